@@ -1,0 +1,139 @@
+"""Benchmark: DOF*stage updates per second of the FP64 DG-SEM wave RK3 stage on one MI355X.
+
+Workload (one GPU): 512 x 512 quads on [-1,1]^2, interior vertices jittered by a seeded uniform
++-15% of the spacing, elements in Hilbert-curve order, N = 8 (81 Legendre-Gauss nodes per
+element, 63,700,992 DOF), reflecting walls (the reference's wall ghost), Gaussian pulses of
+amplitude 1 at (0.1, -0.2) and 0.5 at (-0.4, 0.3), width 0.2. A "step" is one low-storage RK3 step
+(3 fused stage kernels) over the whole mesh, inputs resident in HBM.
+
+Prints one JSON line. Timing: HIP events recorded on the stream the kernels run on, around the
+K timed steps (after W warm-up steps), bracketed by device synchronisation.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+RES, N, JITTER, SEED = 512, 8, 0.15, 2024
+PULSES = [(0.1, -0.2, 0.2, 1.0), (-0.4, 0.3, 0.2, 0.5)]
+CFL = 0.5
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_N8_512x512_traffic.json")
+
+
+def algorithmic_bytes_per_element(n):
+    """FP64 bytes one fused stage moves per element, averaged over the 3 stages: state and RK
+    register (stage 1 reads q, writes q and g; stages 2-3 read and write both) 88*(N+1)^2,
+    neighbour traces read plus own traces written 192*(N+1), vertices and neighbour ids 80."""
+    nf = n + 1
+    return 88 * nf * nf + 192 * nf + 80
+
+
+def cpu_baseline(desc, steps):
+    """The oracle (single-thread C restatement) timed on a bounded sample of the same workload."""
+    from oracle import oracle as O
+    ot = O.Tables(N, 5)
+    om = O.Mesh(ot, desc, N)
+    om.initial_conditions(PULSES)
+    dofs = 3 * om.n
+    t0 = time.perf_counter()
+    om.run_steps(CFL, steps)
+    dt = time.perf_counter() - t0
+    return {"value": dofs * 3 * steps / dt, "unit": "DOF*stage/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} RK3 steps ({3 * steps} stages) of the same {RES}x{RES} N={N} mesh, "
+                      f"oracle/liboracle.so, 1 thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the DG stage "
+              "runs on a single MI355X and multi-GPU sharding is not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ndg_cuda_amd as nd
+
+    if nd.device_count() < 1 or not torch.cuda.is_available():
+        raise SystemExit("bench.py: no GPU visible")
+    desc = nd.square_mesh(RES, jitter=JITTER, seed=SEED)
+    ndg = nd.NDG_t(N, 5)
+    mesh = nd.Mesh2D_t(desc, N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
+    stream = torch.cuda.Stream()
+    mesh.set_stream(stream.cuda_stream)
+    mesh.initial_conditions(PULSES)
+    solver = nd.Solver2D_t(CFL, [0.0])
+    dt = solver.get_delta_t(mesh)
+    t = 0.0
+    for _ in range(args.warmup):
+        solver.step(mesh, t, dt)
+        t += dt
+    mesh.synchronize()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        solver.step(mesh, t, dt)
+        t += dt
+    stop.record(stream)
+    stop.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = start.elapsed_time(stop)
+    p, u, v = mesh.get_solution()
+    finite = bool(all(map(lambda a: a.size and float(abs(a).max()) < 1e3, (p, u, v))))
+    if not finite:
+        raise SystemExit("bench.py: solution blew up")
+
+    dofs = mesh.n_dofs
+    stages = 3 * args.steps
+    value = dofs * stages / (ms / 1e3)
+    launch_s = ms / 1e3 / stages
+    bytes_per_launch = algorithmic_bytes_per_element(N) * mesh.n_elements_
+    achieved = bytes_per_launch / launch_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    out = {
+        "metric": "DOF*stage updates/s (FP64 plain DG-SEM wave RK3 stage)",
+        "value": value,
+        "unit": "DOF*stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{RES}x{RES} quads jittered +-{int(JITTER * 100)}%, Hilbert order, N={N}, "
+                               f"walls, 2 Gaussian pulses, FP64, one RK3 step = 3 fused stage launches",
+                   "elements": mesh.n_elements_, "N": N, "dofs": dofs, "stage_launches": stages,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "ndg::stage_kernel<8,false>", "avg_launch_ms": launch_s * 1e3,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "wall_s_timed_region": wall,
+    }
+    if not args.no_cpu and args.cpu_steps > 0:
+        out["cpu_baseline"] = cpu_baseline(desc, args.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,166 @@
+/*
+ * ndg_amd.h -- C ABI of the MI355X DG-SEM wave-stage library (libndg_amd.so).
+ *
+ * One RK3 time step of the nodal discontinuous-Galerkin spectral-element solver for the
+ * first-order 2D wave system (p, u, v) on Legendre-Gauss nodes, run as HIP kernels on one AMD
+ * Instinct MI355X (gfx950). Each entry point below replaces one piece of the reference host API
+ * of guillaumetousignant/NDG_cuda (cited as file:line into that repository).
+ *
+ * Conventions
+ *  - Every function returns an ndg_status; 0 is success. The reference exits the process on
+ *    errors (e.g. src/meshes/Mesh2D_t.cpp:68-69); this ABI returns a code instead and keeps a
+ *    message readable with ndg_last_error (thread-local).
+ *  - Host pointers are borrowed for the duration of the call only. Device memory is owned by
+ *    the library. All device work of a mesh is ordered on one HIP stream (the mesh's own, or
+ *    one given with ndg_mesh_set_stream). Handles are not thread-safe.
+ *  - Packed field layout ("reference order"): element by element in mesh order, each element's
+ *    (N+1)^2 values in the reference node order q[i*(N+1)+j], i the xi index and j the eta index
+ *    (include/entities/Element2D_t.h:37-39).
+ */
+#ifndef NDG_AMD_H
+#define NDG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int ndg_status;
+#define NDG_OK 0
+#define NDG_ERR_INVALID 1     /* bad argument or inconsistent mesh description */
+#define NDG_ERR_HIP 2         /* a HIP runtime call or kernel launch failed */
+#define NDG_ERR_UNSUPPORTED 3 /* valid input this build does not handle (message says which) */
+#define NDG_ERR_NO_DEVICE 4   /* no HIP device visible */
+
+/* Boundary kinds of boundary (BAR_2) edges. Values follow the CGNS BC types the reference reads
+ * (src/meshes/Mesh2D_t.cpp:689-791) plus the periodic self-interface (:820-875). */
+#define NDG_BC_NONE 0
+#define NDG_BC_WALL 1
+#define NDG_BC_SYMMETRY 2
+#define NDG_BC_INFLOW 3
+#define NDG_BC_OUTFLOW 4
+#define NDG_BC_PERIODIC 5
+
+/* Table selectors (the public tables of NDG_t, include/entities/NDG_t.h:11-26). */
+#define NDG_TABLE_NODES 0
+#define NDG_TABLE_WEIGHTS 1
+#define NDG_TABLE_BARYCENTRIC_WEIGHTS 2
+#define NDG_TABLE_LAGRANGE_LEFT 3
+#define NDG_TABLE_LAGRANGE_RIGHT 4
+#define NDG_TABLE_LAGRANGE_DERIVATIVE_LEFT 5
+#define NDG_TABLE_LAGRANGE_DERIVATIVE_RIGHT 6
+#define NDG_TABLE_DERIVATIVE 7
+#define NDG_TABLE_G_HAT_DERIVATIVE 8
+#define NDG_TABLE_DERIVATIVE_HAT 9
+#define NDG_TABLE_INTERPOLATION 10
+#define NDG_TABLE_POLYNOMIALS 11
+
+/* Initial-condition kinds. */
+#define NDG_IC_PLANE_WAVE 0 /* the reference g(x, 0): include/functions/analytical_solution.tpp:5-11 */
+#define NDG_IC_PULSES 1     /* sum of radial Gaussian pulses, u = v = 0 (no reference counterpart) */
+
+typedef struct ndg_tables ndg_tables;
+typedef struct ndg_mesh ndg_mesh;
+
+/* In-memory form of the mesh file the reference reads (CGNS unstructured zone,
+ * src/meshes/Mesh2D_t.cpp:422-1048): nodes, QUAD_4 domain elements, BAR_2 boundary elements with
+ * a boundary kind, and for periodic bars the donor bar (the CGNS Abutting1to1 point list donor). */
+typedef struct ndg_mesh_desc {
+    int64_t n_nodes;
+    const double *xy;         /* 2*n_nodes: x0, y0, x1, y1, ... */
+    int64_t n_quads;
+    const int64_t *quads;     /* 4*n_quads node ids, counter-clockwise; storage order = element order */
+    const int32_t *quad_N;    /* per-element order, or NULL for initial_N everywhere */
+    int64_t n_bars;
+    const int64_t *bars;      /* 2*n_bars node ids */
+    const int32_t *bar_type;  /* NDG_BC_* per bar */
+    const int64_t *bar_donor; /* per bar: donor bar index for NDG_BC_PERIODIC, else ignored (may be NULL) */
+    int32_t initial_N;
+    int32_t maximum_N;
+    int32_t max_split_level;
+    double tolerance_min;     /* estimator thresholds (reference defaults 1e-6 / 1e-14) */
+    double tolerance_max;
+} ndg_mesh_desc;
+
+/* Parameters of the square-mesh generator, the equivalent of extras/mesh_generator/main.cpp
+ * (--resolution, --x_min ... --y_max, boundary types, --x_periodic / --y_periodic) plus an
+ * optional seeded vertex jitter (no reference counterpart). bc = {bottom, right, top, left}. */
+typedef struct ndg_square_mesh_params {
+    int32_t resolution;       /* power of two */
+    double x_min, x_max, y_min, y_max;
+    int32_t bc[4];
+    int32_t x_periodic, y_periodic;
+    double jitter;            /* interior vertex jitter as a fraction of the spacing (0 = none) */
+    uint64_t seed;
+} ndg_square_mesh_params;
+
+typedef struct ndg_ic_desc {
+    int32_t kind;             /* NDG_IC_* */
+    int32_t n_pulses;
+    const double *pulses;     /* n_pulses x {x0, y0, width, amplitude}: p = a exp(-ln2 r^2 / w^2) */
+} ndg_ic_desc;
+
+/* ---- errors / device ---- */
+/* Copies the last error message of this thread (always NUL-terminated). */
+ndg_status ndg_last_error(char *buf, size_t buf_len);
+/* Number of visible HIP devices (0 without a GPU; never fails on a CPU-only host). */
+ndg_status ndg_device_count(int32_t *count);
+
+/* ---- NDG_t: include/entities/NDG_t.h:11-26, include/entities/NDG_t.tpp:9-59 ---- */
+ndg_status ndg_tables_create(int32_t N_max, int64_t n_interpolation_points, ndg_tables **out);
+ndg_status ndg_tables_destroy(ndg_tables *tables);
+/* Copies one table for order N into out (capacity doubles). *count receives its length. */
+ndg_status ndg_tables_get(const ndg_tables *tables, int32_t which, int32_t N, double *out,
+                          int64_t capacity, int64_t *count);
+
+/* ---- mesh generator: extras/mesh_generator/main.cpp:161-382 ---- */
+ndg_status ndg_square_mesh_sizes(const ndg_square_mesh_params *params, int64_t *n_nodes,
+                                 int64_t *n_quads, int64_t *n_bars);
+/* Fills caller-allocated arrays of the sizes given by ndg_square_mesh_sizes. Elements follow the
+ * Hilbert curve (src/functions/Hilbert.cpp:16-28); local nodes are BR, TR, TL, BL. */
+ndg_status ndg_square_mesh_fill(const ndg_square_mesh_params *params, double *xy, int64_t *quads,
+                                int64_t *bars, int32_t *bar_type, int64_t *bar_donor);
+
+/* ---- Mesh2D_t: include/meshes/Mesh2D_t.h:21 (constructor), :105-128 (phase methods) ---- */
+ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_tables *tables, ndg_mesh **out);
+ndg_status ndg_mesh_destroy(ndg_mesh *mesh);
+/* Orders all later work of this mesh on the given hipStream_t (NULL: the mesh's own stream). */
+ndg_status ndg_mesh_set_stream(ndg_mesh *mesh, void *hip_stream);
+/* n_dofs_per_field = sum over elements of (N+1)^2. */
+ndg_status ndg_mesh_info(const ndg_mesh *mesh, int64_t *n_elements, int64_t *n_faces,
+                         int64_t *n_dofs_per_field, int32_t *N_min, int32_t *N_max);
+/* Mesh2D_t::initial_conditions (src/meshes/Mesh2D_t.cpp:1192-1216); zeroes G and the RK register. */
+ndg_status ndg_set_ic(ndg_mesh *mesh, const ndg_ic_desc *ic);
+/* Mesh2D_t::get_solution's data without interpolation: packed p, u, v (host arrays). */
+ndg_status ndg_set_state(ndg_mesh *mesh, const double *p, const double *u, const double *v);
+ndg_status ndg_get_state(ndg_mesh *mesh, double *p, double *u, double *v);
+/* Element traces (Mesh2D_t::interpolate_to_boundaries, src/meshes/Mesh2D_t.cpp:3794-3801) of the
+ * current state, packed [element][side][field][N+1]. */
+ndg_status ndg_get_traces(ndg_mesh *mesh, double *out);
+/* One stage's right-hand side G at time t (phases interpolate_to_boundaries .. compute_dg_wave_
+ * derivative, include/solvers/Solver2D_t.tpp:46-51), packed; the state is not advanced. */
+ndg_status ndg_compute_rhs(ndg_mesh *mesh, double t, double *Gp, double *Gu, double *Gv);
+/* Solver2D_t::get_delta_t (src/solvers/Solver2D_t.cpp:19-29): a device reduction. */
+ndg_status ndg_get_delta_t(ndg_mesh *mesh, double cfl, double *dt);
+/* One RK3 step (3 stages, include/solvers/Solver2D_t.tpp:44-70) from time t with step dt.
+ * Stream-ordered; returns without waiting for the GPU. */
+ndg_status ndg_step(ndg_mesh *mesh, double t, double dt);
+/* The reference time loop Solver2D_t::solve (include/solvers/Solver2D_t.tpp:37-72) without output
+ * or adaptation: dt from get_delta_t, the last step clipped to t_end, time accumulated on the
+ * host exactly as the reference does. Waits for completion. */
+ndg_status ndg_solve(ndg_mesh *mesh, double cfl, double t_end, double *t_final, int64_t *n_steps);
+/* n_steps steps of the same loop without clipping, from *time_io (updated). Stream-ordered. */
+ndg_status ndg_run_steps(ndg_mesh *mesh, double cfl, int64_t n_steps, double *time_io);
+/* Waits for all work queued on the mesh's stream. */
+ndg_status ndg_synchronize(ndg_mesh *mesh);
+/* Mesh2D_t::estimate_error -> Element2D_t::estimate_p_error (src/entities/Element2D_t.cpp:274-343),
+ * pressure estimator per element. Any output pointer may be NULL. */
+ndg_status ndg_estimate_error(ndg_mesh *mesh, double *error, double *sigma, int32_t *refine,
+                              int32_t *coarsen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,141 @@
+"""ctypes binding of include/ndg_amd.h (libndg_amd.so, built in-tree by ndg_cuda_amd/build.py).
+
+There is no fallback: if the library is missing or a call fails, an exception is raised.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libndg_amd.so")
+
+NDG_OK = 0
+NDG_ERR_INVALID = 1
+NDG_ERR_HIP = 2
+NDG_ERR_UNSUPPORTED = 3
+NDG_ERR_NO_DEVICE = 4
+
+BC_NONE, BC_WALL, BC_SYMMETRY, BC_INFLOW, BC_OUTFLOW, BC_PERIODIC = range(6)
+IC_PLANE_WAVE, IC_PULSES = 0, 1
+
+TABLE_NAMES = ["nodes", "weights", "barycentric_weights", "lagrange_interpolant_left",
+               "lagrange_interpolant_right", "lagrange_interpolant_derivative_left",
+               "lagrange_interpolant_derivative_right", "derivative_matrices",
+               "g_hat_derivative_matrices", "derivative_matrices_hat", "interpolation_matrices",
+               "polynomials"]
+
+# Every symbol include/ndg_amd.h declares (checked by the CPU tests).
+EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tables_destroy",
+            "ndg_tables_get", "ndg_square_mesh_sizes", "ndg_square_mesh_fill", "ndg_mesh_create",
+            "ndg_mesh_destroy", "ndg_mesh_set_stream", "ndg_mesh_info", "ndg_set_ic",
+            "ndg_set_state", "ndg_get_state", "ndg_get_traces", "ndg_compute_rhs",
+            "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_run_steps", "ndg_synchronize",
+            "ndg_estimate_error"]
+
+
+class NDGError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"[ndg status {code}] {message}")
+        self.code = code
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("n_nodes", ctypes.c_int64), ("xy", ctypes.c_void_p),
+                ("n_quads", ctypes.c_int64), ("quads", ctypes.c_void_p), ("quad_N", ctypes.c_void_p),
+                ("n_bars", ctypes.c_int64), ("bars", ctypes.c_void_p), ("bar_type", ctypes.c_void_p),
+                ("bar_donor", ctypes.c_void_p), ("initial_N", ctypes.c_int32),
+                ("maximum_N", ctypes.c_int32), ("max_split_level", ctypes.c_int32),
+                ("tolerance_min", ctypes.c_double), ("tolerance_max", ctypes.c_double)]
+
+
+class SquareMeshParams(ctypes.Structure):
+    _fields_ = [("resolution", ctypes.c_int32), ("x_min", ctypes.c_double), ("x_max", ctypes.c_double),
+                ("y_min", ctypes.c_double), ("y_max", ctypes.c_double), ("bc", ctypes.c_int32 * 4),
+                ("x_periodic", ctypes.c_int32), ("y_periodic", ctypes.c_int32),
+                ("jitter", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class ICDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("n_pulses", ctypes.c_int32), ("pulses", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libndg_amd.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (or python -m "
+                              "ndg_cuda_amd.build) first; there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        sigs = {
+            "ndg_last_error": [ctypes.c_char_p, ctypes.c_size_t],
+            "ndg_device_count": [vp],
+            "ndg_tables_create": [i32, i64, vp],
+            "ndg_tables_destroy": [vp],
+            "ndg_tables_get": [vp, i32, i32, vp, i64, vp],
+            "ndg_square_mesh_sizes": [vp, vp, vp, vp],
+            "ndg_square_mesh_fill": [vp, vp, vp, vp, vp, vp],
+            "ndg_mesh_create": [vp, vp, vp],
+            "ndg_mesh_destroy": [vp],
+            "ndg_mesh_set_stream": [vp, vp],
+            "ndg_mesh_info": [vp, vp, vp, vp, vp, vp],
+            "ndg_set_ic": [vp, vp],
+            "ndg_set_state": [vp, vp, vp, vp],
+            "ndg_get_state": [vp, vp, vp, vp],
+            "ndg_get_traces": [vp, vp],
+            "ndg_compute_rhs": [vp, f64, vp, vp, vp],
+            "ndg_get_delta_t": [vp, f64, vp],
+            "ndg_step": [vp, f64, f64],
+            "ndg_solve": [vp, f64, f64, vp, vp],
+            "ndg_run_steps": [vp, f64, i64, vp],
+            "ndg_synchronize": [vp],
+            "ndg_estimate_error": [vp, vp, vp, vp, vp],
+        }
+        for name, args in sigs.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    lib().ndg_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(status):
+    if status != NDG_OK:
+        raise NDGError(status, last_error())
+
+
+def ptr(a):
+    """Address of a C-contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data
+
+
+def device_count():
+    n = ctypes.c_int32(0)
+    check(lib().ndg_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def as_f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def as_i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def as_i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)

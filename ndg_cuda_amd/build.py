@@ -1,0 +1,58 @@
+"""In-tree build of libndg_amd.so for gfx950 (hipcc), used by __graft_entry__.build().
+
+Kernels (*.hip) are compiled with hipcc's default FP contraction; the host sources (*.cpp), which
+reproduce the reference host arithmetic (tables, face frames, time accumulation), are compiled
+with -ffp-contract=off. Packed FP32 instructions are disabled for all device code.
+"""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libndg_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = ["-std=c++17", "-fPIC", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-O3"] + COMMON
+HOST_FLAGS = ["-O2", "-ffp-contract=off", "-Wall"] + COMMON
+
+
+def _run(cmd):
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    # hipcc's host pass does not know the packed-fp32 target feature and says so; that is harmless
+    noise = [l for l in (proc.stdout + proc.stderr).splitlines()
+             if l.strip() and "packed-fp32-ops" not in l and "warning generated" not in l
+             and "warnings generated" not in l]
+    if proc.returncode != 0:
+        raise RuntimeError("command failed: " + " ".join(cmd) + "\n" + "\n".join(noise))
+    return noise
+
+
+def build(verbose=False):
+    """Compiles every source under csrc/ and links ndg_cuda_amd/libndg_amd.so. Always rebuilds."""
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for name in sorted(os.listdir(CSRC)):
+        src = os.path.join(CSRC, name)
+        stem, ext = os.path.splitext(name)
+        if ext not in (".hip", ".cpp"):
+            continue
+        obj = os.path.join(BUILD, stem + ".o")
+        flags = DEVICE_FLAGS if ext == ".hip" else HOST_FLAGS
+        msgs = _run([HIPCC] + flags + ["-c", src, "-o", obj])
+        if verbose and msgs:
+            print("\n".join(msgs), file=sys.stderr)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,528 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of one low-storage RK3 stage of the DG-SEM wave solver.
+//
+// Data layout in HBM (structure of arrays, element-major, elements in Hilbert-curve order):
+//   q, g   [K][3][(N+1)^2]   state (p, u, v) and RK register, reference node order i*(N+1)+j
+//   trace  [K][4][3][N+1]    element traces of the current state (two buffers, ping-pong)
+//   verts  [K][4][2]         element vertices (metrics are recomputed from them per node)
+//   side_nb[K][4]            neighbour element*4+side, or -(1+index) into the BcSide records
+//
+// One fused stage kernel replaces the reference's per-phase kernels of one stage
+// (include/solvers/Solver2D_t.tcu:45-52): a workgroup stages the (N+1)^2 tiles of EPB elements in
+// LDS, computes the upwind flux on its 4 sides from its own traces (recomputed from the LDS tile)
+// and the neighbours' traces (one HBM read), applies volume + lift, the RK update, and writes the
+// traces of the updated state for the next stage. Per element and stage HBM traffic is the state
+// and RK register (read + write), one read of the neighbours' traces and one write of its own.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace ndg {
+
+template <int N>
+struct Shape {
+    static constexpr int NF = N + 1;
+    static constexpr int NE = NF * NF;
+    static constexpr int EPB = NE >= 256 ? 1 : 256 / NE;  // elements per workgroup
+    static constexpr int THREADS = ((EPB * NE + 63) / 64) * 64;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD walks one
+// contiguous stretch of the Hilbert-ordered elements (neighbours share its L2).
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G) {
+    const int64_t per = G / 8, rem = G % 8, x = b % 8, i = b / 8;
+    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
+// Bilinear metrics (include/functions/quad_metrics.tpp:3-8): x_xi, x_eta, y_xi, y_eta.
+__device__ __forceinline__ void metrics(const double *P, double xi, double eta, double &xxi,
+                                        double &xeta, double &yxi, double &yeta) {
+    xxi = ((1 - eta) * (P[2] - P[0]) + (1 + eta) * (P[4] - P[6])) / 4;
+    xeta = ((1 - xi) * (P[6] - P[0]) + (1 + xi) * (P[4] - P[2])) / 4;
+    yxi = ((1 - eta) * (P[3] - P[1]) + (1 + eta) * (P[5] - P[7])) / 4;
+    yeta = ((1 - xi) * (P[7] - P[1]) + (1 + xi) * (P[5] - P[3])) / 4;
+}
+
+// Edge Jacobian of side s (src/entities/Element2D_t.cpp:756-769): constant along a side of a
+// bilinear element, so it is evaluated once at the side's first node.
+__device__ __forceinline__ double side_scale(const double *P, int s, double x0) {
+    double xxi, xeta, yxi, yeta;
+    if (s == 0) { metrics(P, x0, -1.0, xxi, xeta, yxi, yeta); return sqrt(xxi * xxi + yxi * yxi); }
+    if (s == 1) { metrics(P, 1.0, x0, xxi, xeta, yxi, yeta); return sqrt(xeta * xeta + yeta * yeta); }
+    if (s == 2) { metrics(P, x0, 1.0, xxi, xeta, yxi, yeta); return sqrt(xxi * xxi + yxi * yxi); }
+    metrics(P, -1.0, x0, xxi, xeta, yxi, yeta);
+    return sqrt(xeta * xeta + yeta * yeta);
+}
+
+// The reference plane wave g(x, t) (include/functions/analytical_solution.tpp:5-11).
+__device__ __forceinline__ void plane_wave(double x, double y, double t, double &p, double &u,
+                                           double &v) {
+    const double kx = 0.707106781186548, ky = 0.707106781186548, x0 = -0.2, y0 = -0.2;
+    const double d = 0.120112240878645;
+    const double a = kx * (x - x0) + ky * (y - y0) - kWaveSpeed * t;
+    p = exp(-a * a / (d * d));
+    u = p * kx / kWaveSpeed;
+    v = p * ky / kWaveSpeed;
+}
+
+// Trace k of side s of one element tile (src/entities/Element2D_t.cpp:86-125): sides 2 and 3
+// run counter-clockwise, i.e. reversed with respect to the xi / eta index.
+template <int N>
+__device__ __forceinline__ double side_trace(const double *tile, const double *lm, const double *lp,
+                                             int s, int k) {
+    constexpr int NF = N + 1;
+    double acc = 0.0;
+    if (s == 0) {
+#pragma unroll
+        for (int j = 0; j < NF; ++j) acc += lm[j] * tile[k * NF + j];
+    } else if (s == 1) {
+#pragma unroll
+        for (int j = 0; j < NF; ++j) acc += lp[j] * tile[j * NF + k];
+    } else if (s == 2) {
+#pragma unroll
+        for (int j = 0; j < NF; ++j) acc += lp[j] * tile[(N - k) * NF + j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < NF; ++j) acc += lm[j] * tile[j * NF + (N - k)];
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Traces of the current state (used after the state is set; the stage kernel writes them
+// afterwards).
+template <int N>
+__global__ void __launch_bounds__(Shape<N>::THREADS)
+traces_kernel(const double *__restrict__ tab, const double *__restrict__ q,
+              double *__restrict__ tr, int64_t K) {
+    using S = Shape<N>;
+    constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::THREADS;
+    __shared__ double s_lm[NF], s_lp[NF];
+    __shared__ double s_q[EPB * 3 * NE];
+    const int tid = threadIdx.x;
+    const int64_t e0 = (int64_t)blockIdx.x * EPB;
+    const int nel = (int)min<int64_t>(EPB, K - e0);
+    for (int r = tid; r < NF; r += T) {
+        s_lm[r] = tab[NF * NF + r];
+        s_lp[r] = tab[NF * NF + NF + r];
+    }
+    const double *qg = q + e0 * 3 * NE;
+    for (int r = tid; r < nel * 3 * NE; r += T) s_q[r] = qg[r];
+    __syncthreads();
+    for (int r = tid; r < nel * 12 * NF; r += T) {
+        const int el = r / (12 * NF), rs = r % (12 * NF);
+        const int s = rs / (3 * NF), f = (rs / NF) % 3, k = rs % NF;
+        tr[(e0 + el) * 12 * NF + rs] = side_trace<N>(s_q + el * 3 * NE + f * NE, s_lm, s_lp, s, k);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// One fused RK stage. RHS_ONLY writes G (Solver2D_t::compute_dg_wave_derivative output) and
+// leaves the state alone.
+template <int N, bool RHS_ONLY>
+__global__ void __launch_bounds__(Shape<N>::THREADS)
+stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_nb,
+             const BcSide *__restrict__ bc, const double *__restrict__ tab,
+             double *__restrict__ q, double *__restrict__ g, const double *__restrict__ tr_in,
+             double *__restrict__ tr_out, double *__restrict__ rhs, int64_t K, int first,
+             double a, double gdt, double t_stage) {
+    using S = Shape<N>;
+    constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::THREADS;
+    constexpr double c = kWaveSpeed;
+    __shared__ double s_Dh[NF * NF], s_lm[NF], s_lp[NF], s_w[NF], s_x[NF];
+    __shared__ double s_v[EPB * 8];
+    __shared__ double s_q[EPB * 3 * NE];
+    __shared__ double s_ft[EPB * 3 * NE];
+    __shared__ double s_gt[EPB * 3 * NE];
+    __shared__ double s_fs[EPB * 12 * NF];
+
+    const int tid = threadIdx.x;
+    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t e0 = blk * EPB;
+    const int nel = (int)min<int64_t>(EPB, K - e0);
+
+    for (int r = tid; r < NF * NF; r += T) s_Dh[r] = tab[r];
+    for (int r = tid; r < NF; r += T) {
+        s_lm[r] = tab[NF * NF + r];
+        s_lp[r] = tab[NF * NF + NF + r];
+        s_w[r] = tab[NF * NF + 2 * NF + r];
+        s_x[r] = tab[NF * NF + 3 * NF + r];
+    }
+    for (int r = tid; r < nel * 8; r += T) s_v[r] = verts[e0 * 8 + r];
+    const double *qg = q + e0 * 3 * NE;
+    for (int r = tid; r < nel * 3 * NE; r += T) s_q[r] = qg[r];
+
+    const int el = tid / NE, node = tid % NE;
+    const bool active = tid < nel * NE;
+    double g_old[3] = {0.0, 0.0, 0.0};
+    if (!RHS_ONLY && active && !first) {
+        const double *gg = g + (e0 + el) * 3 * NE + node;
+        g_old[0] = gg[0];
+        g_old[1] = gg[NE];
+        g_old[2] = gg[2 * NE];
+    }
+    __syncthreads();
+
+    // Phase 1a: contravariant volume fluxes at the own node (Solver2D_t.cpp:71-79, 104-112).
+    double jac = 1.0;
+    if (active) {
+        const int i = node / NF, j = node % NF;
+        double xxi, xeta, yxi, yeta;
+        metrics(s_v + el * 8, s_x[i], s_x[j], xxi, xeta, yxi, yeta);
+        jac = xxi * yeta - xeta * yxi;
+        const double *tq = s_q + el * 3 * NE;
+        const double p = tq[node], u = tq[NE + node], v = tq[2 * NE + node];
+        double *ft = s_ft + el * 3 * NE, *gt = s_gt + el * 3 * NE;
+        ft[node] = yeta * (c * u) - xeta * (c * v);
+        ft[NE + node] = yeta * p;
+        ft[2 * NE + node] = -(xeta * p);
+        gt[node] = -yxi * (c * u) + xxi * (c * v);
+        gt[NE + node] = -yxi * p;
+        gt[2 * NE + node] = xxi * p;
+    }
+    // Phase 1b: upwind flux on the element's sides, scaled by the edge Jacobian
+    // (boundary_conditions, project_to_faces, calculate_wave_fluxes, project_to_elements for
+    // conforming faces: Mesh2D_t.cpp:3731-3935, Solver2D_t.cpp:48-63).
+    for (int r = tid; r < nel * 4 * NF; r += T) {
+        const int fe = r / (4 * NF), rs = r % (4 * NF), s = rs / NF, k = rs % NF;
+        const double *tq = s_q + fe * 3 * NE;
+        const double *P = s_v + fe * 8;
+        const double pL = side_trace<N>(tq, s_lm, s_lp, s, k);
+        const double uL = side_trace<N>(tq + NE, s_lm, s_lp, s, k);
+        const double vL = side_trace<N>(tq + 2 * NE, s_lm, s_lp, s, k);
+        const int s1 = (s + 1) & 3;
+        const double ex = P[2 * s1] - P[2 * s], ey = P[2 * s1 + 1] - P[2 * s + 1];
+        const double len = sqrt(ex * ex + ey * ey);
+        double nx = ey / len, ny = -(ex / len);
+        double pR, uR, vR;
+        const int32_t code = side_nb[(e0 + fe) * 4 + s];
+        if (code >= 0) {
+            const double *nb = tr_in + (int64_t)(code >> 2) * 12 * NF + (code & 3) * 3 * NF + (N - k);
+            pR = nb[0];
+            uR = nb[NF];
+            vR = nb[2 * NF];
+        } else {
+            const BcSide rec = bc[-1 - code];
+            nx = rec.nx;
+            ny = rec.ny;
+            if (rec.kind == NDG_BC_WALL || rec.kind == NDG_BC_SYMMETRY) {
+                // ghost = sf * own trace, normal velocity reversed (Mesh2D_t.cpp:4377-4433)
+                pR = pL * rec.sf;
+                const double ux = uL * rec.sf, uy = vL * rec.sf;
+                const double l0 = -(ux * rec.nx + uy * rec.ny);
+                const double l1 = ux * rec.tx + uy * rec.ty;
+                uR = rec.nx * l0 + rec.tx * l1;
+                vR = rec.ny * l0 + rec.ty * l1;
+            } else if (rec.kind == NDG_BC_OUTFLOW) {
+                pR = pL * rec.sf;
+                uR = uL * rec.sf;
+                vR = vL * rec.sf;
+            } else if (rec.kind == NDG_BC_INFLOW) {
+                // ghost trace = g(x, t) at the ghost's nodes, seen reversed (Mesh2D_t.cpp:4575-4599)
+                const double interp = (s_x[N - k] + 1) / 2;
+                const double X = rec.bx * interp + rec.ax * (1 - interp);
+                const double Y = rec.by * interp + rec.ay * (1 - interp);
+                plane_wave(X, Y, t_stage, pR, uR, vR);
+            } else {
+                pR = 0.0;
+                uR = 0.0;
+                vR = 0.0;
+            }
+        }
+        const double wL = pL + c * (uL * nx + vL * ny);
+        const double wR = pR - c * (uR * nx + vR * ny);
+        const double sf = side_scale(P, s, s_x[0]);
+        double *fs = s_fs + fe * 12 * NF + s * 3 * NF + k;
+        fs[0] = (c * (wL - wR) / 2) * sf;
+        fs[NF] = (nx * (wL + wR) / 2) * sf;
+        fs[2 * NF] = (ny * (wL + wR) / 2) * sf;
+    }
+    __syncthreads();
+
+    // Phase 2: D^ contraction along xi and eta plus lift (Solver2D_t.cpp:81-133), RK update
+    // (Solver2D_t.cpp:138-172).
+    if (active) {
+        const int i = node / NF, j = node % NF;
+        const double *ft = s_ft + el * 3 * NE, *gt = s_gt + el * 3 * NE;
+        const double *fs = s_fs + el * 12 * NF;
+        double Gn[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            double dxi = 0.0, deta = 0.0;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) dxi += s_Dh[i * NF + k] * ft[f * NE + k * NF + j];
+            dxi += (fs[3 * NF + f * NF + j] * s_lp[i] + fs[9 * NF + f * NF + (N - j)] * s_lm[i]) / s_w[i];
+#pragma unroll
+            for (int k = 0; k < NF; ++k) deta += s_Dh[j * NF + k] * gt[f * NE + i * NF + k];
+            deta += (fs[6 * NF + f * NF + (N - i)] * s_lp[j] + fs[f * NF + i] * s_lm[j]) / s_w[j];
+            Gn[f] = (-dxi - deta) / jac;
+        }
+        const int64_t off = (e0 + el) * 3 * NE + node;
+        if (RHS_ONLY) {
+            rhs[off] = Gn[0];
+            rhs[off + NE] = Gn[1];
+            rhs[off + 2 * NE] = Gn[2];
+        } else {
+            double *tq = s_q + el * 3 * NE;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const double gn = first ? Gn[f] : a * g_old[f] + Gn[f];
+                const double qn = tq[f * NE + node] + gdt * gn;
+                g[off + f * NE] = gn;
+                q[off + f * NE] = qn;
+                tq[f * NE + node] = qn;
+            }
+        }
+    }
+    if (RHS_ONLY) return;
+    __syncthreads();
+
+    // Phase 3: traces of the updated state for the next stage.
+    for (int r = tid; r < nel * 12 * NF; r += T) {
+        const int fe = r / (12 * NF), rs = r % (12 * NF);
+        const int s = rs / (3 * NF), f = (rs / NF) % 3, k = rs % NF;
+        tr_out[(e0 + fe) * 12 * NF + rs] = side_trace<N>(s_q + fe * 3 * NE + f * NE, s_lm, s_lp, s, k);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Initial conditions (src/meshes/Mesh2D_t.cpp:1192-1216; pulses have no reference counterpart).
+template <int N>
+__global__ void ic_kernel(const double *__restrict__ verts, const double *__restrict__ tab,
+                          double *__restrict__ q, int64_t K, int kind, int n_pulses,
+                          const double *__restrict__ pulses) {
+    constexpr int NF = N + 1, NE = NF * NF;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= K * NE) return;
+    const int64_t e = gid / NE;
+    const int node = (int)(gid % NE), i = node / NF, j = node % NF;
+    const double *x = tab + NF * NF + 3 * NF;
+    const double *P = verts + e * 8;
+    const double xi = x[i], eta = x[j];
+    const double X = (P[0] * (1 - xi) * (1 - eta) + P[2] * (xi + 1) * (1 - eta) +
+                      P[4] * (xi + 1) * (eta + 1) + P[6] * (1 - xi) * (eta + 1)) / 4;
+    const double Y = (P[1] * (1 - xi) * (1 - eta) + P[3] * (xi + 1) * (1 - eta) +
+                      P[5] * (xi + 1) * (eta + 1) + P[7] * (1 - xi) * (eta + 1)) / 4;
+    double p = 0.0, u = 0.0, v = 0.0;
+    if (kind == NDG_IC_PLANE_WAVE) {
+        plane_wave(X, Y, 0.0, p, u, v);
+    } else {
+        for (int k = 0; k < n_pulses; ++k) {
+            const double dx = X - pulses[4 * k], dy = Y - pulses[4 * k + 1], w = pulses[4 * k + 2];
+            p += pulses[4 * k + 3] * exp(-M_LN2 * (dx * dx + dy * dy) / (w * w));
+        }
+    }
+    double *qe = q + e * 3 * NE + node;
+    qe[0] = p;
+    qe[NE] = u;
+    qe[2 * NE] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// CFL time step: min over elements of CFL * delta_min / (N^2 c) (src/solvers/Solver2D_t.cpp:19-29,
+// Element2D_t.cpp:772-775), reduced in two passes on the device; no host round trip per block.
+__device__ __forceinline__ double edge_len(const double *P, int a, int b) {
+#pragma clang fp contract(off)
+    const double dx = P[2 * a] - P[2 * b], dy = P[2 * a + 1] - P[2 * b + 1];
+    return sqrt(dx * dx + dy * dy);
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
+    return v;
+}
+
+__global__ void __launch_bounds__(256)
+dt_partial_kernel(const double *__restrict__ verts, int64_t K, int N, double cfl,
+                  double *__restrict__ partial) {
+#pragma clang fp contract(off)
+    __shared__ double s_min[4];
+    double v = INFINITY;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < K; e += (int64_t)gridDim.x * 256) {
+        const double *P = verts + e * 8;
+        const double d = fmin(fmin(fmin(edge_len(P, 1, 0), edge_len(P, 2, 3)),
+                                   fmin(edge_len(P, 1, 2), edge_len(P, 0, 3))),
+                              fmin(edge_len(P, 1, 3), edge_len(P, 2, 0)));
+        v = fmin(cfl * d / (N * N * kWaveSpeed), v);
+    }
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = fmin(fmin(s_min[0], s_min[1]), fmin(s_min[2], s_min[3]));
+}
+
+__global__ void __launch_bounds__(256)
+dt_final_kernel(const double *__restrict__ partial, int n, double *__restrict__ out) {
+    __shared__ double s_min[4];
+    double v = INFINITY;
+    for (int k = threadIdx.x; k < n; k += 256) v = fmin(v, partial[k]);
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = fmin(fmin(s_min[0], s_min[1]), fmin(s_min[2], s_min[3]));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pressure estimator (src/entities/Element2D_t.cpp:274-343): spectrum of the last
+// min(N+1, 4) modal indices, least-squares fit of its log-decay, error and flags. One thread per
+// element, FP64 throughout. out[e] = {error, sigma, refine, coarsen}.
+template <int N>
+__global__ void __launch_bounds__(128)
+estimate_kernel(const double *__restrict__ tab, const double *__restrict__ q, int64_t K,
+                double tol_min, double tol_max, double *__restrict__ out) {
+    constexpr int NF = N + 1, NE = NF * NF;
+    constexpr int n = NF < 4 ? NF : 4;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= K) return;
+    const double *M = tab + NF * NF + 4 * NF;
+    const double *p = q + e * 3 * NE;
+    double spec[n];
+    for (int ni = 0; ni < n; ++ni) {
+        spec[ni] = 0.0;
+        const int pm = N + ni + 1 - n;
+        for (int i = 0; i < pm; ++i) {
+            double lx = 0.0, ly = 0.0;
+            for (int k = 0; k <= N; ++k)
+                for (int l = 0; l <= N; ++l) {
+                    lx += p[k * NF + l] * M[i * NF + k] * M[pm * NF + l];
+                    ly += p[k * NF + l] * M[pm * NF + k] * M[i * NF + l];
+                }
+            spec[ni] += fabs(lx) + fabs(ly);
+        }
+        double lx = 0.0;
+        for (int k = 0; k <= N; ++k)
+            for (int l = 0; l <= N; ++l) lx += p[k * NF + l] * M[pm * NF + k] * M[pm * NF + l];
+        spec[ni] += fabs(lx);
+    }
+    double xa = 0.0, ya = 0.0;
+    for (int i = 0; i < n; ++i) {
+        xa += N + i + 1 - n;
+        ya += log(spec[i]);
+    }
+    xa /= n;
+    ya /= n;
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const int pp = N + i + 1 - n;
+        num += (pp - xa) * (log(spec[i]) - ya);
+        den += (pp - xa) * (pp - xa);
+    }
+    const double sig = num / den;
+    const double C = exp(ya - sig * xa);
+    const double sigma = fabs(sig);
+    const double err = sqrt(spec[n - 1] * spec[n - 1] + C * C * 0.5 / sigma * exp(-2 * sigma * (N + 1)));
+    out[4 * e + 0] = err;
+    out[4 * e + 1] = sigma;
+    out[4 * e + 2] = err > tol_min ? 1.0 : 0.0;
+    out[4 * e + 3] = err < tol_max ? 1.0 : 0.0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launchers: dispatch on the (uniform) order N.
+#define NDG_FOR_ORDERS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+
+bool order_supported(int N) { return N >= 1 && N <= kMaxOrder; }
+
+template <int N>
+static hipError_t traces_n(const DeviceMesh &m, hipStream_t s) {
+    using S = Shape<N>;
+    const int64_t blocks = (m.K + S::EPB - 1) / S::EPB;
+    hipLaunchKernelGGL(traces_kernel<N>, dim3((unsigned)blocks), dim3(S::THREADS), 0, s, m.tab, m.q,
+                       m.trace[m.cur], m.K);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t stage_n(DeviceMesh &m, int stage, double t_stage, double a, double gdt, hipStream_t s) {
+    using S = Shape<N>;
+    const int64_t blocks = (m.K + S::EPB - 1) / S::EPB;
+    hipLaunchKernelGGL((stage_kernel<N, false>), dim3((unsigned)blocks), dim3(S::THREADS), 0, s, m.verts,
+                       m.side_nb, m.bc, m.tab, m.q, m.g, m.trace[m.cur], m.trace[1 - m.cur], nullptr, m.K,
+                       stage == 0 ? 1 : 0, a, gdt, t_stage);
+    m.cur = 1 - m.cur;
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t rhs_n(DeviceMesh &m, double t, hipStream_t s) {
+    using S = Shape<N>;
+    const int64_t blocks = (m.K + S::EPB - 1) / S::EPB;
+    hipLaunchKernelGGL((stage_kernel<N, true>), dim3((unsigned)blocks), dim3(S::THREADS), 0, s, m.verts,
+                       m.side_nb, m.bc, m.tab, m.q, m.g, m.trace[m.cur], nullptr, m.rhs, m.K, 1, 0.0, 0.0, t);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t ic_n(const DeviceMesh &m, int kind, int n_pulses, const double *pulses, hipStream_t s) {
+    const int64_t total = m.K * (N + 1) * (N + 1);
+    hipLaunchKernelGGL(ic_kernel<N>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, m.verts, m.tab, m.q,
+                       m.K, kind, n_pulses, pulses);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t estimate_n(const DeviceMesh &m, double tol_min, double tol_max, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(estimate_kernel<N>, dim3((unsigned)((m.K + 127) / 128)), dim3(128), 0, s, m.tab, m.q, m.K,
+                       tol_min, tol_max, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_traces(const DeviceMesh &m, hipStream_t s) {
+    switch (m.N) {
+#define X(n) case n: return traces_n<n>(m, s);
+        NDG_FOR_ORDERS(X)
+#undef X
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_stage(DeviceMesh &m, int stage, double t_stage, double a, double gdt, hipStream_t s) {
+    switch (m.N) {
+#define X(n) case n: return stage_n<n>(m, stage, t_stage, a, gdt, s);
+        NDG_FOR_ORDERS(X)
+#undef X
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_rhs(DeviceMesh &m, double t, hipStream_t s) {
+    switch (m.N) {
+#define X(n) case n: return rhs_n<n>(m, t, s);
+        NDG_FOR_ORDERS(X)
+#undef X
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_ic(const DeviceMesh &m, int kind, int n_pulses, const double *pulses, hipStream_t s) {
+    switch (m.N) {
+#define X(n) case n: return ic_n<n>(m, kind, n_pulses, pulses, s);
+        NDG_FOR_ORDERS(X)
+#undef X
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_estimate(const DeviceMesh &m, double tol_min, double tol_max, double *out, hipStream_t s) {
+    switch (m.N) {
+#define X(n) case n: return estimate_n<n>(m, tol_min, tol_max, out, s);
+        NDG_FOR_ORDERS(X)
+#undef X
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_delta_t(const DeviceMesh &m, double cfl, double *dt_dev, hipStream_t s) {
+    int blocks = (int)std::min<int64_t>((m.K + 255) / 256, 1024);
+    if (blocks < 1) blocks = 1;
+    if (m.scratch_len < blocks) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(dt_partial_kernel, dim3(blocks), dim3(256), 0, s, m.verts, m.K, m.N, cfl, m.scratch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(256), 0, s, m.scratch, blocks, dt_dev);
+    return hipGetLastError();
+}
+
+}  // namespace ndg

@@ -1,0 +1,233 @@
+"""Python mirror of the reference host API (SEM::Host / SEM::Device classes of NDG_cuda) over the
+C ABI of libndg_amd. Names, argument meaning and call order follow the reference:
+
+    NDG_t(N_max, n_interpolation_points)                   include/entities/NDG_t.h:11-26
+    Mesh2D_t(mesh, initial_N, maximum_N, n_interpolation_points, max_split_level,
+             adaptivity_interval, load_balancing_interval, tolerance_min, tolerance_max,
+             load_balancing_threshold, NDG)                 include/meshes/Mesh2D_t.h:21
+    mesh.initial_conditions(...)                             src/meshes/Mesh2D_t.cpp:1192-1216
+    Solver2D_t(CFL, output_times, viscosity)                 include/solvers/Solver2D_t.h:14-49
+    solver.solve(NDG, mesh)                                  include/solvers/Solver2D_t.tpp:7-135
+
+Where the reference reads a CGNS/SU2 file, Mesh2D_t takes a MeshDescription (the same content in
+memory); square_mesh() is the equivalent of the reference's mesh generator. Errors raise NDGError
+instead of exiting the process.
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+from ._abi import (BC_INFLOW, BC_NONE, BC_OUTFLOW, BC_PERIODIC, BC_SYMMETRY, BC_WALL,  # noqa: F401
+                   IC_PLANE_WAVE, IC_PULSES, NDGError, TABLE_NAMES, check, lib, ptr)
+
+
+class NDG_t:
+    """Legendre-Gauss tables for every order 0..N_max (FP64, computed on the host)."""
+
+    def __init__(self, N_max: int, n_interpolation_points: int):
+        self.N_max_ = int(N_max)
+        self.N_interpolation_points_ = int(n_interpolation_points)
+        h = ctypes.c_void_p()
+        check(lib().ndg_tables_create(self.N_max_, self.N_interpolation_points_, ctypes.byref(h)))
+        self._h = h
+        for which, name in enumerate(TABLE_NAMES):
+            setattr(self, name + "_", [self._get(which, N) for N in range(self.N_max_ + 1)])
+
+    def _get(self, which, N):
+        count = ctypes.c_int64()
+        check(lib().ndg_tables_get(self._h, which, N, None, 0, ctypes.byref(count)))
+        out = np.empty(count.value, dtype=np.float64)
+        check(lib().ndg_tables_get(self._h, which, N, ptr(out), out.size, None))
+        return out
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _abi._lib is not None:
+            _abi._lib.ndg_tables_destroy(h)
+            self._h = None
+
+
+@dataclass
+class MeshDescription:
+    """Nodes, QUAD_4 elements, BAR_2 boundary elements with kinds, periodic donors."""
+    xy: np.ndarray                       # (n_nodes, 2)
+    quads: np.ndarray                    # (n_quads, 4) int64, counter-clockwise
+    bars: np.ndarray                     # (n_bars, 2) int64
+    bar_type: np.ndarray                 # (n_bars,) int32, BC_*
+    bar_donor: np.ndarray                # (n_bars,) int64 (periodic donor bar or -1)
+    quad_N: Optional[np.ndarray] = None  # (n_quads,) int32 or None
+
+    @property
+    def n_elements(self):
+        return int(self.quads.shape[0])
+
+
+def square_mesh(resolution: int, x_min=-1.0, x_max=1.0, y_min=-1.0, y_max=1.0,
+                bc: Sequence[int] = (BC_WALL, BC_WALL, BC_WALL, BC_WALL),
+                x_periodic=False, y_periodic=False, jitter=0.0, seed=0) -> MeshDescription:
+    """Square mesh in Hilbert-curve order (extras/mesh_generator/main.cpp:161-382); bc is
+    {bottom, right, top, left}; jitter moves interior vertices by a seeded uniform fraction of
+    the spacing."""
+    p = _abi.SquareMeshParams()
+    p.resolution = int(resolution)
+    p.x_min, p.x_max, p.y_min, p.y_max = float(x_min), float(x_max), float(y_min), float(y_max)
+    for k in range(4):
+        p.bc[k] = int(bc[k])
+    p.x_periodic, p.y_periodic = int(bool(x_periodic)), int(bool(y_periodic))
+    p.jitter, p.seed = float(jitter), int(seed)
+    nn, nq, nb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    check(lib().ndg_square_mesh_sizes(ctypes.byref(p), ctypes.byref(nn), ctypes.byref(nq), ctypes.byref(nb)))
+    xy = np.empty((nn.value, 2), dtype=np.float64)
+    quads = np.empty((nq.value, 4), dtype=np.int64)
+    bars = np.empty((nb.value, 2), dtype=np.int64)
+    bar_type = np.empty(nb.value, dtype=np.int32)
+    bar_donor = np.empty(nb.value, dtype=np.int64)
+    check(lib().ndg_square_mesh_fill(ctypes.byref(p), ptr(xy), ptr(quads), ptr(bars), ptr(bar_type), ptr(bar_donor)))
+    return MeshDescription(xy, quads, bars, bar_type, bar_donor)
+
+
+class Mesh2D_t:
+    """Device-resident mesh and solution (one HIP stream)."""
+
+    def __init__(self, mesh: MeshDescription, initial_N: int, maximum_N: int, n_interpolation_points: int,
+                 max_split_level: int, adaptivity_interval: int, load_balancing_interval: int,
+                 tolerance_min: float, tolerance_max: float, load_balancing_threshold: float, NDG: NDG_t,
+                 worker_weights=None):
+        if adaptivity_interval:
+            raise NDGError(_abi.NDG_ERR_UNSUPPORTED, "adaptivity is not available in this build")
+        self.initial_N_ = int(initial_N)
+        self.maximum_N_ = int(maximum_N)
+        self.n_interpolation_points_ = int(n_interpolation_points)
+        self.max_split_level_ = int(max_split_level)
+        self.adaptivity_interval_ = int(adaptivity_interval)
+        self.load_balancing_interval_ = int(load_balancing_interval)
+        self.tolerance_min_ = float(tolerance_min)
+        self.tolerance_max_ = float(tolerance_max)
+        self._ndg = NDG
+        self._arrays = [_abi.as_f64(mesh.xy), _abi.as_i64(mesh.quads), _abi.as_i64(mesh.bars),
+                        _abi.as_i32(mesh.bar_type), _abi.as_i64(mesh.bar_donor),
+                        None if mesh.quad_N is None else _abi.as_i32(mesh.quad_N)]
+        xy, quads, bars, bar_type, bar_donor, quad_N = self._arrays
+        d = _abi.MeshDesc()
+        d.n_nodes, d.xy = xy.shape[0], ptr(xy)
+        d.n_quads, d.quads, d.quad_N = quads.shape[0], ptr(quads), ptr(quad_N)
+        d.n_bars, d.bars, d.bar_type, d.bar_donor = bars.shape[0], ptr(bars), ptr(bar_type), ptr(bar_donor)
+        d.initial_N, d.maximum_N, d.max_split_level = self.initial_N_, self.maximum_N_, self.max_split_level_
+        d.tolerance_min, d.tolerance_max = self.tolerance_min_, self.tolerance_max_
+        h = ctypes.c_void_p()
+        check(lib().ndg_mesh_create(ctypes.byref(d), NDG.handle, ctypes.byref(h)))
+        self._h = h
+        ne, nf, nd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        nmin, nmax = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().ndg_mesh_info(h, ctypes.byref(ne), ctypes.byref(nf), ctypes.byref(nd), ctypes.byref(nmin),
+                                  ctypes.byref(nmax)))
+        self.n_elements_ = ne.value
+        self.n_faces_ = nf.value
+        self.n_dofs_per_field_ = nd.value
+        self.N_ = nmin.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def n_dofs(self):
+        """DOF = 3 * sum over elements of (N+1)^2."""
+        return 3 * self.n_dofs_per_field_
+
+    def set_stream(self, hip_stream_ptr: int):
+        check(lib().ndg_mesh_set_stream(self._h, ctypes.c_void_p(hip_stream_ptr)))
+
+    def initial_conditions(self, pulses=None):
+        """Reference plane wave g(x, 0) when pulses is None, else radial Gaussian pulses given as
+        rows {x0, y0, width, amplitude}."""
+        ic = _abi.ICDesc()
+        arr = None
+        if pulses is None:
+            ic.kind = IC_PLANE_WAVE
+        else:
+            arr = _abi.as_f64(np.asarray(pulses, dtype=np.float64).reshape(-1, 4))
+            ic.kind, ic.n_pulses, ic.pulses = IC_PULSES, arr.shape[0], ptr(arr)
+        check(lib().ndg_set_ic(self._h, ctypes.byref(ic)))
+
+    def set_solution(self, p, u, v):
+        p, u, v = (_abi.as_f64(a) for a in (p, u, v))
+        for a in (p, u, v):
+            if a.size != self.n_dofs_per_field_:
+                raise NDGError(_abi.NDG_ERR_INVALID, "set_solution: wrong field length")
+        check(lib().ndg_set_state(self._h, ptr(p), ptr(u), ptr(v)))
+
+    def get_solution(self):
+        out = [np.empty(self.n_dofs_per_field_, dtype=np.float64) for _ in range(3)]
+        check(lib().ndg_get_state(self._h, *(ptr(a) for a in out)))
+        return tuple(out)
+
+    def interpolate_to_boundaries(self):
+        """Element traces of the current state, shape (K, 4, 3, N+1)."""
+        out = np.empty((self.n_elements_, 4, 3, self.N_ + 1), dtype=np.float64)
+        check(lib().ndg_get_traces(self._h, ptr(out)))
+        return out
+
+    def compute_rhs(self, t=0.0):
+        out = [np.empty(self.n_dofs_per_field_, dtype=np.float64) for _ in range(3)]
+        check(lib().ndg_compute_rhs(self._h, float(t), *(ptr(a) for a in out)))
+        return tuple(out)
+
+    def estimate_error(self):
+        K = self.n_elements_
+        err, sig = np.empty(K), np.empty(K)
+        ref, coa = np.empty(K, dtype=np.int32), np.empty(K, dtype=np.int32)
+        check(lib().ndg_estimate_error(self._h, ptr(err), ptr(sig), ptr(ref), ptr(coa)))
+        return err, sig, ref.astype(bool), coa.astype(bool)
+
+    def synchronize(self):
+        check(lib().ndg_synchronize(self._h))
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _abi._lib is not None:
+            _abi._lib.ndg_mesh_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Solver2D_t:
+    """Low-storage RK3 time stepping on the device (include/solvers/Solver2D_t.h:14-49)."""
+
+    def __init__(self, CFL: float, output_times: Sequence[float] = (), viscosity: float = 0.0):
+        self.CFL_ = float(CFL)
+        self.output_times_ = [float(t) for t in output_times]
+        self.viscosity_ = float(viscosity)
+        self.time_ = 0.0
+        self.n_steps_ = 0
+
+    def get_delta_t(self, mesh: Mesh2D_t) -> float:
+        dt = ctypes.c_double()
+        check(lib().ndg_get_delta_t(mesh.handle, self.CFL_, ctypes.byref(dt)))
+        return dt.value
+
+    def solve(self, NDG: NDG_t, mesh: Mesh2D_t):
+        """Advances to output_times[-1] with the reference's step clipping; returns the number of
+        steps taken."""
+        t_final, steps = ctypes.c_double(), ctypes.c_int64()
+        check(lib().ndg_solve(mesh.handle, self.CFL_, self.output_times_[-1], ctypes.byref(t_final),
+                              ctypes.byref(steps)))
+        self.time_, self.n_steps_ = t_final.value, steps.value
+        return steps.value
+
+    def step(self, mesh: Mesh2D_t, t: float, dt: float):
+        check(lib().ndg_step(mesh.handle, float(t), float(dt)))
+
+    def run_steps(self, mesh: Mesh2D_t, n_steps: int, time: float = 0.0) -> float:
+        t = ctypes.c_double(time)
+        check(lib().ndg_run_steps(mesh.handle, self.CFL_, int(n_steps), ctypes.byref(t)))
+        return t.value

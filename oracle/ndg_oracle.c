@@ -1,0 +1,1286 @@
+/*
+ * ndg_oracle.c -- CPU restatement of the reference DG-SEM wave solver.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle): loaded by the tests, the smoke check and the CPU-baseline
+ * leg of the benchmark, as the checker. Nothing in the product links or calls it.
+ *
+ * Every routine follows one reference routine (cited as file:line into the reference checkout) in
+ * its operation order, so that an x86-64 build without FMA contraction reproduces the reference
+ * host path's arithmetic. Build with -ffp-contract=off.
+ *
+ * Parity pinning (how each part is checked; see tests/test_oracle_*.py):
+ *  - tables (nodes, weights, barycentric weights, l(+-1), l'(+-1), D, D^, G^, interpolation,
+ *    modal matrix): bitwise against fixtures emitted by the compiled reference sources
+ *    (oracle/_ref), plus numpy leggauss, the reference's N=16 known answers and identities.
+ *  - element traces, element/boundary geometry, face geometry, estimator, Hilbert order:
+ *    bitwise against compiled-reference fixtures.
+ *  - upwind flux: closed form (characteristic decomposition) on random states.
+ *  - volume + lift (compute_rhs): free-stream preservation on jittered meshes, exactness of the
+ *    discrete operator on polynomial fields, linearity, antisymmetry of the face coupling.
+ *  - stage + time loop: the reference plane wave with inflow boundaries against its exact
+ *    solution, spectral convergence in N, the periodic Fourier-series exact solution for radial
+ *    pulses, energy decay, and the RK3 amplification factor on a decoupled mode.
+ *  - mortar / p-nonconforming branches: L2-projection exactness for polynomials of degree
+ *    <= min(N_face, N_element) and conservation of the face flux integral.
+ *  The reference's mesh reader and stage phases (src/meshes/Mesh2D_t.cpp, src/solvers/
+ *  Solver2D_t.cpp) cannot be compiled here (they include VTK headers absent from the image), so
+ *  the stage-level functions are pinned by the mathematics above rather than by fixtures.
+ */
+#include "ndg_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* include/helpers/constants.h:15-19 */
+static const double C_WAVE = 1.0;
+static const double XY0_X = -0.2, XY0_Y = -0.2;
+static const double K_X = 0.707106781186548, K_Y = 0.707106781186548;
+static const double D_WIDTH = 0.120112240878645;
+static const int N_LSQ_MAX = 4;
+static const double PI_REF = 3.14159265358979323846;
+
+/* ------------------------------------------------------------------------------------------ */
+/* Polynomials: src/polynomials/LegendrePolynomial_t.cpp                                       */
+/* ------------------------------------------------------------------------------------------ */
+
+/* LegendrePolynomial_t.cpp:8-32 (Kopriva algorithm 22) */
+static void legendre_and_derivative(int N, double x, double *L_N, double *L_N_prime) {
+    if (N == 0) {
+        *L_N = 1.0;
+        *L_N_prime = 0.0;
+    } else if (N == 1) {
+        *L_N = x;
+        *L_N_prime = 1.0;
+    } else {
+        double L_N_2 = 1.0, L_N_1 = x, L_N_2_prime = 0.0, L_N_1_prime = 1.0;
+        for (int k = 2; k <= N; ++k) {
+            *L_N = (2 * k - 1) * x * L_N_1 / k - (k - 1) * L_N_2 / k;
+            *L_N_prime = L_N_2_prime + (2 * k - 1) * L_N_1;
+            L_N_2 = L_N_1;
+            L_N_1 = *L_N;
+            L_N_2_prime = L_N_1_prime;
+            L_N_1_prime = *L_N_prime;
+        }
+    }
+}
+
+/* LegendrePolynomial_t.cpp:34-58 */
+static double legendre(int N, double x) {
+    if (N == 0) return 1.0;
+    if (N == 1) return x;
+    double L_N_2 = 1.0, L_N_1 = x, L_N = 0.0;
+    for (int k = 2; k <= N; ++k) {
+        L_N = (2 * k - 1) * x * L_N_1 / k - (k - 1) * L_N_2 / k;
+        L_N_2 = L_N_1;
+        L_N_1 = L_N;
+    }
+    return L_N;
+}
+
+/* LegendrePolynomial_t.cpp:60-101 (Kopriva algorithm 23): Newton on L_{N+1}, at most 1000
+ * iterations, symmetric fill, centre node 0 for even N. */
+static void nodes_and_weights(int N, double *nodes, double *weights) {
+    for (int i = 0; i < (N + 1) / 2; ++i) {
+        if (N == 1) {
+            nodes[0] = -sqrt(1.0 / 3.0);
+            weights[0] = 1.0;
+            nodes[1] = -nodes[0];
+            weights[1] = weights[0];
+        } else {
+            nodes[i] = -cos(PI_REF * (2 * i + 1) / (2 * N + 2));
+            for (int k = 0; k < 1000; ++k) {
+                double L, Lp;
+                legendre_and_derivative(N + 1, nodes[i], &L, &Lp);
+                const double delta = -L / Lp;
+                nodes[i] += delta;
+                if (fabs(delta) <= DBL_MIN * 2 * fabs(nodes[i])) break;
+            }
+            double dummy, Lp_final;
+            legendre_and_derivative(N + 1, nodes[i], &dummy, &Lp_final);
+            nodes[N - i] = -nodes[i];
+            weights[i] = 2.0 / ((1 - pow(nodes[i], 2)) * pow(Lp_final, 2));
+            weights[N - i] = weights[i];
+        }
+    }
+    if (N == 0) {
+        nodes[0] = 0.0;
+        weights[0] = 2.0;
+    }
+    if (N % 2 == 0) {
+        double dummy, Lp_final;
+        legendre_and_derivative(N + 1, 0.0, &dummy, &Lp_final);
+        nodes[N / 2] = 0.0;
+        weights[N / 2] = 2 / pow(Lp_final, 2);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* NDG tables: include/entities/NDG_t.tpp                                                      */
+/* ------------------------------------------------------------------------------------------ */
+
+struct o_tables {
+    int N_max, n_interp;
+    double **tab[12];
+};
+
+/* NDG_t.tpp:193-199 and src/meshes/Mesh2D_t.cpp:3784-3790 (cppreference almost_equal, 2 ulp) */
+static int almost_equal(double x, double y) {
+    return fabs(x - y) <= DBL_EPSILON * fabs(x + y) * 2 || fabs(x - y) < DBL_MIN;
+}
+
+static int tab_len(const o_tables *t, int which, int N) {
+    const int nf = N + 1;
+    if (which == O_D || which == O_GHAT || which == O_DHAT || which == O_POLY) return nf * nf;
+    if (which == O_INTERP) return nf * t->n_interp;
+    return nf;
+}
+
+o_tables *o_tables_create(int N_max, int n_interp) {
+    o_tables *t = (o_tables *)calloc(1, sizeof(o_tables));
+    t->N_max = N_max;
+    t->n_interp = n_interp;
+    for (int w = 0; w < 12; ++w) {
+        t->tab[w] = (double **)calloc((size_t)N_max + 1, sizeof(double *));
+        for (int N = 0; N <= N_max; ++N) t->tab[w][N] = (double *)calloc((size_t)tab_len(t, w, N), sizeof(double));
+    }
+    /* NDG_t.tpp:42-58: the constructor's per-N sequence */
+    for (int N = 0; N <= N_max; ++N) {
+        const int nf = N + 1;
+        double *x = t->tab[O_NODES][N], *w = t->tab[O_WEIGHTS][N], *lam = t->tab[O_BARY][N];
+        nodes_and_weights(N, x, w);
+        /* NDG_t.tpp:175-188 calculate_barycentric_weights */
+        for (int j = 0; j <= N; ++j) {
+            double xjxi = 1.0;
+            for (int i = 0; i < j; ++i) xjxi *= x[j] - x[i];
+            for (int i = j + 1; i <= N; ++i) xjxi *= x[j] - x[i];
+            lam[j] = 1.0 / xjxi;
+        }
+        /* NDG_t.tpp:327-334 calculate_polynomials */
+        double *M = t->tab[O_POLY][N];
+        for (int i = 0; i <= N; ++i)
+            for (int j = 0; j <= N; ++j) M[i * nf + j] = legendre(i, x[j]) * w[j] * (2 * i + 1) * 0.5;
+        /* NDG_t.tpp:251-259 polynomial_derivative_matrices (off-diagonal) */
+        double *D = t->tab[O_D][N];
+        for (int i = 0; i <= N; ++i)
+            for (int j = 0; j <= N; ++j)
+                if (i != j) D[i * nf + j] = lam[j] / (lam[i] * (x[i] - x[j]));
+        /* NDG_t.tpp:302-325 create_interpolation_matrices */
+        double *I = t->tab[O_INTERP][N];
+        for (int j = 0; j < n_interp; ++j) {
+            int row_has_match = 0;
+            const double xc = 2.0 * j / (n_interp - 1) - 1.0;
+            for (int k = 0; k <= N; ++k) {
+                I[j * nf + k] = 0.0;
+                if (almost_equal(xc, x[k])) {
+                    I[j * nf + k] = 1.0;
+                    row_has_match = 1;
+                }
+            }
+            if (!row_has_match) {
+                double total = 0.0;
+                for (int k = 0; k <= N; ++k) {
+                    I[j * nf + k] = lam[k] / (xc - x[k]);
+                    total += I[j * nf + k];
+                }
+                for (int k = 0; k <= N; ++k) I[j * nf + k] /= total;
+            }
+        }
+        /* NDG_t.tpp:209-249 lagrange interpolants at -1/+1 and their derivatives, normalised */
+        double *ll = t->tab[O_LAG_LEFT][N], *lr = t->tab[O_LAG_RIGHT][N];
+        double *dl = t->tab[O_DLAG_LEFT][N], *dr = t->tab[O_DLAG_RIGHT][N];
+        for (int i = 0; i <= N; ++i) ll[i] = lam[i] / (-1.0 - x[i]);
+        for (int i = 0; i <= N; ++i) lr[i] = lam[i] / (1.0 - x[i]);
+        for (int i = 0; i <= N; ++i) dl[i] = lam[i] / ((-1.0 - x[i]) * (-1.0 - x[i]));
+        for (int i = 0; i <= N; ++i) dr[i] = lam[i] / ((1.0 - x[i]) * (1.0 - x[i]));
+        double s = 0.0;
+        for (int i = 0; i <= N; ++i) s += ll[i];
+        for (int i = 0; i <= N; ++i) ll[i] /= s;
+        s = 0.0;
+        for (int i = 0; i <= N; ++i) s += lr[i];
+        for (int i = 0; i <= N; ++i) lr[i] /= s;
+        s = 0.0;
+        for (int i = 0; i <= N; ++i) s += lam[i] / (-1.0 - x[i]);
+        for (int i = 0; i <= N; ++i) dl[i] /= s;
+        s = 0.0;
+        for (int i = 0; i <= N; ++i) s += lam[i] / (1.0 - x[i]);
+        for (int i = 0; i <= N; ++i) dr[i] /= s;
+        /* NDG_t.tpp:262-274 diagonal by negative row sum */
+        for (int i = 0; i <= N; ++i) {
+            D[i * (N + 2)] = 0.0;
+            for (int j = 0; j < i; ++j) D[i * (N + 2)] -= D[i * nf + j];
+            for (int j = i + 1; j <= N; ++j) D[i * (N + 2)] -= D[i * nf + j];
+        }
+        /* NDG_t.tpp:286-298 polynomial_cg_derivative_matrices */
+        double *Gh = t->tab[O_GHAT][N];
+        for (int j = 0; j <= N; ++j)
+            for (int n = 0; n <= N; ++n) {
+                double sum = 0.0;
+                for (int k = 0; k <= N; ++k) sum += D[k * nf + n] * D[k * nf + j] * w[k];
+                Gh[j * nf + n] = sum / w[j];
+            }
+        /* NDG_t.tpp:276-283 polynomial_derivative_matrices_hat */
+        double *Dh = t->tab[O_DHAT][N];
+        for (int i = 0; i <= N; ++i)
+            for (int j = 0; j <= N; ++j) Dh[i * nf + j] = -D[j * nf + i] * w[j] / w[i];
+    }
+    return t;
+}
+
+void o_tables_destroy(o_tables *t) {
+    if (!t) return;
+    for (int w = 0; w < 12; ++w) {
+        for (int N = 0; N <= t->N_max; ++N) free(t->tab[w][N]);
+        free(t->tab[w]);
+    }
+    free(t);
+}
+
+int o_tables_get(const o_tables *t, int which, int N, double *out) {
+    if (which < 0 || which > 11 || N < 0 || N > t->N_max) return -1;
+    const int n = tab_len(t, which, N);
+    memcpy(out, t->tab[which][N], (size_t)n * sizeof(double));
+    return n;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Geometry functions: include/functions/{quad_map,quad_metrics,analytical_solution}.tpp        */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct { double x, y; } v2;
+
+/* quad_map.tpp:3-8 */
+static v2 quad_map(double xi, double eta, const v2 *P) {
+    v2 r;
+    r.x = (P[0].x * (1 - xi) * (1 - eta) + P[1].x * (xi + 1) * (1 - eta) + P[2].x * (xi + 1) * (eta + 1) +
+           P[3].x * (1 - xi) * (eta + 1)) / 4;
+    r.y = (P[0].y * (1 - xi) * (1 - eta) + P[1].y * (xi + 1) * (1 - eta) + P[2].y * (xi + 1) * (eta + 1) +
+           P[3].y * (1 - xi) * (eta + 1)) / 4;
+    return r;
+}
+
+/* quad_metrics.tpp:3-8: m[0] = {dx/dxi, dx/deta}, m[1] = {dy/dxi, dy/deta} */
+static void quad_metrics(double xi, double eta, const v2 *P, v2 *m) {
+    m[0].x = ((1 - eta) * (P[1].x - P[0].x) + (1 + eta) * (P[2].x - P[3].x)) / 4;
+    m[0].y = ((1 - xi) * (P[3].x - P[0].x) + (1 + xi) * (P[2].x - P[1].x)) / 4;
+    m[1].x = ((1 - eta) * (P[1].y - P[0].y) + (1 + eta) * (P[2].y - P[3].y)) / 4;
+    m[1].y = ((1 - xi) * (P[3].y - P[0].y) + (1 + xi) * (P[2].y - P[1].y)) / 4;
+}
+
+/* analytical_solution.tpp:5-11 */
+static void g_exact(double x, double y, double t, double *q) {
+    const double a = K_X * (x - XY0_X) + K_Y * (y - XY0_Y) - C_WAVE * t;
+    const double p = exp(-a * a / (D_WIDTH * D_WIDTH));
+    q[0] = p;
+    q[1] = p * K_X / C_WAVE;
+    q[2] = p * K_Y / C_WAVE;
+}
+
+static double mag(v2 a) { return sqrt(a.x * a.x + a.y * a.y); }
+static v2 sub(v2 a, v2 b) { v2 r = {a.x - b.x, a.y - b.y}; return r; }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Mesh model: Element2D_t / Face2D_t (include/entities/Element2D_t.h:12-127,                   */
+/* include/entities/Face2D_t.h:18-83)                                                          */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+    int N;
+    int64_t nodes[4];
+    int nf[4];             /* number of faces per side */
+    int64_t *faces[4];
+    double *block;         /* one allocation for all arrays below */
+    double *dxi_dx, *deta_dx, *dxi_dy, *deta_dy, *jac; /* Ne each */
+    double *sf[4];
+    double dmin, cx, cy;
+    double *p, *u, *v, *Gp, *Gu, *Gv, *ip, *iu, *iv;   /* Ne each */
+    double *pe[4], *ue[4], *ve[4];                    /* traces */
+    double *pf[4], *uf[4], *vf[4];                    /* flux traces */
+    double *pflux, *uflux, *vflux, *pder, *uder, *vder; /* line scratch */
+    double error, sigma;
+    int refine, coarsen;
+} o_elem;
+
+typedef struct {
+    int N;
+    int64_t nodes[2];
+    int64_t elements[2];
+    int64_t sides[2];
+    double nx, ny, tx, ty, length, offset[2], scale[2];
+    double *block;
+    double *p[2], *u[2], *v[2], *pflux, *uflux, *vflux;
+} o_face;
+
+struct o_mesh {
+    const o_tables *t;
+    int64_t n_nodes;
+    v2 *xy;
+    int64_t K, n_elements_total;
+    o_elem *el;
+    int64_t F;
+    o_face *fa;
+    int64_t n_wall, n_sym, n_in, n_out, n_if;
+    int64_t *wall, *sym, *in, *out, *if_origin, *if_side, *if_dest;
+    double tol_min, tol_max;
+};
+
+static void elem_alloc_domain(o_elem *e) {
+    const int nf = e->N + 1, ne = nf * nf;
+    const size_t total = (size_t)ne * 14 + (size_t)nf * (4 + 12 + 12 + 6);
+    e->block = (double *)calloc(total, sizeof(double));
+    double *b = e->block;
+    e->dxi_dx = b; b += ne; e->deta_dx = b; b += ne; e->dxi_dy = b; b += ne; e->deta_dy = b; b += ne;
+    e->jac = b; b += ne;
+    e->p = b; b += ne; e->u = b; b += ne; e->v = b; b += ne;
+    e->Gp = b; b += ne; e->Gu = b; b += ne; e->Gv = b; b += ne;
+    e->ip = b; b += ne; e->iu = b; b += ne; e->iv = b; b += ne;
+    for (int s = 0; s < 4; ++s) { e->sf[s] = b; b += nf; }
+    for (int s = 0; s < 4; ++s) { e->pe[s] = b; b += nf; e->ue[s] = b; b += nf; e->ve[s] = b; b += nf; }
+    for (int s = 0; s < 4; ++s) { e->pf[s] = b; b += nf; e->uf[s] = b; b += nf; e->vf[s] = b; b += nf; }
+    e->pflux = b; b += nf; e->uflux = b; b += nf; e->vflux = b; b += nf;
+    e->pder = b; b += nf; e->uder = b; b += nf; e->vder = b; b += nf;
+}
+
+/* Element2D_t.cpp:711-733 allocate_boundary_storage: side 0 only */
+static void elem_alloc_ghost(o_elem *e) {
+    const int nf = e->N + 1;
+    e->block = (double *)calloc((size_t)nf * 4, sizeof(double));
+    double *b = e->block;
+    e->sf[0] = b; b += nf;
+    e->pe[0] = b; b += nf; e->ue[0] = b; b += nf; e->ve[0] = b; b += nf;
+}
+
+static void face_alloc(o_face *f) {
+    const int nf = f->N + 1;
+    f->block = (double *)calloc((size_t)nf * 9, sizeof(double));
+    double *b = f->block;
+    for (int s = 0; s < 2; ++s) { f->p[s] = b; b += nf; f->u[s] = b; b += nf; f->v[s] = b; b += nf; }
+    f->pflux = b; b += nf; f->uflux = b; b += nf; f->vflux = b;
+}
+
+/* Element2D_t.cpp:743-777 compute_geometry */
+static void element_geometry(o_elem *e, const v2 *P, const double *x) {
+    const int N = e->N, nf = N + 1;
+    for (int i = 0; i <= N; ++i) {
+        for (int j = 0; j <= N; ++j) {
+            v2 m[2];
+            quad_metrics(x[i], x[j], P, m);
+            e->dxi_dx[i * nf + j] = m[0].x;
+            e->deta_dx[i * nf + j] = m[0].y;
+            e->dxi_dy[i * nf + j] = m[1].x;
+            e->deta_dy[i * nf + j] = m[1].y;
+            e->jac[i * nf + j] = m[0].x * m[1].y - m[0].y * m[1].x;
+        }
+        v2 mb[2], mr[2], mt[2], ml[2];
+        quad_metrics(x[i], -1.0, P, mb);
+        quad_metrics(1.0, x[i], P, mr);
+        quad_metrics(x[i], 1.0, P, mt);
+        quad_metrics(-1.0, x[i], P, ml);
+        e->sf[0][i] = sqrt(mb[0].x * mb[0].x + mb[1].x * mb[1].x);
+        e->sf[1][i] = sqrt(mr[0].y * mr[0].y + mr[1].y * mr[1].y);
+        e->sf[2][i] = sqrt(mt[0].x * mt[0].x + mt[1].x * mt[1].x);
+        e->sf[3][i] = sqrt(ml[0].y * ml[0].y + ml[1].y * ml[1].y);
+    }
+    const double a = fmin(mag(sub(P[1], P[0])), mag(sub(P[2], P[3])));
+    const double b = fmin(mag(sub(P[1], P[2])), mag(sub(P[0], P[3])));
+    const double c = fmin(mag(sub(P[1], P[3])), mag(sub(P[2], P[0])));
+    e->dmin = fmin(fmin(a, b), c);
+    e->cx = (P[0].x + P[1].x + P[2].x + P[3].x) / 4;
+    e->cy = (P[0].y + P[1].y + P[2].y + P[3].y) / 4;
+}
+
+/* Element2D_t.cpp:779-790 compute_boundary_geometry */
+static void boundary_geometry(o_elem *e, const v2 *P, const double *x) {
+    for (int i = 0; i <= e->N; ++i) {
+        v2 mb[2];
+        quad_metrics(x[i], -1.0, P, mb);
+        e->sf[0][i] = sqrt(mb[0].x * mb[0].x + mb[1].x * mb[1].x);
+    }
+    e->dmin = mag(sub(P[1], P[0]));
+    e->cx = (P[0].x + P[1].x) / 2;
+    e->cy = (P[0].y + P[1].y) / 2;
+}
+
+/* Face2D_t.cpp:56-81 compute_geometry. en = {L side start, L side end, R side start, R side end} */
+static void face_geometry(o_face *f, const v2 *fn, const v2 *en) {
+    v2 t = sub(fn[1], fn[0]);
+    f->length = mag(t);
+    t.x /= f->length;
+    t.y /= f->length;
+    v2 n = {t.y, -t.x};
+    const v2 lt = sub(en[1], en[0]);
+    const double sign = copysign(1.0, t.x * lt.x + t.y * lt.y);
+    n.x *= sign; n.y *= sign;
+    t.x *= sign; t.y *= sign;
+    f->nx = n.x; f->ny = n.y; f->tx = t.x; f->ty = t.y;
+    const v2 d0 = sub(en[1], en[0]), d1 = sub(en[3], en[2]);
+    const v2 fd = sub(fn[1], fn[0]);
+    const v2 ed0 = sub(fn[0], en[0]), ed1 = sub(fn[1], en[2]);
+    f->offset[0] = mag(ed0) / mag(d0);
+    f->offset[1] = mag(ed1) / mag(d1);
+    f->scale[0] = mag(fd) / mag(d0);
+    f->scale[1] = mag(fd) / mag(d1);
+}
+
+static void face_add_to_node(int64_t **lists, int *counts, int *caps, int64_t node, int64_t face) {
+    if (counts[node] == caps[node]) {
+        caps[node] = caps[node] ? 2 * caps[node] : 4;
+        lists[node] = (int64_t *)realloc(lists[node], (size_t)caps[node] * sizeof(int64_t));
+    }
+    lists[node][counts[node]++] = face;
+}
+
+static void push(int64_t **arr, int64_t *n, int64_t v) {
+    *arr = (int64_t *)realloc(*arr, (size_t)(*n + 1) * sizeof(int64_t));
+    (*arr)[(*n)++] = v;
+}
+
+o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const double *xy,
+                      int64_t n_quads, const int64_t *quads, const int32_t *quad_N, int64_t n_bars,
+                      const int64_t *bars, const int32_t *bar_type, const int64_t *bar_donor,
+                      double tol_min, double tol_max) {
+    o_mesh *m = (o_mesh *)calloc(1, sizeof(o_mesh));
+    m->t = t;
+    m->tol_min = tol_min;
+    m->tol_max = tol_max;
+    m->n_nodes = n_nodes;
+    m->xy = (v2 *)malloc((size_t)n_nodes * sizeof(v2));
+    for (int64_t i = 0; i < n_nodes; ++i) { m->xy[i].x = xy[2 * i]; m->xy[i].y = xy[2 * i + 1]; }
+    m->K = n_quads;
+    m->n_elements_total = n_quads + n_bars;
+    m->el = (o_elem *)calloc((size_t)m->n_elements_total, sizeof(o_elem));
+    /* src/meshes/Mesh2D_t.cpp:676-707: domain elements, then ghost elements with nodes {a,b,b,a} */
+    for (int64_t i = 0; i < n_quads; ++i) {
+        o_elem *e = &m->el[i];
+        e->N = quad_N ? quad_N[i] : initial_N;
+        for (int k = 0; k < 4; ++k) e->nodes[k] = quads[4 * i + k];
+    }
+    for (int64_t b = 0; b < n_bars; ++b) {
+        o_elem *e = &m->el[n_quads + b];
+        e->N = initial_N;
+        e->nodes[0] = bars[2 * b];
+        e->nodes[1] = bars[2 * b + 1];
+        e->nodes[2] = bars[2 * b + 1];
+        e->nodes[3] = bars[2 * b];
+    }
+    /* src/meshes/Mesh2D_t.cpp:1110-1190 build_faces */
+    int64_t **n2f = (int64_t **)calloc((size_t)n_nodes, sizeof(int64_t *));
+    int *n2f_count = (int *)calloc((size_t)n_nodes, sizeof(int));
+    int *n2f_cap = (int *)calloc((size_t)n_nodes, sizeof(int));
+    int64_t cap = 2 * n_quads + n_bars + 4;
+    m->fa = (o_face *)calloc((size_t)cap, sizeof(o_face));
+    int64_t (*e2f)[4] = (int64_t(*)[4])malloc((size_t)m->n_elements_total * sizeof(int64_t[4]));
+    m->F = 0;
+    for (int64_t i = 0; i < n_quads; ++i) {
+        for (int j = 0; j < 4; ++j) {
+            const int64_t a = m->el[i].nodes[j], b = m->el[i].nodes[(j + 1) % 4];
+            int found = 0;
+            for (int k = 0; k < n2f_count[a]; ++k) {
+                o_face *f = &m->fa[n2f[a][k]];
+                if (f->nodes[0] == b && f->nodes[1] == a) {
+                    found = 1;
+                    f->elements[1] = i;
+                    f->sides[1] = j;
+                    e2f[i][j] = n2f[a][k];
+                    break;
+                }
+            }
+            if (!found) {
+                if (m->F == cap) {
+                    cap *= 2;
+                    m->fa = (o_face *)realloc(m->fa, (size_t)cap * sizeof(o_face));
+                }
+                e2f[i][j] = m->F;
+                face_add_to_node(n2f, n2f_count, n2f_cap, a, m->F);
+                if (b != a) face_add_to_node(n2f, n2f_count, n2f_cap, b, m->F);
+                o_face *f = &m->fa[m->F++];
+                memset(f, 0, sizeof(*f));
+                f->N = initial_N;
+                f->nodes[0] = a; f->nodes[1] = b;
+                f->elements[0] = i; f->elements[1] = -1;
+                f->sides[0] = j; f->sides[1] = -1;
+            }
+        }
+    }
+    for (int64_t i = n_quads; i < m->n_elements_total; ++i) {
+        o_elem *e = &m->el[i];
+        const int64_t a = e->nodes[0], b = e->nodes[1];
+        e2f[i][0] = -1;
+        for (int k = 0; k < n2f_count[a]; ++k) {
+            o_face *f = &m->fa[n2f[a][k]];
+            if (f->nodes[0] == b && f->nodes[1] == a) {
+                f->elements[1] = i; f->sides[1] = 0; e2f[i][0] = n2f[a][k];
+                break;
+            }
+            if (f->nodes[0] == a && f->nodes[1] == b) {
+                f->elements[1] = i; f->sides[1] = 0; e2f[i][0] = n2f[a][k];
+                e->nodes[0] = b; e->nodes[1] = a; /* "inside out" boundary element */
+                break;
+            }
+        }
+    }
+    for (int64_t i = 0; i < n_nodes; ++i) free(n2f[i]);
+    free(n2f); free(n2f_count); free(n2f_cap);
+
+    /* boundary lists (src/meshes/Mesh2D_t.cpp:715-791) and self interfaces (:820-875) */
+    for (int64_t b = 0; b < n_bars; ++b) {
+        const int64_t g = n_quads + b;
+        switch (bar_type[b]) {
+            case O_BC_WALL: push(&m->wall, &m->n_wall, g); break;
+            case O_BC_SYMMETRY: push(&m->sym, &m->n_sym, g); break;
+            case O_BC_INFLOW: push(&m->in, &m->n_in, g); break;
+            case O_BC_OUTFLOW: push(&m->out, &m->n_out, g); break;
+            case O_BC_PERIODIC: {
+                const int64_t donor_ghost = n_quads + bar_donor[b];
+                const int64_t fi = e2f[donor_ghost][0];
+                const int side_index = m->fa[fi].elements[0] == donor_ghost;
+                int64_t tmp = m->n_if;
+                push(&m->if_origin, &tmp, m->fa[fi].elements[side_index]);
+                tmp = m->n_if;
+                push(&m->if_side, &tmp, m->fa[fi].sides[side_index]);
+                push(&m->if_dest, &m->n_if, g);
+                break;
+            }
+            default: break;
+        }
+    }
+
+    /* orders after (p-)adaptation semantics: ghost N follows its source (src/meshes/Mesh2D_t.cpp:
+     * 7482-7520), face N = max(N_L, N_R) (:7522-7538) */
+    for (int64_t b = 0; b < n_bars; ++b) {
+        const int64_t g = n_quads + b;
+        const int64_t fi = e2f[g][0];
+        if (fi < 0) continue;
+        const int64_t src = m->fa[fi].elements[0];
+        int Ng = m->el[g].N;
+        if (bar_type[b] == O_BC_PERIODIC) continue;
+        if (m->el[src].N > Ng) Ng = m->el[src].N;
+        m->el[g].N = Ng;
+    }
+    for (int64_t k = 0; k < m->n_if; ++k) m->el[m->if_dest[k]].N = m->el[m->if_origin[k]].N;
+    for (int64_t f = 0; f < m->F; ++f) {
+        o_face *fc = &m->fa[f];
+        const int NL = m->el[fc->elements[0]].N;
+        const int NR = fc->elements[1] >= 0 ? m->el[fc->elements[1]].N : NL;
+        fc->N = NL > NR ? NL : NR;
+    }
+
+    /* storage, face lists, geometry (Mesh2D_t.cpp:1017-1030, 4026-4052, 4200-4235) */
+    for (int64_t i = 0; i < m->n_elements_total; ++i) {
+        o_elem *e = &m->el[i];
+        if (i < n_quads) {
+            elem_alloc_domain(e);
+            for (int s = 0; s < 4; ++s) {
+                e->nf[s] = 1;
+                e->faces[s] = (int64_t *)malloc(sizeof(int64_t));
+                e->faces[s][0] = e2f[i][s];
+            }
+            v2 P[4];
+            for (int k = 0; k < 4; ++k) P[k] = m->xy[e->nodes[k]];
+            element_geometry(e, P, t->tab[O_NODES][e->N]);
+        } else {
+            elem_alloc_ghost(e);
+            e->nf[0] = 1;
+            e->faces[0] = (int64_t *)malloc(sizeof(int64_t));
+            e->faces[0][0] = e2f[i][0];
+            v2 P[4];
+            for (int k = 0; k < 4; ++k) P[k] = m->xy[e->nodes[k]];
+            boundary_geometry(e, P, t->tab[O_NODES][e->N]);
+        }
+    }
+    for (int64_t f = 0; f < m->F; ++f) {
+        o_face *fc = &m->fa[f];
+        face_alloc(fc);
+        const o_elem *L = &m->el[fc->elements[0]];
+        const o_elem *R = &m->el[fc->elements[1]];
+        const int sL = (int)fc->sides[0], sR = (int)fc->sides[1];
+        v2 fn[2] = {m->xy[fc->nodes[0]], m->xy[fc->nodes[1]]};
+        v2 en[4] = {m->xy[L->nodes[sL]], m->xy[L->nodes[(sL + 1) % 4]], m->xy[R->nodes[sR]],
+                    m->xy[R->nodes[(sR + 1) % 4]]};
+        face_geometry(fc, fn, en);
+    }
+    free(e2f);
+    return m;
+}
+
+void o_mesh_destroy(o_mesh *m) {
+    if (!m) return;
+    for (int64_t i = 0; i < m->n_elements_total; ++i) {
+        free(m->el[i].block);
+        for (int s = 0; s < 4; ++s) free(m->el[i].faces[s]);
+    }
+    for (int64_t f = 0; f < m->F; ++f) free(m->fa[f].block);
+    free(m->el); free(m->fa); free(m->xy);
+    free(m->wall); free(m->sym); free(m->in); free(m->out);
+    free(m->if_origin); free(m->if_side); free(m->if_dest);
+    free(m);
+}
+
+int64_t o_mesh_n_elements(const o_mesh *m) { return m->K; }
+int64_t o_mesh_n_faces(const o_mesh *m) { return m->F; }
+int64_t o_mesh_n_ghosts(const o_mesh *m) { return m->n_elements_total - m->K; }
+int64_t o_mesh_n_nodes_total(const o_mesh *m) {
+    int64_t n = 0;
+    for (int64_t i = 0; i < m->K; ++i) n += (int64_t)(m->el[i].N + 1) * (m->el[i].N + 1);
+    return n;
+}
+void o_mesh_get_N(const o_mesh *m, int32_t *N_out) {
+    for (int64_t i = 0; i < m->K; ++i) N_out[i] = m->el[i].N;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Initial conditions: src/meshes/Mesh2D_t.cpp:1192-1216                                       */
+/* ------------------------------------------------------------------------------------------ */
+
+static void ic_common(o_mesh *m, int kind, int n, const double *pulses) {
+    for (int64_t ei = 0; ei < m->K; ++ei) {
+        o_elem *e = &m->el[ei];
+        const int N = e->N, nf = N + 1;
+        const double *x = m->t->tab[O_NODES][N];
+        v2 P[4];
+        for (int k = 0; k < 4; ++k) P[k] = m->xy[e->nodes[k]];
+        for (int i = 0; i <= N; ++i)
+            for (int j = 0; j <= N; ++j) {
+                const v2 X = quad_map(x[i], x[j], P);
+                double q[3];
+                if (kind == 0) {
+                    g_exact(X.x, X.y, 0.0, q);
+                } else {
+                    q[0] = 0.0; q[1] = 0.0; q[2] = 0.0;
+                    for (int k = 0; k < n; ++k) {
+                        const double dx = X.x - pulses[4 * k], dy = X.y - pulses[4 * k + 1];
+                        const double w = pulses[4 * k + 2];
+                        q[0] += pulses[4 * k + 3] * exp(-M_LN2 * (dx * dx + dy * dy) / (w * w));
+                    }
+                }
+                e->p[i * nf + j] = q[0];
+                e->u[i * nf + j] = q[1];
+                e->v[i * nf + j] = q[2];
+                e->Gp[i * nf + j] = 0; e->Gu[i * nf + j] = 0; e->Gv[i * nf + j] = 0;
+                e->ip[i * nf + j] = 0; e->iu[i * nf + j] = 0; e->iv[i * nf + j] = 0;
+            }
+    }
+}
+
+void o_initial_conditions_plane(o_mesh *m) { ic_common(m, 0, 0, NULL); }
+void o_initial_conditions_pulses(o_mesh *m, int n, const double *pulses) { ic_common(m, 1, n, pulses); }
+
+void o_set_state(o_mesh *m, const double *p, const double *u, const double *v) {
+    int64_t off = 0;
+    for (int64_t i = 0; i < m->K; ++i) {
+        o_elem *e = &m->el[i];
+        const int ne = (e->N + 1) * (e->N + 1);
+        memcpy(e->p, p + off, (size_t)ne * sizeof(double));
+        memcpy(e->u, u + off, (size_t)ne * sizeof(double));
+        memcpy(e->v, v + off, (size_t)ne * sizeof(double));
+        off += ne;
+    }
+}
+
+static void get3(const o_mesh *m, int which, double *a, double *b, double *c) {
+    int64_t off = 0;
+    for (int64_t i = 0; i < m->K; ++i) {
+        const o_elem *e = &m->el[i];
+        const int ne = (e->N + 1) * (e->N + 1);
+        memcpy(a + off, which ? e->Gp : e->p, (size_t)ne * sizeof(double));
+        memcpy(b + off, which ? e->Gu : e->u, (size_t)ne * sizeof(double));
+        memcpy(c + off, which ? e->Gv : e->v, (size_t)ne * sizeof(double));
+        off += ne;
+    }
+}
+void o_get_state(const o_mesh *m, double *p, double *u, double *v) { get3(m, 0, p, u, v); }
+void o_get_rhs(const o_mesh *m, double *Gp, double *Gu, double *Gv) { get3(m, 1, Gp, Gu, Gv); }
+
+static void get_sides(const o_mesh *m, int flux, double *out) {
+    int64_t off = 0;
+    for (int64_t i = 0; i < m->K; ++i) {
+        const o_elem *e = &m->el[i];
+        const int nf = e->N + 1;
+        for (int s = 0; s < 4; ++s) {
+            memcpy(out + off, flux ? e->pf[s] : e->pe[s], (size_t)nf * sizeof(double)); off += nf;
+            memcpy(out + off, flux ? e->uf[s] : e->ue[s], (size_t)nf * sizeof(double)); off += nf;
+            memcpy(out + off, flux ? e->vf[s] : e->ve[s], (size_t)nf * sizeof(double)); off += nf;
+        }
+    }
+}
+void o_get_traces(const o_mesh *m, double *out) { get_sides(m, 0, out); }
+void o_get_flux_traces(const o_mesh *m, double *out) { get_sides(m, 1, out); }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Solver: src/solvers/Solver2D_t.cpp                                                          */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Solver2D_t.cpp:19-29 (single rank: the MPI_Allreduce(MIN) is the identity) */
+double o_get_delta_t(const o_mesh *m, double cfl) {
+    double dt = INFINITY;
+    for (int64_t i = 0; i < m->K; ++i) {
+        const o_elem *e = &m->el[i];
+        dt = fmin(cfl * e->dmin / (e->N * e->N * C_WAVE), dt);
+    }
+    return dt;
+}
+
+/* Element2D_t.cpp:86-125 interpolate_to_boundaries */
+static void element_traces(o_elem *e, const double *lm, const double *lp) {
+    const int N = e->N, nf = N + 1;
+    for (int i = 0; i <= N; ++i) {
+        e->pe[0][i] = 0; e->pe[2][N - i] = 0; e->pe[1][i] = 0; e->pe[3][N - i] = 0;
+        e->ue[0][i] = 0; e->ue[2][N - i] = 0; e->ue[1][i] = 0; e->ue[3][N - i] = 0;
+        e->ve[0][i] = 0; e->ve[2][N - i] = 0; e->ve[1][i] = 0; e->ve[3][N - i] = 0;
+        for (int j = 0; j <= N; ++j) {
+            e->pe[0][i] += lm[j] * e->p[i * nf + j];
+            e->pe[2][N - i] += lp[j] * e->p[i * nf + j];
+            e->pe[1][i] += lp[j] * e->p[j * nf + i];
+            e->pe[3][N - i] += lm[j] * e->p[j * nf + i];
+            e->ue[0][i] += lm[j] * e->u[i * nf + j];
+            e->ue[2][N - i] += lp[j] * e->u[i * nf + j];
+            e->ue[1][i] += lp[j] * e->u[j * nf + i];
+            e->ue[3][N - i] += lm[j] * e->u[j * nf + i];
+            e->ve[0][i] += lm[j] * e->v[i * nf + j];
+            e->ve[2][N - i] += lp[j] * e->v[i * nf + j];
+            e->ve[1][i] += lp[j] * e->v[j * nf + i];
+            e->ve[3][N - i] += lm[j] * e->v[j * nf + i];
+        }
+    }
+}
+
+/* Accumulate the mortar projection of a neighbour trace onto ghost side 0, the shared body of
+ * compute_wall_boundaries / compute_symmetry_boundaries / compute_outflow_boundaries
+ * (src/meshes/Mesh2D_t.cpp:4383-4418, 4656-4691). The "only one neighbour" branch of those
+ * functions tests faces_[0].size() == 0 and is never taken (a ghost has exactly one face). */
+static void ghost_project(const o_mesh *m, o_elem *g, int64_t gi) {
+    const o_tables *t = m->t;
+    const int Ng = g->N;
+    for (int i = 0; i <= Ng; ++i) { g->pe[0][i] = 0; g->ue[0][i] = 0; g->ve[0][i] = 0; }
+    for (int fk = 0; fk < g->nf[0]; ++fk) {
+        const o_face *f = &m->fa[g->faces[0][fk]];
+        const int face_side = f->elements[0] == gi;
+        const o_elem *nb = &m->el[f->elements[face_side]];
+        const int ns = (int)f->sides[face_side];
+        const int Nn = nb->N;
+        const double *xn = t->tab[O_NODES][Nn], *xg = t->tab[O_NODES][Ng];
+        const double *wn = t->tab[O_WEIGHTS][Nn], *wg = t->tab[O_WEIGHTS][Ng];
+        const double *lg = t->tab[O_BARY][Ng];
+        for (int j = 0; j <= Nn; ++j) {
+            const double coord = (xn[Nn - j] + 1) * f->scale[face_side] + 2 * f->offset[face_side] - 1;
+            int found = 0;
+            for (int i = 0; i <= Ng; ++i) {
+                if (almost_equal(coord, xg[i])) {
+                    g->pe[0][i] += wn[j] / wg[i] * nb->pe[ns][j] * g->sf[0][i];
+                    g->ue[0][i] += wn[j] / wg[i] * nb->ue[ns][j] * g->sf[0][i];
+                    g->ve[0][i] += wn[j] / wg[i] * nb->ve[ns][j] * g->sf[0][i];
+                    found = 1;
+                    break;
+                }
+            }
+            if (!found) {
+                double s = 0.0;
+                for (int i = 0; i <= Ng; ++i) s += lg[i] / (coord - xg[i]);
+                for (int i = 0; i <= Ng; ++i) {
+                    const double T = lg[i] / ((coord - xg[i]) * s);
+                    g->pe[0][i] += T * wn[j] / wg[i] * nb->pe[ns][j] * g->sf[0][i];
+                    g->ue[0][i] += T * wn[j] / wg[i] * nb->ue[ns][j] * g->sf[0][i];
+                    g->ve[0][i] += T * wn[j] / wg[i] * nb->ve[ns][j] * g->sf[0][i];
+                }
+            }
+        }
+    }
+}
+
+/* src/meshes/Mesh2D_t.cpp:4298-4436 compute_wall_boundaries (symmetry :4438-4573 is identical) */
+static void wall_boundary(const o_mesh *m, int64_t gi) {
+    o_elem *g = &m->el[gi];
+    ghost_project(m, g, gi);
+    const o_face *f = &m->fa[g->faces[0][0]];
+    for (int k = 0; k <= g->N; ++k) {
+        const double ux = g->ue[0][k], uy = g->ve[0][k];
+        const double l0 = -(ux * f->nx + uy * f->ny);
+        const double l1 = ux * f->tx + uy * f->ty;
+        g->ue[0][k] = f->nx * l0 + f->tx * l1;
+        g->ve[0][k] = f->ny * l0 + f->ty * l1;
+    }
+}
+
+/* src/meshes/Mesh2D_t.cpp:4575-4599 compute_inflow_boundaries */
+static void inflow_boundary(const o_mesh *m, int64_t gi, double time) {
+    o_elem *g = &m->el[gi];
+    const v2 a = m->xy[g->nodes[0]], b = m->xy[g->nodes[1]];
+    const double *x = m->t->tab[O_NODES][g->N];
+    for (int k = 0; k <= g->N; ++k) {
+        const double interp = (x[k] + 1) / 2;
+        const double X = b.x * interp + a.x * (1 - interp);
+        const double Y = b.y * interp + a.y * (1 - interp);
+        double q[3];
+        g_exact(X, Y, time, q);
+        g->pe[0][k] = q[0];
+        g->ue[0][k] = q[1];
+        g->ve[0][k] = q[2];
+    }
+}
+
+/* src/meshes/Mesh2D_t.cpp:3731-3782 boundary_conditions (+ :4712-4730 local_interfaces) */
+static void boundary_conditions(o_mesh *m, double time) {
+    for (int64_t k = 0; k < m->n_wall; ++k) wall_boundary(m, m->wall[k]);
+    for (int64_t k = 0; k < m->n_sym; ++k) wall_boundary(m, m->sym[k]);
+    for (int64_t k = 0; k < m->n_in; ++k) inflow_boundary(m, m->in[k], time);
+    for (int64_t k = 0; k < m->n_out; ++k) ghost_project(m, &m->el[m->out[k]], m->out[k]);
+    for (int64_t k = 0; k < m->n_if; ++k) {
+        const o_elem *src = &m->el[m->if_origin[k]];
+        o_elem *dst = &m->el[m->if_dest[k]];
+        const int s = (int)m->if_side[k];
+        for (int i = 0; i <= src->N; ++i) {
+            dst->pe[0][i] = src->pe[s][i];
+            dst->ue[0][i] = src->ue[s][i];
+            dst->ve[0][i] = src->ve[s][i];
+        }
+    }
+}
+
+static int next_side(int s) { return (s + 1) * !(s + 1 == 4); }
+
+/* src/meshes/Mesh2D_t.cpp:3803-3893 project_to_faces */
+static void project_to_faces(o_mesh *m) {
+    const o_tables *t = m->t;
+    for (int64_t fi = 0; fi < m->F; ++fi) {
+        o_face *f = &m->fa[fi];
+        const int Nf = f->N;
+        const double *xf = t->tab[O_NODES][Nf];
+        for (int side = 0; side < 2; ++side) {
+            const o_elem *e = &m->el[f->elements[side]];
+            const int es = (int)f->sides[side];
+            const int conforming = side == 0
+                ? (f->N == e->N && f->nodes[0] == e->nodes[es] && f->nodes[1] == e->nodes[next_side(es)])
+                : (f->N == e->N && f->nodes[1] == e->nodes[es] && f->nodes[0] == e->nodes[next_side(es)]);
+            if (conforming) {
+                for (int i = 0; i <= Nf; ++i) {
+                    const int o = side == 0 ? i : Nf - i;
+                    f->p[side][o] = e->pe[es][i];
+                    f->u[side][o] = e->ue[es][i];
+                    f->v[side][o] = e->ve[es][i];
+                }
+            } else {
+                const double *xe = t->tab[O_NODES][e->N], *le = t->tab[O_BARY][e->N];
+                for (int i = 0; i <= Nf; ++i) {
+                    const double coord = (xf[side == 0 ? i : Nf - i] + 1) * f->scale[side] + 2 * f->offset[side] - 1;
+                    double pn = 0.0, un = 0.0, vn = 0.0, den = 0.0;
+                    for (int j = 0; j <= e->N; ++j) {
+                        if (almost_equal(coord, xe[j])) {
+                            pn = e->pe[es][j]; un = e->ue[es][j]; vn = e->ve[es][j];
+                            den = 1.0;
+                            break;
+                        }
+                        const double tt = le[j] / (coord - xe[j]);
+                        pn += tt * e->pe[es][j];
+                        un += tt * e->ue[es][j];
+                        vn += tt * e->ve[es][j];
+                        den += tt;
+                    }
+                    f->p[side][i] = pn / den;
+                    f->u[side][i] = un / den;
+                    f->v[side][i] = vn / den;
+                }
+            }
+        }
+    }
+}
+
+/* Solver2D_t.cpp:48-63 calculate_wave_fluxes */
+static void wave_fluxes(o_mesh *m) {
+    for (int64_t fi = 0; fi < m->F; ++fi) {
+        o_face *f = &m->fa[fi];
+        for (int i = 0; i <= f->N; ++i) {
+            const double wL = f->p[0][i] + C_WAVE * (f->u[0][i] * f->nx + f->v[0][i] * f->ny);
+            const double wR = f->p[1][i] - C_WAVE * (f->u[1][i] * f->nx + f->v[1][i] * f->ny);
+            f->pflux[i] = C_WAVE * (wL - wR) / 2;
+            f->uflux[i] = f->nx * (wL + wR) / 2;
+            f->vflux[i] = f->ny * (wL + wR) / 2;
+        }
+    }
+}
+
+void o_wave_flux(const double *in, double *out) {
+    const double nx = in[6], ny = in[7];
+    const double wL = in[0] + C_WAVE * (in[1] * nx + in[2] * ny);
+    const double wR = in[3] - C_WAVE * (in[4] * nx + in[5] * ny);
+    out[0] = C_WAVE * (wL - wR) / 2;
+    out[1] = nx * (wL + wR) / 2;
+    out[2] = ny * (wL + wR) / 2;
+}
+
+/* src/meshes/Mesh2D_t.cpp:3895-4006 project_to_elements */
+static void project_to_elements(o_mesh *m) {
+    const o_tables *t = m->t;
+    for (int64_t ei = 0; ei < m->K; ++ei) {
+        o_elem *e = &m->el[ei];
+        const int Ne = e->N;
+        for (int s = 0; s < 4; ++s) {
+            const o_face *f0 = &m->fa[e->faces[s][0]];
+            if (e->nf[s] == 1 && f0->N == Ne && f0->nodes[0] == e->nodes[s] && f0->nodes[1] == e->nodes[next_side(s)]) {
+                for (int j = 0; j <= f0->N; ++j) {
+                    e->pf[s][j] = f0->pflux[j] * e->sf[s][j];
+                    e->uf[s][j] = f0->uflux[j] * e->sf[s][j];
+                    e->vf[s][j] = f0->vflux[j] * e->sf[s][j];
+                }
+            } else if (e->nf[s] == 1 && f0->N == Ne && f0->nodes[1] == e->nodes[s] && f0->nodes[0] == e->nodes[next_side(s)]) {
+                for (int j = 0; j <= f0->N; ++j) {
+                    e->pf[s][f0->N - j] = -f0->pflux[j] * e->sf[s][j];
+                    e->uf[s][f0->N - j] = -f0->uflux[j] * e->sf[s][j];
+                    e->vf[s][f0->N - j] = -f0->vflux[j] * e->sf[s][j];
+                }
+            } else {
+                for (int i = 0; i <= Ne; ++i) { e->pf[s][i] = 0.0; e->uf[s][i] = 0.0; e->vf[s][i] = 0.0; }
+                const double *xe = t->tab[O_NODES][Ne], *we = t->tab[O_WEIGHTS][Ne], *le = t->tab[O_BARY][Ne];
+                for (int fk = 0; fk < e->nf[s]; ++fk) {
+                    const o_face *f = &m->fa[e->faces[s][fk]];
+                    const double *xf = t->tab[O_NODES][f->N], *wf = t->tab[O_WEIGHTS][f->N];
+                    const int fwd = ei == f->elements[0];
+                    const int fs = fwd ? 0 : 1;
+                    for (int j = 0; j <= f->N; ++j) {
+                        const double coord = (xf[fwd ? j : f->N - j] + 1) * f->scale[fs] + 2 * f->offset[fs] - 1;
+                        int found = 0;
+                        for (int i = 0; i <= Ne; ++i) {
+                            if (almost_equal(coord, xe[i])) {
+                                if (fwd) {
+                                    e->pf[s][i] += wf[j] / we[i] * f->pflux[j] * e->sf[s][i] * f->scale[0];
+                                    e->uf[s][i] += wf[j] / we[i] * f->uflux[j] * e->sf[s][i] * f->scale[0];
+                                    e->vf[s][i] += wf[j] / we[i] * f->vflux[j] * e->sf[s][i] * f->scale[0];
+                                } else {
+                                    e->pf[s][i] += -wf[j] / we[i] * f->pflux[j] * e->sf[s][i] * f->scale[1];
+                                    e->uf[s][i] += -wf[j] / we[i] * f->uflux[j] * e->sf[s][i] * f->scale[1];
+                                    e->vf[s][i] += -wf[j] / we[i] * f->vflux[j] * e->sf[s][i] * f->scale[1];
+                                }
+                                found = 1;
+                                break;
+                            }
+                        }
+                        if (!found) {
+                            double ssum = 0.0;
+                            for (int i = 0; i <= Ne; ++i) ssum += le[i] / (coord - xe[i]);
+                            for (int i = 0; i <= Ne; ++i) {
+                                const double T = le[i] / ((coord - xe[i]) * ssum);
+                                if (fwd) {
+                                    e->pf[s][i] += T * wf[j] / we[i] * f->pflux[j] * e->sf[s][i] * f->scale[0];
+                                    e->uf[s][i] += T * wf[j] / we[i] * f->uflux[j] * e->sf[s][i] * f->scale[0];
+                                    e->vf[s][i] += T * wf[j] / we[i] * f->vflux[j] * e->sf[s][i] * f->scale[0];
+                                } else {
+                                    e->pf[s][i] += -T * wf[j] / we[i] * f->pflux[j] * e->sf[s][i] * f->scale[1];
+                                    e->uf[s][i] += -T * wf[j] / we[i] * f->uflux[j] * e->sf[s][i] * f->scale[1];
+                                    e->vf[s][i] += -T * wf[j] / we[i] * f->vflux[j] * e->sf[s][i] * f->scale[1];
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* Solver2D_t.cpp:39-46 matrix_vector_multiply */
+static void matvec(int N, const double *A, const double *x, double *y) {
+    for (int i = 0; i <= N; ++i) {
+        y[i] = 0.0;
+        for (int j = 0; j <= N; ++j) y[i] += A[i * (N + 1) + j] * x[j];
+    }
+}
+
+/* Solver2D_t.cpp:66-136 compute_dg_wave_derivative (x_flux = {c u, p, 0}, y_flux = {c v, 0, p},
+ * :31-37) */
+static void dg_derivative(o_mesh *m) {
+    const o_tables *t = m->t;
+    for (int64_t ei = 0; ei < m->K; ++ei) {
+        o_elem *e = &m->el[ei];
+        const int N = e->N, nf = N + 1;
+        const double *w = t->tab[O_WEIGHTS][N], *Dh = t->tab[O_DHAT][N];
+        const double *ll = t->tab[O_LAG_LEFT][N], *lr = t->tab[O_LAG_RIGHT][N];
+        for (int j = 0; j <= N; ++j) {
+            for (int i = 0; i <= N; ++i) {
+                const int k = i * nf + j;
+                const double fx0 = C_WAVE * e->u[k], fx1 = e->p[k], fx2 = 0.0;
+                const double fy0 = C_WAVE * e->v[k], fy1 = 0.0, fy2 = e->p[k];
+                e->pflux[i] = e->deta_dy[k] * fx0 - e->deta_dx[k] * fy0;
+                e->uflux[i] = e->deta_dy[k] * fx1 - e->deta_dx[k] * fy1;
+                e->vflux[i] = e->deta_dy[k] * fx2 - e->deta_dx[k] * fy2;
+            }
+            matvec(N, Dh, e->pflux, e->pder);
+            matvec(N, Dh, e->uflux, e->uder);
+            matvec(N, Dh, e->vflux, e->vder);
+            for (int i = 0; i <= N; ++i) {
+                e->pder[i] += (e->pf[1][j] * lr[i] + e->pf[3][N - j] * ll[i]) / w[i];
+                e->uder[i] += (e->uf[1][j] * lr[i] + e->uf[3][N - j] * ll[i]) / w[i];
+                e->vder[i] += (e->vf[1][j] * lr[i] + e->vf[3][N - j] * ll[i]) / w[i];
+            }
+            for (int i = 0; i <= N; ++i) {
+                e->Gp[i * nf + j] = -e->pder[i];
+                e->Gu[i * nf + j] = -e->uder[i];
+                e->Gv[i * nf + j] = -e->vder[i];
+            }
+        }
+        for (int i = 0; i <= N; ++i) {
+            for (int j = 0; j <= N; ++j) {
+                const int k = i * nf + j;
+                const double fx0 = C_WAVE * e->u[k], fx1 = e->p[k], fx2 = 0.0;
+                const double fy0 = C_WAVE * e->v[k], fy1 = 0.0, fy2 = e->p[k];
+                e->pflux[j] = -e->dxi_dy[k] * fx0 + e->dxi_dx[k] * fy0;
+                e->uflux[j] = -e->dxi_dy[k] * fx1 + e->dxi_dx[k] * fy1;
+                e->vflux[j] = -e->dxi_dy[k] * fx2 + e->dxi_dx[k] * fy2;
+            }
+            matvec(N, Dh, e->pflux, e->pder);
+            matvec(N, Dh, e->uflux, e->uder);
+            matvec(N, Dh, e->vflux, e->vder);
+            for (int j = 0; j <= N; ++j) {
+                e->pder[j] += (e->pf[2][N - i] * lr[j] + e->pf[0][i] * ll[j]) / w[j];
+                e->uder[j] += (e->uf[2][N - i] * lr[j] + e->uf[0][i] * ll[j]) / w[j];
+                e->vder[j] += (e->vf[2][N - i] * lr[j] + e->vf[0][i] * ll[j]) / w[j];
+            }
+            for (int j = 0; j <= N; ++j) {
+                const int k = i * nf + j;
+                e->Gp[k] = (e->Gp[k] - e->pder[j]) / e->jac[k];
+                e->Gu[k] = (e->Gu[k] - e->uder[j]) / e->jac[k];
+                e->Gv[k] = (e->Gv[k] - e->vder[j]) / e->jac[k];
+            }
+        }
+    }
+}
+
+/* Solver2D_t.cpp:138-172 rk3_first_step / rk3_step (a = 0 marks the first stage) */
+static void rk3_update(o_mesh *m, int first, double dt, double a, double g) {
+    for (int64_t ei = 0; ei < m->K; ++ei) {
+        o_elem *e = &m->el[ei];
+        const int ne = (e->N + 1) * (e->N + 1);
+        for (int k = 0; k < ne; ++k) {
+            if (first) {
+                e->ip[k] = e->Gp[k]; e->iu[k] = e->Gu[k]; e->iv[k] = e->Gv[k];
+            } else {
+                e->ip[k] = a * e->ip[k] + e->Gp[k];
+                e->iu[k] = a * e->iu[k] + e->Gu[k];
+                e->iv[k] = a * e->iv[k] + e->Gv[k];
+            }
+            e->p[k] += g * dt * e->ip[k];
+            e->u[k] += g * dt * e->iu[k];
+            e->v[k] += g * dt * e->iv[k];
+        }
+    }
+}
+
+void o_compute_rhs(o_mesh *m, double time) {
+    for (int64_t ei = 0; ei < m->K; ++ei) {
+        o_elem *e = &m->el[ei];
+        element_traces(e, m->t->tab[O_LAG_LEFT][e->N], m->t->tab[O_LAG_RIGHT][e->N]);
+    }
+    boundary_conditions(m, time);
+    project_to_faces(m);
+    wave_fluxes(m);
+    project_to_elements(m);
+    dg_derivative(m);
+}
+
+/* include/solvers/Solver2D_t.tpp:16-18 (Williamson 2N-storage RK3) and :44-72 */
+static const double AM[3] = {0, -5.0 / 9.0, -153.0 / 128.0};
+static const double BM[3] = {0, 1.0 / 3.0, 0.75};
+static const double GM[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+
+void o_step(o_mesh *m, double time, double dt) {
+    for (int s = 0; s < 3; ++s) {
+        o_compute_rhs(m, time + BM[s] * dt);
+        rk3_update(m, s == 0, dt, AM[s], GM[s]);
+    }
+}
+
+int64_t o_solve(o_mesh *m, double cfl, double t_end, double *t_final) {
+    double time = 0.0;
+    int64_t steps = 0;
+    while (time < t_end) {
+        ++steps;
+        double dt = o_get_delta_t(m, cfl);
+        if (time + dt > t_end) dt = t_end - time;
+        o_step(m, time, dt);
+        time += dt;
+    }
+    if (t_final) *t_final = time;
+    return steps;
+}
+
+void o_run_steps(o_mesh *m, double cfl, int64_t n_steps, double *time_io) {
+    double time = time_io ? *time_io : 0.0;
+    for (int64_t k = 0; k < n_steps; ++k) {
+        const double dt = o_get_delta_t(m, cfl);
+        o_step(m, time, dt);
+        time += dt;
+    }
+    if (time_io) *time_io = time;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Estimator: src/entities/Element2D_t.cpp:274-343                                             */
+/* ------------------------------------------------------------------------------------------ */
+
+static void estimate_p(int N, const double *p, const double *poly, double tol_min, double tol_max,
+                       double *err_out, double *sigma_out, int *refine, int *coarsen, double *spectrum) {
+    const int nf = N + 1;
+    const int n = (N + 1) < N_LSQ_MAX ? (N + 1) : N_LSQ_MAX;
+    for (int ni = 0; ni < n; ++ni) {
+        spectrum[ni] = 0.0;
+        const double pd = N + ni + 1 - n;
+        const int pi_ = (int)pd;
+        for (int i = 0; i < pd; ++i) {
+            double lx = 0.0, ly = 0.0;
+            for (int k = 0; k <= N; ++k)
+                for (int l = 0; l <= N; ++l) {
+                    lx += p[k * nf + l] * poly[i * nf + k] * poly[pi_ * nf + l];
+                    ly += p[k * nf + l] * poly[pi_ * nf + k] * poly[i * nf + l];
+                }
+            spectrum[ni] += fabs(lx) + fabs(ly);
+        }
+        double lx = 0.0;
+        for (int k = 0; k <= N; ++k)
+            for (int l = 0; l <= N; ++l) lx += p[k * nf + l] * poly[pi_ * nf + k] * poly[pi_ * nf + l];
+        spectrum[ni] += fabs(lx);
+    }
+    /* Element2D_t.cpp:319-343 exponential_decay */
+    double x_avg = 0.0, y_avg = 0.0;
+    for (int i = 0; i < n; ++i) {
+        x_avg += N + i + 1 - n;
+        y_avg += log(spectrum[i]);
+    }
+    x_avg /= n;
+    y_avg /= n;
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const int pp = N + i + 1 - n;
+        num += (pp - x_avg) * (log(spectrum[i]) - y_avg);
+        den += (pp - x_avg) * (pp - x_avg);
+    }
+    const double sigma_signed = num / den;
+    const double C = exp(y_avg - sigma_signed * x_avg);
+    const double sigma = fabs(sigma_signed);
+    const double err = sqrt(spectrum[n - 1] * spectrum[n - 1] + C * C * 0.5 / sigma * exp(-2 * sigma * (N + 1)));
+    *err_out = err;
+    *sigma_out = sigma;
+    *refine = err > tol_min;
+    *coarsen = err < tol_max;
+}
+
+void o_estimate_error(o_mesh *m, double *error, double *sigma, int32_t *refine, int32_t *coarsen) {
+    for (int64_t ei = 0; ei < m->K; ++ei) {
+        o_elem *e = &m->el[ei];
+        double spec[16];
+        estimate_p(e->N, e->p, m->t->tab[O_POLY][e->N], m->tol_min, m->tol_max, &e->error, &e->sigma,
+                   &e->refine, &e->coarsen, spec);
+        if (error) error[ei] = e->error;
+        if (sigma) sigma[ei] = e->sigma;
+        if (refine) refine[ei] = e->refine;
+        if (coarsen) coarsen[ei] = e->coarsen;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Element / face level entry points                                                           */
+/* ------------------------------------------------------------------------------------------ */
+
+void o_element_traces(const o_tables *t, int N, const double *p, const double *u, const double *v,
+                      double *out) {
+    o_elem e;
+    memset(&e, 0, sizeof(e));
+    e.N = N;
+    elem_alloc_domain(&e);
+    const int nf = N + 1;
+    memcpy(e.p, p, (size_t)nf * nf * sizeof(double));
+    memcpy(e.u, u, (size_t)nf * nf * sizeof(double));
+    memcpy(e.v, v, (size_t)nf * nf * sizeof(double));
+    element_traces(&e, t->tab[O_LAG_LEFT][N], t->tab[O_LAG_RIGHT][N]);
+    for (int s = 0; s < 4; ++s) {
+        memcpy(out + (s * 3 + 0) * nf, e.pe[s], (size_t)nf * sizeof(double));
+        memcpy(out + (s * 3 + 1) * nf, e.ue[s], (size_t)nf * sizeof(double));
+        memcpy(out + (s * 3 + 2) * nf, e.ve[s], (size_t)nf * sizeof(double));
+    }
+    free(e.block);
+}
+
+void o_element_geometry(const o_tables *t, int N, const double *pts, double *out) {
+    o_elem e;
+    memset(&e, 0, sizeof(e));
+    e.N = N;
+    elem_alloc_domain(&e);
+    v2 P[4];
+    for (int k = 0; k < 4; ++k) { P[k].x = pts[2 * k]; P[k].y = pts[2 * k + 1]; }
+    element_geometry(&e, P, t->tab[O_NODES][N]);
+    const int nf = N + 1, ne = nf * nf;
+    double *o = out;
+    memcpy(o, e.dxi_dx, ne * sizeof(double)); o += ne;
+    memcpy(o, e.deta_dx, ne * sizeof(double)); o += ne;
+    memcpy(o, e.dxi_dy, ne * sizeof(double)); o += ne;
+    memcpy(o, e.deta_dy, ne * sizeof(double)); o += ne;
+    memcpy(o, e.jac, ne * sizeof(double)); o += ne;
+    for (int s = 0; s < 4; ++s) { memcpy(o, e.sf[s], nf * sizeof(double)); o += nf; }
+    o[0] = e.dmin; o[1] = e.cx; o[2] = e.cy;
+    free(e.block);
+}
+
+void o_boundary_geometry(const o_tables *t, int N, const double *pts, double *out) {
+    o_elem e;
+    memset(&e, 0, sizeof(e));
+    e.N = N;
+    elem_alloc_ghost(&e);
+    v2 P[4];
+    for (int k = 0; k < 4; ++k) { P[k].x = pts[2 * k]; P[k].y = pts[2 * k + 1]; }
+    boundary_geometry(&e, P, t->tab[O_NODES][N]);
+    memcpy(out, e.sf[0], (size_t)(N + 1) * sizeof(double));
+    out[N + 1] = e.dmin; out[N + 2] = e.cx; out[N + 3] = e.cy;
+    free(e.block);
+}
+
+void o_face_geometry(const double *face_nodes, const double *element_nodes, double *out) {
+    o_face f;
+    memset(&f, 0, sizeof(f));
+    v2 fn[2], en[4];
+    for (int k = 0; k < 2; ++k) { fn[k].x = face_nodes[2 * k]; fn[k].y = face_nodes[2 * k + 1]; }
+    for (int k = 0; k < 4; ++k) { en[k].x = element_nodes[2 * k]; en[k].y = element_nodes[2 * k + 1]; }
+    face_geometry(&f, fn, en);
+    out[0] = f.nx; out[1] = f.ny; out[2] = f.tx; out[3] = f.ty; out[4] = f.length;
+    out[5] = f.offset[0]; out[6] = f.offset[1]; out[7] = f.scale[0]; out[8] = f.scale[1];
+}
+
+void o_element_estimate(const o_tables *t, int N, const double *p, double tol_min, double tol_max,
+                        double *out) {
+    double spec[16] = {0};
+    double err, sigma;
+    int refine, coarsen;
+    estimate_p(N, p, t->tab[O_POLY][N], tol_min, tol_max, &err, &sigma, &refine, &coarsen, spec);
+    out[0] = err; out[1] = sigma; out[2] = refine; out[3] = coarsen;
+    const int n = (N + 1) < N_LSQ_MAX ? (N + 1) : N_LSQ_MAX;
+    for (int i = 0; i < n; ++i) out[4 + i] = spec[i];
+}
+
+/* src/functions/Hilbert.cpp:16-40 */
+static void hilbert_rot(int n, int *x, int *y, int rx, int ry) {
+    if (ry == 0) {
+        if (rx == 1) { *x = n - 1 - *x; *y = n - 1 - *y; }
+        const int tmp = *x; *x = *y; *y = tmp;
+    }
+}
+
+void o_hilbert_d2xy(int n, int d, int *xy) {
+    int x = 0, y = 0;
+    for (int s = 1; s < n; s *= 2) {
+        const int rx = 1 & (d / 2);
+        const int ry = 1 & (d ^ rx);
+        hilbert_rot(s, &x, &y, rx, ry);
+        x += s * rx;
+        y += s * ry;
+        d /= 4;
+    }
+    xy[0] = x; xy[1] = y;
+}
+
+/* src/functions/Hilbert.cpp:3-14 */
+int o_hilbert_xy2d(int n, int x, int y) {
+    int d = 0;
+    for (int s = n / 2; s > 0; s /= 2) {
+        const int rx = (x & s) > 0, ry = (y & s) > 0;
+        d += s * s * ((3 * rx) ^ ry);
+        hilbert_rot(n, &x, &y, rx, ry);
+    }
+    return d;
+}

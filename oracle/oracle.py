@@ -1,0 +1,224 @@
+"""ctypes wrapper of oracle/liboracle.so -- THE ORACLE (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module, and only
+as the checker (or as the timed CPU restatement). The product (ndg_cuda_amd) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+O_NODES, O_WEIGHTS, O_BARY, O_LAG_LEFT, O_LAG_RIGHT, O_DLAG_LEFT, O_DLAG_RIGHT, O_D, O_GHAT, O_DHAT, \
+    O_INTERP, O_POLY = range(12)
+TABLE_KEYS = ["nodes", "weights", "bary", "lag_left", "lag_right", "dlag_left", "dlag_right", "D",
+              "Ghat", "Dhat", "interp", "poly"]
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        sig = {
+            "o_tables_create": ([i32, i32], vp), "o_tables_destroy": ([vp], None),
+            "o_tables_get": ([vp, i32, i32, vp], i32),
+            "o_mesh_create": ([vp, i32, i64, vp, i64, vp, vp, i64, vp, vp, vp, f64, f64], vp),
+            "o_mesh_destroy": ([vp], None), "o_mesh_n_elements": ([vp], i64),
+            "o_mesh_n_faces": ([vp], i64), "o_mesh_n_ghosts": ([vp], i64),
+            "o_mesh_n_nodes_total": ([vp], i64), "o_mesh_get_N": ([vp, vp], None),
+            "o_initial_conditions_plane": ([vp], None),
+            "o_initial_conditions_pulses": ([vp, i32, vp], None),
+            "o_set_state": ([vp, vp, vp, vp], None), "o_get_state": ([vp, vp, vp, vp], None),
+            "o_get_rhs": ([vp, vp, vp, vp], None), "o_get_traces": ([vp, vp], None),
+            "o_get_flux_traces": ([vp, vp], None), "o_get_delta_t": ([vp, f64], f64),
+            "o_compute_rhs": ([vp, f64], None), "o_step": ([vp, f64, f64], None),
+            "o_solve": ([vp, f64, f64, vp], i64), "o_run_steps": ([vp, f64, i64, vp], None),
+            "o_estimate_error": ([vp, vp, vp, vp, vp], None),
+            "o_element_traces": ([vp, i32, vp, vp, vp, vp], None),
+            "o_element_geometry": ([vp, i32, vp, vp], None),
+            "o_boundary_geometry": ([vp, i32, vp, vp], None),
+            "o_face_geometry": ([vp, vp, vp], None),
+            "o_element_estimate": ([vp, i32, vp, f64, f64, vp], None),
+            "o_wave_flux": ([vp, vp], None), "o_hilbert_d2xy": ([i32, i32, vp], None),
+            "o_hilbert_xy2d": ([i32, i32, i32], i32),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Tables:
+    def __init__(self, N_max, n_interp):
+        self.N_max, self.n_interp = N_max, n_interp
+        self.h = lib().o_tables_create(N_max, n_interp)
+
+    def get(self, which, N):
+        nf = N + 1
+        n = nf * nf if which in (O_D, O_GHAT, O_DHAT, O_POLY) else (nf * self.n_interp if which == O_INTERP else nf)
+        out = np.empty(n)
+        lib().o_tables_get(self.h, which, N, _p(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().o_tables_destroy(self.h)
+            self.h = None
+
+
+class Mesh:
+    """Reference-model mesh built from the same description the product takes."""
+
+    def __init__(self, tables, desc, initial_N, tol_min=1e-6, tol_max=1e-14):
+        self.tables = tables
+        xy = _f(desc.xy)
+        quads = np.ascontiguousarray(desc.quads, dtype=np.int64)
+        bars = np.ascontiguousarray(desc.bars, dtype=np.int64)
+        bt = np.ascontiguousarray(desc.bar_type, dtype=np.int32)
+        bd = np.ascontiguousarray(desc.bar_donor, dtype=np.int64)
+        qn = None if desc.quad_N is None else np.ascontiguousarray(desc.quad_N, dtype=np.int32)
+        self._keep = (xy, quads, bars, bt, bd, qn)
+        self.h = lib().o_mesh_create(tables.h, initial_N, xy.shape[0], _p(xy), quads.shape[0], _p(quads), _p(qn),
+                                     bars.shape[0], _p(bars), _p(bt), _p(bd), tol_min, tol_max)
+        self.K = lib().o_mesh_n_elements(self.h)
+        self.n = lib().o_mesh_n_nodes_total(self.h)
+        self.N = np.empty(self.K, dtype=np.int32)
+        lib().o_mesh_get_N(self.h, _p(self.N))
+
+    @property
+    def n_faces(self):
+        return lib().o_mesh_n_faces(self.h)
+
+    def initial_conditions(self, pulses=None):
+        if pulses is None:
+            lib().o_initial_conditions_plane(self.h)
+        else:
+            arr = _f(np.asarray(pulses, dtype=np.float64).reshape(-1, 4))
+            lib().o_initial_conditions_pulses(self.h, arr.shape[0], _p(arr))
+
+    def set_state(self, p, u, v):
+        p, u, v = _f(p), _f(u), _f(v)
+        lib().o_set_state(self.h, _p(p), _p(u), _p(v))
+
+    def get_state(self):
+        out = [np.empty(self.n) for _ in range(3)]
+        lib().o_get_state(self.h, *(_p(a) for a in out))
+        return tuple(out)
+
+    def get_rhs(self):
+        out = [np.empty(self.n) for _ in range(3)]
+        lib().o_get_rhs(self.h, *(_p(a) for a in out))
+        return tuple(out)
+
+    def _sides(self, fn):
+        out = np.empty(int(np.sum(12 * (self.N.astype(np.int64) + 1))))
+        fn(self.h, _p(out))
+        return out
+
+    def get_traces(self):
+        return self._sides(lib().o_get_traces)
+
+    def get_flux_traces(self):
+        return self._sides(lib().o_get_flux_traces)
+
+    def delta_t(self, cfl):
+        return lib().o_get_delta_t(self.h, cfl)
+
+    def compute_rhs(self, t=0.0):
+        lib().o_compute_rhs(self.h, t)
+        return self.get_rhs()
+
+    def step(self, t, dt):
+        lib().o_step(self.h, t, dt)
+
+    def solve(self, cfl, t_end):
+        tf = ctypes.c_double()
+        n = lib().o_solve(self.h, cfl, t_end, ctypes.byref(tf))
+        return n, tf.value
+
+    def run_steps(self, cfl, n_steps, time=0.0):
+        t = ctypes.c_double(time)
+        lib().o_run_steps(self.h, cfl, n_steps, ctypes.byref(t))
+        return t.value
+
+    def estimate_error(self):
+        err, sig = np.empty(self.K), np.empty(self.K)
+        ref, coa = np.empty(self.K, dtype=np.int32), np.empty(self.K, dtype=np.int32)
+        lib().o_estimate_error(self.h, _p(err), _p(sig), _p(ref), _p(coa))
+        return err, sig, ref.astype(bool), coa.astype(bool)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().o_mesh_destroy(self.h)
+            self.h = None
+
+
+def element_traces(tables, N, p, u, v):
+    out = np.empty(12 * (N + 1))
+    p, u, v = _f(p), _f(u), _f(v)
+    lib().o_element_traces(tables.h, N, _p(p), _p(u), _p(v), _p(out))
+    return out
+
+
+def element_geometry(tables, N, pts):
+    nf = N + 1
+    out = np.empty(5 * nf * nf + 4 * nf + 3)
+    pts = _f(pts)
+    lib().o_element_geometry(tables.h, N, _p(pts), _p(out))
+    return out
+
+
+def boundary_geometry(tables, N, pts):
+    out = np.empty(N + 4)
+    pts = _f(pts)
+    lib().o_boundary_geometry(tables.h, N, _p(pts), _p(out))
+    return out
+
+
+def face_geometry(face_nodes, element_nodes):
+    out = np.empty(9)
+    a, b = _f(face_nodes), _f(element_nodes)
+    lib().o_face_geometry(_p(a), _p(b), _p(out))
+    return out
+
+
+def element_estimate(tables, N, p, tol_min=1e-6, tol_max=1e-14):
+    out = np.zeros(8)
+    p = _f(p)
+    lib().o_element_estimate(tables.h, N, _p(p), tol_min, tol_max, _p(out))
+    return out
+
+
+def wave_flux(state8):
+    out = np.empty(3)
+    s = _f(state8)
+    lib().o_wave_flux(_p(s), _p(out))
+    return out
+
+
+def hilbert_d2xy(n, d):
+    xy = np.zeros(2, dtype=np.int32)
+    lib().o_hilbert_d2xy(n, d, _p(xy))
+    return int(xy[0]), int(xy[1])

@@ -1,0 +1,208 @@
+"""Parity of the HIP path (called through the C ABI) with the oracle on the same inputs.
+
+Bar: relative max error max|a - b| / max|b| per field <= 1e-12 for FP64 (the kernels contract
+multiply-adds into FMAs, the oracle does not, so agreement is to roundoff rather than bitwise).
+At the full roofline size (512 x 512, N = 8) the oracle is too slow, so size-independent
+properties are checked instead: linearity, conservation and energy decay.
+"""
+import numpy as np
+import pytest
+
+import exact
+import ndg_cuda_amd as nd
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def say(*args):
+    print("[gpu]", *args, flush=True)
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return nd.NDG_t(16, 5), O.Tables(16, 5)
+
+
+def build(tables, res, N, bc=(nd.BC_WALL,) * 4, periodic=False, jitter=0.0, seed=1):
+    ndg, ot = tables
+    desc = nd.square_mesh(res, bc=bc, x_periodic=periodic, y_periodic=periodic, jitter=jitter, seed=seed)
+    gm = nd.Mesh2D_t(desc, N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
+    om = O.Mesh(ot, desc, N)
+    return desc, gm, om
+
+
+def random_state(n, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.normal(size=n) for _ in range(3)]
+
+
+MIXED_BC = (nd.BC_WALL, nd.BC_INFLOW, nd.BC_OUTFLOW, nd.BC_SYMMETRY)
+
+
+def test_device_present():
+    assert nd.device_count() >= 1, "GPU tests need a visible HIP device"
+
+
+@pytest.mark.parametrize("N", [1, 2, 4, 6, 8, 12, 16])
+def test_traces(tables, N):
+    say("traces N", N)
+    desc, gm, om = build(tables, 4, N, jitter=0.2)
+    q = random_state(gm.n_dofs_per_field_, N)
+    gm.set_solution(*q)
+    om.set_state(*q)
+    om.compute_rhs(0.0)
+    ref = om.get_traces().reshape(gm.n_elements_, 4, 3, N + 1)
+    got = gm.interpolate_to_boundaries()
+    assert rel(got, ref) < TOL
+
+
+CASES = [
+    ("periodic-jitter", dict(periodic=True, jitter=0.2), 6),
+    ("wall-jitter", dict(jitter=0.2), 8),
+    ("mixed-bc", dict(bc=MIXED_BC, jitter=0.1), 5),
+    ("inflow", dict(bc=(nd.BC_INFLOW,) * 4), 7),
+    ("outflow", dict(bc=(nd.BC_OUTFLOW,) * 4, jitter=0.1), 4),
+    ("zero-ghost", dict(bc=(nd.BC_NONE,) * 4), 3),
+]
+
+
+@pytest.mark.parametrize("name,kw,N", CASES)
+def test_rhs(tables, name, kw, N):
+    say("rhs", name, "N", N)
+    desc, gm, om = build(tables, 8, N, **kw)
+    q = random_state(gm.n_dofs_per_field_, 17)
+    gm.set_solution(*q)
+    om.set_state(*q)
+    t = 0.13
+    ref = om.compute_rhs(t)
+    got = gm.compute_rhs(t)
+    for g, r in zip(got, ref):
+        assert rel(g, r) < TOL, name
+
+
+@pytest.mark.parametrize("name,kw,N", CASES)
+def test_steps(tables, name, kw, N):
+    say("steps", name, "N", N)
+    desc, gm, om = build(tables, 8, N, **kw)
+    if "inflow" in name or kw.get("bc") == MIXED_BC:
+        gm.initial_conditions(None)
+        om.initial_conditions(None)
+    else:
+        pulses = [(0.1, -0.2, 0.2, 1.0), (-0.4, 0.3, 0.2, 0.5)]
+        gm.initial_conditions(pulses)
+        om.initial_conditions(pulses)
+    assert rel(np.concatenate(gm.get_solution()), np.concatenate(om.get_state())) < TOL
+    solver = nd.Solver2D_t(0.5, [0.0])
+    t_gpu = solver.run_steps(gm, 7, 0.0)
+    t_cpu = om.run_steps(0.5, 7, 0.0)
+    assert t_gpu == t_cpu
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL, name
+
+
+def test_delta_t_matches(tables):
+    desc, gm, om = build(tables, 16, 8, jitter=0.15)
+    solver = nd.Solver2D_t(0.5, [0.0])
+    assert solver.get_delta_t(gm) == om.delta_t(0.5)
+
+
+def test_config1_full_solve(tables):
+    """16 x 16 periodic, N = 6, CFL 0.5, pulse at (0.1, -0.2) width 0.2, t_end = 0.5: the reference
+    loop takes 289 steps (accumulated time falls short of 0.5 after 288)."""
+    say("config 1")
+    desc, gm, om = build(tables, 16, 6, periodic=True)
+    pulse = [(0.1, -0.2, 0.2, 1.0)]
+    gm.initial_conditions(pulse)
+    om.initial_conditions(pulse)
+    solver = nd.Solver2D_t(0.5, [0.5])
+    steps = solver.solve(None, gm)
+    n_ref, t_ref = om.solve(0.5, 0.5)
+    assert steps == n_ref == 289 and solver.time_ == t_ref == 0.5
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL
+
+
+def test_config1_exact_solution(tables):
+    """Same configuration with the periodised pulse against the Fourier-series exact solution."""
+    ndg, ot = tables
+    desc, gm, _ = build(tables, 16, 6, periodic=True)
+    base = (0.1, -0.2, 0.2, 1.0)
+    gm.initial_conditions(exact.periodic_images(*base))
+    solver = nd.Solver2D_t(0.5, [0.5])
+    solver.solve(None, gm)
+    X, Y = exact.node_coordinates(desc, ot.get(O.O_NODES, 6))
+    pe, ue, ve = exact.periodic_pulse(X, Y, solver.time_, [base], n_modes=24)
+    p, u, v = gm.get_solution()
+    assert max(np.abs(p - pe).max(), np.abs(u - ue).max(), np.abs(v - ve).max()) < 5e-7
+
+
+def test_config2_reduced(tables):
+    """Config-2 form (jittered walls, N = 8, two pulses) at 32 x 32 for 20 steps."""
+    say("config 2 reduced")
+    desc, gm, om = build(tables, 32, 8, jitter=0.15, seed=2024)
+    pulses = [(0.1, -0.2, 0.2, 1.0), (-0.4, 0.3, 0.2, 0.5)]
+    gm.initial_conditions(pulses)
+    om.initial_conditions(pulses)
+    solver = nd.Solver2D_t(0.5, [0.0])
+    solver.run_steps(gm, 20)
+    om.run_steps(0.5, 20)
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL
+
+
+def test_estimator(tables):
+    desc, gm, om = build(tables, 8, 8, periodic=True, jitter=0.1)
+    pulses = [(0.1, -0.2, 0.15, 1.0)]
+    gm.initial_conditions(pulses)
+    om.initial_conditions(pulses)
+    eg, sg, rg, cg = gm.estimate_error()
+    eo, so, ro, co = om.estimate_error()
+    assert rel(eg, eo) < 1e-12 and rel(sg, so) < 1e-12
+    assert np.array_equal(rg, ro) and np.array_equal(cg, co)
+
+
+def test_full_size_properties(tables):
+    """512 x 512 elements, N = 8 (the roofline size): linearity of G on jittered walls and
+    conservation of sum(w w J G_p) on a periodic mesh, without the oracle."""
+    say("full size 512x512 N=8")
+    ndg, ot = tables
+    N, res = 8, 512
+    desc = nd.square_mesh(res, jitter=0.15, seed=5)
+    gm = nd.Mesh2D_t(desc, N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
+    n = gm.n_dofs_per_field_
+    rng = np.random.default_rng(0)
+    q1 = [rng.standard_normal(n) for _ in range(3)]
+    q2 = [rng.standard_normal(n) for _ in range(3)]
+    gm.set_solution(*q1)
+    G1 = gm.compute_rhs(0.0)
+    gm.set_solution(*q2)
+    G2 = gm.compute_rhs(0.0)
+    gm.set_solution(*[0.5 * a - 2.0 * b for a, b in zip(q1, q2)])
+    G12 = gm.compute_rhs(0.0)
+    for g12, g1, g2 in zip(G12, G1, G2):
+        assert rel(g12, 0.5 * g1 - 2.0 * g2) < 1e-12
+    del gm
+    say("full size periodic conservation")
+    desc = nd.square_mesh(res, x_periodic=True, y_periodic=True, jitter=0.15, seed=6)
+    gm = nd.Mesh2D_t(desc, N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
+    gm.set_solution(*q1)
+    Gp, _, _ = gm.compute_rhs(0.0)
+    w = ot.get(O.O_WEIGHTS, N)
+    x = ot.get(O.O_NODES, N)
+    P = desc.xy[desc.quads]                                      # (K, 4, 2)
+    XI, ETA = np.meshgrid(x, x, indexing="ij")
+    XI, ETA = XI.ravel()[None, :], ETA.ravel()[None, :]
+    d = lambda a, b, c: P[:, a, c:c + 1] - P[:, b, c:c + 1]
+    xxi = ((1 - ETA) * d(1, 0, 0) + (1 + ETA) * d(2, 3, 0)) / 4
+    xeta = ((1 - XI) * d(3, 0, 0) + (1 + XI) * d(2, 1, 0)) / 4
+    yxi = ((1 - ETA) * d(1, 0, 1) + (1 + ETA) * d(2, 3, 1)) / 4
+    yeta = ((1 - XI) * d(3, 0, 1) + (1 + XI) * d(2, 1, 1)) / 4
+    W = (np.outer(w, w).ravel()[None, :] * (xxi * yeta - xeta * yxi)).ravel()
+    assert abs(np.sum(W * Gp)) < 1e-11 * np.sum(W * np.abs(Gp))
