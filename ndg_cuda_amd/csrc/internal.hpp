@@ -29,10 +29,14 @@ struct BcSide {
 };
 
 // Per-order constant tables in device memory, packed as Dhat[Nf*Nf], l(-1)[Nf], l(+1)[Nf], w[Nf],
-// x[Nf], then the modal matrix poly[Nf*Nf] used by the estimator.
+// x[Nf], 1/w[Nf], then the modal matrix poly[Nf*Nf] used by the estimator.
 inline int64_t table_pack_len(int N) {
     const int nf = N + 1;
-    return (int64_t)nf * nf * 2 + 4 * nf;
+    return (int64_t)nf * nf * 2 + 5 * nf;
+}
+inline int64_t table_poly_offset(int N) {
+    const int nf = N + 1;
+    return (int64_t)nf * nf + 5 * nf;
 }
 
 struct Tables {

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <climits>
 #include <cmath>
 
 #include "internal.hpp"
@@ -119,171 +120,321 @@ traces_kernel(const double *__restrict__ tab, const double *__restrict__ q,
 }
 
 // ---------------------------------------------------------------------------------------------
-// One fused RK stage. RHS_ONLY writes G (Solver2D_t::compute_dg_wave_derivative output) and
-// leaves the state alone.
+// One fused RK stage, line-per-thread. A workgroup owns EPB elements; thread (element, field f,
+// line l) handles one (N+1)-point line of one field, so the D^ contractions run in registers with
+// the D^ rows as wave-uniform (scalar) loads, and LDS carries only the element tiles:
+//   P0  issue every HBM load of the tile at once: state -> LDS, RK register -> registers,
+//       neighbour traces of the thread's flux points -> registers
+//   P1  per-element metric lines and the element's own traces (row l and column l of field f)
+//   P2  upwind flux at the 4(N+1) side points, times the edge Jacobian
+//       (boundary_conditions / project_to_faces / calculate_wave_fluxes / project_to_elements,
+//       Mesh2D_t.cpp:3731-3935, Solver2D_t.cpp:48-63)
+//   P3  xi lines: D^ (y_eta f - x_eta g) + lift (Solver2D_t.cpp:71-100)
+//   P4  eta lines: D^ (-y_xi f + x_xi g) + lift, G = (-d_xi - d_eta)/J (:104-133), RK update
+//       (Solver2D_t.cpp:138-172)
+//   P5  coalesced stores of q and g, traces of the updated state for the next stage.
+// RHS_ONLY writes G (the compute_dg_wave_derivative output) and leaves the state alone.
+template <int N>
+struct LineShape {
+    static constexpr int NF = N + 1;
+    static constexpr int NE = NF * NF;
+    static constexpr int T = 256;
+    static constexpr int TPE = 3 * NF;  // threads per element
+    static constexpr int EPB = T / TPE;
+    static constexpr int FLUX_PER_T = (EPB * 4 * NF + T - 1) / T;
+};
+
 template <int N, bool RHS_ONLY>
-__global__ void __launch_bounds__(Shape<N>::THREADS)
+__global__ void __launch_bounds__(256)
 stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_nb,
              const BcSide *__restrict__ bc, const double *__restrict__ tab,
              double *__restrict__ q, double *__restrict__ g, const double *__restrict__ tr_in,
              double *__restrict__ tr_out, double *__restrict__ rhs, int64_t K, int first,
              double a, double gdt, double t_stage) {
-    using S = Shape<N>;
-    constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::THREADS;
+    using S = LineShape<N>;
+    constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::T, FPT = S::FLUX_PER_T;
     constexpr double c = kWaveSpeed;
-    __shared__ double s_Dh[NF * NF], s_lm[NF], s_lp[NF], s_w[NF], s_x[NF];
-    __shared__ double s_v[EPB * 8];
     __shared__ double s_q[EPB * 3 * NE];
-    __shared__ double s_ft[EPB * 3 * NE];
-    __shared__ double s_gt[EPB * 3 * NE];
-    __shared__ double s_fs[EPB * 12 * NF];
+    __shared__ double s_d[EPB * 3 * NE];
+    __shared__ double s_tr[EPB * 12 * NF];
+    __shared__ double s_met[EPB * 4 * NF];
+    __shared__ double s_ij[EPB * NE];      // 1/J per node
+    __shared__ double s_side[EPB * 4 * 3];  // outward normal and edge Jacobian per side
+    __shared__ double s_v[EPB * 8];
+    const double *Dh = tab;
+    const double *lm = tab + NF * NF;
+    const double *lp = lm + NF;
+    const double *x = lp + 2 * NF;
+    const double *winv = x + NF;
 
     const int tid = threadIdx.x;
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t e0 = blk * EPB;
     const int nel = (int)min<int64_t>(EPB, K - e0);
+    const int el = tid / (3 * NF), f = (tid / NF) % 3, l = tid % NF;
+    const bool active = tid < nel * 3 * NF;
 
-    for (int r = tid; r < NF * NF; r += T) s_Dh[r] = tab[r];
-    for (int r = tid; r < NF; r += T) {
-        s_lm[r] = tab[NF * NF + r];
-        s_lp[r] = tab[NF * NF + NF + r];
-        s_w[r] = tab[NF * NF + 2 * NF + r];
-        s_x[r] = tab[NF * NF + 3 * NF + r];
+    // P0: all HBM reads of the tile are issued before any is waited on (fixed trip counts, fully
+    // unrolled, register staged); the neighbour-trace loads wait only for the connectivity loads
+    // issued first.
+    int32_t code[FPT];
+#pragma unroll
+    for (int it = 0; it < FPT; ++it) {
+        const int item = tid + it * T;
+        code[it] = INT32_MIN;
+        if (item < nel * 4 * NF) code[it] = side_nb[(e0 + item / NF / 4) * 4 + (item / NF) % 4];
     }
-    for (int r = tid; r < nel * 8; r += T) s_v[r] = verts[e0 * 8 + r];
+    constexpr int QPT = (EPB * 3 * NE + T - 1) / T;
+    double qreg[QPT];
     const double *qg = q + e0 * 3 * NE;
-    for (int r = tid; r < nel * 3 * NE; r += T) s_q[r] = qg[r];
-
-    const int el = tid / NE, node = tid % NE;
-    const bool active = tid < nel * NE;
-    double g_old[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int it = 0; it < QPT; ++it) {
+        const int r = tid + it * T;
+        qreg[it] = r < nel * 3 * NE ? qg[r] : 0.0;
+    }
+    constexpr int VPT = (EPB * 8 + T - 1) / T;
+    double vreg[VPT];
+#pragma unroll
+    for (int it = 0; it < VPT; ++it) {
+        const int r = tid + it * T;
+        vreg[it] = r < nel * 8 ? verts[e0 * 8 + r] : 0.0;
+    }
+    double g_old[NF];
     if (!RHS_ONLY && active && !first) {
-        const double *gg = g + (e0 + el) * 3 * NE + node;
-        g_old[0] = gg[0];
-        g_old[1] = gg[NE];
-        g_old[2] = gg[2 * NE];
+        const double *gg = g + (e0 + el) * 3 * NE + f * NE + l * NF;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) g_old[j] = gg[j];
+    }
+    double nbv[FPT][3];
+#pragma unroll
+    for (int it = 0; it < FPT; ++it) {
+        nbv[it][0] = nbv[it][1] = nbv[it][2] = 0.0;
+        const int32_t cd = code[it];
+        if (cd >= 0) {
+            const int k = (tid + it * T) % NF;
+            const double *nb = tr_in + (int64_t)(cd >> 2) * 12 * NF + (cd & 3) * 3 * NF + (N - k);
+            nbv[it][0] = nb[0];
+            nbv[it][1] = nb[NF];
+            nbv[it][2] = nb[2 * NF];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < VPT; ++it) {
+        const int r = tid + it * T;
+        if (r < nel * 8) s_v[r] = vreg[it];
+    }
+#pragma unroll
+    for (int it = 0; it < QPT; ++it) {
+        const int r = tid + it * T;
+        if (r < nel * 3 * NE) s_q[r] = qreg[it];
     }
     __syncthreads();
 
-    // Phase 1a: contravariant volume fluxes at the own node (Solver2D_t.cpp:71-79, 104-112).
-    double jac = 1.0;
+    // P1
     if (active) {
-        const int i = node / NF, j = node % NF;
-        double xxi, xeta, yxi, yeta;
-        metrics(s_v + el * 8, s_x[i], s_x[j], xxi, xeta, yxi, yeta);
-        jac = xxi * yeta - xeta * yxi;
-        const double *tq = s_q + el * 3 * NE;
-        const double p = tq[node], u = tq[NE + node], v = tq[2 * NE + node];
-        double *ft = s_ft + el * 3 * NE, *gt = s_gt + el * 3 * NE;
-        ft[node] = yeta * (c * u) - xeta * (c * v);
-        ft[NE + node] = yeta * p;
-        ft[2 * NE + node] = -(xeta * p);
-        gt[node] = -yxi * (c * u) + xxi * (c * v);
-        gt[NE + node] = -yxi * p;
-        gt[2 * NE + node] = xxi * p;
+        const double *P = s_v + el * 8;
+        double *met = s_met + el * 4 * NF;
+        if (f == 0) {  // x_eta, y_eta depend on xi only (include/functions/quad_metrics.tpp:3-8)
+            const double xi = x[l];
+            met[l] = ((1 - xi) * (P[6] - P[0]) + (1 + xi) * (P[4] - P[2])) / 4;
+            met[NF + l] = ((1 - xi) * (P[7] - P[1]) + (1 + xi) * (P[5] - P[3])) / 4;
+        } else if (f == 1) {  // x_xi, y_xi depend on eta only
+            const double eta = x[l];
+            met[2 * NF + l] = ((1 - eta) * (P[2] - P[0]) + (1 + eta) * (P[4] - P[6])) / 4;
+            met[3 * NF + l] = ((1 - eta) * (P[3] - P[1]) + (1 + eta) * (P[5] - P[7])) / 4;
+        } else if (l < 4) {  // side l: outward normal of the edge P_l -> P_l+1 and its Jacobian
+            const int s = l, s1 = (s + 1) & 3;
+            const double ex = P[2 * s1] - P[2 * s], ey = P[2 * s1 + 1] - P[2 * s + 1];
+            const double len = sqrt(ex * ex + ey * ey);
+            double *sd = s_side + (el * 4 + s) * 3;
+            sd[0] = ey / len;
+            sd[1] = -(ex / len);
+            sd[2] = side_scale(P, s, x[0]);
+        }
+        {  // 1/J at nodes (l, j), j = f, f+3, ... (src/entities/Element2D_t.cpp:744-754)
+            const double xi = x[l];
+            const double xeta = ((1 - xi) * (P[6] - P[0]) + (1 + xi) * (P[4] - P[2])) / 4;
+            const double yeta = ((1 - xi) * (P[7] - P[1]) + (1 + xi) * (P[5] - P[3])) / 4;
+            for (int j = f; j < NF; j += 3) {
+                const double eta = x[j];
+                const double xxi = ((1 - eta) * (P[2] - P[0]) + (1 + eta) * (P[4] - P[6])) / 4;
+                const double yxi = ((1 - eta) * (P[3] - P[1]) + (1 + eta) * (P[5] - P[7])) / 4;
+                s_ij[el * NE + l * NF + j] = 1.0 / (xxi * yeta - xeta * yxi);
+            }
+        }
+        const double *tq = s_q + el * 3 * NE + f * NE;
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            const double row = tq[l * NF + j], col = tq[j * NF + l];
+            t0 += lm[j] * row;
+            t2 += lp[j] * row;
+            t1 += lp[j] * col;
+            t3 += lm[j] * col;
+        }
+        double *tr = s_tr + el * 12 * NF + f * NF;
+        tr[l] = t0;
+        tr[6 * NF + (N - l)] = t2;
+        tr[3 * NF + l] = t1;
+        tr[9 * NF + (N - l)] = t3;
     }
-    // Phase 1b: upwind flux on the element's sides, scaled by the edge Jacobian
-    // (boundary_conditions, project_to_faces, calculate_wave_fluxes, project_to_elements for
-    // conforming faces: Mesh2D_t.cpp:3731-3935, Solver2D_t.cpp:48-63).
-    for (int r = tid; r < nel * 4 * NF; r += T) {
-        const int fe = r / (4 * NF), rs = r % (4 * NF), s = rs / NF, k = rs % NF;
-        const double *tq = s_q + fe * 3 * NE;
-        const double *P = s_v + fe * 8;
-        const double pL = side_trace<N>(tq, s_lm, s_lp, s, k);
-        const double uL = side_trace<N>(tq + NE, s_lm, s_lp, s, k);
-        const double vL = side_trace<N>(tq + 2 * NE, s_lm, s_lp, s, k);
-        const int s1 = (s + 1) & 3;
-        const double ex = P[2 * s1] - P[2 * s], ey = P[2 * s1 + 1] - P[2 * s + 1];
-        const double len = sqrt(ex * ex + ey * ey);
-        double nx = ey / len, ny = -(ex / len);
-        double pR, uR, vR;
-        const int32_t code = side_nb[(e0 + fe) * 4 + s];
-        if (code >= 0) {
-            const double *nb = tr_in + (int64_t)(code >> 2) * 12 * NF + (code & 3) * 3 * NF + (N - k);
-            pR = nb[0];
-            uR = nb[NF];
-            vR = nb[2 * NF];
-        } else {
-            const BcSide rec = bc[-1 - code];
-            nx = rec.nx;
-            ny = rec.ny;
-            if (rec.kind == NDG_BC_WALL || rec.kind == NDG_BC_SYMMETRY) {
-                // ghost = sf * own trace, normal velocity reversed (Mesh2D_t.cpp:4377-4433)
-                pR = pL * rec.sf;
-                const double ux = uL * rec.sf, uy = vL * rec.sf;
-                const double l0 = -(ux * rec.nx + uy * rec.ny);
-                const double l1 = ux * rec.tx + uy * rec.ty;
-                uR = rec.nx * l0 + rec.tx * l1;
-                vR = rec.ny * l0 + rec.ty * l1;
-            } else if (rec.kind == NDG_BC_OUTFLOW) {
-                pR = pL * rec.sf;
-                uR = uL * rec.sf;
-                vR = vL * rec.sf;
-            } else if (rec.kind == NDG_BC_INFLOW) {
-                // ghost trace = g(x, t) at the ghost's nodes, seen reversed (Mesh2D_t.cpp:4575-4599)
-                const double interp = (s_x[N - k] + 1) / 2;
-                const double X = rec.bx * interp + rec.ax * (1 - interp);
-                const double Y = rec.by * interp + rec.ay * (1 - interp);
-                plane_wave(X, Y, t_stage, pR, uR, vR);
+    __syncthreads();
+
+    // P2
+#pragma unroll
+    for (int it = 0; it < FPT; ++it) {
+        const int item = tid + it * T;
+        if (item < nel * 4 * NF) {
+            const int fe = item / (4 * NF), s = (item / NF) % 4, k = item % NF;
+            double *tr = s_tr + fe * 12 * NF + s * 3 * NF + k;
+            const double pL = tr[0], uL = tr[NF], vL = tr[2 * NF];
+            const double *sd = s_side + (fe * 4 + s) * 3;
+            double nx = sd[0], ny = sd[1];
+            const double sf = sd[2];
+            double pR, uR, vR;
+            const int32_t cd = code[it];
+            if (cd >= 0) {
+                pR = nbv[it][0];
+                uR = nbv[it][1];
+                vR = nbv[it][2];
             } else {
-                pR = 0.0;
-                uR = 0.0;
-                vR = 0.0;
+                const BcSide rec = bc[-1 - cd];
+                nx = rec.nx;
+                ny = rec.ny;
+                if (rec.kind == NDG_BC_WALL || rec.kind == NDG_BC_SYMMETRY) {
+                    // ghost = sf * own trace, normal velocity reversed (Mesh2D_t.cpp:4377-4433)
+                    pR = pL * rec.sf;
+                    const double ux = uL * rec.sf, uy = vL * rec.sf;
+                    const double l0 = -(ux * rec.nx + uy * rec.ny);
+                    const double l1 = ux * rec.tx + uy * rec.ty;
+                    uR = rec.nx * l0 + rec.tx * l1;
+                    vR = rec.ny * l0 + rec.ty * l1;
+                } else if (rec.kind == NDG_BC_OUTFLOW) {
+                    pR = pL * rec.sf;
+                    uR = uL * rec.sf;
+                    vR = vL * rec.sf;
+                } else if (rec.kind == NDG_BC_INFLOW) {
+                    // ghost trace = g(x, t) at the ghost's nodes, seen reversed (Mesh2D_t.cpp:4575-4599)
+                    const double interp = (x[N - k] + 1) / 2;
+                    const double X = rec.bx * interp + rec.ax * (1 - interp);
+                    const double Y = rec.by * interp + rec.ay * (1 - interp);
+                    plane_wave(X, Y, t_stage, pR, uR, vR);
+                } else {
+                    pR = 0.0;
+                    uR = 0.0;
+                    vR = 0.0;
+                }
             }
+            const double wL = pL + c * (uL * nx + vL * ny);
+            const double wR = pR - c * (uR * nx + vR * ny);
+            tr[0] = (c * (wL - wR) / 2) * sf;
+            tr[NF] = (nx * (wL + wR) / 2) * sf;
+            tr[2 * NF] = (ny * (wL + wR) / 2) * sf;
         }
-        const double wL = pL + c * (uL * nx + vL * ny);
-        const double wR = pR - c * (uR * nx + vR * ny);
-        const double sf = side_scale(P, s, s_x[0]);
-        double *fs = s_fs + fe * 12 * NF + s * 3 * NF + k;
-        fs[0] = (c * (wL - wR) / 2) * sf;
-        fs[NF] = (nx * (wL + wR) / 2) * sf;
-        fs[2 * NF] = (ny * (wL + wR) / 2) * sf;
     }
     __syncthreads();
 
-    // Phase 2: D^ contraction along xi and eta plus lift (Solver2D_t.cpp:81-133), RK update
-    // (Solver2D_t.cpp:138-172).
+    const double *tq = s_q + el * 3 * NE;
+    const double *met = s_met + el * 4 * NF;
+    const double *fs = s_tr + el * 12 * NF + f * NF;
+    // P3: thread owns xi-line j = l of field f
     if (active) {
-        const int i = node / NF, j = node % NF;
-        const double *ft = s_ft + el * 3 * NE, *gt = s_gt + el * 3 * NE;
-        const double *fs = s_fs + el * 12 * NF;
-        double Gn[3];
+        const int j = l;
+        double F[NF];
 #pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            double dxi = 0.0, deta = 0.0;
-#pragma unroll
-            for (int k = 0; k < NF; ++k) dxi += s_Dh[i * NF + k] * ft[f * NE + k * NF + j];
-            dxi += (fs[3 * NF + f * NF + j] * s_lp[i] + fs[9 * NF + f * NF + (N - j)] * s_lm[i]) / s_w[i];
-#pragma unroll
-            for (int k = 0; k < NF; ++k) deta += s_Dh[j * NF + k] * gt[f * NE + i * NF + k];
-            deta += (fs[6 * NF + f * NF + (N - i)] * s_lp[j] + fs[f * NF + i] * s_lm[j]) / s_w[j];
-            Gn[f] = (-dxi - deta) / jac;
+        for (int k = 0; k < NF; ++k) {
+            const int node = k * NF + j;
+            const double xeta = met[k], yeta = met[NF + k];
+            if (f == 0) F[k] = yeta * (c * tq[NE + node]) - xeta * (c * tq[2 * NE + node]);
+            else if (f == 1) F[k] = yeta * tq[node];
+            else F[k] = -(xeta * tq[node]);
         }
-        const int64_t off = (e0 + el) * 3 * NE + node;
-        if (RHS_ONLY) {
-            rhs[off] = Gn[0];
-            rhs[off + NE] = Gn[1];
-            rhs[off + 2 * NE] = Gn[2];
-        } else {
-            double *tq = s_q + el * 3 * NE;
+        const double Fr = fs[3 * NF + j], Fl = fs[9 * NF + (N - j)];
+        double *d = s_d + el * 3 * NE + f * NE + j;
 #pragma unroll
-            for (int f = 0; f < 3; ++f) {
-                const double gn = first ? Gn[f] : a * g_old[f] + Gn[f];
-                const double qn = tq[f * NE + node] + gdt * gn;
-                g[off + f * NE] = gn;
-                q[off + f * NE] = qn;
-                tq[f * NE + node] = qn;
+        for (int i = 0; i < NF; ++i) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) acc += Dh[i * NF + k] * F[k];
+            acc += (Fr * lp[i] + Fl * lm[i]) * winv[i];
+            d[i * NF] = acc;
+        }
+    }
+    __syncthreads();
+
+    // P4: thread owns eta-line i = l of field f
+    double res[NF], qn[NF];
+    if (active) {
+        const int i = l;
+        double G[NF];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            const int node = i * NF + k;
+            const double xxi = met[2 * NF + k], yxi = met[3 * NF + k];
+            if (f == 0) G[k] = -yxi * (c * tq[NE + node]) + xxi * (c * tq[2 * NE + node]);
+            else if (f == 1) G[k] = -yxi * tq[node];
+            else G[k] = xxi * tq[node];
+        }
+        const double Ft = fs[6 * NF + (N - i)], Fb = fs[i];
+        const double *d = s_d + el * 3 * NE + f * NE + i * NF;
+        const double *qrow = tq + f * NE + i * NF;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) acc += Dh[j * NF + k] * G[k];
+            acc += (Ft * lp[j] + Fb * lm[j]) * winv[j];
+            const double Gv = (-d[j] - acc) * s_ij[el * NE + i * NF + j];
+            if (RHS_ONLY) {
+                res[j] = Gv;
+            } else {
+                res[j] = first ? Gv : a * g_old[j] + Gv;
+                qn[j] = qrow[j] + gdt * res[j];
             }
         }
     }
-    if (RHS_ONLY) return;
+    __syncthreads();
+    if (active) {
+        double *d = s_d + el * 3 * NE + f * NE + l * NF;
+        double *qw = s_q + el * 3 * NE + f * NE + l * NF;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            d[j] = res[j];
+            if (!RHS_ONLY) qw[j] = qn[j];
+        }
+    }
     __syncthreads();
 
-    // Phase 3: traces of the updated state for the next stage.
-    for (int r = tid; r < nel * 12 * NF; r += T) {
-        const int fe = r / (12 * NF), rs = r % (12 * NF);
-        const int s = rs / (3 * NF), f = (rs / NF) % 3, k = rs % NF;
-        tr_out[(e0 + fe) * 12 * NF + rs] = side_trace<N>(s_q + fe * 3 * NE + f * NE, s_lm, s_lp, s, k);
+    // P5
+    if (RHS_ONLY) {
+        double *rg = rhs + e0 * 3 * NE;
+        for (int r = tid; r < nel * 3 * NE; r += T) rg[r] = s_d[r];
+        return;
+    }
+    {
+        double *qg = q + e0 * 3 * NE;
+        double *gg = g + e0 * 3 * NE;
+        for (int r = tid; r < nel * 3 * NE; r += T) {
+            qg[r] = s_q[r];
+            gg[r] = s_d[r];
+        }
+    }
+    if (active) {
+        const double *tqf = s_q + el * 3 * NE + f * NE;
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            const double row = tqf[l * NF + j], col = tqf[j * NF + l];
+            t0 += lm[j] * row;
+            t2 += lp[j] * row;
+            t1 += lp[j] * col;
+            t3 += lm[j] * col;
+        }
+        double *to = tr_out + (e0 + el) * 12 * NF + f * NF;
+        to[l] = t0;
+        to[6 * NF + (N - l)] = t2;
+        to[3 * NF + l] = t1;
+        to[9 * NF + (N - l)] = t3;
     }
 }
 
@@ -376,7 +527,7 @@ estimate_kernel(const double *__restrict__ tab, const double *__restrict__ q, in
     constexpr int n = NF < 4 ? NF : 4;
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= K) return;
-    const double *M = tab + NF * NF + 4 * NF;
+    const double *M = tab + NF * NF + 5 * NF;
     const double *p = q + e * 3 * NE;
     double spec[n];
     for (int ni = 0; ni < n; ++ni) {
@@ -436,9 +587,9 @@ static hipError_t traces_n(const DeviceMesh &m, hipStream_t s) {
 
 template <int N>
 static hipError_t stage_n(DeviceMesh &m, int stage, double t_stage, double a, double gdt, hipStream_t s) {
-    using S = Shape<N>;
+    using S = LineShape<N>;
     const int64_t blocks = (m.K + S::EPB - 1) / S::EPB;
-    hipLaunchKernelGGL((stage_kernel<N, false>), dim3((unsigned)blocks), dim3(S::THREADS), 0, s, m.verts,
+    hipLaunchKernelGGL((stage_kernel<N, false>), dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts,
                        m.side_nb, m.bc, m.tab, m.q, m.g, m.trace[m.cur], m.trace[1 - m.cur], nullptr, m.K,
                        stage == 0 ? 1 : 0, a, gdt, t_stage);
     m.cur = 1 - m.cur;
@@ -447,9 +598,9 @@ static hipError_t stage_n(DeviceMesh &m, int stage, double t_stage, double a, do
 
 template <int N>
 static hipError_t rhs_n(DeviceMesh &m, double t, hipStream_t s) {
-    using S = Shape<N>;
+    using S = LineShape<N>;
     const int64_t blocks = (m.K + S::EPB - 1) / S::EPB;
-    hipLaunchKernelGGL((stage_kernel<N, true>), dim3((unsigned)blocks), dim3(S::THREADS), 0, s, m.verts,
+    hipLaunchKernelGGL((stage_kernel<N, true>), dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts,
                        m.side_nb, m.bc, m.tab, m.q, m.g, m.trace[m.cur], nullptr, m.rhs, m.K, 1, 0.0, 0.0, t);
     return hipGetLastError();
 }

@@ -166,9 +166,9 @@ std::vector<double> pack_tables(const Tables &t, int N) {
     out.insert(out.end(), Dh.begin(), Dh.end());
     for (int which : {NDG_TABLE_LAGRANGE_LEFT, NDG_TABLE_LAGRANGE_RIGHT, NDG_TABLE_WEIGHTS, NDG_TABLE_NODES})
         out.insert(out.end(), t.tab[which][N].begin(), t.tab[which][N].end());
+    for (int i = 0; i < nf; ++i) out.push_back(1.0 / t.tab[NDG_TABLE_WEIGHTS][N][i]);
     const auto &M = t.tab[NDG_TABLE_POLYNOMIALS][N];
     out.insert(out.end(), M.begin(), M.end());
-    (void)nf;
     return out;
 }
 

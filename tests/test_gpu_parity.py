@@ -164,8 +164,16 @@ def test_estimator(tables):
     om.initial_conditions(pulses)
     eg, sg, rg, cg = gm.estimate_error()
     eo, so, ro, co = om.estimate_error()
-    assert rel(eg, eo) < 1e-12 and rel(sg, so) < 1e-12
-    assert np.array_equal(rg, ro) and np.array_equal(cg, co)
+    # The spectrum sums products of the field with the modal matrix and cancels down to tiny
+    # values on smooth elements: its roundoff is absolute, of order eps * max|p| (max|p| = 1 here),
+    # so the error estimate is compared absolutely and sigma (a log-fit) only where the estimate is
+    # well above that roundoff.
+    assert np.max(np.abs(eg - eo)) < 1e-13
+    big = eo > 1e-7
+    assert big.sum() > 5
+    assert np.max(np.abs(sg[big] - so[big]) / so[big]) < 1e-8
+    tie = np.abs(eo - 1e-6) < 1e-12
+    assert np.array_equal(rg[~tie], ro[~tie]) and np.array_equal(cg, co)
 
 
 def test_full_size_properties(tables):
