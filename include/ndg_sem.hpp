@@ -1,0 +1,260 @@
+// ndg_sem.hpp -- header-only C++ mirror of the reference host API over the C ABI (ndg_amd.h).
+//
+// Class and member names, constructor arguments and call order follow guillaumetousignant/NDG_cuda's
+// SEM::Device classes:
+//   SEM::Device::Entities::NDG_t<Polynomial>(N_max, n_interpolation_points)   include/entities/NDG_t.cuh:11-29
+//   SEM::Device::Meshes::Mesh2D_t(mesh, initial_N, maximum_N, n_interpolation_points,
+//       max_split_level, adaptivity_interval, load_balancing_interval, tolerance_min,
+//       tolerance_max, load_balancing_threshold, NDG, stream)              include/meshes/Mesh2D_t.cuh:23
+//   mesh.initial_conditions(...)                                              src/meshes/Mesh2D_t.cpp:1192-1216
+//   SEM::Device::Solvers::Solver2D_t(CFL, output_times, viscosity)            include/solvers/Solver2D_t.cuh
+//   solver.solve(NDG, mesh)                                                   include/solvers/Solver2D_t.tcu:7-135
+// Differences: the mesh comes from an in-memory description (or the square generator) instead of
+// a CGNS/SU2 path, no VTK writer is attached, and errors throw SEM::Device::Error (carrying the
+// status code) instead of calling exit().
+#ifndef NDG_SEM_HPP
+#define NDG_SEM_HPP
+
+#include <array>
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ndg_amd.h"
+
+namespace SEM {
+namespace Device {
+
+class Error : public std::runtime_error {
+  public:
+    Error(ndg_status code, const std::string &msg) : std::runtime_error(msg), code_(code) {}
+    ndg_status code() const { return code_; }
+
+  private:
+    ndg_status code_;
+};
+
+inline void check(ndg_status status) {
+    if (status != NDG_OK) {
+        char buf[1024];
+        ndg_last_error(buf, sizeof buf);
+        throw Error(status, buf);
+    }
+}
+
+namespace Polynomials {
+// Legendre-Gauss nodes (the only polynomial family on the reference's 2D path).
+struct LegendrePolynomial_t {};
+}  // namespace Polynomials
+
+namespace Entities {
+
+template <typename Polynomial>
+class NDG_t {
+  public:
+    NDG_t(int N_max, std::size_t n_interpolation_points)
+        : N_max_(N_max), N_interpolation_points_(n_interpolation_points) {
+        check(ndg_tables_create(N_max, (int64_t)n_interpolation_points, &handle_));
+        std::vector<std::vector<double>> *tables[12] = {
+            &nodes_, &weights_, &barycentric_weights_, &lagrange_interpolant_left_,
+            &lagrange_interpolant_right_, &lagrange_interpolant_derivative_left_,
+            &lagrange_interpolant_derivative_right_, &derivative_matrices_, &g_hat_derivative_matrices_,
+            &derivative_matrices_hat_, &interpolation_matrices_, &polynomials_};
+        for (int which = 0; which < 12; ++which) {
+            tables[which]->resize(N_max + 1);
+            for (int N = 0; N <= N_max; ++N) {
+                int64_t count = 0;
+                check(ndg_tables_get(handle_, which, N, nullptr, 0, &count));
+                (*tables[which])[N].resize((std::size_t)count);
+                check(ndg_tables_get(handle_, which, N, (*tables[which])[N].data(), count, nullptr));
+            }
+        }
+    }
+    NDG_t(const NDG_t &) = delete;
+    NDG_t &operator=(const NDG_t &) = delete;
+    ~NDG_t() { ndg_tables_destroy(handle_); }
+
+    const ndg_tables *handle() const { return handle_; }
+
+    int N_max_;
+    std::size_t N_interpolation_points_;
+    std::vector<std::vector<double>> nodes_;
+    std::vector<std::vector<double>> weights_;
+    std::vector<std::vector<double>> barycentric_weights_;
+    std::vector<std::vector<double>> lagrange_interpolant_left_;
+    std::vector<std::vector<double>> lagrange_interpolant_right_;
+    std::vector<std::vector<double>> lagrange_interpolant_derivative_left_;
+    std::vector<std::vector<double>> lagrange_interpolant_derivative_right_;
+    std::vector<std::vector<double>> derivative_matrices_;
+    std::vector<std::vector<double>> g_hat_derivative_matrices_;
+    std::vector<std::vector<double>> derivative_matrices_hat_;
+    std::vector<std::vector<double>> interpolation_matrices_;
+    std::vector<std::vector<double>> polynomials_;
+
+  private:
+    ndg_tables *handle_ = nullptr;
+};
+
+}  // namespace Entities
+
+namespace Meshes {
+
+// Content of the mesh file the reference reads (src/meshes/Mesh2D_t.cpp:422-1048).
+struct MeshDescription {
+    std::vector<double> xy;         // 2 per node
+    std::vector<int64_t> quads;     // 4 per element, counter-clockwise, element order = storage order
+    std::vector<int32_t> quad_N;    // empty: initial_N everywhere
+    std::vector<int64_t> bars;      // 2 per boundary element
+    std::vector<int32_t> bar_type;  // NDG_BC_*
+    std::vector<int64_t> bar_donor; // periodic donor bar or -1
+};
+
+// extras/mesh_generator/main.cpp:161-382 (square, Hilbert order, optional periodicity and jitter).
+inline MeshDescription square_mesh(const ndg_square_mesh_params &params) {
+    int64_t nn = 0, nq = 0, nb = 0;
+    check(ndg_square_mesh_sizes(&params, &nn, &nq, &nb));
+    MeshDescription m;
+    m.xy.resize(2 * nn);
+    m.quads.resize(4 * nq);
+    m.bars.resize(2 * nb);
+    m.bar_type.resize(nb);
+    m.bar_donor.resize(nb);
+    check(ndg_square_mesh_fill(&params, m.xy.data(), m.quads.data(), m.bars.data(), m.bar_type.data(),
+                               m.bar_donor.data()));
+    return m;
+}
+
+class Mesh2D_t {
+  public:
+    template <typename Polynomial>
+    Mesh2D_t(const MeshDescription &mesh, int initial_N, int maximum_N, std::size_t n_interpolation_points,
+             int max_split_level, std::size_t adaptivity_interval, std::size_t load_balancing_interval,
+             double tolerance_min, double tolerance_max, double load_balancing_threshold,
+             const Entities::NDG_t<Polynomial> &NDG, void *stream = nullptr)
+        : initial_N_(initial_N), maximum_N_(maximum_N), n_interpolation_points_(n_interpolation_points),
+          max_split_level_(max_split_level), adaptivity_interval_(adaptivity_interval),
+          load_balancing_interval_(load_balancing_interval), tolerance_min_(tolerance_min),
+          tolerance_max_(tolerance_max), load_balancing_threshold_(load_balancing_threshold) {
+        if (adaptivity_interval != 0) throw Error(NDG_ERR_UNSUPPORTED, "Mesh2D_t: adaptivity is not available in this build");
+        ndg_mesh_desc d{};
+        d.n_nodes = (int64_t)(mesh.xy.size() / 2);
+        d.xy = mesh.xy.data();
+        d.n_quads = (int64_t)(mesh.quads.size() / 4);
+        d.quads = mesh.quads.data();
+        d.quad_N = mesh.quad_N.empty() ? nullptr : mesh.quad_N.data();
+        d.n_bars = (int64_t)(mesh.bars.size() / 2);
+        d.bars = mesh.bars.data();
+        d.bar_type = mesh.bar_type.data();
+        d.bar_donor = mesh.bar_donor.data();
+        d.initial_N = initial_N;
+        d.maximum_N = maximum_N;
+        d.max_split_level = max_split_level;
+        d.tolerance_min = tolerance_min;
+        d.tolerance_max = tolerance_max;
+        check(ndg_mesh_create(&d, NDG.handle(), &handle_));
+        if (stream) check(ndg_mesh_set_stream(handle_, stream));
+        int64_t faces = 0, dofs = 0, ne = 0;
+        int32_t nmin = 0, nmax = 0;
+        check(ndg_mesh_info(handle_, &ne, &faces, &dofs, &nmin, &nmax));
+        n_elements_ = (std::size_t)ne;
+        n_faces_ = (std::size_t)faces;
+        n_dofs_per_field_ = (std::size_t)dofs;
+    }
+    Mesh2D_t(const Mesh2D_t &) = delete;
+    Mesh2D_t &operator=(const Mesh2D_t &) = delete;
+    ~Mesh2D_t() { ndg_mesh_destroy(handle_); }
+
+    // The reference plane wave g(x, 0).
+    void initial_conditions() {
+        ndg_ic_desc ic{NDG_IC_PLANE_WAVE, 0, nullptr};
+        check(ndg_set_ic(handle_, &ic));
+    }
+    // Radial Gaussian pulses {x0, y0, width, amplitude}; u = v = 0.
+    void initial_conditions(const std::vector<std::array<double, 4>> &pulses) {
+        ndg_ic_desc ic{NDG_IC_PULSES, (int32_t)pulses.size(), pulses.empty() ? nullptr : pulses[0].data()};
+        check(ndg_set_ic(handle_, &ic));
+    }
+    // Packed fields in reference node order.
+    void get_solution(std::vector<double> &p, std::vector<double> &u, std::vector<double> &v) {
+        p.resize(n_dofs_per_field_);
+        u.resize(n_dofs_per_field_);
+        v.resize(n_dofs_per_field_);
+        check(ndg_get_state(handle_, p.data(), u.data(), v.data()));
+    }
+    void set_solution(const std::vector<double> &p, const std::vector<double> &u, const std::vector<double> &v) {
+        if (p.size() != n_dofs_per_field_ || u.size() != n_dofs_per_field_ || v.size() != n_dofs_per_field_)
+            throw Error(NDG_ERR_INVALID, "Mesh2D_t::set_solution: wrong field length");
+        check(ndg_set_state(handle_, p.data(), u.data(), v.data()));
+    }
+    // Traces of the current state, [element][side][field][N+1].
+    void interpolate_to_boundaries(std::vector<double> &traces) {
+        int32_t nmin = 0, nmax = 0;
+        check(ndg_mesh_info(handle_, nullptr, nullptr, nullptr, &nmin, &nmax));
+        traces.resize(n_elements_ * 12 * (std::size_t)(nmax + 1));
+        check(ndg_get_traces(handle_, traces.data()));
+    }
+    void estimate_error(std::vector<double> &error, std::vector<double> &sigma, std::vector<int32_t> &refine,
+                        std::vector<int32_t> &coarsen) {
+        error.resize(n_elements_);
+        sigma.resize(n_elements_);
+        refine.resize(n_elements_);
+        coarsen.resize(n_elements_);
+        check(ndg_estimate_error(handle_, error.data(), sigma.data(), refine.data(), coarsen.data()));
+    }
+
+    ndg_mesh *handle() { return handle_; }
+
+    int initial_N_;
+    int maximum_N_;
+    std::size_t n_interpolation_points_;
+    int max_split_level_;
+    std::size_t adaptivity_interval_;
+    std::size_t load_balancing_interval_;
+    double tolerance_min_;
+    double tolerance_max_;
+    double load_balancing_threshold_;
+    std::size_t n_elements_ = 0;
+    std::size_t n_faces_ = 0;
+    std::size_t n_dofs_per_field_ = 0;
+
+  private:
+    ndg_mesh *handle_ = nullptr;
+};
+
+}  // namespace Meshes
+
+namespace Solvers {
+
+class Solver2D_t {
+  public:
+    Solver2D_t(double CFL, std::vector<double> output_times, double viscosity)
+        : CFL_(CFL), output_times_(std::move(output_times)), viscosity_(viscosity) {}
+
+    // Advances to output_times_.back() (include/solvers/Solver2D_t.tpp:37-72); no output is written.
+    template <typename Polynomial>
+    void solve(const Entities::NDG_t<Polynomial> &NDG, Meshes::Mesh2D_t &mesh) {
+        (void)NDG;
+        if (output_times_.empty()) throw Error(NDG_ERR_INVALID, "Solver2D_t::solve: no output times");
+        check(ndg_solve(mesh.handle(), CFL_, output_times_.back(), &time_, &n_steps_));
+    }
+
+    double get_delta_t(Meshes::Mesh2D_t &mesh) const {
+        double dt = 0.0;
+        check(ndg_get_delta_t(mesh.handle(), CFL_, &dt));
+        return dt;
+    }
+
+    double CFL_;
+    std::vector<double> output_times_;
+    double viscosity_;
+    double time_ = 0.0;
+    int64_t n_steps_ = 0;
+};
+
+}  // namespace Solvers
+}  // namespace Device
+}  // namespace SEM
+
+#endif

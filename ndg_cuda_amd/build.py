@@ -51,7 +51,21 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_host_driver()
     return LIB
+
+
+HOST_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "host_api_main.cpp")
+HOST_DRIVER = os.path.join(ROOT, "tests", "cpp", "host_api_main")
+
+
+def build_host_driver():
+    """Links the C++ host-API driver (tests/cpp/host_api_main.cpp) against libndg_amd.so with g++."""
+    if not os.path.exists(HOST_DRIVER_SRC):
+        return None
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), HOST_DRIVER_SRC,
+          "-L" + PKG, "-lndg_amd", "-Wl,-rpath,$ORIGIN/../../ndg_cuda_amd", "-o", HOST_DRIVER])
+    return HOST_DRIVER
 
 
 if __name__ == "__main__":

@@ -1,0 +1,89 @@
+"""Summaries of rocprofv3 output (rocpd SQLite databases) for the stage kernels.
+
+    python -m ndg_cuda_amd.profiling trace <run_results.db> [--kernel SUBSTR]
+    python -m ndg_cuda_amd.profiling traffic --fetch DB --write DB --copy-fetch DB --copy-write DB
+    python -m ndg_cuda_amd.profiling counters DB [DB ...] [--kernel SUBSTR]
+
+`trace` prints per-kernel call counts and durations from a --kernel-trace run. `traffic` turns one
+FETCH_SIZE run and one WRITE_SIZE run of the benchmark into HBM bytes per launch of the kernel,
+each scaled by what the same counter reads for a plain 1 GiB device copy (bench.py
+--copy-calibration): scale = bytes the copy moves / counter reading for the copy.
+"""
+import argparse
+import glob
+import json
+import os
+import sqlite3
+import sys
+
+
+def _db(path):
+    if os.path.isdir(path):
+        found = glob.glob(os.path.join(path, "**", "*.db"), recursive=True)
+        if not found:
+            raise SystemExit(f"no .db under {path}")
+        path = found[0]
+    return sqlite3.connect(path)
+
+
+def kernel_stats(path, kernel=None):
+    c = _db(path)
+    rows = c.execute("select name, count(*), avg(duration), min(duration), max(duration), sum(duration) "
+                     "from kernels group by name order by sum(duration) desc").fetchall()
+    out = []
+    for name, n, avg, mn, mx, tot in rows:
+        if kernel and kernel not in name:
+            continue
+        out.append({"kernel": name, "calls": n, "avg_ns": avg, "min_ns": mn, "max_ns": mx, "total_ns": tot})
+    return out
+
+
+def counter_means(path, kernel):
+    c = _db(path)
+    rows = c.execute("select counter_name, count(*), avg(value) from counters_collection "
+                     "where kernel_name like ? group by counter_name", (f"%{kernel}%",)).fetchall()
+    return {name: {"dispatches": n, "mean": v} for name, n, v in rows}
+
+
+def traffic(fetch, write, copy_fetch, copy_write, kernel, copy_kernel="copyBuffer", copy_bytes=1 << 30):
+    f = counter_means(fetch, kernel)["FETCH_SIZE"]["mean"]
+    w = counter_means(write, kernel)["WRITE_SIZE"]["mean"]
+    cf = counter_means(copy_fetch, copy_kernel)["FETCH_SIZE"]["mean"]
+    cw = counter_means(copy_write, copy_kernel)["WRITE_SIZE"]["mean"]
+    fs, ws = copy_bytes / (cf * 1024.0), copy_bytes / (cw * 1024.0)
+    read_b, write_b = f * 1024.0 * fs, w * 1024.0 * ws
+    return {"kernel": kernel, "fetch_kib_per_dispatch": f, "write_kib_per_dispatch": w,
+            "copy_fetch_kib": cf, "copy_write_kib": cw, "fetch_scale": fs, "write_scale": ws,
+            "hbm_read_bytes_per_launch": read_b, "hbm_write_bytes_per_launch": write_b,
+            "hbm_bytes_per_launch": read_b + write_b}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("trace")
+    t.add_argument("db")
+    t.add_argument("--kernel")
+    tr = sub.add_parser("traffic")
+    for k in ("fetch", "write", "copy_fetch", "copy_write"):
+        tr.add_argument("--" + k.replace("_", "-"), required=True)
+    tr.add_argument("--kernel", default="stage_kernel<8, false>")
+    cc = sub.add_parser("counters")
+    cc.add_argument("dbs", nargs="+")
+    cc.add_argument("--kernel", default="stage_kernel<8, false>")
+    a = ap.parse_args(argv)
+    if a.cmd == "trace":
+        for r in kernel_stats(a.db, a.kernel):
+            print(f"{r['calls']:6d} calls  avg {r['avg_ns'] / 1e3:10.2f} us  min {r['min_ns'] / 1e3:10.2f}  "
+                  f"max {r['max_ns'] / 1e3:10.2f}  total {r['total_ns'] / 1e6:9.3f} ms  {r['kernel'][:90]}")
+    elif a.cmd == "traffic":
+        print(json.dumps(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.kernel), indent=1))
+    else:
+        for db in a.dbs:
+            for name, v in sorted(counter_means(db, a.kernel).items()):
+                print(f"{name:28s} {v['mean']:.6g}  ({v['dispatches']} dispatches)")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
