@@ -120,12 +120,15 @@ traces_kernel(const double *__restrict__ tab, const double *__restrict__ q,
 }
 
 // ---------------------------------------------------------------------------------------------
-// One fused RK stage, line-per-thread. A workgroup owns EPB elements; thread (element, field f,
-// line l) handles one (N+1)-point line of one field, so the D^ contractions run in registers with
-// the D^ rows as wave-uniform (scalar) loads, and LDS carries only the element tiles:
-//   P0  issue every HBM load of the tile at once: state -> LDS, RK register -> registers,
-//       neighbour traces of the thread's flux points -> registers
-//   P1  per-element metric lines and the element's own traces (row l and column l of field f)
+// One fused RK stage. A workgroup is three waves, one per field (p, u, v); lane (element e, line l)
+// of the wave for field f handles one (N+1)-point line of that field for EPB = floor(64/(N+1))
+// elements. Every field-dependent branch is therefore wave-uniform (a scalar branch, no masked
+// lanes), the D^ contractions run in registers with the D^ rows as wave-uniform scalar loads, and
+// LDS carries the element tiles:
+//   P0  every HBM read of the tile is issued before any is waited on: state tile -> LDS, the
+//       thread's RK-register line and the neighbour traces of its flux points -> registers
+//   P1  the element's own traces (row l and column l of field f); wave 0 the metric lines along
+//       xi, wave 1 along eta, wave 2 the side normals and edge Jacobians
 //   P2  upwind flux at the 4(N+1) side points, times the edge Jacobian
 //       (boundary_conditions / project_to_faces / calculate_wave_fluxes / project_to_elements,
 //       Mesh2D_t.cpp:3731-3935, Solver2D_t.cpp:48-63)
@@ -138,27 +141,36 @@ template <int N>
 struct LineShape {
     static constexpr int NF = N + 1;
     static constexpr int NE = NF * NF;
-    static constexpr int T = 256;
-    static constexpr int TPE = 3 * NF;  // threads per element
-    static constexpr int EPB = T / TPE;
-    static constexpr int FLUX_PER_T = (EPB * 4 * NF + T - 1) / T;
+    static constexpr int T = 192;                            // one wave per field
+    static constexpr int EPB = 64 / NF;                       // elements per workgroup
+    static constexpr int FPT = (EPB * 4 * NF + T - 1) / T;    // flux points per thread
+    static constexpr int QPT = (EPB * 3 * NE + T - 1) / T;    // state values per thread (tile load)
+    static constexpr int VPT = (EPB * 8 + T - 1) / T;         // vertex coordinates per thread
 };
 
+// 1/x from the hardware reciprocal estimate and two Newton steps (quadratic convergence from its
+// ~2^-29 relative error to the rounding error of the last step).
+__device__ __forceinline__ double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
 template <int N, bool RHS_ONLY>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(192)
 stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_nb,
              const BcSide *__restrict__ bc, const double *__restrict__ tab,
              double *__restrict__ q, double *__restrict__ g, const double *__restrict__ tr_in,
              double *__restrict__ tr_out, double *__restrict__ rhs, int64_t K, int first,
              double a, double gdt, double t_stage) {
     using S = LineShape<N>;
-    constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::T, FPT = S::FLUX_PER_T;
+    constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::T, FPT = S::FPT;
     constexpr double c = kWaveSpeed;
     __shared__ double s_q[EPB * 3 * NE];
     __shared__ double s_d[EPB * 3 * NE];
     __shared__ double s_tr[EPB * 12 * NF];
     __shared__ double s_met[EPB * 4 * NF];
-    __shared__ double s_ij[EPB * NE];      // 1/J per node
     __shared__ double s_side[EPB * 4 * 3];  // outward normal and edge Jacobian per side
     __shared__ double s_v[EPB * 8];
     const double *Dh = tab;
@@ -168,42 +180,42 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
     const double *winv = x + NF;
 
     const int tid = threadIdx.x;
+    const int f = __builtin_amdgcn_readfirstlane(tid >> 6);  // field of this wave
+    const int lane = tid & 63;
+    const int el = lane / NF, l = lane % NF;
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t e0 = blk * EPB;
     const int nel = (int)min<int64_t>(EPB, K - e0);
-    const int el = tid / (3 * NF), f = (tid / NF) % 3, l = tid % NF;
-    const bool active = tid < nel * 3 * NF;
+    const bool active = el < nel;
 
-    // P0: all HBM reads of the tile are issued before any is waited on (fixed trip counts, fully
-    // unrolled, register staged); the neighbour-trace loads wait only for the connectivity loads
-    // issued first.
+    // P0
     int32_t code[FPT];
 #pragma unroll
     for (int it = 0; it < FPT; ++it) {
         const int item = tid + it * T;
-        code[it] = INT32_MIN;
-        if (item < nel * 4 * NF) code[it] = side_nb[(e0 + item / NF / 4) * 4 + (item / NF) % 4];
+        code[it] = item < nel * 4 * NF ? side_nb[(e0 + item / NF / 4) * 4 + (item / NF) % 4] : INT32_MIN;
     }
-    constexpr int QPT = (EPB * 3 * NE + T - 1) / T;
-    double qreg[QPT];
+    double qreg[S::QPT];
     const double *qg = q + e0 * 3 * NE;
 #pragma unroll
-    for (int it = 0; it < QPT; ++it) {
+    for (int it = 0; it < S::QPT; ++it) {
         const int r = tid + it * T;
         qreg[it] = r < nel * 3 * NE ? qg[r] : 0.0;
     }
-    constexpr int VPT = (EPB * 8 + T - 1) / T;
-    double vreg[VPT];
+    double vreg[S::VPT];
 #pragma unroll
-    for (int it = 0; it < VPT; ++it) {
+    for (int it = 0; it < S::VPT; ++it) {
         const int r = tid + it * T;
         vreg[it] = r < nel * 8 ? verts[e0 * 8 + r] : 0.0;
     }
     double g_old[NF];
-    if (!RHS_ONLY && active && !first) {
+    if (!RHS_ONLY && !first && active) {
         const double *gg = g + (e0 + el) * 3 * NE + f * NE + l * NF;
 #pragma unroll
         for (int j = 0; j < NF; ++j) g_old[j] = gg[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < NF; ++j) g_old[j] = 0.0;
     }
     double nbv[FPT][3];
 #pragma unroll
@@ -219,12 +231,12 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
         }
     }
 #pragma unroll
-    for (int it = 0; it < VPT; ++it) {
+    for (int it = 0; it < S::VPT; ++it) {
         const int r = tid + it * T;
         if (r < nel * 8) s_v[r] = vreg[it];
     }
 #pragma unroll
-    for (int it = 0; it < QPT; ++it) {
+    for (int it = 0; it < S::QPT; ++it) {
         const int r = tid + it * T;
         if (r < nel * 3 * NE) s_q[r] = qreg[it];
     }
@@ -250,17 +262,6 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
             sd[0] = ey / len;
             sd[1] = -(ex / len);
             sd[2] = side_scale(P, s, x[0]);
-        }
-        {  // 1/J at nodes (l, j), j = f, f+3, ... (src/entities/Element2D_t.cpp:744-754)
-            const double xi = x[l];
-            const double xeta = ((1 - xi) * (P[6] - P[0]) + (1 + xi) * (P[4] - P[2])) / 4;
-            const double yeta = ((1 - xi) * (P[7] - P[1]) + (1 + xi) * (P[5] - P[3])) / 4;
-            for (int j = f; j < NF; j += 3) {
-                const double eta = x[j];
-                const double xxi = ((1 - eta) * (P[2] - P[0]) + (1 + eta) * (P[4] - P[6])) / 4;
-                const double yxi = ((1 - eta) * (P[3] - P[1]) + (1 + eta) * (P[5] - P[7])) / 4;
-                s_ij[el * NE + l * NF + j] = 1.0 / (xxi * yeta - xeta * yxi);
-            }
         }
         const double *tq = s_q + el * 3 * NE + f * NE;
         double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
@@ -362,61 +363,69 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
     }
     __syncthreads();
 
-    // P4: thread owns eta-line i = l of field f
-    double res[NF], qn[NF];
+    // P4: thread owns eta-line i = l of field f. The new RK register goes straight back into its
+    // own row of s_d (read by no other thread); the new state waits for the barrier.
+    double qn[NF];
     if (active) {
         const int i = l;
-        double G[NF];
+        double Gt[NF];
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
             const int node = i * NF + k;
             const double xxi = met[2 * NF + k], yxi = met[3 * NF + k];
-            if (f == 0) G[k] = -yxi * (c * tq[NE + node]) + xxi * (c * tq[2 * NE + node]);
-            else if (f == 1) G[k] = -yxi * tq[node];
-            else G[k] = xxi * tq[node];
+            if (f == 0) Gt[k] = -yxi * (c * tq[NE + node]) + xxi * (c * tq[2 * NE + node]);
+            else if (f == 1) Gt[k] = -yxi * tq[node];
+            else Gt[k] = xxi * tq[node];
         }
         const double Ft = fs[6 * NF + (N - i)], Fb = fs[i];
-        const double *d = s_d + el * 3 * NE + f * NE + i * NF;
+        const double xeta_i = met[i], yeta_i = met[NF + i];
+        double *d = s_d + el * 3 * NE + f * NE + i * NF;
         const double *qrow = tq + f * NE + i * NF;
 #pragma unroll
         for (int j = 0; j < NF; ++j) {
             double acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < NF; ++k) acc += Dh[j * NF + k] * G[k];
+            for (int k = 0; k < NF; ++k) acc += Dh[j * NF + k] * Gt[k];
             acc += (Ft * lp[j] + Fb * lm[j]) * winv[j];
-            const double Gv = (-d[j] - acc) * s_ij[el * NE + i * NF + j];
+            const double jac = met[2 * NF + j] * yeta_i - xeta_i * met[3 * NF + j];
+            const double Gv = (-d[j] - acc) * recip(jac);
             if (RHS_ONLY) {
-                res[j] = Gv;
+                d[j] = Gv;
             } else {
-                res[j] = first ? Gv : a * g_old[j] + Gv;
-                qn[j] = qrow[j] + gdt * res[j];
+                const double gn = first ? Gv : a * g_old[j] + Gv;
+                d[j] = gn;
+                qn[j] = qrow[j] + gdt * gn;
             }
         }
     }
     __syncthreads();
-    if (active) {
-        double *d = s_d + el * 3 * NE + f * NE + l * NF;
+    if (!RHS_ONLY && active) {
         double *qw = s_q + el * 3 * NE + f * NE + l * NF;
 #pragma unroll
-        for (int j = 0; j < NF; ++j) {
-            d[j] = res[j];
-            if (!RHS_ONLY) qw[j] = qn[j];
-        }
+        for (int j = 0; j < NF; ++j) qw[j] = qn[j];
     }
     __syncthreads();
 
     // P5
     if (RHS_ONLY) {
         double *rg = rhs + e0 * 3 * NE;
-        for (int r = tid; r < nel * 3 * NE; r += T) rg[r] = s_d[r];
+#pragma unroll
+        for (int it = 0; it < S::QPT; ++it) {
+            const int r = tid + it * T;
+            if (r < nel * 3 * NE) rg[r] = s_d[r];
+        }
         return;
     }
     {
-        double *qg = q + e0 * 3 * NE;
-        double *gg = g + e0 * 3 * NE;
-        for (int r = tid; r < nel * 3 * NE; r += T) {
-            qg[r] = s_q[r];
-            gg[r] = s_d[r];
+        double *qw = q + e0 * 3 * NE;
+        double *gw = g + e0 * 3 * NE;
+#pragma unroll
+        for (int it = 0; it < S::QPT; ++it) {
+            const int r = tid + it * T;
+            if (r < nel * 3 * NE) {
+                qw[r] = s_q[r];
+                gw[r] = s_d[r];
+            }
         }
     }
     if (active) {
