@@ -15,6 +15,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -31,6 +33,13 @@ def algorithmic_bytes_per_element(n):
     neighbour traces read plus own traces written 192*(N+1), vertices and neighbour ids 80."""
     nf = n + 1
     return 88 * nf * nf + 192 * nf + 80
+
+
+def algorithmic_bytes_fp32c(n):
+    """The FP32 form per element and stage: state as two floats read and written 48*(N+1)^2, RK
+    register as floats 20*(N+1)^2 (2/3 read + written), float traces 96*(N+1), vertices and ids 80."""
+    nf = n + 1
+    return 68 * nf * nf + 96 * nf + 80
 
 
 def cpu_baseline(desc, steps):
@@ -55,6 +64,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true", help="skip the FP32-compensated leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -83,34 +93,39 @@ def main():
     mesh = nd.Mesh2D_t(desc, N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
     stream = torch.cuda.Stream()
     mesh.set_stream(stream.cuda_stream)
-    mesh.initial_conditions(PULSES)
     solver = nd.Solver2D_t(CFL, [0.0])
-    dt = solver.get_delta_t(mesh)
-    t = 0.0
-    for _ in range(args.warmup):
-        solver.step(mesh, t, dt)
-        t += dt
-    mesh.synchronize()
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    wall0 = time.perf_counter()
-    start.record(stream)
-    for _ in range(args.steps):
-        solver.step(mesh, t, dt)
-        t += dt
-    stop.record(stream)
-    stop.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    ms = start.elapsed_time(stop)
-    p, u, v = mesh.get_solution()
-    finite = bool(all(map(lambda a: a.size and float(abs(a).max()) < 1e3, (p, u, v))))
-    if not finite:
-        raise SystemExit("bench.py: solution blew up")
+
+    def timed_run(precision):
+        """IC, W warm-up steps, K timed steps; returns (elapsed ms, wall s, final state)."""
+        mesh.set_precision("fp64")
+        mesh.initial_conditions(PULSES)
+        mesh.set_precision(precision)
+        dt = solver.get_delta_t(mesh)
+        t = 0.0
+        for _ in range(args.warmup):
+            solver.step(mesh, t, dt)
+            t += dt
+        mesh.synchronize()
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        wall0 = time.perf_counter()
+        start.record(stream)
+        for _ in range(args.steps):
+            solver.step(mesh, t, dt)
+            t += dt
+        stop.record(stream)
+        stop.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - wall0
+        state = mesh.get_solution()
+        if not all(a.size and float(abs(a).max()) < 1e3 for a in state):
+            raise SystemExit(f"bench.py: {precision} solution blew up")
+        return start.elapsed_time(stop), wall, state
 
     dofs = mesh.n_dofs
     stages = 3 * args.steps
+    ms, wall, ref_state = timed_run("fp64")
     value = dofs * stages / (ms / 1e3)
     launch_s = ms / 1e3 / stages
     bytes_per_launch = algorithmic_bytes_per_element(N) * mesh.n_elements_
@@ -119,6 +134,22 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    forms = {}
+    if not args.fp64_only:
+        ms32, _, st32 = timed_run("fp32c")
+        diff = np.concatenate([a - b for a, b in zip(st32, ref_state)])
+        refv = np.concatenate(ref_state)
+        b32 = algorithmic_bytes_fp32c(N) * mesh.n_elements_
+        forms["fp32c"] = {
+            "value": dofs * stages / (ms32 / 1e3), "ms_per_step": ms32 / args.steps,
+            "l2_rel_vs_fp64": float(np.linalg.norm(diff) / np.linalg.norm(refv)),
+            "max_rel_vs_fp64": float(np.abs(diff).max() / np.abs(refv).max()),
+            "roofline": {"bound": "hbm", "achieved": b32 / (ms32 / 1e3 / stages) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": b32 / (ms32 / 1e3 / stages) / 1e9 / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_launch": b32, "kernel": "ndg::stage_c32_kernel<8>"},
+            "note": "FP32 right-hand side and RK register, state as a two-sum-compensated float pair; "
+                    "error is against the FP64 run of the same steps"}
     out = {
         "metric": "DOF*stage updates/s (FP64 plain DG-SEM wave RK3 stage)",
         "value": value,
@@ -140,6 +171,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "ndg::stage_kernel<8,false>", "avg_launch_ms": launch_s * 1e3,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
+        "forms": forms,
         "wall_s_timed_region": wall,
     }
     if not args.no_cpu and args.cpu_steps > 0:

@@ -57,6 +57,12 @@ typedef int ndg_status;
 #define NDG_TABLE_INTERPOLATION 10
 #define NDG_TABLE_POLYNOMIALS 11
 
+/* Arithmetic forms of the stage (ndg_mesh_set_precision). */
+#define NDG_PRECISION_FP64 0             /* the reference's FP64 arithmetic (default) */
+#define NDG_PRECISION_FP32_COMPENSATED 1 /* FP32 right-hand side and RK register; state kept as a
+                                            two-sum-compensated pair of floats (no reference
+                                            counterpart) */
+
 /* Initial-condition kinds. */
 #define NDG_IC_PLANE_WAVE 0 /* the reference g(x, 0): include/functions/analytical_solution.tpp:5-11 */
 #define NDG_IC_PULSES 1     /* sum of radial Gaussian pulses, u = v = 0 (no reference counterpart) */
@@ -131,6 +137,9 @@ ndg_status ndg_mesh_set_stream(ndg_mesh *mesh, void *hip_stream);
 /* n_dofs_per_field = sum over elements of (N+1)^2. */
 ndg_status ndg_mesh_info(const ndg_mesh *mesh, int64_t *n_elements, int64_t *n_faces,
                          int64_t *n_dofs_per_field, int32_t *N_min, int32_t *N_max);
+/* Selects the arithmetic form of later steps (NDG_PRECISION_*); the current state carries over.
+ * No reference counterpart: the reference is FP64 only (include/helpers/float_types.h:4-5). */
+ndg_status ndg_mesh_set_precision(ndg_mesh *mesh, int32_t precision);
 /* Mesh2D_t::initial_conditions (src/meshes/Mesh2D_t.cpp:1192-1216); zeroes G and the RK register. */
 ndg_status ndg_set_ic(ndg_mesh *mesh, const ndg_ic_desc *ic);
 /* Mesh2D_t::get_solution's data without interpolation: packed p, u, v (host arrays). */

@@ -166,6 +166,9 @@ class Mesh2D_t {
     Mesh2D_t &operator=(const Mesh2D_t &) = delete;
     ~Mesh2D_t() { ndg_mesh_destroy(handle_); }
 
+    // NDG_PRECISION_FP64 (the reference arithmetic) or NDG_PRECISION_FP32_COMPENSATED.
+    void set_precision(int32_t precision) { check(ndg_mesh_set_precision(handle_, precision)); }
+
     // The reference plane wave g(x, 0).
     void initial_conditions() {
         ndg_ic_desc ic{NDG_IC_PLANE_WAVE, 0, nullptr};

@@ -18,6 +18,7 @@ NDG_ERR_NO_DEVICE = 4
 
 BC_NONE, BC_WALL, BC_SYMMETRY, BC_INFLOW, BC_OUTFLOW, BC_PERIODIC = range(6)
 IC_PLANE_WAVE, IC_PULSES = 0, 1
+PRECISION_FP64, PRECISION_FP32_COMPENSATED = 0, 1
 
 TABLE_NAMES = ["nodes", "weights", "barycentric_weights", "lagrange_interpolant_left",
                "lagrange_interpolant_right", "lagrange_interpolant_derivative_left",
@@ -28,7 +29,7 @@ TABLE_NAMES = ["nodes", "weights", "barycentric_weights", "lagrange_interpolant_
 # Every symbol include/ndg_amd.h declares (checked by the CPU tests).
 EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tables_destroy",
             "ndg_tables_get", "ndg_square_mesh_sizes", "ndg_square_mesh_fill", "ndg_mesh_create",
-            "ndg_mesh_destroy", "ndg_mesh_set_stream", "ndg_mesh_info", "ndg_set_ic",
+            "ndg_mesh_destroy", "ndg_mesh_set_stream", "ndg_mesh_info", "ndg_mesh_set_precision", "ndg_set_ic",
             "ndg_set_state", "ndg_get_state", "ndg_get_traces", "ndg_compute_rhs",
             "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_run_steps", "ndg_synchronize",
             "ndg_estimate_error"]
@@ -84,6 +85,7 @@ def lib():
             "ndg_mesh_destroy": [vp],
             "ndg_mesh_set_stream": [vp, vp],
             "ndg_mesh_info": [vp, vp, vp, vp, vp, vp],
+            "ndg_mesh_set_precision": [vp, i32],
             "ndg_set_ic": [vp, vp],
             "ndg_set_state": [vp, vp, vp, vp],
             "ndg_get_state": [vp, vp, vp, vp],

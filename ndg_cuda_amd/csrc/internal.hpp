@@ -65,6 +65,12 @@ struct DeviceMesh {
     double *scratch = nullptr;   // reductions
     int64_t scratch_len = 0;
     int cur = 0;                 // trace buffer holding the traces of the current state
+    // FP32 form with compensated state (allocated by ndg_mesh_set_precision): the state is the
+    // unevaluated sum qh + ql of two floats per value, the RK register and traces are floats.
+    int precision = NDG_PRECISION_FP64;
+    float *qh = nullptr, *ql = nullptr, *g32 = nullptr;  // [K][3][Ne]
+    float *trace32[2] = {nullptr, nullptr};              // [K][4][3][Nf]
+    float *tab32 = nullptr;                              // packed tables for N, as floats
 };
 
 // Kernel launchers (kernels.hip). All return hipSuccess or the first launch error.
@@ -72,6 +78,13 @@ hipError_t launch_traces(const DeviceMesh &m, hipStream_t s);
 hipError_t launch_stage(DeviceMesh &m, int stage, double t_stage, double a, double gdt,
                         hipStream_t s);
 hipError_t launch_rhs(DeviceMesh &m, double t, hipStream_t s);
+// FP32 compensated form: conversions between the FP64 state (q, g) and (qh, ql, g32), traces of
+// qh, and the stage.
+hipError_t launch_to_c32(DeviceMesh &m, hipStream_t s);
+hipError_t launch_from_c32(DeviceMesh &m, hipStream_t s);
+hipError_t launch_traces_c32(const DeviceMesh &m, hipStream_t s);
+hipError_t launch_stage_c32(DeviceMesh &m, int stage, double t_stage, double a, double gdt,
+                            hipStream_t s);
 hipError_t launch_ic(const DeviceMesh &m, int kind, int n_pulses, const double *pulses_dev,
                      hipStream_t s);
 hipError_t launch_delta_t(const DeviceMesh &m, double cfl, double *dt_dev, hipStream_t s);

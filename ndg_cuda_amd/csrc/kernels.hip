@@ -208,11 +208,13 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
         const int r = tid + it * T;
         vreg[it] = r < nel * 8 ? verts[e0 * 8 + r] : 0.0;
     }
+    // The RK register is stored transposed within an element ([e][f][j][i]), so that the row
+    // owner's line of it is read and written with consecutive lanes on consecutive addresses.
     double g_old[NF];
     if (!RHS_ONLY && !first && active) {
-        const double *gg = g + (e0 + el) * 3 * NE + f * NE + l * NF;
+        const double *gg = g + (e0 + el) * 3 * NE + f * NE + l;
 #pragma unroll
-        for (int j = 0; j < NF; ++j) g_old[j] = gg[j];
+        for (int j = 0; j < NF; ++j) g_old[j] = gg[j * NF];
     } else {
 #pragma unroll
         for (int j = 0; j < NF; ++j) g_old[j] = 0.0;
@@ -363,8 +365,8 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
     }
     __syncthreads();
 
-    // P4: thread owns eta-line i = l of field f. The new RK register goes straight back into its
-    // own row of s_d (read by no other thread); the new state waits for the barrier.
+    // P4: thread owns eta-line i = l of field f. The new RK register goes straight to HBM
+    // (transposed layout, coalesced); the new state waits for the barrier.
     double qn[NF];
     if (active) {
         const int i = l;
@@ -393,7 +395,7 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
                 d[j] = Gv;
             } else {
                 const double gn = first ? Gv : a * g_old[j] + Gv;
-                d[j] = gn;
+                g[(e0 + el) * 3 * NE + f * NE + j * NF + i] = gn;
                 qn[j] = qrow[j] + gdt * gn;
             }
         }
@@ -418,14 +420,10 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
     }
     {
         double *qw = q + e0 * 3 * NE;
-        double *gw = g + e0 * 3 * NE;
 #pragma unroll
         for (int it = 0; it < S::QPT; ++it) {
             const int r = tid + it * T;
-            if (r < nel * 3 * NE) {
-                qw[r] = s_q[r];
-                gw[r] = s_d[r];
-            }
+            if (r < nel * 3 * NE) qw[r] = s_q[r];
         }
     }
     if (active) {
@@ -440,6 +438,366 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
             t3 += lm[j] * col;
         }
         double *to = tr_out + (e0 + el) * 12 * NF + f * NF;
+        to[l] = t0;
+        to[6 * NF + (N - l)] = t2;
+        to[3 * NF + l] = t1;
+        to[9 * NF + (N - l)] = t3;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP32 form with a compensated state. The right-hand side (traces, flux, volume, lift) and the RK
+// register are FP32; the state is the unevaluated sum qh + ql of two floats, updated with an
+// error-free two-sum so that the accumulation over many steps keeps ~48 bits. Geometry (metric
+// lines, normals, edge Jacobians) is evaluated in FP64 from the FP64 vertices and rounded once.
+// Same thread layout and phases as stage_kernel.
+
+// qh + ql += d exactly up to the final rounding of ql (Knuth two-sum, then fast two-sum
+// renormalisation); contraction is off so that the error terms are exact.
+__device__ __forceinline__ void two_sum_add(float &hi, float &lo, float d) {
+#pragma clang fp contract(off)
+    const float s = hi + d;
+    const float bb = s - hi;
+    const float err = (hi - (s - bb)) + (d - bb);
+    const float l = lo + err;
+    const float h = s + l;
+    lo = l - (h - s);
+    hi = h;
+}
+
+// q and qh are in reference node order, ql, g and g32 transposed within each (element, field)
+// block; nf = N + 1.
+__global__ void to_c32_kernel(const double *__restrict__ q, const double *__restrict__ g,
+                              float *__restrict__ qh, float *__restrict__ ql, float *__restrict__ g32,
+                              int64_t n, int nf) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int ne = nf * nf, node = (int)(r % ne), i = node / nf, j = node % nf;
+    const int64_t t = r - node + j * nf + i;
+    const double v = q[r];
+    const float h = (float)v;
+    qh[r] = h;
+    ql[t] = (float)(v - (double)h);
+    g32[r] = (float)g[r];
+}
+
+__global__ void from_c32_kernel(const float *__restrict__ qh, const float *__restrict__ ql,
+                                const float *__restrict__ g32, double *__restrict__ q,
+                                double *__restrict__ g, int64_t n, int nf) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int ne = nf * nf, node = (int)(r % ne), i = node / nf, j = node % nf;
+    const int64_t t = r - node + j * nf + i;
+    q[r] = (double)qh[r] + (double)ql[t];
+    g[r] = (double)g32[r];
+}
+
+template <int N>
+__global__ void __launch_bounds__(192)
+traces_c32_kernel(const float *__restrict__ tab, const float *__restrict__ qh, float *__restrict__ tr,
+                  int64_t K) {
+    using S = LineShape<N>;
+    constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::T;
+    __shared__ float s_q[EPB * 3 * NE];
+    const float *lm = tab + NF * NF;
+    const float *lp = lm + NF;
+    const int tid = threadIdx.x;
+    const int f = tid >> 6, lane = tid & 63, el = lane / NF, l = lane % NF;
+    const int64_t e0 = (int64_t)blockIdx.x * EPB;
+    const int nel = (int)min<int64_t>(EPB, K - e0);
+    for (int r = tid; r < nel * 3 * NE; r += T) s_q[r] = qh[e0 * 3 * NE + r];
+    __syncthreads();
+    if (el < nel) {
+        const float *tq = s_q + el * 3 * NE + f * NE;
+        float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
+        for (int j = 0; j < NF; ++j) {
+            const float row = tq[l * NF + j], col = tq[j * NF + l];
+            t0 += lm[j] * row;
+            t2 += lp[j] * row;
+            t1 += lp[j] * col;
+            t3 += lm[j] * col;
+        }
+        float *to = tr + (e0 + el) * 12 * NF + f * NF;
+        to[l] = t0;
+        to[6 * NF + (N - l)] = t2;
+        to[3 * NF + l] = t1;
+        to[9 * NF + (N - l)] = t3;
+    }
+}
+
+template <int N>
+__global__ void __launch_bounds__(192)
+stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_nb,
+                 const BcSide *__restrict__ bc, const double *__restrict__ tab64,
+                 const float *__restrict__ tab, float *__restrict__ qh, float *__restrict__ ql,
+                 float *__restrict__ g, const float *__restrict__ tr_in, float *__restrict__ tr_out,
+                 int64_t K, int first, float a, float gdt, double t_stage) {
+    using S = LineShape<N>;
+    constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::T, FPT = S::FPT;
+    __shared__ float s_q[EPB * 3 * NE];
+    __shared__ float s_d[EPB * 3 * NE];
+    __shared__ float s_tr[EPB * 12 * NF];
+    __shared__ float s_met[EPB * 4 * NF];
+    __shared__ float s_side[EPB * 4 * 3];
+    __shared__ double s_v[EPB * 8];
+    const float *Dh = tab;
+    const float *lm = tab + NF * NF;
+    const float *lp = lm + NF;
+    const float *winv = lp + 3 * NF;
+    const double *x64 = tab64 + NF * NF + 3 * NF;
+
+    const int tid = threadIdx.x;
+    const int f = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int el = lane / NF, l = lane % NF;
+    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t e0 = blk * EPB;
+    const int nel = (int)min<int64_t>(EPB, K - e0);
+    const bool active = el < nel;
+
+    // P0
+    int32_t code[FPT];
+#pragma unroll
+    for (int it = 0; it < FPT; ++it) {
+        const int item = tid + it * T;
+        code[it] = item < nel * 4 * NF ? side_nb[(e0 + item / NF / 4) * 4 + (item / NF) % 4] : INT32_MIN;
+    }
+    float qreg[S::QPT];
+    const float *qg = qh + e0 * 3 * NE;
+#pragma unroll
+    for (int it = 0; it < S::QPT; ++it) {
+        const int r = tid + it * T;
+        qreg[it] = r < nel * 3 * NE ? qg[r] : 0.f;
+    }
+    double vreg[S::VPT];
+#pragma unroll
+    for (int it = 0; it < S::VPT; ++it) {
+        const int r = tid + it * T;
+        vreg[it] = r < nel * 8 ? verts[e0 * 8 + r] : 0.0;
+    }
+    // RK register and low words stored transposed within an element ([e][f][j][i]): the row owner's
+    // lines are read and written with consecutive lanes on consecutive addresses.
+    float g_old[NF], lo_row[NF];
+    if (active) {
+        const int64_t off = (e0 + el) * 3 * NE + f * NE + l;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            g_old[j] = first ? 0.f : g[off + j * NF];
+            lo_row[j] = ql[off + j * NF];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < NF; ++j) g_old[j] = lo_row[j] = 0.f;
+    }
+    float nbv[FPT][3];
+#pragma unroll
+    for (int it = 0; it < FPT; ++it) {
+        nbv[it][0] = nbv[it][1] = nbv[it][2] = 0.f;
+        const int32_t cd = code[it];
+        if (cd >= 0) {
+            const int k = (tid + it * T) % NF;
+            const float *nb = tr_in + (int64_t)(cd >> 2) * 12 * NF + (cd & 3) * 3 * NF + (N - k);
+            nbv[it][0] = nb[0];
+            nbv[it][1] = nb[NF];
+            nbv[it][2] = nb[2 * NF];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < S::VPT; ++it) {
+        const int r = tid + it * T;
+        if (r < nel * 8) s_v[r] = vreg[it];
+    }
+#pragma unroll
+    for (int it = 0; it < S::QPT; ++it) {
+        const int r = tid + it * T;
+        if (r < nel * 3 * NE) s_q[r] = qreg[it];
+    }
+    __syncthreads();
+
+    // P1 (geometry in FP64, rounded once)
+    if (active) {
+        const double *P = s_v + el * 8;
+        float *met = s_met + el * 4 * NF;
+        if (f == 0) {
+            const double xi = x64[l];
+            met[l] = (float)(((1 - xi) * (P[6] - P[0]) + (1 + xi) * (P[4] - P[2])) / 4);
+            met[NF + l] = (float)(((1 - xi) * (P[7] - P[1]) + (1 + xi) * (P[5] - P[3])) / 4);
+        } else if (f == 1) {
+            const double eta = x64[l];
+            met[2 * NF + l] = (float)(((1 - eta) * (P[2] - P[0]) + (1 + eta) * (P[4] - P[6])) / 4);
+            met[3 * NF + l] = (float)(((1 - eta) * (P[3] - P[1]) + (1 + eta) * (P[5] - P[7])) / 4);
+        } else if (l < 4) {
+            const int s = l, s1 = (s + 1) & 3;
+            const double ex = P[2 * s1] - P[2 * s], ey = P[2 * s1 + 1] - P[2 * s + 1];
+            const double len = sqrt(ex * ex + ey * ey);
+            float *sd = s_side + (el * 4 + s) * 3;
+            sd[0] = (float)(ey / len);
+            sd[1] = (float)(-(ex / len));
+            sd[2] = (float)side_scale(P, s, x64[0]);
+        }
+        const float *tq = s_q + el * 3 * NE + f * NE;
+        float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            const float row = tq[l * NF + j], col = tq[j * NF + l];
+            t0 += lm[j] * row;
+            t2 += lp[j] * row;
+            t1 += lp[j] * col;
+            t3 += lm[j] * col;
+        }
+        float *tr = s_tr + el * 12 * NF + f * NF;
+        tr[l] = t0;
+        tr[6 * NF + (N - l)] = t2;
+        tr[3 * NF + l] = t1;
+        tr[9 * NF + (N - l)] = t3;
+    }
+    __syncthreads();
+
+    // P2
+#pragma unroll
+    for (int it = 0; it < FPT; ++it) {
+        const int item = tid + it * T;
+        if (item < nel * 4 * NF) {
+            const int fe = item / (4 * NF), s = (item / NF) % 4, k = item % NF;
+            float *tr = s_tr + fe * 12 * NF + s * 3 * NF + k;
+            const float pL = tr[0], uL = tr[NF], vL = tr[2 * NF];
+            const float *sd = s_side + (fe * 4 + s) * 3;
+            float nx = sd[0], ny = sd[1];
+            const float sf = sd[2];
+            float pR, uR, vR;
+            const int32_t cd = code[it];
+            if (cd >= 0) {
+                pR = nbv[it][0];
+                uR = nbv[it][1];
+                vR = nbv[it][2];
+            } else {
+                const BcSide rec = bc[-1 - cd];
+                nx = (float)rec.nx;
+                ny = (float)rec.ny;
+                const float tx = (float)rec.tx, ty = (float)rec.ty, gs = (float)rec.sf;
+                if (rec.kind == NDG_BC_WALL || rec.kind == NDG_BC_SYMMETRY) {
+                    pR = pL * gs;
+                    const float ux = uL * gs, uy = vL * gs;
+                    const float l0 = -(ux * nx + uy * ny);
+                    const float l1 = ux * tx + uy * ty;
+                    uR = nx * l0 + tx * l1;
+                    vR = ny * l0 + ty * l1;
+                } else if (rec.kind == NDG_BC_OUTFLOW) {
+                    pR = pL * gs;
+                    uR = uL * gs;
+                    vR = vL * gs;
+                } else if (rec.kind == NDG_BC_INFLOW) {
+                    const double interp = (x64[N - k] + 1) / 2;
+                    const double X = rec.bx * interp + rec.ax * (1 - interp);
+                    const double Y = rec.by * interp + rec.ay * (1 - interp);
+                    double p64, u64, v64;
+                    plane_wave(X, Y, t_stage, p64, u64, v64);
+                    pR = (float)p64;
+                    uR = (float)u64;
+                    vR = (float)v64;
+                } else {
+                    pR = uR = vR = 0.f;
+                }
+            }
+            const float wL = pL + (uL * nx + vL * ny);
+            const float wR = pR - (uR * nx + vR * ny);
+            tr[0] = ((wL - wR) * 0.5f) * sf;
+            tr[NF] = (nx * (wL + wR) * 0.5f) * sf;
+            tr[2 * NF] = (ny * (wL + wR) * 0.5f) * sf;
+        }
+    }
+    __syncthreads();
+
+    const float *tq = s_q + el * 3 * NE;
+    const float *met = s_met + el * 4 * NF;
+    const float *fs = s_tr + el * 12 * NF + f * NF;
+    // P3
+    if (active) {
+        const int j = l;
+        float F[NF];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            const int node = k * NF + j;
+            const float xeta = met[k], yeta = met[NF + k];
+            if (f == 0) F[k] = yeta * tq[NE + node] - xeta * tq[2 * NE + node];
+            else if (f == 1) F[k] = yeta * tq[node];
+            else F[k] = -(xeta * tq[node]);
+        }
+        const float Fr = fs[3 * NF + j], Fl = fs[9 * NF + (N - j)];
+        float *d = s_d + el * 3 * NE + f * NE + j;
+#pragma unroll
+        for (int i = 0; i < NF; ++i) {
+            float acc = 0.f;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) acc += Dh[i * NF + k] * F[k];
+            acc += (Fr * lp[i] + Fl * lm[i]) * winv[i];
+            d[i * NF] = acc;
+        }
+    }
+    __syncthreads();
+
+    // P4
+    float hn[NF];
+    if (active) {
+        const int i = l;
+        float Gt[NF];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            const int node = i * NF + k;
+            const float xxi = met[2 * NF + k], yxi = met[3 * NF + k];
+            if (f == 0) Gt[k] = -yxi * tq[NE + node] + xxi * tq[2 * NE + node];
+            else if (f == 1) Gt[k] = -yxi * tq[node];
+            else Gt[k] = xxi * tq[node];
+        }
+        const float Ft = fs[6 * NF + (N - i)], Fb = fs[i];
+        const float xeta_i = met[i], yeta_i = met[NF + i];
+        float *d = s_d + el * 3 * NE + f * NE + i * NF;
+        const float *qrow = tq + f * NE + i * NF;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            float acc = 0.f;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) acc += Dh[j * NF + k] * Gt[k];
+            acc += (Ft * lp[j] + Fb * lm[j]) * winv[j];
+            const float jac = met[2 * NF + j] * yeta_i - xeta_i * met[3 * NF + j];
+            const float Gv = (-d[j] - acc) / jac;
+            const float gn = first ? Gv : a * g_old[j] + Gv;
+            g[(e0 + el) * 3 * NE + f * NE + j * NF + i] = gn;
+            float h = qrow[j], lo = lo_row[j];
+            two_sum_add(h, lo, gdt * gn);
+            hn[j] = h;
+            ql[(e0 + el) * 3 * NE + f * NE + j * NF + i] = lo;
+        }
+    }
+    __syncthreads();
+    if (active) {
+        float *qw = s_q + el * 3 * NE + f * NE + l * NF;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) qw[j] = hn[j];
+    }
+    __syncthreads();
+
+    // P5
+    {
+        float *qw = qh + e0 * 3 * NE;
+#pragma unroll
+        for (int it = 0; it < S::QPT; ++it) {
+            const int r = tid + it * T;
+            if (r < nel * 3 * NE) qw[r] = s_q[r];
+        }
+    }
+    if (active) {
+        const float *tqf = s_q + el * 3 * NE + f * NE;
+        float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            const float row = tqf[l * NF + j], col = tqf[j * NF + l];
+            t0 += lm[j] * row;
+            t2 += lp[j] * row;
+            t1 += lp[j] * col;
+            t3 += lm[j] * col;
+        }
+        float *to = tr_out + (e0 + el) * 12 * NF + f * NF;
         to[l] = t0;
         to[6 * NF + (N - l)] = t2;
         to[3 * NF + l] = t1;
@@ -672,6 +1030,58 @@ hipError_t launch_estimate(const DeviceMesh &m, double tol_min, double tol_max, 
 #undef X
     }
     return hipErrorInvalidValue;
+}
+
+template <int N>
+static hipError_t traces_c32_n(const DeviceMesh &m, hipStream_t s) {
+    using S = LineShape<N>;
+    const int64_t blocks = (m.K + S::EPB - 1) / S::EPB;
+    hipLaunchKernelGGL(traces_c32_kernel<N>, dim3((unsigned)blocks), dim3(S::T), 0, s, m.tab32, m.qh,
+                       m.trace32[m.cur], m.K);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t stage_c32_n(DeviceMesh &m, int stage, double t_stage, double a, double gdt, hipStream_t s) {
+    using S = LineShape<N>;
+    const int64_t blocks = (m.K + S::EPB - 1) / S::EPB;
+    hipLaunchKernelGGL(stage_c32_kernel<N>, dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts, m.side_nb, m.bc,
+                       m.tab, m.tab32, m.qh, m.ql, m.g32, m.trace32[m.cur], m.trace32[1 - m.cur], m.K,
+                       stage == 0 ? 1 : 0, (float)a, (float)gdt, t_stage);
+    m.cur = 1 - m.cur;
+    return hipGetLastError();
+}
+
+hipError_t launch_traces_c32(const DeviceMesh &m, hipStream_t s) {
+    switch (m.N) {
+#define X(n) case n: return traces_c32_n<n>(m, s);
+        NDG_FOR_ORDERS(X)
+#undef X
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_stage_c32(DeviceMesh &m, int stage, double t_stage, double a, double gdt, hipStream_t s) {
+    switch (m.N) {
+#define X(n) case n: return stage_c32_n<n>(m, stage, t_stage, a, gdt, s);
+        NDG_FOR_ORDERS(X)
+#undef X
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_to_c32(DeviceMesh &m, hipStream_t s) {
+    const int64_t n = m.K * 3 * (int64_t)(m.N + 1) * (m.N + 1);
+    hipLaunchKernelGGL(to_c32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m.q, m.g, m.qh, m.ql,
+                       m.g32, n, m.N + 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_from_c32(DeviceMesh &m, hipStream_t s) {
+    const int64_t n = m.K * 3 * (int64_t)(m.N + 1) * (m.N + 1);
+    hipLaunchKernelGGL(from_c32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m.qh, m.ql, m.g32,
+                       m.q, m.g, n, m.N + 1);
+    return hipGetLastError();
 }
 
 hipError_t launch_delta_t(const DeviceMesh &m, double cfl, double *dt_dev, hipStream_t s) {

@@ -159,7 +159,8 @@ void free_mesh(ndg_mesh *m) {
     DeviceMesh &d = m->d;
     for (void *p : {(void *)d.verts, (void *)d.side_nb, (void *)d.bc, (void *)d.tab, (void *)d.q, (void *)d.g,
                     (void *)d.trace[0], (void *)d.trace[1], (void *)d.rhs, (void *)d.scratch, (void *)m->dt_dev,
-                    (void *)m->est_dev, (void *)m->pulses_dev})
+                    (void *)m->est_dev, (void *)m->pulses_dev, (void *)d.qh, (void *)d.ql, (void *)d.g32,
+                    (void *)d.trace32[0], (void *)d.trace32[1], (void *)d.tab32})
         if (p) (void)hipFree(p);
     if (m->own_stream) (void)hipStreamDestroy(m->own_stream);
     delete m;
@@ -371,7 +372,54 @@ ndg_status copy_out3(ndg_mesh *m, const double *dev, double *a, double *b, doubl
     return NDG_OK;
 }
 
+// FP32 compensated form: refresh the FP64 state, RK register and traces from (qh, ql, g32).
+ndg_status refresh_fp64(ndg_mesh *m) {
+    if (m->d.precision != NDG_PRECISION_FP32_COMPENSATED) return NDG_OK;
+    HIP_TRY(launch_from_c32(m->d, m->stream), "from_c32 kernel");
+    HIP_TRY(launch_traces(m->d, m->stream), "traces kernel");
+    return NDG_OK;
+}
+
+// After the FP64 state changed: rebuild (qh, ql, g32) and their traces.
+ndg_status refresh_c32(ndg_mesh *m) {
+    if (m->d.precision != NDG_PRECISION_FP32_COMPENSATED) return NDG_OK;
+    HIP_TRY(launch_to_c32(m->d, m->stream), "to_c32 kernel");
+    HIP_TRY(launch_traces_c32(m->d, m->stream), "traces_c32 kernel");
+    return NDG_OK;
+}
+
 }  // namespace
+
+extern "C" ndg_status ndg_mesh_set_precision(ndg_mesh *mesh, int32_t precision) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_mesh_set_precision: null mesh");
+    if (precision != NDG_PRECISION_FP64 && precision != NDG_PRECISION_FP32_COMPENSATED)
+        return fail(NDG_ERR_INVALID, "ndg_mesh_set_precision: unknown precision");
+    DeviceMesh &d = mesh->d;
+    if (precision == d.precision) return NDG_OK;
+    if (precision == NDG_PRECISION_FP32_COMPENSATED) {
+        const int64_t nf = d.N + 1, ne = nf * nf;
+        if (!d.qh) {
+            HIP_TRY(hipMalloc((void **)&d.qh, sizeof(float) * 3 * ne * d.K), "hipMalloc qh");
+            HIP_TRY(hipMalloc((void **)&d.ql, sizeof(float) * 3 * ne * d.K), "hipMalloc ql");
+            HIP_TRY(hipMalloc((void **)&d.g32, sizeof(float) * 3 * ne * d.K), "hipMalloc g32");
+            HIP_TRY(hipMalloc((void **)&d.trace32[0], sizeof(float) * 12 * nf * d.K), "hipMalloc traces32");
+            HIP_TRY(hipMalloc((void **)&d.trace32[1], sizeof(float) * 12 * nf * d.K), "hipMalloc traces32");
+            const std::vector<double> t64 = pack_tables(mesh->tables->t, d.N);
+            std::vector<float> t32(t64.begin(), t64.end());
+            HIP_TRY(hipMalloc((void **)&d.tab32, sizeof(float) * t32.size()), "hipMalloc tab32");
+            HIP_TRY(hipMemcpy(d.tab32, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice), "upload tab32");
+        }
+        d.precision = NDG_PRECISION_FP32_COMPENSATED;
+        ndg_status st = refresh_c32(mesh);
+        if (st != NDG_OK) return st;
+    } else {
+        ndg_status st = refresh_fp64(mesh);
+        if (st != NDG_OK) return st;
+        d.precision = NDG_PRECISION_FP64;
+    }
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_mesh_set_precision sync");
+    return NDG_OK;
+}
 
 extern "C" ndg_status ndg_set_ic(ndg_mesh *mesh, const ndg_ic_desc *ic) {
     if (!mesh || !ic) return fail(NDG_ERR_INVALID, "ndg_set_ic: null argument");
@@ -393,6 +441,8 @@ extern "C" ndg_status ndg_set_ic(ndg_mesh *mesh, const ndg_ic_desc *ic) {
     HIP_TRY(launch_ic(d, ic->kind, np, mesh->pulses_dev, mesh->stream), "ic kernel");
     HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * 3 * ne * d.K, mesh->stream), "hipMemset g");
     HIP_TRY(launch_traces(d, mesh->stream), "traces kernel");
+    ndg_status st = refresh_c32(mesh);
+    if (st != NDG_OK) return st;
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_set_ic sync");
     return NDG_OK;
 }
@@ -409,17 +459,23 @@ extern "C" ndg_status ndg_set_state(ndg_mesh *mesh, const double *p, const doubl
     HIP_TRY(hipMemcpyAsync(d.q, mesh->staging.data(), sizeof(double) * 3 * ne * K, hipMemcpyHostToDevice, mesh->stream), "upload state");
     HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * 3 * ne * K, mesh->stream), "hipMemset g");
     HIP_TRY(launch_traces(d, mesh->stream), "traces kernel");
+    ndg_status st = refresh_c32(mesh);
+    if (st != NDG_OK) return st;
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_set_state sync");
     return NDG_OK;
 }
 
 extern "C" ndg_status ndg_get_state(ndg_mesh *mesh, double *p, double *u, double *v) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_get_state: null mesh");
+    ndg_status st = refresh_fp64(mesh);
+    if (st != NDG_OK) return st;
     return copy_out3(mesh, mesh->d.q, p, u, v, "download state");
 }
 
 extern "C" ndg_status ndg_get_traces(ndg_mesh *mesh, double *out) {
     if (!mesh || !out) return fail(NDG_ERR_INVALID, "ndg_get_traces: null argument");
+    ndg_status st = refresh_fp64(mesh);
+    if (st != NDG_OK) return st;
     const DeviceMesh &d = mesh->d;
     const int64_t n = 12 * (int64_t)(d.N + 1) * d.K;
     HIP_TRY(hipMemcpyAsync(out, d.trace[d.cur], sizeof(double) * n, hipMemcpyDeviceToHost, mesh->stream), "download traces");
@@ -429,6 +485,8 @@ extern "C" ndg_status ndg_get_traces(ndg_mesh *mesh, double *out) {
 
 extern "C" ndg_status ndg_compute_rhs(ndg_mesh *mesh, double t, double *Gp, double *Gu, double *Gv) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_compute_rhs: null mesh");
+    ndg_status st = refresh_fp64(mesh);
+    if (st != NDG_OK) return st;
     DeviceMesh &d = mesh->d;
     const int64_t ne = (int64_t)(d.N + 1) * (d.N + 1);
     if (!d.rhs) HIP_TRY(hipMalloc((void **)&d.rhs, sizeof(double) * 3 * ne * d.K), "hipMalloc rhs");
@@ -447,9 +505,11 @@ extern "C" ndg_status ndg_get_delta_t(ndg_mesh *mesh, double cfl, double *dt) {
 
 extern "C" ndg_status ndg_step(ndg_mesh *mesh, double t, double dt) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_step: null mesh");
+    const bool c32 = mesh->d.precision == NDG_PRECISION_FP32_COMPENSATED;
     for (int s = 0; s < 3; ++s) {
         const double ts = t + kBm[s] * dt;
-        HIP_TRY(launch_stage(mesh->d, s, ts, kAm[s], kGm[s] * dt, mesh->stream), "stage kernel");
+        if (c32) HIP_TRY(launch_stage_c32(mesh->d, s, ts, kAm[s], kGm[s] * dt, mesh->stream), "stage_c32 kernel");
+        else HIP_TRY(launch_stage(mesh->d, s, ts, kAm[s], kGm[s] * dt, mesh->stream), "stage kernel");
     }
     return NDG_OK;
 }
@@ -498,6 +558,8 @@ extern "C" ndg_status ndg_synchronize(ndg_mesh *mesh) {
 extern "C" ndg_status ndg_estimate_error(ndg_mesh *mesh, double *error, double *sigma, int32_t *refine,
                                          int32_t *coarsen) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_estimate_error: null mesh");
+    ndg_status st = refresh_fp64(mesh);
+    if (st != NDG_OK) return st;
     DeviceMesh &d = mesh->d;
     if (!mesh->est_dev) HIP_TRY(hipMalloc((void **)&mesh->est_dev, sizeof(double) * 4 * d.K), "hipMalloc estimator");
     HIP_TRY(launch_estimate(d, mesh->tol_min, mesh->tol_max, mesh->est_dev, mesh->stream), "estimator kernel");

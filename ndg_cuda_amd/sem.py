@@ -142,6 +142,13 @@ class Mesh2D_t:
         """DOF = 3 * sum over elements of (N+1)^2."""
         return 3 * self.n_dofs_per_field_
 
+    def set_precision(self, precision: str):
+        """'fp64' (the reference arithmetic) or 'fp32c' (FP32 right-hand side and RK register, state
+        kept as a compensated pair of floats)."""
+        code = {"fp64": _abi.PRECISION_FP64, "fp32c": _abi.PRECISION_FP32_COMPENSATED}[precision]
+        check(lib().ndg_mesh_set_precision(self._h, code))
+        self.precision_ = precision
+
     def set_stream(self, hip_stream_ptr: int):
         check(lib().ndg_mesh_set_stream(self._h, ctypes.c_void_p(hip_stream_ptr)))
 

@@ -214,3 +214,43 @@ def test_full_size_properties(tables):
     yeta = ((1 - XI) * d(3, 0, 1) + (1 + XI) * d(2, 1, 1)) / 4
     W = (np.outer(w, w).ravel()[None, :] * (xxi * yeta - xeta * yxi)).ravel()
     assert abs(np.sum(W * Gp)) < 1e-11 * np.sum(W * np.abs(Gp))
+
+
+# ---------------------------------------------------------------------------------------------
+# FP32 form with compensated state (no reference counterpart): checked against the FP64 oracle
+# with a tolerance set by FP32 rounding of the right-hand side, and against the exact solution.
+
+def test_fp32c_close_to_fp64_oracle(tables):
+    say("fp32c vs oracle, jittered walls N=7")
+    desc, gm, om = build(tables, 16, 7, jitter=0.15, seed=9)
+    pulses = [(0.1, -0.2, 0.2, 1.0), (-0.4, 0.3, 0.2, 0.5)]
+    gm.initial_conditions(pulses)
+    om.initial_conditions(pulses)
+    gm.set_precision("fp32c")
+    solver = nd.Solver2D_t(0.5, [0.0])
+    solver.run_steps(gm, 40)
+    om.run_steps(0.5, 40)
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < 2e-5
+    # switching back continues in FP64 from the compensated state
+    gm.set_precision("fp64")
+    solver.run_steps(gm, 2)
+    om.run_steps(0.5, 2)
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < 2e-5
+
+
+def test_fp32c_exact_solution_config1(tables):
+    """Config 1 (16x16 periodic, N = 6, t = 0.5) in the FP32 form: its distance to the Fourier exact
+    solution stays at the FP64 discretisation error (5e-7 bound used for FP64 above)."""
+    ndg, ot = tables
+    desc, gm, _ = build(tables, 16, 6, periodic=True)
+    base = (0.1, -0.2, 0.2, 1.0)
+    gm.initial_conditions(exact.periodic_images(*base))
+    gm.set_precision("fp32c")
+    solver = nd.Solver2D_t(0.5, [0.5])
+    solver.solve(None, gm)
+    X, Y = exact.node_coordinates(desc, ot.get(O.O_NODES, 6))
+    pe, ue, ve = exact.periodic_pulse(X, Y, solver.time_, [base], n_modes=24)
+    p, u, v = gm.get_solution()
+    assert max(np.abs(p - pe).max(), np.abs(u - ue).max(), np.abs(v - ve).max()) < 1e-6
