@@ -63,6 +63,12 @@ typedef int ndg_status;
                                             two-sum-compensated pair of floats (no reference
                                             counterpart) */
 
+/* Launch modes of the RK step (ndg_mesh_set_graph). */
+#define NDG_GRAPH_OFF 0        /* direct launches on the mesh stream (default) */
+#define NDG_GRAPH_SEQUENTIAL 1 /* one captured hipGraph per step; order groups one after another */
+#define NDG_GRAPH_PARALLEL 2   /* one captured hipGraph per step; the order groups of a stage are
+                                  independent graph branches that may run concurrently */
+
 /* Initial-condition kinds. */
 #define NDG_IC_PLANE_WAVE 0 /* the reference g(x, 0): include/functions/analytical_solution.tpp:5-11 */
 #define NDG_IC_PULSES 1     /* sum of radial Gaussian pulses, u = v = 0 (no reference counterpart) */
@@ -164,6 +170,22 @@ ndg_status ndg_solve(ndg_mesh *mesh, double cfl, double t_end, double *t_final, 
 ndg_status ndg_run_steps(ndg_mesh *mesh, double cfl, int64_t n_steps, double *time_io);
 /* Waits for all work queued on the mesh's stream. */
 ndg_status ndg_synchronize(ndg_mesh *mesh);
+/* Replaces Solver2D_t::solve's per-phase launches (include/solvers/Solver2D_t.tpp:44-70, one
+ * launch per phase and per element kernel) by a captured hipGraph per RK step (mortar faces, then
+ * each order group's fused stage kernel, three times). The stage times are read from a device
+ * clock that the graph advances, so one graph serves every step of size dt; it is recaptured when
+ * dt or the precision changes. No reference counterpart for the mode switch itself. */
+ndg_status ndg_mesh_set_graph(ndg_mesh *mesh, int32_t mode);
+/* Times every candidate launch shape of each order group's stage kernel (elements per workgroup
+ * from the shape's maximum down to 1, XCD-contiguous block order on or off) on this mesh, `repeats`
+ * launches each, and keeps the fastest for the current precision. The state is left unchanged.
+ * No reference counterpart (the reference launches every element kernel with 32 threads per
+ * block, include/solvers/Solver2D_t.tcu:45-52). */
+ndg_status ndg_mesh_autotune(ndg_mesh *mesh, int32_t repeats);
+/* The launch shape in use per order group (ascending N): orders, elements per workgroup, block
+ * order flag. Any output may be NULL; *n_groups receives the group count. */
+ndg_status ndg_mesh_tuning(const ndg_mesh *mesh, int32_t *orders, int32_t *epb, int32_t *remap,
+                           int32_t capacity, int32_t *n_groups);
 /* Mesh2D_t::estimate_error -> Element2D_t::estimate_p_error (src/entities/Element2D_t.cpp:274-343),
  * pressure estimator per element. Any output pointer may be NULL. */
 ndg_status ndg_estimate_error(ndg_mesh *mesh, double *error, double *sigma, int32_t *refine,

@@ -161,6 +161,9 @@ class Mesh2D_t {
         n_elements_ = (std::size_t)ne;
         n_faces_ = (std::size_t)faces;
         n_dofs_per_field_ = (std::size_t)dofs;
+        n_trace_values_ = 0;
+        for (std::size_t e = 0; e < n_elements_; ++e)
+            n_trace_values_ += 12 * (std::size_t)((mesh.quad_N.empty() ? initial_N : mesh.quad_N[e]) + 1);
     }
     Mesh2D_t(const Mesh2D_t &) = delete;
     Mesh2D_t &operator=(const Mesh2D_t &) = delete;
@@ -191,13 +194,15 @@ class Mesh2D_t {
             throw Error(NDG_ERR_INVALID, "Mesh2D_t::set_solution: wrong field length");
         check(ndg_set_state(handle_, p.data(), u.data(), v.data()));
     }
-    // Traces of the current state, [element][side][field][N+1].
+    // Traces of the current state, [element][side][field][N_e+1] packed element after element.
     void interpolate_to_boundaries(std::vector<double> &traces) {
-        int32_t nmin = 0, nmax = 0;
-        check(ndg_mesh_info(handle_, nullptr, nullptr, nullptr, &nmin, &nmax));
-        traces.resize(n_elements_ * 12 * (std::size_t)(nmax + 1));
+        traces.resize(n_trace_values_);
         check(ndg_get_traces(handle_, traces.data()));
     }
+    // NDG_GRAPH_OFF, NDG_GRAPH_SEQUENTIAL or NDG_GRAPH_PARALLEL (captured RK steps).
+    void set_graph(int32_t mode) { check(ndg_mesh_set_graph(handle_, mode)); }
+    // Times the launch shapes of each order group's stage kernel and keeps the fastest.
+    void autotune(int32_t repeats = 5) { check(ndg_mesh_autotune(handle_, repeats)); }
     void estimate_error(std::vector<double> &error, std::vector<double> &sigma, std::vector<int32_t> &refine,
                         std::vector<int32_t> &coarsen) {
         error.resize(n_elements_);
@@ -208,6 +213,11 @@ class Mesh2D_t {
     }
 
     ndg_mesh *handle() { return handle_; }
+
+  private:
+    std::size_t n_trace_values_ = 0;
+
+  public:
 
     int initial_N_;
     int maximum_N_;

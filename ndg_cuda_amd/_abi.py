@@ -19,6 +19,7 @@ NDG_ERR_NO_DEVICE = 4
 BC_NONE, BC_WALL, BC_SYMMETRY, BC_INFLOW, BC_OUTFLOW, BC_PERIODIC = range(6)
 IC_PLANE_WAVE, IC_PULSES = 0, 1
 PRECISION_FP64, PRECISION_FP32_COMPENSATED = 0, 1
+GRAPH_OFF, GRAPH_SEQUENTIAL, GRAPH_PARALLEL = 0, 1, 2
 
 TABLE_NAMES = ["nodes", "weights", "barycentric_weights", "lagrange_interpolant_left",
                "lagrange_interpolant_right", "lagrange_interpolant_derivative_left",
@@ -32,7 +33,7 @@ EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tabl
             "ndg_mesh_destroy", "ndg_mesh_set_stream", "ndg_mesh_info", "ndg_mesh_set_precision", "ndg_set_ic",
             "ndg_set_state", "ndg_get_state", "ndg_get_traces", "ndg_compute_rhs",
             "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_run_steps", "ndg_synchronize",
-            "ndg_estimate_error"]
+            "ndg_estimate_error", "ndg_mesh_set_graph", "ndg_mesh_autotune", "ndg_mesh_tuning"]
 
 
 class NDGError(RuntimeError):
@@ -97,6 +98,9 @@ def lib():
             "ndg_run_steps": [vp, f64, i64, vp],
             "ndg_synchronize": [vp],
             "ndg_estimate_error": [vp, vp, vp, vp, vp],
+            "ndg_mesh_set_graph": [vp, i32],
+            "ndg_mesh_autotune": [vp, i32],
+            "ndg_mesh_tuning": [vp, vp, vp, vp, i32, vp],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)

@@ -16,16 +16,34 @@ constexpr double kWaveSpeed = 1.0;  // include/helpers/constants.h:15
 constexpr int kMaxOrder = 16;       // largest order with a compiled kernel instance
 
 // Boundary (ghost) side record. One per element side whose neighbour is a wall, symmetry,
-// inflow or outflow ghost element. n/t are the face normal and tangent of the reference
-// (Face2D_t::compute_geometry with the domain element on the left); a/b the ghost element's
-// first two nodes after face building; sf the ghost's edge scaling factor (half the edge
-// length, src/entities/Element2D_t.cpp:779-790).
+// inflow or outflow ghost element, or whose face is non-conforming (kind kKindMortar). n/t are the
+// face normal and tangent of the reference (Face2D_t::compute_geometry with the domain element on
+// the left); a/b the ghost element's first two nodes after face building; sf the ghost's edge
+// scaling factor (half the edge length, src/entities/Element2D_t.cpp:779-790). For mortar sides
+// `slot` is the offset (in values) of the side's projected flux [3][N+1] in the side-flux buffer.
 struct BcSide {
     double nx, ny, tx, ty;
     double ax, ay, bx, by;
     double sf;
     int32_t kind;
     int32_t pad;
+    int64_t slot;
+};
+
+// Element side whose face is non-conforming: its element flux trace is produced by the mortar
+// kernel (interpolation to the face, flux, L2 projection back) before the stage kernels run.
+constexpr int32_t kKindMortar = 100;
+
+// One non-conforming face (src/meshes/Mesh2D_t.cpp:3803-4006). L is the element
+// that created the face in the reference's face building, R the other element (or, for a periodic
+// side, the element whose trace the ghost copies, in which case only L is projected: slotR = -1).
+struct MortarFace {
+    int64_t trL, trR;        // offsets (values) of the L / R side traces [3][N+1] in the trace buffer
+    int64_t slotL, slotR;    // offsets of the L / R projected side fluxes, slotR = -1 if one-sided
+    double nx, ny;           // face normal (outward from L)
+    double offset[2], scale[2];
+    double sfL, sfR;         // edge Jacobians of the L / R element sides
+    int32_t NL, NR, Nf, pad;
 };
 
 // Per-order constant tables in device memory, packed as Dhat[Nf*Nf], l(-1)[Nf], l(+1)[Nf], w[Nf],
@@ -39,6 +57,10 @@ inline int64_t table_poly_offset(int N) {
     return (int64_t)nf * nf + 5 * nf;
 }
 
+// 1D tables of every order 0..kMaxOrder for the mortar kernel: nodes, weights and barycentric
+// weights of order n at tab1d[(3 * n + which) * kTab1dStride].
+constexpr int kTab1dStride = kMaxOrder + 1;
+
 struct Tables {
     int N_max = 0;
     int64_t n_interp = 0;
@@ -49,45 +71,82 @@ void build_tables(int N_max, int64_t n_interp, Tables &t);
 // Packs the device table block for order N (layout above).
 std::vector<double> pack_tables(const Tables &t, int N);
 
-// Device-side description of a uniform-order conforming mesh.
-struct DeviceMesh {
+// Elements of one order, contiguous in device order (Hilbert order kept within the group).
+struct Group {
     int N = 0;
     int64_t K = 0;
-    double *verts = nullptr;     // [K][4][2]
-    int32_t *side_nb = nullptr;  // [K][4]: nb*4+side (>=0) or -(1+bc index)
+    int64_t e_begin = 0;     // first element (device order)
+    int64_t q_off = 0;       // offset (values) into q / g: 3 (N+1)^2 per element
+    int64_t tr_off = 0;      // offset (values) into the trace buffers: 12 (N+1) per element
+    double *tab = nullptr;   // packed tables for N
+    float *tab32 = nullptr;
+    // Launch shape of the group's stage kernel per precision (index NDG_PRECISION_*): elements
+    // per workgroup (0 = the shape's maximum) and the XCD-contiguous block order on/off.
+    struct Tune {
+        int epb = 0;
+        int remap = 1;
+    } tune[2];
+};
+
+// Device-side description of a mesh: elements grouped by order, each group stored as the
+// uniform-order layout [K_g][3][(N+1)^2] (state, RK register) and [K_g][4][3][N+1] (traces).
+struct DeviceMesh {
+    std::vector<Group> groups;
+    int64_t K = 0;
+    int N_min = 0, N_max = 0;
+    int64_t n_q = 0, n_tr = 0;   // values per state buffer / trace buffer
+    double *verts = nullptr;     // [K][4][2], device order
+    int32_t *side_nb = nullptr;  // [K][4]: group-local nb*4+side (>=0) or -(1+bc index)
     BcSide *bc = nullptr;        // [n_bc]
     int64_t n_bc = 0;
-    double *tab = nullptr;       // packed tables for N
-    double *q = nullptr;         // [K][3][Ne]
-    double *g = nullptr;         // [K][3][Ne]
-    double *trace[2] = {nullptr, nullptr};  // [K][4][3][Nf]
-    double *rhs = nullptr;       // [K][3][Ne], compute_rhs only (allocated on demand)
+    MortarFace *faces = nullptr; // [n_faces]
+    int64_t n_faces = 0;
+    double *side_flux = nullptr; // projected mortar side fluxes
+    float *side_flux32 = nullptr;
+    int64_t n_side_flux = 0;
+    double *tab1d = nullptr;     // 1D tables of all orders (mortar kernel)
+    double *q = nullptr;
+    double *g = nullptr;         // RK register, transposed within each (element, field) block
+    double *trace[2] = {nullptr, nullptr};
+    double *rhs = nullptr;       // compute_rhs only (allocated on demand)
     double *scratch = nullptr;   // reductions
     int64_t scratch_len = 0;
     int cur = 0;                 // trace buffer holding the traces of the current state
+    double *clock = nullptr;     // [4]: step start time, then the 3 stage times
     // FP32 form with compensated state (allocated by ndg_mesh_set_precision): the state is the
     // unevaluated sum qh + ql of two floats per value, the RK register and traces are floats.
     int precision = NDG_PRECISION_FP64;
-    float *qh = nullptr, *ql = nullptr, *g32 = nullptr;  // [K][3][Ne]
-    float *trace32[2] = {nullptr, nullptr};              // [K][4][3][Nf]
-    float *tab32 = nullptr;                              // packed tables for N, as floats
+    float *qh = nullptr, *ql = nullptr, *g32 = nullptr;
+    float *trace32[2] = {nullptr, nullptr};
 };
 
 // Kernel launchers (kernels.hip). All return hipSuccess or the first launch error.
 hipError_t launch_traces(const DeviceMesh &m, hipStream_t s);
-hipError_t launch_stage(DeviceMesh &m, int stage, double t_stage, double a, double gdt,
-                        hipStream_t s);
+// Stage s of the RK step (mortar faces, then every order group, in the mesh's precision); the
+// stage time is read from clock[1 + s] (see launch_clock_begin). Flips `cur`.
+hipError_t launch_stage(DeviceMesh &m, int stage, double a, double gdt, hipStream_t s);
 hipError_t launch_rhs(DeviceMesh &m, double t, hipStream_t s);
 // FP32 compensated form: conversions between the FP64 state (q, g) and (qh, ql, g32), traces of
 // qh, and the stage.
 hipError_t launch_to_c32(DeviceMesh &m, hipStream_t s);
 hipError_t launch_from_c32(DeviceMesh &m, hipStream_t s);
 hipError_t launch_traces_c32(const DeviceMesh &m, hipStream_t s);
-hipError_t launch_stage_c32(DeviceMesh &m, int stage, double t_stage, double a, double gdt,
-                            hipStream_t s);
+// One order group's stage kernel (of the mesh's precision); does not flip `cur`.
+hipError_t launch_group_stage(DeviceMesh &m, int group, int stage, double a, double gdt, hipStream_t s);
+// Device clock: begin writes the 3 stage times of a step starting at t (set != 0) or at clock[0];
+// end advances clock[0] by dt. Both are single-thread kernels, so a captured step replays with
+// the time advancing on the device.
+hipError_t launch_clock_begin(DeviceMesh &m, double t, int set, double dt, hipStream_t s);
+hipError_t launch_clock_end(DeviceMesh &m, double dt, hipStream_t s);
+// Autotuning: the largest elements-per-workgroup of the stage kernel for order N, and one launch
+// of group g's stage kernel that leaves the state and the current traces unchanged.
+int stage_max_epb(int N);
+hipError_t launch_group_probe(DeviceMesh &m, int group, hipStream_t s);
 hipError_t launch_ic(const DeviceMesh &m, int kind, int n_pulses, const double *pulses_dev,
                      hipStream_t s);
 hipError_t launch_delta_t(const DeviceMesh &m, double cfl, double *dt_dev, hipStream_t s);
+// Mortar faces: interpolation to the face, upwind flux, projection to the element sides.
+hipError_t launch_mortar(DeviceMesh &m, hipStream_t s);
 hipError_t launch_estimate(const DeviceMesh &m, double tol_min, double tol_max, double *out_dev,
                            hipStream_t s);
 bool order_supported(int N);

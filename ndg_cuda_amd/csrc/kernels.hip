@@ -137,6 +137,9 @@ traces_kernel(const double *__restrict__ tab, const double *__restrict__ q,
 //       (Solver2D_t.cpp:138-172)
 //   P5  coalesced stores of q and g, traces of the updated state for the next stage.
 // RHS_ONLY writes G (the compute_dg_wave_derivative output) and leaves the state alone.
+// A workgroup takes epb <= EPB consecutive elements (fewer gives more workgroups for a small
+// order group); remap != 0 applies the XCD-contiguous block order. t_ptr points at the stage time
+// in device memory (the inflow ghost reads it), so that a captured step can be replayed.
 template <int N>
 struct LineShape {
     static constexpr int NF = N + 1;
@@ -162,8 +165,9 @@ __global__ void __launch_bounds__(192)
 stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_nb,
              const BcSide *__restrict__ bc, const double *__restrict__ tab,
              double *__restrict__ q, double *__restrict__ g, const double *__restrict__ tr_in,
-             double *__restrict__ tr_out, double *__restrict__ rhs, int64_t K, int first,
-             double a, double gdt, double t_stage) {
+             double *__restrict__ tr_out, double *__restrict__ rhs, const double *__restrict__ side_flux,
+             int64_t K, int first, double a, double gdt, const double *__restrict__ t_ptr, int epb,
+             int remap) {
     using S = LineShape<N>;
     constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::T, FPT = S::FPT;
     constexpr double c = kWaveSpeed;
@@ -183,9 +187,9 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
     const int f = __builtin_amdgcn_readfirstlane(tid >> 6);  // field of this wave
     const int lane = tid & 63;
     const int el = lane / NF, l = lane % NF;
-    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t e0 = blk * EPB;
-    const int nel = (int)min<int64_t>(EPB, K - e0);
+    const int64_t blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    const int64_t e0 = blk * epb;
+    const int nel = (int)min<int64_t>(epb, K - e0);
     const bool active = el < nel;
 
     // P0
@@ -302,6 +306,13 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
                 vR = nbv[it][2];
             } else {
                 const BcSide rec = bc[-1 - cd];
+                if (rec.kind == kKindMortar) {  // projected by the mortar kernel (already x sf)
+                    const double *sfl = side_flux + rec.slot + k;
+                    tr[0] = sfl[0];
+                    tr[NF] = sfl[NF];
+                    tr[2 * NF] = sfl[2 * NF];
+                    continue;
+                }
                 nx = rec.nx;
                 ny = rec.ny;
                 if (rec.kind == NDG_BC_WALL || rec.kind == NDG_BC_SYMMETRY) {
@@ -321,7 +332,7 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
                     const double interp = (x[N - k] + 1) / 2;
                     const double X = rec.bx * interp + rec.ax * (1 - interp);
                     const double Y = rec.by * interp + rec.ay * (1 - interp);
-                    plane_wave(X, Y, t_stage, pR, uR, vR);
+                    plane_wave(X, Y, *t_ptr, pR, uR, vR);
                 } else {
                     pR = 0.0;
                     uR = 0.0;
@@ -531,7 +542,8 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
                  const BcSide *__restrict__ bc, const double *__restrict__ tab64,
                  const float *__restrict__ tab, float *__restrict__ qh, float *__restrict__ ql,
                  float *__restrict__ g, const float *__restrict__ tr_in, float *__restrict__ tr_out,
-                 int64_t K, int first, float a, float gdt, double t_stage) {
+                 const float *__restrict__ side_flux, int64_t K, int first, float a, float gdt,
+                 const double *__restrict__ t_ptr, int epb, int remap) {
     using S = LineShape<N>;
     constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::T, FPT = S::FPT;
     __shared__ float s_q[EPB * 3 * NE];
@@ -550,9 +562,9 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
     const int f = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     const int el = lane / NF, l = lane % NF;
-    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t e0 = blk * EPB;
-    const int nel = (int)min<int64_t>(EPB, K - e0);
+    const int64_t blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    const int64_t e0 = blk * epb;
+    const int nel = (int)min<int64_t>(epb, K - e0);
     const bool active = el < nel;
 
     // P0
@@ -672,6 +684,13 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
                 vR = nbv[it][2];
             } else {
                 const BcSide rec = bc[-1 - cd];
+                if (rec.kind == kKindMortar) {
+                    const float *sfl = side_flux + rec.slot + k;
+                    tr[0] = sfl[0];
+                    tr[NF] = sfl[NF];
+                    tr[2 * NF] = sfl[2 * NF];
+                    continue;
+                }
                 nx = (float)rec.nx;
                 ny = (float)rec.ny;
                 const float tx = (float)rec.tx, ty = (float)rec.ty, gs = (float)rec.sf;
@@ -691,7 +710,7 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
                     const double X = rec.bx * interp + rec.ax * (1 - interp);
                     const double Y = rec.by * interp + rec.ay * (1 - interp);
                     double p64, u64, v64;
-                    plane_wave(X, Y, t_stage, p64, u64, v64);
+                    plane_wave(X, Y, *t_ptr, p64, u64, v64);
                     pR = (float)p64;
                     uR = (float)u64;
                     vR = (float)v64;
@@ -938,160 +957,366 @@ estimate_kernel(const double *__restrict__ tab, const double *__restrict__ q, in
 }
 
 // ---------------------------------------------------------------------------------------------
+// Device clock of a step: clock[0] is the time at the start of the step, clock[1 + s] the time of
+// stage s, t + b_s dt (include/solvers/Solver2D_t.tpp:16, :45-63), rounded as the host loop does
+// (no contraction). set != 0 starts from t, else from clock[0]; clock_end advances clock[0] by dt.
+__global__ void clock_begin_kernel(double *clock, double t, int set, double dt) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    const double t0 = set ? t : clock[0];
+    clock[0] = t0;
+    clock[1] = t0 + 0.0 * dt;
+    clock[2] = t0 + (1.0 / 3.0) * dt;
+    clock[3] = t0 + 0.75 * dt;
+}
+
+__global__ void clock_end_kernel(double *clock, double dt) {
+#pragma clang fp contract(off)
+    if (threadIdx.x == 0) clock[0] = clock[0] + dt;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Non-conforming (mortar) faces, one thread per face, FP64 arithmetic (the traces and the output are
+// of type T: double, or float in the FP32 form). Restates, for one face:
+//   project_to_faces (src/meshes/Mesh2D_t.cpp:3803-3893): each side's trace is copied when the
+//     element order equals the face order, else interpolated barycentrically at
+//     xi = (x_face + 1) scale + 2 offset - 1 with the exact-node shortcut (right side reversed);
+//   calculate_wave_fluxes (src/solvers/Solver2D_t.cpp:48-63);
+//   project_to_elements (src/meshes/Mesh2D_t.cpp:3895-4003): conforming copy (left forward, right
+//     backward and negated) or the Gauss-quadrature L2 projection, times the edge Jacobian and the
+//     face scale, accumulated over face nodes in order.
+
+__device__ __forceinline__ bool ulp2_equal(double a, double b) {
+    return fabs(a - b) <= DBL_EPSILON * fabs(a + b) * 2 || fabs(a - b) < DBL_MIN;
+}
+
+template <typename T>
+__device__ void mortar_side_state(const T *tr, int Ne, int Nf, const double *xf, const double *xe,
+                                  const double *le, double scale, double offset, bool reversed,
+                                  double out[3][kMaxOrder + 1]) {
+    if (Ne == Nf) {
+        for (int i = 0; i <= Nf; ++i)
+            for (int f = 0; f < 3; ++f) out[f][i] = (double)tr[f * (Ne + 1) + (reversed ? Nf - i : i)];
+        return;
+    }
+    for (int i = 0; i <= Nf; ++i) {
+        const double coord = (xf[reversed ? Nf - i : i] + 1) * scale + 2 * offset - 1;
+        double num[3] = {0.0, 0.0, 0.0}, den = 0.0;
+        for (int j = 0; j <= Ne; ++j) {
+            if (ulp2_equal(coord, xe[j])) {
+                for (int f = 0; f < 3; ++f) num[f] = (double)tr[f * (Ne + 1) + j];
+                den = 1.0;
+                break;
+            }
+            const double t = le[j] / (coord - xe[j]);
+            for (int f = 0; f < 3; ++f) num[f] += t * (double)tr[f * (Ne + 1) + j];
+            den += t;
+        }
+        for (int f = 0; f < 3; ++f) out[f][i] = num[f] / den;
+    }
+}
+
+template <typename T>
+__device__ void mortar_project(const double F[3][kMaxOrder + 1], int Ne, int Nf, const double *xf,
+                               const double *wf, const double *xe, const double *we, const double *le,
+                               double sf, double scale, double offset, bool backward, T *out) {
+    if (Ne == Nf) {
+        for (int j = 0; j <= Nf; ++j)
+            for (int f = 0; f < 3; ++f) {
+                if (backward) out[f * (Ne + 1) + (Nf - j)] = (T)(-F[f][j] * sf);
+                else out[f * (Ne + 1) + j] = (T)(F[f][j] * sf);
+            }
+        return;
+    }
+    double acc[3][kMaxOrder + 1];
+    for (int i = 0; i <= Ne; ++i) acc[0][i] = acc[1][i] = acc[2][i] = 0.0;
+    for (int j = 0; j <= Nf; ++j) {
+        const double coord = (xf[backward ? Nf - j : j] + 1) * scale + 2 * offset - 1;
+        int hit = -1;
+        for (int i = 0; i <= Ne; ++i)
+            if (ulp2_equal(coord, xe[i])) { hit = i; break; }
+        if (hit >= 0) {
+            const int i = hit;
+            const double wr = (backward ? -wf[j] : wf[j]) / we[i];
+            for (int f = 0; f < 3; ++f) acc[f][i] += wr * F[f][j] * sf * scale;
+        } else {
+            double ssum = 0.0;
+            for (int i = 0; i <= Ne; ++i) ssum += le[i] / (coord - xe[i]);
+            for (int i = 0; i <= Ne; ++i) {
+                const double Tc = le[i] / ((coord - xe[i]) * ssum);
+                const double wr = (backward ? -Tc : Tc) * wf[j] / we[i];
+                for (int f = 0; f < 3; ++f) acc[f][i] += wr * F[f][j] * sf * scale;
+            }
+        }
+    }
+    for (int i = 0; i <= Ne; ++i)
+        for (int f = 0; f < 3; ++f) out[f * (Ne + 1) + i] = (T)acc[f][i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+mortar_kernel(const MortarFace *__restrict__ faces, int64_t n_faces, const double *__restrict__ tab1d,
+              const T *__restrict__ tr, T *__restrict__ side_flux) {
+    const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (fi >= n_faces) return;
+    const MortarFace fc = faces[fi];
+    const double *xf = tab1d + (3 * fc.Nf + 0) * kTab1dStride, *wf = tab1d + (3 * fc.Nf + 1) * kTab1dStride;
+    const double *xL = tab1d + (3 * fc.NL + 0) * kTab1dStride, *wL = tab1d + (3 * fc.NL + 1) * kTab1dStride;
+    const double *lL = tab1d + (3 * fc.NL + 2) * kTab1dStride;
+    const double *xR = tab1d + (3 * fc.NR + 0) * kTab1dStride, *wR = tab1d + (3 * fc.NR + 1) * kTab1dStride;
+    const double *lR = tab1d + (3 * fc.NR + 2) * kTab1dStride;
+    double sL[3][kMaxOrder + 1], sR[3][kMaxOrder + 1];
+    mortar_side_state(tr + fc.trL, fc.NL, fc.Nf, xf, xL, lL, fc.scale[0], fc.offset[0], false, sL);
+    mortar_side_state(tr + fc.trR, fc.NR, fc.Nf, xf, xR, lR, fc.scale[1], fc.offset[1], true, sR);
+    const double c = kWaveSpeed;
+    for (int i = 0; i <= fc.Nf; ++i) {
+        const double wl = sL[0][i] + c * (sL[1][i] * fc.nx + sL[2][i] * fc.ny);
+        const double wr = sR[0][i] - c * (sR[1][i] * fc.nx + sR[2][i] * fc.ny);
+        sL[0][i] = c * (wl - wr) / 2;      // the face flux overwrites the left state
+        sL[1][i] = fc.nx * (wl + wr) / 2;
+        sL[2][i] = fc.ny * (wl + wr) / 2;
+    }
+    mortar_project<T>(sL, fc.NL, fc.Nf, xf, wf, xL, wL, lL, fc.sfL, fc.scale[0], fc.offset[0], false,
+                      side_flux + fc.slotL);
+    if (fc.slotR >= 0)
+        mortar_project<T>(sL, fc.NR, fc.Nf, xf, wf, xR, wR, lR, fc.sfR, fc.scale[1], fc.offset[1], true,
+                          side_flux + fc.slotR);
+}
+
+// ---------------------------------------------------------------------------------------------
 // Launchers: dispatch on the (uniform) order N.
 #define NDG_FOR_ORDERS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
-
 bool order_supported(int N) { return N >= 1 && N <= kMaxOrder; }
 
+#define CASES_1(F, ...) \
+    case 1: return F<1>(__VA_ARGS__); case 2: return F<2>(__VA_ARGS__); case 3: return F<3>(__VA_ARGS__); \
+    case 4: return F<4>(__VA_ARGS__); case 5: return F<5>(__VA_ARGS__); case 6: return F<6>(__VA_ARGS__); \
+    case 7: return F<7>(__VA_ARGS__); case 8: return F<8>(__VA_ARGS__); case 9: return F<9>(__VA_ARGS__); \
+    case 10: return F<10>(__VA_ARGS__); case 11: return F<11>(__VA_ARGS__); case 12: return F<12>(__VA_ARGS__); \
+    case 13: return F<13>(__VA_ARGS__); case 14: return F<14>(__VA_ARGS__); case 15: return F<15>(__VA_ARGS__); \
+    case 16: return F<16>(__VA_ARGS__);
+
+// Launchers: every per-element kernel runs once per order group, on the group's slices of the
+// device buffers.
+
 template <int N>
-static hipError_t traces_n(const DeviceMesh &m, hipStream_t s) {
+static hipError_t traces_n(const DeviceMesh &m, const Group &G, hipStream_t s) {
     using S = Shape<N>;
-    const int64_t blocks = (m.K + S::EPB - 1) / S::EPB;
-    hipLaunchKernelGGL(traces_kernel<N>, dim3((unsigned)blocks), dim3(S::THREADS), 0, s, m.tab, m.q,
-                       m.trace[m.cur], m.K);
+    const int64_t blocks = (G.K + S::EPB - 1) / S::EPB;
+    hipLaunchKernelGGL(traces_kernel<N>, dim3((unsigned)blocks), dim3(S::THREADS), 0, s, G.tab, m.q + G.q_off,
+                       m.trace[m.cur] + G.tr_off, G.K);
     return hipGetLastError();
 }
 
+// Elements per workgroup actually used for group G (a tuning choice, at most the shape's EPB).
 template <int N>
-static hipError_t stage_n(DeviceMesh &m, int stage, double t_stage, double a, double gdt, hipStream_t s) {
+static int group_epb(const Group &G, int precision) {
+    const int e = G.tune[precision].epb;
+    return e >= 1 && e <= LineShape<N>::EPB ? e : LineShape<N>::EPB;
+}
+
+template <int N>
+static hipError_t stage_n(DeviceMesh &m, const Group &G, int stage, double a, double gdt, hipStream_t s) {
     using S = LineShape<N>;
-    const int64_t blocks = (m.K + S::EPB - 1) / S::EPB;
-    hipLaunchKernelGGL((stage_kernel<N, false>), dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts,
-                       m.side_nb, m.bc, m.tab, m.q, m.g, m.trace[m.cur], m.trace[1 - m.cur], nullptr, m.K,
-                       stage == 0 ? 1 : 0, a, gdt, t_stage);
-    m.cur = 1 - m.cur;
+    const int epb = group_epb<N>(G, NDG_PRECISION_FP64);
+    const int64_t blocks = (G.K + epb - 1) / epb;
+    hipLaunchKernelGGL((stage_kernel<N, false>), dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts + G.e_begin * 8,
+                       m.side_nb + G.e_begin * 4, m.bc, G.tab, m.q + G.q_off, m.g + G.q_off,
+                       m.trace[m.cur] + G.tr_off, m.trace[1 - m.cur] + G.tr_off, nullptr, m.side_flux, G.K,
+                       stage == 0 ? 1 : 0, a, gdt, m.clock + 1 + stage, epb, G.tune[NDG_PRECISION_FP64].remap);
     return hipGetLastError();
 }
 
 template <int N>
-static hipError_t rhs_n(DeviceMesh &m, double t, hipStream_t s) {
+static hipError_t rhs_n(DeviceMesh &m, const Group &G, hipStream_t s) {
     using S = LineShape<N>;
-    const int64_t blocks = (m.K + S::EPB - 1) / S::EPB;
-    hipLaunchKernelGGL((stage_kernel<N, true>), dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts,
-                       m.side_nb, m.bc, m.tab, m.q, m.g, m.trace[m.cur], nullptr, m.rhs, m.K, 1, 0.0, 0.0, t);
+    const int64_t blocks = (G.K + S::EPB - 1) / S::EPB;
+    hipLaunchKernelGGL((stage_kernel<N, true>), dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts + G.e_begin * 8,
+                       m.side_nb + G.e_begin * 4, m.bc, G.tab, m.q + G.q_off, m.g + G.q_off,
+                       m.trace[m.cur] + G.tr_off, nullptr, m.rhs + G.q_off, m.side_flux, G.K, 1, 0.0, 0.0,
+                       m.clock + 1, S::EPB, 1);
     return hipGetLastError();
 }
 
 template <int N>
-static hipError_t ic_n(const DeviceMesh &m, int kind, int n_pulses, const double *pulses, hipStream_t s) {
-    const int64_t total = m.K * (N + 1) * (N + 1);
-    hipLaunchKernelGGL(ic_kernel<N>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, m.verts, m.tab, m.q,
-                       m.K, kind, n_pulses, pulses);
+static hipError_t ic_n(const DeviceMesh &m, const Group &G, int kind, int n_pulses, const double *pulses,
+                       hipStream_t s) {
+    const int64_t total = G.K * (N + 1) * (N + 1);
+    hipLaunchKernelGGL(ic_kernel<N>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, m.verts + G.e_begin * 8,
+                       G.tab, m.q + G.q_off, G.K, kind, n_pulses, pulses);
     return hipGetLastError();
 }
 
 template <int N>
-static hipError_t estimate_n(const DeviceMesh &m, double tol_min, double tol_max, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(estimate_kernel<N>, dim3((unsigned)((m.K + 127) / 128)), dim3(128), 0, s, m.tab, m.q, m.K,
-                       tol_min, tol_max, out);
+static hipError_t estimate_n(const DeviceMesh &m, const Group &G, double tol_min, double tol_max, double *out,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(estimate_kernel<N>, dim3((unsigned)((G.K + 127) / 128)), dim3(128), 0, s, G.tab,
+                       m.q + G.q_off, G.K, tol_min, tol_max, out + 4 * G.e_begin);
     return hipGetLastError();
+}
+
+template <int N>
+static hipError_t traces_c32_n(const DeviceMesh &m, const Group &G, hipStream_t s) {
+    using S = LineShape<N>;
+    const int64_t blocks = (G.K + S::EPB - 1) / S::EPB;
+    hipLaunchKernelGGL(traces_c32_kernel<N>, dim3((unsigned)blocks), dim3(S::T), 0, s, G.tab32, m.qh + G.q_off,
+                       m.trace32[m.cur] + G.tr_off, G.K);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t stage_c32_n(DeviceMesh &m, const Group &G, int stage, double a, double gdt, hipStream_t s) {
+    using S = LineShape<N>;
+    const int epb = group_epb<N>(G, NDG_PRECISION_FP32_COMPENSATED);
+    const int64_t blocks = (G.K + epb - 1) / epb;
+    hipLaunchKernelGGL(stage_c32_kernel<N>, dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts + G.e_begin * 8,
+                       m.side_nb + G.e_begin * 4, m.bc, G.tab, G.tab32, m.qh + G.q_off, m.ql + G.q_off,
+                       m.g32 + G.q_off, m.trace32[m.cur] + G.tr_off, m.trace32[1 - m.cur] + G.tr_off,
+                       m.side_flux32, G.K, stage == 0 ? 1 : 0, (float)a, (float)gdt, m.clock + 1 + stage, epb,
+                       G.tune[NDG_PRECISION_FP32_COMPENSATED].remap);
+    return hipGetLastError();
+}
+
+template <int N>
+static int max_epb_n() { return LineShape<N>::EPB; }
+
+int stage_max_epb(int N) {
+    switch (N) { CASES_1(max_epb_n) }
+    return 0;
+}
+
+hipError_t launch_group_stage(DeviceMesh &m, int group, int stage, double a, double gdt, hipStream_t s) {
+    const Group &G = m.groups[group];
+    if (m.precision == NDG_PRECISION_FP32_COMPENSATED) {
+        switch (G.N) { CASES_1(stage_c32_n, m, G, stage, a, gdt, s) }
+    } else {
+        switch (G.N) { CASES_1(stage_n, m, G, stage, a, gdt, s) }
+    }
+    return hipErrorInvalidValue;
+}
+
+// Autotuning probe: stage 0 form with a = 0 and gdt = 0, so that the state and the current traces
+// are left as they are.
+hipError_t launch_group_probe(DeviceMesh &m, int group, hipStream_t s) {
+    return launch_group_stage(m, group, 0, 0.0, 0.0, s);
+}
+
+
+
+template <typename Fn>
+static hipError_t for_groups(const DeviceMesh &m, Fn &&fn) {
+    for (const Group &G : m.groups) {
+        const hipError_t e = fn(G);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_traces(const DeviceMesh &m, hipStream_t s) {
-    switch (m.N) {
-#define X(n) case n: return traces_n<n>(m, s);
-        NDG_FOR_ORDERS(X)
-#undef X
-    }
-    return hipErrorInvalidValue;
+    return for_groups(m, [&](const Group &G) -> hipError_t {
+        switch (G.N) { CASES_1(traces_n, m, G, s) }
+        return hipErrorInvalidValue;
+    });
 }
 
-hipError_t launch_stage(DeviceMesh &m, int stage, double t_stage, double a, double gdt, hipStream_t s) {
-    switch (m.N) {
-#define X(n) case n: return stage_n<n>(m, stage, t_stage, a, gdt, s);
-        NDG_FOR_ORDERS(X)
-#undef X
-    }
-    return hipErrorInvalidValue;
+hipError_t launch_mortar(DeviceMesh &m, hipStream_t s) {
+    if (m.n_faces == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((m.n_faces + 63) / 64);
+    if (m.precision == NDG_PRECISION_FP32_COMPENSATED)
+        hipLaunchKernelGGL(mortar_kernel<float>, dim3(blocks), dim3(64), 0, s, m.faces, m.n_faces, m.tab1d,
+                           m.trace32[m.cur], m.side_flux32);
+    else
+        hipLaunchKernelGGL(mortar_kernel<double>, dim3(blocks), dim3(64), 0, s, m.faces, m.n_faces, m.tab1d,
+                           m.trace[m.cur], m.side_flux);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage(DeviceMesh &m, int stage, double a, double gdt, hipStream_t s) {
+    hipError_t e = launch_mortar(m, s);
+    if (e != hipSuccess) return e;
+    for (int g = 0; g < (int)m.groups.size(); ++g)
+        if ((e = launch_group_stage(m, g, stage, a, gdt, s)) != hipSuccess) return e;
+    m.cur = 1 - m.cur;
+    return hipSuccess;
 }
 
 hipError_t launch_rhs(DeviceMesh &m, double t, hipStream_t s) {
-    switch (m.N) {
-#define X(n) case n: return rhs_n<n>(m, t, s);
-        NDG_FOR_ORDERS(X)
-#undef X
-    }
-    return hipErrorInvalidValue;
+    hipLaunchKernelGGL(clock_begin_kernel, dim3(1), dim3(64), 0, s, m.clock, t, 1, 0.0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = launch_mortar(m, s)) != hipSuccess) return e;
+    return for_groups(m, [&](const Group &G) -> hipError_t {
+        switch (G.N) { CASES_1(rhs_n, m, G, s) }
+        return hipErrorInvalidValue;
+    });
 }
 
 hipError_t launch_ic(const DeviceMesh &m, int kind, int n_pulses, const double *pulses, hipStream_t s) {
-    switch (m.N) {
-#define X(n) case n: return ic_n<n>(m, kind, n_pulses, pulses, s);
-        NDG_FOR_ORDERS(X)
-#undef X
-    }
-    return hipErrorInvalidValue;
+    return for_groups(m, [&](const Group &G) -> hipError_t {
+        switch (G.N) { CASES_1(ic_n, m, G, kind, n_pulses, pulses, s) }
+        return hipErrorInvalidValue;
+    });
 }
 
 hipError_t launch_estimate(const DeviceMesh &m, double tol_min, double tol_max, double *out, hipStream_t s) {
-    switch (m.N) {
-#define X(n) case n: return estimate_n<n>(m, tol_min, tol_max, out, s);
-        NDG_FOR_ORDERS(X)
-#undef X
-    }
-    return hipErrorInvalidValue;
-}
-
-template <int N>
-static hipError_t traces_c32_n(const DeviceMesh &m, hipStream_t s) {
-    using S = LineShape<N>;
-    const int64_t blocks = (m.K + S::EPB - 1) / S::EPB;
-    hipLaunchKernelGGL(traces_c32_kernel<N>, dim3((unsigned)blocks), dim3(S::T), 0, s, m.tab32, m.qh,
-                       m.trace32[m.cur], m.K);
-    return hipGetLastError();
-}
-
-template <int N>
-static hipError_t stage_c32_n(DeviceMesh &m, int stage, double t_stage, double a, double gdt, hipStream_t s) {
-    using S = LineShape<N>;
-    const int64_t blocks = (m.K + S::EPB - 1) / S::EPB;
-    hipLaunchKernelGGL(stage_c32_kernel<N>, dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts, m.side_nb, m.bc,
-                       m.tab, m.tab32, m.qh, m.ql, m.g32, m.trace32[m.cur], m.trace32[1 - m.cur], m.K,
-                       stage == 0 ? 1 : 0, (float)a, (float)gdt, t_stage);
-    m.cur = 1 - m.cur;
-    return hipGetLastError();
+    return for_groups(m, [&](const Group &G) -> hipError_t {
+        switch (G.N) { CASES_1(estimate_n, m, G, tol_min, tol_max, out, s) }
+        return hipErrorInvalidValue;
+    });
 }
 
 hipError_t launch_traces_c32(const DeviceMesh &m, hipStream_t s) {
-    switch (m.N) {
-#define X(n) case n: return traces_c32_n<n>(m, s);
-        NDG_FOR_ORDERS(X)
-#undef X
-    }
-    return hipErrorInvalidValue;
+    return for_groups(m, [&](const Group &G) -> hipError_t {
+        switch (G.N) { CASES_1(traces_c32_n, m, G, s) }
+        return hipErrorInvalidValue;
+    });
 }
 
-hipError_t launch_stage_c32(DeviceMesh &m, int stage, double t_stage, double a, double gdt, hipStream_t s) {
-    switch (m.N) {
-#define X(n) case n: return stage_c32_n<n>(m, stage, t_stage, a, gdt, s);
-        NDG_FOR_ORDERS(X)
-#undef X
-    }
-    return hipErrorInvalidValue;
-}
 
+// q and qh are in reference node order; ql, g and g32 transposed within each (element, field)
+// block. Run per group (the transposition depends on N).
 hipError_t launch_to_c32(DeviceMesh &m, hipStream_t s) {
-    const int64_t n = m.K * 3 * (int64_t)(m.N + 1) * (m.N + 1);
-    hipLaunchKernelGGL(to_c32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m.q, m.g, m.qh, m.ql,
-                       m.g32, n, m.N + 1);
-    return hipGetLastError();
+    return for_groups(m, [&](const Group &G) -> hipError_t {
+        const int64_t n = G.K * 3 * (int64_t)(G.N + 1) * (G.N + 1);
+        hipLaunchKernelGGL(to_c32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m.q + G.q_off,
+                           m.g + G.q_off, m.qh + G.q_off, m.ql + G.q_off, m.g32 + G.q_off, n, G.N + 1);
+        return hipGetLastError();
+    });
 }
 
 hipError_t launch_from_c32(DeviceMesh &m, hipStream_t s) {
-    const int64_t n = m.K * 3 * (int64_t)(m.N + 1) * (m.N + 1);
-    hipLaunchKernelGGL(from_c32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m.qh, m.ql, m.g32,
-                       m.q, m.g, n, m.N + 1);
+    return for_groups(m, [&](const Group &G) -> hipError_t {
+        const int64_t n = G.K * 3 * (int64_t)(G.N + 1) * (G.N + 1);
+        hipLaunchKernelGGL(from_c32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m.qh + G.q_off,
+                           m.ql + G.q_off, m.g32 + G.q_off, m.q + G.q_off, m.g + G.q_off, n, G.N + 1);
+        return hipGetLastError();
+    });
+}
+
+// One partial minimum per workgroup of every group (at most 1024 in total), then one final pass.
+hipError_t launch_delta_t(const DeviceMesh &m, double cfl, double *dt_dev, hipStream_t s) {
+    int used = 0;
+    for (const Group &G : m.groups) {
+        const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((G.K + 255) / 256,
+                                                                      1024 / (int64_t)m.groups.size()));
+        if (used + blocks > m.scratch_len) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(dt_partial_kernel, dim3(blocks), dim3(256), 0, s, m.verts + G.e_begin * 8, G.K, G.N, cfl,
+                           m.scratch + used);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        used += blocks;
+    }
+    hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(256), 0, s, m.scratch, used, dt_dev);
     return hipGetLastError();
 }
 
-hipError_t launch_delta_t(const DeviceMesh &m, double cfl, double *dt_dev, hipStream_t s) {
-    int blocks = (int)std::min<int64_t>((m.K + 255) / 256, 1024);
-    if (blocks < 1) blocks = 1;
-    if (m.scratch_len < blocks) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(dt_partial_kernel, dim3(blocks), dim3(256), 0, s, m.verts, m.K, m.N, cfl, m.scratch);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(256), 0, s, m.scratch, blocks, dt_dev);
+hipError_t launch_clock_begin(DeviceMesh &m, double t, int set, double dt, hipStream_t s) {
+    hipLaunchKernelGGL(clock_begin_kernel, dim3(1), dim3(64), 0, s, m.clock, t, set, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_clock_end(DeviceMesh &m, double dt, hipStream_t s) {
+    hipLaunchKernelGGL(clock_end_kernel, dim3(1), dim3(64), 0, s, m.clock, dt);
     return hipGetLastError();
 }
 

@@ -40,10 +40,6 @@ constexpr double kAm[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
 constexpr double kBm[3] = {0.0, 1.0 / 3.0, 0.75};
 constexpr double kGm[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 
-bool ulp2_equal(double a, double b) {
-    return std::fabs(a - b) <= DBL_EPSILON * std::fabs(a + b) * 2 || std::fabs(a - b) < DBL_MIN;
-}
-
 }  // namespace
 
 ndg_status ndg::set_error(ndg_status code, const char *msg) { return fail(code, msg); }
@@ -63,6 +59,19 @@ struct ndg_mesh {
     double *est_dev = nullptr;
     double *pulses_dev = nullptr;
     std::vector<double> staging;
+    // Element e of the caller's order is element dev_of[e] of the device order (grouped by N).
+    std::vector<int64_t> dev_of;
+    std::vector<int32_t> order;      // N of element e (caller's order)
+    std::vector<int64_t> host_off;   // offset of element e in a packed per-field host array
+    int64_t n_dofs = 0;              // values per field
+    // Captured RK step (ndg_mesh_set_graph): one executable graph per starting trace buffer, for
+    // the step size and precision it was captured with.
+    int graph_mode = NDG_GRAPH_OFF;
+    hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+    double graph_dt = 0.0;
+    int graph_precision = -1;
+    std::vector<hipStream_t> side_streams;   // branches of the parallel capture (default priority)
+    std::vector<hipEvent_t> side_events;     // [0] fork, [1 + g] join of group g
 };
 
 extern "C" ndg_status ndg_last_error(char *buf, size_t buf_len) {
@@ -123,24 +132,43 @@ struct v2 {
 double mag(double x, double y) { return std::sqrt(x * x + y * y); }
 
 // Face geometry with the domain element on the left (src/entities/Face2D_t.cpp:56-66).
-void face_frame(v2 f0, v2 f1, v2 l0, v2 l1, BcSide &rec) {
-    double tx = f1.x - f0.x, ty = f1.y - f0.y;
+void face_frame(v2 f0, v2 f1, v2 l0, v2 l1, double &nx, double &ny, double &tx, double &ty) {
+    tx = f1.x - f0.x;
+    ty = f1.y - f0.y;
     const double len = mag(tx, ty);
     tx /= len;
     ty /= len;
-    double nx = ty, ny = -tx;
+    nx = ty;
+    ny = -tx;
     const double lx = l1.x - l0.x, ly = l1.y - l0.y;
     const double sign = std::copysign(1.0, tx * lx + ty * ly);
     nx *= sign; ny *= sign; tx *= sign; ty *= sign;
-    rec.nx = nx; rec.ny = ny; rec.tx = tx; rec.ty = ty;
 }
 
-// Ghost element side-0 scaling factor for nodes {P0, P1, P2, P3}
-// (src/entities/Element2D_t.cpp:779-786 with include/functions/quad_metrics.tpp:3-8 at eta = -1).
-double ghost_scale(v2 P0, v2 P1, v2 P2, v2 P3) {
-    const double eta = -1.0;
-    const double mx = ((1 - eta) * (P1.x - P0.x) + (1 + eta) * (P2.x - P3.x)) / 4;
-    const double my = ((1 - eta) * (P1.y - P0.y) + (1 + eta) * (P2.y - P3.y)) / 4;
+// Face offsets and scales of the two sides (src/entities/Face2D_t.cpp:68-81): en = {L side start,
+// L side end, R side start, R side end}, face nodes f0, f1.
+void face_offsets(v2 f0, v2 f1, const v2 en[4], double offset[2], double scale[2]) {
+    const double d0 = mag(en[1].x - en[0].x, en[1].y - en[0].y);
+    const double d1 = mag(en[3].x - en[2].x, en[3].y - en[2].y);
+    const double fd = mag(f1.x - f0.x, f1.y - f0.y);
+    offset[0] = mag(f0.x - en[0].x, f0.y - en[0].y) / d0;
+    offset[1] = mag(f1.x - en[2].x, f1.y - en[2].y) / d1;
+    scale[0] = fd / d0;
+    scale[1] = fd / d1;
+}
+
+// Edge Jacobian of side s of the bilinear quad P (include/functions/quad_metrics.tpp:3-8 on the
+// side, src/entities/Element2D_t.cpp:768-772); it is the same at every node of the side.
+double side_scale(const v2 P[4], int s) {
+    if (s == 0 || s == 2) {
+        const double eta = s == 0 ? -1.0 : 1.0;
+        const double mx = ((1 - eta) * (P[1].x - P[0].x) + (1 + eta) * (P[2].x - P[3].x)) / 4;
+        const double my = ((1 - eta) * (P[1].y - P[0].y) + (1 + eta) * (P[2].y - P[3].y)) / 4;
+        return std::sqrt(mx * mx + my * my);
+    }
+    const double xi = s == 1 ? 1.0 : -1.0;
+    const double mx = ((1 - xi) * (P[3].x - P[0].x) + (1 + xi) * (P[2].x - P[1].x)) / 4;
+    const double my = ((1 - xi) * (P[3].y - P[0].y) + (1 + xi) * (P[2].y - P[1].y)) / 4;
     return std::sqrt(mx * mx + my * my);
 }
 
@@ -157,13 +185,38 @@ ndg_status device_upload(T **dst, const std::vector<T> &src, const char *what) {
 void free_mesh(ndg_mesh *m) {
     if (!m) return;
     DeviceMesh &d = m->d;
-    for (void *p : {(void *)d.verts, (void *)d.side_nb, (void *)d.bc, (void *)d.tab, (void *)d.q, (void *)d.g,
-                    (void *)d.trace[0], (void *)d.trace[1], (void *)d.rhs, (void *)d.scratch, (void *)m->dt_dev,
-                    (void *)m->est_dev, (void *)m->pulses_dev, (void *)d.qh, (void *)d.ql, (void *)d.g32,
-                    (void *)d.trace32[0], (void *)d.trace32[1], (void *)d.tab32})
+    std::vector<void *> ptrs = {(void *)d.verts, (void *)d.side_nb, (void *)d.bc, (void *)d.faces,
+                                (void *)d.side_flux, (void *)d.side_flux32, (void *)d.tab1d, (void *)d.q,
+                                (void *)d.g, (void *)d.trace[0], (void *)d.trace[1], (void *)d.rhs,
+                                (void *)d.scratch, (void *)m->dt_dev, (void *)m->est_dev, (void *)m->pulses_dev,
+                                (void *)d.qh, (void *)d.ql, (void *)d.g32, (void *)d.trace32[0],
+                                (void *)d.trace32[1], (void *)d.clock};
+    for (const Group &G : d.groups) {
+        ptrs.push_back(G.tab);
+        ptrs.push_back(G.tab32);
+    }
+    for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (hipGraphExec_t &x : m->step_exec)
+        if (x) (void)hipGraphExecDestroy(x);
+    for (hipEvent_t ev : m->side_events) (void)hipEventDestroy(ev);
+    for (hipStream_t st : m->side_streams) (void)hipStreamDestroy(st);
     if (m->own_stream) (void)hipStreamDestroy(m->own_stream);
     delete m;
+}
+
+// The ghost projection of a conforming boundary face (src/meshes/Mesh2D_t.cpp:4383-4418) is the
+// reversed copy of the element trace up to roundoff: the face coordinate of ghost node j is the
+// mirrored element node and the weight ratio w[j] / w[N - j] is 1 (the kernels use the copy). This
+// checks that both hold to 1e-13 for the tables in use.
+bool ghost_copy_is_close(const Tables &t, int N) {
+    const auto &x = t.tab[NDG_TABLE_NODES][N];
+    const auto &w = t.tab[NDG_TABLE_WEIGHTS][N];
+    for (int j = 0; j <= N; ++j) {
+        const double coord = (x[N - j] + 1) * 1.0 + 2 * 0.0 - 1;
+        if (std::fabs(coord - x[N - j]) > 1e-13 || std::fabs(w[j] / w[N - j] - 1.0) > 1e-13) return false;
+    }
+    return true;
 }
 
 }  // namespace
@@ -176,26 +229,24 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
     if (B > 0 && (!desc->bars || !desc->bar_type)) return fail(NDG_ERR_INVALID, "ndg_mesh_create: bars without types");
     if (NN >= (int64_t(1) << 32)) return fail(NDG_ERR_INVALID, "ndg_mesh_create: too many nodes");
     if (K > (int64_t(1) << 29)) return fail(NDG_ERR_INVALID, "ndg_mesh_create: too many elements");
-    int N = desc->initial_N;
-    if (desc->quad_N) {
-        N = desc->quad_N[0];
-        for (int64_t e = 1; e < K; ++e)
-            if (desc->quad_N[e] != N)
-                return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: elements of different orders need the mortar "
-                                                 "(mixed-order) path, which this build does not provide yet");
-        if (N != desc->initial_N)
-            return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: element orders differ from initial_N (ghost and face "
-                                             "orders would differ from the element order)");
+    std::vector<int32_t> order((size_t)K, desc->initial_N);
+    if (desc->quad_N) std::copy(desc->quad_N, desc->quad_N + K, order.begin());
+    for (int64_t e = 0; e < K; ++e) {
+        const int N = order[e];
+        if (!order_supported(N))
+            return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: element order outside the compiled range 1..16");
+        if (N > tables->t.N_max) return fail(NDG_ERR_INVALID, "ndg_mesh_create: element order exceeds the tables' N_max");
+        if (N < desc->initial_N)
+            return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: element order below initial_N (boundary ghosts would "
+                                             "carry a higher order than their element)");
     }
-    if (!order_supported(N)) return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: order N outside the compiled range 1..16");
-    if (N > tables->t.N_max) return fail(NDG_ERR_INVALID, "ndg_mesh_create: N exceeds the tables' N_max");
-
     for (int64_t e = 0; e < 4 * K; ++e)
         if (desc->quads[e] < 0 || desc->quads[e] >= NN) return fail(NDG_ERR_INVALID, "ndg_mesh_create: quad node id out of range");
     for (int64_t b = 0; b < 2 * B; ++b)
         if (desc->bars[b] < 0 || desc->bars[b] >= NN) return fail(NDG_ERR_INVALID, "ndg_mesh_create: bar node id out of range");
 
     auto P = [&](int64_t node) { return v2{desc->xy[2 * node], desc->xy[2 * node + 1]}; };
+    auto corner = [&](int64_t e, int k) { return P(desc->quads[4 * e + (k & 3)]); };
 
     // Directed edges of the domain elements (build_faces, src/meshes/Mesh2D_t.cpp:1110-1154).
     std::unordered_map<uint64_t, int64_t> edges;
@@ -206,14 +257,17 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
             if (!edges.emplace(edge_key(a, b), e * 4 + s).second)
                 return fail(NDG_ERR_INVALID, "ndg_mesh_create: an edge is used twice with the same orientation");
         }
-    std::vector<int32_t> side_nb((size_t)(4 * K), INT32_MIN);
+    // link[e*4+s]: the element side whose trace is the exterior state of side s of element e (a
+    // neighbour, or the donor side of a periodic pair), or -(1 + boundary record) / INT64_MIN.
+    std::vector<int64_t> link((size_t)(4 * K), INT64_MIN);
+    std::vector<char> periodic_link((size_t)(4 * K), 0);
     int64_t n_faces = 0;
     for (int64_t e = 0; e < K; ++e)
         for (int s = 0; s < 4; ++s) {
             const int64_t a = desc->quads[4 * e + s], b = desc->quads[4 * e + (s + 1) % 4];
             auto it = edges.find(edge_key(b, a));
             if (it != edges.end()) {
-                side_nb[4 * e + s] = (int32_t)it->second;
+                link[4 * e + s] = it->second;
                 if (it->second > e * 4 + s) ++n_faces;
             }
         }
@@ -232,7 +286,7 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
     std::vector<int64_t> side_bar((size_t)(4 * K), -1);
     for (int64_t b = 0; b < B; ++b) {
         const int64_t es = bar_side[b];
-        if (side_nb[es] != INT32_MIN || side_bar[es] != -1)
+        if (link[es] != INT64_MIN || side_bar[es] != -1)
             return fail(NDG_ERR_INVALID, "ndg_mesh_create: a boundary bar lies on an interior or already bounded edge");
         side_bar[es] = b;
     }
@@ -246,45 +300,34 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
             const int64_t d = desc->bar_donor[b];
             if (d < 0 || d >= B) return fail(NDG_ERR_INVALID, "ndg_mesh_create: periodic donor out of range");
             // local_interfaces (Mesh2D_t.cpp:4712-4730): the ghost carries the donor side's trace
-            // unchanged, so the side couples directly to the donor's element side.
-            side_nb[es] = (int32_t)bar_side[d];
+            // unchanged (and takes the donor element's order), so the side couples to the donor
+            // element side directly.
+            link[es] = bar_side[d];
+            periodic_link[es] = 1;
             continue;
         }
         if (kind < NDG_BC_NONE || kind > NDG_BC_OUTFLOW) return fail(NDG_ERR_INVALID, "ndg_mesh_create: unknown boundary kind");
         BcSide rec{};
         rec.kind = kind;
+        rec.slot = -1;
         // ghost nodes {a, b, b, a} with a = P(s+1), b = P(s) after face attachment
         const v2 ga = P(u1), gb = P(u0);
         rec.ax = ga.x; rec.ay = ga.y; rec.bx = gb.x; rec.by = gb.y;
         const int64_t bar_a = desc->bars[2 * b];
-        v2 G0, G1, G2, G3;
+        v2 G[4];
         if (bar_a == u1) {  // bar already reversed with respect to the element edge
-            G0 = ga; G1 = gb; G2 = gb; G3 = ga;
+            G[0] = ga; G[1] = gb; G[2] = gb; G[3] = ga;
         } else {            // "inside out" bar: only its first two nodes are swapped
-            G0 = ga; G1 = gb; G2 = ga; G3 = gb;
+            G[0] = ga; G[1] = gb; G[2] = ga; G[3] = gb;
         }
-        rec.sf = ghost_scale(G0, G1, G2, G3);
-        face_frame(P(u0), P(u1), P(u0), P(u1), rec);
-        side_nb[es] = -(int32_t)(1 + bcs.size());
+        rec.sf = side_scale(G, 0);
+        face_frame(P(u0), P(u1), P(u0), P(u1), rec.nx, rec.ny, rec.tx, rec.ty);
+        link[es] = -(int64_t)(1 + bcs.size());
         bcs.push_back(rec);
     }
     for (int64_t es = 0; es < 4 * K; ++es)
-        if (side_nb[es] == INT32_MIN)
+        if (link[es] == INT64_MIN)
             return fail(NDG_ERR_INVALID, "ndg_mesh_create: element side has neither a neighbour nor a boundary bar");
-
-    // The ghost projection of a conforming boundary face (src/meshes/Mesh2D_t.cpp:4383-4418) must
-    // reduce to the reversed copy: check the exact-node shortcut and symmetric weights.
-    const auto &x = tables->t.tab[NDG_TABLE_NODES][N];
-    const auto &w = tables->t.tab[NDG_TABLE_WEIGHTS][N];
-    for (int j = 0; j <= N; ++j) {
-        const double coord = (x[N - j] + 1) * 1.0 + 2 * 0.0 - 1;
-        int hit = -1;
-        for (int i = 0; i <= N; ++i)
-            if (ulp2_equal(coord, x[i])) { hit = i; break; }
-        if (hit != N - j || w[j] / w[N - j] != 1.0)
-            return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: boundary ghost projection is not a plain reversed copy "
-                                             "for this order");
-    }
 
     int32_t ndev = 0;
     ndg_device_count(&ndev);
@@ -295,34 +338,160 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
     m->tol_min = desc->tolerance_min;
     m->tol_max = desc->tolerance_max;
     m->n_faces = n_faces;
+    m->order = order;
+    DeviceMesh &d = m->d;
+    d.K = K;
+
+    // Order groups: elements sorted by N (stable, so each group keeps the caller's order, e.g. a
+    // space-filling curve).
+    std::vector<int64_t> perm((size_t)K);
+    for (int64_t e = 0; e < K; ++e) perm[e] = e;
+    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return order[a] < order[b]; });
+    m->dev_of.assign((size_t)K, 0);
+    for (int64_t k = 0; k < K; ++k) m->dev_of[perm[k]] = k;
+    std::vector<int> group_of_N(kMaxOrder + 1, -1);
+    for (int64_t k = 0; k < K; ++k) {
+        const int N = order[perm[k]];
+        if (group_of_N[N] < 0) {
+            Group G;
+            G.N = N;
+            G.e_begin = k;
+            if (!d.groups.empty()) {
+                const Group &H = d.groups.back();
+                G.q_off = H.q_off + 3 * (int64_t)(H.N + 1) * (H.N + 1) * H.K;
+                G.tr_off = H.tr_off + 12 * (int64_t)(H.N + 1) * H.K;
+            }
+            group_of_N[N] = (int)d.groups.size();
+            d.groups.push_back(G);
+        }
+        ++d.groups[group_of_N[N]].K;
+    }
+    {
+        const Group &H = d.groups.back();
+        d.n_q = H.q_off + 3 * (int64_t)(H.N + 1) * (H.N + 1) * H.K;
+        d.n_tr = H.tr_off + 12 * (int64_t)(H.N + 1) * H.K;
+    }
+    d.N_min = d.groups.front().N;
+    d.N_max = d.groups.back().N;
+    for (const Group &G : d.groups)
+        if (!ghost_copy_is_close(tables->t, G.N))
+            return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: boundary ghost projection is not a reversed copy for "
+                                             "this order");
+    m->host_off.assign((size_t)K, 0);
+    int64_t acc = 0;
+    for (int64_t e = 0; e < K; ++e) {
+        m->host_off[e] = acc;
+        acc += (int64_t)(order[e] + 1) * (order[e] + 1);
+    }
+    m->n_dofs = acc;
+    auto group_of = [&](int64_t e) -> const Group & { return d.groups[group_of_N[order[e]]]; };
+    auto local_of = [&](int64_t e) { return m->dev_of[e] - group_of(e).e_begin; };
+    auto trace_at = [&](int64_t e, int s) {
+        const int64_t nf = order[e] + 1;
+        return group_of(e).tr_off + local_of(e) * 12 * nf + s * 3 * nf;
+    };
+
+    // Device connectivity. Sides whose exterior state has another order are non-conforming: they
+    // get a mortar record, and one mortar face per pair (two-sided for a neighbour, one-sided for
+    // a periodic donor) fills their flux.
+    std::vector<int32_t> side_nb((size_t)(4 * K));
+    std::vector<MortarFace> faces;
+    std::vector<int64_t> mortar_rec((size_t)(4 * K), -1);
+    int64_t n_slot = 0;
+    auto add_mortar_side = [&](int64_t es) {
+        BcSide rec{};
+        rec.kind = kKindMortar;
+        rec.slot = n_slot;
+        n_slot += 3 * (int64_t)(order[es / 4] + 1);
+        mortar_rec[es] = (int64_t)bcs.size();
+        bcs.push_back(rec);
+        return rec.slot;
+    };
+    for (int64_t e = 0; e < K; ++e)
+        for (int s = 0; s < 4; ++s) {
+            const int64_t es = 4 * e + s, l = link[es];
+            if (l < 0 || order[l / 4] == order[e]) continue;
+            const int64_t r = l / 4;
+            const int rs = (int)(l % 4);
+            const bool periodic = periodic_link[es] != 0;
+            if (!periodic && r < e) continue;  // the face belongs to its first element
+            MortarFace fc{};
+            fc.NL = order[e];
+            fc.NR = order[r];
+            fc.Nf = std::max(fc.NL, fc.NR);
+            fc.trL = trace_at(e, s);
+            fc.trR = trace_at(r, rs);
+            const v2 f0 = corner(e, s), f1 = corner(e, s + 1);
+            double tx, ty;
+            face_frame(f0, f1, f0, f1, fc.nx, fc.ny, tx, ty);
+            v2 en[4] = {f0, f1, f1, f0};
+            if (!periodic) {
+                en[2] = corner(r, rs);
+                en[3] = corner(r, rs + 1);
+            }
+            face_offsets(f0, f1, en, fc.offset, fc.scale);
+            v2 PL[4] = {corner(e, 0), corner(e, 1), corner(e, 2), corner(e, 3)};
+            fc.sfL = side_scale(PL, s);
+            fc.slotL = add_mortar_side(es);
+            fc.slotR = -1;
+            if (!periodic) {
+                v2 PR[4] = {corner(r, 0), corner(r, 1), corner(r, 2), corner(r, 3)};
+                fc.sfR = side_scale(PR, rs);
+                fc.slotR = add_mortar_side(l);
+            }
+            faces.push_back(fc);
+        }
+    for (int64_t e = 0; e < K; ++e)
+        for (int s = 0; s < 4; ++s) {
+            const int64_t es = 4 * e + s, l = link[es];
+            int32_t code;
+            if (mortar_rec[es] >= 0) code = -(int32_t)(1 + mortar_rec[es]);
+            else if (l < 0) code = (int32_t)l;
+            else code = (int32_t)(local_of(l / 4) * 4 + l % 4);
+            side_nb[4 * m->dev_of[e] + s] = code;
+        }
+
     HIP_TRY(hipStreamCreateWithFlags(&m->own_stream, hipStreamNonBlocking), "hipStreamCreate");
     m->stream = m->own_stream;
-    DeviceMesh &d = m->d;
-    d.N = N;
-    d.K = K;
     std::vector<double> verts((size_t)(8 * K));
     for (int64_t e = 0; e < K; ++e)
         for (int k = 0; k < 4; ++k) {
-            const v2 p = P(desc->quads[4 * e + k]);
-            verts[8 * e + 2 * k] = p.x;
-            verts[8 * e + 2 * k + 1] = p.y;
+            const v2 p = corner(e, k);
+            verts[8 * m->dev_of[e] + 2 * k] = p.x;
+            verts[8 * m->dev_of[e] + 2 * k + 1] = p.y;
         }
     ndg_status st;
     if ((st = device_upload(&d.verts, verts, "upload vertices")) != NDG_OK) return st;
     if ((st = device_upload(&d.side_nb, side_nb, "upload connectivity")) != NDG_OK) return st;
     if ((st = device_upload(&d.bc, bcs, "upload boundary records")) != NDG_OK) return st;
     d.n_bc = (int64_t)bcs.size();
-    if ((st = device_upload(&d.tab, pack_tables(tables->t, N), "upload tables")) != NDG_OK) return st;
-    const int64_t nf = N + 1, ne = nf * nf;
-    HIP_TRY(hipMalloc((void **)&d.q, sizeof(double) * 3 * ne * K), "hipMalloc q");
-    HIP_TRY(hipMalloc((void **)&d.g, sizeof(double) * 3 * ne * K), "hipMalloc g");
-    HIP_TRY(hipMalloc((void **)&d.trace[0], sizeof(double) * 12 * nf * K), "hipMalloc traces");
-    HIP_TRY(hipMalloc((void **)&d.trace[1], sizeof(double) * 12 * nf * K), "hipMalloc traces");
+    if ((st = device_upload(&d.faces, faces, "upload mortar faces")) != NDG_OK) return st;
+    d.n_faces = (int64_t)faces.size();
+    d.n_side_flux = n_slot;
+    if (n_slot > 0) HIP_TRY(hipMalloc((void **)&d.side_flux, sizeof(double) * n_slot), "hipMalloc side fluxes");
+    if (!faces.empty()) {
+        std::vector<double> t1((size_t)(3 * (kMaxOrder + 1) * kTab1dStride), 0.0);
+        for (int n = 0; n <= std::min(kMaxOrder, tables->t.N_max); ++n)
+            for (int i = 0; i <= n; ++i) {
+                t1[(3 * n + 0) * kTab1dStride + i] = tables->t.tab[NDG_TABLE_NODES][n][i];
+                t1[(3 * n + 1) * kTab1dStride + i] = tables->t.tab[NDG_TABLE_WEIGHTS][n][i];
+                t1[(3 * n + 2) * kTab1dStride + i] = tables->t.tab[NDG_TABLE_BARYCENTRIC_WEIGHTS][n][i];
+            }
+        if ((st = device_upload(&d.tab1d, t1, "upload 1D tables")) != NDG_OK) return st;
+    }
+    for (Group &G : d.groups)
+        if ((st = device_upload(&G.tab, pack_tables(tables->t, G.N), "upload tables")) != NDG_OK) return st;
+    HIP_TRY(hipMalloc((void **)&d.q, sizeof(double) * d.n_q), "hipMalloc q");
+    HIP_TRY(hipMalloc((void **)&d.g, sizeof(double) * d.n_q), "hipMalloc g");
+    HIP_TRY(hipMalloc((void **)&d.trace[0], sizeof(double) * d.n_tr), "hipMalloc traces");
+    HIP_TRY(hipMalloc((void **)&d.trace[1], sizeof(double) * d.n_tr), "hipMalloc traces");
     d.scratch_len = 1024;
     HIP_TRY(hipMalloc((void **)&d.scratch, sizeof(double) * d.scratch_len), "hipMalloc scratch");
     HIP_TRY(hipMalloc((void **)&m->dt_dev, sizeof(double)), "hipMalloc dt");
-    HIP_TRY(hipMemsetAsync(d.q, 0, sizeof(double) * 3 * ne * K, m->stream), "hipMemset q");
-    HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * 3 * ne * K, m->stream), "hipMemset g");
+    HIP_TRY(hipMalloc((void **)&d.clock, sizeof(double) * 4), "hipMalloc clock");
+    HIP_TRY(hipMemsetAsync(d.clock, 0, sizeof(double) * 4, m->stream), "hipMemset clock");
+    HIP_TRY(hipMemsetAsync(d.q, 0, sizeof(double) * d.n_q, m->stream), "hipMemset q");
+    HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * d.n_q, m->stream), "hipMemset g");
     HIP_TRY(launch_traces(d, m->stream), "traces kernel");
     HIP_TRY(hipStreamSynchronize(m->stream), "mesh create sync");
     *out = m.release();
@@ -345,12 +514,11 @@ extern "C" ndg_status ndg_mesh_set_stream(ndg_mesh *mesh, void *hip_stream) {
 extern "C" ndg_status ndg_mesh_info(const ndg_mesh *mesh, int64_t *n_elements, int64_t *n_faces,
                                     int64_t *n_dofs_per_field, int32_t *N_min, int32_t *N_max) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_mesh_info: null mesh");
-    const int64_t ne = (int64_t)(mesh->d.N + 1) * (mesh->d.N + 1);
     if (n_elements) *n_elements = mesh->d.K;
     if (n_faces) *n_faces = mesh->n_faces;
-    if (n_dofs_per_field) *n_dofs_per_field = ne * mesh->d.K;
-    if (N_min) *N_min = mesh->d.N;
-    if (N_max) *N_max = mesh->d.N;
+    if (n_dofs_per_field) *n_dofs_per_field = mesh->n_dofs;
+    if (N_min) *N_min = mesh->d.N_min;
+    if (N_max) *N_max = mesh->d.N_max;
     return NDG_OK;
 }
 
@@ -359,16 +527,32 @@ extern "C" ndg_status ndg_mesh_info(const ndg_mesh *mesh, int64_t *n_elements, i
 
 namespace {
 
-// device [K][3][Ne] <-> packed per-field host arrays
-ndg_status copy_out3(ndg_mesh *m, const double *dev, double *a, double *b, double *c, const char *what) {
-    const int64_t ne = (int64_t)(m->d.N + 1) * (m->d.N + 1), K = m->d.K;
-    m->staging.resize((size_t)(3 * ne * K));
-    HIP_TRY(hipMemcpyAsync(m->staging.data(), dev, sizeof(double) * 3 * ne * K, hipMemcpyDeviceToHost, m->stream), what);
-    HIP_TRY(hipStreamSynchronize(m->stream), what);
-    double *outs[3] = {a, b, c};
-    for (int64_t e = 0; e < K; ++e)
-        for (int f = 0; f < 3; ++f)
-            if (outs[f]) std::memcpy(outs[f] + e * ne, m->staging.data() + (e * 3 + f) * ne, sizeof(double) * ne);
+// Device layout [element (device order)][3][(N+1)^2] <-> packed per-field host arrays in the
+// caller's element order. download: device -> host, else host -> device.
+ndg_status copy_state(ndg_mesh *m, double *dev, double *const fields[3], bool download, const char *what) {
+    const DeviceMesh &d = m->d;
+    m->staging.resize((size_t)d.n_q);
+    if (download) {
+        HIP_TRY(hipMemcpyAsync(m->staging.data(), dev, sizeof(double) * d.n_q, hipMemcpyDeviceToHost, m->stream), what);
+        HIP_TRY(hipStreamSynchronize(m->stream), what);
+    }
+    for (int64_t e = 0; e < d.K; ++e) {
+        const int N = m->order[e];
+        const int64_t ne = (int64_t)(N + 1) * (N + 1);
+        const Group *G = nullptr;
+        for (const Group &H : d.groups)
+            if (H.N == N) { G = &H; break; }
+        double *blk = m->staging.data() + G->q_off + (m->dev_of[e] - G->e_begin) * 3 * ne;
+        for (int f = 0; f < 3; ++f) {
+            if (!fields[f]) continue;
+            if (download) std::memcpy(fields[f] + m->host_off[e], blk + f * ne, sizeof(double) * ne);
+            else std::memcpy(blk + f * ne, fields[f] + m->host_off[e], sizeof(double) * ne);
+        }
+    }
+    if (!download) {
+        HIP_TRY(hipMemcpyAsync(dev, m->staging.data(), sizeof(double) * d.n_q, hipMemcpyHostToDevice, m->stream), what);
+        HIP_TRY(hipStreamSynchronize(m->stream), what);
+    }
     return NDG_OK;
 }
 
@@ -397,17 +581,20 @@ extern "C" ndg_status ndg_mesh_set_precision(ndg_mesh *mesh, int32_t precision) 
     DeviceMesh &d = mesh->d;
     if (precision == d.precision) return NDG_OK;
     if (precision == NDG_PRECISION_FP32_COMPENSATED) {
-        const int64_t nf = d.N + 1, ne = nf * nf;
         if (!d.qh) {
-            HIP_TRY(hipMalloc((void **)&d.qh, sizeof(float) * 3 * ne * d.K), "hipMalloc qh");
-            HIP_TRY(hipMalloc((void **)&d.ql, sizeof(float) * 3 * ne * d.K), "hipMalloc ql");
-            HIP_TRY(hipMalloc((void **)&d.g32, sizeof(float) * 3 * ne * d.K), "hipMalloc g32");
-            HIP_TRY(hipMalloc((void **)&d.trace32[0], sizeof(float) * 12 * nf * d.K), "hipMalloc traces32");
-            HIP_TRY(hipMalloc((void **)&d.trace32[1], sizeof(float) * 12 * nf * d.K), "hipMalloc traces32");
-            const std::vector<double> t64 = pack_tables(mesh->tables->t, d.N);
-            std::vector<float> t32(t64.begin(), t64.end());
-            HIP_TRY(hipMalloc((void **)&d.tab32, sizeof(float) * t32.size()), "hipMalloc tab32");
-            HIP_TRY(hipMemcpy(d.tab32, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice), "upload tab32");
+            HIP_TRY(hipMalloc((void **)&d.qh, sizeof(float) * d.n_q), "hipMalloc qh");
+            HIP_TRY(hipMalloc((void **)&d.ql, sizeof(float) * d.n_q), "hipMalloc ql");
+            HIP_TRY(hipMalloc((void **)&d.g32, sizeof(float) * d.n_q), "hipMalloc g32");
+            HIP_TRY(hipMalloc((void **)&d.trace32[0], sizeof(float) * d.n_tr), "hipMalloc traces32");
+            HIP_TRY(hipMalloc((void **)&d.trace32[1], sizeof(float) * d.n_tr), "hipMalloc traces32");
+            if (d.n_side_flux > 0)
+                HIP_TRY(hipMalloc((void **)&d.side_flux32, sizeof(float) * d.n_side_flux), "hipMalloc side fluxes");
+            for (Group &G : d.groups) {
+                const std::vector<double> t64 = pack_tables(mesh->tables->t, G.N);
+                std::vector<float> t32(t64.begin(), t64.end());
+                HIP_TRY(hipMalloc((void **)&G.tab32, sizeof(float) * t32.size()), "hipMalloc tab32");
+                HIP_TRY(hipMemcpy(G.tab32, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice), "upload tab32");
+            }
         }
         d.precision = NDG_PRECISION_FP32_COMPENSATED;
         ndg_status st = refresh_c32(mesh);
@@ -437,9 +624,8 @@ extern "C" ndg_status ndg_set_ic(ndg_mesh *mesh, const ndg_ic_desc *ic) {
         HIP_TRY(hipMalloc((void **)&mesh->pulses_dev, sizeof(double) * 4 * np), "hipMalloc pulses");
         HIP_TRY(hipMemcpy(mesh->pulses_dev, ic->pulses, sizeof(double) * 4 * np, hipMemcpyHostToDevice), "upload pulses");
     }
-    const int64_t ne = (int64_t)(d.N + 1) * (d.N + 1);
     HIP_TRY(launch_ic(d, ic->kind, np, mesh->pulses_dev, mesh->stream), "ic kernel");
-    HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * 3 * ne * d.K, mesh->stream), "hipMemset g");
+    HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * d.n_q, mesh->stream), "hipMemset g");
     HIP_TRY(launch_traces(d, mesh->stream), "traces kernel");
     ndg_status st = refresh_c32(mesh);
     if (st != NDG_OK) return st;
@@ -450,17 +636,13 @@ extern "C" ndg_status ndg_set_ic(ndg_mesh *mesh, const ndg_ic_desc *ic) {
 extern "C" ndg_status ndg_set_state(ndg_mesh *mesh, const double *p, const double *u, const double *v) {
     if (!mesh || !p || !u || !v) return fail(NDG_ERR_INVALID, "ndg_set_state: null argument");
     DeviceMesh &d = mesh->d;
-    const int64_t ne = (int64_t)(d.N + 1) * (d.N + 1), K = d.K;
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_set_state sync");
-    mesh->staging.resize((size_t)(3 * ne * K));
-    const double *ins[3] = {p, u, v};
-    for (int64_t e = 0; e < K; ++e)
-        for (int f = 0; f < 3; ++f) std::memcpy(mesh->staging.data() + (e * 3 + f) * ne, ins[f] + e * ne, sizeof(double) * ne);
-    HIP_TRY(hipMemcpyAsync(d.q, mesh->staging.data(), sizeof(double) * 3 * ne * K, hipMemcpyHostToDevice, mesh->stream), "upload state");
-    HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * 3 * ne * K, mesh->stream), "hipMemset g");
-    HIP_TRY(launch_traces(d, mesh->stream), "traces kernel");
-    ndg_status st = refresh_c32(mesh);
+    double *const ins[3] = {const_cast<double *>(p), const_cast<double *>(u), const_cast<double *>(v)};
+    ndg_status st = copy_state(mesh, d.q, ins, false, "upload state");
     if (st != NDG_OK) return st;
+    HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * d.n_q, mesh->stream), "hipMemset g");
+    HIP_TRY(launch_traces(d, mesh->stream), "traces kernel");
+    if ((st = refresh_c32(mesh)) != NDG_OK) return st;
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_set_state sync");
     return NDG_OK;
 }
@@ -469,7 +651,8 @@ extern "C" ndg_status ndg_get_state(ndg_mesh *mesh, double *p, double *u, double
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_get_state: null mesh");
     ndg_status st = refresh_fp64(mesh);
     if (st != NDG_OK) return st;
-    return copy_out3(mesh, mesh->d.q, p, u, v, "download state");
+    double *const outs[3] = {p, u, v};
+    return copy_state(mesh, mesh->d.q, outs, true, "download state");
 }
 
 extern "C" ndg_status ndg_get_traces(ndg_mesh *mesh, double *out) {
@@ -477,9 +660,22 @@ extern "C" ndg_status ndg_get_traces(ndg_mesh *mesh, double *out) {
     ndg_status st = refresh_fp64(mesh);
     if (st != NDG_OK) return st;
     const DeviceMesh &d = mesh->d;
-    const int64_t n = 12 * (int64_t)(d.N + 1) * d.K;
-    HIP_TRY(hipMemcpyAsync(out, d.trace[d.cur], sizeof(double) * n, hipMemcpyDeviceToHost, mesh->stream), "download traces");
+    mesh->staging.resize((size_t)d.n_tr);
+    HIP_TRY(hipMemcpyAsync(mesh->staging.data(), d.trace[d.cur], sizeof(double) * d.n_tr, hipMemcpyDeviceToHost,
+                           mesh->stream), "download traces");
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_get_traces sync");
+    int64_t o = 0;
+    for (int64_t e = 0; e < d.K; ++e) {
+        const int N = mesh->order[e];
+        const int64_t len = 12 * (int64_t)(N + 1);
+        for (const Group &G : d.groups)
+            if (G.N == N) {
+                std::memcpy(out + o, mesh->staging.data() + G.tr_off + (mesh->dev_of[e] - G.e_begin) * len,
+                            sizeof(double) * len);
+                break;
+            }
+        o += len;
+    }
     return NDG_OK;
 }
 
@@ -488,10 +684,10 @@ extern "C" ndg_status ndg_compute_rhs(ndg_mesh *mesh, double t, double *Gp, doub
     ndg_status st = refresh_fp64(mesh);
     if (st != NDG_OK) return st;
     DeviceMesh &d = mesh->d;
-    const int64_t ne = (int64_t)(d.N + 1) * (d.N + 1);
-    if (!d.rhs) HIP_TRY(hipMalloc((void **)&d.rhs, sizeof(double) * 3 * ne * d.K), "hipMalloc rhs");
+    if (!d.rhs) HIP_TRY(hipMalloc((void **)&d.rhs, sizeof(double) * d.n_q), "hipMalloc rhs");
     HIP_TRY(launch_rhs(d, t, mesh->stream), "rhs kernel");
-    return copy_out3(mesh, d.rhs, Gp, Gu, Gv, "download rhs");
+    double *const outs[3] = {Gp, Gu, Gv};
+    return copy_state(mesh, d.rhs, outs, true, "download rhs");
 }
 
 extern "C" ndg_status ndg_get_delta_t(ndg_mesh *mesh, double cfl, double *dt) {
@@ -503,13 +699,191 @@ extern "C" ndg_status ndg_get_delta_t(ndg_mesh *mesh, double cfl, double *dt) {
     return NDG_OK;
 }
 
+namespace {
+
+void drop_graphs(ndg_mesh *m) {
+    for (hipGraphExec_t &x : m->step_exec)
+        if (x) {
+            (void)hipGraphExecDestroy(x);
+            x = nullptr;
+        }
+    m->graph_precision = -1;
+}
+
+// One RK step from the device clock (clock_begin with set = 0), launched on s; with `parallel`
+// the order groups of each stage are forked onto side streams and joined (graph branches).
+ndg_status launch_step_body(ndg_mesh *m, double dt, bool parallel, hipStream_t s) {
+    DeviceMesh &d = m->d;
+    HIP_TRY(launch_clock_begin(d, 0.0, 0, dt, s), "clock kernel");
+    const int ng = (int)d.groups.size();
+    for (int st = 0; st < 3; ++st) {
+        HIP_TRY(launch_mortar(d, s), "mortar kernel");
+        if (parallel && ng > 1) {
+            HIP_TRY(hipEventRecord(m->side_events[0], s), "fork");
+            for (int g = 0; g < ng; ++g) {
+                HIP_TRY(hipStreamWaitEvent(m->side_streams[g], m->side_events[0], 0), "fork wait");
+                HIP_TRY(launch_group_stage(d, g, st, kAm[st], kGm[st] * dt, m->side_streams[g]), "stage kernel");
+                HIP_TRY(hipEventRecord(m->side_events[1 + g], m->side_streams[g]), "join");
+                HIP_TRY(hipStreamWaitEvent(s, m->side_events[1 + g], 0), "join wait");
+            }
+        } else {
+            for (int g = 0; g < ng; ++g)
+                HIP_TRY(launch_group_stage(d, g, st, kAm[st], kGm[st] * dt, s), "stage kernel");
+        }
+        d.cur = 1 - d.cur;
+    }
+    HIP_TRY(launch_clock_end(d, dt, s), "clock kernel");
+    return NDG_OK;
+}
+
+ndg_status ensure_side_streams(ndg_mesh *m) {
+    const size_t ng = m->d.groups.size();
+    while (m->side_streams.size() < ng) {
+        hipStream_t st;
+        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        m->side_streams.push_back(st);
+    }
+    while (m->side_events.size() < ng + 1) {
+        hipEvent_t ev;
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+        m->side_events.push_back(ev);
+    }
+    return NDG_OK;
+}
+
+// Captures the step graphs for dt (both starting trace buffers); leaves `cur` as it was.
+ndg_status capture_steps(ndg_mesh *m, double dt) {
+    drop_graphs(m);
+    const bool parallel = m->graph_mode == NDG_GRAPH_PARALLEL;
+    if (parallel) {
+        ndg_status st = ensure_side_streams(m);
+        if (st != NDG_OK) return st;
+    }
+    DeviceMesh &d = m->d;
+    const int cur0 = d.cur;
+    for (int k = 0; k < 2; ++k) {
+        d.cur = k;
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        ndg_status st = launch_step_body(m, dt, parallel, m->stream);
+        hipGraph_t captured = nullptr;
+        const hipError_t e = hipStreamEndCapture(m->stream, &captured);
+        graph = captured;
+        if (st != NDG_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            d.cur = cur0;
+            return st;
+        }
+        if (e != hipSuccess) {
+            d.cur = cur0;
+            return hip_fail(e, "hipStreamEndCapture");
+        }
+        const hipError_t ei = hipGraphInstantiate(&m->step_exec[k], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            d.cur = cur0;
+            return hip_fail(ei, "hipGraphInstantiate");
+        }
+    }
+    d.cur = cur0;
+    m->graph_dt = dt;
+    m->graph_precision = d.precision;
+    return NDG_OK;
+}
+
+// One step starting from the device clock: a graph replay when graphs are on, else direct
+// launches.
+ndg_status step_from_clock(ndg_mesh *m, double dt) {
+    DeviceMesh &d = m->d;
+    if (m->graph_mode == NDG_GRAPH_OFF) return launch_step_body(m, dt, false, m->stream);
+    if (!m->step_exec[0] || m->graph_dt != dt || m->graph_precision != d.precision) {
+        ndg_status st = capture_steps(m, dt);
+        if (st != NDG_OK) return st;
+    }
+    HIP_TRY(hipGraphLaunch(m->step_exec[d.cur], m->stream), "hipGraphLaunch");
+    d.cur = 1 - d.cur;  // three stages flip the trace buffer three times
+    return NDG_OK;
+}
+
+}  // namespace
+
 extern "C" ndg_status ndg_step(ndg_mesh *mesh, double t, double dt) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_step: null mesh");
-    const bool c32 = mesh->d.precision == NDG_PRECISION_FP32_COMPENSATED;
-    for (int s = 0; s < 3; ++s) {
-        const double ts = t + kBm[s] * dt;
-        if (c32) HIP_TRY(launch_stage_c32(mesh->d, s, ts, kAm[s], kGm[s] * dt, mesh->stream), "stage_c32 kernel");
-        else HIP_TRY(launch_stage(mesh->d, s, ts, kAm[s], kGm[s] * dt, mesh->stream), "stage kernel");
+    HIP_TRY(launch_clock_begin(mesh->d, t, 1, dt, mesh->stream), "clock kernel");
+    return step_from_clock(mesh, dt);
+}
+
+extern "C" ndg_status ndg_mesh_set_graph(ndg_mesh *mesh, int32_t mode) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_mesh_set_graph: null mesh");
+    if (mode != NDG_GRAPH_OFF && mode != NDG_GRAPH_SEQUENTIAL && mode != NDG_GRAPH_PARALLEL)
+        return fail(NDG_ERR_INVALID, "ndg_mesh_set_graph: unknown mode");
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_mesh_set_graph sync");
+    drop_graphs(mesh);
+    mesh->graph_mode = mode;
+    return NDG_OK;
+}
+
+extern "C" ndg_status ndg_mesh_autotune(ndg_mesh *mesh, int32_t repeats) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_mesh_autotune: null mesh");
+    if (repeats < 1) return fail(NDG_ERR_INVALID, "ndg_mesh_autotune: repeats must be >= 1");
+    DeviceMesh &d = mesh->d;
+    const int prec = d.precision;
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_mesh_autotune sync");
+    drop_graphs(mesh);
+    // the probes read the mortar side fluxes of the current traces
+    HIP_TRY(launch_mortar(d, mesh->stream), "mortar kernel");
+    hipEvent_t ev[2];
+    HIP_TRY(hipEventCreate(&ev[0]), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&ev[1]), "hipEventCreate");
+    ndg_status status = NDG_OK;
+    for (int gi = 0; gi < (int)d.groups.size() && status == NDG_OK; ++gi) {
+        Group &G = d.groups[gi];
+        const int emax = stage_max_epb(G.N);
+        std::vector<int> epbs;
+        for (int e : {emax, (emax + 1) / 2, (emax + 3) / 4, 1})
+            if (std::find(epbs.begin(), epbs.end(), e) == epbs.end()) epbs.push_back(e);
+        float best = INFINITY;
+        Group::Tune best_t = G.tune[prec];
+        for (int e : epbs)
+            for (int remap : {1, 0}) {
+                G.tune[prec].epb = e;
+                G.tune[prec].remap = remap;
+                hipError_t he = launch_group_probe(d, gi, mesh->stream);
+                if (he == hipSuccess) he = hipEventRecord(ev[0], mesh->stream);
+                for (int r = 0; r < repeats && he == hipSuccess; ++r) he = launch_group_probe(d, gi, mesh->stream);
+                if (he == hipSuccess) he = hipEventRecord(ev[1], mesh->stream);
+                if (he == hipSuccess) he = hipEventSynchronize(ev[1]);
+                float ms = 0.0f;
+                if (he == hipSuccess) he = hipEventElapsedTime(&ms, ev[0], ev[1]);
+                if (he != hipSuccess) {
+                    status = hip_fail(he, "autotune probe");
+                    break;
+                }
+                if (ms < best) {
+                    best = ms;
+                    best_t = G.tune[prec];
+                }
+            }
+        G.tune[prec] = best_t;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    return status;
+}
+
+extern "C" ndg_status ndg_mesh_tuning(const ndg_mesh *mesh, int32_t *orders, int32_t *epb, int32_t *remap,
+                                      int32_t capacity, int32_t *n_groups) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_mesh_tuning: null mesh");
+    const DeviceMesh &d = mesh->d;
+    if (n_groups) *n_groups = (int32_t)d.groups.size();
+    if (capacity < (int32_t)d.groups.size() && (orders || epb || remap))
+        return fail(NDG_ERR_INVALID, "ndg_mesh_tuning: output too small");
+    for (size_t g = 0; g < d.groups.size(); ++g) {
+        const Group &G = d.groups[g];
+        const int e = G.tune[d.precision].epb;
+        if (orders) orders[g] = G.N;
+        if (epb) epb[g] = e >= 1 ? e : stage_max_epb(G.N);
+        if (remap) remap[g] = G.tune[d.precision].remap;
     }
     return NDG_OK;
 }
@@ -521,11 +895,16 @@ extern "C" ndg_status ndg_solve(ndg_mesh *mesh, double cfl, double t_end, double
     if (st != NDG_OK) return st;
     double time = 0.0;
     int64_t steps = 0;
+    HIP_TRY(launch_clock_begin(mesh->d, time, 1, dt0, mesh->stream), "clock kernel");
     while (time < t_end) {
         ++steps;
         double dt = dt0;
-        if (time + dt > t_end) dt = t_end - time;
-        if ((st = ndg_step(mesh, time, dt)) != NDG_OK) return st;
+        if (time + dt > t_end) {
+            dt = t_end - time;
+            if ((st = ndg_step(mesh, time, dt)) != NDG_OK) return st;
+        } else if ((st = step_from_clock(mesh, dt)) != NDG_OK) {
+            return st;
+        }
         time += dt;
     }
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_solve sync");
@@ -541,8 +920,9 @@ extern "C" ndg_status ndg_run_steps(ndg_mesh *mesh, double cfl, int64_t n_steps,
     ndg_status st = ndg_get_delta_t(mesh, cfl, &dt);
     if (st != NDG_OK) return st;
     double time = time_io ? *time_io : 0.0;
+    HIP_TRY(launch_clock_begin(mesh->d, time, 1, dt, mesh->stream), "clock kernel");
     for (int64_t k = 0; k < n_steps; ++k) {
-        if ((st = ndg_step(mesh, time, dt)) != NDG_OK) return st;
+        if ((st = step_from_clock(mesh, dt)) != NDG_OK) return st;
         time += dt;
     }
     if (time_io) *time_io = time;
@@ -568,10 +948,11 @@ extern "C" ndg_status ndg_estimate_error(ndg_mesh *mesh, double *error, double *
             "download estimator");
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_estimate_error sync");
     for (int64_t e = 0; e < d.K; ++e) {
-        if (error) error[e] = host[4 * e];
-        if (sigma) sigma[e] = host[4 * e + 1];
-        if (refine) refine[e] = host[4 * e + 2] != 0.0;
-        if (coarsen) coarsen[e] = host[4 * e + 3] != 0.0;
+        const double *h = host.data() + 4 * mesh->dev_of[e];
+        if (error) error[e] = h[0];
+        if (sigma) sigma[e] = h[1];
+        if (refine) refine[e] = h[2] != 0.0;
+        if (coarsen) coarsen[e] = h[3] != 0.0;
     }
     return NDG_OK;
 }

@@ -132,6 +132,10 @@ class Mesh2D_t:
         self.n_faces_ = nf.value
         self.n_dofs_per_field_ = nd.value
         self.N_ = nmin.value
+        self.N_max_ = nmax.value
+        # element orders in the caller's element order
+        self.orders_ = (np.full(self.n_elements_, self.initial_N_, dtype=np.int32) if quad_N is None
+                        else quad_N.copy())
 
     @property
     def handle(self):
@@ -148,6 +152,25 @@ class Mesh2D_t:
         code = {"fp64": _abi.PRECISION_FP64, "fp32c": _abi.PRECISION_FP32_COMPENSATED}[precision]
         check(lib().ndg_mesh_set_precision(self._h, code))
         self.precision_ = precision
+
+    def set_graph(self, mode: str):
+        """'off' (direct launches), 'sequential' or 'parallel' (one captured hipGraph per RK step,
+        order groups in sequence or as concurrent branches)."""
+        code = {"off": _abi.GRAPH_OFF, "sequential": _abi.GRAPH_SEQUENTIAL, "parallel": _abi.GRAPH_PARALLEL}[mode]
+        check(lib().ndg_mesh_set_graph(self._h, code))
+
+    def autotune(self, repeats: int = 5):
+        """Times the launch shapes of every order group's stage kernel and keeps the fastest;
+        returns [(N, elements per workgroup, XCD block order)] per group."""
+        check(lib().ndg_mesh_autotune(self._h, int(repeats)))
+        return self.tuning()
+
+    def tuning(self):
+        n = ctypes.c_int32()
+        check(lib().ndg_mesh_tuning(self._h, None, None, None, 0, ctypes.byref(n)))
+        o, e, r = (np.empty(n.value, dtype=np.int32) for _ in range(3))
+        check(lib().ndg_mesh_tuning(self._h, ptr(o), ptr(e), ptr(r), n.value, ctypes.byref(n)))
+        return [(int(a), int(b), int(c)) for a, b, c in zip(o, e, r)]
 
     def set_stream(self, hip_stream_ptr: int):
         check(lib().ndg_mesh_set_stream(self._h, ctypes.c_void_p(hip_stream_ptr)))
@@ -177,10 +200,15 @@ class Mesh2D_t:
         return tuple(out)
 
     def interpolate_to_boundaries(self):
-        """Element traces of the current state, shape (K, 4, 3, N+1)."""
-        out = np.empty((self.n_elements_, 4, 3, self.N_ + 1), dtype=np.float64)
-        check(lib().ndg_get_traces(self._h, ptr(out)))
-        return out
+        """Element traces of the current state: shape (K, 4, 3, N+1) when every element has the
+        same order, else a list of K arrays of shape (4, 3, N_e+1)."""
+        nf = self.orders_.astype(np.int64) + 1
+        flat = np.empty(int(12 * nf.sum()), dtype=np.float64)
+        check(lib().ndg_get_traces(self._h, ptr(flat)))
+        if self.N_ == self.N_max_:
+            return flat.reshape(self.n_elements_, 4, 3, self.N_ + 1)
+        offs = np.concatenate([[0], np.cumsum(12 * nf)])
+        return [flat[offs[e]:offs[e + 1]].reshape(4, 3, nf[e]) for e in range(self.n_elements_)]
 
     def compute_rhs(self, t=0.0):
         out = [np.empty(self.n_dofs_per_field_, dtype=np.float64) for _ in range(3)]

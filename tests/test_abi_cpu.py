@@ -94,7 +94,8 @@ def test_invalid_input_is_a_status_not_an_exit():
         nd.NDG_t(-1, 5)
     ndg = nd.NDG_t(8, 5)
     desc = nd.square_mesh(4)
-    desc.quad_N = np.array([4] * 15 + [6], dtype=np.int32)
+    # an element below initial_N would meet a boundary ghost of higher order
+    desc.quad_N = np.array([4] * 15 + [3], dtype=np.int32)
     with pytest.raises(nd.NDGError) as e:
         nd.Mesh2D_t(desc, 4, 8, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
     assert e.value.code == _abi.NDG_ERR_UNSUPPORTED

@@ -1,0 +1,188 @@
+"""Meshes whose elements have different orders (p-nonconforming faces): the HIP path, called
+through the C ABI, against the oracle on the same inputs.
+
+Elements are grouped by order on the device and the faces between orders go through the mortar
+kernel (interpolation to the face order, upwind flux, L2 projection back). Bar: relative max error
+<= 1e-12 per field, as for uniform meshes.
+"""
+import numpy as np
+import pytest
+
+import ndg_cuda_amd as nd
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+PULSES = [(0.1, -0.2, 0.2, 1.0), (-0.4, 0.3, 0.2, 0.5)]
+MIXED_BC = (nd.BC_WALL, nd.BC_INFLOW, nd.BC_OUTFLOW, nd.BC_SYMMETRY)
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def say(*args):
+    print("[gpu]", *args, flush=True)
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return nd.NDG_t(16, 5), O.Tables(16, 5)
+
+
+def build(tables, res, orders, initial_N, bc=(nd.BC_WALL,) * 4, periodic=False, jitter=0.0, seed=1):
+    ndg, ot = tables
+    desc = nd.square_mesh(res, bc=bc, x_periodic=periodic, y_periodic=periodic, jitter=jitter, seed=seed)
+    desc.quad_N = np.asarray(orders, dtype=np.int32)
+    gm = nd.Mesh2D_t(desc, initial_N, 16, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
+    om = O.Mesh(ot, desc, initial_N)
+    return desc, gm, om
+
+
+def random_orders(K, choices, seed):
+    return np.random.default_rng(seed).choice(choices, size=K).astype(np.int32)
+
+
+CASES = [
+    ("periodic-jitter", dict(periodic=True, jitter=0.2), (4, 5, 6, 8)),
+    ("wall-jitter", dict(jitter=0.2), (3, 4, 7)),
+    ("mixed-bc", dict(bc=MIXED_BC, jitter=0.1), (2, 3, 5)),
+    ("outflow", dict(bc=(nd.BC_OUTFLOW,) * 4, jitter=0.1), (6, 9)),
+    ("zero-ghost-high", dict(bc=(nd.BC_NONE,) * 4), (10, 12, 16)),
+]
+
+
+@pytest.mark.parametrize("name,kw,choices", CASES)
+def test_mixed_rhs(tables, name, kw, choices):
+    say("mixed rhs", name, choices)
+    orders = random_orders(64, choices, 3)
+    desc, gm, om = build(tables, 8, orders, min(choices), **kw)
+    assert gm.n_dofs_per_field_ == int(((orders + 1) ** 2).sum())
+    rng = np.random.default_rng(5)
+    q = [rng.normal(size=gm.n_dofs_per_field_) for _ in range(3)]
+    gm.set_solution(*q)
+    om.set_state(*q)
+    for g, r in zip(gm.get_solution(), q):
+        assert np.array_equal(g, r)
+    t = 0.21
+    ref = om.compute_rhs(t)
+    got = gm.compute_rhs(t)
+    for g, r in zip(got, ref):
+        assert rel(g, r) < TOL, name
+    # element traces come back in the caller's element order, each of its own order
+    tr_ref = om.get_traces()
+    tr = np.concatenate([a.ravel() for a in gm.interpolate_to_boundaries()])
+    assert rel(tr, tr_ref) < TOL
+
+
+@pytest.mark.parametrize("name,kw,choices", CASES)
+def test_mixed_steps(tables, name, kw, choices):
+    say("mixed steps", name, choices)
+    orders = random_orders(64, choices, 11)
+    desc, gm, om = build(tables, 8, orders, min(choices), **kw)
+    if kw.get("bc") == MIXED_BC:
+        gm.initial_conditions(None)
+        om.initial_conditions(None)
+    else:
+        gm.initial_conditions(PULSES)
+        om.initial_conditions(PULSES)
+    solver = nd.Solver2D_t(0.5, [0.0])
+    assert solver.get_delta_t(gm) == om.delta_t(0.5)
+    t_gpu = solver.run_steps(gm, 9, 0.0)
+    t_cpu = om.run_steps(0.5, 9, 0.0)
+    assert t_gpu == t_cpu
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL, name
+
+
+def test_mixed_blocks(tables):
+    """Orders laid out in blocks (an order jump along whole lines of faces) on a jittered periodic
+    mesh, run to t = 0.3 with the reference time loop."""
+    say("mixed blocks periodic N 4/8")
+    res = 16
+    orders = np.where(np.arange(res * res) % 64 < 32, 4, 8).astype(np.int32)
+    desc, gm, om = build(tables, res, orders, 4, periodic=True, jitter=0.1, seed=4)
+    gm.initial_conditions(PULSES)
+    om.initial_conditions(PULSES)
+    solver = nd.Solver2D_t(0.5, [0.3])
+    solver.solve(None, gm)
+    steps_cpu, t_cpu = om.solve(0.5, 0.3)
+    assert solver.time_ == t_cpu
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL
+
+
+def test_mixed_estimator(tables):
+    say("mixed estimator")
+    orders = random_orders(64, (6, 8, 10), 2)
+    desc, gm, om = build(tables, 8, orders, 6, jitter=0.1)
+    gm.initial_conditions(PULSES)
+    om.initial_conditions(PULSES)
+    err, sig, refine, coarsen = gm.estimate_error()
+    e_ref, s_ref, r_ref, c_ref = om.estimate_error()
+    assert np.max(np.abs(err - e_ref)) < 1e-13
+    big = e_ref > 1e-7
+    assert np.max(np.abs(sig[big] - s_ref[big]) / np.abs(s_ref[big])) < 1e-8
+
+
+def test_mixed_fp32c(tables):
+    say("mixed fp32c")
+    orders = random_orders(256, (4, 6, 7), 8)
+    desc, gm, om = build(tables, 16, orders, 4, jitter=0.15, seed=9)
+    gm.initial_conditions(PULSES)
+    om.initial_conditions(PULSES)
+    gm.set_precision("fp32c")
+    solver = nd.Solver2D_t(0.5, [0.0])
+    solver.run_steps(gm, 30)
+    om.run_steps(0.5, 30)
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < 2e-5
+
+
+@pytest.mark.parametrize("mode", ["sequential", "parallel"])
+def test_graph_steps_after_autotune(tables, mode):
+    """Captured-step replays (device clock drives the inflow ghosts' time) after autotuning, on a
+    mixed-order mesh with every boundary kind; the autotuner leaves the state bit-identical."""
+    say("graph", mode)
+    orders = random_orders(256, (2, 4, 5, 7), 21)
+    desc, gm, om = build(tables, 16, orders, 2, bc=MIXED_BC, jitter=0.1, seed=6)
+    gm.initial_conditions(None)
+    om.initial_conditions(None)
+    before = np.concatenate(gm.get_solution())
+    tuning = gm.autotune(2)
+    say("tuning", tuning)
+    assert [t[0] for t in tuning] == [2, 4, 5, 7]
+    assert np.array_equal(np.concatenate(gm.get_solution()), before)
+    gm.set_graph(mode)
+    solver = nd.Solver2D_t(0.5, [0.0])
+    t_gpu = solver.run_steps(gm, 6, 0.0)
+    t_cpu = om.run_steps(0.5, 6, 0.0)
+    assert t_gpu == t_cpu
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL
+    # direct steps interleaved with replays continue from the same clock
+    t_gpu = solver.run_steps(gm, 3, t_gpu)
+    t_cpu = om.run_steps(0.5, 3, t_cpu)
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL
+
+
+def test_graph_solve_config1(tables):
+    """The reference time loop in graph mode: 288 replayed steps plus the clipped 289th launched
+    directly (16 x 16 periodic, N = 6, t = 0.5)."""
+    say("graph solve config 1")
+    ndg, ot = tables
+    desc = nd.square_mesh(16, x_periodic=True, y_periodic=True)
+    gm = nd.Mesh2D_t(desc, 6, 6, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
+    om = O.Mesh(ot, desc, 6)
+    pulse = [(0.1, -0.2, 0.2, 1.0)]
+    gm.initial_conditions(pulse)
+    om.initial_conditions(pulse)
+    gm.set_graph("parallel")
+    solver = nd.Solver2D_t(0.5, [0.5])
+    solver.solve(None, gm)
+    steps, t_cpu = om.solve(0.5, 0.5)
+    assert solver.n_steps_ == steps == 289 and solver.time_ == t_cpu
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL
