@@ -191,6 +191,43 @@ ndg_status ndg_mesh_tuning(const ndg_mesh *mesh, int32_t *orders, int32_t *epb, 
 ndg_status ndg_estimate_error(ndg_mesh *mesh, double *error, double *sigma, int32_t *refine,
                               int32_t *coarsen);
 
+/* ---------------------------------------------------------------------------------------------
+ * Wave speed and the discrete adjoint (no reference counterpart; the reference's c is the
+ * compile-time constant 1, include/helpers/constants.h:15).
+ *
+ * With wave speed c the system is p_t + c div(u, v) = 0, (u, v)_t + c grad p = 0: the right-hand
+ * side is c times the reference operator, the CFL step scales by 1/c, and the plane wave moves
+ * with speed c. c = 1 is the reference arithmetic bit for bit. */
+ndg_status ndg_mesh_set_wave_speed(ndg_mesh *mesh, double c);
+
+/* Pressure receivers: element (caller's order) and reference coordinates (xi, eta) in [-1, 1]^2
+ * of each point; p there is the element's Lagrange interpolant. */
+typedef struct ndg_receivers {
+    int32_t n;
+    const int64_t *element;
+    const double *xi_eta; /* [n][2] */
+} ndg_receivers;
+
+/* Advances the mesh n_steps RK3 steps of size dt and records p at the receivers after every step:
+ * p_out[step][receiver]. Single-order meshes with interior/periodic sides, FP64. */
+ndg_status ndg_record_receivers(ndg_mesh *mesh, const ndg_receivers *rec, double dt, int64_t n_steps,
+                                double *p_out);
+
+/* Misfit J = 1/2 sum_steps sum_receivers (p_r(t_n) - data[n][r])^2 over n_steps steps of size dt
+ * (frozen) from the mesh's current state, and its gradient with respect to the initial fields
+ * (packed like ndg_get_state) and to c, by the discrete adjoint of the RK3 scheme: a forward pass
+ * keeping a checkpoint every checkpoint_interval steps, then per segment a recompute from its
+ * checkpoint that keeps the stage states, and the transposed stages backwards. The mesh state is
+ * the initial one again on return. data may be NULL (zero); any output may be NULL. */
+ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *rec, double dt, int64_t n_steps,
+                                int64_t checkpoint_interval, const double *data, double *J, double *dJ_dp0,
+                                double *dJ_du0, double *dJ_dv0, double *dJ_dc);
+
+/* lambda = G^T mu for the stage right-hand side G (ndg_compute_rhs without boundary terms), packed
+ * fields; for dot-product tests of the adjoint. */
+ndg_status ndg_apply_rhs_transpose(ndg_mesh *mesh, const double *mp, const double *mu, const double *mv,
+                                   double *lp, double *lu, double *lv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,7 +33,8 @@ EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tabl
             "ndg_mesh_destroy", "ndg_mesh_set_stream", "ndg_mesh_info", "ndg_mesh_set_precision", "ndg_set_ic",
             "ndg_set_state", "ndg_get_state", "ndg_get_traces", "ndg_compute_rhs",
             "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_run_steps", "ndg_synchronize",
-            "ndg_estimate_error", "ndg_mesh_set_graph", "ndg_mesh_autotune", "ndg_mesh_tuning"]
+            "ndg_estimate_error", "ndg_mesh_set_graph", "ndg_mesh_autotune", "ndg_mesh_tuning",
+            "ndg_mesh_set_wave_speed", "ndg_record_receivers", "ndg_adjoint_gradient", "ndg_apply_rhs_transpose"]
 
 
 class NDGError(RuntimeError):
@@ -56,6 +57,10 @@ class SquareMeshParams(ctypes.Structure):
                 ("y_min", ctypes.c_double), ("y_max", ctypes.c_double), ("bc", ctypes.c_int32 * 4),
                 ("x_periodic", ctypes.c_int32), ("y_periodic", ctypes.c_int32),
                 ("jitter", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class Receivers(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("element", ctypes.c_void_p), ("xi_eta", ctypes.c_void_p)]
 
 
 class ICDesc(ctypes.Structure):
@@ -101,6 +106,10 @@ def lib():
             "ndg_mesh_set_graph": [vp, i32],
             "ndg_mesh_autotune": [vp, i32],
             "ndg_mesh_tuning": [vp, vp, vp, vp, i32, vp],
+            "ndg_mesh_set_wave_speed": [vp, f64],
+            "ndg_record_receivers": [vp, vp, f64, i64, vp],
+            "ndg_adjoint_gradient": [vp, vp, f64, i64, i64, vp, vp, vp, vp, vp, vp],
+            "ndg_apply_rhs_transpose": [vp, vp, vp, vp, vp, vp, vp],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)

@@ -1,0 +1,427 @@
+// adjoint.hip -- discrete adjoint (transpose) of the fused RK3 stage, for gradients of a receiver
+// misfit with respect to the initial fields and the wave speed c. No reference counterpart.
+//
+// The forward stage (kernels.hip) is, per element, with nu = mu-adjoint scaled below:
+//   G_f = c (-d_xi - d_eta) / J,
+//   d_xi[i][j]  = sum_k Dh[i][k] F_f[k][j]  + (F*_1[j] lp[i] + F*_3[N-j] lm[i]) / w[i],
+//   d_eta[i][j] = sum_k Dh[j][k] Gt_f[i][k] + (F*_2[N-i] lp[j] + F*_0[i] lm[j]) / w[j],
+//   F_p = y_eta u - x_eta v, F_u = y_eta p, F_v = -x_eta p (x_eta, y_eta functions of xi_i),
+//   Gt_p = -y_xi u + x_xi v, Gt_u = -y_xi p, Gt_v = x_xi p (x_xi, y_xi functions of eta_j),
+//   F*_s[k] = sf_s flux(own trace (s, k), neighbour trace (s', N - k)), the upwind flux with
+//   w_L = p_L + u_L.n, w_R = p_R - u_R.n: F*p = (w_L - w_R)/2, F*u = n_x (w_L + w_R)/2, F*v = n_y ...,
+//   traces tr0[i] = sum_j lm[j] q[i][j], tr2[N-i] = sum_j lp[j] q[i][j],
+//          tr1[j] = sum_i lp[i] q[i][j], tr3[N-j] = sum_i lm[i] q[i][j]
+// (kernels.hip stage_kernel; src/solvers/Solver2D_t.cpp:48-136, Element2D_t.cpp:86-125).
+//
+// Transpose applied to an adjoint mu (per node and field): nu = -c mu / J, then
+//   Fbar_f[k][j] = sum_i Dh[i][k] nu_f[i][j],  Gbar_f[i][k] = sum_j Dh[j][k] nu_f[i][j],
+//   rho_1[j] = sum_i lp[i] nu[i][j] / w[i],  rho_3[N-j] = sum_i lm[i] nu[i][j] / w[i],
+//   rho_0[i] = sum_j lm[j] nu[i][j] / w[j],  rho_2[N-i] = sum_j lp[j] nu[i][j] / w[j]
+// (rho_s is the adjoint of the side flux F*_s); per side point omega_L = sf (rho_p + n.rho_uv)/2,
+// omega_R = sf (-rho_p + n.rho_uv)/2; the own trace adjoint is (omega_L, n omega_L) plus what the
+// neighbour exports for this point, chi = (omega_R, -n omega_R) computed on its side; finally the
+// traces' and fluxes' transposes scatter into lambda:
+//   dl_p += y_eta Fbar_u - x_eta Fbar_v - y_xi Gbar_u + x_xi Gbar_v,
+//   dl_u += y_eta Fbar_p - y_xi Gbar_p,  dl_v += -x_eta Fbar_p + x_xi Gbar_p,
+//   dl_f[i][j] += lm[j] tau0_f[i] + lp[j] tau2_f[N-i] + lp[i] tau1_f[j] + lm[i] tau3_f[N-j].
+//
+// Reverse RK stage s (forward: g' = a_s g + G(q), q' = q + gs g' with gs = gamma_s dt), given the
+// adjoints (lambda, mu) of (q', g'):  mu~ = mu + gs lambda;  lambda <- lambda + G^T mu~;
+// mu <- a_s mu~.  The c-derivative of the misfit gathers <mu~, dG/dc q_s> = <G^T mu~, q_s> / c
+// over every stage (G is linear in c), from the stage input state q_s.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace ndg {
+
+// Flags of adjoint_stage_kernel.
+constexpr int kAdjMuIn = 1;     // mu input is non-zero (stages after the first reverse stage of a step)
+constexpr int kAdjMuOut = 2;    // store mu <- a_s mu~ (a_s != 0)
+constexpr int kAdjExport = 4;   // export chi for the next reverse stage
+constexpr int kAdjApply = 8;    // do the transpose (off: export only)
+
+template <int N>
+__global__ void __launch_bounds__(192)
+adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_nb,
+                     const double *__restrict__ tab, double *__restrict__ lam, double *__restrict__ mu,
+                     const double *__restrict__ chi_in, double *__restrict__ chi_out,
+                     const double *__restrict__ qs, double *__restrict__ cpart, int64_t K, int flags,
+                     double gs, double as, double gnext, double cw) {
+    using S = LineShape<N>;
+    constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::T, FPT = S::FPT, QPT = S::QPT;
+    __shared__ double s_l[EPB * 3 * NE];   // lambda (then lambda')
+    __shared__ double s_n[EPB * 3 * NE];   // nu = -c mu~ / J
+    __shared__ double s_a[EPB * 3 * NE];   // d lambda
+    __shared__ double s_rho[EPB * 12 * NF];
+    __shared__ double s_met[EPB * 4 * NF];
+    __shared__ double s_side[EPB * 4 * 3];
+    __shared__ double s_v[EPB * 8];
+    __shared__ double s_red[3];
+    const double *Dh = tab;
+    const double *lm = tab + NF * NF;
+    const double *lp = lm + NF;
+    const double *x = lp + 2 * NF;
+    const double *winv = x + NF;
+
+    const int tid = threadIdx.x;
+    const int f = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int el = lane / NF, l = lane % NF;
+    const int64_t e0 = (int64_t)blockIdx.x * EPB;
+    const int nel = (int)min<int64_t>(EPB, K - e0);
+    const bool active = el < nel;
+    const bool apply = flags & kAdjApply;
+
+    // P0: connectivity and neighbour exports of the thread's flux points, lambda and mu tiles,
+    // vertices.
+    double nbv[FPT][3];
+#pragma unroll
+    for (int it = 0; it < FPT; ++it) {
+        nbv[it][0] = nbv[it][1] = nbv[it][2] = 0.0;
+        const int item = tid + it * T;
+        if (apply && item < nel * 4 * NF) {
+            const int32_t cd = side_nb[(e0 + item / NF / 4) * 4 + (item / NF) % 4];
+            const int k = item % NF;
+            if (cd >= 0) {  // the host admits interior and periodic sides only
+                const double *nb = chi_in + (int64_t)(cd >> 2) * 12 * NF + (cd & 3) * 3 * NF + (N - k);
+                nbv[it][0] = nb[0];
+                nbv[it][1] = nb[NF];
+                nbv[it][2] = nb[2 * NF];
+            }
+        }
+    }
+    double mreg[QPT];
+    const double *lg = lam + e0 * 3 * NE;
+    const double *mg = mu + e0 * 3 * NE;
+#pragma unroll
+    for (int it = 0; it < QPT; ++it) {
+        const int r = tid + it * T;
+        const bool in = r < nel * 3 * NE;
+        if (in) s_l[r] = lg[r];
+        mreg[it] = (in && (flags & kAdjMuIn)) ? mg[r] : 0.0;
+    }
+    for (int r = tid; r < nel * 8; r += T) s_v[r] = verts[e0 * 8 + r];
+    __syncthreads();
+
+    // P1: metric lines, side normals and edge Jacobians (as the forward stage).
+    if (active) {
+        const double *P = s_v + el * 8;
+        double *met = s_met + el * 4 * NF;
+        if (f == 0) {
+            const double xi = x[l];
+            met[l] = ((1 - xi) * (P[6] - P[0]) + (1 + xi) * (P[4] - P[2])) / 4;
+            met[NF + l] = ((1 - xi) * (P[7] - P[1]) + (1 + xi) * (P[5] - P[3])) / 4;
+        } else if (f == 1) {
+            const double eta = x[l];
+            met[2 * NF + l] = ((1 - eta) * (P[2] - P[0]) + (1 + eta) * (P[4] - P[6])) / 4;
+            met[3 * NF + l] = ((1 - eta) * (P[3] - P[1]) + (1 + eta) * (P[5] - P[7])) / 4;
+        } else if (l < 4) {
+            const int s = l, s1 = (s + 1) & 3;
+            const double ex = P[2 * s1] - P[2 * s], ey = P[2 * s1 + 1] - P[2 * s + 1];
+            const double len = sqrt(ex * ex + ey * ey);
+            double *sd = s_side + (el * 4 + s) * 3;
+            sd[0] = ey / len;
+            sd[1] = -(ex / len);
+            sd[2] = side_scale(P, s, x[0]);
+        }
+    }
+    __syncthreads();
+
+    auto jac_at = [&](int e, int node) {
+        const double *met = s_met + e * 4 * NF;
+        const int a = node / NF, b = node % NF;
+        return met[2 * NF + b] * met[NF + a] - met[a] * met[3 * NF + b];
+    };
+
+    if (apply) {
+        // P2: mu~ = mu + gs lambda; mu' = a_s mu~ (stored); nu = -c mu~ / J.
+        double *mo = mu + e0 * 3 * NE;
+#pragma unroll
+        for (int it = 0; it < QPT; ++it) {
+            const int r = tid + it * T;
+            if (r < nel * 3 * NE) {
+                const double mt = mreg[it] + gs * s_l[r];
+                mreg[it] = as * mt;
+                if (flags & kAdjMuOut) mo[r] = mreg[it];
+                s_n[r] = -cw * mt * recip(jac_at(r / (3 * NE), r % NE));
+            }
+        }
+        __syncthreads();
+
+        // P3: xi lines (thread owns column j = l): volume transpose along xi into d lambda,
+        // lift transposes of sides 1 and 3. Field p takes Fbar_u and Fbar_v, fields u and v take
+        // Fbar_p (a wave-uniform branch).
+        if (active) {
+            const int j = l;
+            const double *nb = s_n + el * 3 * NE;
+            const double *met = s_met + el * 4 * NF;
+            double own[NF];
+#pragma unroll
+            for (int i = 0; i < NF; ++i) own[i] = nb[f * NE + i * NF + j];
+            double r1 = 0.0, r3 = 0.0;
+#pragma unroll
+            for (int i = 0; i < NF; ++i) {
+                r1 += lp[i] * winv[i] * own[i];
+                r3 += lm[i] * winv[i] * own[i];
+            }
+            double *rho = s_rho + el * 12 * NF + f * NF;
+            rho[3 * NF + j] = r1;
+            rho[9 * NF + (N - j)] = r3;
+            double *da = s_a + el * 3 * NE + f * NE + j;
+            if (f == 0) {
+                double ou[NF], ov[NF];
+#pragma unroll
+                for (int i = 0; i < NF; ++i) {
+                    ou[i] = nb[NE + i * NF + j];
+                    ov[i] = nb[2 * NE + i * NF + j];
+                }
+#pragma unroll
+                for (int a = 0; a < NF; ++a) {
+                    double Fu = 0.0, Fv = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NF; ++i) {
+                        Fu += Dh[i * NF + a] * ou[i];
+                        Fv += Dh[i * NF + a] * ov[i];
+                    }
+                    da[a * NF] = met[NF + a] * Fu - met[a] * Fv;
+                }
+            } else {
+                double op[NF];
+#pragma unroll
+                for (int i = 0; i < NF; ++i) op[i] = nb[i * NF + j];
+#pragma unroll
+                for (int a = 0; a < NF; ++a) {
+                    double Fp = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NF; ++i) Fp += Dh[i * NF + a] * op[i];
+                    da[a * NF] = f == 1 ? met[NF + a] * Fp : -(met[a] * Fp);
+                }
+            }
+        }
+        __syncthreads();
+
+        // P4: eta lines (thread owns row i = l): volume transpose along eta added to d lambda,
+        // lift transposes of sides 0 and 2.
+        if (active) {
+            const int i = l;
+            const double *nb = s_n + el * 3 * NE;
+            const double *met = s_met + el * 4 * NF;
+            double own[NF];
+#pragma unroll
+            for (int j = 0; j < NF; ++j) own[j] = nb[f * NE + i * NF + j];
+            double r0 = 0.0, r2 = 0.0;
+#pragma unroll
+            for (int j = 0; j < NF; ++j) {
+                r0 += lm[j] * winv[j] * own[j];
+                r2 += lp[j] * winv[j] * own[j];
+            }
+            double *rho = s_rho + el * 12 * NF + f * NF;
+            rho[i] = r0;
+            rho[6 * NF + (N - i)] = r2;
+            double *da = s_a + el * 3 * NE + f * NE + i * NF;
+            if (f == 0) {
+                double ou[NF], ov[NF];
+#pragma unroll
+                for (int j = 0; j < NF; ++j) {
+                    ou[j] = nb[NE + i * NF + j];
+                    ov[j] = nb[2 * NE + i * NF + j];
+                }
+#pragma unroll
+                for (int b = 0; b < NF; ++b) {
+                    double Gu = 0.0, Gv = 0.0;
+#pragma unroll
+                    for (int j = 0; j < NF; ++j) {
+                        Gu += Dh[j * NF + b] * ou[j];
+                        Gv += Dh[j * NF + b] * ov[j];
+                    }
+                    da[b] += -(met[3 * NF + b] * Gu) + met[2 * NF + b] * Gv;
+                }
+            } else {
+                double op[NF];
+#pragma unroll
+                for (int j = 0; j < NF; ++j) op[j] = nb[i * NF + j];
+#pragma unroll
+                for (int b = 0; b < NF; ++b) {
+                    double Gp = 0.0;
+#pragma unroll
+                    for (int j = 0; j < NF; ++j) Gp += Dh[j * NF + b] * op[j];
+                    da[b] += f == 1 ? -(met[3 * NF + b] * Gp) : met[2 * NF + b] * Gp;
+                }
+            }
+        }
+        __syncthreads();
+
+        // P5: flux transpose at the side points: own trace adjoint tau = (omega_L, n omega_L)
+        // plus the neighbour's export for this point.
+#pragma unroll
+        for (int it = 0; it < FPT; ++it) {
+            const int item = tid + it * T;
+            if (item < nel * 4 * NF) {
+                const int fe = item / (4 * NF), s = (item / NF) % 4, k = item % NF;
+                double *rho = s_rho + fe * 12 * NF + s * 3 * NF + k;
+                const double *sd = s_side + (fe * 4 + s) * 3;
+                const double nx = sd[0], ny = sd[1], sf = sd[2];
+                const double wl = sf * ((rho[0] + nx * rho[NF] + ny * rho[2 * NF]) / 2);
+                rho[0] = wl + nbv[it][0];
+                rho[NF] = nx * wl + nbv[it][1];
+                rho[2 * NF] = ny * wl + nbv[it][2];
+            }
+        }
+        __syncthreads();
+
+        // P6: trace transposes (thread owns row i = l); lambda' = lambda + d lambda; the
+        // c-derivative term <d lambda, q_s>.
+        double dot = 0.0;
+        if (active) {
+            const int i = l;
+            const double *tau = s_rho + el * 12 * NF + f * NF;
+            const double t0 = tau[i], t2 = tau[6 * NF + (N - i)];
+            double *da = s_a + el * 3 * NE + f * NE + i * NF;
+            double *lw = s_l + el * 3 * NE + f * NE + i * NF;
+            const double *qrow = qs ? qs + (e0 + el) * 3 * NE + f * NE + i * NF : nullptr;
+#pragma unroll
+            for (int b = 0; b < NF; ++b) {
+                const double dl = da[b] + lm[b] * t0 + lp[b] * t2 + lp[i] * tau[3 * NF + b] +
+                                  lm[i] * tau[9 * NF + (N - b)];
+                lw[b] += dl;
+                if (qrow) dot += dl * qrow[b];
+            }
+        }
+        if (cpart) {
+            for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
+            if (lane == 0) s_red[f] = dot;
+        }
+        __syncthreads();
+        if (cpart && tid == 0) cpart[blockIdx.x] += (s_red[0] + s_red[1]) + s_red[2];
+
+        double *lo = lam + e0 * 3 * NE;
+#pragma unroll
+        for (int it = 0; it < QPT; ++it) {
+            const int r = tid + it * T;
+            if (r < nel * 3 * NE) lo[r] = s_l[r];
+        }
+    }
+    if (!(flags & kAdjExport)) return;
+
+    // P7: nu of the next reverse stage, from mu' + gnext lambda'.
+#pragma unroll
+    for (int it = 0; it < QPT; ++it) {
+        const int r = tid + it * T;
+        if (r < nel * 3 * NE)
+            s_n[r] = -cw * (mreg[it] + gnext * s_l[r]) * recip(jac_at(r / (3 * NE), r % NE));
+    }
+    __syncthreads();
+    // P8: lift transposes of all four sides (thread owns column l and row l).
+    if (active) {
+        const double *nf_ = s_n + el * 3 * NE + f * NE;
+        double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+#pragma unroll
+        for (int m = 0; m < NF; ++m) {
+            const double col = nf_[m * NF + l], row = nf_[l * NF + m];
+            r1 += lp[m] * winv[m] * col;
+            r3 += lm[m] * winv[m] * col;
+            r0 += lm[m] * winv[m] * row;
+            r2 += lp[m] * winv[m] * row;
+        }
+        double *rho = s_rho + el * 12 * NF + f * NF;
+        rho[l] = r0;
+        rho[3 * NF + l] = r1;
+        rho[6 * NF + (N - l)] = r2;
+        rho[9 * NF + (N - l)] = r3;
+    }
+    __syncthreads();
+    // P9: the export chi = (omega_R, -n omega_R) of every side point.
+#pragma unroll
+    for (int it = 0; it < FPT; ++it) {
+        const int item = tid + it * T;
+        if (item < nel * 4 * NF) {
+            const int fe = item / (4 * NF), s = (item / NF) % 4, k = item % NF;
+            const double *rho = s_rho + fe * 12 * NF + s * 3 * NF + k;
+            const double *sd = s_side + (fe * 4 + s) * 3;
+            const double nx = sd[0], ny = sd[1], sf = sd[2];
+            const double wr = sf * ((-rho[0] + nx * rho[NF] + ny * rho[2 * NF]) / 2);
+            double *co = chi_out + (e0 + fe) * 12 * NF + s * 3 * NF + k;
+            co[0] = wr;
+            co[NF] = -(nx * wr);
+            co[2 * NF] = -(ny * wr);
+        }
+    }
+}
+
+// Receivers: p at point r = sum over the element's nodes of w_r[node] p[node] (one wave per
+// receiver), and the misfit source lambda_p += res_r w_r (one workgroup, receivers in order so
+// that receivers sharing an element add without races).
+__global__ void __launch_bounds__(64)
+record_kernel(const double *__restrict__ q, const int64_t *__restrict__ elem, const double *__restrict__ w,
+              int ne, double *__restrict__ out) {
+    const int r = blockIdx.x;
+    const double *pe = q + elem[r] * 3 * ne;
+    const double *wr = w + (int64_t)r * ne;
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < ne; k += 64) acc += wr[k] * pe[k];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (threadIdx.x == 0) out[r] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+source_kernel(double *__restrict__ lam, const int64_t *__restrict__ elem, const double *__restrict__ w,
+              const double *__restrict__ res, int n, int ne) {
+    for (int r = 0; r < n; ++r) {
+        double *le = lam + elem[r] * 3 * ne;
+        const double *wr = w + (int64_t)r * ne;
+        const double c = res[r];
+        for (int k = threadIdx.x; k < ne; k += 256) le[k] += c * wr[k];
+        __syncthreads();
+    }
+}
+
+template <int N>
+static hipError_t adjoint_n(DeviceMesh &m, const AdjointArgs &A, hipStream_t s) {
+    using S = LineShape<N>;
+    const Group &G = m.groups[0];
+    const int64_t blocks = (G.K + S::EPB - 1) / S::EPB;
+    hipLaunchKernelGGL(adjoint_stage_kernel<N>, dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts, m.side_nb, G.tab,
+                       A.lam, A.mu, A.chi_in, A.chi_out, A.qs, A.cpart, G.K, A.flags, A.gs, A.as, A.gnext,
+                       m.wave_speed);
+    return hipGetLastError();
+}
+
+#define ADJ_CASES(F, ...)                                                                                    \
+    case 1: return F<1>(__VA_ARGS__); case 2: return F<2>(__VA_ARGS__); case 3: return F<3>(__VA_ARGS__);   \
+    case 4: return F<4>(__VA_ARGS__); case 5: return F<5>(__VA_ARGS__); case 6: return F<6>(__VA_ARGS__);   \
+    case 7: return F<7>(__VA_ARGS__); case 8: return F<8>(__VA_ARGS__); case 9: return F<9>(__VA_ARGS__);   \
+    case 10: return F<10>(__VA_ARGS__); case 11: return F<11>(__VA_ARGS__); case 12: return F<12>(__VA_ARGS__); \
+    case 13: return F<13>(__VA_ARGS__); case 14: return F<14>(__VA_ARGS__); case 15: return F<15>(__VA_ARGS__); \
+    case 16: return F<16>(__VA_ARGS__);
+
+hipError_t launch_adjoint_stage(DeviceMesh &m, const AdjointArgs &A, hipStream_t s) {
+    if (m.groups.size() != 1) return hipErrorInvalidValue;
+    switch (m.groups[0].N) { ADJ_CASES(adjoint_n, m, A, s) }
+    return hipErrorInvalidValue;
+}
+
+int64_t adjoint_blocks(const DeviceMesh &m) {
+    const int N = m.groups[0].N;
+    const int epb = 64 / (N + 1);
+    return (m.groups[0].K + epb - 1) / epb;
+}
+
+hipError_t launch_record(const DeviceMesh &m, const int64_t *elem, const double *w, int n, double *out,
+                         hipStream_t s) {
+    const int ne = (m.groups[0].N + 1) * (m.groups[0].N + 1);
+    hipLaunchKernelGGL(record_kernel, dim3(n), dim3(64), 0, s, m.q, elem, w, ne, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_source(double *lam, const DeviceMesh &m, const int64_t *elem, const double *w,
+                         const double *res, int n, hipStream_t s) {
+    const int ne = (m.groups[0].N + 1) * (m.groups[0].N + 1);
+    hipLaunchKernelGGL(source_kernel, dim3(1), dim3(256), 0, s, lam, elem, w, res, n, ne);
+    return hipGetLastError();
+}
+
+}  // namespace ndg

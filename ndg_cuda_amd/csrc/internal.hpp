@@ -12,7 +12,7 @@
 
 namespace ndg {
 
-constexpr double kWaveSpeed = 1.0;  // include/helpers/constants.h:15
+constexpr double kWaveSpeed = 1.0;  // include/helpers/constants.h:15 (the flux form; see wave_speed)
 constexpr int kMaxOrder = 16;       // largest order with a compiled kernel instance
 
 // Boundary (ghost) side record. One per element side whose neighbour is a wall, symmetry,
@@ -113,6 +113,10 @@ struct DeviceMesh {
     int64_t scratch_len = 0;
     int cur = 0;                 // trace buffer holding the traces of the current state
     double *clock = nullptr;     // [4]: step start time, then the 3 stage times
+    // Wave speed c of the symmetric system p_t + c div(u, v) = 0, (u, v)_t + c grad p = 0: the
+    // right-hand side is c times the reference operator (whose constant c is 1), so c = 1 is the
+    // reference arithmetic bit for bit.
+    double wave_speed = 1.0;
     // FP32 form with compensated state (allocated by ndg_mesh_set_precision): the state is the
     // unevaluated sum qh + ql of two floats per value, the RK register and traces are floats.
     int precision = NDG_PRECISION_FP64;
@@ -150,6 +154,27 @@ hipError_t launch_mortar(DeviceMesh &m, hipStream_t s);
 hipError_t launch_estimate(const DeviceMesh &m, double tol_min, double tol_max, double *out_dev,
                            hipStream_t s);
 bool order_supported(int N);
+
+// Adjoint (transpose) stage on a single-order mesh with interior and periodic sides only
+// (adjoint.hip). flags: 1 mu input non-zero, 2 store mu <- as mu~, 4 export chi for the next
+// reverse stage (from mu' + gnext lambda'), 8 apply the transpose (else export only).
+struct AdjointArgs {
+    double *lam = nullptr, *mu = nullptr;
+    const double *chi_in = nullptr;
+    double *chi_out = nullptr;
+    const double *qs = nullptr;  // stage input state for the c-derivative (may be null)
+    double *cpart = nullptr;     // per-workgroup accumulators of <d lambda, q_s> (may be null)
+    int flags = 0;
+    double gs = 0.0, as = 0.0, gnext = 0.0;
+};
+hipError_t launch_adjoint_stage(DeviceMesh &m, const AdjointArgs &A, hipStream_t s);
+int64_t adjoint_blocks(const DeviceMesh &m);
+// Receivers (element in device order, (N+1)^2 interpolation weights each): p at the receivers of
+// the current state, and lambda_p += res_r w_r.
+hipError_t launch_record(const DeviceMesh &m, const int64_t *elem, const double *w, int n, double *out,
+                         hipStream_t s);
+hipError_t launch_source(double *lam, const DeviceMesh &m, const int64_t *elem, const double *w,
+                         const double *res, int n, hipStream_t s);
 
 // Records msg as the calling thread's last error and returns code.
 ndg_status set_error(ndg_status code, const char *msg);

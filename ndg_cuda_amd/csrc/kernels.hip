@@ -18,6 +18,7 @@
 #include <climits>
 #include <cmath>
 
+#include "device.hpp"
 #include "internal.hpp"
 
 namespace ndg {
@@ -30,42 +31,15 @@ struct Shape {
     static constexpr int THREADS = ((EPB * NE + 63) / 64) * 64;
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD walks one
-// contiguous stretch of the Hilbert-ordered elements (neighbours share its L2).
-__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G) {
-    const int64_t per = G / 8, rem = G % 8, x = b % 8, i = b / 8;
-    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
-}
-
-// Bilinear metrics (include/functions/quad_metrics.tpp:3-8): x_xi, x_eta, y_xi, y_eta.
-__device__ __forceinline__ void metrics(const double *P, double xi, double eta, double &xxi,
-                                        double &xeta, double &yxi, double &yeta) {
-    xxi = ((1 - eta) * (P[2] - P[0]) + (1 + eta) * (P[4] - P[6])) / 4;
-    xeta = ((1 - xi) * (P[6] - P[0]) + (1 + xi) * (P[4] - P[2])) / 4;
-    yxi = ((1 - eta) * (P[3] - P[1]) + (1 + eta) * (P[5] - P[7])) / 4;
-    yeta = ((1 - xi) * (P[7] - P[1]) + (1 + xi) * (P[5] - P[3])) / 4;
-}
-
-// Edge Jacobian of side s (src/entities/Element2D_t.cpp:756-769): constant along a side of a
-// bilinear element, so it is evaluated once at the side's first node.
-__device__ __forceinline__ double side_scale(const double *P, int s, double x0) {
-    double xxi, xeta, yxi, yeta;
-    if (s == 0) { metrics(P, x0, -1.0, xxi, xeta, yxi, yeta); return sqrt(xxi * xxi + yxi * yxi); }
-    if (s == 1) { metrics(P, 1.0, x0, xxi, xeta, yxi, yeta); return sqrt(xeta * xeta + yeta * yeta); }
-    if (s == 2) { metrics(P, x0, 1.0, xxi, xeta, yxi, yeta); return sqrt(xxi * xxi + yxi * yxi); }
-    metrics(P, -1.0, x0, xxi, xeta, yxi, yeta);
-    return sqrt(xeta * xeta + yeta * yeta);
-}
-
 // The reference plane wave g(x, t) (include/functions/analytical_solution.tpp:5-11).
-__device__ __forceinline__ void plane_wave(double x, double y, double t, double &p, double &u,
+__device__ __forceinline__ void plane_wave(double x, double y, double t, double cw, double &p, double &u,
                                            double &v) {
     const double kx = 0.707106781186548, ky = 0.707106781186548, x0 = -0.2, y0 = -0.2;
     const double d = 0.120112240878645;
-    const double a = kx * (x - x0) + ky * (y - y0) - kWaveSpeed * t;
+    const double a = kx * (x - x0) + ky * (y - y0) - cw * t;
     p = exp(-a * a / (d * d));
-    u = p * kx / kWaveSpeed;
-    v = p * ky / kWaveSpeed;
+    u = p * kx;  // the reference's p k / c at c = 1 (and the plane wave of the speed-c system)
+    v = p * ky;
 }
 
 // Trace k of side s of one element tile (src/entities/Element2D_t.cpp:86-125): sides 2 and 3
@@ -140,25 +114,7 @@ traces_kernel(const double *__restrict__ tab, const double *__restrict__ q,
 // A workgroup takes epb <= EPB consecutive elements (fewer gives more workgroups for a small
 // order group); remap != 0 applies the XCD-contiguous block order. t_ptr points at the stage time
 // in device memory (the inflow ghost reads it), so that a captured step can be replayed.
-template <int N>
-struct LineShape {
-    static constexpr int NF = N + 1;
-    static constexpr int NE = NF * NF;
-    static constexpr int T = 192;                            // one wave per field
-    static constexpr int EPB = 64 / NF;                       // elements per workgroup
-    static constexpr int FPT = (EPB * 4 * NF + T - 1) / T;    // flux points per thread
-    static constexpr int QPT = (EPB * 3 * NE + T - 1) / T;    // state values per thread (tile load)
-    static constexpr int VPT = (EPB * 8 + T - 1) / T;         // vertex coordinates per thread
-};
 
-// 1/x from the hardware reciprocal estimate and two Newton steps (quadratic convergence from its
-// ~2^-29 relative error to the rounding error of the last step).
-__device__ __forceinline__ double recip(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.0), r);
-    r = fma(r, fma(-x, r, 1.0), r);
-    return r;
-}
 
 template <int N, bool RHS_ONLY>
 __global__ void __launch_bounds__(192)
@@ -167,7 +123,7 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
              double *__restrict__ q, double *__restrict__ g, const double *__restrict__ tr_in,
              double *__restrict__ tr_out, double *__restrict__ rhs, const double *__restrict__ side_flux,
              int64_t K, int first, double a, double gdt, const double *__restrict__ t_ptr, int epb,
-             int remap) {
+             int remap, double cw) {
     using S = LineShape<N>;
     constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::T, FPT = S::FPT;
     constexpr double c = kWaveSpeed;
@@ -332,7 +288,7 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
                     const double interp = (x[N - k] + 1) / 2;
                     const double X = rec.bx * interp + rec.ax * (1 - interp);
                     const double Y = rec.by * interp + rec.ay * (1 - interp);
-                    plane_wave(X, Y, *t_ptr, pR, uR, vR);
+                    plane_wave(X, Y, *t_ptr, cw, pR, uR, vR);
                 } else {
                     pR = 0.0;
                     uR = 0.0;
@@ -401,7 +357,7 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
             for (int k = 0; k < NF; ++k) acc += Dh[j * NF + k] * Gt[k];
             acc += (Ft * lp[j] + Fb * lm[j]) * winv[j];
             const double jac = met[2 * NF + j] * yeta_i - xeta_i * met[3 * NF + j];
-            const double Gv = (-d[j] - acc) * recip(jac);
+            const double Gv = cw * ((-d[j] - acc) * recip(jac));
             if (RHS_ONLY) {
                 d[j] = Gv;
             } else {
@@ -543,7 +499,7 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
                  const float *__restrict__ tab, float *__restrict__ qh, float *__restrict__ ql,
                  float *__restrict__ g, const float *__restrict__ tr_in, float *__restrict__ tr_out,
                  const float *__restrict__ side_flux, int64_t K, int first, float a, float gdt,
-                 const double *__restrict__ t_ptr, int epb, int remap) {
+                 const double *__restrict__ t_ptr, int epb, int remap, double cw) {
     using S = LineShape<N>;
     constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::T, FPT = S::FPT;
     __shared__ float s_q[EPB * 3 * NE];
@@ -710,7 +666,7 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
                     const double X = rec.bx * interp + rec.ax * (1 - interp);
                     const double Y = rec.by * interp + rec.ay * (1 - interp);
                     double p64, u64, v64;
-                    plane_wave(X, Y, *t_ptr, p64, u64, v64);
+                    plane_wave(X, Y, *t_ptr, cw, p64, u64, v64);
                     pR = (float)p64;
                     uR = (float)u64;
                     vR = (float)v64;
@@ -779,7 +735,7 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
             for (int k = 0; k < NF; ++k) acc += Dh[j * NF + k] * Gt[k];
             acc += (Ft * lp[j] + Fb * lm[j]) * winv[j];
             const float jac = met[2 * NF + j] * yeta_i - xeta_i * met[3 * NF + j];
-            const float Gv = (-d[j] - acc) / jac;
+            const float Gv = (float)cw * ((-d[j] - acc) / jac);
             const float gn = first ? Gv : a * g_old[j] + Gv;
             g[(e0 + el) * 3 * NE + f * NE + j * NF + i] = gn;
             float h = qrow[j], lo = lo_row[j];
@@ -844,7 +800,7 @@ __global__ void ic_kernel(const double *__restrict__ verts, const double *__rest
                       P[5] * (xi + 1) * (eta + 1) + P[7] * (1 - xi) * (eta + 1)) / 4;
     double p = 0.0, u = 0.0, v = 0.0;
     if (kind == NDG_IC_PLANE_WAVE) {
-        plane_wave(X, Y, 0.0, p, u, v);
+        plane_wave(X, Y, 0.0, 1.0, p, u, v);
     } else {
         for (int k = 0; k < n_pulses; ++k) {
             const double dx = X - pulses[4 * k], dy = Y - pulses[4 * k + 1], w = pulses[4 * k + 2];
@@ -872,7 +828,7 @@ __device__ __forceinline__ double wave_min(double v) {
 }
 
 __global__ void __launch_bounds__(256)
-dt_partial_kernel(const double *__restrict__ verts, int64_t K, int N, double cfl,
+dt_partial_kernel(const double *__restrict__ verts, int64_t K, int N, double cfl, double cw,
                   double *__restrict__ partial) {
 #pragma clang fp contract(off)
     __shared__ double s_min[4];
@@ -882,7 +838,7 @@ dt_partial_kernel(const double *__restrict__ verts, int64_t K, int N, double cfl
         const double d = fmin(fmin(fmin(edge_len(P, 1, 0), edge_len(P, 2, 3)),
                                    fmin(edge_len(P, 1, 2), edge_len(P, 0, 3))),
                               fmin(edge_len(P, 1, 3), edge_len(P, 2, 0)));
-        v = fmin(cfl * d / (N * N * kWaveSpeed), v);
+        v = fmin(cfl * d / (N * N * cw), v);
     }
     v = wave_min(v);
     if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = v;
@@ -1123,7 +1079,8 @@ static hipError_t stage_n(DeviceMesh &m, const Group &G, int stage, double a, do
     hipLaunchKernelGGL((stage_kernel<N, false>), dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts + G.e_begin * 8,
                        m.side_nb + G.e_begin * 4, m.bc, G.tab, m.q + G.q_off, m.g + G.q_off,
                        m.trace[m.cur] + G.tr_off, m.trace[1 - m.cur] + G.tr_off, nullptr, m.side_flux, G.K,
-                       stage == 0 ? 1 : 0, a, gdt, m.clock + 1 + stage, epb, G.tune[NDG_PRECISION_FP64].remap);
+                       stage == 0 ? 1 : 0, a, gdt, m.clock + 1 + stage, epb, G.tune[NDG_PRECISION_FP64].remap,
+                       m.wave_speed);
     return hipGetLastError();
 }
 
@@ -1134,7 +1091,7 @@ static hipError_t rhs_n(DeviceMesh &m, const Group &G, hipStream_t s) {
     hipLaunchKernelGGL((stage_kernel<N, true>), dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts + G.e_begin * 8,
                        m.side_nb + G.e_begin * 4, m.bc, G.tab, m.q + G.q_off, m.g + G.q_off,
                        m.trace[m.cur] + G.tr_off, nullptr, m.rhs + G.q_off, m.side_flux, G.K, 1, 0.0, 0.0,
-                       m.clock + 1, S::EPB, 1);
+                       m.clock + 1, S::EPB, 1, m.wave_speed);
     return hipGetLastError();
 }
 
@@ -1173,7 +1130,7 @@ static hipError_t stage_c32_n(DeviceMesh &m, const Group &G, int stage, double a
                        m.side_nb + G.e_begin * 4, m.bc, G.tab, G.tab32, m.qh + G.q_off, m.ql + G.q_off,
                        m.g32 + G.q_off, m.trace32[m.cur] + G.tr_off, m.trace32[1 - m.cur] + G.tr_off,
                        m.side_flux32, G.K, stage == 0 ? 1 : 0, (float)a, (float)gdt, m.clock + 1 + stage, epb,
-                       G.tune[NDG_PRECISION_FP32_COMPENSATED].remap);
+                       G.tune[NDG_PRECISION_FP32_COMPENSATED].remap, m.wave_speed);
     return hipGetLastError();
 }
 
@@ -1301,7 +1258,7 @@ hipError_t launch_delta_t(const DeviceMesh &m, double cfl, double *dt_dev, hipSt
                                                                       1024 / (int64_t)m.groups.size()));
         if (used + blocks > m.scratch_len) return hipErrorInvalidValue;
         hipLaunchKernelGGL(dt_partial_kernel, dim3(blocks), dim3(256), 0, s, m.verts + G.e_begin * 8, G.K, G.N, cfl,
-                           m.scratch + used);
+                           m.wave_speed, m.scratch + used);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         used += blocks;

@@ -956,3 +956,262 @@ extern "C" ndg_status ndg_estimate_error(ndg_mesh *mesh, double *error, double *
     }
     return NDG_OK;
 }
+
+// ------------------------------------------------------------------------------------------
+// Wave speed and the adjoint form (no reference counterpart)
+
+extern "C" ndg_status ndg_mesh_set_wave_speed(ndg_mesh *mesh, double c) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_mesh_set_wave_speed: null mesh");
+    if (!(c > 0.0) || !std::isfinite(c)) return fail(NDG_ERR_INVALID, "ndg_mesh_set_wave_speed: c must be positive");
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_mesh_set_wave_speed sync");
+    drop_graphs(mesh);
+    mesh->d.wave_speed = c;
+    return NDG_OK;
+}
+
+namespace {
+
+// Device buffer freed on scope exit (adjoint temporaries).
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <typename T>
+    T *as() const { return (T *)p; }
+};
+
+ndg_status dev_alloc(DevBuf &b, size_t bytes, const char *what) {
+    HIP_TRY(hipMalloc(&b.p, bytes), what);
+    return NDG_OK;
+}
+
+// Lagrange interpolating polynomials of the order-N Gauss nodes at xi (barycentric form with the
+// exact-node case, include/entities/NDG_t.tpp:209-226).
+void lagrange_at(const Tables &t, int N, double xi, std::vector<double> &out) {
+    const auto &x = t.tab[NDG_TABLE_NODES][N];
+    const auto &lam = t.tab[NDG_TABLE_BARYCENTRIC_WEIGHTS][N];
+    out.assign((size_t)N + 1, 0.0);
+    for (int i = 0; i <= N; ++i)
+        if (xi == x[i]) {
+            out[i] = 1.0;
+            return;
+        }
+    double den = 0.0;
+    for (int i = 0; i <= N; ++i) {
+        out[i] = lam[i] / (xi - x[i]);
+        den += out[i];
+    }
+    for (int i = 0; i <= N; ++i) out[i] /= den;
+}
+
+// The adjoint and the receivers need one order group, no boundary records and no mortar faces.
+ndg_status check_adjoint_mesh(const ndg_mesh *m, const char *who) {
+    const DeviceMesh &d = m->d;
+    if (d.groups.size() != 1 || d.n_bc != 0 || d.n_faces != 0)
+        return fail(NDG_ERR_UNSUPPORTED, std::string(who) + ": needs a single-order mesh whose sides are all interior "
+                                                            "or periodic");
+    if (d.precision != NDG_PRECISION_FP64) return fail(NDG_ERR_UNSUPPORTED, std::string(who) + ": FP64 precision only");
+    return NDG_OK;
+}
+
+struct DeviceReceivers {
+    DevBuf elem, w;
+    int n = 0;
+};
+
+ndg_status upload_receivers(const ndg_mesh *m, const ndg_receivers *rec, DeviceReceivers &out) {
+    if (!rec || rec->n < 1 || !rec->element || !rec->xi_eta) return fail(NDG_ERR_INVALID, "receivers: empty list");
+    const int N = m->d.groups[0].N, ne = (N + 1) * (N + 1);
+    std::vector<int64_t> elem((size_t)rec->n);
+    std::vector<double> w((size_t)rec->n * ne);
+    std::vector<double> lx, ly;
+    for (int r = 0; r < rec->n; ++r) {
+        const int64_t e = rec->element[r];
+        const double xi = rec->xi_eta[2 * r], eta = rec->xi_eta[2 * r + 1];
+        if (e < 0 || e >= m->d.K) return fail(NDG_ERR_INVALID, "receivers: element out of range");
+        if (!(xi >= -1.0 && xi <= 1.0 && eta >= -1.0 && eta <= 1.0))
+            return fail(NDG_ERR_INVALID, "receivers: reference coordinates outside [-1, 1]");
+        elem[r] = m->dev_of[e];
+        lagrange_at(m->tables->t, N, xi, lx);
+        lagrange_at(m->tables->t, N, eta, ly);
+        for (int i = 0; i <= N; ++i)
+            for (int j = 0; j <= N; ++j) w[(size_t)r * ne + i * (N + 1) + j] = lx[i] * ly[j];
+    }
+    ndg_status st;
+    if ((st = dev_alloc(out.elem, elem.size() * sizeof(int64_t), "hipMalloc receivers")) != NDG_OK) return st;
+    if ((st = dev_alloc(out.w, w.size() * sizeof(double), "hipMalloc receivers")) != NDG_OK) return st;
+    HIP_TRY(hipMemcpy(out.elem.p, elem.data(), elem.size() * sizeof(int64_t), hipMemcpyHostToDevice), "upload receivers");
+    HIP_TRY(hipMemcpy(out.w.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice), "upload receivers");
+    out.n = rec->n;
+    return NDG_OK;
+}
+
+}  // namespace
+
+extern "C" ndg_status ndg_record_receivers(ndg_mesh *mesh, const ndg_receivers *rec, double dt, int64_t n_steps,
+                                           double *p_out) {
+    if (!mesh || !p_out) return fail(NDG_ERR_INVALID, "ndg_record_receivers: null argument");
+    if (n_steps < 0 || !(dt > 0.0)) return fail(NDG_ERR_INVALID, "ndg_record_receivers: bad step count or dt");
+    ndg_status st = check_adjoint_mesh(mesh, "ndg_record_receivers");
+    if (st != NDG_OK) return st;
+    DeviceReceivers R;
+    if ((st = upload_receivers(mesh, rec, R)) != NDG_OK) return st;
+    DevBuf out;
+    if (n_steps == 0) return NDG_OK;
+    if ((st = dev_alloc(out, sizeof(double) * R.n * n_steps, "hipMalloc records")) != NDG_OK) return st;
+    DeviceMesh &d = mesh->d;
+    HIP_TRY(launch_clock_begin(d, 0.0, 1, dt, mesh->stream), "clock kernel");
+    for (int64_t n = 0; n < n_steps; ++n) {
+        if ((st = step_from_clock(mesh, dt)) != NDG_OK) return st;
+        HIP_TRY(launch_record(d, R.elem.as<int64_t>(), R.w.as<double>(), R.n, out.as<double>() + n * R.n,
+                              mesh->stream), "record kernel");
+    }
+    HIP_TRY(hipMemcpyAsync(p_out, out.p, sizeof(double) * R.n * n_steps, hipMemcpyDeviceToHost, mesh->stream),
+            "download records");
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_record_receivers sync");
+    return NDG_OK;
+}
+
+extern "C" ndg_status ndg_apply_rhs_transpose(ndg_mesh *mesh, const double *mp, const double *mu, const double *mv,
+                                              double *lp, double *lu, double *lv) {
+    if (!mesh || !mp || !mu || !mv) return fail(NDG_ERR_INVALID, "ndg_apply_rhs_transpose: null argument");
+    ndg_status st = check_adjoint_mesh(mesh, "ndg_apply_rhs_transpose");
+    if (st != NDG_OK) return st;
+    DeviceMesh &d = mesh->d;
+    DevBuf lam, muv, chi;
+    if ((st = dev_alloc(lam, sizeof(double) * d.n_q, "hipMalloc adjoint")) != NDG_OK) return st;
+    if ((st = dev_alloc(muv, sizeof(double) * d.n_q, "hipMalloc adjoint")) != NDG_OK) return st;
+    if ((st = dev_alloc(chi, sizeof(double) * d.n_tr, "hipMalloc adjoint")) != NDG_OK) return st;
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_apply_rhs_transpose sync");
+    double *const ins[3] = {const_cast<double *>(mp), const_cast<double *>(mu), const_cast<double *>(mv)};
+    if ((st = copy_state(mesh, muv.as<double>(), ins, false, "upload adjoint")) != NDG_OK) return st;
+    HIP_TRY(hipMemsetAsync(lam.p, 0, sizeof(double) * d.n_q, mesh->stream), "hipMemset adjoint");
+    AdjointArgs A;
+    A.lam = lam.as<double>();
+    A.mu = muv.as<double>();
+    A.chi_out = chi.as<double>();
+    A.flags = 1 | 4;  // mu input, export only
+    HIP_TRY(launch_adjoint_stage(d, A, mesh->stream), "adjoint export kernel");
+    A.chi_in = chi.as<double>();
+    A.chi_out = nullptr;
+    A.flags = 1 | 8;  // mu input, apply
+    HIP_TRY(launch_adjoint_stage(d, A, mesh->stream), "adjoint kernel");
+    double *const outs[3] = {lp, lu, lv};
+    return copy_state(mesh, lam.as<double>(), outs, true, "download adjoint");
+}
+
+extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *rec, double dt, int64_t n_steps,
+                                           int64_t checkpoint_interval, const double *data, double *J,
+                                           double *dJ_dp0, double *dJ_du0, double *dJ_dv0, double *dJ_dc) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_adjoint_gradient: null mesh");
+    if (n_steps < 1 || !(dt > 0.0) || checkpoint_interval < 1)
+        return fail(NDG_ERR_INVALID, "ndg_adjoint_gradient: bad step count, dt or checkpoint interval");
+    ndg_status st = check_adjoint_mesh(mesh, "ndg_adjoint_gradient");
+    if (st != NDG_OK) return st;
+    DeviceMesh &d = mesh->d;
+    hipStream_t s = mesh->stream;
+    DeviceReceivers R;
+    if ((st = upload_receivers(mesh, rec, R)) != NDG_OK) return st;
+    const int64_t n_ck = (n_steps + checkpoint_interval - 1) / checkpoint_interval;
+    const int64_t seg = std::min(checkpoint_interval, n_steps);
+    const size_t qb = sizeof(double) * d.n_q;
+    DevBuf ck, store, lam, mu, chi0, chi1, cpart, recs, res;
+    if ((st = dev_alloc(ck, qb * n_ck, "hipMalloc checkpoints")) != NDG_OK) return st;
+    if ((st = dev_alloc(store, qb * 3 * seg, "hipMalloc stage store")) != NDG_OK) return st;
+    if ((st = dev_alloc(lam, qb, "hipMalloc adjoint")) != NDG_OK) return st;
+    if ((st = dev_alloc(mu, qb, "hipMalloc adjoint")) != NDG_OK) return st;
+    if ((st = dev_alloc(chi0, sizeof(double) * d.n_tr, "hipMalloc adjoint")) != NDG_OK) return st;
+    if ((st = dev_alloc(chi1, sizeof(double) * d.n_tr, "hipMalloc adjoint")) != NDG_OK) return st;
+    const int64_t nblk = adjoint_blocks(d);
+    if ((st = dev_alloc(cpart, sizeof(double) * nblk, "hipMalloc adjoint")) != NDG_OK) return st;
+    if ((st = dev_alloc(recs, sizeof(double) * R.n * n_steps, "hipMalloc records")) != NDG_OK) return st;
+    if ((st = dev_alloc(res, sizeof(double) * R.n * n_steps, "hipMalloc residuals")) != NDG_OK) return st;
+    auto ckp = [&](int64_t k) { return ck.as<double>() + k * d.n_q; };
+    auto stp = [&](int64_t j, int stage) { return store.as<double>() + (j * 3 + stage) * d.n_q; };
+
+    // Forward pass with checkpoints; p at the receivers after every step.
+    HIP_TRY(launch_clock_begin(d, 0.0, 1, dt, s), "clock kernel");
+    for (int64_t n = 0; n < n_steps; ++n) {
+        if (n % checkpoint_interval == 0)
+            HIP_TRY(hipMemcpyAsync(ckp(n / checkpoint_interval), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
+        if ((st = step_from_clock(mesh, dt)) != NDG_OK) return st;
+        HIP_TRY(launch_record(d, R.elem.as<int64_t>(), R.w.as<double>(), R.n, recs.as<double>() + n * R.n, s),
+                "record kernel");
+    }
+    std::vector<double> hres((size_t)(R.n * n_steps));
+    HIP_TRY(hipMemcpyAsync(hres.data(), recs.p, sizeof(double) * hres.size(), hipMemcpyDeviceToHost, s), "download records");
+    HIP_TRY(hipStreamSynchronize(s), "adjoint sync");
+    double Jv = 0.0;
+    for (size_t k = 0; k < hres.size(); ++k) {
+        hres[k] -= data ? data[k] : 0.0;
+        Jv += hres[k] * hres[k];
+    }
+    Jv /= 2;
+    HIP_TRY(hipMemcpyAsync(res.p, hres.data(), sizeof(double) * hres.size(), hipMemcpyHostToDevice, s), "upload residuals");
+
+    // Reverse sweep: recompute each segment from its checkpoint keeping the stage inputs, then
+    // run the transposed stages backwards.
+    HIP_TRY(hipMemsetAsync(lam.p, 0, qb, s), "hipMemset adjoint");
+    HIP_TRY(hipMemsetAsync(mu.p, 0, qb, s), "hipMemset adjoint");
+    HIP_TRY(hipMemsetAsync(cpart.p, 0, sizeof(double) * nblk, s), "hipMemset adjoint");
+    double *chi[2] = {chi0.as<double>(), chi1.as<double>()};
+    int cc = 0;
+    auto add_source_and_export = [&](int64_t n) -> ndg_status {
+        HIP_TRY(launch_source(lam.as<double>(), d, R.elem.as<int64_t>(), R.w.as<double>(),
+                              res.as<double>() + n * R.n, R.n, s), "source kernel");
+        AdjointArgs A;
+        A.lam = lam.as<double>();
+        A.mu = mu.as<double>();
+        A.chi_out = chi[1 - cc];
+        A.flags = 4;  // export only, mu = 0 at a step boundary
+        A.gnext = kGm[2] * dt;
+        HIP_TRY(launch_adjoint_stage(d, A, s), "adjoint export kernel");
+        cc = 1 - cc;
+        return NDG_OK;
+    };
+    if ((st = add_source_and_export(n_steps - 1)) != NDG_OK) return st;
+    for (int64_t k = n_ck - 1; k >= 0; --k) {
+        const int64_t n0 = k * checkpoint_interval, n1 = std::min(n0 + checkpoint_interval, n_steps);
+        HIP_TRY(hipMemcpyAsync(d.q, ckp(k), qb, hipMemcpyDeviceToDevice, s), "restore checkpoint");
+        HIP_TRY(launch_traces(d, s), "traces kernel");
+        for (int64_t j = n0; j < n1; ++j)
+            for (int stage = 0; stage < 3; ++stage) {
+                HIP_TRY(hipMemcpyAsync(stp(j - n0, stage), d.q, qb, hipMemcpyDeviceToDevice, s), "stage store");
+                HIP_TRY(launch_stage(d, stage, kAm[stage], kGm[stage] * dt, s), "stage kernel");
+            }
+        for (int64_t j = n1 - 1; j >= n0; --j) {
+            for (int stage = 2; stage >= 0; --stage) {
+                AdjointArgs A;
+                A.lam = lam.as<double>();
+                A.mu = mu.as<double>();
+                A.chi_in = chi[cc];
+                A.chi_out = chi[1 - cc];
+                A.qs = stp(j - n0, stage);
+                A.cpart = cpart.as<double>();
+                A.flags = 8 | (stage != 2 ? 1 : 0) | (stage != 0 ? 2 | 4 : 0);
+                A.gs = kGm[stage] * dt;
+                A.as = kAm[stage];
+                A.gnext = stage > 0 ? kGm[stage - 1] * dt : 0.0;
+                HIP_TRY(launch_adjoint_stage(d, A, s), "adjoint kernel");
+                if (stage != 0) cc = 1 - cc;
+            }
+            if (j >= 1 && (st = add_source_and_export(j - 1)) != NDG_OK) return st;
+        }
+    }
+
+    // Outputs; the mesh state goes back to the initial fields.
+    std::vector<double> hc((size_t)nblk);
+    HIP_TRY(hipMemcpyAsync(hc.data(), cpart.p, sizeof(double) * nblk, hipMemcpyDeviceToHost, s), "download c-gradient");
+    double *const outs[3] = {dJ_dp0, dJ_du0, dJ_dv0};
+    if ((st = copy_state(mesh, lam.as<double>(), outs, true, "download gradient")) != NDG_OK) return st;
+    double dc = 0.0;
+    for (double v : hc) dc += v;
+    HIP_TRY(hipMemcpyAsync(d.q, ckp(0), qb, hipMemcpyDeviceToDevice, s), "restore state");
+    HIP_TRY(hipMemsetAsync(d.g, 0, qb, s), "hipMemset g");
+    HIP_TRY(launch_traces(d, s), "traces kernel");
+    HIP_TRY(hipStreamSynchronize(s), "ndg_adjoint_gradient sync");
+    if (J) *J = Jv;
+    if (dJ_dc) *dJ_dc = dc / d.wave_speed;
+    return NDG_OK;
+}

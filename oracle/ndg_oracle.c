@@ -277,6 +277,20 @@ static void g_exact(double x, double y, double t, double *q) {
     q[2] = p * K_Y / C_WAVE;
 }
 
+/* The plane wave of the speed-c system (the reference's g at c = 1): argument k.(x - x0) - c t,
+ * velocity p k. */
+static void g_exact_c(double x, double y, double t, double c, double *q) {
+    if (c == 1.0) {
+        g_exact(x, y, t, q);
+        return;
+    }
+    const double a = K_X * (x - XY0_X) + K_Y * (y - XY0_Y) - c * t;
+    const double p = exp(-a * a / (D_WIDTH * D_WIDTH));
+    q[0] = p;
+    q[1] = p * K_X;
+    q[2] = p * K_Y;
+}
+
 static double mag(v2 a) { return sqrt(a.x * a.x + a.y * a.y); }
 static v2 sub(v2 a, v2 b) { v2 r = {a.x - b.x, a.y - b.y}; return r; }
 
@@ -323,6 +337,7 @@ struct o_mesh {
     int64_t n_wall, n_sym, n_in, n_out, n_if;
     int64_t *wall, *sym, *in, *out, *if_origin, *if_side, *if_dest;
     double tol_min, tol_max;
+    double c; /* wave-speed scale of the right-hand side (1 = the reference) */
 };
 
 static void elem_alloc_domain(o_elem *e) {
@@ -444,6 +459,7 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
     m->t = t;
     m->tol_min = tol_min;
     m->tol_max = tol_max;
+    m->c = 1.0;
     m->n_nodes = n_nodes;
     m->xy = (v2 *)malloc((size_t)n_nodes * sizeof(v2));
     for (int64_t i = 0; i < n_nodes; ++i) { m->xy[i].x = xy[2 * i]; m->xy[i].y = xy[2 * i + 1]; }
@@ -716,7 +732,7 @@ double o_get_delta_t(const o_mesh *m, double cfl) {
     double dt = INFINITY;
     for (int64_t i = 0; i < m->K; ++i) {
         const o_elem *e = &m->el[i];
-        dt = fmin(cfl * e->dmin / (e->N * e->N * C_WAVE), dt);
+        dt = fmin(cfl * e->dmin / (e->N * e->N * C_WAVE * m->c), dt);
     }
     return dt;
 }
@@ -812,7 +828,7 @@ static void inflow_boundary(const o_mesh *m, int64_t gi, double time) {
         const double X = b.x * interp + a.x * (1 - interp);
         const double Y = b.y * interp + a.y * (1 - interp);
         double q[3];
-        g_exact(X, Y, time, q);
+        g_exact_c(X, Y, time, m->c, q);
         g->pe[0][k] = q[0];
         g->ue[0][k] = q[1];
         g->ve[0][k] = q[2];
@@ -1041,8 +1057,16 @@ static void dg_derivative(o_mesh *m) {
                 e->Gv[k] = (e->Gv[k] - e->vder[j]) / e->jac[k];
             }
         }
+        /* wave speed c: the symmetric speed-c system's operator is c times the c = 1 operator */
+        for (int k = 0; k < nf * nf; ++k) {
+            e->Gp[k] *= m->c;
+            e->Gu[k] *= m->c;
+            e->Gv[k] *= m->c;
+        }
     }
 }
+
+void o_mesh_set_wave_speed(o_mesh *m, double c) { m->c = c; }
 
 /* Solver2D_t.cpp:138-172 rk3_first_step / rk3_step (a = 0 marks the first stage) */
 static void rk3_update(o_mesh *m, int first, double dt, double a, double g) {

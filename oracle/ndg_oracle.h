@@ -79,6 +79,10 @@ double o_get_delta_t(const o_mesh *m, double cfl);
 /* Phases 1-6 of one RK stage at time t: traces, boundary conditions, face projection, fluxes,
  * element projection, volume + lift -> G. */
 void o_compute_rhs(o_mesh *m, double t);
+/* Wave speed c of the symmetric system (p_t + c div u = 0, u_t + c grad p = 0): scales the
+ * right-hand side by c and the CFL step by 1/c; the reference is c = 1 (no reference counterpart
+ * for other values). */
+void o_mesh_set_wave_speed(o_mesh *m, double c);
 /* One full RK3 step (3 stages) from time t with step dt. */
 void o_step(o_mesh *m, double t, double dt);
 /* The reference time loop (include/solvers/Solver2D_t.tpp:37-72) without output or adaptation.

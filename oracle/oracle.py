@@ -43,7 +43,7 @@ def lib():
             "o_set_state": ([vp, vp, vp, vp], None), "o_get_state": ([vp, vp, vp, vp], None),
             "o_get_rhs": ([vp, vp, vp, vp], None), "o_get_traces": ([vp, vp], None),
             "o_get_flux_traces": ([vp, vp], None), "o_get_delta_t": ([vp, f64], f64),
-            "o_compute_rhs": ([vp, f64], None), "o_step": ([vp, f64, f64], None),
+            "o_compute_rhs": ([vp, f64], None), "o_mesh_set_wave_speed": ([vp, f64], None), "o_step": ([vp, f64, f64], None),
             "o_solve": ([vp, f64, f64, vp], i64), "o_run_steps": ([vp, f64, i64, vp], None),
             "o_estimate_error": ([vp, vp, vp, vp, vp], None),
             "o_element_traces": ([vp, i32, vp, vp, vp, vp], None),
@@ -142,6 +142,9 @@ class Mesh:
 
     def get_flux_traces(self):
         return self._sides(lib().o_get_flux_traces)
+
+    def set_wave_speed(self, c):
+        lib().o_mesh_set_wave_speed(self.h, float(c))
 
     def delta_t(self, cfl):
         return lib().o_get_delta_t(self.h, cfl)
