@@ -126,7 +126,7 @@ def batched_leg(args, nd, torch, stream):
     diff = max(float(np.abs(a - b).max()) for a, b in zip(got, ref))
     orders = desc.quad_N
     nf = orders.astype(np.int64) + 1
-    bytes_launch = int((88 * nf * nf + 192 * nf + 80).sum()) + mortar_bytes(orders, desc)
+    bytes_launch = int((88 * nf * nf + 192 * nf + 80).sum()) + int(mortar_bytes(orders, desc))
     stage_s = ms_graph / 1e3 / (3 * steps)
     counts = {int(n): int((orders == n).sum()) for n in np.unique(orders)}
     return {"value": dofs * 3 * steps / (ms_graph / 1e3), "ms_per_step": ms_graph / steps,
@@ -140,6 +140,92 @@ def batched_leg(args, nd, torch, stream):
                          "unit": "GB/s", "frac": bytes_launch / stage_s / 1e9 / HBM_PEAK_GBS,
                          "algorithmic_bytes_per_stage": bytes_launch,
                          "note": "all launches of one stage (mortar kernel + one stage kernel per order)"}}
+
+
+def timed_steps(solver, mesh, stream, torch, steps, t=0.0):
+    """Runs `steps` RK3 steps on the mesh stream between two HIP events; returns ms."""
+    mesh.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    solver.run_steps(mesh, steps, t)
+    stop.record(stream)
+    stop.synchronize()
+    return start.elapsed_time(stop)
+
+
+def precision_leg(nd, torch, stream):
+    """1024 x 1024 periodic, N = 7, 4 seeded pulses, 500 steps: FP64 and the FP32 compensated
+    form on the same initial state, with the FP32 form's error against the FP64 run."""
+    res, n, steps = 1024, 7, 500
+    rng = np.random.default_rng(4)
+    pulses = np.column_stack([rng.uniform(-0.7, 0.7, 4), rng.uniform(-0.7, 0.7, 4), rng.uniform(0.1, 0.2, 4),
+                              rng.uniform(0.2, 1.0, 4)])
+    desc = nd.square_mesh(res, x_periodic=True, y_periodic=True)
+    mesh = nd.Mesh2D_t(desc, n, n, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, nd.NDG_t(n, 5))
+    mesh.set_stream(stream.cuda_stream)
+    solver = nd.Solver2D_t(CFL, [0.0])
+    out = {"workload": f"{res}x{res} periodic, N={n}, 4 seeded pulses, {steps} steps", "dofs": mesh.n_dofs}
+    states = {}
+    for prec in ("fp64", "fp32c"):
+        mesh.set_precision("fp64")
+        mesh.initial_conditions(pulses)
+        mesh.set_precision(prec)
+        ms = timed_steps(solver, mesh, stream, torch, steps)
+        states[prec] = np.concatenate(mesh.get_solution())
+        nf = n + 1
+        b = (88 * nf * nf + 192 * nf + 80) if prec == "fp64" else (68 * nf * nf + 96 * nf + 80)
+        b *= mesh.n_elements_
+        out[prec] = {"value": mesh.n_dofs * 3 * steps / (ms / 1e3), "ms_per_step": ms / steps,
+                     "roofline_frac": b / (ms / 1e3 / (3 * steps)) / 1e9 / HBM_PEAK_GBS}
+    d = states["fp32c"] - states["fp64"]
+    out["fp32c"]["l2_rel_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(states["fp64"]))
+    out["fp32c"]["max_rel_vs_fp64"] = float(np.abs(d).max() / np.abs(states["fp64"]).max())
+    mesh.close()
+    return out
+
+
+def adjoint_leg(nd, torch, stream):
+    """128 x 128 periodic, N = 8, 2000 steps, checkpoints every 50, 32 seeded receivers, data from a
+    c = 1.05 run of two pulses: gradient time against one forward solve, and one central finite
+    difference (eps = 1e-6) along a seeded direction."""
+    from ndg_cuda_amd import adjoint as A
+    res, n, steps, interval = 128, 8, 2000, 50
+    desc = nd.square_mesh(res, x_periodic=True, y_periodic=True)
+    mesh = nd.Mesh2D_t(desc, n, n, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, nd.NDG_t(n, 5))
+    mesh.set_stream(stream.cuda_stream)
+    rec = A.Receivers.seeded(mesh.n_elements_, 32, 2024)
+    dt = 0.5 * (2.0 / res) / (n * n)
+    mesh.initial_conditions([(0.1, -0.2, 0.2, 1.0), (-0.4, 0.3, 0.2, 0.5)])
+    A.set_wave_speed(mesh, 1.05)
+    data = A.record_receivers(mesh, rec, dt, steps)
+    A.set_wave_speed(mesh, 1.0)
+    mesh.initial_conditions([(0.1, -0.2, 0.2, 1.0)])
+    q0 = [np.array(a) for a in mesh.get_solution()]
+    t0 = time.perf_counter()
+    A.record_receivers(mesh, rec, dt, steps)
+    t_fwd = time.perf_counter() - t0
+    mesh.set_solution(*q0)
+    t0 = time.perf_counter()
+    J, gp, gu, gv, gc = A.adjoint_gradient(mesh, rec, dt, steps, interval, data)
+    t_adj = time.perf_counter() - t0
+    rng = np.random.default_rng(8)
+    d = [rng.normal(size=q0[0].size) for _ in range(3)]
+    eps = 1e-6
+
+    def misfit(sign):
+        mesh.set_solution(*(a + sign * eps * b for a, b in zip(q0, d)))
+        A.set_wave_speed(mesh, 1.0 + sign * eps)
+        p = A.record_receivers(mesh, rec, dt, steps)
+        return 0.5 * float(((p - data) ** 2).sum())
+
+    fd = (misfit(1) - misfit(-1)) / (2 * eps)
+    ad = float(gp @ d[0] + gu @ d[1] + gv @ d[2]) + gc
+    mesh.close()
+    return {"workload": f"{res}x{res} periodic, N={n}, {steps} steps, checkpoints every {interval}, 32 receivers",
+            "J": J, "dJ_dc": gc, "forward_s": t_fwd, "gradient_s": t_adj, "gradient_over_forward": t_adj / t_fwd,
+            "fd_rel_diff": abs(ad - fd) / abs(fd),
+            "note": "wall times of the library calls (host-synchronous), forward = one recorded 2000-step run"}
 
 
 def main():
@@ -223,6 +309,8 @@ def main():
     forms = {}
     if not args.fp64_only:
         forms["batched"] = batched_leg(args, nd, torch, stream)
+        forms["precision_1024"] = precision_leg(nd, torch, stream)
+        forms["adjoint_128"] = adjoint_leg(nd, torch, stream)
         ms32, _, st32 = timed_run("fp32c")
         diff = np.concatenate([a - b for a, b in zip(st32, ref_state)])
         refv = np.concatenate(ref_state)

@@ -162,12 +162,26 @@ ndg_status ndg_get_delta_t(ndg_mesh *mesh, double cfl, double *dt);
 /* One RK3 step (3 stages, include/solvers/Solver2D_t.tpp:44-70) from time t with step dt.
  * Stream-ordered; returns without waiting for the GPU. */
 ndg_status ndg_step(ndg_mesh *mesh, double t, double dt);
-/* The reference time loop Solver2D_t::solve (include/solvers/Solver2D_t.tpp:37-72) without output
- * or adaptation: dt from get_delta_t, the last step clipped to t_end, time accumulated on the
- * host exactly as the reference does. Waits for completion. */
+/* The reference time loop Solver2D_t::solve (include/solvers/Solver2D_t.tpp:37-100) without
+ * output: dt from get_delta_t, the last step clipped to t_end, time accumulated on the host exactly
+ * as the reference does, and the mesh's adaptivity (ndg_mesh_set_adaptivity). Waits for
+ * completion. */
 ndg_status ndg_solve(ndg_mesh *mesh, double cfl, double t_end, double *t_final, int64_t *n_steps);
 /* n_steps steps of the same loop without clipping, from *time_io (updated). Stream-ordered. */
 ndg_status ndg_run_steps(ndg_mesh *mesh, double cfl, int64_t n_steps, double *time_io);
+/* Mesh2D_t::adapt without splitting (src/meshes/Mesh2D_t.cpp:1644-1706, p_adapt :5026-5049):
+ * estimates the error, then raises by 2 the order of every element with refine && sigma >= 1 &&
+ * N + 2 <= N_max (Element2D_t.cpp:802-812), interpolating its state to the new nodes
+ * (Element2D_t.cpp:464-579); ghosts and faces follow (adjust_boundaries / adjust_interfaces /
+ * adjust_faces, :7482-7538), including the reference's zero edge scaling factor of a resized
+ * ghost. Returns NDG_ERR_UNSUPPORTED (and changes nothing) when an element would be split
+ * (refine && sigma < 1 && max_split_level > 0). *n_raised (may be NULL) gets the count. */
+ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split_level, int64_t *n_raised);
+/* The order of every element (caller's order); it changes with ndg_adapt. */
+ndg_status ndg_mesh_orders(const ndg_mesh *mesh, int32_t *orders);
+/* Mesh2D_t's adaptivity_interval: ndg_solve then adapts every that many steps with the tables'
+ * N_max and the description's max_split_level (include/solvers/Solver2D_t.tpp:88-96). 0 = off. */
+ndg_status ndg_mesh_set_adaptivity(ndg_mesh *mesh, int64_t adaptivity_interval);
 /* Waits for all work queued on the mesh's stream. */
 ndg_status ndg_synchronize(ndg_mesh *mesh);
 /* Replaces Solver2D_t::solve's per-phase launches (include/solvers/Solver2D_t.tpp:44-70, one

@@ -137,7 +137,6 @@ class Mesh2D_t {
           max_split_level_(max_split_level), adaptivity_interval_(adaptivity_interval),
           load_balancing_interval_(load_balancing_interval), tolerance_min_(tolerance_min),
           tolerance_max_(tolerance_max), load_balancing_threshold_(load_balancing_threshold) {
-        if (adaptivity_interval != 0) throw Error(NDG_ERR_UNSUPPORTED, "Mesh2D_t: adaptivity is not available in this build");
         ndg_mesh_desc d{};
         d.n_nodes = (int64_t)(mesh.xy.size() / 2);
         d.xy = mesh.xy.data();
@@ -161,9 +160,26 @@ class Mesh2D_t {
         n_elements_ = (std::size_t)ne;
         n_faces_ = (std::size_t)faces;
         n_dofs_per_field_ = (std::size_t)dofs;
+        if (adaptivity_interval != 0) check(ndg_mesh_set_adaptivity(handle_, (int64_t)adaptivity_interval));
+        refresh();
+    }
+
+    // Sizes and element orders (they change with adapt and with an adaptive solve).
+    void refresh() {
+        int64_t dofs = 0;
+        check(ndg_mesh_info(handle_, nullptr, nullptr, &dofs, nullptr, nullptr));
+        n_dofs_per_field_ = (std::size_t)dofs;
+        orders_.resize(n_elements_);
+        check(ndg_mesh_orders(handle_, orders_.data()));
         n_trace_values_ = 0;
-        for (std::size_t e = 0; e < n_elements_; ++e)
-            n_trace_values_ += 12 * (std::size_t)((mesh.quad_N.empty() ? initial_N : mesh.quad_N[e]) + 1);
+        for (int32_t N : orders_) n_trace_values_ += 12 * (std::size_t)(N + 1);
+    }
+    // Mesh2D_t::adapt without splitting (N_max: the tables' maximum order).
+    std::int64_t adapt(int N_max) {
+        int64_t raised = 0;
+        check(ndg_adapt(handle_, N_max, max_split_level_, &raised));
+        refresh();
+        return raised;
     }
     Mesh2D_t(const Mesh2D_t &) = delete;
     Mesh2D_t &operator=(const Mesh2D_t &) = delete;
@@ -214,6 +230,8 @@ class Mesh2D_t {
 
     ndg_mesh *handle() { return handle_; }
 
+    std::vector<int32_t> orders_;
+
   private:
     std::size_t n_trace_values_ = 0;
 
@@ -251,6 +269,7 @@ class Solver2D_t {
         (void)NDG;
         if (output_times_.empty()) throw Error(NDG_ERR_INVALID, "Solver2D_t::solve: no output times");
         check(ndg_solve(mesh.handle(), CFL_, output_times_.back(), &time_, &n_steps_));
+        mesh.refresh();
     }
 
     double get_delta_t(Meshes::Mesh2D_t &mesh) const {

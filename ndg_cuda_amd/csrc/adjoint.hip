@@ -119,14 +119,16 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
             const double eta = x[l];
             met[2 * NF + l] = ((1 - eta) * (P[2] - P[0]) + (1 + eta) * (P[4] - P[6])) / 4;
             met[3 * NF + l] = ((1 - eta) * (P[3] - P[1]) + (1 + eta) * (P[5] - P[7])) / 4;
-        } else if (l < 4) {
-            const int s = l, s1 = (s + 1) & 3;
-            const double ex = P[2 * s1] - P[2 * s], ey = P[2 * s1 + 1] - P[2 * s + 1];
-            const double len = sqrt(ex * ex + ey * ey);
-            double *sd = s_side + (el * 4 + s) * 3;
-            sd[0] = ey / len;
-            sd[1] = -(ex / len);
-            sd[2] = side_scale(P, s, x[0]);
+        } else {  // sides l, l + N + 1, ... (N + 1 lanes cover the 4 sides)
+            for (int s = l; s < 4; s += NF) {
+                const int s1 = (s + 1) & 3;
+                const double ex = P[2 * s1] - P[2 * s], ey = P[2 * s1 + 1] - P[2 * s + 1];
+                const double len = sqrt(ex * ex + ey * ey);
+                double *sd = s_side + (el * 4 + s) * 3;
+                sd[0] = ey / len;
+                sd[1] = -(ex / len);
+                sd[2] = side_scale(P, s, x[0]);
+            }
         }
     }
     __syncthreads();

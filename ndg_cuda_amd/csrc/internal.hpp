@@ -176,6 +176,15 @@ hipError_t launch_record(const DeviceMesh &m, const int64_t *elem, const double 
 hipError_t launch_source(double *lam, const DeviceMesh &m, const int64_t *elem, const double *w,
                          const double *res, int n, hipStream_t s);
 
+// Solution transfer between two layouts after an order change (one record per element, offsets
+// in values of q): copy, or interpolation to the new order (transfer_kernel).
+struct TransferRec {
+    int64_t old_off, new_off;
+    int32_t N_old, N_new;
+};
+hipError_t launch_transfer(const double *q_old, double *q_new, const TransferRec *recs, int64_t K,
+                           const double *tab1d, hipStream_t s);
+
 // Records msg as the calling thread's last error and returns code.
 ndg_status set_error(ndg_status code, const char *msg);
 

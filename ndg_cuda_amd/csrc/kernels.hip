@@ -216,14 +216,16 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
             const double eta = x[l];
             met[2 * NF + l] = ((1 - eta) * (P[2] - P[0]) + (1 + eta) * (P[4] - P[6])) / 4;
             met[3 * NF + l] = ((1 - eta) * (P[3] - P[1]) + (1 + eta) * (P[5] - P[7])) / 4;
-        } else if (l < 4) {  // side l: outward normal of the edge P_l -> P_l+1 and its Jacobian
-            const int s = l, s1 = (s + 1) & 3;
-            const double ex = P[2 * s1] - P[2 * s], ey = P[2 * s1 + 1] - P[2 * s + 1];
-            const double len = sqrt(ex * ex + ey * ey);
-            double *sd = s_side + (el * 4 + s) * 3;
-            sd[0] = ey / len;
-            sd[1] = -(ex / len);
-            sd[2] = side_scale(P, s, x[0]);
+        } else {  // sides l, l + N + 1, ...: outward normal of the edge P_s -> P_s+1 and its Jacobian
+            for (int s = l; s < 4; s += NF) {
+                const int s1 = (s + 1) & 3;
+                const double ex = P[2 * s1] - P[2 * s], ey = P[2 * s1 + 1] - P[2 * s + 1];
+                const double len = sqrt(ex * ex + ey * ey);
+                double *sd = s_side + (el * 4 + s) * 3;
+                sd[0] = ey / len;
+                sd[1] = -(ex / len);
+                sd[2] = side_scale(P, s, x[0]);
+            }
         }
         const double *tq = s_q + el * 3 * NE + f * NE;
         double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
@@ -594,14 +596,16 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
             const double eta = x64[l];
             met[2 * NF + l] = (float)(((1 - eta) * (P[2] - P[0]) + (1 + eta) * (P[4] - P[6])) / 4);
             met[3 * NF + l] = (float)(((1 - eta) * (P[3] - P[1]) + (1 + eta) * (P[5] - P[7])) / 4);
-        } else if (l < 4) {
-            const int s = l, s1 = (s + 1) & 3;
-            const double ex = P[2 * s1] - P[2 * s], ey = P[2 * s1 + 1] - P[2 * s + 1];
-            const double len = sqrt(ex * ex + ey * ey);
-            float *sd = s_side + (el * 4 + s) * 3;
-            sd[0] = (float)(ey / len);
-            sd[1] = (float)(-(ex / len));
-            sd[2] = (float)side_scale(P, s, x64[0]);
+        } else {  // sides l, l + N + 1, ... (N + 1 lanes cover the 4 sides)
+            for (int s = l; s < 4; s += NF) {
+                const int s1 = (s + 1) & 3;
+                const double ex = P[2 * s1] - P[2 * s], ey = P[2 * s1 + 1] - P[2 * s + 1];
+                const double len = sqrt(ex * ex + ey * ey);
+                float *sd = s_side + (el * 4 + s) * 3;
+                sd[0] = (float)(ey / len);
+                sd[1] = (float)(-(ex / len));
+                sd[2] = (float)side_scale(P, s, x64[0]);
+            }
         }
         const float *tq = s_q + el * 3 * NE + f * NE;
         float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
@@ -1041,6 +1045,76 @@ mortar_kernel(const MortarFace *__restrict__ faces, int64_t n_faces, const doubl
 
 // ---------------------------------------------------------------------------------------------
 // Launchers: dispatch on the (uniform) order N.
+// ---------------------------------------------------------------------------------------------
+// Solution transfer after an order change, one workgroup per element: a copy when the order is
+// unchanged, else Element2D_t::interpolate_from for the same geometry (src/entities/
+// Element2D_t.cpp:464-579): the old element's node when both coordinates match a node (almost_
+// equal, the last matching index wins), a 1D barycentric interpolation along the other coordinate
+// when one matches, else the tensor barycentric interpolation.
+__global__ void __launch_bounds__(256)
+transfer_kernel(const double *__restrict__ q_old, double *__restrict__ q_new,
+                const TransferRec *__restrict__ recs, const double *__restrict__ tab1d) {
+    const TransferRec R = recs[blockIdx.x];
+    const int No = R.N_old, Nn = R.N_new, nfo = No + 1, nf = Nn + 1, neo = nfo * nfo, ne = nf * nf;
+    const double *src = q_old + R.old_off;
+    double *dst = q_new + R.new_off;
+    if (No == Nn) {
+        for (int r = threadIdx.x; r < 3 * ne; r += 256) dst[r] = src[r];
+        return;
+    }
+    const double *xo = tab1d + (3 * No + 0) * kTab1dStride, *lo = tab1d + (3 * No + 2) * kTab1dStride;
+    const double *xn = tab1d + (3 * Nn + 0) * kTab1dStride;
+    for (int r = threadIdx.x; r < 3 * ne; r += 256) {
+        const int f = r / ne, node = r % ne, i = node / nf, j = node % nf;
+        const double X = xn[i], Y = xn[j];
+        int col = -1, row = -1;
+        for (int m = 0; m <= No; ++m) {
+            if (ulp2_equal(X, xo[m])) col = m;
+            if (ulp2_equal(Y, xo[m])) row = m;
+        }
+        const double *sf = src + f * neo;
+        double v;
+        if (row >= 0 && col >= 0) {
+            v = sf[col * nfo + row];
+        } else if (row >= 0) {
+            double num = 0.0, den = 0.0;
+            for (int m = 0; m <= No; ++m) {
+                const double t = lo[m] / (X - xo[m]);
+                num += t * sf[m * nfo + row];
+                den += t;
+            }
+            v = num / den;
+        } else if (col >= 0) {
+            double num = 0.0, den = 0.0;
+            for (int n = 0; n <= No; ++n) {
+                const double t = lo[n] / (Y - xo[n]);
+                num += t * sf[col * nfo + n];
+                den += t;
+            }
+            v = num / den;
+        } else {
+            double dx = 0.0, dy = 0.0;
+            for (int k = 0; k <= No; ++k) {
+                dx += lo[k] / (X - xo[k]);
+                dy += lo[k] / (Y - xo[k]);
+            }
+            v = 0.0;
+            for (int m = 0; m <= No; ++m) {
+                const double tx = lo[m] / ((X - xo[m]) * dx);
+                for (int n = 0; n <= No; ++n) v += sf[m * nfo + n] * tx * (lo[n] / ((Y - xo[n]) * dy));
+            }
+        }
+        dst[r] = v;
+    }
+}
+
+hipError_t launch_transfer(const double *q_old, double *q_new, const TransferRec *recs, int64_t K,
+                           const double *tab1d, hipStream_t s) {
+    if (K <= 0) return hipSuccess;
+    hipLaunchKernelGGL(transfer_kernel, dim3((unsigned)K), dim3(256), 0, s, q_old, q_new, recs, tab1d);
+    return hipGetLastError();
+}
+
 #define NDG_FOR_ORDERS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
 bool order_supported(int N) { return N >= 1 && N <= kMaxOrder; }
 

@@ -72,6 +72,14 @@ struct ndg_mesh {
     int graph_precision = -1;
     std::vector<hipStream_t> side_streams;   // branches of the parallel capture (default priority)
     std::vector<hipEvent_t> side_events;     // [0] fork, [1 + g] join of group g
+    // Host topology, kept for rebuilding the device mesh when orders change (adaptation).
+    std::vector<double> host_verts;          // [K][4][2], caller's order
+    std::vector<int64_t> link;               // exterior side of each element side (see ndg_mesh_create)
+    std::vector<char> periodic_link;
+    std::vector<BcSide> bc_base;             // boundary (ghost) records
+    std::vector<char> ghost_zero;            // per record: ghost storage resized to zero by an adaptation
+    int initial_N = 0, maximum_N = 0, max_split_level = 0;
+    int64_t adapt_interval = 0;
 };
 
 extern "C" ndg_status ndg_last_error(char *buf, size_t buf_len) {
@@ -182,20 +190,25 @@ ndg_status device_upload(T **dst, const std::vector<T> &src, const char *what) {
     return NDG_OK;
 }
 
-void free_mesh(ndg_mesh *m) {
-    if (!m) return;
-    DeviceMesh &d = m->d;
+void free_device(DeviceMesh &d) {
     std::vector<void *> ptrs = {(void *)d.verts, (void *)d.side_nb, (void *)d.bc, (void *)d.faces,
                                 (void *)d.side_flux, (void *)d.side_flux32, (void *)d.tab1d, (void *)d.q,
                                 (void *)d.g, (void *)d.trace[0], (void *)d.trace[1], (void *)d.rhs,
-                                (void *)d.scratch, (void *)m->dt_dev, (void *)m->est_dev, (void *)m->pulses_dev,
-                                (void *)d.qh, (void *)d.ql, (void *)d.g32, (void *)d.trace32[0],
-                                (void *)d.trace32[1], (void *)d.clock};
+                                (void *)d.scratch, (void *)d.qh, (void *)d.ql, (void *)d.g32,
+                                (void *)d.trace32[0], (void *)d.trace32[1], (void *)d.clock};
     for (const Group &G : d.groups) {
         ptrs.push_back(G.tab);
         ptrs.push_back(G.tab32);
     }
     for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    d = DeviceMesh();
+}
+
+void free_mesh(ndg_mesh *m) {
+    if (!m) return;
+    free_device(m->d);
+    for (void *p : {(void *)m->dt_dev, (void *)m->est_dev, (void *)m->pulses_dev})
         if (p) (void)hipFree(p);
     for (hipGraphExec_t &x : m->step_exec)
         if (x) (void)hipGraphExecDestroy(x);
@@ -217,6 +230,175 @@ bool ghost_copy_is_close(const Tables &t, int N) {
         if (std::fabs(coord - x[N - j]) > 1e-13 || std::fabs(w[j] / w[N - j] - 1.0) > 1e-13) return false;
     }
     return true;
+}
+
+}  // namespace
+
+namespace {
+
+// Builds the device mesh for the given element orders from the host topology kept in m: order
+// groups, connectivity (group-local neighbour codes, boundary records with the ghost scale zeroed
+// where an adaptation resized the ghost, mortar records and faces between different orders),
+// vertices, tables, and the state / trace buffers (uninitialised). Sets m->order, m->dev_of,
+// m->host_off, m->n_dofs. The clock is not allocated here.
+ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMesh &d) {
+    const Tables &tab = m->tables->t;
+    const int64_t K = (int64_t)order.size();
+    const std::vector<int64_t> &link = m->link;
+    const std::vector<char> &periodic_link = m->periodic_link;
+    auto corner = [&](int64_t e, int k) {
+        const int kk = k & 3;
+        return v2{m->host_verts[8 * e + 2 * kk], m->host_verts[8 * e + 2 * kk + 1]};
+    };
+    d = DeviceMesh();
+    d.K = K;
+    m->order = order;
+
+    // Order groups: elements sorted by N (stable, so each group keeps the caller's order, e.g. a
+    // space-filling curve).
+    std::vector<int64_t> perm((size_t)K);
+    for (int64_t e = 0; e < K; ++e) perm[e] = e;
+    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return order[a] < order[b]; });
+    m->dev_of.assign((size_t)K, 0);
+    for (int64_t k = 0; k < K; ++k) m->dev_of[perm[k]] = k;
+    std::vector<int> group_of_N(kMaxOrder + 1, -1);
+    for (int64_t k = 0; k < K; ++k) {
+        const int N = order[perm[k]];
+        if (group_of_N[N] < 0) {
+            Group G;
+            G.N = N;
+            G.e_begin = k;
+            if (!d.groups.empty()) {
+                const Group &H = d.groups.back();
+                G.q_off = H.q_off + 3 * (int64_t)(H.N + 1) * (H.N + 1) * H.K;
+                G.tr_off = H.tr_off + 12 * (int64_t)(H.N + 1) * H.K;
+            }
+            group_of_N[N] = (int)d.groups.size();
+            d.groups.push_back(G);
+        }
+        ++d.groups[group_of_N[N]].K;
+    }
+    {
+        const Group &H = d.groups.back();
+        d.n_q = H.q_off + 3 * (int64_t)(H.N + 1) * (H.N + 1) * H.K;
+        d.n_tr = H.tr_off + 12 * (int64_t)(H.N + 1) * H.K;
+    }
+    d.N_min = d.groups.front().N;
+    d.N_max = d.groups.back().N;
+    for (const Group &G : d.groups)
+        if (!ghost_copy_is_close(tab, G.N))
+            return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: boundary ghost projection is not a reversed copy for "
+                                             "this order");
+    m->host_off.assign((size_t)K, 0);
+    int64_t acc = 0;
+    for (int64_t e = 0; e < K; ++e) {
+        m->host_off[e] = acc;
+        acc += (int64_t)(order[e] + 1) * (order[e] + 1);
+    }
+    m->n_dofs = acc;
+    auto group_of = [&](int64_t e) -> const Group & { return d.groups[group_of_N[order[e]]]; };
+    auto local_of = [&](int64_t e) { return m->dev_of[e] - group_of(e).e_begin; };
+    auto trace_at = [&](int64_t e, int s) {
+        const int64_t nf = order[e] + 1;
+        return group_of(e).tr_off + local_of(e) * 12 * nf + s * 3 * nf;
+    };
+
+    // Boundary records; a ghost whose storage an adaptation resized has a zero scale.
+    std::vector<BcSide> bcs = m->bc_base;
+    for (size_t b = 0; b < bcs.size(); ++b)
+        if (m->ghost_zero[b]) bcs[b].sf = 0.0;
+
+    // Device connectivity. Sides whose exterior state has another order are non-conforming: they
+    // get a mortar record, and one mortar face per pair (two-sided for a neighbour, one-sided for
+    // a periodic donor) fills their flux.
+    std::vector<int32_t> side_nb((size_t)(4 * K));
+    std::vector<MortarFace> faces;
+    std::vector<int64_t> mortar_rec((size_t)(4 * K), -1);
+    int64_t n_slot = 0;
+    auto add_mortar_side = [&](int64_t es) {
+        BcSide rec{};
+        rec.kind = kKindMortar;
+        rec.slot = n_slot;
+        n_slot += 3 * (int64_t)(order[es / 4] + 1);
+        mortar_rec[es] = (int64_t)bcs.size();
+        bcs.push_back(rec);
+        return rec.slot;
+    };
+    for (int64_t e = 0; e < K; ++e)
+        for (int s = 0; s < 4; ++s) {
+            const int64_t es = 4 * e + s, l = link[es];
+            if (l < 0 || order[l / 4] == order[e]) continue;
+            const int64_t r = l / 4;
+            const int rs = (int)(l % 4);
+            const bool periodic = periodic_link[es] != 0;
+            if (!periodic && r < e) continue;  // the face belongs to its first element
+            MortarFace fc{};
+            fc.NL = order[e];
+            fc.NR = order[r];
+            fc.Nf = std::max(fc.NL, fc.NR);
+            fc.trL = trace_at(e, s);
+            fc.trR = trace_at(r, rs);
+            const v2 f0 = corner(e, s), f1 = corner(e, s + 1);
+            double tx, ty;
+            face_frame(f0, f1, f0, f1, fc.nx, fc.ny, tx, ty);
+            v2 en[4] = {f0, f1, f1, f0};
+            if (!periodic) {
+                en[2] = corner(r, rs);
+                en[3] = corner(r, rs + 1);
+            }
+            face_offsets(f0, f1, en, fc.offset, fc.scale);
+            v2 PL[4] = {corner(e, 0), corner(e, 1), corner(e, 2), corner(e, 3)};
+            fc.sfL = side_scale(PL, s);
+            fc.slotL = add_mortar_side(es);
+            fc.slotR = -1;
+            if (!periodic) {
+                v2 PR[4] = {corner(r, 0), corner(r, 1), corner(r, 2), corner(r, 3)};
+                fc.sfR = side_scale(PR, rs);
+                fc.slotR = add_mortar_side(l);
+            }
+            faces.push_back(fc);
+        }
+    for (int64_t e = 0; e < K; ++e)
+        for (int s = 0; s < 4; ++s) {
+            const int64_t es = 4 * e + s, l = link[es];
+            int32_t code;
+            if (mortar_rec[es] >= 0) code = -(int32_t)(1 + mortar_rec[es]);
+            else if (l < 0) code = (int32_t)l;
+            else code = (int32_t)(local_of(l / 4) * 4 + l % 4);
+            side_nb[4 * m->dev_of[e] + s] = code;
+        }
+
+    std::vector<double> verts((size_t)(8 * K));
+    for (int64_t e = 0; e < K; ++e)
+        for (int k = 0; k < 8; ++k) verts[8 * m->dev_of[e] + k] = m->host_verts[8 * e + k];
+    ndg_status st;
+    if ((st = device_upload(&d.verts, verts, "upload vertices")) != NDG_OK) return st;
+    if ((st = device_upload(&d.side_nb, side_nb, "upload connectivity")) != NDG_OK) return st;
+    if ((st = device_upload(&d.bc, bcs, "upload boundary records")) != NDG_OK) return st;
+    d.n_bc = (int64_t)bcs.size();
+    if ((st = device_upload(&d.faces, faces, "upload mortar faces")) != NDG_OK) return st;
+    d.n_faces = (int64_t)faces.size();
+    d.n_side_flux = n_slot;
+    if (n_slot > 0) HIP_TRY(hipMalloc((void **)&d.side_flux, sizeof(double) * n_slot), "hipMalloc side fluxes");
+    {
+        std::vector<double> t1((size_t)(3 * (kMaxOrder + 1) * kTab1dStride), 0.0);
+        for (int n = 0; n <= std::min(kMaxOrder, tab.N_max); ++n)
+            for (int i = 0; i <= n; ++i) {
+                t1[(3 * n + 0) * kTab1dStride + i] = tab.tab[NDG_TABLE_NODES][n][i];
+                t1[(3 * n + 1) * kTab1dStride + i] = tab.tab[NDG_TABLE_WEIGHTS][n][i];
+                t1[(3 * n + 2) * kTab1dStride + i] = tab.tab[NDG_TABLE_BARYCENTRIC_WEIGHTS][n][i];
+            }
+        if ((st = device_upload(&d.tab1d, t1, "upload 1D tables")) != NDG_OK) return st;
+    }
+    for (Group &G : d.groups)
+        if ((st = device_upload(&G.tab, pack_tables(tab, G.N), "upload tables")) != NDG_OK) return st;
+    HIP_TRY(hipMalloc((void **)&d.q, sizeof(double) * d.n_q), "hipMalloc q");
+    HIP_TRY(hipMalloc((void **)&d.g, sizeof(double) * d.n_q), "hipMalloc g");
+    HIP_TRY(hipMalloc((void **)&d.trace[0], sizeof(double) * d.n_tr), "hipMalloc traces");
+    HIP_TRY(hipMalloc((void **)&d.trace[1], sizeof(double) * d.n_tr), "hipMalloc traces");
+    d.scratch_len = 1024;
+    HIP_TRY(hipMalloc((void **)&d.scratch, sizeof(double) * d.scratch_len), "hipMalloc scratch");
+    return NDG_OK;
 }
 
 }  // namespace
@@ -338,156 +520,26 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
     m->tol_min = desc->tolerance_min;
     m->tol_max = desc->tolerance_max;
     m->n_faces = n_faces;
-    m->order = order;
-    DeviceMesh &d = m->d;
-    d.K = K;
-
-    // Order groups: elements sorted by N (stable, so each group keeps the caller's order, e.g. a
-    // space-filling curve).
-    std::vector<int64_t> perm((size_t)K);
-    for (int64_t e = 0; e < K; ++e) perm[e] = e;
-    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return order[a] < order[b]; });
-    m->dev_of.assign((size_t)K, 0);
-    for (int64_t k = 0; k < K; ++k) m->dev_of[perm[k]] = k;
-    std::vector<int> group_of_N(kMaxOrder + 1, -1);
-    for (int64_t k = 0; k < K; ++k) {
-        const int N = order[perm[k]];
-        if (group_of_N[N] < 0) {
-            Group G;
-            G.N = N;
-            G.e_begin = k;
-            if (!d.groups.empty()) {
-                const Group &H = d.groups.back();
-                G.q_off = H.q_off + 3 * (int64_t)(H.N + 1) * (H.N + 1) * H.K;
-                G.tr_off = H.tr_off + 12 * (int64_t)(H.N + 1) * H.K;
-            }
-            group_of_N[N] = (int)d.groups.size();
-            d.groups.push_back(G);
-        }
-        ++d.groups[group_of_N[N]].K;
-    }
-    {
-        const Group &H = d.groups.back();
-        d.n_q = H.q_off + 3 * (int64_t)(H.N + 1) * (H.N + 1) * H.K;
-        d.n_tr = H.tr_off + 12 * (int64_t)(H.N + 1) * H.K;
-    }
-    d.N_min = d.groups.front().N;
-    d.N_max = d.groups.back().N;
-    for (const Group &G : d.groups)
-        if (!ghost_copy_is_close(tables->t, G.N))
-            return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: boundary ghost projection is not a reversed copy for "
-                                             "this order");
-    m->host_off.assign((size_t)K, 0);
-    int64_t acc = 0;
-    for (int64_t e = 0; e < K; ++e) {
-        m->host_off[e] = acc;
-        acc += (int64_t)(order[e] + 1) * (order[e] + 1);
-    }
-    m->n_dofs = acc;
-    auto group_of = [&](int64_t e) -> const Group & { return d.groups[group_of_N[order[e]]]; };
-    auto local_of = [&](int64_t e) { return m->dev_of[e] - group_of(e).e_begin; };
-    auto trace_at = [&](int64_t e, int s) {
-        const int64_t nf = order[e] + 1;
-        return group_of(e).tr_off + local_of(e) * 12 * nf + s * 3 * nf;
-    };
-
-    // Device connectivity. Sides whose exterior state has another order are non-conforming: they
-    // get a mortar record, and one mortar face per pair (two-sided for a neighbour, one-sided for
-    // a periodic donor) fills their flux.
-    std::vector<int32_t> side_nb((size_t)(4 * K));
-    std::vector<MortarFace> faces;
-    std::vector<int64_t> mortar_rec((size_t)(4 * K), -1);
-    int64_t n_slot = 0;
-    auto add_mortar_side = [&](int64_t es) {
-        BcSide rec{};
-        rec.kind = kKindMortar;
-        rec.slot = n_slot;
-        n_slot += 3 * (int64_t)(order[es / 4] + 1);
-        mortar_rec[es] = (int64_t)bcs.size();
-        bcs.push_back(rec);
-        return rec.slot;
-    };
-    for (int64_t e = 0; e < K; ++e)
-        for (int s = 0; s < 4; ++s) {
-            const int64_t es = 4 * e + s, l = link[es];
-            if (l < 0 || order[l / 4] == order[e]) continue;
-            const int64_t r = l / 4;
-            const int rs = (int)(l % 4);
-            const bool periodic = periodic_link[es] != 0;
-            if (!periodic && r < e) continue;  // the face belongs to its first element
-            MortarFace fc{};
-            fc.NL = order[e];
-            fc.NR = order[r];
-            fc.Nf = std::max(fc.NL, fc.NR);
-            fc.trL = trace_at(e, s);
-            fc.trR = trace_at(r, rs);
-            const v2 f0 = corner(e, s), f1 = corner(e, s + 1);
-            double tx, ty;
-            face_frame(f0, f1, f0, f1, fc.nx, fc.ny, tx, ty);
-            v2 en[4] = {f0, f1, f1, f0};
-            if (!periodic) {
-                en[2] = corner(r, rs);
-                en[3] = corner(r, rs + 1);
-            }
-            face_offsets(f0, f1, en, fc.offset, fc.scale);
-            v2 PL[4] = {corner(e, 0), corner(e, 1), corner(e, 2), corner(e, 3)};
-            fc.sfL = side_scale(PL, s);
-            fc.slotL = add_mortar_side(es);
-            fc.slotR = -1;
-            if (!periodic) {
-                v2 PR[4] = {corner(r, 0), corner(r, 1), corner(r, 2), corner(r, 3)};
-                fc.sfR = side_scale(PR, rs);
-                fc.slotR = add_mortar_side(l);
-            }
-            faces.push_back(fc);
-        }
-    for (int64_t e = 0; e < K; ++e)
-        for (int s = 0; s < 4; ++s) {
-            const int64_t es = 4 * e + s, l = link[es];
-            int32_t code;
-            if (mortar_rec[es] >= 0) code = -(int32_t)(1 + mortar_rec[es]);
-            else if (l < 0) code = (int32_t)l;
-            else code = (int32_t)(local_of(l / 4) * 4 + l % 4);
-            side_nb[4 * m->dev_of[e] + s] = code;
-        }
-
-    HIP_TRY(hipStreamCreateWithFlags(&m->own_stream, hipStreamNonBlocking), "hipStreamCreate");
-    m->stream = m->own_stream;
-    std::vector<double> verts((size_t)(8 * K));
+    m->initial_N = desc->initial_N;
+    m->maximum_N = desc->maximum_N;
+    m->max_split_level = desc->max_split_level;
+    m->link = std::move(link);
+    m->periodic_link = std::move(periodic_link);
+    m->bc_base = std::move(bcs);
+    m->ghost_zero.assign(m->bc_base.size(), 0);
+    m->host_verts.resize((size_t)(8 * K));
     for (int64_t e = 0; e < K; ++e)
         for (int k = 0; k < 4; ++k) {
             const v2 p = corner(e, k);
-            verts[8 * m->dev_of[e] + 2 * k] = p.x;
-            verts[8 * m->dev_of[e] + 2 * k + 1] = p.y;
+            m->host_verts[8 * e + 2 * k] = p.x;
+            m->host_verts[8 * e + 2 * k + 1] = p.y;
         }
-    ndg_status st;
-    if ((st = device_upload(&d.verts, verts, "upload vertices")) != NDG_OK) return st;
-    if ((st = device_upload(&d.side_nb, side_nb, "upload connectivity")) != NDG_OK) return st;
-    if ((st = device_upload(&d.bc, bcs, "upload boundary records")) != NDG_OK) return st;
-    d.n_bc = (int64_t)bcs.size();
-    if ((st = device_upload(&d.faces, faces, "upload mortar faces")) != NDG_OK) return st;
-    d.n_faces = (int64_t)faces.size();
-    d.n_side_flux = n_slot;
-    if (n_slot > 0) HIP_TRY(hipMalloc((void **)&d.side_flux, sizeof(double) * n_slot), "hipMalloc side fluxes");
-    if (!faces.empty()) {
-        std::vector<double> t1((size_t)(3 * (kMaxOrder + 1) * kTab1dStride), 0.0);
-        for (int n = 0; n <= std::min(kMaxOrder, tables->t.N_max); ++n)
-            for (int i = 0; i <= n; ++i) {
-                t1[(3 * n + 0) * kTab1dStride + i] = tables->t.tab[NDG_TABLE_NODES][n][i];
-                t1[(3 * n + 1) * kTab1dStride + i] = tables->t.tab[NDG_TABLE_WEIGHTS][n][i];
-                t1[(3 * n + 2) * kTab1dStride + i] = tables->t.tab[NDG_TABLE_BARYCENTRIC_WEIGHTS][n][i];
-            }
-        if ((st = device_upload(&d.tab1d, t1, "upload 1D tables")) != NDG_OK) return st;
-    }
-    for (Group &G : d.groups)
-        if ((st = device_upload(&G.tab, pack_tables(tables->t, G.N), "upload tables")) != NDG_OK) return st;
-    HIP_TRY(hipMalloc((void **)&d.q, sizeof(double) * d.n_q), "hipMalloc q");
-    HIP_TRY(hipMalloc((void **)&d.g, sizeof(double) * d.n_q), "hipMalloc g");
-    HIP_TRY(hipMalloc((void **)&d.trace[0], sizeof(double) * d.n_tr), "hipMalloc traces");
-    HIP_TRY(hipMalloc((void **)&d.trace[1], sizeof(double) * d.n_tr), "hipMalloc traces");
-    d.scratch_len = 1024;
-    HIP_TRY(hipMalloc((void **)&d.scratch, sizeof(double) * d.scratch_len), "hipMalloc scratch");
+    HIP_TRY(hipStreamCreateWithFlags(&m->own_stream, hipStreamNonBlocking), "hipStreamCreate");
+    m->stream = m->own_stream;
     HIP_TRY(hipMalloc((void **)&m->dt_dev, sizeof(double)), "hipMalloc dt");
+    ndg_status st = build_device(m.get(), order, m->d);
+    if (st != NDG_OK) return st;
+    DeviceMesh &d = m->d;
     HIP_TRY(hipMalloc((void **)&d.clock, sizeof(double) * 4), "hipMalloc clock");
     HIP_TRY(hipMemsetAsync(d.clock, 0, sizeof(double) * 4, m->stream), "hipMemset clock");
     HIP_TRY(hipMemsetAsync(d.q, 0, sizeof(double) * d.n_q, m->stream), "hipMemset q");
@@ -888,6 +940,109 @@ extern "C" ndg_status ndg_mesh_tuning(const ndg_mesh *mesh, int32_t *orders, int
     return NDG_OK;
 }
 
+extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split_level, int64_t *n_raised) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_adapt: null mesh");
+    DeviceMesh &d = mesh->d;
+    const int64_t K = d.K;
+    std::vector<double> err((size_t)K), sig((size_t)K);
+    std::vector<int32_t> ref((size_t)K), coa((size_t)K);
+    ndg_status st = ndg_estimate_error(mesh, err.data(), sig.data(), ref.data(), coa.data());
+    if (st != NDG_OK) return st;
+    // Element2D_t.cpp:802-812 (elements are never split here, so their split level is 0)
+    for (int64_t e = 0; e < K; ++e)
+        if (ref[e] && sig[e] < 1.0 && 0 < max_split_level)
+            return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: an element would be split (h-refinement is not provided; "
+                                             "use max_split_level 0)");
+    std::vector<int32_t> order = mesh->order;
+    int64_t raised = 0;
+    for (int64_t e = 0; e < K; ++e)
+        if (ref[e] && sig[e] >= 1.0 && order[e] + 2 <= N_max) {
+            order[e] += 2;
+            ++raised;
+        }
+    if (n_raised) *n_raised = raised;
+    if (raised == 0) return NDG_OK;
+    for (int64_t e = 0; e < K; ++e)
+        if (order[e] > kMaxOrder || order[e] > mesh->tables->t.N_max)
+            return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a raised order exceeds the compiled range or the tables");
+    // adjust_boundaries (Mesh2D_t.cpp:7482-7504): the ghost of a raised element is resized, which
+    // zeroes its edge scaling factor (Element2D_t.cpp:735-741).
+    for (int64_t es = 0; es < 4 * K; ++es) {
+        const int64_t l = mesh->link[es];
+        if (l < 0 && l != INT64_MIN && order[es / 4] != mesh->order[es / 4]) mesh->ghost_zero[(size_t)(-l - 1)] = 1;
+    }
+    const int precision = d.precision;
+    if (precision == NDG_PRECISION_FP32_COMPENSATED) {
+        if ((st = ndg_mesh_set_precision(mesh, NDG_PRECISION_FP64)) != NDG_OK) return st;
+    }
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_adapt sync");
+    drop_graphs(mesh);
+    const std::vector<int32_t> old_order = mesh->order;
+    const std::vector<int64_t> old_dev = mesh->dev_of, old_host_off = mesh->host_off;
+    const int64_t old_dofs = mesh->n_dofs;
+    DeviceMesh nd;
+    st = build_device(mesh, order, nd);
+    if (st != NDG_OK) {
+        free_device(nd);
+        mesh->order = old_order;
+        mesh->dev_of = old_dev;
+        mesh->host_off = old_host_off;
+        mesh->n_dofs = old_dofs;
+        return st;
+    }
+    auto offset = [](const DeviceMesh &dm, int N, int64_t dev) -> int64_t {
+        for (const Group &G : dm.groups)
+            if (G.N == N) return G.q_off + (dev - G.e_begin) * 3 * (int64_t)(N + 1) * (N + 1);
+        return -1;
+    };
+    std::vector<TransferRec> recs((size_t)K);
+    for (int64_t e = 0; e < K; ++e) {
+        TransferRec &R = recs[(size_t)mesh->dev_of[e]];
+        R.old_off = offset(d, old_order[e], old_dev[e]);
+        R.new_off = offset(nd, order[e], mesh->dev_of[e]);
+        R.N_old = old_order[e];
+        R.N_new = order[e];
+    }
+    TransferRec *recs_dev = nullptr;
+    st = device_upload(&recs_dev, recs, "upload transfer records");
+    if (st == NDG_OK) {
+        const hipError_t e1 = launch_transfer(d.q, nd.q, recs_dev, K, nd.tab1d, mesh->stream);
+        const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(mesh->stream) : e1;
+        if (e2 != hipSuccess) st = hip_fail(e2, "transfer kernel");
+    }
+    if (recs_dev) (void)hipFree(recs_dev);
+    if (st != NDG_OK) {
+        free_device(nd);
+        return st;
+    }
+    nd.clock = d.clock;
+    d.clock = nullptr;
+    nd.wave_speed = d.wave_speed;
+    nd.cur = 0;
+    free_device(d);
+    d = nd;
+    HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * d.n_q, mesh->stream), "hipMemset g");
+    HIP_TRY(launch_traces(d, mesh->stream), "traces kernel");
+    if (precision == NDG_PRECISION_FP32_COMPENSATED) {
+        if ((st = ndg_mesh_set_precision(mesh, precision)) != NDG_OK) return st;
+    }
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_adapt sync");
+    return NDG_OK;
+}
+
+extern "C" ndg_status ndg_mesh_orders(const ndg_mesh *mesh, int32_t *orders) {
+    if (!mesh || !orders) return fail(NDG_ERR_INVALID, "ndg_mesh_orders: null argument");
+    std::copy(mesh->order.begin(), mesh->order.end(), orders);
+    return NDG_OK;
+}
+
+extern "C" ndg_status ndg_mesh_set_adaptivity(ndg_mesh *mesh, int64_t adaptivity_interval) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_mesh_set_adaptivity: null mesh");
+    if (adaptivity_interval < 0) return fail(NDG_ERR_INVALID, "ndg_mesh_set_adaptivity: negative interval");
+    mesh->adapt_interval = adaptivity_interval;
+    return NDG_OK;
+}
+
 extern "C" ndg_status ndg_solve(ndg_mesh *mesh, double cfl, double t_end, double *t_final, int64_t *n_steps) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_solve: null mesh");
     double dt0 = 0.0;
@@ -906,6 +1061,16 @@ extern "C" ndg_status ndg_solve(ndg_mesh *mesh, double cfl, double t_end, double
             return st;
         }
         time += dt;
+        // include/solvers/Solver2D_t.tpp:88-96: estimate_error + adapt every adaptivity_interval
+        // steps, with the tables' N_max; Δt then follows the new orders.
+        if (mesh->adapt_interval > 0 && steps % mesh->adapt_interval == 0) {
+            int64_t raised = 0;
+            if ((st = ndg_adapt(mesh, mesh->tables->t.N_max, mesh->max_split_level, &raised)) != NDG_OK) return st;
+            if (raised > 0) {
+                if ((st = ndg_get_delta_t(mesh, cfl, &dt0)) != NDG_OK) return st;
+                HIP_TRY(launch_clock_begin(mesh->d, time, 1, dt0, mesh->stream), "clock kernel");
+            }
+        }
     }
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_solve sync");
     if (t_final) *t_final = time;

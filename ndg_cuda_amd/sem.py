@@ -100,8 +100,6 @@ class Mesh2D_t:
                  max_split_level: int, adaptivity_interval: int, load_balancing_interval: int,
                  tolerance_min: float, tolerance_max: float, load_balancing_threshold: float, NDG: NDG_t,
                  worker_weights=None):
-        if adaptivity_interval:
-            raise NDGError(_abi.NDG_ERR_UNSUPPORTED, "adaptivity is not available in this build")
         self.initial_N_ = int(initial_N)
         self.maximum_N_ = int(maximum_N)
         self.n_interpolation_points_ = int(n_interpolation_points)
@@ -130,12 +128,31 @@ class Mesh2D_t:
                                   ctypes.byref(nmax)))
         self.n_elements_ = ne.value
         self.n_faces_ = nf.value
+        self._refresh()
+        if adaptivity_interval:
+            check(lib().ndg_mesh_set_adaptivity(h, int(adaptivity_interval)))
+
+    def _refresh(self):
+        """Re-reads the sizes and element orders (they change with adaptation)."""
+        ne, nf, nd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        nmin, nmax = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().ndg_mesh_info(self._h, ctypes.byref(ne), ctypes.byref(nf), ctypes.byref(nd),
+                                  ctypes.byref(nmin), ctypes.byref(nmax)))
         self.n_dofs_per_field_ = nd.value
         self.N_ = nmin.value
         self.N_max_ = nmax.value
-        # element orders in the caller's element order
-        self.orders_ = (np.full(self.n_elements_, self.initial_N_, dtype=np.int32) if quad_N is None
-                        else quad_N.copy())
+        self.orders_ = np.empty(self.n_elements_, dtype=np.int32)
+        check(lib().ndg_mesh_orders(self._h, ptr(self.orders_)))
+
+    def adapt(self, N_max=None, max_split_level=None):
+        """Mesh2D_t::adapt without splitting: estimate, then raise by 2 the order of the elements
+        that would p-refine. Returns the number of raised elements."""
+        n = ctypes.c_int64()
+        check(lib().ndg_adapt(self._h, int(self._ndg.N_max_ if N_max is None else N_max),
+                              int(self.max_split_level_ if max_split_level is None else max_split_level),
+                              ctypes.byref(n)))
+        self._refresh()
+        return n.value
 
     @property
     def handle(self):
@@ -257,6 +274,7 @@ class Solver2D_t:
         check(lib().ndg_solve(mesh.handle, self.CFL_, self.output_times_[-1], ctypes.byref(t_final),
                               ctypes.byref(steps)))
         self.time_, self.n_steps_ = t_final.value, steps.value
+        mesh._refresh()
         return steps.value
 
     def step(self, mesh: Mesh2D_t, t: float, dt: float):

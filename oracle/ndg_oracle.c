@@ -1200,6 +1200,150 @@ void o_estimate_error(o_mesh *m, double *error, double *sigma, int32_t *refine, 
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* p-adaptation: src/meshes/Mesh2D_t.cpp:1644-1706 (the branch without splitting), :5026-5049,  */
+/* :7482-7538; Element2D_t.cpp:464-579, :735-741, :802-812                                      */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Element2D_t.cpp:464-579 interpolate_from (same geometry, other order): at each new node, the
+ * old element's exact node when both coordinates match (almost_equal, the last matching index
+ * wins), a 1D barycentric interpolation along the other coordinate when one matches, else the
+ * tensor barycentric interpolation. p, u, v and G are interpolated; the RK register starts at 0. */
+static void interpolate_p(const o_tables *t, const o_elem *o, o_elem *e) {
+    const int No = o->N, N = e->N, nfo = No + 1, nf = N + 1;
+    const double *xo = t->tab[O_NODES][No], *lo = t->tab[O_BARY][No], *xn = t->tab[O_NODES][N];
+    const double *src[6] = {o->p, o->u, o->v, o->Gp, o->Gu, o->Gv};
+    double *dst[6] = {e->p, e->u, e->v, e->Gp, e->Gu, e->Gv};
+    for (int i = 0; i <= N; ++i)
+        for (int j = 0; j <= N; ++j) {
+            const double X = xn[i], Y = xn[j];
+            int col = -1, row = -1;
+            for (int m = 0; m <= No; ++m) {
+                if (almost_equal(X, xo[m])) col = m;
+                if (almost_equal(Y, xo[m])) row = m;
+            }
+            const int k = i * nf + j;
+            if (row != -1 && col != -1) {
+                for (int f = 0; f < 6; ++f) dst[f][k] = src[f][col * nfo + row];
+            } else if (row != -1) {
+                double num[6] = {0, 0, 0, 0, 0, 0}, den = 0.0;
+                for (int m = 0; m <= No; ++m) {
+                    const double tt = lo[m] / (X - xo[m]);
+                    for (int f = 0; f < 6; ++f) num[f] += tt * src[f][m * nfo + row];
+                    den += tt;
+                }
+                for (int f = 0; f < 6; ++f) dst[f][k] = num[f] / den;
+            } else if (col != -1) {
+                double num[6] = {0, 0, 0, 0, 0, 0}, den = 0.0;
+                for (int n = 0; n <= No; ++n) {
+                    const double tt = lo[n] / (Y - xo[n]);
+                    for (int f = 0; f < 6; ++f) num[f] += tt * src[f][col * nfo + n];
+                    den += tt;
+                }
+                for (int f = 0; f < 6; ++f) dst[f][k] = num[f] / den;
+            } else {
+                for (int f = 0; f < 6; ++f) dst[f][k] = 0.0;
+                double dx = 0.0, dy = 0.0;
+                for (int q = 0; q <= No; ++q) {
+                    dx += lo[q] / (X - xo[q]);
+                    dy += lo[q] / (Y - xo[q]);
+                }
+                for (int m = 0; m <= No; ++m) {
+                    const double tx = lo[m] / ((X - xo[m]) * dx);
+                    for (int n = 0; n <= No; ++n) {
+                        const double ty = lo[n] / ((Y - xo[n]) * dy);
+                        for (int f = 0; f < 6; ++f) dst[f][k] += src[f][m * nfo + n] * tx * ty;
+                    }
+                }
+            }
+        }
+}
+
+/* Element2D_t.cpp:735-741 resize_boundary_storage: the ghost's traces and edge scaling factor
+ * become zero vectors of the new length; nothing recomputes the scaling factor on this path. */
+static void ghost_resize(o_elem *g, int N) {
+    free(g->block);
+    g->N = N;
+    elem_alloc_ghost(g);
+}
+
+int64_t o_adapt(o_mesh *m, int N_max, int max_split_level) {
+    /* Element2D_t.cpp:802-812: h-refinement (not restated) is refine && sigma < 1 && split level
+     * (0 here: elements are never split) < max_split_level */
+    for (int64_t i = 0; i < m->K; ++i)
+        if (m->el[i].refine && m->el[i].sigma < 1.0 && 0 < max_split_level) return -1;
+    int64_t count = 0;
+    for (int64_t i = 0; i < m->K; ++i) {
+        o_elem *o = &m->el[i];
+        if (!(o->refine && o->sigma >= 1.0 && o->N + 2 <= N_max)) continue;
+        o_elem e = *o;
+        e.N = o->N + 2;
+        elem_alloc_domain(&e);
+        interpolate_p(m->t, o, &e);
+        v2 P[4];
+        for (int k = 0; k < 4; ++k) P[k] = m->xy[e.nodes[k]];
+        element_geometry(&e, P, m->t->tab[O_NODES][e.N]);
+        free(o->block);
+        *o = e;
+        ++count;
+    }
+    /* adjust_boundaries (Mesh2D_t.cpp:7482-7504) for the wall, symmetry, inflow, outflow lists */
+    int64_t *lists[4] = {m->wall, m->sym, m->in, m->out};
+    const int64_t counts[4] = {m->n_wall, m->n_sym, m->n_in, m->n_out};
+    for (int l = 0; l < 4; ++l)
+        for (int64_t k = 0; k < counts[l]; ++k) {
+            o_elem *g = &m->el[lists[l][k]];
+            int N = g->N;
+            for (int fk = 0; fk < g->nf[0]; ++fk) {
+                const o_face *f = &m->fa[g->faces[0][fk]];
+                const int side = f->elements[0] == lists[l][k];
+                const int Ns = m->el[f->elements[side]].N;
+                if (Ns > N) N = Ns;
+            }
+            if (N != g->N) ghost_resize(g, N);
+        }
+    /* adjust_interfaces (:7506-7520) */
+    for (int64_t k = 0; k < m->n_if; ++k) {
+        o_elem *g = &m->el[m->if_dest[k]];
+        const int N = m->el[m->if_origin[k]].N;
+        if (N != g->N) ghost_resize(g, N);
+    }
+    /* adjust_faces (:7522-7538) */
+    for (int64_t fi = 0; fi < m->F; ++fi) {
+        o_face *f = &m->fa[fi];
+        const int NL = m->el[f->elements[0]].N, NR = m->el[f->elements[1]].N;
+        const int N = NL > NR ? NL : NR;
+        if (N != f->N) {
+            free(f->block);
+            f->N = N;
+            face_alloc(f);
+        }
+    }
+    return count;
+}
+
+/* include/solvers/Solver2D_t.tpp:37-100 with adaptation: Δt from get_delta_t every step, the last
+ * step clipped to t_end, and every adapt_interval steps estimate_error + adapt. Returns the step
+ * count, or -1 if an adaptation would split an element. */
+int64_t o_solve_adaptive(o_mesh *m, double cfl, double t_end, int64_t adapt_interval, int N_max,
+                         int max_split_level, double *t_final) {
+    double time = 0.0;
+    int64_t steps = 0;
+    while (time < t_end) {
+        ++steps;
+        double dt = o_get_delta_t(m, cfl);
+        if (time + dt > t_end) dt = t_end - time;
+        o_step(m, time, dt);
+        time += dt;
+        if (adapt_interval > 0 && steps % adapt_interval == 0) {
+            o_estimate_error(m, NULL, NULL, NULL, NULL);
+            if (o_adapt(m, N_max, max_split_level) < 0) return -1;
+        }
+    }
+    if (t_final) *t_final = time;
+    return steps;
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* Element / face level entry points                                                           */
 /* ------------------------------------------------------------------------------------------ */
 

@@ -93,6 +93,15 @@ void o_run_steps(o_mesh *m, double cfl, int64_t n_steps, double *time_io);
 
 /* Pressure estimator per domain element (src/entities/Element2D_t.cpp:274-343). */
 void o_estimate_error(o_mesh *m, double *error, double *sigma, int32_t *refine, int32_t *coarsen);
+/* One adaptation without splitting (src/meshes/Mesh2D_t.cpp:1644-1706, p_adapt :5026-5049): from
+ * the last estimate, elements with refine && sigma >= 1 && N + 2 <= N_max are raised by 2 orders
+ * (solution interpolated, geometry recomputed); ghosts and faces follow (their ghost storage,
+ * including the edge scaling factor, becomes zero when its order changes). Returns the number of
+ * raised elements, or -1 (nothing changed) if an element would split. */
+int64_t o_adapt(o_mesh *m, int N_max, int max_split_level);
+/* The reference time loop with an adaptation every adapt_interval steps; -1 if one would split. */
+int64_t o_solve_adaptive(o_mesh *m, double cfl, double t_end, int64_t adapt_interval, int N_max,
+                         int max_split_level, double *t_final);
 
 /* ---- element / face level entry points (for known-answer and fixture checks) ---- */
 /* traces of one element: q = p,u,v each Nf*Nf; out = [4][3][Nf] */

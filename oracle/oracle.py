@@ -43,7 +43,10 @@ def lib():
             "o_set_state": ([vp, vp, vp, vp], None), "o_get_state": ([vp, vp, vp, vp], None),
             "o_get_rhs": ([vp, vp, vp, vp], None), "o_get_traces": ([vp, vp], None),
             "o_get_flux_traces": ([vp, vp], None), "o_get_delta_t": ([vp, f64], f64),
-            "o_compute_rhs": ([vp, f64], None), "o_mesh_set_wave_speed": ([vp, f64], None), "o_step": ([vp, f64, f64], None),
+            "o_compute_rhs": ([vp, f64], None), "o_mesh_set_wave_speed": ([vp, f64], None),
+            "o_adapt": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int64),
+            "o_solve_adaptive": ([vp, f64, f64, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_double)], ctypes.c_int64), "o_step": ([vp, f64, f64], None),
             "o_solve": ([vp, f64, f64, vp], i64), "o_run_steps": ([vp, f64, i64, vp], None),
             "o_estimate_error": ([vp, vp, vp, vp, vp], None),
             "o_element_traces": ([vp, i32, vp, vp, vp, vp], None),
@@ -165,6 +168,24 @@ class Mesh:
         t = ctypes.c_double(time)
         lib().o_run_steps(self.h, cfl, n_steps, ctypes.byref(t))
         return t.value
+
+    def adapt(self, N_max, max_split_level=0):
+        """One adaptation from the last estimate; returns the number of raised elements (-1: an
+        element would split) and refreshes the element orders."""
+        n = lib().o_adapt(self.h, int(N_max), int(max_split_level))
+        self._refresh()
+        return n
+
+    def solve_adaptive(self, cfl, t_end, adapt_interval, N_max, max_split_level=0):
+        tf = ctypes.c_double()
+        n = lib().o_solve_adaptive(self.h, cfl, t_end, int(adapt_interval), int(N_max), int(max_split_level),
+                                   ctypes.byref(tf))
+        self._refresh()
+        return n, tf.value
+
+    def _refresh(self):
+        self.n = lib().o_mesh_n_nodes_total(self.h)
+        lib().o_mesh_get_N(self.h, _p(self.N))
 
     def estimate_error(self):
         err, sig = np.empty(self.K), np.empty(self.K)

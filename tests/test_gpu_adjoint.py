@@ -187,7 +187,7 @@ def test_autograd_function(tables):
     gm.set_solution(*(t.detach().numpy() for t in q0))
     A.set_wave_speed(gm, 1.0)
     Jr, gp, gu, gv, gc = A.adjoint_gradient(gm, rec, dt, n_steps, 5, data)
-    assert float(J) == Jr
+    assert float(J.detach()) == Jr
     for t, g in zip(q0, (gp, gu, gv)):
         assert np.array_equal(t.grad.numpy(), 2.0 * g)
     assert float(c.grad) == 2.0 * gc

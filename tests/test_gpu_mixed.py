@@ -186,3 +186,29 @@ def test_graph_solve_config1(tables):
     assert solver.n_steps_ == steps == 289 and solver.time_ == t_cpu
     for g, r in zip(gm.get_solution(), om.get_state()):
         assert rel(g, r) < TOL
+
+
+@pytest.mark.parametrize("periodic,bc", [(True, (nd.BC_WALL,) * 4), (False, MIXED_BC)])
+def test_adaptive_solve_matches_oracle(tables, periodic, bc):
+    """The reference time loop with p-adaptation every 4 steps (estimate, raise by 2 where
+    refine && sigma >= 1, interpolate, ghosts and faces follow) against the oracle's restatement."""
+    say("adaptive solve", periodic, bc)
+    ndg, ot = tables
+    desc = nd.square_mesh(8, bc=bc, x_periodic=periodic, y_periodic=periodic, jitter=0.1, seed=7)
+    gm = nd.Mesh2D_t(desc, 3, 9, 5, 0, 4, 0, 1e-4, 1e-14, 0.0, nd.NDG_t(9, 5))
+    om = O.Mesh(O.Tables(9, 5), desc, 3, tol_min=1e-4, tol_max=1e-14)
+    if periodic:
+        gm.initial_conditions(PULSES)
+        om.initial_conditions(PULSES)
+    else:
+        gm.initial_conditions(None)
+        om.initial_conditions(None)
+    solver = nd.Solver2D_t(0.5, [0.2])
+    solver.solve(None, gm)
+    steps, t_cpu = om.solve_adaptive(0.5, 0.2, 4, 9, 0)
+    say("steps", steps, "orders", np.bincount(om.N))
+    assert solver.n_steps_ == steps and solver.time_ == t_cpu
+    assert np.array_equal(gm.orders_, om.N)
+    assert om.N.max() > 3
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL

@@ -70,6 +70,9 @@ CASES = [
     ("inflow", dict(bc=(nd.BC_INFLOW,) * 4), 7),
     ("outflow", dict(bc=(nd.BC_OUTFLOW,) * 4, jitter=0.1), 4),
     ("zero-ghost", dict(bc=(nd.BC_NONE,) * 4), 3),
+    # N + 1 < 4: fewer lanes per line than element sides
+    ("low-order-walls", dict(jitter=0.2), 1),
+    ("low-order-mixed-bc", dict(bc=MIXED_BC, jitter=0.1), 2),
 ]
 
 

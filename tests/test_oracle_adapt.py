@@ -1,0 +1,126 @@
+"""The oracle's p-adaptation (src/meshes/Mesh2D_t.cpp:1644-1706, :5026-5049, :7482-7538;
+Element2D_t.cpp:464-579, :735-741, :802-812) pinned by mathematics (CPU only).
+
+- interpolate_from to order N + 2 reproduces the old element polynomial at the new nodes (checked
+  against a numpy Legendre-Vandermonde evaluation, independent of the barycentric formula);
+- the decisions follow would_p_refine / would_h_refine (refine and sigma >= 1 and N + 2 <= N_max;
+  a split request makes the no-split branch refuse);
+- after adaptation the mesh is the same operator as a mesh built directly at the new orders with
+  the same state, except that ghosts whose order changed have zero storage (the reference
+  resizes them without recomputing the edge scaling factor): with every element raised, walls act
+  as zero ghosts, which a directly built mesh with BC_NONE reproduces exactly; periodic meshes are
+  unchanged in kind.
+"""
+import numpy as np
+import pytest
+
+import exact
+import ndg_cuda_amd as nd
+from oracle import oracle as O
+
+
+def smooth_state(desc, tables, N):
+    X, Y = exact.node_coordinates(desc, tables.get(O.O_NODES, N))
+    return [np.exp(0.7 * X) * np.cos(1.3 * Y), np.sin(X + 0.3 * Y), np.cos(2 * X * Y)]
+
+
+def legendre_eval(x_old, vals, x_new):
+    """Evaluates the degree-N tensor interpolant of vals (nodes x_old) at x_new x x_new through a
+    Legendre Vandermonde solve."""
+    V = np.polynomial.legendre.legvander(x_old, len(x_old) - 1)
+    W = np.polynomial.legendre.legvander(x_new, len(x_old) - 1)
+    C = np.linalg.solve(V, np.linalg.solve(V, vals).T).T  # coefficients c[a][b]
+    return W @ C @ W.T
+
+
+def mesh(tables, res, N, bc=(nd.BC_WALL,) * 4, periodic=False, jitter=0.1, tol_min=1e-14):
+    desc = nd.square_mesh(res, bc=bc, x_periodic=periodic, y_periodic=periodic, jitter=jitter, seed=3)
+    return desc, O.Mesh(tables, desc, N, tol_min=tol_min, tol_max=1e-300)
+
+
+def test_p_refinement_interpolates_the_element_polynomial(oracle_tables):
+    N = 5
+    desc, om = mesh(oracle_tables, 4, N)
+    q = smooth_state(desc, oracle_tables, N)
+    om.set_state(*q)
+    err, sigma, refine, _ = om.estimate_error()
+    assert refine.all() and (sigma >= 1).all()
+    assert om.adapt(N_max=N + 2) == om.K
+    assert (om.N == N + 2).all()
+    x_old, x_new = oracle_tables.get(O.O_NODES, N), oracle_tables.get(O.O_NODES, N + 2)
+    new = om.get_state()
+    ne_old, ne_new = (N + 1) ** 2, (N + 3) ** 2
+    for f in range(3):
+        for e in range(om.K):
+            old = q[f][e * ne_old:(e + 1) * ne_old].reshape(N + 1, N + 1)
+            ref = legendre_eval(x_old, old, x_new).ravel()
+            got = new[f][e * ne_new:(e + 1) * ne_new]
+            assert np.max(np.abs(got - ref)) < 1e-13 * max(1.0, np.abs(ref).max())
+
+
+def test_decisions_follow_the_reference_rules(oracle_tables):
+    N = 4
+    desc, om = mesh(oracle_tables, 4, N)
+    q = smooth_state(desc, oracle_tables, N)
+    om.set_state(*q)
+    om.estimate_error()
+    assert om.adapt(N_max=N + 1) == 0                  # N + 2 > N_max: nothing changes
+    assert (om.N == N).all()
+    # a rough field (sigma < 1 somewhere) asks for splitting, which the no-split branch refuses
+    rng = np.random.default_rng(0)
+    om.set_state(*(rng.normal(size=om.n) for _ in range(3)))
+    _, sigma, refine, _ = om.estimate_error()
+    assert (refine & (sigma < 1)).any()
+    assert om.adapt(N_max=12, max_split_level=1) == -1
+    assert (om.N == N).all()
+    # with max_split_level = 0 the same estimate raises only the refine & sigma >= 1 elements
+    expect = int((refine & (sigma >= 1)).sum())
+    assert om.adapt(N_max=12, max_split_level=0) == expect
+    assert int((om.N == N + 2).sum()) == expect
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_adapted_mesh_equals_directly_built_mesh(oracle_tables, periodic):
+    """Every element raised: the adapted operator equals the directly built one at N + 2 with the
+    same state, walls turned into zero ghosts (their storage was resized to zero)."""
+    N = 4
+    bc = (nd.BC_WALL,) * 4
+    desc, om = mesh(oracle_tables, 4, N, bc=bc, periodic=periodic)
+    om.set_state(*smooth_state(desc, oracle_tables, N))
+    om.estimate_error()
+    assert om.adapt(N_max=N + 2) == om.K
+    state = om.get_state()
+    desc2 = nd.square_mesh(4, bc=(nd.BC_NONE,) * 4 if not periodic else bc, x_periodic=periodic,
+                           y_periodic=periodic, jitter=0.1, seed=3)
+    fresh = O.Mesh(oracle_tables, desc2, N + 2)
+    fresh.set_state(*state)
+    for a, b in zip(om.compute_rhs(0.0), fresh.compute_rhs(0.0)):
+        assert np.array_equal(a, b)
+
+
+def test_solve_adaptive_runs_the_reference_loop(oracle_tables):
+    """Adaptation every 3 steps changes Δt (recomputed each step from the new orders) and keeps
+    the total pressure integral (the periodic upwind scheme conserves it, interpolation to a
+    higher order preserves the element polynomial)."""
+    N = 3
+    desc, om = mesh(oracle_tables, 8, N, periodic=True, jitter=0.0, tol_min=1e-4)
+    om.initial_conditions([(0.1, -0.2, 0.2, 1.0)])
+
+    def mass():
+        total = 0.0
+        p = om.get_state()[0]
+        off = 0
+        for e in range(om.K):
+            n = om.N[e]
+            w = oracle_tables.get(O.O_WEIGHTS, n)
+            ne = (n + 1) ** 2
+            # uniform mesh: J = (h/2)^2 everywhere
+            total += float(np.outer(w, w).ravel() @ p[off:off + ne]) * (0.25 / 2) ** 2
+            off += ne
+        return total
+
+    m0 = mass()
+    steps, t = om.solve_adaptive(0.5, 0.05, 3, N_max=9)
+    assert steps > 0 and t == pytest.approx(0.05, abs=0)
+    assert om.N.max() > N
+    assert abs(mass() - m0) < 1e-12 * abs(m0)
