@@ -43,6 +43,7 @@ constexpr int kAdjMuIn = 1;     // mu input is non-zero (stages after the first 
 constexpr int kAdjMuOut = 2;    // store mu <- a_s mu~ (a_s != 0)
 constexpr int kAdjExport = 4;   // export chi for the next reverse stage
 constexpr int kAdjApply = 8;    // do the transpose (off: export only)
+constexpr int kAdjSource = 16;  // add the receivers' misfit source to lambda' before the export
 
 template <int N>
 __global__ void __launch_bounds__(192)
@@ -50,12 +51,12 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
                      const double *__restrict__ tab, double *__restrict__ lam, double *__restrict__ mu,
                      const double *__restrict__ chi_in, double *__restrict__ chi_out,
                      const double *__restrict__ qs, double *__restrict__ cpart, int64_t K, int flags,
-                     double gs, double as, double gnext, double cw) {
+                     double gs, double as, double gnext, double cw, const int64_t *__restrict__ rec_elem,
+                     const double *__restrict__ rec_w, const double *__restrict__ rec_res, int n_rec) {
     using S = LineShape<N>;
     constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::T, FPT = S::FPT, QPT = S::QPT;
-    __shared__ double s_l[EPB * 3 * NE];   // lambda (then lambda')
     __shared__ double s_n[EPB * 3 * NE];   // nu = -c mu~ / J
-    __shared__ double s_a[EPB * 3 * NE];   // d lambda
+    __shared__ double s_a[EPB * 3 * NE];   // d lambda, then lambda' = lambda + d lambda
     __shared__ double s_rho[EPB * 12 * NF];
     __shared__ double s_met[EPB * 4 * NF];
     __shared__ double s_side[EPB * 4 * 3];
@@ -94,14 +95,14 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
             }
         }
     }
-    double mreg[QPT];
+    double mreg[QPT], lreg[QPT];
     const double *lg = lam + e0 * 3 * NE;
     const double *mg = mu + e0 * 3 * NE;
 #pragma unroll
     for (int it = 0; it < QPT; ++it) {
         const int r = tid + it * T;
         const bool in = r < nel * 3 * NE;
-        if (in) s_l[r] = lg[r];
+        lreg[it] = in ? lg[r] : 0.0;
         mreg[it] = (in && (flags & kAdjMuIn)) ? mg[r] : 0.0;
     }
     for (int r = tid; r < nel * 8; r += T) s_v[r] = verts[e0 * 8 + r];
@@ -146,7 +147,7 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         for (int it = 0; it < QPT; ++it) {
             const int r = tid + it * T;
             if (r < nel * 3 * NE) {
-                const double mt = mreg[it] + gs * s_l[r];
+                const double mt = mreg[it] + gs * lreg[it];
                 mreg[it] = as * mt;
                 if (flags & kAdjMuOut) mo[r] = mreg[it];
                 s_n[r] = -cw * mt * recip(jac_at(r / (3 * NE), r % NE));
@@ -181,7 +182,7 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
                     ou[i] = nb[NE + i * NF + j];
                     ov[i] = nb[2 * NE + i * NF + j];
                 }
-#pragma unroll
+#pragma unroll 1
                 for (int a = 0; a < NF; ++a) {
                     double Fu = 0.0, Fv = 0.0;
 #pragma unroll
@@ -195,7 +196,7 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
                 double op[NF];
 #pragma unroll
                 for (int i = 0; i < NF; ++i) op[i] = nb[i * NF + j];
-#pragma unroll
+#pragma unroll 1
                 for (int a = 0; a < NF; ++a) {
                     double Fp = 0.0;
 #pragma unroll
@@ -232,7 +233,7 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
                     ou[j] = nb[NE + i * NF + j];
                     ov[j] = nb[2 * NE + i * NF + j];
                 }
-#pragma unroll
+#pragma unroll 1
                 for (int b = 0; b < NF; ++b) {
                     double Gu = 0.0, Gv = 0.0;
 #pragma unroll
@@ -246,7 +247,7 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
                 double op[NF];
 #pragma unroll
                 for (int j = 0; j < NF; ++j) op[j] = nb[i * NF + j];
-#pragma unroll
+#pragma unroll 1
                 for (int b = 0; b < NF; ++b) {
                     double Gp = 0.0;
 #pragma unroll
@@ -282,16 +283,25 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
             const int i = l;
             const double *tau = s_rho + el * 12 * NF + f * NF;
             const double t0 = tau[i], t2 = tau[6 * NF + (N - i)];
-            double *da = s_a + el * 3 * NE + f * NE + i * NF;
-            double *lw = s_l + el * 3 * NE + f * NE + i * NF;
+            double *lw = s_a + el * 3 * NE + f * NE + i * NF;  // d lambda in, lambda' out
+            const double *lrow = lam + (e0 + el) * 3 * NE + f * NE + i * NF;
             const double *qrow = qs ? qs + (e0 + el) * 3 * NE + f * NE + i * NF : nullptr;
 #pragma unroll
             for (int b = 0; b < NF; ++b) {
-                const double dl = da[b] + lm[b] * t0 + lp[b] * t2 + lp[i] * tau[3 * NF + b] +
+                const double dl = lw[b] + lm[b] * t0 + lp[b] * t2 + lp[i] * tau[3 * NF + b] +
                                   lm[i] * tau[9 * NF + (N - b)];
-                lw[b] += dl;
+                lw[b] = lrow[b] + dl;
                 if (qrow) dot += dl * qrow[b];
             }
+            // misfit source of the step end this stage reaches: lambda_p += res_r w_r
+            if ((flags & kAdjSource) && f == 0)
+                for (int r = 0; r < n_rec; ++r)
+                    if (rec_elem[r] == e0 + el) {
+                        const double *w = rec_w + (int64_t)r * NE + i * NF;
+                        const double c = rec_res[r];
+#pragma unroll
+                        for (int b = 0; b < NF; ++b) lw[b] += c * w[b];
+                    }
         }
         if (cpart) {
             for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
@@ -304,7 +314,7 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
 #pragma unroll
         for (int it = 0; it < QPT; ++it) {
             const int r = tid + it * T;
-            if (r < nel * 3 * NE) lo[r] = s_l[r];
+            if (r < nel * 3 * NE) lo[r] = lreg[it] = s_a[r];
         }
     }
     if (!(flags & kAdjExport)) return;
@@ -314,7 +324,7 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
     for (int it = 0; it < QPT; ++it) {
         const int r = tid + it * T;
         if (r < nel * 3 * NE)
-            s_n[r] = -cw * (mreg[it] + gnext * s_l[r]) * recip(jac_at(r / (3 * NE), r % NE));
+            s_n[r] = -cw * (mreg[it] + gnext * lreg[it]) * recip(jac_at(r / (3 * NE), r % NE));
     }
     __syncthreads();
     // P8: lift transposes of all four sides (thread owns column l and row l).
@@ -388,7 +398,7 @@ static hipError_t adjoint_n(DeviceMesh &m, const AdjointArgs &A, hipStream_t s) 
     const int64_t blocks = (G.K + S::EPB - 1) / S::EPB;
     hipLaunchKernelGGL(adjoint_stage_kernel<N>, dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts, m.side_nb, G.tab,
                        A.lam, A.mu, A.chi_in, A.chi_out, A.qs, A.cpart, G.K, A.flags, A.gs, A.as, A.gnext,
-                       m.wave_speed);
+                       m.wave_speed, A.rec_elem, A.rec_w, A.rec_res, A.n_rec);
     return hipGetLastError();
 }
 

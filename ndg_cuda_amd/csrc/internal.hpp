@@ -20,7 +20,8 @@ constexpr int kMaxOrder = 16;       // largest order with a compiled kernel inst
 // face normal and tangent of the reference (Face2D_t::compute_geometry with the domain element on
 // the left); a/b the ghost element's first two nodes after face building; sf the ghost's edge
 // scaling factor (half the edge length, src/entities/Element2D_t.cpp:779-790). For mortar sides
-// `slot` is the offset (in values) of the side's projected flux [3][N+1] in the side-flux buffer.
+// `slot` is the offset (in values) of the side's projected flux [3][N+1] in the side-flux buffer
+// (a side covered by two half faces adds the one at `slot2`).
 struct BcSide {
     double nx, ny, tx, ty;
     double ax, ay, bx, by;
@@ -28,6 +29,7 @@ struct BcSide {
     int32_t kind;
     int32_t pad;
     int64_t slot;
+    int64_t slot2 = -1;  // a larger hanging side: the second half face's projected flux
 };
 
 // Element side whose face is non-conforming: its element flux trace is produced by the mortar
@@ -44,6 +46,10 @@ struct MortarFace {
     double offset[2], scale[2];
     double sfL, sfR;         // edge Jacobians of the L / R element sides
     int32_t NL, NR, Nf, pad;
+    // Offsets into the mortar matrix buffer of the L and R interpolation matrices ([Nf+1][N+1],
+    // element trace -> face nodes, R reversed) and projection matrices ([N+1][Nf+1], face flux ->
+    // element side, R negated and reversed, face scale included); -1 where the side is a plain copy.
+    int64_t mat[4];
 };
 
 // Per-order constant tables in device memory, packed as Dhat[Nf*Nf], l(-1)[Nf], l(+1)[Nf], w[Nf],
@@ -104,7 +110,8 @@ struct DeviceMesh {
     double *side_flux = nullptr; // projected mortar side fluxes
     float *side_flux32 = nullptr;
     int64_t n_side_flux = 0;
-    double *tab1d = nullptr;     // 1D tables of all orders (mortar kernel)
+    double *tab1d = nullptr;     // 1D tables of all orders (solution transfer)
+    double *mortar_mats = nullptr;  // interpolation / projection matrices of the mortar faces
     double *q = nullptr;
     double *g = nullptr;         // RK register, transposed within each (element, field) block
     double *trace[2] = {nullptr, nullptr};
@@ -113,6 +120,8 @@ struct DeviceMesh {
     int64_t scratch_len = 0;
     int cur = 0;                 // trace buffer holding the traces of the current state
     double *clock = nullptr;     // [4]: step start time, then the 3 stage times
+    double *q_out = nullptr;     // when set, FP64 stage kernels write the new state here (not in q)
+    bool has_inflow = false;     // the stage time is read only by inflow ghosts
     // Wave speed c of the symmetric system p_t + c div(u, v) = 0, (u, v)_t + c grad p = 0: the
     // right-hand side is c times the reference operator (whose constant c is 1), so c = 1 is the
     // reference arithmetic bit for bit.
@@ -139,7 +148,8 @@ hipError_t launch_traces_c32(const DeviceMesh &m, hipStream_t s);
 hipError_t launch_group_stage(DeviceMesh &m, int group, int stage, double a, double gdt, hipStream_t s);
 // Device clock: begin writes the 3 stage times of a step starting at t (set != 0) or at clock[0];
 // end advances clock[0] by dt. Both are single-thread kernels, so a captured step replays with
-// the time advancing on the device.
+// the time advancing on the device. Meshes without inflow ghosts never read the time, and the
+// clock kernels are skipped for them.
 hipError_t launch_clock_begin(DeviceMesh &m, double t, int set, double dt, hipStream_t s);
 hipError_t launch_clock_end(DeviceMesh &m, double dt, hipStream_t s);
 // Autotuning: the largest elements-per-workgroup of the stage kernel for order N, and one launch
@@ -157,7 +167,8 @@ bool order_supported(int N);
 
 // Adjoint (transpose) stage on a single-order mesh with interior and periodic sides only
 // (adjoint.hip). flags: 1 mu input non-zero, 2 store mu <- as mu~, 4 export chi for the next
-// reverse stage (from mu' + gnext lambda'), 8 apply the transpose (else export only).
+// reverse stage (from mu' + gnext lambda'), 8 apply the transpose (else export only), 16 add the
+// receivers' misfit source to lambda' (after the transpose, before the export).
 struct AdjointArgs {
     double *lam = nullptr, *mu = nullptr;
     const double *chi_in = nullptr;
@@ -166,6 +177,10 @@ struct AdjointArgs {
     double *cpart = nullptr;     // per-workgroup accumulators of <d lambda, q_s> (may be null)
     int flags = 0;
     double gs = 0.0, as = 0.0, gnext = 0.0;
+    // flag 16: receivers (element in device order, weights, residuals) added to lambda'
+    const int64_t *rec_elem = nullptr;
+    const double *rec_w = nullptr, *rec_res = nullptr;
+    int n_rec = 0;
 };
 hipError_t launch_adjoint_stage(DeviceMesh &m, const AdjointArgs &A, hipStream_t s);
 int64_t adjoint_blocks(const DeviceMesh &m);

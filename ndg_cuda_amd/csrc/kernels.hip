@@ -111,6 +111,8 @@ traces_kernel(const double *__restrict__ tab, const double *__restrict__ q,
 //       (Solver2D_t.cpp:138-172)
 //   P5  coalesced stores of q and g, traces of the updated state for the next stage.
 // RHS_ONLY writes G (the compute_dg_wave_derivative output) and leaves the state alone.
+// The new state goes to q_out, which is q for the in-place step (q and q_out are therefore not
+// restrict; every value of q a workgroup reads precedes, through barriers, its writes).
 // A workgroup takes epb <= EPB consecutive elements (fewer gives more workgroups for a small
 // order group); remap != 0 applies the XCD-contiguous block order. t_ptr points at the stage time
 // in device memory (the inflow ghost reads it), so that a captured step can be replayed.
@@ -120,7 +122,7 @@ template <int N, bool RHS_ONLY>
 __global__ void __launch_bounds__(192)
 stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_nb,
              const BcSide *__restrict__ bc, const double *__restrict__ tab,
-             double *__restrict__ q, double *__restrict__ g, const double *__restrict__ tr_in,
+             const double *q, double *q_out, double *__restrict__ g, const double *__restrict__ tr_in,
              double *__restrict__ tr_out, double *__restrict__ rhs, const double *__restrict__ side_flux,
              int64_t K, int first, double a, double gdt, const double *__restrict__ t_ptr, int epb,
              int remap, double cw) {
@@ -266,9 +268,16 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
                 const BcSide rec = bc[-1 - cd];
                 if (rec.kind == kKindMortar) {  // projected by the mortar kernel (already x sf)
                     const double *sfl = side_flux + rec.slot + k;
-                    tr[0] = sfl[0];
-                    tr[NF] = sfl[NF];
-                    tr[2 * NF] = sfl[2 * NF];
+                    double f0 = sfl[0], f1 = sfl[NF], f2 = sfl[2 * NF];
+                    if (rec.slot2 >= 0) {  // a side covered by two half faces
+                        const double *s2 = side_flux + rec.slot2 + k;
+                        f0 += s2[0];
+                        f1 += s2[NF];
+                        f2 += s2[2 * NF];
+                    }
+                    tr[0] = f0;
+                    tr[NF] = f1;
+                    tr[2 * NF] = f2;
                     continue;
                 }
                 nx = rec.nx;
@@ -388,7 +397,7 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
         return;
     }
     {
-        double *qw = q + e0 * 3 * NE;
+        double *qw = q_out + e0 * 3 * NE;
 #pragma unroll
         for (int it = 0; it < S::QPT; ++it) {
             const int r = tid + it * T;
@@ -646,9 +655,16 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
                 const BcSide rec = bc[-1 - cd];
                 if (rec.kind == kKindMortar) {
                     const float *sfl = side_flux + rec.slot + k;
-                    tr[0] = sfl[0];
-                    tr[NF] = sfl[NF];
-                    tr[2 * NF] = sfl[2 * NF];
+                    float f0 = sfl[0], f1 = sfl[NF], f2 = sfl[2 * NF];
+                    if (rec.slot2 >= 0) {
+                        const float *s2 = side_flux + rec.slot2 + k;
+                        f0 += s2[0];
+                        f1 += s2[NF];
+                        f2 += s2[2 * NF];
+                    }
+                    tr[0] = f0;
+                    tr[NF] = f1;
+                    tr[2 * NF] = f2;
                     continue;
                 }
                 nx = (float)rec.nx;
@@ -950,97 +966,77 @@ __device__ __forceinline__ bool ulp2_equal(double a, double b) {
     return fabs(a - b) <= DBL_EPSILON * fabs(a + b) * 2 || fabs(a - b) < DBL_MIN;
 }
 
-template <typename T>
-__device__ void mortar_side_state(const T *tr, int Ne, int Nf, const double *xf, const double *xe,
-                                  const double *le, double scale, double offset, bool reversed,
-                                  double out[3][kMaxOrder + 1]) {
-    if (Ne == Nf) {
-        for (int i = 0; i <= Nf; ++i)
-            for (int f = 0; f < 3; ++f) out[f][i] = (double)tr[f * (Ne + 1) + (reversed ? Nf - i : i)];
-        return;
-    }
-    for (int i = 0; i <= Nf; ++i) {
-        const double coord = (xf[reversed ? Nf - i : i] + 1) * scale + 2 * offset - 1;
-        double num[3] = {0.0, 0.0, 0.0}, den = 0.0;
-        for (int j = 0; j <= Ne; ++j) {
-            if (ulp2_equal(coord, xe[j])) {
-                for (int f = 0; f < 3; ++f) num[f] = (double)tr[f * (Ne + 1) + j];
-                den = 1.0;
-                break;
-            }
-            const double t = le[j] / (coord - xe[j]);
-            for (int f = 0; f < 3; ++f) num[f] += t * (double)tr[f * (Ne + 1) + j];
-            den += t;
-        }
-        for (int f = 0; f < 3; ++f) out[f][i] = num[f] / den;
-    }
-}
-
-template <typename T>
-__device__ void mortar_project(const double F[3][kMaxOrder + 1], int Ne, int Nf, const double *xf,
-                               const double *wf, const double *xe, const double *we, const double *le,
-                               double sf, double scale, double offset, bool backward, T *out) {
-    if (Ne == Nf) {
-        for (int j = 0; j <= Nf; ++j)
-            for (int f = 0; f < 3; ++f) {
-                if (backward) out[f * (Ne + 1) + (Nf - j)] = (T)(-F[f][j] * sf);
-                else out[f * (Ne + 1) + j] = (T)(F[f][j] * sf);
-            }
-        return;
-    }
-    double acc[3][kMaxOrder + 1];
-    for (int i = 0; i <= Ne; ++i) acc[0][i] = acc[1][i] = acc[2][i] = 0.0;
-    for (int j = 0; j <= Nf; ++j) {
-        const double coord = (xf[backward ? Nf - j : j] + 1) * scale + 2 * offset - 1;
-        int hit = -1;
-        for (int i = 0; i <= Ne; ++i)
-            if (ulp2_equal(coord, xe[i])) { hit = i; break; }
-        if (hit >= 0) {
-            const int i = hit;
-            const double wr = (backward ? -wf[j] : wf[j]) / we[i];
-            for (int f = 0; f < 3; ++f) acc[f][i] += wr * F[f][j] * sf * scale;
-        } else {
-            double ssum = 0.0;
-            for (int i = 0; i <= Ne; ++i) ssum += le[i] / (coord - xe[i]);
-            for (int i = 0; i <= Ne; ++i) {
-                const double Tc = le[i] / ((coord - xe[i]) * ssum);
-                const double wr = (backward ? -Tc : Tc) * wf[j] / we[i];
-                for (int f = 0; f < 3; ++f) acc[f][i] += wr * F[f][j] * sf * scale;
-            }
-        }
-    }
-    for (int i = 0; i <= Ne; ++i)
-        for (int f = 0; f < 3; ++f) out[f * (Ne + 1) + i] = (T)acc[f][i];
-}
-
+// One wave per face: lanes 0..31 handle the left side, lanes 32..63 the right side (face node i
+// for the interpolation, element node i for the projection); the face states and flux go through
+// LDS. The interpolation and projection matrices are built on the host with the reference's
+// exact-node shortcuts (see MortarFace).
 template <typename T>
 __global__ void __launch_bounds__(64)
-mortar_kernel(const MortarFace *__restrict__ faces, int64_t n_faces, const double *__restrict__ tab1d,
+mortar_kernel(const MortarFace *__restrict__ faces, const double *__restrict__ mats,
               const T *__restrict__ tr, T *__restrict__ side_flux) {
-    const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (fi >= n_faces) return;
-    const MortarFace fc = faces[fi];
-    const double *xf = tab1d + (3 * fc.Nf + 0) * kTab1dStride, *wf = tab1d + (3 * fc.Nf + 1) * kTab1dStride;
-    const double *xL = tab1d + (3 * fc.NL + 0) * kTab1dStride, *wL = tab1d + (3 * fc.NL + 1) * kTab1dStride;
-    const double *lL = tab1d + (3 * fc.NL + 2) * kTab1dStride;
-    const double *xR = tab1d + (3 * fc.NR + 0) * kTab1dStride, *wR = tab1d + (3 * fc.NR + 1) * kTab1dStride;
-    const double *lR = tab1d + (3 * fc.NR + 2) * kTab1dStride;
-    double sL[3][kMaxOrder + 1], sR[3][kMaxOrder + 1];
-    mortar_side_state(tr + fc.trL, fc.NL, fc.Nf, xf, xL, lL, fc.scale[0], fc.offset[0], false, sL);
-    mortar_side_state(tr + fc.trR, fc.NR, fc.Nf, xf, xR, lR, fc.scale[1], fc.offset[1], true, sR);
-    const double c = kWaveSpeed;
-    for (int i = 0; i <= fc.Nf; ++i) {
-        const double wl = sL[0][i] + c * (sL[1][i] * fc.nx + sL[2][i] * fc.ny);
-        const double wr = sR[0][i] - c * (sR[1][i] * fc.nx + sR[2][i] * fc.ny);
-        sL[0][i] = c * (wl - wr) / 2;      // the face flux overwrites the left state
-        sL[1][i] = fc.nx * (wl + wr) / 2;
-        sL[2][i] = fc.ny * (wl + wr) / 2;
+    __shared__ double s_st[2][3][kMaxOrder + 1];  // face states, left and right
+    __shared__ double s_f[3][kMaxOrder + 1];      // face flux
+    const MortarFace fc = faces[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int side = lane >> 5, i = lane & 31;
+    const int Nf = fc.Nf;
+    const int Ne = side ? fc.NR : fc.NL;
+    const T *te = tr + (side ? fc.trR : fc.trL);
+    // project_to_faces (src/meshes/Mesh2D_t.cpp:3803-3893)
+    if (i <= Nf) {
+        const int64_t mi = fc.mat[side];
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        if (mi < 0) {
+            const int k = side ? Nf - i : i;
+            v0 = (double)te[k];
+            v1 = (double)te[(Ne + 1) + k];
+            v2 = (double)te[2 * (Ne + 1) + k];
+        } else {
+            const double *row = mats + mi + (int64_t)i * (Ne + 1);
+            for (int j = 0; j <= Ne; ++j) {
+                const double w = row[j];
+                v0 += w * (double)te[j];
+                v1 += w * (double)te[(Ne + 1) + j];
+                v2 += w * (double)te[2 * (Ne + 1) + j];
+            }
+        }
+        s_st[side][0][i] = v0;
+        s_st[side][1][i] = v1;
+        s_st[side][2][i] = v2;
     }
-    mortar_project<T>(sL, fc.NL, fc.Nf, xf, wf, xL, wL, lL, fc.sfL, fc.scale[0], fc.offset[0], false,
-                      side_flux + fc.slotL);
-    if (fc.slotR >= 0)
-        mortar_project<T>(sL, fc.NR, fc.Nf, xf, wf, xR, wR, lR, fc.sfR, fc.scale[1], fc.offset[1], true,
-                          side_flux + fc.slotR);
+    __syncthreads();
+    // calculate_wave_fluxes (src/solvers/Solver2D_t.cpp:48-63)
+    if (lane <= Nf) {
+        const double c = kWaveSpeed;
+        const double wl = s_st[0][0][lane] + c * (s_st[0][1][lane] * fc.nx + s_st[0][2][lane] * fc.ny);
+        const double wr = s_st[1][0][lane] - c * (s_st[1][1][lane] * fc.nx + s_st[1][2][lane] * fc.ny);
+        s_f[0][lane] = c * (wl - wr) / 2;
+        s_f[1][lane] = fc.nx * (wl + wr) / 2;
+        s_f[2][lane] = fc.ny * (wl + wr) / 2;
+    }
+    __syncthreads();
+    // project_to_elements (src/meshes/Mesh2D_t.cpp:3895-4006), times the edge Jacobian
+    const int64_t slot = side ? fc.slotR : fc.slotL;
+    if (slot < 0 || i > Ne) return;
+    const double sf = side ? fc.sfR : fc.sfL;
+    T *out = side_flux + slot;
+    const int64_t mp = fc.mat[2 + side];
+    if (mp < 0) {
+        for (int f = 0; f < 3; ++f)
+            out[f * (Ne + 1) + i] = side ? (T)(-s_f[f][Nf - i] * sf) : (T)(s_f[f][i] * sf);
+        return;
+    }
+    const double *row = mats + mp + (int64_t)i * (Nf + 1);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int j = 0; j <= Nf; ++j) {
+        const double w = row[j];
+        a0 += w * s_f[0][j];
+        a1 += w * s_f[1][j];
+        a2 += w * s_f[2][j];
+    }
+    out[i] = (T)(a0 * sf);
+    out[(Ne + 1) + i] = (T)(a1 * sf);
+    out[2 * (Ne + 1) + i] = (T)(a2 * sf);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1151,9 +1147,9 @@ static hipError_t stage_n(DeviceMesh &m, const Group &G, int stage, double a, do
     const int epb = group_epb<N>(G, NDG_PRECISION_FP64);
     const int64_t blocks = (G.K + epb - 1) / epb;
     hipLaunchKernelGGL((stage_kernel<N, false>), dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts + G.e_begin * 8,
-                       m.side_nb + G.e_begin * 4, m.bc, G.tab, m.q + G.q_off, m.g + G.q_off,
-                       m.trace[m.cur] + G.tr_off, m.trace[1 - m.cur] + G.tr_off, nullptr, m.side_flux, G.K,
-                       stage == 0 ? 1 : 0, a, gdt, m.clock + 1 + stage, epb, G.tune[NDG_PRECISION_FP64].remap,
+                       m.side_nb + G.e_begin * 4, m.bc, G.tab, m.q + G.q_off, (m.q_out ? m.q_out : m.q) + G.q_off,
+                       m.g + G.q_off, m.trace[m.cur] + G.tr_off, m.trace[1 - m.cur] + G.tr_off, nullptr, m.side_flux,
+                       G.K, stage == 0 ? 1 : 0, a, gdt, m.clock + 1 + stage, epb, G.tune[NDG_PRECISION_FP64].remap,
                        m.wave_speed);
     return hipGetLastError();
 }
@@ -1163,7 +1159,7 @@ static hipError_t rhs_n(DeviceMesh &m, const Group &G, hipStream_t s) {
     using S = LineShape<N>;
     const int64_t blocks = (G.K + S::EPB - 1) / S::EPB;
     hipLaunchKernelGGL((stage_kernel<N, true>), dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts + G.e_begin * 8,
-                       m.side_nb + G.e_begin * 4, m.bc, G.tab, m.q + G.q_off, m.g + G.q_off,
+                       m.side_nb + G.e_begin * 4, m.bc, G.tab, m.q + G.q_off, m.q + G.q_off, m.g + G.q_off,
                        m.trace[m.cur] + G.tr_off, nullptr, m.rhs + G.q_off, m.side_flux, G.K, 1, 0.0, 0.0,
                        m.clock + 1, S::EPB, 1, m.wave_speed);
     return hipGetLastError();
@@ -1252,12 +1248,12 @@ hipError_t launch_traces(const DeviceMesh &m, hipStream_t s) {
 
 hipError_t launch_mortar(DeviceMesh &m, hipStream_t s) {
     if (m.n_faces == 0) return hipSuccess;
-    const unsigned blocks = (unsigned)((m.n_faces + 63) / 64);
+    const unsigned blocks = (unsigned)m.n_faces;
     if (m.precision == NDG_PRECISION_FP32_COMPENSATED)
-        hipLaunchKernelGGL(mortar_kernel<float>, dim3(blocks), dim3(64), 0, s, m.faces, m.n_faces, m.tab1d,
+        hipLaunchKernelGGL(mortar_kernel<float>, dim3(blocks), dim3(64), 0, s, m.faces, m.mortar_mats,
                            m.trace32[m.cur], m.side_flux32);
     else
-        hipLaunchKernelGGL(mortar_kernel<double>, dim3(blocks), dim3(64), 0, s, m.faces, m.n_faces, m.tab1d,
+        hipLaunchKernelGGL(mortar_kernel<double>, dim3(blocks), dim3(64), 0, s, m.faces, m.mortar_mats,
                            m.trace[m.cur], m.side_flux);
     return hipGetLastError();
 }
@@ -1272,8 +1268,7 @@ hipError_t launch_stage(DeviceMesh &m, int stage, double a, double gdt, hipStrea
 }
 
 hipError_t launch_rhs(DeviceMesh &m, double t, hipStream_t s) {
-    hipLaunchKernelGGL(clock_begin_kernel, dim3(1), dim3(64), 0, s, m.clock, t, 1, 0.0);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_clock_begin(m, t, 1, 0.0, s);
     if (e != hipSuccess) return e;
     if ((e = launch_mortar(m, s)) != hipSuccess) return e;
     return for_groups(m, [&](const Group &G) -> hipError_t {
@@ -1342,11 +1337,13 @@ hipError_t launch_delta_t(const DeviceMesh &m, double cfl, double *dt_dev, hipSt
 }
 
 hipError_t launch_clock_begin(DeviceMesh &m, double t, int set, double dt, hipStream_t s) {
+    if (!m.has_inflow) return hipSuccess;  // nothing reads the stage time
     hipLaunchKernelGGL(clock_begin_kernel, dim3(1), dim3(64), 0, s, m.clock, t, set, dt);
     return hipGetLastError();
 }
 
 hipError_t launch_clock_end(DeviceMesh &m, double dt, hipStream_t s) {
+    if (!m.has_inflow) return hipSuccess;
     hipLaunchKernelGGL(clock_end_kernel, dim3(1), dim3(64), 0, s, m.clock, dt);
     return hipGetLastError();
 }

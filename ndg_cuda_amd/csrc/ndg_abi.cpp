@@ -7,7 +7,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <tuple>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -76,6 +78,11 @@ struct ndg_mesh {
     std::vector<double> host_verts;          // [K][4][2], caller's order
     std::vector<int64_t> link;               // exterior side of each element side (see ndg_mesh_create)
     std::vector<char> periodic_link;
+    // Hanging (2:1) sides: 1 = a small side whose exterior is half of a larger element's side
+    // (link = that side), 2 = the larger side (link = its first small side, hang_second = the
+    // second, in the order of the larger side).
+    std::vector<char> hang_kind;
+    std::vector<int64_t> hang_second;
     std::vector<BcSide> bc_base;             // boundary (ghost) records
     std::vector<char> ghost_zero;            // per record: ghost storage resized to zero by an adaptation
     int initial_N = 0, maximum_N = 0, max_split_level = 0;
@@ -182,6 +189,90 @@ double side_scale(const v2 P[4], int s) {
 
 uint64_t edge_key(int64_t a, int64_t b) { return ((uint64_t)a << 32) | (uint64_t)b; }
 
+// Element2D_t::almost_equal / Mesh2D_t::almost_equal with 2 ulp (src/meshes/Mesh2D_t.cpp:3785-3792)
+bool ulp2_equal(double a, double b) {
+    return std::fabs(a - b) <= DBL_EPSILON * std::fabs(a + b) * 2 || std::fabs(a - b) < DBL_MIN;
+}
+
+// Interpolation (element side -> face nodes) and projection (face -> element side) matrices of one
+// side of a mortar face, with the reference's arithmetic structure: project_to_faces' barycentric
+// interpolation at xi = (x_face + 1) scale + 2 offset - 1 with the exact-node shortcut
+// (src/meshes/Mesh2D_t.cpp:3820-3890), and project_to_elements' Gauss-quadrature L2 projection
+// T_i w_j / w_i with the same shortcut, negated for the face's right side (:3935-4003). The
+// right side walks the face nodes backwards. Matrices are shared between faces with equal keys.
+class MortarMatrices {
+  public:
+    explicit MortarMatrices(const Tables &t) : t_(t) {}
+    int64_t interp(int Ne, int Nf, int side, double offset, double scale) {
+        return get(Ne, Nf, side, offset, scale, 0);
+    }
+    int64_t proj(int Ne, int Nf, int side, double offset, double scale) {
+        return get(Ne, Nf, side, offset, scale, 1);
+    }
+    const std::vector<double> &data() const { return data_; }
+
+  private:
+    int64_t get(int Ne, int Nf, int side, double offset, double scale, int which) {
+        const auto key = std::make_tuple(Ne, Nf, side, offset, scale, which);
+        auto it = index_.find(key);
+        if (it != index_.end()) return it->second;
+        const int64_t at = (int64_t)data_.size();
+        const auto &xf = t_.tab[NDG_TABLE_NODES][Nf], &wf = t_.tab[NDG_TABLE_WEIGHTS][Nf];
+        const auto &xe = t_.tab[NDG_TABLE_NODES][Ne], &we = t_.tab[NDG_TABLE_WEIGHTS][Ne];
+        const auto &le = t_.tab[NDG_TABLE_BARYCENTRIC_WEIGHTS][Ne];
+        if (which == 0) {
+            data_.resize(data_.size() + (size_t)(Nf + 1) * (Ne + 1), 0.0);
+            for (int i = 0; i <= Nf; ++i) {
+                double *row = data_.data() + at + (int64_t)i * (Ne + 1);
+                const double coord = (xf[side ? Nf - i : i] + 1) * scale + 2 * offset - 1;
+                int hit = -1;
+                for (int j = 0; j <= Ne; ++j)
+                    if (ulp2_equal(coord, xe[j])) {
+                        hit = j;
+                        break;
+                    }
+                if (hit >= 0) {
+                    row[hit] = 1.0;
+                    continue;
+                }
+                double den = 0.0;
+                for (int j = 0; j <= Ne; ++j) {
+                    row[j] = le[j] / (coord - xe[j]);
+                    den += row[j];
+                }
+                for (int j = 0; j <= Ne; ++j) row[j] /= den;
+            }
+        } else {
+            data_.resize(data_.size() + (size_t)(Ne + 1) * (Nf + 1), 0.0);
+            double *P = data_.data() + at;
+            for (int j = 0; j <= Nf; ++j) {
+                const double coord = (xf[side ? Nf - j : j] + 1) * scale + 2 * offset - 1;
+                int hit = -1;
+                for (int k = 0; k <= Ne; ++k)
+                    if (ulp2_equal(coord, xe[k])) {
+                        hit = k;
+                        break;
+                    }
+                if (hit >= 0) {
+                    P[(int64_t)hit * (Nf + 1) + j] += (side ? -wf[j] : wf[j]) / we[hit] * scale;
+                    continue;
+                }
+                double ssum = 0.0;
+                for (int k = 0; k <= Ne; ++k) ssum += le[k] / (coord - xe[k]);
+                for (int i = 0; i <= Ne; ++i) {
+                    const double T = le[i] / ((coord - xe[i]) * ssum);
+                    P[(int64_t)i * (Nf + 1) + j] += (side ? -T : T) * wf[j] / we[i] * scale;
+                }
+            }
+        }
+        index_[key] = at;
+        return at;
+    }
+    const Tables &t_;
+    std::vector<double> data_;
+    std::map<std::tuple<int, int, int, double, double, int>, int64_t> index_;
+};
+
 template <typename T>
 ndg_status device_upload(T **dst, const std::vector<T> &src, const char *what) {
     if (src.empty()) return NDG_OK;
@@ -192,7 +283,8 @@ ndg_status device_upload(T **dst, const std::vector<T> &src, const char *what) {
 
 void free_device(DeviceMesh &d) {
     std::vector<void *> ptrs = {(void *)d.verts, (void *)d.side_nb, (void *)d.bc, (void *)d.faces,
-                                (void *)d.side_flux, (void *)d.side_flux32, (void *)d.tab1d, (void *)d.q,
+                                (void *)d.side_flux, (void *)d.side_flux32, (void *)d.tab1d,
+                                (void *)d.mortar_mats, (void *)d.q,
                                 (void *)d.g, (void *)d.trace[0], (void *)d.trace[1], (void *)d.rhs,
                                 (void *)d.scratch, (void *)d.qh, (void *)d.ql, (void *)d.g32,
                                 (void *)d.trace32[0], (void *)d.trace32[1], (void *)d.clock};
@@ -307,6 +399,7 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
     std::vector<BcSide> bcs = m->bc_base;
     for (size_t b = 0; b < bcs.size(); ++b)
         if (m->ghost_zero[b]) bcs[b].sf = 0.0;
+    for (const BcSide &b : bcs) d.has_inflow = d.has_inflow || b.kind == NDG_BC_INFLOW;
 
     // Device connectivity. Sides whose exterior state has another order are non-conforming: they
     // get a mortar record, and one mortar face per pair (two-sided for a neighbour, one-sided for
@@ -324,14 +417,23 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
         bcs.push_back(rec);
         return rec.slot;
     };
+    // A larger hanging side sums two projected half-face fluxes (slot, slot2).
+    for (int64_t es = 0; es < 4 * K; ++es)
+        if (m->hang_kind[es] == 2) {
+            add_mortar_side(es);
+            bcs.back().slot2 = n_slot;
+            n_slot += 3 * (int64_t)(order[es / 4] + 1);
+        }
     for (int64_t e = 0; e < K; ++e)
         for (int s = 0; s < 4; ++s) {
             const int64_t es = 4 * e + s, l = link[es];
-            if (l < 0 || order[l / 4] == order[e]) continue;
+            const int hk = m->hang_kind[es];
+            if (hk == 2) continue;  // built from its two halves
+            if (l < 0 || (hk == 0 && order[l / 4] == order[e])) continue;
             const int64_t r = l / 4;
             const int rs = (int)(l % 4);
             const bool periodic = periodic_link[es] != 0;
-            if (!periodic && r < e) continue;  // the face belongs to its first element
+            if (hk == 0 && !periodic && r < e) continue;  // the face belongs to its first element
             MortarFace fc{};
             fc.NL = order[e];
             fc.NR = order[r];
@@ -351,12 +453,25 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
             fc.sfL = side_scale(PL, s);
             fc.slotL = add_mortar_side(es);
             fc.slotR = -1;
-            if (!periodic) {
+            if (hk == 1) {  // half of the larger side l
+                v2 PR[4] = {corner(r, 0), corner(r, 1), corner(r, 2), corner(r, 3)};
+                fc.sfR = side_scale(PR, rs);
+                const BcSide &big = bcs[(size_t)mortar_rec[l]];
+                fc.slotR = link[l] == es ? big.slot : big.slot2;
+            } else if (!periodic) {
                 v2 PR[4] = {corner(r, 0), corner(r, 1), corner(r, 2), corner(r, 3)};
                 fc.sfR = side_scale(PR, rs);
                 fc.slotR = add_mortar_side(l);
             }
             faces.push_back(fc);
+        }
+    MortarMatrices mats(tab);
+    for (MortarFace &fc : faces)
+        for (int side = 0; side < 2; ++side) {
+            const int Ne = side ? fc.NR : fc.NL;
+            const bool copy = Ne == fc.Nf && fc.offset[side] == 0.0 && fc.scale[side] == 1.0;
+            fc.mat[side] = copy ? -1 : mats.interp(Ne, fc.Nf, side, fc.offset[side], fc.scale[side]);
+            fc.mat[2 + side] = copy ? -1 : mats.proj(Ne, fc.Nf, side, fc.offset[side], fc.scale[side]);
         }
     for (int64_t e = 0; e < K; ++e)
         for (int s = 0; s < 4; ++s) {
@@ -364,6 +479,7 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
             int32_t code;
             if (mortar_rec[es] >= 0) code = -(int32_t)(1 + mortar_rec[es]);
             else if (l < 0) code = (int32_t)l;
+            else if (m->hang_kind[es]) return fail(NDG_ERR_INVALID, "ndg_mesh_create: hanging side without a mortar");
             else code = (int32_t)(local_of(l / 4) * 4 + l % 4);
             side_nb[4 * m->dev_of[e] + s] = code;
         }
@@ -377,6 +493,7 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
     if ((st = device_upload(&d.bc, bcs, "upload boundary records")) != NDG_OK) return st;
     d.n_bc = (int64_t)bcs.size();
     if ((st = device_upload(&d.faces, faces, "upload mortar faces")) != NDG_OK) return st;
+    if ((st = device_upload(&d.mortar_mats, mats.data(), "upload mortar matrices")) != NDG_OK) return st;
     d.n_faces = (int64_t)faces.size();
     d.n_side_flux = n_slot;
     if (n_slot > 0) HIP_TRY(hipMalloc((void **)&d.side_flux, sizeof(double) * n_slot), "hipMalloc side fluxes");
@@ -507,6 +624,44 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
         link[es] = -(int64_t)(1 + bcs.size());
         bcs.push_back(rec);
     }
+    // Hanging (2:1) sides: a side a -> b with neither a neighbour nor a bar whose two halves are
+    // the unmatched sides b -> m and m -> a of two other elements, m its midpoint (what
+    // h-refinement leaves, src/meshes/Mesh2D_t.cpp:5661-5985).
+    std::vector<char> hang_kind((size_t)(4 * K), 0);
+    std::vector<int64_t> hang_second((size_t)(4 * K), -1);
+    {
+        std::unordered_map<int64_t, std::vector<int64_t>> out_edges;  // node -> element sides starting there
+        for (int64_t es = 0; es < 4 * K; ++es)
+            if (link[es] == INT64_MIN) out_edges[desc->quads[es]].push_back(es);
+        auto end_node = [&](int64_t es) { return desc->quads[4 * (es / 4) + (es % 4 + 1) % 4]; };
+        for (int64_t es = 0; es < 4 * K; ++es) {
+            if (link[es] != INT64_MIN || hang_kind[es]) continue;
+            const int64_t a = desc->quads[es], b = end_node(es);
+            auto it = out_edges.find(b);
+            if (it == out_edges.end()) continue;
+            for (int64_t s1 : it->second) {  // b -> m
+                if (s1 == es || hang_kind[s1] || link[s1] != INT64_MIN) continue;
+                const int64_t mnode = end_node(s1);
+                const v2 pa = P(a), pb = P(b), pm = P(mnode);
+                const double len = mag(pb.x - pa.x, pb.y - pa.y);
+                if (std::fabs(pm.x - (pa.x + pb.x) / 2) > 1e-12 * len || std::fabs(pm.y - (pa.y + pb.y) / 2) > 1e-12 * len)
+                    continue;
+                auto jt = out_edges.find(mnode);
+                if (jt == out_edges.end()) continue;
+                int64_t s2 = -1;  // m -> a
+                for (int64_t c : jt->second)
+                    if (end_node(c) == a && !hang_kind[c] && link[c] == INT64_MIN) s2 = c;
+                if (s2 < 0) continue;
+                hang_kind[es] = 2;
+                link[es] = s2;          // the half starting at a first
+                hang_second[es] = s1;
+                hang_kind[s1] = hang_kind[s2] = 1;
+                link[s1] = link[s2] = es;
+                n_faces += 2;           // the two half faces
+                break;
+            }
+        }
+    }
     for (int64_t es = 0; es < 4 * K; ++es)
         if (link[es] == INT64_MIN)
             return fail(NDG_ERR_INVALID, "ndg_mesh_create: element side has neither a neighbour nor a boundary bar");
@@ -525,6 +680,8 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
     m->max_split_level = desc->max_split_level;
     m->link = std::move(link);
     m->periodic_link = std::move(periodic_link);
+    m->hang_kind = std::move(hang_kind);
+    m->hang_second = std::move(hang_second);
     m->bc_base = std::move(bcs);
     m->ghost_zero.assign(m->bc_base.size(), 0);
     m->host_verts.resize((size_t)(8 * K));
@@ -1294,16 +1451,47 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     if ((st = dev_alloc(res, sizeof(double) * R.n * n_steps, "hipMalloc residuals")) != NDG_OK) return st;
     auto ckp = [&](int64_t k) { return ck.as<double>() + k * d.n_q; };
     auto stp = [&](int64_t j, int stage) { return store.as<double>() + (j * 3 + stage) * d.n_q; };
+    double *const q_main = d.q;
 
-    // Forward pass with checkpoints; p at the receivers after every step.
+    // Steps n0..n1-1 from the state at q_start, keeping every stage input: the stage kernels write
+    // their output state into the next store slot (q_start itself stands for slot (n0, 0)); the
+    // output of the last step goes to the mesh's own state buffer. Records the receivers after
+    // every step when asked.
+    auto run_stored = [&](const double *q_start, int64_t n0, int64_t n1, bool record) -> ndg_status {
+        d.q = const_cast<double *>(q_start);
+        HIP_TRY(launch_traces(d, s), "traces kernel");
+        for (int64_t j = n0; j < n1; ++j)
+            for (int stage = 0; stage < 3; ++stage) {
+                double *in = (j == n0 && stage == 0) ? const_cast<double *>(q_start) : stp(j - n0, stage);
+                double *out = stage < 2 ? stp(j - n0, stage + 1) : (j + 1 < n1 ? stp(j + 1 - n0, 0) : q_main);
+                d.q = in;
+                d.q_out = out;
+                HIP_TRY(launch_stage(d, stage, kAm[stage], kGm[stage] * dt, s), "stage kernel");
+                if (stage == 2 && record) {
+                    d.q = out;
+                    HIP_TRY(launch_record(d, R.elem.as<int64_t>(), R.w.as<double>(), R.n,
+                                          recs.as<double>() + j * R.n, s), "record kernel");
+                }
+            }
+        d.q = q_main;
+        d.q_out = nullptr;
+        return NDG_OK;
+    };
+    auto stage_input = [&](int64_t k, int64_t j, int stage) -> const double * {
+        return (j == k * checkpoint_interval && stage == 0) ? ckp(k) : stp(j - k * checkpoint_interval, stage);
+    };
+
+    // Forward pass: checkpoints; the last segment keeps its stage inputs right away.
     HIP_TRY(launch_clock_begin(d, 0.0, 1, dt, s), "clock kernel");
-    for (int64_t n = 0; n < n_steps; ++n) {
+    for (int64_t n = 0; n < (n_ck - 1) * checkpoint_interval; ++n) {
         if (n % checkpoint_interval == 0)
             HIP_TRY(hipMemcpyAsync(ckp(n / checkpoint_interval), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
         if ((st = step_from_clock(mesh, dt)) != NDG_OK) return st;
         HIP_TRY(launch_record(d, R.elem.as<int64_t>(), R.w.as<double>(), R.n, recs.as<double>() + n * R.n, s),
                 "record kernel");
     }
+    HIP_TRY(hipMemcpyAsync(ckp(n_ck - 1), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
+    if ((st = run_stored(ckp(n_ck - 1), (n_ck - 1) * checkpoint_interval, n_steps, true)) != NDG_OK) return st;
     std::vector<double> hres((size_t)(R.n * n_steps));
     HIP_TRY(hipMemcpyAsync(hres.data(), recs.p, sizeof(double) * hres.size(), hipMemcpyDeviceToHost, s), "download records");
     HIP_TRY(hipStreamSynchronize(s), "adjoint sync");
@@ -1315,54 +1503,52 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     Jv /= 2;
     HIP_TRY(hipMemcpyAsync(res.p, hres.data(), sizeof(double) * hres.size(), hipMemcpyHostToDevice, s), "upload residuals");
 
-    // Reverse sweep: recompute each segment from its checkpoint keeping the stage inputs, then
-    // run the transposed stages backwards.
+    // Reverse sweep: per segment (the last one is already stored) recompute from its checkpoint
+    // keeping the stage inputs, then the transposed stages backwards. The misfit source of each
+    // step end is added by the stage-0 kernel of the following step (in reverse order), which
+    // also exports for the next stage 2.
     HIP_TRY(hipMemsetAsync(lam.p, 0, qb, s), "hipMemset adjoint");
     HIP_TRY(hipMemsetAsync(mu.p, 0, qb, s), "hipMemset adjoint");
     HIP_TRY(hipMemsetAsync(cpart.p, 0, sizeof(double) * nblk, s), "hipMemset adjoint");
     double *chi[2] = {chi0.as<double>(), chi1.as<double>()};
     int cc = 0;
-    auto add_source_and_export = [&](int64_t n) -> ndg_status {
-        HIP_TRY(launch_source(lam.as<double>(), d, R.elem.as<int64_t>(), R.w.as<double>(),
-                              res.as<double>() + n * R.n, R.n, s), "source kernel");
+    HIP_TRY(launch_source(lam.as<double>(), d, R.elem.as<int64_t>(), R.w.as<double>(),
+                          res.as<double>() + (n_steps - 1) * R.n, R.n, s), "source kernel");
+    {
         AdjointArgs A;
         A.lam = lam.as<double>();
         A.mu = mu.as<double>();
-        A.chi_out = chi[1 - cc];
-        A.flags = 4;  // export only, mu = 0 at a step boundary
+        A.chi_out = chi[1];
+        A.flags = 4;  // export only, mu = 0 at a step end
         A.gnext = kGm[2] * dt;
         HIP_TRY(launch_adjoint_stage(d, A, s), "adjoint export kernel");
-        cc = 1 - cc;
-        return NDG_OK;
-    };
-    if ((st = add_source_and_export(n_steps - 1)) != NDG_OK) return st;
+        cc = 1;
+    }
     for (int64_t k = n_ck - 1; k >= 0; --k) {
         const int64_t n0 = k * checkpoint_interval, n1 = std::min(n0 + checkpoint_interval, n_steps);
-        HIP_TRY(hipMemcpyAsync(d.q, ckp(k), qb, hipMemcpyDeviceToDevice, s), "restore checkpoint");
-        HIP_TRY(launch_traces(d, s), "traces kernel");
-        for (int64_t j = n0; j < n1; ++j)
-            for (int stage = 0; stage < 3; ++stage) {
-                HIP_TRY(hipMemcpyAsync(stp(j - n0, stage), d.q, qb, hipMemcpyDeviceToDevice, s), "stage store");
-                HIP_TRY(launch_stage(d, stage, kAm[stage], kGm[stage] * dt, s), "stage kernel");
-            }
-        for (int64_t j = n1 - 1; j >= n0; --j) {
+        if (k != n_ck - 1 && (st = run_stored(ckp(k), n0, n1, false)) != NDG_OK) return st;
+        for (int64_t j = n1 - 1; j >= n0; --j)
             for (int stage = 2; stage >= 0; --stage) {
                 AdjointArgs A;
                 A.lam = lam.as<double>();
                 A.mu = mu.as<double>();
                 A.chi_in = chi[cc];
                 A.chi_out = chi[1 - cc];
-                A.qs = stp(j - n0, stage);
+                A.qs = stage_input(k, j, stage);
                 A.cpart = cpart.as<double>();
-                A.flags = 8 | (stage != 2 ? 1 : 0) | (stage != 0 ? 2 | 4 : 0);
+                const bool more = stage != 0 || j >= 1;  // a reverse stage follows
+                A.flags = 8 | (stage != 2 ? 1 : 0) | (stage != 0 ? 2 : 0) | (more ? 4 : 0) |
+                          (stage == 0 && j >= 1 ? 16 : 0);
                 A.gs = kGm[stage] * dt;
                 A.as = kAm[stage];
-                A.gnext = stage > 0 ? kGm[stage - 1] * dt : 0.0;
+                A.gnext = stage > 0 ? kGm[stage - 1] * dt : kGm[2] * dt;
+                A.rec_elem = R.elem.as<int64_t>();
+                A.rec_w = R.w.as<double>();
+                A.rec_res = j >= 1 ? res.as<double>() + (j - 1) * R.n : nullptr;
+                A.n_rec = R.n;
                 HIP_TRY(launch_adjoint_stage(d, A, s), "adjoint kernel");
-                if (stage != 0) cc = 1 - cc;
+                if (more) cc = 1 - cc;
             }
-            if (j >= 1 && (st = add_source_and_export(j - 1)) != NDG_OK) return st;
-        }
     }
 
     // Outputs; the mesh state goes back to the initial fields.

@@ -536,6 +536,56 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
             }
         }
     }
+    /* Hanging (2:1) sides: a domain side a -> b without a neighbour or a ghost, covered by the
+     * unmatched sides b -> m and m -> a of two elements meeting at its midpoint m (the configuration
+     * h-refinement leaves behind, src/meshes/Mesh2D_t.cpp:5661-5985). Those two sides' faces take
+     * the larger element as their right element (its side then has two faces, the one starting at
+     * a first) and the larger side's own face is dropped. */
+    int64_t (*hang)[2] = (int64_t(*)[2])malloc((size_t)(4 * n_quads + 1) * sizeof(int64_t[2]));
+    char *dropped = (char *)calloc((size_t)m->F + 1, 1);
+    for (int64_t es = 0; es < 4 * n_quads; ++es) hang[es][0] = hang[es][1] = -1;
+    for (int64_t i = 0; i < n_quads; ++i)
+        for (int j = 0; j < 4; ++j) {
+            const int64_t fb = e2f[i][j];
+            if (m->fa[fb].elements[1] != -1 || m->fa[fb].elements[0] != i) continue;
+            const int64_t a = m->fa[fb].nodes[0], b = m->fa[fb].nodes[1];
+            for (int k = 0; k < n2f_count[b]; ++k) {
+                const int64_t f1 = n2f[b][k];
+                if (m->fa[f1].nodes[0] != b || m->fa[f1].elements[1] != -1 || f1 == fb) continue;
+                const int64_t mid = m->fa[f1].nodes[1];
+                int64_t f2 = -1;
+                for (int q = 0; q < n2f_count[mid]; ++q) {
+                    const o_face *c = &m->fa[n2f[mid][q]];
+                    if (c->nodes[0] == mid && c->nodes[1] == a && c->elements[1] == -1) f2 = n2f[mid][q];
+                }
+                if (f2 < 0) continue;
+                const double mx = (m->xy[a].x + m->xy[b].x) / 2, my = (m->xy[a].y + m->xy[b].y) / 2;
+                const double len = mag(sub(m->xy[b], m->xy[a]));
+                if (fabs(m->xy[mid].x - mx) > 1e-12 * len || fabs(m->xy[mid].y - my) > 1e-12 * len) continue;
+                m->fa[f1].elements[1] = i; m->fa[f1].sides[1] = j;
+                m->fa[f2].elements[1] = i; m->fa[f2].sides[1] = j;
+                hang[4 * i + j][0] = f2;
+                hang[4 * i + j][1] = f1;
+                dropped[fb] = 1;
+                break;
+            }
+        }
+    {   /* compact the face array */
+        int64_t *remap = (int64_t *)malloc((size_t)(m->F + 1) * sizeof(int64_t));
+        int64_t nf = 0;
+        for (int64_t f = 0; f < m->F; ++f) {
+            remap[f] = dropped[f] ? -1 : nf;
+            if (!dropped[f]) m->fa[nf++] = m->fa[f];
+        }
+        m->F = nf;
+        for (int64_t i = 0; i < m->n_elements_total; ++i)
+            for (int j = 0; j < (i < n_quads ? 4 : 1); ++j)
+                if (e2f[i][j] >= 0) e2f[i][j] = remap[e2f[i][j]];
+        for (int64_t es = 0; es < 4 * n_quads; ++es)
+            if (hang[es][0] >= 0) { hang[es][0] = remap[hang[es][0]]; hang[es][1] = remap[hang[es][1]]; }
+        free(remap);
+    }
+    free(dropped);
     for (int64_t i = 0; i < n_nodes; ++i) free(n2f[i]);
     free(n2f); free(n2f_count); free(n2f_cap);
 
@@ -588,6 +638,13 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
         if (i < n_quads) {
             elem_alloc_domain(e);
             for (int s = 0; s < 4; ++s) {
+                if (hang[4 * i + s][0] >= 0) {
+                    e->nf[s] = 2;
+                    e->faces[s] = (int64_t *)malloc(2 * sizeof(int64_t));
+                    e->faces[s][0] = hang[4 * i + s][0];
+                    e->faces[s][1] = hang[4 * i + s][1];
+                    continue;
+                }
                 e->nf[s] = 1;
                 e->faces[s] = (int64_t *)malloc(sizeof(int64_t));
                 e->faces[s][0] = e2f[i][s];
@@ -617,6 +674,7 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
         face_geometry(fc, fn, en);
     }
     free(e2f);
+    free(hang);
     return m;
 }
 

@@ -212,3 +212,60 @@ def test_adaptive_solve_matches_oracle(tables, periodic, bc):
     assert om.N.max() > 3
     for g, r in zip(gm.get_solution(), om.get_state()):
         assert rel(g, r) < TOL
+
+
+HANGING = [
+    ("periodic-uniform", dict(periodic=True, jitter=0.15), (5,)),
+    ("periodic-mixed", dict(periodic=True, jitter=0.1), (3, 5, 6)),
+    ("walls-mixed", dict(jitter=0.1), (4, 6)),
+]
+
+
+def hanging_build(tables, res, kw, choices, seed):
+    import meshes
+    ndg, ot = tables
+    desc = nd.square_mesh(res, x_periodic=kw.get("periodic", False), y_periodic=kw.get("periodic", False),
+                          jitter=kw.get("jitter", 0.0), seed=seed)
+    rng = np.random.default_rng(seed)
+    inner = meshes.interior_elements(desc)
+    desc.quad_N = rng.choice(choices, size=res * res).astype(np.int32)
+    fine = meshes.split_elements(desc, rng.choice(inner, size=len(inner) // 3, replace=False))
+    gm = nd.Mesh2D_t(fine, min(choices), 16, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
+    om = O.Mesh(ot, fine, min(choices))
+    return fine, gm, om
+
+
+@pytest.mark.parametrize("name,kw,choices", HANGING)
+def test_hanging_rhs_and_steps(tables, name, kw, choices):
+    """2:1 hanging faces (a side covered by two half faces of split neighbours): the interpolation
+    to the half faces with offset/scale, and the sum of the two projected half-face fluxes."""
+    say("hanging", name, choices)
+    desc, gm, om = hanging_build(tables, 8, kw, choices, 13)
+    assert gm.n_faces_ == om.n_faces
+    rng = np.random.default_rng(4)
+    q = [rng.normal(size=gm.n_dofs_per_field_) for _ in range(3)]
+    gm.set_solution(*q)
+    om.set_state(*q)
+    for g, r in zip(gm.compute_rhs(0.0), om.compute_rhs(0.0)):
+        assert rel(g, r) < TOL, name
+    gm.initial_conditions(PULSES)
+    om.initial_conditions(PULSES)
+    solver = nd.Solver2D_t(0.5, [0.0])
+    t_gpu = solver.run_steps(gm, 8, 0.0)
+    t_cpu = om.run_steps(0.5, 8, 0.0)
+    assert t_gpu == t_cpu
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL, name
+
+
+def test_hanging_fp32c(tables):
+    say("hanging fp32c")
+    desc, gm, om = hanging_build(tables, 8, dict(periodic=True, jitter=0.1), (4, 6), 17)
+    gm.initial_conditions(PULSES)
+    om.initial_conditions(PULSES)
+    gm.set_precision("fp32c")
+    solver = nd.Solver2D_t(0.5, [0.0])
+    solver.run_steps(gm, 20)
+    om.run_steps(0.5, 20)
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < 2e-5

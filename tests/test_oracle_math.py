@@ -328,3 +328,71 @@ def test_estimator_synthetic_decay(oracle_tables):
     assert abs(out[1] - sigma) < 1e-10
     zero = O.element_estimate(oracle_tables, N, np.zeros(nf * nf))
     assert np.isnan(zero[1]) and zero[2] == 0.0   # log(0): sigma NaN, no refinement (F6)
+
+
+# ---------------------------------------------------------------------------------------------
+# Hanging (2:1) faces: the h-non-conforming mortar (A12/A14 with offset/scale and two faces per
+# element side, Mesh2D_t.cpp:3803-4006)
+
+def hanging_mesh(res, N, mixed, jitter, seed):
+    import meshes
+    desc = nd.square_mesh(res, x_periodic=True, y_periodic=True, jitter=jitter, seed=seed)
+    rng = np.random.default_rng(seed)
+    inner = meshes.interior_elements(desc)
+    chosen = rng.choice(inner, size=len(inner) // 3, replace=False)
+    if mixed:
+        desc.quad_N = rng.choice([N, N + 2, N + 4], size=res * res).astype(np.int32)
+    fine = meshes.split_elements(desc, chosen)
+    T = O.Tables(16, 5)
+    return fine, T, O.Mesh(T, fine, N)
+
+
+def test_hanging_faces_are_built():
+    desc, T, m = hanging_mesh(8, 4, False, 0.0, 1)
+    # every split element's unsplit neighbour has sides covered by two small faces; the face count
+    # is that of the conforming fine mesh minus one face per hanging pair
+    K = desc.quads.shape[0]
+    assert m.K == K
+    edges = {}
+    for q in desc.quads:
+        for s in range(4):
+            edges[(int(q[s]), int(q[(s + 1) % 4]))] = True
+    unmatched = sum(1 for (a, b) in edges if (b, a) not in edges)
+    n_bars = desc.bars.shape[0]
+    # unmatched directed edges: boundary sides (n_bars) + 3 per hanging side (big + two small)
+    n_hanging = (unmatched - n_bars) // 3
+    assert n_hanging > 0
+    assert m.n_faces == (4 * K - unmatched) // 2 + n_bars + 2 * n_hanging
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_hanging_free_stream_and_polynomials(mixed):
+    """Constant states give G = 0 and continuous polynomial fields of degree <= N give the exact
+    -div f on every element whose faces are interior: the interpolation to the half faces and the
+    L2 projection back are exact for these degrees."""
+    desc, T, m = hanging_mesh(8, 4, mixed, 0.0, 3)
+    n = m.n
+    m.set_state(np.full(n, 0.75), np.zeros(n), np.zeros(n))
+    assert max(np.abs(g).max() for g in m.compute_rhs(0.0)) < 1e-11
+    Ns = m.N
+    X, Y = packed_nodes(desc, T, Ns)
+    p = X**2 + 3 * X * Y - Y**3 + 0.5
+    u = Y**2 - X * Y + 2 * X**3
+    v = X * Y + X**2 * Y
+    m.set_state(p, u, v)
+    Gp, Gu, Gv = m.compute_rhs(0.0)
+    ex = (-((-Y + 6 * X**2) + (X + X**2)), -(2 * X + 3 * Y), -(3 * X - 3 * Y**2))
+    mask = interior_mask(desc, Ns, 8)
+    for got, ref in zip((Gp, Gu, Gv), ex):
+        assert np.max(np.abs(got[mask] - ref[mask])) < 1e-9
+
+
+@pytest.mark.parametrize("mixed,jitter", [(False, 0.2), (True, 0.1)])
+def test_hanging_conservation(mixed, jitter):
+    desc, T, m = hanging_mesh(8, 4, mixed, jitter, 5)
+    rng = np.random.default_rng(9)
+    n = m.n
+    m.set_state(rng.normal(size=n), rng.normal(size=n), rng.normal(size=n))
+    Gp, Gu, Gv = m.compute_rhs(0.0)
+    W = quadrature_weights(desc, T, m.N)
+    assert abs(np.sum(W * Gp)) < 1e-12 * np.sum(W * np.abs(Gp))
