@@ -276,32 +276,30 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         }
         __syncthreads();
 
-        // P6: trace transposes (thread owns row i = l); lambda' = lambda + d lambda; the
-        // c-derivative term <d lambda, q_s>.
-        double dot = 0.0;
+        // P6: trace transposes added to d lambda (thread owns row i = l).
         if (active) {
             const int i = l;
             const double *tau = s_rho + el * 12 * NF + f * NF;
             const double t0 = tau[i], t2 = tau[6 * NF + (N - i)];
-            double *lw = s_a + el * 3 * NE + f * NE + i * NF;  // d lambda in, lambda' out
-            const double *lrow = lam + (e0 + el) * 3 * NE + f * NE + i * NF;
-            const double *qrow = qs ? qs + (e0 + el) * 3 * NE + f * NE + i * NF : nullptr;
+            double *da = s_a + el * 3 * NE + f * NE + i * NF;
 #pragma unroll
-            for (int b = 0; b < NF; ++b) {
-                const double dl = lw[b] + lm[b] * t0 + lp[b] * t2 + lp[i] * tau[3 * NF + b] +
-                                  lm[i] * tau[9 * NF + (N - b)];
-                lw[b] = lrow[b] + dl;
-                if (qrow) dot += dl * qrow[b];
+            for (int b = 0; b < NF; ++b)
+                da[b] += lm[b] * t0 + lp[b] * t2 + lp[i] * tau[3 * NF + b] + lm[i] * tau[9 * NF + (N - b)];
+        }
+        __syncthreads();
+
+        // P6b (coalesced): lambda' = lambda + d lambda, and the c-derivative term <d lambda, q_s>.
+        double dot = 0.0;
+        const double *qg = qs ? qs + e0 * 3 * NE : nullptr;
+#pragma unroll
+        for (int it = 0; it < QPT; ++it) {
+            const int r = tid + it * T;
+            if (r < nel * 3 * NE) {
+                const double dl = s_a[r];
+                if (qg) dot += dl * qg[r];
+                lreg[it] += dl;
+                s_a[r] = lreg[it];
             }
-            // misfit source of the step end this stage reaches: lambda_p += res_r w_r
-            if ((flags & kAdjSource) && f == 0)
-                for (int r = 0; r < n_rec; ++r)
-                    if (rec_elem[r] == e0 + el) {
-                        const double *w = rec_w + (int64_t)r * NE + i * NF;
-                        const double c = rec_res[r];
-#pragma unroll
-                        for (int b = 0; b < NF; ++b) lw[b] += c * w[b];
-                    }
         }
         if (cpart) {
             for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
@@ -309,12 +307,29 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         }
         __syncthreads();
         if (cpart && tid == 0) cpart[blockIdx.x] += (s_red[0] + s_red[1]) + s_red[2];
+        // misfit source of the step end this stage reaches: lambda_p += res_r w_r (receivers in
+        // order, each node always handled by the same thread)
+        if (flags & kAdjSource) {
+            for (int r = 0; r < n_rec; ++r) {
+                const int64_t le = rec_elem[r] - e0;
+                if (le < 0 || le >= nel) continue;
+                const double *w = rec_w + (int64_t)r * NE;
+                const double c = rec_res[r];
+                for (int k = tid; k < NE; k += T) s_a[le * 3 * NE + k] += c * w[k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int it = 0; it < QPT; ++it) {
+                const int r = tid + it * T;
+                if (r < nel * 3 * NE) lreg[it] = s_a[r];
+            }
+        }
 
         double *lo = lam + e0 * 3 * NE;
 #pragma unroll
         for (int it = 0; it < QPT; ++it) {
             const int r = tid + it * T;
-            if (r < nel * 3 * NE) lo[r] = lreg[it] = s_a[r];
+            if (r < nel * 3 * NE) lo[r] = lreg[it];
         }
     }
     if (!(flags & kAdjExport)) return;

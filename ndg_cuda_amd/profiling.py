@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 output (rocpd SQLite databases) for the stage kernels.
 
-    python -m ndg_cuda_amd.profiling trace <run_results.db> [--kernel SUBSTR]
+    python -m ndg_cuda_amd.profiling trace <run_results.db> [--kernel SUBSTR] [--min-us T] [--max-us T]
     python -m ndg_cuda_amd.profiling traffic --fetch DB --write DB --copy-fetch DB --copy-write DB
     python -m ndg_cuda_amd.profiling counters DB [DB ...] [--kernel SUBSTR]
 
@@ -26,10 +26,11 @@ def _db(path):
     return sqlite3.connect(path)
 
 
-def kernel_stats(path, kernel=None):
+def kernel_stats(path, kernel=None, min_us=0.0, max_us=float("inf")):
     c = _db(path)
     rows = c.execute("select name, count(*), avg(duration), min(duration), max(duration), sum(duration) "
-                     "from kernels group by name order by sum(duration) desc").fetchall()
+                     "from kernels where duration >= ? and duration <= ? group by name order by sum(duration) desc",
+                     (min_us * 1e3, min(max_us * 1e3, 1e18))).fetchall()
     out = []
     for name, n, avg, mn, mx, tot in rows:
         if kernel and kernel not in name:
@@ -64,6 +65,8 @@ def main(argv=None):
     t = sub.add_parser("trace")
     t.add_argument("db")
     t.add_argument("--kernel")
+    t.add_argument("--min-us", type=float, default=0.0, help="only dispatches at least this long")
+    t.add_argument("--max-us", type=float, default=float("inf"), help="only dispatches at most this long")
     tr = sub.add_parser("traffic")
     for k in ("fetch", "write", "copy_fetch", "copy_write"):
         tr.add_argument("--" + k.replace("_", "-"), required=True)
@@ -73,7 +76,7 @@ def main(argv=None):
     cc.add_argument("--kernel", default="stage_kernel<8, false>")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
-        for r in kernel_stats(a.db, a.kernel):
+        for r in kernel_stats(a.db, a.kernel, a.min_us, a.max_us):
             print(f"{r['calls']:6d} calls  avg {r['avg_ns'] / 1e3:10.2f} us  min {r['min_ns'] / 1e3:10.2f}  "
                   f"max {r['max_ns'] / 1e3:10.2f}  total {r['total_ns'] / 1e6:9.3f} ms  {r['kernel'][:90]}")
     elif a.cmd == "traffic":
