@@ -314,6 +314,7 @@ typedef struct {
     double *pflux, *uflux, *vflux, *pder, *uder, *vder; /* line scratch */
     double error, sigma;
     int refine, coarsen;
+    int status, rotation, split_level; /* Hilbert status/rotation (Element2D_t.h:62-66) */
 } o_elem;
 
 typedef struct {
@@ -338,6 +339,9 @@ struct o_mesh {
     int64_t *wall, *sym, *in, *out, *if_origin, *if_side, *if_dest;
     double tol_min, tol_max;
     double c; /* wave-speed scale of the right-hand side (1 = the reference) */
+    int initial_N;
+    int32_t *bar_type;  /* per ghost (bar order): kind and periodic donor bar */
+    int64_t *bar_donor;
 };
 
 static void elem_alloc_domain(o_elem *e) {
@@ -451,6 +455,68 @@ static void push(int64_t **arr, int64_t *n, int64_t v) {
     (*arr)[(*n)++] = v;
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* Hilbert curve statuses (src/functions/Hilbert_splitting.cpp; Status H, A, R, B = 0..3)       */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Hilbert_splitting.cpp:3-6, :8-11, :13-21 */
+static int first_status(int outgoing) { static const int t[4] = {3, 0, 1, 2}; return t[outgoing]; }
+static int last_status(int incoming) { static const int t[4] = {1, 2, 3, 0}; return t[incoming]; }
+static int mid_status(int incoming, int outgoing) {
+    static const int t[4][4] = {{0, 0, 1, 1}, {3, 3, 2, 2}, {3, 3, 2, 2}, {0, 0, 1, 1}};
+    return t[incoming][outgoing];
+}
+/* Hilbert_splitting.cpp:23-49 child_order[rotation][status] */
+static const int CHILD_ORDER[4][4][4] = {
+    {{0, 3, 2, 1}, {0, 1, 2, 3}, {2, 1, 0, 3}, {2, 3, 0, 1}},
+    {{3, 2, 1, 0}, {1, 2, 3, 0}, {1, 0, 3, 2}, {3, 0, 1, 2}},
+    {{2, 1, 0, 3}, {2, 3, 0, 1}, {0, 3, 2, 1}, {0, 1, 2, 3}},
+    {{1, 0, 3, 2}, {3, 0, 1, 2}, {3, 2, 1, 0}, {1, 2, 3, 0}}};
+/* Hilbert_splitting.cpp:51-81 child_statuses[rotation][status] (H A R B = 0 1 2 3) */
+static const int CHILD_STATUS[4][4][4] = {
+    {{1, 3, 0, 0}, {0, 1, 1, 2}, {2, 2, 3, 1}, {3, 0, 2, 3}},
+    {{3, 0, 0, 1}, {1, 1, 2, 0}, {2, 3, 1, 2}, {0, 2, 3, 3}},
+    {{0, 0, 1, 3}, {1, 2, 0, 1}, {3, 1, 2, 2}, {2, 3, 3, 0}},
+    {{0, 1, 3, 0}, {2, 0, 1, 1}, {1, 2, 2, 3}, {3, 3, 0, 2}}};
+
+/* side through which the neighbouring centre d lies (Mesh2D_t.cpp:4083-4100) */
+static int exit_side(double dx, double dy) {
+    if (fabs(dy) > fabs(dx)) return dy < 0 ? 0 : 2;
+    return dx < 0 ? 3 : 1;
+}
+
+/* src/meshes/Mesh2D_t.cpp:4055-4178 compute_element_status */
+static void element_status(o_mesh *m) {
+    static const double PI = 3.14159265358979323846;
+    const double targets[4] = {-3 * PI / 4, -PI / 4, PI / 4, 3 * PI / 4};
+    const int64_t K = m->K;
+    for (int64_t i = 0; i < K; ++i) {
+        o_elem *e = &m->el[i];
+        long rotation = 0;
+        for (int k = 0; k < 4; ++k) {
+            const double dx = m->xy[e->nodes[k]].x - e->cx, dy = m->xy[e->nodes[k]].y - e->cy;
+            const double angle = atan2(dy, dx);
+            const double offset = atan2(sin(angle - targets[k]), cos(angle - targets[k]));
+            const long n_turns = lround(offset * 2 / PI + 4 * (offset < 0));
+            rotation += n_turns * (n_turns < 4);
+        }
+        rotation /= 4;
+        e->rotation = (int)rotation;
+        if (K == 1) {
+            e->status = 0;
+        } else if (i == 0) {
+            e->status = first_status(exit_side(m->el[1].cx - e->cx, m->el[1].cy - e->cy));
+        } else if (i == K - 1) {
+            e->status = last_status(exit_side(m->el[i - 1].cx - e->cx, m->el[i - 1].cy - e->cy));
+        } else {
+            const int in = exit_side(m->el[i - 1].cx - e->cx, m->el[i - 1].cy - e->cy);
+            const int out = exit_side(m->el[i + 1].cx - e->cx, m->el[i + 1].cy - e->cy);
+            e->status = mid_status(in, out);
+        }
+        e->split_level = 0;
+    }
+}
+
 o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const double *xy,
                       int64_t n_quads, const int64_t *quads, const int32_t *quad_N, int64_t n_bars,
                       const int64_t *bars, const int32_t *bar_type, const int64_t *bar_donor,
@@ -460,6 +526,13 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
     m->tol_min = tol_min;
     m->tol_max = tol_max;
     m->c = 1.0;
+    m->initial_N = initial_N;
+    m->bar_type = (int32_t *)malloc((size_t)(n_bars + 1) * sizeof(int32_t));
+    m->bar_donor = (int64_t *)malloc((size_t)(n_bars + 1) * sizeof(int64_t));
+    for (int64_t b = 0; b < n_bars; ++b) {
+        m->bar_type[b] = bar_type[b];
+        m->bar_donor[b] = bar_donor ? bar_donor[b] : -1;
+    }
     m->n_nodes = n_nodes;
     m->xy = (v2 *)malloc((size_t)n_nodes * sizeof(v2));
     for (int64_t i = 0; i < n_nodes; ++i) { m->xy[i].x = xy[2 * i]; m->xy[i].y = xy[2 * i + 1]; }
@@ -662,6 +735,7 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
             boundary_geometry(e, P, t->tab[O_NODES][e->N]);
         }
     }
+    element_status(m);
     for (int64_t f = 0; f < m->F; ++f) {
         o_face *fc = &m->fa[f];
         face_alloc(fc);
@@ -688,6 +762,7 @@ void o_mesh_destroy(o_mesh *m) {
     free(m->el); free(m->fa); free(m->xy);
     free(m->wall); free(m->sym); free(m->in); free(m->out);
     free(m->if_origin); free(m->if_side); free(m->if_dest);
+    free(m->bar_type); free(m->bar_donor);
     free(m);
 }
 
@@ -1324,11 +1399,13 @@ static void ghost_resize(o_elem *g, int N) {
     elem_alloc_ghost(g);
 }
 
+static int64_t adapt_split(o_mesh *m, int N_max, int max_split_level);
+
 int64_t o_adapt(o_mesh *m, int N_max, int max_split_level) {
-    /* Element2D_t.cpp:802-812: h-refinement (not restated) is refine && sigma < 1 && split level
-     * (0 here: elements are never split) < max_split_level */
+    /* Element2D_t.cpp:802-812 would_h_refine: refine && sigma < 1 && split_level < max */
     for (int64_t i = 0; i < m->K; ++i)
-        if (m->el[i].refine && m->el[i].sigma < 1.0 && 0 < max_split_level) return -1;
+        if (m->el[i].refine && m->el[i].sigma < 1.0 && m->el[i].split_level < max_split_level)
+            return adapt_split(m, N_max, max_split_level);
     int64_t count = 0;
     for (int64_t i = 0; i < m->K; ++i) {
         o_elem *o = &m->el[i];
@@ -1377,6 +1454,302 @@ int64_t o_adapt(o_mesh *m, int N_max, int max_split_level) {
         }
     }
     return count;
+}
+
+/* quad_map.tpp:3-8 */
+static v2 quad_map_pt(double xi, double eta, const v2 *P) {
+    v2 r;
+    r.x = (P[0].x * (1 - xi) * (1 - eta) + P[1].x * (xi + 1) * (1 - eta) + P[2].x * (xi + 1) * (eta + 1) +
+           P[3].x * (1 - xi) * (eta + 1)) / 4;
+    r.y = (P[0].y * (1 - xi) * (1 - eta) + P[1].y * (xi + 1) * (1 - eta) + P[2].y * (xi + 1) * (eta + 1) +
+           P[3].y * (1 - xi) * (eta + 1)) / 4;
+    return r;
+}
+
+static double cross2(v2 a, v2 b) { return a.x * b.y - a.y * b.x; }
+
+/* inverse_quad_map.tpp:46-88 (closed-form inverse of the bilinear map) */
+static v2 inverse_quad_map(v2 g, const v2 *P) {
+    const v2 e = sub(P[1], P[0]), f = sub(P[3], P[0]);
+    const v2 gg = {P[0].x - P[1].x + P[2].x - P[3].x, P[0].y - P[1].y + P[2].y - P[3].y};
+    const v2 h = sub(g, P[0]);
+    const double k2 = cross2(gg, f);
+    const double k1 = cross2(e, f) + cross2(h, gg);
+    const double k0 = cross2(h, e);
+    v2 r;
+    if (fabs(k2) < DBL_MIN) {
+        const double ux = e.x * k1 - gg.x * k0, uy = e.y * k1 - gg.y * k0;
+        const double u = fabs(ux) > fabs(uy) ? (h.x * k1 + f.x * k0) / ux : (h.y * k1 + f.y * k0) / uy;
+        r.x = u * 2 - 1;
+        r.y = -k0 / k1 * 2 - 1;
+        return r;
+    }
+    double w = k1 * k1 - 4 * k0 * k2;
+    if (w < 0.0) { r.x = -1; r.y = -1; return r; }
+    w = sqrt(w);
+    const double ik2 = 1 / (2 * k2);
+    double v = (-k1 - w) * ik2;
+    double ux = e.x + gg.x * v, uy = e.y + gg.y * v;
+    double u = fabs(ux) > fabs(uy) ? (h.x - f.x * v) / ux : (h.y - f.y * v) / uy;
+    if (u < 0.0 || u > 1.0 || v < 0.0 || v > 1.0) {
+        v = (-k1 + w) * ik2;
+        ux = e.x + gg.x * v;
+        uy = e.y + gg.y * v;
+        u = fabs(ux) > fabs(uy) ? (h.x - f.x * v) / ux : (h.y - f.y * v) / uy;
+    }
+    r.x = u * 2 - 1;
+    r.y = v * 2 - 1;
+    return r;
+}
+
+/* Element2D_t.cpp:345-462 interpolate_from (other geometry): each new node is mapped to the other
+ * element's reference coordinates (quad_map, inverse_quad_map), then the same exact-node /
+ * row / column / tensor barycentric branches as interpolate_p. */
+static void interpolate_h(const o_tables *t, const o_elem *o, const v2 *Po, o_elem *e, const v2 *Pe) {
+    const int No = o->N, N = e->N, nfo = No + 1, nf = N + 1;
+    const double *xo = t->tab[O_NODES][No], *lo = t->tab[O_BARY][No], *xn = t->tab[O_NODES][N];
+    const double *src[6] = {o->p, o->u, o->v, o->Gp, o->Gu, o->Gv};
+    double *dst[6] = {e->p, e->u, e->v, e->Gp, e->Gu, e->Gv};
+    for (int i = 0; i <= N; ++i)
+        for (int j = 0; j <= N; ++j) {
+            const v2 gpt = quad_map_pt(xn[i], xn[j], Pe);
+            const v2 loc = inverse_quad_map(gpt, Po);
+            const double X = loc.x, Y = loc.y;
+            int col = -1, row = -1;
+            for (int mm = 0; mm <= No; ++mm) {
+                if (almost_equal(X, xo[mm])) col = mm;
+                if (almost_equal(Y, xo[mm])) row = mm;
+            }
+            const int k = i * nf + j;
+            if (row != -1 && col != -1) {
+                for (int f = 0; f < 6; ++f) dst[f][k] = src[f][col * nfo + row];
+            } else if (row != -1) {
+                double num[6] = {0, 0, 0, 0, 0, 0}, den = 0.0;
+                for (int mm = 0; mm <= No; ++mm) {
+                    const double tt = lo[mm] / (X - xo[mm]);
+                    for (int f = 0; f < 6; ++f) num[f] += tt * src[f][mm * nfo + row];
+                    den += tt;
+                }
+                for (int f = 0; f < 6; ++f) dst[f][k] = num[f] / den;
+            } else if (col != -1) {
+                double num[6] = {0, 0, 0, 0, 0, 0}, den = 0.0;
+                for (int n = 0; n <= No; ++n) {
+                    const double tt = lo[n] / (Y - xo[n]);
+                    for (int f = 0; f < 6; ++f) num[f] += tt * src[f][col * nfo + n];
+                    den += tt;
+                }
+                for (int f = 0; f < 6; ++f) dst[f][k] = num[f] / den;
+            } else {
+                for (int f = 0; f < 6; ++f) dst[f][k] = 0.0;
+                double dx = 0.0, dy = 0.0;
+                for (int q = 0; q <= No; ++q) {
+                    dx += lo[q] / (X - xo[q]);
+                    dy += lo[q] / (Y - xo[q]);
+                }
+                for (int mm = 0; mm <= No; ++mm) {
+                    const double tx = lo[mm] / ((X - xo[mm]) * dx);
+                    for (int n = 0; n <= No; ++n) {
+                        const double ty = lo[n] / ((Y - xo[n]) * dy);
+                        for (int f = 0; f < 6; ++f) dst[f][k] += src[f][mm * nfo + n] * tx * ty;
+                    }
+                }
+            }
+        }
+}
+
+static int64_t edge_mid_find(const int64_t *keys, int64_t n, int64_t a, int64_t b) {
+    const int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+    for (int64_t k = 0; k < n; ++k)
+        if (keys[3 * k] == lo && keys[3 * k + 1] == hi) return keys[3 * k + 2];
+    return -1;
+}
+
+/* The splitting branch of Mesh2D_t::adapt (src/meshes/Mesh2D_t.cpp:1993-2268, hp_adapt
+ * :5186-5659, split_boundaries :6232-6343): elements with would_h_refine become four children
+ * (corner s: {corner s, midpoint of side s, centre, midpoint of side s-1} in local positions s,
+ * s+1, s+2, s+3), placed at position child_order[s] of the Hilbert child order, with the parent's
+ * order, split level + 1, child status and the parent's rotation, state interpolated from the
+ * parent; elements with would_p_refine are raised by 2 orders; a boundary ghost on a split side
+ * becomes two ghosts (geometry recomputed), a ghost whose order changes is resized and its
+ * geometry recomputed. The new topology (faces, 2:1 hanging sides) is built from the new element
+ * and ghost lists as at mesh creation. Splitting an element with a periodic side, or one that
+ * would leave a side covered by more than two faces, is not restated (returns -2, nothing
+ * changed). Returns the number of split plus raised elements. */
+static int64_t adapt_split(o_mesh *m, int N_max, int max_split_level) {
+    const int64_t K = m->K, NB = m->n_elements_total - m->K;
+    char *split = (char *)calloc((size_t)K + 1, 1);
+    char *raise = (char *)calloc((size_t)K + 1, 1);
+    int64_t n_split = 0, n_raise = 0;
+    for (int64_t i = 0; i < K; ++i) {
+        const o_elem *e = &m->el[i];
+        if (e->refine && e->sigma < 1.0 && e->split_level < max_split_level) { split[i] = 1; ++n_split; }
+        else if (e->refine && e->sigma >= 1.0 && e->N + 2 <= N_max) { raise[i] = 1; ++n_raise; }
+    }
+    /* limits of this restatement */
+    for (int64_t i = 0; i < K; ++i) {
+        if (!split[i]) continue;
+        const o_elem *e = &m->el[i];
+        for (int s = 0; s < 4; ++s)
+            for (int fk = 0; fk < e->nf[s]; ++fk) {
+                const o_face *f = &m->fa[e->faces[s][fk]];
+                const int64_t other = f->elements[0] == i ? f->elements[1] : f->elements[0];
+                if (other >= K && m->bar_type[other - K] == O_BC_PERIODIC) { free(split); free(raise); return -2; }
+                /* a half of a larger unsplit neighbour's side would be split again */
+                if (other < K && !split[other] && e->nf[s] == 1) {
+                    const o_elem *nb = &m->el[other];
+                    const int ns = (int)(f->elements[0] == i ? f->sides[1] : f->sides[0]);
+                    if (nb->nf[ns] == 2) { free(split); free(raise); return -2; }
+                }
+            }
+    }
+    /* new nodes: existing, then per split element its side midpoints (shared) and centre */
+    int64_t cap_nodes = m->n_nodes + 5 * n_split + 1;
+    double *xy = (double *)malloc((size_t)cap_nodes * 2 * sizeof(double));
+    for (int64_t k = 0; k < m->n_nodes; ++k) { xy[2 * k] = m->xy[k].x; xy[2 * k + 1] = m->xy[k].y; }
+    int64_t n_nodes = m->n_nodes;
+    int64_t *mids = (int64_t *)malloc((size_t)(4 * K + 4 * n_split + 1) * 3 * sizeof(int64_t));
+    int64_t n_mids = 0;
+    /* midpoints already present: hanging sides (two faces) */
+    for (int64_t i = 0; i < K; ++i)
+        for (int s = 0; s < 4; ++s)
+            if (m->el[i].nf[s] == 2) {
+                const int64_t a = m->el[i].nodes[s], b = m->el[i].nodes[(s + 1) % 4];
+                mids[3 * n_mids] = a < b ? a : b; mids[3 * n_mids + 1] = a < b ? b : a;
+                mids[3 * n_mids + 2] = m->fa[m->el[i].faces[s][0]].nodes[0];
+                ++n_mids;
+            }
+    const int64_t Knew = K + 3 * n_split;
+    int64_t *quads = (int64_t *)malloc((size_t)Knew * 4 * sizeof(int64_t));
+    int32_t *qN = (int32_t *)malloc((size_t)Knew * sizeof(int32_t));
+    int64_t *parent = (int64_t *)malloc((size_t)Knew * sizeof(int64_t));
+    int *child_status = (int *)malloc((size_t)Knew * sizeof(int));
+    int64_t pos = 0;
+    for (int64_t i = 0; i < K; ++i) {
+        const o_elem *e = &m->el[i];
+        if (!split[i]) {
+            for (int k = 0; k < 4; ++k) quads[4 * pos + k] = e->nodes[k];
+            qN[pos] = e->N + (raise[i] ? 2 : 0);
+            parent[pos] = i;
+            child_status[pos] = -1;
+            ++pos;
+            continue;
+        }
+        int64_t mid[4];
+        for (int s = 0; s < 4; ++s) {
+            const int64_t a = e->nodes[s], b = e->nodes[(s + 1) % 4];
+            int64_t md = edge_mid_find(mids, n_mids, a, b);
+            if (md < 0) {
+                md = n_nodes++;
+                xy[2 * md] = (xy[2 * a] + xy[2 * b]) / 2;
+                xy[2 * md + 1] = (xy[2 * a + 1] + xy[2 * b + 1]) / 2;
+                mids[3 * n_mids] = a < b ? a : b; mids[3 * n_mids + 1] = a < b ? b : a; mids[3 * n_mids + 2] = md;
+                ++n_mids;
+            }
+            mid[s] = md;
+        }
+        const int64_t c = n_nodes++;
+        double cx = 0.0, cy = 0.0;
+        for (int k = 0; k < 4; ++k) { cx += xy[2 * e->nodes[k]]; cy += xy[2 * e->nodes[k] + 1]; }
+        xy[2 * c] = cx / 4;
+        xy[2 * c + 1] = cy / 4;
+        const int *order = CHILD_ORDER[e->rotation][e->status];
+        const int *stat = CHILD_STATUS[e->rotation][e->status];
+        for (int s = 0; s < 4; ++s) {
+            const int64_t at = pos + order[s];
+            quads[4 * at + s] = e->nodes[s];
+            quads[4 * at + (s + 1) % 4] = mid[s];
+            quads[4 * at + (s + 2) % 4] = c;
+            quads[4 * at + (s + 3) % 4] = mid[(s + 3) % 4];
+            qN[at] = e->N;
+            parent[at] = i;
+            child_status[at] = stat[s];
+        }
+        pos += 4;
+    }
+    /* ghosts: split on split sides (two consecutive ghosts), else kept; resized ones and new ones
+     * get fresh geometry, the others keep a zero scaling factor an earlier resize left */
+    int64_t *bars = (int64_t *)malloc((size_t)(2 * NB + 1) * 2 * sizeof(int64_t));
+    int32_t *btype = (int32_t *)malloc((size_t)(2 * NB + 1) * sizeof(int32_t));
+    int64_t *bdonor = (int64_t *)malloc((size_t)(2 * NB + 1) * sizeof(int64_t));
+    char *bzero = (char *)calloc((size_t)(2 * NB + 1), 1);
+    int64_t *bremap = (int64_t *)malloc((size_t)(NB + 1) * sizeof(int64_t));
+    int64_t nb = 0;
+    for (int64_t b = 0; b < NB; ++b) {
+        const o_elem *g = &m->el[K + b];
+        const o_face *f = &m->fa[g->faces[0][0]];
+        const int64_t src = f->elements[0];
+        bremap[b] = nb;
+        if (split[src]) {
+            const int64_t md = edge_mid_find(mids, n_mids, g->nodes[0], g->nodes[1]);
+            bars[2 * nb] = g->nodes[0]; bars[2 * nb + 1] = md; btype[nb] = m->bar_type[b]; bdonor[nb] = -1; ++nb;
+            bars[2 * nb] = md; bars[2 * nb + 1] = g->nodes[1]; btype[nb] = m->bar_type[b]; bdonor[nb] = -1; ++nb;
+        } else {
+            bars[2 * nb] = g->nodes[0]; bars[2 * nb + 1] = g->nodes[1];
+            btype[nb] = m->bar_type[b];
+            bdonor[nb] = m->bar_donor[b];
+            bzero[nb] = !raise[src] && g->sf[0][0] == 0.0;
+            ++nb;
+        }
+    }
+    for (int64_t b = 0; b < nb; ++b)
+        if (btype[b] == O_BC_PERIODIC) bdonor[b] = bremap[bdonor[b]];
+    o_mesh *nm = o_mesh_create(m->t, m->initial_N, n_nodes, xy, Knew, quads, qN, nb, bars, btype, bdonor,
+                               m->tol_min, m->tol_max);
+    nm->c = m->c;
+    for (int64_t k = 0; k < Knew; ++k) {
+        o_elem *e = &nm->el[k];
+        const o_elem *o = &m->el[parent[k]];
+        v2 Pe[4], Po[4];
+        for (int q = 0; q < 4; ++q) { Pe[q] = nm->xy[e->nodes[q]]; Po[q] = m->xy[o->nodes[q]]; }
+        if (child_status[k] >= 0) {
+            e->status = child_status[k];
+            e->rotation = o->rotation;
+            e->split_level = o->split_level + 1;
+            interpolate_h(m->t, o, Po, e, Pe);
+        } else {
+            e->status = o->status;
+            e->rotation = o->rotation;
+            e->split_level = o->split_level;
+            if (e->N != o->N) {
+                interpolate_p(m->t, o, e);
+            } else {
+                const int ne = (e->N + 1) * (e->N + 1);
+                memcpy(e->p, o->p, ne * sizeof(double)); memcpy(e->u, o->u, ne * sizeof(double));
+                memcpy(e->v, o->v, ne * sizeof(double)); memcpy(e->Gp, o->Gp, ne * sizeof(double));
+                memcpy(e->Gu, o->Gu, ne * sizeof(double)); memcpy(e->Gv, o->Gv, ne * sizeof(double));
+            }
+        }
+    }
+    for (int64_t b = 0; b < nb; ++b)
+        if (bzero[b]) {
+            o_elem *g = &nm->el[Knew + b];
+            for (int k = 0; k <= g->N; ++k) g->sf[0][k] = 0.0;
+        }
+    /* swap the new mesh in */
+    o_mesh tmp = *m;
+    *m = *nm;
+    *nm = tmp;
+    o_mesh_destroy(nm);
+    free(xy); free(mids); free(quads); free(qN); free(parent); free(child_status);
+    free(bars); free(btype); free(bdonor); free(bzero); free(bremap); free(split); free(raise);
+    return n_split + n_raise;
+}
+
+void o_mesh_get_elements(const o_mesh *m, int64_t *nodes, int32_t *status, int32_t *rotation, double *xy) {
+    for (int64_t i = 0; i < m->K; ++i) {
+        for (int k = 0; k < 4; ++k) nodes[4 * i + k] = m->el[i].nodes[k];
+        if (status) status[i] = m->el[i].status;
+        if (rotation) rotation[i] = m->el[i].rotation;
+    }
+    if (xy)
+        for (int64_t k = 0; k < m->n_nodes; ++k) { xy[2 * k] = m->xy[k].x; xy[2 * k + 1] = m->xy[k].y; }
+}
+int64_t o_mesh_n_nodes(const o_mesh *m) { return m->n_nodes; }
+void o_mesh_set_flags(o_mesh *m, const int32_t *refine, const double *sigma) {
+    for (int64_t i = 0; i < m->K; ++i) {
+        m->el[i].refine = refine[i];
+        m->el[i].sigma = sigma[i];
+    }
 }
 
 /* include/solvers/Solver2D_t.tpp:37-100 with adaptation: Δt from get_delta_t every step, the last

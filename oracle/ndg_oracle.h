@@ -93,13 +93,21 @@ void o_run_steps(o_mesh *m, double cfl, int64_t n_steps, double *time_io);
 
 /* Pressure estimator per domain element (src/entities/Element2D_t.cpp:274-343). */
 void o_estimate_error(o_mesh *m, double *error, double *sigma, int32_t *refine, int32_t *coarsen);
-/* One adaptation without splitting (src/meshes/Mesh2D_t.cpp:1644-1706, p_adapt :5026-5049): from
- * the last estimate, elements with refine && sigma >= 1 && N + 2 <= N_max are raised by 2 orders
- * (solution interpolated, geometry recomputed); ghosts and faces follow (their ghost storage,
- * including the edge scaling factor, becomes zero when its order changes). Returns the number of
- * raised elements, or -1 (nothing changed) if an element would split. */
+/* One adaptation (src/meshes/Mesh2D_t.cpp:1644-2268) from the last estimate. Without a split
+ * request (p_adapt, :5026-5049): elements with refine && sigma >= 1 && N + 2 <= N_max are raised by
+ * 2 orders (solution interpolated, geometry recomputed); ghosts and faces follow (a resized ghost's
+ * storage, including its edge scaling factor, becomes zero). With one (hp_adapt): elements with
+ * refine && sigma < 1 && split_level < max_split_level split into 4 children in Hilbert order, the
+ * others p-refine as above. Returns the number of changed elements, or -2 (nothing changed) for a
+ * split this restatement does not cover (a periodic side, or a side that would get more than two
+ * faces). */
 int64_t o_adapt(o_mesh *m, int N_max, int max_split_level);
-/* The reference time loop with an adaptation every adapt_interval steps; -1 if one would split. */
+/* Element nodes (4 per element), Hilbert status and rotation (may be NULL), node coordinates. */
+void o_mesh_get_elements(const o_mesh *m, int64_t *nodes, int32_t *status, int32_t *rotation, double *xy);
+int64_t o_mesh_n_nodes(const o_mesh *m);
+/* Sets the estimator's refine flags and decay rates directly (tests of the adaptation rules). */
+void o_mesh_set_flags(o_mesh *m, const int32_t *refine, const double *sigma);
+/* The reference time loop with an adaptation every adapt_interval steps; -1 if one is not covered. */
 int64_t o_solve_adaptive(o_mesh *m, double cfl, double t_end, int64_t adapt_interval, int N_max,
                          int max_split_level, double *t_final);
 

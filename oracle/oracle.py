@@ -45,6 +45,8 @@ def lib():
             "o_get_flux_traces": ([vp, vp], None), "o_get_delta_t": ([vp, f64], f64),
             "o_compute_rhs": ([vp, f64], None), "o_mesh_set_wave_speed": ([vp, f64], None),
             "o_adapt": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int64),
+            "o_mesh_get_elements": ([vp, vp, vp, vp, vp], None), "o_mesh_n_nodes": ([vp], ctypes.c_int64),
+            "o_mesh_set_flags": ([vp, vp, vp], None),
             "o_solve_adaptive": ([vp, f64, f64, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_double)], ctypes.c_int64), "o_step": ([vp, f64, f64], None),
             "o_solve": ([vp, f64, f64, vp], i64), "o_run_steps": ([vp, f64, i64, vp], None),
@@ -183,7 +185,22 @@ class Mesh:
         self._refresh()
         return n, tf.value
 
+    def set_flags(self, refine, sigma):
+        r = np.ascontiguousarray(refine, dtype=np.int32)
+        sg = _f(sigma)
+        lib().o_mesh_set_flags(self.h, _p(r), _p(sg))
+
+    def elements(self):
+        """(element nodes (K, 4), Hilbert status (K,), rotation (K,), node coordinates (n, 2))."""
+        nodes = np.empty((self.K, 4), dtype=np.int64)
+        st, rot = np.empty(self.K, dtype=np.int32), np.empty(self.K, dtype=np.int32)
+        xy = np.empty((lib().o_mesh_n_nodes(self.h), 2))
+        lib().o_mesh_get_elements(self.h, _p(nodes), _p(st), _p(rot), _p(xy))
+        return nodes, st, rot, xy
+
     def _refresh(self):
+        self.K = lib().o_mesh_n_elements(self.h)
+        self.N = np.empty(self.K, dtype=np.int32)
         self.n = lib().o_mesh_n_nodes_total(self.h)
         lib().o_mesh_get_N(self.h, _p(self.N))
 

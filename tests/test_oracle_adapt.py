@@ -1,5 +1,6 @@
-"""The oracle's p-adaptation (src/meshes/Mesh2D_t.cpp:1644-1706, :5026-5049, :7482-7538;
-Element2D_t.cpp:464-579, :735-741, :802-812) pinned by mathematics (CPU only).
+"""The oracle's adaptation (src/meshes/Mesh2D_t.cpp:1644-2268, :5026-5049, :5186-5659,
+:6232-6343, :7482-7538; Element2D_t.cpp:345-579, :735-741, :802-812; Hilbert_splitting.cpp)
+pinned by mathematics (CPU only).
 
 - interpolate_from to order N + 2 reproduces the old element polynomial at the new nodes (checked
   against a numpy Legendre-Vandermonde evaluation, independent of the barycentric formula);
@@ -9,7 +10,10 @@ Element2D_t.cpp:464-579, :735-741, :802-812) pinned by mathematics (CPU only).
   the same state, except that ghosts whose order changed have zero storage (the reference
   resizes them without recomputing the edge scaling factor): with every element raised, walls act
   as zero ghosts, which a directly built mesh with BC_NONE reproduces exactly; periodic meshes are
-  unchanged in kind.
+  unchanged in kind;
+- splitting keeps the Hilbert curve continuous (element statuses, child order and child status
+  tables), carries polynomial fields exactly to the children, and keeps the operator's exactness
+  on the resulting hanging faces.
 """
 import numpy as np
 import pytest
@@ -66,14 +70,12 @@ def test_decisions_follow_the_reference_rules(oracle_tables):
     om.estimate_error()
     assert om.adapt(N_max=N + 1) == 0                  # N + 2 > N_max: nothing changes
     assert (om.N == N).all()
-    # a rough field (sigma < 1 somewhere) asks for splitting, which the no-split branch refuses
+    # a rough field (sigma < 1 somewhere): with max_split_level = 0 the estimate only raises the
+    # refine & sigma >= 1 elements (the no-split branch)
     rng = np.random.default_rng(0)
     om.set_state(*(rng.normal(size=om.n) for _ in range(3)))
     _, sigma, refine, _ = om.estimate_error()
     assert (refine & (sigma < 1)).any()
-    assert om.adapt(N_max=12, max_split_level=1) == -1
-    assert (om.N == N).all()
-    # with max_split_level = 0 the same estimate raises only the refine & sigma >= 1 elements
     expect = int((refine & (sigma >= 1)).sum())
     assert om.adapt(N_max=12, max_split_level=0) == expect
     assert int((om.N == N + 2).sum()) == expect
@@ -124,3 +126,88 @@ def test_solve_adaptive_runs_the_reference_loop(oracle_tables):
     assert steps > 0 and t == pytest.approx(0.05, abs=0)
     assert om.N.max() > N
     assert abs(mass() - m0) < 1e-12 * abs(m0)
+
+
+# ---------------------------------------------------------------------------------------------
+# Splitting (hp_adapt, Mesh2D_t.cpp:5186-5659; Hilbert child order, Hilbert_splitting.cpp)
+
+def split_all(om, level_cap):
+    om.set_flags(np.ones(om.K, dtype=np.int32), np.zeros(om.K))
+    return om.adapt(N_max=16, max_split_level=level_cap)
+
+
+def assert_hilbert_path(om):
+    """Consecutive elements share an edge: the element order is still a continuous curve."""
+    nodes, _, _, _ = om.elements()
+    for k in range(om.K - 1):
+        assert len(set(nodes[k].tolist()) & set(nodes[k + 1].tolist())) == 2, k
+
+
+@pytest.mark.parametrize("res", [4, 8])
+def test_split_keeps_the_hilbert_curve(oracle_tables, res):
+    """Statuses from compute_element_status and the child order / child status tables: splitting
+    every element once and then again keeps the element sequence a continuous path."""
+    desc, om = mesh(oracle_tables, res, 3, jitter=0.0)
+    assert_hilbert_path(om)
+    assert split_all(om, 2) == res * res
+    assert om.K == 4 * res * res
+    assert_hilbert_path(om)
+    assert split_all(om, 2) == 4 * res * res
+    assert om.K == 16 * res * res
+    assert_hilbert_path(om)
+    assert split_all(om, 2) == 0 or om.K == 16 * res * res
+
+
+def test_split_interpolates_and_conserves(oracle_tables):
+    """Children get the parent polynomial at their nodes (inverse_quad_map + barycentric
+    interpolation), so a polynomial field of degree <= N is carried exactly, and the operator on
+    the split mesh (hanging faces where a neighbour is not split) keeps its polynomial exactness."""
+    N = 4
+    desc, om = mesh(oracle_tables, 8, N, periodic=True, jitter=0.0)
+    import test_oracle_math as tm
+    X, Y = tm.packed_nodes(desc, oracle_tables, om.N)
+    f = lambda x, y: (x**2 + 3 * x * y - y**3 + 0.5, y**2 - x * y + 2 * x**3, x * y + x**2 * y)
+    om.set_state(*f(X, Y))
+    inner = np.zeros(om.K, dtype=np.int32)
+    inner[[e for e in range(om.K) if e % 3 == 0]] = 1
+    # splitting must avoid the periodic boundary: keep only interior elements
+    import meshes
+    interior = set(meshes.interior_elements(desc))
+    inner = np.array([1 if (e in interior and e % 3 == 0) else 0 for e in range(om.K)], dtype=np.int32)
+    om.set_flags(inner, np.zeros(om.K))
+    assert om.adapt(N_max=16, max_split_level=1) == int(inner.sum())
+    nodes, _, _, xy = om.elements()
+    fine = nd.MeshDescription(xy, nodes, desc.bars, desc.bar_type, desc.bar_donor)
+    Xf, Yf = tm.packed_nodes(fine, oracle_tables, om.N)
+    for got, ref in zip(om.get_state(), f(Xf, Yf)):
+        assert np.max(np.abs(got - ref)) < 1e-12
+    Gp, Gu, Gv = om.compute_rhs(0.0)
+    ex = (-((-Yf + 6 * Xf**2) + (Xf + Xf**2)), -(2 * Xf + 3 * Yf), -(3 * Xf - 3 * Yf**2))
+    mask = tm.interior_mask(fine, om.N, 8)
+    for got, ref in zip((Gp, Gu, Gv), ex):
+        assert np.max(np.abs(got[mask] - ref[mask])) < 1e-9
+
+
+def test_split_walls_and_limits(oracle_tables):
+    """Splitting an element on a wall splits its ghost in two (geometry recomputed); a periodic
+    side or a half side of an unsplit neighbour is outside this restatement."""
+    desc, om = mesh(oracle_tables, 4, 3, jitter=0.1)
+    flags = np.zeros(om.K, dtype=np.int32)
+    flags[0] = 1  # the first Hilbert element is a corner element: two wall sides
+    om.set_flags(flags, np.zeros(om.K))
+    assert om.adapt(N_max=16, max_split_level=1) == 1
+    assert om.K == 16 + 3
+    om.set_state(*(np.ones(om.n) for _ in range(3)))
+    assert all(np.isfinite(g).all() for g in om.compute_rhs(0.0))
+    # with max level 1 the four children stay, every other element splits (their sides facing the
+    # children become conforming again)
+    om.set_flags(np.ones(om.K, dtype=np.int32), np.zeros(om.K))
+    assert om.adapt(N_max=16, max_split_level=1) == 15
+    assert om.K == 64
+    assert_hilbert_path(om)
+    om.set_state(*(np.ones(om.n) for _ in range(3)))
+    assert all(np.isfinite(g).all() for g in om.compute_rhs(0.0))
+    # periodic sides are refused
+    d2, om2 = mesh(oracle_tables, 4, 3, periodic=True)
+    om2.set_flags(np.ones(om2.K, dtype=np.int32), np.zeros(om2.K))
+    assert om2.adapt(N_max=16, max_split_level=1) == -2
