@@ -151,6 +151,12 @@ ndg_status ndg_set_ic(ndg_mesh *mesh, const ndg_ic_desc *ic);
 /* Mesh2D_t::get_solution's data without interpolation: packed p, u, v (host arrays). */
 ndg_status ndg_set_state(ndg_mesh *mesh, const double *p, const double *u, const double *v);
 ndg_status ndg_get_state(ndg_mesh *mesh, double *p, double *u, double *v);
+/* Mesh2D_t::get_solution (src/meshes/Mesh2D_t.cpp:4237-4249, Element2D_t::interpolate_solution
+ * src/entities/Element2D_t.cpp:581-604): the state interpolated to the tables' n x n uniform output
+ * points of every element, with their coordinates. Host arrays of n_elements * n * n values,
+ * element e's point (i, j) at e * n * n + i * n + j (caller's element order); a null array is
+ * skipped. Requires n >= 2. */
+ndg_status ndg_get_solution(ndg_mesh *mesh, double *x, double *y, double *p, double *u, double *v);
 /* Element traces (Mesh2D_t::interpolate_to_boundaries, src/meshes/Mesh2D_t.cpp:3794-3801) of the
  * current state, packed [element][side][field][N+1]. */
 ndg_status ndg_get_traces(ndg_mesh *mesh, double *out);

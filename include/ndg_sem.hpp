@@ -154,27 +154,24 @@ class Mesh2D_t {
         d.tolerance_max = tolerance_max;
         check(ndg_mesh_create(&d, NDG.handle(), &handle_));
         if (stream) check(ndg_mesh_set_stream(handle_, stream));
-        int64_t faces = 0, dofs = 0, ne = 0;
-        int32_t nmin = 0, nmax = 0;
-        check(ndg_mesh_info(handle_, &ne, &faces, &dofs, &nmin, &nmax));
-        n_elements_ = (std::size_t)ne;
-        n_faces_ = (std::size_t)faces;
-        n_dofs_per_field_ = (std::size_t)dofs;
         if (adaptivity_interval != 0) check(ndg_mesh_set_adaptivity(handle_, (int64_t)adaptivity_interval));
         refresh();
     }
 
     // Sizes and element orders (they change with adapt and with an adaptive solve).
     void refresh() {
-        int64_t dofs = 0;
-        check(ndg_mesh_info(handle_, nullptr, nullptr, &dofs, nullptr, nullptr));
+        int64_t ne = 0, faces = 0, dofs = 0;
+        check(ndg_mesh_info(handle_, &ne, &faces, &dofs, nullptr, nullptr));
+        n_elements_ = (std::size_t)ne;
+        n_faces_ = (std::size_t)faces;
         n_dofs_per_field_ = (std::size_t)dofs;
         orders_.resize(n_elements_);
         check(ndg_mesh_orders(handle_, orders_.data()));
         n_trace_values_ = 0;
         for (int32_t N : orders_) n_trace_values_ += 12 * (std::size_t)(N + 1);
     }
-    // Mesh2D_t::adapt without splitting (N_max: the tables' maximum order).
+    // Mesh2D_t::adapt (N_max: the tables' maximum order): splits the elements that would h-refine
+    // (when any would) and raises by 2 those that would p-refine; the element count may change.
     std::int64_t adapt(int N_max) {
         int64_t raised = 0;
         check(ndg_adapt(handle_, N_max, max_split_level_, &raised));
@@ -210,6 +207,18 @@ class Mesh2D_t {
             throw Error(NDG_ERR_INVALID, "Mesh2D_t::set_solution: wrong field length");
         check(ndg_set_state(handle_, p.data(), u.data(), v.data()));
     }
+    // Mesh2D_t::get_solution (src/meshes/Mesh2D_t.cpp:4237-4249): fills x_output_, y_output_,
+    // p_output_, u_output_, v_output_ with the values at the n x n uniform output points of every
+    // element (n = the tables' n_interpolation_points), element e's point (i, j) at
+    // e * n * n + i * n + j.
+    template <typename Polynomial>
+    void get_solution(const Entities::NDG_t<Polynomial> &NDG) {
+        const std::size_t size = n_elements_ * NDG.N_interpolation_points_ * NDG.N_interpolation_points_;
+        for (std::vector<double> *a : {&x_output_, &y_output_, &p_output_, &u_output_, &v_output_}) a->resize(size);
+        check(ndg_get_solution(handle_, x_output_.data(), y_output_.data(), p_output_.data(), u_output_.data(),
+                               v_output_.data()));
+    }
+    std::vector<double> x_output_, y_output_, p_output_, u_output_, v_output_;
     // Traces of the current state, [element][side][field][N_e+1] packed element after element.
     void interpolate_to_boundaries(std::vector<double> &traces) {
         traces.resize(n_trace_values_);

@@ -31,7 +31,7 @@ TABLE_NAMES = ["nodes", "weights", "barycentric_weights", "lagrange_interpolant_
 EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tables_destroy",
             "ndg_tables_get", "ndg_square_mesh_sizes", "ndg_square_mesh_fill", "ndg_mesh_create",
             "ndg_mesh_destroy", "ndg_mesh_set_stream", "ndg_mesh_info", "ndg_mesh_set_precision", "ndg_set_ic",
-            "ndg_set_state", "ndg_get_state", "ndg_get_traces", "ndg_compute_rhs",
+            "ndg_set_state", "ndg_get_state", "ndg_get_solution", "ndg_get_traces", "ndg_compute_rhs",
             "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_run_steps", "ndg_synchronize",
             "ndg_estimate_error", "ndg_mesh_set_graph", "ndg_mesh_autotune", "ndg_mesh_tuning",
             "ndg_mesh_set_wave_speed", "ndg_record_receivers", "ndg_adjoint_gradient", "ndg_apply_rhs_transpose",
@@ -96,6 +96,7 @@ def lib():
             "ndg_set_ic": [vp, vp],
             "ndg_set_state": [vp, vp, vp, vp],
             "ndg_get_state": [vp, vp, vp, vp],
+            "ndg_get_solution": [vp, vp, vp, vp, vp, vp],
             "ndg_get_traces": [vp, vp],
             "ndg_compute_rhs": [vp, f64, vp, vp, vp],
             "ndg_get_delta_t": [vp, f64, vp],

@@ -191,14 +191,26 @@ hipError_t launch_record(const DeviceMesh &m, const int64_t *elem, const double 
 hipError_t launch_source(double *lam, const DeviceMesh &m, const int64_t *elem, const double *w,
                          const double *res, int n, hipStream_t s);
 
-// Solution transfer between two layouts after an order change (one record per element, offsets
-// in values of q): copy, or interpolation to the new order (transfer_kernel).
+// Solution transfer between two layouts after adaptation (one record per new element, offsets in
+// values of q): copy, interpolation to a new order, or interpolation at given reference
+// coordinates of the old element (a child of a split element).
+// One element of the uniform-grid output (Mesh2D_t::get_solution), in the caller's element order.
+struct OutputRec {
+    int64_t q_off;   // the element's state block in DeviceMesh::q
+    int64_t dev;     // its device index (vertices)
+    int64_t i_off;   // its order's [n][N+1] interpolation matrix in the concatenated table
+    int32_t N, pad;
+};
+hipError_t launch_output(const double *q, const double *verts, const OutputRec *recs, const double *interp, int n,
+                         int64_t K, double *out, hipStream_t s);
+
 struct TransferRec {
     int64_t old_off, new_off;
     int32_t N_old, N_new;
+    int64_t coord_off;  // >= 0: the new nodes' reference coordinates in the old element, [(N+1)^2][2]
 };
-hipError_t launch_transfer(const double *q_old, double *q_new, const TransferRec *recs, int64_t K,
-                           const double *tab1d, hipStream_t s);
+hipError_t launch_transfer(const double *q_old, double *q_new, const TransferRec *recs, const double *coords,
+                           int64_t K, const double *tab1d, hipStream_t s);
 
 // Records msg as the calling thread's last error and returns code.
 ndg_status set_error(ndg_status code, const char *msg);

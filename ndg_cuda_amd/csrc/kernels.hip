@@ -1042,19 +1042,21 @@ mortar_kernel(const MortarFace *__restrict__ faces, const double *__restrict__ m
 // ---------------------------------------------------------------------------------------------
 // Launchers: dispatch on the (uniform) order N.
 // ---------------------------------------------------------------------------------------------
-// Solution transfer after an order change, one workgroup per element: a copy when the order is
-// unchanged, else Element2D_t::interpolate_from for the same geometry (src/entities/
-// Element2D_t.cpp:464-579): the old element's node when both coordinates match a node (almost_
-// equal, the last matching index wins), a 1D barycentric interpolation along the other coordinate
-// when one matches, else the tensor barycentric interpolation.
+// Solution transfer after adaptation, one workgroup per new element: a copy when neither order nor
+// geometry changes, else Element2D_t::interpolate_from (src/entities/Element2D_t.cpp:345-579) at
+// the new nodes (same geometry) or at the given reference coordinates in the old element (a child):
+// the old element's node when both coordinates match a node (almost_equal, the last matching
+// index wins), a 1D barycentric interpolation along the other coordinate when one matches, else
+// the tensor barycentric interpolation.
 __global__ void __launch_bounds__(256)
 transfer_kernel(const double *__restrict__ q_old, double *__restrict__ q_new,
-                const TransferRec *__restrict__ recs, const double *__restrict__ tab1d) {
+                const TransferRec *__restrict__ recs, const double *__restrict__ coords,
+                const double *__restrict__ tab1d) {
     const TransferRec R = recs[blockIdx.x];
     const int No = R.N_old, Nn = R.N_new, nfo = No + 1, nf = Nn + 1, neo = nfo * nfo, ne = nf * nf;
     const double *src = q_old + R.old_off;
     double *dst = q_new + R.new_off;
-    if (No == Nn) {
+    if (No == Nn && R.coord_off < 0) {
         for (int r = threadIdx.x; r < 3 * ne; r += 256) dst[r] = src[r];
         return;
     }
@@ -1062,7 +1064,8 @@ transfer_kernel(const double *__restrict__ q_old, double *__restrict__ q_new,
     const double *xn = tab1d + (3 * Nn + 0) * kTab1dStride;
     for (int r = threadIdx.x; r < 3 * ne; r += 256) {
         const int f = r / ne, node = r % ne, i = node / nf, j = node % nf;
-        const double X = xn[i], Y = xn[j];
+        const double X = R.coord_off >= 0 ? coords[R.coord_off + 2 * node] : xn[i];
+        const double Y = R.coord_off >= 0 ? coords[R.coord_off + 2 * node + 1] : xn[j];
         int col = -1, row = -1;
         for (int m = 0; m <= No; ++m) {
             if (ulp2_equal(X, xo[m])) col = m;
@@ -1104,10 +1107,56 @@ transfer_kernel(const double *__restrict__ q_old, double *__restrict__ q_new,
     }
 }
 
-hipError_t launch_transfer(const double *q_old, double *q_new, const TransferRec *recs, int64_t K,
-                           const double *tab1d, hipStream_t s) {
+hipError_t launch_transfer(const double *q_old, double *q_new, const TransferRec *recs, const double *coords,
+                           int64_t K, const double *tab1d, hipStream_t s) {
     if (K <= 0) return hipSuccess;
-    hipLaunchKernelGGL(transfer_kernel, dim3((unsigned)K), dim3(256), 0, s, q_old, q_new, recs, tab1d);
+    hipLaunchKernelGGL(transfer_kernel, dim3((unsigned)K), dim3(256), 0, s, q_old, q_new, recs, coords, tab1d);
+    return hipGetLastError();
+}
+
+// Mesh2D_t::get_solution (src/meshes/Mesh2D_t.cpp:4237-4249) with
+// Element2D_t::interpolate_solution (src/entities/Element2D_t.cpp:581-604): for each element (one
+// workgroup, caller's order) and output point (i, j) of the n x n uniform grid, x and y by the
+// bilinear map at ((i / (n - 1)) * 2 - 1, (j / (n - 1)) * 2 - 1), and each field as
+// sum_m sum_k q[m][k] I[i][m] I[j][k] with the order's uniform-grid interpolation matrix I.
+// Output layout [x, y, p, u, v][element][i * n + j].
+__global__ __launch_bounds__(64) void output_kernel(const double *__restrict__ q, const double *__restrict__ verts,
+                                                    const OutputRec *__restrict__ recs,
+                                                    const double *__restrict__ interp, int n, int64_t K,
+                                                    double *__restrict__ out) {
+    const int64_t e = blockIdx.x;
+    const OutputRec R = recs[e];
+    const int nf = R.N + 1, ne = nf * nf, nn = n * n;
+    const double *I = interp + R.i_off;
+    const double *P = verts + 8 * R.dev;
+    const double *src = q + R.q_off;
+    const int64_t plane = K * nn;
+    for (int r = threadIdx.x; r < nn; r += 64) {
+        const int i = r / n, j = r % n;
+        const double xi = (double)i / (double)(n - 1) * 2 - 1, eta = (double)j / (double)(n - 1) * 2 - 1;
+        const double w0 = (1 - xi) * (1 - eta), w1 = (xi + 1) * (1 - eta), w2 = (xi + 1) * (eta + 1),
+                     w3 = (1 - xi) * (eta + 1);
+        const double x = (P[0] * w0 + P[2] * w1 + P[4] * w2 + P[6] * w3) / 4;
+        const double y = (P[1] * w0 + P[3] * w1 + P[5] * w2 + P[7] * w3) / 4;
+        double f[3] = {0.0, 0.0, 0.0};
+        for (int m = 0; m < nf; ++m) {
+            const double a = I[i * nf + m];
+            for (int k = 0; k < nf; ++k) {
+                const double b = I[j * nf + k];
+                for (int c = 0; c < 3; ++c) f[c] += src[c * ne + m * nf + k] * a * b;
+            }
+        }
+        const int64_t at = e * nn + r;
+        out[at] = x;
+        out[plane + at] = y;
+        for (int c = 0; c < 3; ++c) out[(2 + c) * plane + at] = f[c];
+    }
+}
+
+hipError_t launch_output(const double *q, const double *verts, const OutputRec *recs, const double *interp, int n,
+                         int64_t K, double *out, hipStream_t s) {
+    if (K <= 0) return hipSuccess;
+    hipLaunchKernelGGL(output_kernel, dim3((unsigned)K), dim3(64), 0, s, q, verts, recs, interp, n, K, out);
     return hipGetLastError();
 }
 

@@ -50,6 +50,26 @@ struct ndg_tables {
     Tables t;
 };
 
+// Host description of the mesh (kept for rebuilds after adaptation).
+struct HostMesh {
+    std::vector<double> xy;        // [n_nodes][2]
+    std::vector<int64_t> quads;    // [K][4]
+    std::vector<int64_t> bars;     // [B][2]
+    std::vector<int32_t> bar_type; // [B]
+    std::vector<int64_t> bar_donor;  // [B] (empty: no periodic bars)
+};
+
+struct Topology {
+    std::vector<int64_t> link;
+    std::vector<char> periodic_link;
+    std::vector<char> hang_kind;
+    std::vector<int64_t> hang_second;
+    std::vector<BcSide> bcs;       // boundary (ghost) records
+    std::vector<int64_t> bc_bar;   // bar of each record
+    std::vector<int64_t> bar_side; // element side of each bar
+    int64_t n_faces = 0;
+};
+
 struct ndg_mesh {
     DeviceMesh d;
     const ndg_tables *tables = nullptr;
@@ -59,6 +79,7 @@ struct ndg_mesh {
     double tol_min = 1e-6, tol_max = 1e-14;
     double *dt_dev = nullptr;
     double *est_dev = nullptr;
+    int64_t est_len = 0;
     double *pulses_dev = nullptr;
     std::vector<double> staging;
     // Element e of the caller's order is element dev_of[e] of the device order (grouped by N).
@@ -84,7 +105,11 @@ struct ndg_mesh {
     std::vector<char> hang_kind;
     std::vector<int64_t> hang_second;
     std::vector<BcSide> bc_base;             // boundary (ghost) records
-    std::vector<char> ghost_zero;            // per record: ghost storage resized to zero by an adaptation
+    std::vector<int64_t> bc_bar;             // bar (ghost) of each record
+    std::vector<int64_t> bar_side;           // element side of each bar
+    std::vector<char> bar_zero;              // per bar: ghost storage resized to zero by an adaptation
+    HostMesh host;                           // the description the topology was built from
+    std::vector<int32_t> status, rotation, split_level;  // Hilbert status per element
     int initial_N = 0, maximum_N = 0, max_split_level = 0;
     int64_t adapt_interval = 0;
 };
@@ -398,7 +423,7 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
     // Boundary records; a ghost whose storage an adaptation resized has a zero scale.
     std::vector<BcSide> bcs = m->bc_base;
     for (size_t b = 0; b < bcs.size(); ++b)
-        if (m->ghost_zero[b]) bcs[b].sf = 0.0;
+        if (m->bar_zero[(size_t)m->bc_bar[b]]) bcs[b].sf = 0.0;
     for (const BcSide &b : bcs) d.has_inflow = d.has_inflow || b.kind == NDG_BC_INFLOW;
 
     // Device connectivity. Sides whose exterior state has another order are non-conforming: they
@@ -520,50 +545,97 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
 
 }  // namespace
 
-extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_tables *tables, ndg_mesh **out) {
-    if (!desc || !tables || !out) return fail(NDG_ERR_INVALID, "ndg_mesh_create: null argument");
-    *out = nullptr;
-    const int64_t K = desc->n_quads, B = desc->n_bars, NN = desc->n_nodes;
-    if (K <= 0 || NN <= 0 || !desc->xy || !desc->quads) return fail(NDG_ERR_INVALID, "ndg_mesh_create: empty mesh");
-    if (B > 0 && (!desc->bars || !desc->bar_type)) return fail(NDG_ERR_INVALID, "ndg_mesh_create: bars without types");
-    if (NN >= (int64_t(1) << 32)) return fail(NDG_ERR_INVALID, "ndg_mesh_create: too many nodes");
-    if (K > (int64_t(1) << 29)) return fail(NDG_ERR_INVALID, "ndg_mesh_create: too many elements");
-    std::vector<int32_t> order((size_t)K, desc->initial_N);
-    if (desc->quad_N) std::copy(desc->quad_N, desc->quad_N + K, order.begin());
-    for (int64_t e = 0; e < K; ++e) {
-        const int N = order[e];
-        if (!order_supported(N))
-            return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: element order outside the compiled range 1..16");
-        if (N > tables->t.N_max) return fail(NDG_ERR_INVALID, "ndg_mesh_create: element order exceeds the tables' N_max");
-        if (N < desc->initial_N)
-            return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: element order below initial_N (boundary ghosts would "
-                                             "carry a higher order than their element)");
-    }
-    for (int64_t e = 0; e < 4 * K; ++e)
-        if (desc->quads[e] < 0 || desc->quads[e] >= NN) return fail(NDG_ERR_INVALID, "ndg_mesh_create: quad node id out of range");
-    for (int64_t b = 0; b < 2 * B; ++b)
-        if (desc->bars[b] < 0 || desc->bars[b] >= NN) return fail(NDG_ERR_INVALID, "ndg_mesh_create: bar node id out of range");
+namespace {
 
-    auto P = [&](int64_t node) { return v2{desc->xy[2 * node], desc->xy[2 * node + 1]}; };
-    auto corner = [&](int64_t e, int k) { return P(desc->quads[4 * e + (k & 3)]); };
+// Hilbert-curve status of the elements (src/functions/Hilbert_splitting.cpp, statuses H, A, R, B =
+// 0..3): tables of the first / last / inner element status by entry and exit side, and of the
+// children's order and statuses by parent rotation and status.
+constexpr int kFirstStatus[4] = {3, 0, 1, 2};
+constexpr int kLastStatus[4] = {1, 2, 3, 0};
+constexpr int kInnerStatus[4][4] = {{0, 0, 1, 1}, {3, 3, 2, 2}, {3, 3, 2, 2}, {0, 0, 1, 1}};
+constexpr int kChildOrder[4][4][4] = {
+    {{0, 3, 2, 1}, {0, 1, 2, 3}, {2, 1, 0, 3}, {2, 3, 0, 1}},
+    {{3, 2, 1, 0}, {1, 2, 3, 0}, {1, 0, 3, 2}, {3, 0, 1, 2}},
+    {{2, 1, 0, 3}, {2, 3, 0, 1}, {0, 3, 2, 1}, {0, 1, 2, 3}},
+    {{1, 0, 3, 2}, {3, 0, 1, 2}, {3, 2, 1, 0}, {1, 2, 3, 0}}};
+constexpr int kChildStatus[4][4][4] = {
+    {{1, 3, 0, 0}, {0, 1, 1, 2}, {2, 2, 3, 1}, {3, 0, 2, 3}},
+    {{3, 0, 0, 1}, {1, 1, 2, 0}, {2, 3, 1, 2}, {0, 2, 3, 3}},
+    {{0, 0, 1, 3}, {1, 2, 0, 1}, {3, 1, 2, 2}, {2, 3, 3, 0}},
+    {{0, 1, 3, 0}, {2, 0, 1, 1}, {1, 2, 2, 3}, {3, 3, 0, 2}}};
+
+// The side of an element through which the centre offset (dx, dy) points (the element's bottom,
+// right, top, left are sides 0..3).
+int side_towards(double dx, double dy) {
+    if (std::fabs(dy) > std::fabs(dx)) return dy < 0 ? 0 : 2;
+    return dx < 0 ? 3 : 1;
+}
+
+// compute_element_status (src/meshes/Mesh2D_t.cpp:4055-4178): rotation from the corner angles
+// around the centre, status from the sides through which the curve enters (previous element) and
+// leaves (next element).
+void element_statuses(const HostMesh &h, std::vector<int32_t> &status, std::vector<int32_t> &rotation) {
+    const double pi = 3.14159265358979323846;
+    const double targets[4] = {-3 * pi / 4, -pi / 4, pi / 4, 3 * pi / 4};
+    const int64_t K = (int64_t)h.quads.size() / 4;
+    std::vector<double> cx((size_t)K), cy((size_t)K);
+    for (int64_t e = 0; e < K; ++e) {
+        double x = 0.0, y = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            x += h.xy[2 * h.quads[4 * e + k]];
+            y += h.xy[2 * h.quads[4 * e + k] + 1];
+        }
+        cx[e] = x / 4;
+        cy[e] = y / 4;
+    }
+    status.assign((size_t)K, 0);
+    rotation.assign((size_t)K, 0);
+    for (int64_t e = 0; e < K; ++e) {
+        long rot = 0;
+        for (int k = 0; k < 4; ++k) {
+            const int64_t node = h.quads[4 * e + k];
+            const double angle = std::atan2(h.xy[2 * node + 1] - cy[e], h.xy[2 * node] - cx[e]);
+            const double off = std::atan2(std::sin(angle - targets[k]), std::cos(angle - targets[k]));
+            const long turns = std::lround(off * 2 / pi + 4 * (off < 0));
+            rot += turns * (turns < 4);
+        }
+        rotation[e] = (int32_t)(rot / 4);
+        if (K == 1) status[e] = 0;
+        else if (e == 0) status[e] = kFirstStatus[side_towards(cx[1] - cx[0], cy[1] - cy[0])];
+        else if (e == K - 1) status[e] = kLastStatus[side_towards(cx[e - 1] - cx[e], cy[e - 1] - cy[e])];
+        else
+            status[e] = kInnerStatus[side_towards(cx[e - 1] - cx[e], cy[e - 1] - cy[e])]
+                                    [side_towards(cx[e + 1] - cx[e], cy[e + 1] - cy[e])];
+    }
+}
+
+// Faces of the description: conforming neighbours (build_faces, src/meshes/Mesh2D_t.cpp:1110-1154),
+// boundary ghosts, periodic couplings and 2:1 hanging sides, as element-side links.
+ndg_status build_topology(const HostMesh &h, Topology &t) {
+    const int64_t K = (int64_t)h.quads.size() / 4, B = (int64_t)h.bar_type.size();
+    auto P = [&](int64_t node) { return v2{h.xy[2 * node], h.xy[2 * node + 1]}; };
+    auto corner = [&](int64_t e, int k) { return P(h.quads[4 * e + (k & 3)]); };
 
     // Directed edges of the domain elements (build_faces, src/meshes/Mesh2D_t.cpp:1110-1154).
     std::unordered_map<uint64_t, int64_t> edges;
     edges.reserve((size_t)(4 * K) * 2);
     for (int64_t e = 0; e < K; ++e)
         for (int s = 0; s < 4; ++s) {
-            const int64_t a = desc->quads[4 * e + s], b = desc->quads[4 * e + (s + 1) % 4];
+            const int64_t a = h.quads[4 * e + s], b = h.quads[4 * e + (s + 1) % 4];
             if (!edges.emplace(edge_key(a, b), e * 4 + s).second)
-                return fail(NDG_ERR_INVALID, "ndg_mesh_create: an edge is used twice with the same orientation");
+                return fail(NDG_ERR_INVALID, "mesh topology: an edge is used twice with the same orientation");
         }
     // link[e*4+s]: the element side whose trace is the exterior state of side s of element e (a
     // neighbour, or the donor side of a periodic pair), or -(1 + boundary record) / INT64_MIN.
-    std::vector<int64_t> link((size_t)(4 * K), INT64_MIN);
-    std::vector<char> periodic_link((size_t)(4 * K), 0);
-    int64_t n_faces = 0;
+    std::vector<int64_t> &link = t.link;
+    std::vector<char> &periodic_link = t.periodic_link;
+    link.assign((size_t)(4 * K), INT64_MIN);
+    periodic_link.assign((size_t)(4 * K), 0);
+    int64_t &n_faces = t.n_faces;
+    n_faces = 0;
     for (int64_t e = 0; e < K; ++e)
         for (int s = 0; s < 4; ++s) {
-            const int64_t a = desc->quads[4 * e + s], b = desc->quads[4 * e + (s + 1) % 4];
+            const int64_t a = h.quads[4 * e + s], b = h.quads[4 * e + (s + 1) % 4];
             auto it = edges.find(edge_key(b, a));
             if (it != edges.end()) {
                 link[4 * e + s] = it->second;
@@ -572,32 +644,36 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
         }
     // Ghost (bar) elements attach to the domain side with the same two nodes
     // (src/meshes/Mesh2D_t.cpp:1156-1181). Their nodes after attachment are {P(s+1), P(s)}.
-    std::vector<int64_t> bar_side((size_t)B, -1);
+    std::vector<int64_t> &bar_side = t.bar_side;
+    bar_side.assign((size_t)B, -1);
     for (int64_t b = 0; b < B; ++b) {
-        const int64_t a = desc->bars[2 * b], c = desc->bars[2 * b + 1];
+        const int64_t a = h.bars[2 * b], c = h.bars[2 * b + 1];
         auto it = edges.find(edge_key(c, a));
         if (it == edges.end()) it = edges.find(edge_key(a, c));
-        if (it == edges.end()) return fail(NDG_ERR_INVALID, "ndg_mesh_create: boundary bar matches no element edge");
+        if (it == edges.end()) return fail(NDG_ERR_INVALID, "mesh topology: boundary bar matches no element edge");
         bar_side[b] = it->second;
         ++n_faces;
     }
-    std::vector<BcSide> bcs;
+    std::vector<BcSide> &bcs = t.bcs;
+    std::vector<int64_t> &bc_bar = t.bc_bar;
+    bcs.clear();
+    bc_bar.clear();
     std::vector<int64_t> side_bar((size_t)(4 * K), -1);
     for (int64_t b = 0; b < B; ++b) {
         const int64_t es = bar_side[b];
         if (link[es] != INT64_MIN || side_bar[es] != -1)
-            return fail(NDG_ERR_INVALID, "ndg_mesh_create: a boundary bar lies on an interior or already bounded edge");
+            return fail(NDG_ERR_INVALID, "mesh topology: a boundary bar lies on an interior or already bounded edge");
         side_bar[es] = b;
     }
     for (int64_t b = 0; b < B; ++b) {
         const int64_t es = bar_side[b], e = es / 4;
         const int s = (int)(es % 4);
-        const int64_t u0 = desc->quads[4 * e + s], u1 = desc->quads[4 * e + (s + 1) % 4];
-        const int kind = desc->bar_type[b];
+        const int64_t u0 = h.quads[4 * e + s], u1 = h.quads[4 * e + (s + 1) % 4];
+        const int kind = h.bar_type[b];
         if (kind == NDG_BC_PERIODIC) {
-            if (!desc->bar_donor) return fail(NDG_ERR_INVALID, "ndg_mesh_create: periodic bar without donor array");
-            const int64_t d = desc->bar_donor[b];
-            if (d < 0 || d >= B) return fail(NDG_ERR_INVALID, "ndg_mesh_create: periodic donor out of range");
+            if (h.bar_donor.empty()) return fail(NDG_ERR_INVALID, "mesh topology: periodic bar without donor array");
+            const int64_t d = h.bar_donor[b];
+            if (d < 0 || d >= B) return fail(NDG_ERR_INVALID, "mesh topology: periodic donor out of range");
             // local_interfaces (Mesh2D_t.cpp:4712-4730): the ghost carries the donor side's trace
             // unchanged (and takes the donor element's order), so the side couples to the donor
             // element side directly.
@@ -605,14 +681,14 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
             periodic_link[es] = 1;
             continue;
         }
-        if (kind < NDG_BC_NONE || kind > NDG_BC_OUTFLOW) return fail(NDG_ERR_INVALID, "ndg_mesh_create: unknown boundary kind");
+        if (kind < NDG_BC_NONE || kind > NDG_BC_OUTFLOW) return fail(NDG_ERR_INVALID, "mesh topology: unknown boundary kind");
         BcSide rec{};
         rec.kind = kind;
         rec.slot = -1;
         // ghost nodes {a, b, b, a} with a = P(s+1), b = P(s) after face attachment
         const v2 ga = P(u1), gb = P(u0);
         rec.ax = ga.x; rec.ay = ga.y; rec.bx = gb.x; rec.by = gb.y;
-        const int64_t bar_a = desc->bars[2 * b];
+        const int64_t bar_a = h.bars[2 * b];
         v2 G[4];
         if (bar_a == u1) {  // bar already reversed with respect to the element edge
             G[0] = ga; G[1] = gb; G[2] = gb; G[3] = ga;
@@ -623,20 +699,23 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
         face_frame(P(u0), P(u1), P(u0), P(u1), rec.nx, rec.ny, rec.tx, rec.ty);
         link[es] = -(int64_t)(1 + bcs.size());
         bcs.push_back(rec);
+        bc_bar.push_back(b);
     }
     // Hanging (2:1) sides: a side a -> b with neither a neighbour nor a bar whose two halves are
     // the unmatched sides b -> m and m -> a of two other elements, m its midpoint (what
     // h-refinement leaves, src/meshes/Mesh2D_t.cpp:5661-5985).
-    std::vector<char> hang_kind((size_t)(4 * K), 0);
-    std::vector<int64_t> hang_second((size_t)(4 * K), -1);
+    std::vector<char> &hang_kind = t.hang_kind;
+    std::vector<int64_t> &hang_second = t.hang_second;
+    hang_kind.assign((size_t)(4 * K), 0);
+    hang_second.assign((size_t)(4 * K), -1);
     {
         std::unordered_map<int64_t, std::vector<int64_t>> out_edges;  // node -> element sides starting there
         for (int64_t es = 0; es < 4 * K; ++es)
-            if (link[es] == INT64_MIN) out_edges[desc->quads[es]].push_back(es);
-        auto end_node = [&](int64_t es) { return desc->quads[4 * (es / 4) + (es % 4 + 1) % 4]; };
+            if (link[es] == INT64_MIN) out_edges[h.quads[es]].push_back(es);
+        auto end_node = [&](int64_t es) { return h.quads[4 * (es / 4) + (es % 4 + 1) % 4]; };
         for (int64_t es = 0; es < 4 * K; ++es) {
             if (link[es] != INT64_MIN || hang_kind[es]) continue;
-            const int64_t a = desc->quads[es], b = end_node(es);
+            const int64_t a = h.quads[es], b = end_node(es);
             auto it = out_edges.find(b);
             if (it == out_edges.end()) continue;
             for (int64_t s1 : it->second) {  // b -> m
@@ -664,33 +743,78 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
     }
     for (int64_t es = 0; es < 4 * K; ++es)
         if (link[es] == INT64_MIN)
-            return fail(NDG_ERR_INVALID, "ndg_mesh_create: element side has neither a neighbour nor a boundary bar");
+            return fail(NDG_ERR_INVALID, "mesh topology: element side has neither a neighbour nor a boundary bar");
+    return NDG_OK;
+}
 
-    int32_t ndev = 0;
-    ndg_device_count(&ndev);
-    if (ndev <= 0) return fail(NDG_ERR_NO_DEVICE, "ndg_mesh_create: no HIP device visible");
+}  // namespace
 
+extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_tables *tables, ndg_mesh **out) {
+    if (!desc || !tables || !out) return fail(NDG_ERR_INVALID, "ndg_mesh_create: null argument");
+    *out = nullptr;
+    const int64_t K = desc->n_quads, B = desc->n_bars, NN = desc->n_nodes;
+    if (K <= 0 || NN <= 0 || !desc->xy || !desc->quads) return fail(NDG_ERR_INVALID, "ndg_mesh_create: empty mesh");
+    if (B > 0 && (!desc->bars || !desc->bar_type)) return fail(NDG_ERR_INVALID, "ndg_mesh_create: bars without types");
+    if (NN >= (int64_t(1) << 32)) return fail(NDG_ERR_INVALID, "ndg_mesh_create: too many nodes");
+    if (K > (int64_t(1) << 29)) return fail(NDG_ERR_INVALID, "ndg_mesh_create: too many elements");
+    std::vector<int32_t> order((size_t)K, desc->initial_N);
+    if (desc->quad_N) std::copy(desc->quad_N, desc->quad_N + K, order.begin());
+    for (int64_t e = 0; e < K; ++e) {
+        const int N = order[e];
+        if (!order_supported(N))
+            return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: element order outside the compiled range 1..16");
+        if (N > tables->t.N_max) return fail(NDG_ERR_INVALID, "ndg_mesh_create: element order exceeds the tables' N_max");
+        if (N < desc->initial_N)
+            return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: element order below initial_N (boundary ghosts would "
+                                             "carry a higher order than their element)");
+    }
+    for (int64_t e = 0; e < 4 * K; ++e)
+        if (desc->quads[e] < 0 || desc->quads[e] >= NN) return fail(NDG_ERR_INVALID, "ndg_mesh_create: quad node id out of range");
+    for (int64_t b = 0; b < 2 * B; ++b)
+        if (desc->bars[b] < 0 || desc->bars[b] >= NN) return fail(NDG_ERR_INVALID, "ndg_mesh_create: bar node id out of range");
+
+    HostMesh host;
+    host.xy.assign(desc->xy, desc->xy + 2 * NN);
+    host.quads.assign(desc->quads, desc->quads + 4 * K);
+    if (B > 0) {
+        host.bars.assign(desc->bars, desc->bars + 2 * B);
+        host.bar_type.assign(desc->bar_type, desc->bar_type + B);
+        if (desc->bar_donor) host.bar_donor.assign(desc->bar_donor, desc->bar_donor + B);
+    }
+    Topology topo;
+    ndg_status tst = build_topology(host, topo);
+    if (tst != NDG_OK) return tst;
     std::unique_ptr<ndg_mesh, void (*)(ndg_mesh *)> m(new ndg_mesh, free_mesh);
     m->tables = tables;
     m->tol_min = desc->tolerance_min;
     m->tol_max = desc->tolerance_max;
-    m->n_faces = n_faces;
+    m->n_faces = topo.n_faces;
     m->initial_N = desc->initial_N;
     m->maximum_N = desc->maximum_N;
     m->max_split_level = desc->max_split_level;
-    m->link = std::move(link);
-    m->periodic_link = std::move(periodic_link);
-    m->hang_kind = std::move(hang_kind);
-    m->hang_second = std::move(hang_second);
-    m->bc_base = std::move(bcs);
-    m->ghost_zero.assign(m->bc_base.size(), 0);
+    m->link = std::move(topo.link);
+    m->periodic_link = std::move(topo.periodic_link);
+    m->hang_kind = std::move(topo.hang_kind);
+    m->hang_second = std::move(topo.hang_second);
+    m->bc_base = std::move(topo.bcs);
+    m->bc_bar = std::move(topo.bc_bar);
+    m->bar_side = std::move(topo.bar_side);
+    m->bar_zero.assign(host.bar_type.size(), 0);
+    element_statuses(host, m->status, m->rotation);
+    m->split_level.assign((size_t)K, 0);
     m->host_verts.resize((size_t)(8 * K));
     for (int64_t e = 0; e < K; ++e)
         for (int k = 0; k < 4; ++k) {
-            const v2 p = corner(e, k);
-            m->host_verts[8 * e + 2 * k] = p.x;
-            m->host_verts[8 * e + 2 * k + 1] = p.y;
+            const int64_t node = host.quads[4 * e + k];
+            m->host_verts[8 * e + 2 * k] = host.xy[2 * node];
+            m->host_verts[8 * e + 2 * k + 1] = host.xy[2 * node + 1];
         }
+    m->host = std::move(host);
+
+    // Everything above is host work; the device is needed from here on.
+    int32_t ndev = 0;
+    ndg_device_count(&ndev);
+    if (ndev <= 0) return fail(NDG_ERR_NO_DEVICE, "ndg_mesh_create: no HIP device visible");
     HIP_TRY(hipStreamCreateWithFlags(&m->own_stream, hipStreamNonBlocking), "hipStreamCreate");
     m->stream = m->own_stream;
     HIP_TRY(hipMalloc((void **)&m->dt_dev, sizeof(double)), "hipMalloc dt");
@@ -862,6 +986,59 @@ extern "C" ndg_status ndg_get_state(ndg_mesh *mesh, double *p, double *u, double
     if (st != NDG_OK) return st;
     double *const outs[3] = {p, u, v};
     return copy_state(mesh, mesh->d.q, outs, true, "download state");
+}
+
+extern "C" ndg_status ndg_get_solution(ndg_mesh *mesh, double *x, double *y, double *p, double *u, double *v) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_get_solution: null mesh");
+    const Tables &tab = mesh->tables->t;
+    const int64_t n = tab.n_interp;
+    if (n < 2) return fail(NDG_ERR_INVALID, "ndg_get_solution: the tables have fewer than 2 output points");
+    if (n > 4096) return fail(NDG_ERR_INVALID, "ndg_get_solution: too many output points");
+    ndg_status st = refresh_fp64(mesh);
+    if (st != NDG_OK) return st;
+    const DeviceMesh &d = mesh->d;
+    const int64_t K = d.K, nn = n * n;
+    std::vector<double> interp;
+    std::vector<int64_t> i_off(kMaxOrder + 1, -1);
+    for (const Group &G : d.groups) {
+        i_off[G.N] = (int64_t)interp.size();
+        const auto &I = tab.tab[NDG_TABLE_INTERPOLATION][G.N];
+        interp.insert(interp.end(), I.begin(), I.end());
+    }
+    std::vector<OutputRec> recs((size_t)K);
+    for (int64_t e = 0; e < K; ++e) {
+        const int N = mesh->order[e];
+        const Group *G = nullptr;
+        for (const Group &H : d.groups)
+            if (H.N == N) G = &H;
+        OutputRec &R = recs[(size_t)e];
+        R.q_off = G->q_off + (mesh->dev_of[e] - G->e_begin) * 3 * (int64_t)(N + 1) * (N + 1);
+        R.dev = mesh->dev_of[e];
+        R.i_off = i_off[N];
+        R.N = N;
+        R.pad = 0;
+    }
+    OutputRec *recs_dev = nullptr;
+    double *interp_dev = nullptr, *out_dev = nullptr;
+    st = device_upload(&recs_dev, recs, "upload output records");
+    if (st == NDG_OK) st = device_upload(&interp_dev, interp, "upload interpolation matrices");
+    if (st == NDG_OK) {
+        const hipError_t e = hipMalloc((void **)&out_dev, sizeof(double) * 5 * K * nn);
+        if (e != hipSuccess) st = hip_fail(e, "hipMalloc output");
+    }
+    if (st == NDG_OK) {
+        hipError_t e = launch_output(d.q, d.verts, recs_dev, interp_dev, (int)n, K, out_dev, mesh->stream);
+        double *const outs[5] = {x, y, p, u, v};
+        for (int f = 0; f < 5 && e == hipSuccess; ++f)
+            if (outs[f])
+                e = hipMemcpyAsync(outs[f], out_dev + f * K * nn, sizeof(double) * K * nn, hipMemcpyDeviceToHost,
+                                   mesh->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(mesh->stream);
+        if (e != hipSuccess) st = hip_fail(e, "output kernel");
+    }
+    for (void *ptr : {(void *)recs_dev, (void *)interp_dev, (void *)out_dev})
+        if (ptr) (void)hipFree(ptr);
+    return st;
 }
 
 extern "C" ndg_status ndg_get_traces(ndg_mesh *mesh, double *out) {
@@ -1097,6 +1274,269 @@ extern "C" ndg_status ndg_mesh_tuning(const ndg_mesh *mesh, int32_t *orders, int
     return NDG_OK;
 }
 
+namespace {
+
+// quad_map (include/functions/quad_map.tpp:3-8)
+v2 quad_point(double xi, double eta, const v2 P[4]) {
+    return v2{(P[0].x * (1 - xi) * (1 - eta) + P[1].x * (xi + 1) * (1 - eta) + P[2].x * (xi + 1) * (eta + 1) +
+               P[3].x * (1 - xi) * (eta + 1)) / 4,
+              (P[0].y * (1 - xi) * (1 - eta) + P[1].y * (xi + 1) * (1 - eta) + P[2].y * (xi + 1) * (eta + 1) +
+               P[3].y * (1 - xi) * (eta + 1)) / 4};
+}
+
+double cross(v2 a, v2 b) { return a.x * b.y - a.y * b.x; }
+
+// inverse_quad_map (include/functions/inverse_quad_map.tpp:46-88): the bilinear map's inverse by
+// its quadratic, the root inside the unit square.
+v2 reference_point(v2 g, const v2 P[4]) {
+    const v2 e{P[1].x - P[0].x, P[1].y - P[0].y}, f{P[3].x - P[0].x, P[3].y - P[0].y};
+    const v2 gg{P[0].x - P[1].x + P[2].x - P[3].x, P[0].y - P[1].y + P[2].y - P[3].y};
+    const v2 h{g.x - P[0].x, g.y - P[0].y};
+    const double k2 = cross(gg, f), k1 = cross(e, f) + cross(h, gg), k0 = cross(h, e);
+    if (std::fabs(k2) < DBL_MIN) {
+        const double ux = e.x * k1 - gg.x * k0, uy = e.y * k1 - gg.y * k0;
+        const double u = std::fabs(ux) > std::fabs(uy) ? (h.x * k1 + f.x * k0) / ux : (h.y * k1 + f.y * k0) / uy;
+        return v2{u * 2 - 1, -k0 / k1 * 2 - 1};
+    }
+    double w = k1 * k1 - 4 * k0 * k2;
+    if (w < 0.0) return v2{-1, -1};
+    w = std::sqrt(w);
+    const double ik2 = 1 / (2 * k2);
+    double v = (-k1 - w) * ik2;
+    double ux = e.x + gg.x * v, uy = e.y + gg.y * v;
+    double u = std::fabs(ux) > std::fabs(uy) ? (h.x - f.x * v) / ux : (h.y - f.y * v) / uy;
+    if (u < 0.0 || u > 1.0 || v < 0.0 || v > 1.0) {
+        v = (-k1 + w) * ik2;
+        ux = e.x + gg.x * v;
+        uy = e.y + gg.y * v;
+        u = std::fabs(ux) > std::fabs(uy) ? (h.x - f.x * v) / ux : (h.y - f.y * v) / uy;
+    }
+    return v2{u * 2 - 1, v * 2 - 1};
+}
+
+// The splitting branch of Mesh2D_t::adapt (src/meshes/Mesh2D_t.cpp:1993-2268; hp_adapt
+// :5186-5659; split_boundaries :6232-6343): every element with would_h_refine becomes four children
+// (corner s: {corner s, midpoint of side s, centre, midpoint of side s - 1} at local positions s,
+// s + 1, s + 2, s + 3) placed in the Hilbert child order, with the parent's order, split level + 1,
+// the child status and the parent's rotation; would_p_refine elements are raised by 2 orders; a
+// boundary ghost on a split side becomes two ghosts (fresh geometry), a ghost whose element is
+// raised gets fresh geometry. The faces are rebuilt from the new elements and ghosts. Elements
+// with a periodic side, or whose split would cover a side of an unsplit neighbour with more than
+// two faces, are refused (NDG_ERR_UNSUPPORTED, nothing changed).
+ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const std::vector<double> &sig,
+                       int32_t N_max, int32_t max_split_level, int64_t *n_changed) {
+    const HostMesh &h = mesh->host;
+    const int64_t K = (int64_t)h.quads.size() / 4, B = (int64_t)h.bar_type.size();
+    std::vector<char> split((size_t)K, 0), raise((size_t)K, 0);
+    int64_t n_split = 0, n_raise = 0;
+    for (int64_t e = 0; e < K; ++e) {
+        if (ref[e] && sig[e] < 1.0 && mesh->split_level[e] < max_split_level) split[e] = 1, ++n_split;
+        else if (ref[e] && sig[e] >= 1.0 && mesh->order[e] + 2 <= N_max) raise[e] = 1, ++n_raise;
+    }
+    for (int64_t e = 0; e < K; ++e) {
+        if (raise[e] && (mesh->order[e] + 2 > kMaxOrder || mesh->order[e] + 2 > mesh->tables->t.N_max))
+            return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a raised order exceeds the compiled range or the tables");
+        if (!split[e]) continue;
+        for (int s = 0; s < 4; ++s) {
+            const int64_t es = 4 * e + s;
+            if (mesh->periodic_link[es])
+                return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: splitting an element with a periodic side is not provided");
+            if (mesh->hang_kind[es] == 1 && !split[mesh->link[es] / 4])
+                return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a split would leave more than two faces on a side");
+        }
+    }
+    auto node_xy = [&](const std::vector<double> &xy, int64_t n) { return v2{xy[2 * n], xy[2 * n + 1]}; };
+    auto end_node = [&](int64_t es) { return h.quads[4 * (es / 4) + (es % 4 + 1) % 4]; };
+    HostMesh nh;
+    nh.xy = h.xy;
+    std::unordered_map<uint64_t, int64_t> mids;
+    auto mid_key = [](int64_t a, int64_t b) { return edge_key(std::min(a, b), std::max(a, b)); };
+    for (int64_t es = 0; es < 4 * K; ++es)
+        if (mesh->hang_kind[es] == 2) mids[mid_key(h.quads[es], end_node(es))] = h.quads[mesh->link[es]];
+    auto midpoint = [&](int64_t a, int64_t b) {
+        auto it = mids.find(mid_key(a, b));
+        if (it != mids.end()) return it->second;
+        const int64_t md = (int64_t)nh.xy.size() / 2;
+        nh.xy.push_back((nh.xy[2 * a] + nh.xy[2 * b]) / 2);
+        nh.xy.push_back((nh.xy[2 * a + 1] + nh.xy[2 * b + 1]) / 2);
+        mids[mid_key(a, b)] = md;
+        return md;
+    };
+    const int64_t Knew = K + 3 * n_split;
+    nh.quads.assign((size_t)(4 * Knew), -1);
+    std::vector<int32_t> order((size_t)Knew), status((size_t)Knew), rotation((size_t)Knew), level((size_t)Knew);
+    std::vector<int64_t> parent((size_t)Knew);
+    std::vector<char> child((size_t)Knew, 0);
+    int64_t pos = 0;
+    for (int64_t e = 0; e < K; ++e) {
+        const int64_t *q = &h.quads[4 * e];
+        if (!split[e]) {
+            for (int k = 0; k < 4; ++k) nh.quads[4 * pos + k] = q[k];
+            order[pos] = mesh->order[e] + (raise[e] ? 2 : 0);
+            status[pos] = mesh->status[e];
+            rotation[pos] = mesh->rotation[e];
+            level[pos] = mesh->split_level[e];
+            parent[pos] = e;
+            ++pos;
+            continue;
+        }
+        int64_t md[4];
+        for (int s = 0; s < 4; ++s) md[s] = midpoint(q[s], q[(s + 1) % 4]);
+        double cx = 0.0, cy = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            cx += nh.xy[2 * q[k]];
+            cy += nh.xy[2 * q[k] + 1];
+        }
+        const int64_t c = (int64_t)nh.xy.size() / 2;
+        nh.xy.push_back(cx / 4);
+        nh.xy.push_back(cy / 4);
+        const int *co = kChildOrder[mesh->rotation[e]][mesh->status[e]];
+        const int *cs = kChildStatus[mesh->rotation[e]][mesh->status[e]];
+        for (int s = 0; s < 4; ++s) {
+            const int64_t at = pos + co[s];
+            nh.quads[4 * at + s] = q[s];
+            nh.quads[4 * at + (s + 1) % 4] = md[s];
+            nh.quads[4 * at + (s + 2) % 4] = c;
+            nh.quads[4 * at + (s + 3) % 4] = md[(s + 3) % 4];
+            order[at] = mesh->order[e];
+            status[at] = cs[s];
+            rotation[at] = mesh->rotation[e];
+            level[at] = mesh->split_level[e] + 1;
+            parent[at] = e;
+            child[at] = 1;
+        }
+        pos += 4;
+    }
+    std::vector<char> bar_zero;
+    std::vector<int64_t> bar_new((size_t)B, -1);
+    for (int64_t b = 0; b < B; ++b) {
+        const int64_t es = mesh->bar_side[b], e = es / 4;
+        bar_new[b] = (int64_t)nh.bar_type.size();
+        if (split[e]) {  // two ghosts: (P(s+1), m) and (m, P(s))
+            const int64_t u0 = h.quads[es], u1 = end_node(es), md = midpoint(u0, u1);
+            const int64_t halves[2][2] = {{u1, md}, {md, u0}};
+            for (const auto &pr : halves) {
+                nh.bars.push_back(pr[0]);
+                nh.bars.push_back(pr[1]);
+                nh.bar_type.push_back(h.bar_type[b]);
+                if (!h.bar_donor.empty()) nh.bar_donor.push_back(-1);
+                bar_zero.push_back(0);
+            }
+        } else {
+            nh.bars.push_back(h.bars[2 * b]);
+            nh.bars.push_back(h.bars[2 * b + 1]);
+            nh.bar_type.push_back(h.bar_type[b]);
+            if (!h.bar_donor.empty()) nh.bar_donor.push_back(h.bar_donor[b]);
+            bar_zero.push_back(mesh->bar_zero[b] && !raise[e]);
+        }
+    }
+    for (size_t b = 0; b < nh.bar_donor.size(); ++b)
+        if (nh.bar_type[b] == NDG_BC_PERIODIC) nh.bar_donor[b] = bar_new[(size_t)nh.bar_donor[b]];
+    Topology topo;
+    ndg_status st = build_topology(nh, topo);
+    if (st != NDG_OK) return st;
+
+    // Child reference coordinates in the parent (quad_map, inverse_quad_map) for the transfer.
+    const std::vector<int32_t> old_order = mesh->order;
+    const std::vector<int64_t> old_dev = mesh->dev_of;
+    std::vector<double> coords;
+    std::vector<int64_t> coord_at((size_t)Knew, -1);
+    for (int64_t k = 0; k < Knew; ++k) {
+        if (!child[k]) continue;
+        v2 Pc[4], Pp[4];
+        for (int q = 0; q < 4; ++q) {
+            Pc[q] = node_xy(nh.xy, nh.quads[4 * k + q]);
+            Pp[q] = node_xy(h.xy, h.quads[4 * parent[k] + q]);
+        }
+        const auto &x = mesh->tables->t.tab[NDG_TABLE_NODES][order[k]];
+        coord_at[k] = (int64_t)coords.size();
+        for (int i = 0; i <= order[k]; ++i)
+            for (int j = 0; j <= order[k]; ++j) {
+                const v2 r = reference_point(quad_point(x[i], x[j], Pc), Pp);
+                coords.push_back(r.x);
+                coords.push_back(r.y);
+            }
+    }
+    const int precision = mesh->d.precision;
+    if (precision == NDG_PRECISION_FP32_COMPENSATED) {
+        if ((st = ndg_mesh_set_precision(mesh, NDG_PRECISION_FP64)) != NDG_OK) return st;
+    }
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_adapt sync");
+    drop_graphs(mesh);
+    // commit the new host topology and build the device mesh
+    DeviceMesh &d = mesh->d;
+    mesh->host = std::move(nh);
+    mesh->link = std::move(topo.link);
+    mesh->periodic_link = std::move(topo.periodic_link);
+    mesh->hang_kind = std::move(topo.hang_kind);
+    mesh->hang_second = std::move(topo.hang_second);
+    mesh->bc_base = std::move(topo.bcs);
+    mesh->bc_bar = std::move(topo.bc_bar);
+    mesh->bar_side = std::move(topo.bar_side);
+    mesh->bar_zero = std::move(bar_zero);
+    mesh->n_faces = topo.n_faces;
+    mesh->status = std::move(status);
+    mesh->rotation = std::move(rotation);
+    mesh->split_level = std::move(level);
+    mesh->host_verts.assign((size_t)(8 * Knew), 0.0);
+    for (int64_t k = 0; k < Knew; ++k)
+        for (int q = 0; q < 4; ++q) {
+            const v2 p = node_xy(mesh->host.xy, mesh->host.quads[4 * k + q]);
+            mesh->host_verts[8 * k + 2 * q] = p.x;
+            mesh->host_verts[8 * k + 2 * q + 1] = p.y;
+        }
+    DeviceMesh nd;
+    if ((st = build_device(mesh, order, nd)) != NDG_OK) {
+        free_device(nd);
+        return st;
+    }
+    auto offset = [](const DeviceMesh &dm, int N, int64_t dev) -> int64_t {
+        for (const Group &G : dm.groups)
+            if (G.N == N) return G.q_off + (dev - G.e_begin) * 3 * (int64_t)(N + 1) * (N + 1);
+        return -1;
+    };
+    std::vector<TransferRec> recs((size_t)Knew);
+    for (int64_t k = 0; k < Knew; ++k) {
+        TransferRec &R = recs[(size_t)mesh->dev_of[k]];
+        const int64_t e = parent[k];
+        R.old_off = offset(d, old_order[e], old_dev[e]);
+        R.new_off = offset(nd, order[k], mesh->dev_of[k]);
+        R.N_old = old_order[e];
+        R.N_new = order[k];
+        R.coord_off = coord_at[k];
+    }
+    TransferRec *recs_dev = nullptr;
+    double *coords_dev = nullptr;
+    st = device_upload(&recs_dev, recs, "upload transfer records");
+    if (st == NDG_OK && !coords.empty()) st = device_upload(&coords_dev, coords, "upload child coordinates");
+    if (st == NDG_OK) {
+        const hipError_t e1 = launch_transfer(d.q, nd.q, recs_dev, coords_dev, Knew, nd.tab1d, mesh->stream);
+        const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(mesh->stream) : e1;
+        if (e2 != hipSuccess) st = hip_fail(e2, "transfer kernel");
+    }
+    if (recs_dev) (void)hipFree(recs_dev);
+    if (coords_dev) (void)hipFree(coords_dev);
+    if (st != NDG_OK) {
+        free_device(nd);
+        return st;
+    }
+    nd.clock = d.clock;
+    d.clock = nullptr;
+    nd.wave_speed = d.wave_speed;
+    free_device(d);
+    d = nd;
+    HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * d.n_q, mesh->stream), "hipMemset g");
+    HIP_TRY(launch_traces(d, mesh->stream), "traces kernel");
+    if (precision == NDG_PRECISION_FP32_COMPENSATED) {
+        if ((st = ndg_mesh_set_precision(mesh, precision)) != NDG_OK) return st;
+    }
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_adapt sync");
+    if (n_changed) *n_changed = n_split + n_raise;
+    return NDG_OK;
+}
+
+}  // namespace
+
 extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split_level, int64_t *n_raised) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_adapt: null mesh");
     DeviceMesh &d = mesh->d;
@@ -1105,11 +1545,10 @@ extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split
     std::vector<int32_t> ref((size_t)K), coa((size_t)K);
     ndg_status st = ndg_estimate_error(mesh, err.data(), sig.data(), ref.data(), coa.data());
     if (st != NDG_OK) return st;
-    // Element2D_t.cpp:802-812 (elements are never split here, so their split level is 0)
+    // Element2D_t.cpp:802-812: any would_h_refine takes the splitting branch (hp_adapt)
     for (int64_t e = 0; e < K; ++e)
-        if (ref[e] && sig[e] < 1.0 && 0 < max_split_level)
-            return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: an element would be split (h-refinement is not provided; "
-                                             "use max_split_level 0)");
+        if (ref[e] && sig[e] < 1.0 && mesh->split_level[e] < max_split_level)
+            return adapt_split(mesh, ref, sig, N_max, max_split_level, n_raised);
     std::vector<int32_t> order = mesh->order;
     int64_t raised = 0;
     for (int64_t e = 0; e < K; ++e)
@@ -1126,7 +1565,8 @@ extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split
     // zeroes its edge scaling factor (Element2D_t.cpp:735-741).
     for (int64_t es = 0; es < 4 * K; ++es) {
         const int64_t l = mesh->link[es];
-        if (l < 0 && l != INT64_MIN && order[es / 4] != mesh->order[es / 4]) mesh->ghost_zero[(size_t)(-l - 1)] = 1;
+        if (l < 0 && l != INT64_MIN && order[es / 4] != mesh->order[es / 4])
+            mesh->bar_zero[(size_t)mesh->bc_bar[(size_t)(-l - 1)]] = 1;
     }
     const int precision = d.precision;
     if (precision == NDG_PRECISION_FP32_COMPENSATED) {
@@ -1159,11 +1599,12 @@ extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split
         R.new_off = offset(nd, order[e], mesh->dev_of[e]);
         R.N_old = old_order[e];
         R.N_new = order[e];
+        R.coord_off = -1;
     }
     TransferRec *recs_dev = nullptr;
     st = device_upload(&recs_dev, recs, "upload transfer records");
     if (st == NDG_OK) {
-        const hipError_t e1 = launch_transfer(d.q, nd.q, recs_dev, K, nd.tab1d, mesh->stream);
+        const hipError_t e1 = launch_transfer(d.q, nd.q, recs_dev, nullptr, K, nd.tab1d, mesh->stream);
         const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(mesh->stream) : e1;
         if (e2 != hipSuccess) st = hip_fail(e2, "transfer kernel");
     }
@@ -1263,7 +1704,12 @@ extern "C" ndg_status ndg_estimate_error(ndg_mesh *mesh, double *error, double *
     ndg_status st = refresh_fp64(mesh);
     if (st != NDG_OK) return st;
     DeviceMesh &d = mesh->d;
-    if (!mesh->est_dev) HIP_TRY(hipMalloc((void **)&mesh->est_dev, sizeof(double) * 4 * d.K), "hipMalloc estimator");
+    if (mesh->est_len < d.K) {
+        if (mesh->est_dev) HIP_TRY(hipFree(mesh->est_dev), "hipFree");
+        mesh->est_dev = nullptr;
+        HIP_TRY(hipMalloc((void **)&mesh->est_dev, sizeof(double) * 4 * d.K), "hipMalloc estimator");
+        mesh->est_len = d.K;
+    }
     HIP_TRY(launch_estimate(d, mesh->tol_min, mesh->tol_max, mesh->est_dev, mesh->stream), "estimator kernel");
     std::vector<double> host((size_t)(4 * d.K));
     HIP_TRY(hipMemcpyAsync(host.data(), mesh->est_dev, sizeof(double) * 4 * d.K, hipMemcpyDeviceToHost, mesh->stream),

@@ -122,12 +122,6 @@ class Mesh2D_t:
         h = ctypes.c_void_p()
         check(lib().ndg_mesh_create(ctypes.byref(d), NDG.handle, ctypes.byref(h)))
         self._h = h
-        ne, nf, nd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-        nmin, nmax = ctypes.c_int32(), ctypes.c_int32()
-        check(lib().ndg_mesh_info(h, ctypes.byref(ne), ctypes.byref(nf), ctypes.byref(nd), ctypes.byref(nmin),
-                                  ctypes.byref(nmax)))
-        self.n_elements_ = ne.value
-        self.n_faces_ = nf.value
         self._refresh()
         if adaptivity_interval:
             check(lib().ndg_mesh_set_adaptivity(h, int(adaptivity_interval)))
@@ -138,6 +132,8 @@ class Mesh2D_t:
         nmin, nmax = ctypes.c_int32(), ctypes.c_int32()
         check(lib().ndg_mesh_info(self._h, ctypes.byref(ne), ctypes.byref(nf), ctypes.byref(nd),
                                   ctypes.byref(nmin), ctypes.byref(nmax)))
+        self.n_elements_ = ne.value
+        self.n_faces_ = nf.value
         self.n_dofs_per_field_ = nd.value
         self.N_ = nmin.value
         self.N_max_ = nmax.value
@@ -145,8 +141,9 @@ class Mesh2D_t:
         check(lib().ndg_mesh_orders(self._h, ptr(self.orders_)))
 
     def adapt(self, N_max=None, max_split_level=None):
-        """Mesh2D_t::adapt without splitting: estimate, then raise by 2 the order of the elements
-        that would p-refine. Returns the number of raised elements."""
+        """Mesh2D_t::adapt: estimate, then either split the elements that would h-refine (when
+        any would, the hp_adapt branch) or raise by 2 the order of those that would p-refine.
+        Returns the number of changed elements."""
         n = ctypes.c_int64()
         check(lib().ndg_adapt(self._h, int(self._ndg.N_max_ if N_max is None else N_max),
                               int(self.max_split_level_ if max_split_level is None else max_split_level),
@@ -214,6 +211,17 @@ class Mesh2D_t:
     def get_solution(self):
         out = [np.empty(self.n_dofs_per_field_, dtype=np.float64) for _ in range(3)]
         check(lib().ndg_get_state(self._h, *(ptr(a) for a in out)))
+        return tuple(out)
+
+    def get_interpolated_solution(self):
+        """Mesh2D_t::get_solution (src/meshes/Mesh2D_t.cpp:4237-4249): x, y, p, u, v at the n x n
+        uniform output points of every element (n = the tables' n_interpolation_points), arrays of
+        shape (n_elements, n, n). Also kept as x_output_, y_output_, p_output_, u_output_,
+        v_output_ like the reference's members."""
+        n = self._ndg.N_interpolation_points_
+        out = [np.empty((self.n_elements_, n, n), dtype=np.float64) for _ in range(5)]
+        check(lib().ndg_get_solution(self._h, *(ptr(a) for a in out)))
+        self.x_output_, self.y_output_, self.p_output_, self.u_output_, self.v_output_ = out
         return tuple(out)
 
     def interpolate_to_boundaries(self):

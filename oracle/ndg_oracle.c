@@ -841,6 +841,37 @@ static void get3(const o_mesh *m, int which, double *a, double *b, double *c) {
 void o_get_state(const o_mesh *m, double *p, double *u, double *v) { get3(m, 0, p, u, v); }
 void o_get_rhs(const o_mesh *m, double *Gp, double *Gu, double *Gv) { get3(m, 1, Gp, Gu, Gv); }
 
+/* Mesh2D_t::get_solution (src/meshes/Mesh2D_t.cpp:4237-4249) and Element2D_t::interpolate_solution
+ * (src/entities/Element2D_t.cpp:581-604): per element, x and y by quad_map at the uniform points
+ * (i/(n-1))*2-1 and each field as sum_m sum_k q[m][k] I[i][m] I[j][k]. */
+void o_get_solution(const o_mesh *m, double *x, double *y, double *p, double *u, double *v) {
+    const int n = m->t->n_interp;
+    for (int64_t k = 0; k < m->K; ++k) {
+        const o_elem *e = &m->el[k];
+        const int nf = e->N + 1;
+        const double *I = m->t->tab[O_INTERP][e->N];
+        const v2 P[4] = {m->xy[e->nodes[0]], m->xy[e->nodes[1]], m->xy[e->nodes[2]], m->xy[e->nodes[3]]};
+        const int64_t off = k * (int64_t)n * n;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                const v2 g = quad_map((double)i / (double)(n - 1) * 2 - 1, (double)j / (double)(n - 1) * 2 - 1, P);
+                const int64_t at = off + (int64_t)i * n + j;
+                x[at] = g.x;
+                y[at] = g.y;
+                double sp = 0.0, su = 0.0, sv = 0.0;
+                for (int a = 0; a <= e->N; ++a)
+                    for (int b = 0; b <= e->N; ++b) {
+                        sp += e->p[a * nf + b] * I[i * nf + a] * I[j * nf + b];
+                        su += e->u[a * nf + b] * I[i * nf + a] * I[j * nf + b];
+                        sv += e->v[a * nf + b] * I[i * nf + a] * I[j * nf + b];
+                    }
+                p[at] = sp;
+                u[at] = su;
+                v[at] = sv;
+            }
+    }
+}
+
 static void get_sides(const o_mesh *m, int flux, double *out) {
     int64_t off = 0;
     for (int64_t i = 0; i < m->K; ++i) {

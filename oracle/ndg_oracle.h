@@ -71,6 +71,9 @@ void o_set_state(o_mesh *m, const double *p, const double *u, const double *v);
 void o_get_state(const o_mesh *m, double *p, double *u, double *v);
 void o_get_rhs(const o_mesh *m, double *Gp, double *Gu, double *Gv);
 /* Element traces p,u,v on the 4 sides, packed [element][side][field][Nf]. */
+/* Mesh2D_t::get_solution: x, y, p, u, v at the tables' n x n uniform output points per element,
+ * element k's point (i, j) at k*n*n + i*n + j. */
+void o_get_solution(const o_mesh *m, double *x, double *y, double *p, double *u, double *v);
 void o_get_traces(const o_mesh *m, double *out);
 /* Element flux traces, packed [element][side][field][Nf]. */
 void o_get_flux_traces(const o_mesh *m, double *out);

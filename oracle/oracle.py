@@ -42,6 +42,7 @@ def lib():
             "o_initial_conditions_pulses": ([vp, i32, vp], None),
             "o_set_state": ([vp, vp, vp, vp], None), "o_get_state": ([vp, vp, vp, vp], None),
             "o_get_rhs": ([vp, vp, vp, vp], None), "o_get_traces": ([vp, vp], None),
+            "o_get_solution": ([vp, vp, vp, vp, vp, vp], None),
             "o_get_flux_traces": ([vp, vp], None), "o_get_delta_t": ([vp, f64], f64),
             "o_compute_rhs": ([vp, f64], None), "o_mesh_set_wave_speed": ([vp, f64], None),
             "o_adapt": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int64),
@@ -203,6 +204,13 @@ class Mesh:
         self.N = np.empty(self.K, dtype=np.int32)
         self.n = lib().o_mesh_n_nodes_total(self.h)
         lib().o_mesh_get_N(self.h, _p(self.N))
+
+    def get_solution(self, n):
+        """x, y, p, u, v at the n x n uniform output points of every element, shape (K, n, n)
+        (n: the tables' interpolation point count)."""
+        out = [np.empty((self.K, n, n)) for _ in range(5)]
+        lib().o_get_solution(self.h, *(_p(a) for a in out))
+        return tuple(out)
 
     def estimate_error(self):
         err, sig = np.empty(self.K), np.empty(self.K)

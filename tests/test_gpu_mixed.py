@@ -214,6 +214,32 @@ def test_adaptive_solve_matches_oracle(tables, periodic, bc):
         assert rel(g, r) < TOL
 
 
+@pytest.mark.parametrize("bc,ic", [((nd.BC_WALL,) * 4, PULSES), (MIXED_BC, None),
+                                   ((nd.BC_WALL,) * 4, [(0.1, -0.2, 0.08, 1.0)])])
+def test_adaptive_split_solve_matches_oracle(tables, bc, ic):
+    """hp-adaptation every 4 steps with max_split_level = 1: elements whose estimate asks for h
+    refinement become four children in Hilbert order (state interpolated through the inverse
+    bilinear map), split ghosts become two, the others are raised by 2; hanging faces appear
+    between split and unsplit elements. Element count, orders, time and state against the
+    oracle's restatement."""
+    say("adaptive split solve", bc, ic is None)
+    desc = nd.square_mesh(8, bc=bc, jitter=0.1, seed=7)
+    gm = nd.Mesh2D_t(desc, 3, 9, 5, 1, 4, 0, 1e-4, 1e-14, 0.0, nd.NDG_t(9, 5))
+    om = O.Mesh(O.Tables(9, 5), desc, 3, tol_min=1e-4, tol_max=1e-14)
+    gm.initial_conditions(ic)
+    om.initial_conditions(ic)
+    solver = nd.Solver2D_t(0.5, [0.2])
+    solver.solve(None, gm)
+    steps, t_cpu = om.solve_adaptive(0.5, 0.2, 4, 9, 1)
+    say("steps", steps, "K", om.K, "orders", np.bincount(om.N))
+    assert om.K > 64
+    assert gm.n_elements_ == om.K and gm.n_faces_ == om.n_faces
+    assert solver.n_steps_ == steps and solver.time_ == t_cpu
+    assert np.array_equal(gm.orders_, om.N)
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL
+
+
 HANGING = [
     ("periodic-uniform", dict(periodic=True, jitter=0.15), (5,)),
     ("periodic-mixed", dict(periodic=True, jitter=0.1), (3, 5, 6)),
@@ -269,3 +295,21 @@ def test_hanging_fp32c(tables):
     om.run_steps(0.5, 20)
     for g, r in zip(gm.get_solution(), om.get_state()):
         assert rel(g, r) < 2e-5
+
+
+def test_output_interpolation(tables):
+    """Mesh2D_t::get_solution on a mesh with mixed orders and hanging faces after a few steps:
+    x, y, p, u, v at the uniform output points against the oracle."""
+    say("output interpolation")
+    desc, gm, om = hanging_build(tables, 8, dict(jitter=0.1), (3, 5, 6), 21)
+    gm.initial_conditions(PULSES)
+    om.initial_conditions(PULSES)
+    nd.Solver2D_t(0.5, [0.0]).run_steps(gm, 3)
+    om.run_steps(0.5, 3)
+    got = gm.get_interpolated_solution()
+    ref = om.get_solution(5)
+    assert got[0].shape == (om.K, 5, 5)
+    for g, r in zip(got[:2], ref[:2]):
+        assert np.max(np.abs(g - r)) < 1e-15
+    for g, r in zip(got[2:], ref[2:]):
+        assert rel(g, r) < TOL
