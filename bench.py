@@ -185,12 +185,12 @@ def precision_leg(nd, torch, stream):
     return out
 
 
-def adjoint_leg(nd, torch, stream):
+def adjoint_leg(nd, torch, stream, steps=2000):
     """128 x 128 periodic, N = 8, 2000 steps, checkpoints every 50, 32 seeded receivers, data from a
     c = 1.05 run of two pulses: gradient time against one forward solve, and one central finite
-    difference (eps = 1e-6) along a seeded direction."""
+    difference (eps = 1e-6) along a seeded direction. (Fewer steps only for profiler runs.)"""
     from ndg_cuda_amd import adjoint as A
-    res, n, steps, interval = 128, 8, 2000, 50
+    res, n, interval = 128, 8, 50
     desc = nd.square_mesh(res, x_periodic=True, y_periodic=True)
     mesh = nd.Mesh2D_t(desc, n, n, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, nd.NDG_t(n, 5))
     mesh.set_stream(stream.cuda_stream)
@@ -228,6 +228,18 @@ def adjoint_leg(nd, torch, stream):
             "note": "wall times of the library calls (host-synchronous), forward = one recorded 2000-step run"}
 
 
+def separate_forms(args, legs, nd, torch, stream):
+    """The form legs that run on workloads of their own (the FP32 leg reuses the main mesh)."""
+    forms = {}
+    if "batched" in legs:
+        forms["batched"] = batched_leg(args, nd, torch, stream)
+    if "precision" in legs:
+        forms["precision_1024"] = precision_leg(nd, torch, stream)
+    if "adjoint" in legs:
+        forms["adjoint_128"] = adjoint_leg(nd, torch, stream, args.adjoint_steps)
+    return forms
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,7 +247,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--fp64-only", action="store_true", help="skip the FP32-compensated leg")
+    ap.add_argument("--fp64-only", action="store_true", help="skip every form leg (same as --forms none)")
+    ap.add_argument("--forms", default="batched,precision,adjoint,fp32c",
+                    help="comma-separated form legs to run after the main line (or 'none')")
+    ap.add_argument("--forms-only", action="store_true",
+                    help="profiler runs: skip the main line and print only the selected form legs")
+    ap.add_argument("--adjoint-steps", type=int, default=2000,
+                    help="time steps of the adjoint leg (2000 is the workload; fewer for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -259,10 +277,14 @@ def main():
 
     if nd.device_count() < 1 or not torch.cuda.is_available():
         raise SystemExit("bench.py: no GPU visible")
+    stream = torch.cuda.Stream()
+    legs = set() if args.fp64_only else set(args.forms.split(","))
+    if args.forms_only:
+        print(json.dumps({"forms": separate_forms(args, legs, nd, torch, stream)}))
+        return
     desc = nd.square_mesh(RES, jitter=JITTER, seed=SEED)
     ndg = nd.NDG_t(N, 5)
     mesh = nd.Mesh2D_t(desc, N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
-    stream = torch.cuda.Stream()
     mesh.set_stream(stream.cuda_stream)
     solver = nd.Solver2D_t(CFL, [0.0])
 
@@ -306,11 +328,8 @@ def main():
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
-    forms = {}
-    if not args.fp64_only:
-        forms["batched"] = batched_leg(args, nd, torch, stream)
-        forms["precision_1024"] = precision_leg(nd, torch, stream)
-        forms["adjoint_128"] = adjoint_leg(nd, torch, stream)
+    forms = separate_forms(args, legs, nd, torch, stream)
+    if "fp32c" in legs:
         ms32, _, st32 = timed_run("fp32c")
         diff = np.concatenate([a - b for a, b in zip(st32, ref_state)])
         refv = np.concatenate(ref_state)

@@ -62,11 +62,11 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
     __shared__ double s_side[EPB * 4 * 3];
     __shared__ double s_v[EPB * 8];
     __shared__ double s_red[3];
-    const double *Dh = tab;
     const double *lm = tab + NF * NF;
     const double *lp = lm + NF;
     const double *x = lp + 2 * NF;
     const double *winv = x + NF;
+    const double *DhT = tab + 2 * NF * NF + 5 * NF;  // DhT[a][i] = Dh[i][a]
 
     const int tid = threadIdx.x;
     const int f = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -102,8 +102,11 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
     for (int it = 0; it < QPT; ++it) {
         const int r = tid + it * T;
         const bool in = r < nel * 3 * NE;
-        lreg[it] = in ? lg[r] : 0.0;
-        mreg[it] = (in && (flags & kAdjMuIn)) ? mg[r] : 0.0;
+        const int rr = in ? r : 0;  // in-range address for every lane, value selected below
+        const double lv = lg[rr];
+        const double mv = (flags & kAdjMuIn) ? mg[rr] : 0.0;
+        lreg[it] = in ? lv : 0.0;
+        mreg[it] = in ? mv : 0.0;
     }
     for (int r = tid; r < nel * 8; r += T) s_v[r] = verts[e0 * 8 + r];
     __syncthreads();
@@ -134,132 +137,77 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
     }
     __syncthreads();
 
-    auto jac_at = [&](int e, int node) {
+    // -c / J at node (a, b) of element e (J from the metric lines, as the forward stage).
+    auto scale_at = [&](int e, int a, int b) {
         const double *met = s_met + e * 4 * NF;
-        const int a = node / NF, b = node % NF;
-        return met[2 * NF + b] * met[NF + a] - met[a] * met[3 * NF + b];
+        return -cw * recip(met[2 * NF + b] * met[NF + a] - met[a] * met[3 * NF + b]);
     };
 
     if (apply) {
-        // P2: mu~ = mu + gs lambda; mu' = a_s mu~ (stored); nu = -c mu~ / J.
+        // P2: mu~ = mu + gs lambda (to s_n; scaled to nu = -c mu~ / J where it is read);
+        // mu' = a_s mu~ (stored). Lanes past the last element hold zeros and store nothing.
         double *mo = mu + e0 * 3 * NE;
 #pragma unroll
         for (int it = 0; it < QPT; ++it) {
             const int r = tid + it * T;
+            const double mt = mreg[it] + gs * lreg[it];
+            mreg[it] = as * mt;
             if (r < nel * 3 * NE) {
-                const double mt = mreg[it] + gs * lreg[it];
-                mreg[it] = as * mt;
                 if (flags & kAdjMuOut) mo[r] = mreg[it];
-                s_n[r] = -cw * mt * recip(jac_at(r / (3 * NE), r % NE));
+                s_n[r] = mt;
             }
         }
         __syncthreads();
 
-        // P3: xi lines (thread owns column j = l): volume transpose along xi into d lambda,
-        // lift transposes of sides 1 and 3. Field p takes Fbar_u and Fbar_v, fields u and v take
-        // Fbar_p (a wave-uniform branch).
+        // P3: per field f, the lift transposes rho of the four sides and the volume transpose
+        // Fbar_f[a][l] = sum_i Dh[i][a] nu_f[i][l] (column l, into s_a); row l of nu_f stays in
+        // registers for P4. One contraction per field and direction: the same work in each wave.
+        double row[NF];
         if (active) {
-            const int j = l;
-            const double *nb = s_n + el * 3 * NE;
-            const double *met = s_met + el * 4 * NF;
-            double own[NF];
+            const double *nb = s_n + el * 3 * NE + f * NE;
+            double col[NF];
 #pragma unroll
-            for (int i = 0; i < NF; ++i) own[i] = nb[f * NE + i * NF + j];
-            double r1 = 0.0, r3 = 0.0;
+            for (int m = 0; m < NF; ++m) {
+                col[m] = nb[m * NF + l] * scale_at(el, m, l);
+                row[m] = nb[l * NF + m] * scale_at(el, l, m);
+            }
+            double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
 #pragma unroll
-            for (int i = 0; i < NF; ++i) {
-                r1 += lp[i] * winv[i] * own[i];
-                r3 += lm[i] * winv[i] * own[i];
+            for (int m = 0; m < NF; ++m) {
+                r1 += lp[m] * winv[m] * col[m];
+                r3 += lm[m] * winv[m] * col[m];
+                r0 += lm[m] * winv[m] * row[m];
+                r2 += lp[m] * winv[m] * row[m];
             }
             double *rho = s_rho + el * 12 * NF + f * NF;
-            rho[3 * NF + j] = r1;
-            rho[9 * NF + (N - j)] = r3;
-            double *da = s_a + el * 3 * NE + f * NE + j;
-            if (f == 0) {
-                double ou[NF], ov[NF];
-#pragma unroll
-                for (int i = 0; i < NF; ++i) {
-                    ou[i] = nb[NE + i * NF + j];
-                    ov[i] = nb[2 * NE + i * NF + j];
-                }
+            rho[l] = r0;
+            rho[3 * NF + l] = r1;
+            rho[6 * NF + (N - l)] = r2;
+            rho[9 * NF + (N - l)] = r3;
+            double *fb = s_a + el * 3 * NE + f * NE + l;
 #pragma unroll 1
-                for (int a = 0; a < NF; ++a) {
-                    double Fu = 0.0, Fv = 0.0;
+            for (int a = 0; a < NF; ++a) {
+                double F = 0.0;
 #pragma unroll
-                    for (int i = 0; i < NF; ++i) {
-                        Fu += Dh[i * NF + a] * ou[i];
-                        Fv += Dh[i * NF + a] * ov[i];
-                    }
-                    da[a * NF] = met[NF + a] * Fu - met[a] * Fv;
-                }
-            } else {
-                double op[NF];
-#pragma unroll
-                for (int i = 0; i < NF; ++i) op[i] = nb[i * NF + j];
-#pragma unroll 1
-                for (int a = 0; a < NF; ++a) {
-                    double Fp = 0.0;
-#pragma unroll
-                    for (int i = 0; i < NF; ++i) Fp += Dh[i * NF + a] * op[i];
-                    da[a * NF] = f == 1 ? met[NF + a] * Fp : -(met[a] * Fp);
-                }
+                for (int i = 0; i < NF; ++i) F += DhT[a * NF + i] * col[i];
+                fb[a * NF] = F;
             }
         }
         __syncthreads();
 
-        // P4: eta lines (thread owns row i = l): volume transpose along eta added to d lambda,
-        // lift transposes of sides 0 and 2.
+        // P4: Gbar_f[l][b] = sum_j Dh[j][b] nu_f[l][j] in place of row l of nu_f (only this
+        // thread reads that row from here on); flux transpose at the side points: own trace
+        // adjoint tau = (omega_L, n omega_L) plus the neighbour's export for this point.
         if (active) {
-            const int i = l;
-            const double *nb = s_n + el * 3 * NE;
-            const double *met = s_met + el * 4 * NF;
-            double own[NF];
-#pragma unroll
-            for (int j = 0; j < NF; ++j) own[j] = nb[f * NE + i * NF + j];
-            double r0 = 0.0, r2 = 0.0;
-#pragma unroll
-            for (int j = 0; j < NF; ++j) {
-                r0 += lm[j] * winv[j] * own[j];
-                r2 += lp[j] * winv[j] * own[j];
-            }
-            double *rho = s_rho + el * 12 * NF + f * NF;
-            rho[i] = r0;
-            rho[6 * NF + (N - i)] = r2;
-            double *da = s_a + el * 3 * NE + f * NE + i * NF;
-            if (f == 0) {
-                double ou[NF], ov[NF];
-#pragma unroll
-                for (int j = 0; j < NF; ++j) {
-                    ou[j] = nb[NE + i * NF + j];
-                    ov[j] = nb[2 * NE + i * NF + j];
-                }
+            double *gs_ = s_n + el * 3 * NE + f * NE + l * NF;
 #pragma unroll 1
-                for (int b = 0; b < NF; ++b) {
-                    double Gu = 0.0, Gv = 0.0;
+            for (int b = 0; b < NF; ++b) {
+                double G = 0.0;
 #pragma unroll
-                    for (int j = 0; j < NF; ++j) {
-                        Gu += Dh[j * NF + b] * ou[j];
-                        Gv += Dh[j * NF + b] * ov[j];
-                    }
-                    da[b] += -(met[3 * NF + b] * Gu) + met[2 * NF + b] * Gv;
-                }
-            } else {
-                double op[NF];
-#pragma unroll
-                for (int j = 0; j < NF; ++j) op[j] = nb[i * NF + j];
-#pragma unroll 1
-                for (int b = 0; b < NF; ++b) {
-                    double Gp = 0.0;
-#pragma unroll
-                    for (int j = 0; j < NF; ++j) Gp += Dh[j * NF + b] * op[j];
-                    da[b] += f == 1 ? -(met[3 * NF + b] * Gp) : met[2 * NF + b] * Gp;
-                }
+                for (int j = 0; j < NF; ++j) G += DhT[b * NF + j] * row[j];
+                gs_[b] = G;
             }
         }
-        __syncthreads();
-
-        // P5: flux transpose at the side points: own trace adjoint tau = (omega_L, n omega_L)
-        // plus the neighbour's export for this point.
 #pragma unroll
         for (int it = 0; it < FPT; ++it) {
             const int item = tid + it * T;
@@ -276,30 +224,52 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         }
         __syncthreads();
 
-        // P6: trace transposes added to d lambda (thread owns row i = l).
+        // P5 (thread owns row i = l of field f): d lambda_f[i][b] from the volume transposes of
+        // the fields that f's fluxes depend on, plus the trace transposes:
+        //   p: y_eta(i) Fbar_u - x_eta(i) Fbar_v - y_xi(b) Gbar_u + x_xi(b) Gbar_v,
+        //   u: y_eta(i) Fbar_p - y_xi(b) Gbar_p,   v: -x_eta(i) Fbar_p + x_xi(b) Gbar_p.
+        double da[NF];
         if (active) {
             const int i = l;
+            const double *Fb = s_a + el * 3 * NE + i * NF;
+            const double *Gb = s_n + el * 3 * NE + i * NF;
+            const double *met = s_met + el * 4 * NF;
             const double *tau = s_rho + el * 12 * NF + f * NF;
             const double t0 = tau[i], t2 = tau[6 * NF + (N - i)];
-            double *da = s_a + el * 3 * NE + f * NE + i * NF;
+            const double ye = met[NF + i], xe = met[i], lpi = lp[i], lmi = lm[i];
 #pragma unroll
-            for (int b = 0; b < NF; ++b)
-                da[b] += lm[b] * t0 + lp[b] * t2 + lp[i] * tau[3 * NF + b] + lm[i] * tau[9 * NF + (N - b)];
+            for (int b = 0; b < NF; ++b) {
+                const double yx = met[3 * NF + b], xx = met[2 * NF + b];
+                double v;
+                if (f == 0)
+                    v = ye * Fb[NE + b] - xe * Fb[2 * NE + b] - yx * Gb[NE + b] + xx * Gb[2 * NE + b];
+                else if (f == 1)
+                    v = ye * Fb[b] - yx * Gb[b];
+                else
+                    v = -(xe * Fb[b]) + xx * Gb[b];
+                da[b] = v + lm[b] * t0 + lp[b] * t2 + lpi * tau[3 * NF + b] + lmi * tau[9 * NF + (N - b)];
+            }
+        }
+        __syncthreads();
+        if (active) {
+            double *dst = s_a + el * 3 * NE + f * NE + l * NF;
+#pragma unroll
+            for (int b = 0; b < NF; ++b) dst[b] = da[b];
         }
         __syncthreads();
 
-        // P6b (coalesced): lambda' = lambda + d lambda, and the c-derivative term <d lambda, q_s>.
+        // P6 (coalesced): lambda' = lambda + d lambda, and the c-derivative term <d lambda, q_s>.
         double dot = 0.0;
         const double *qg = qs ? qs + e0 * 3 * NE : nullptr;
 #pragma unroll
         for (int it = 0; it < QPT; ++it) {
             const int r = tid + it * T;
-            if (r < nel * 3 * NE) {
-                const double dl = s_a[r];
-                if (qg) dot += dl * qg[r];
-                lreg[it] += dl;
-                s_a[r] = lreg[it];
-            }
+            const bool in = r < nel * 3 * NE;
+            const int rr = in ? r : 0;
+            const double dl = in ? s_a[rr] : 0.0;
+            if (qg) dot += dl * qg[rr];
+            lreg[it] += dl;
+            if (in) s_a[r] = lreg[it];
         }
         if (cpart) {
             for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
@@ -321,7 +291,9 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
 #pragma unroll
             for (int it = 0; it < QPT; ++it) {
                 const int r = tid + it * T;
-                if (r < nel * 3 * NE) lreg[it] = s_a[r];
+                const bool in = r < nel * 3 * NE;
+                const double v = s_a[in ? r : 0];
+                lreg[it] = in ? v : 0.0;
             }
         }
 
@@ -334,12 +306,12 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
     }
     if (!(flags & kAdjExport)) return;
 
-    // P7: nu of the next reverse stage, from mu' + gnext lambda'.
+    // P7: mu~ of the next reverse stage, mu' + gnext lambda' (scaled to nu where it is read).
 #pragma unroll
     for (int it = 0; it < QPT; ++it) {
         const int r = tid + it * T;
-        if (r < nel * 3 * NE)
-            s_n[r] = -cw * (mreg[it] + gnext * lreg[it]) * recip(jac_at(r / (3 * NE), r % NE));
+        const double v = mreg[it] + gnext * lreg[it];
+        if (r < nel * 3 * NE) s_n[r] = v;
     }
     __syncthreads();
     // P8: lift transposes of all four sides (thread owns column l and row l).
@@ -348,7 +320,7 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
 #pragma unroll
         for (int m = 0; m < NF; ++m) {
-            const double col = nf_[m * NF + l], row = nf_[l * NF + m];
+            const double col = nf_[m * NF + l] * scale_at(el, m, l), row = nf_[l * NF + m] * scale_at(el, l, m);
             r1 += lp[m] * winv[m] * col;
             r3 += lm[m] * winv[m] * col;
             r0 += lm[m] * winv[m] * row;

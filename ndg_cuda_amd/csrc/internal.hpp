@@ -53,8 +53,13 @@ struct MortarFace {
 };
 
 // Per-order constant tables in device memory, packed as Dhat[Nf*Nf], l(-1)[Nf], l(+1)[Nf], w[Nf],
-// x[Nf], 1/w[Nf], then the modal matrix poly[Nf*Nf] used by the estimator.
+// x[Nf], 1/w[Nf], then the modal matrix poly[Nf*Nf] used by the estimator and the transpose
+// Dhat^T[Nf*Nf] used by the adjoint (its columns as contiguous rows).
 inline int64_t table_pack_len(int N) {
+    const int nf = N + 1;
+    return (int64_t)nf * nf * 3 + 5 * nf;
+}
+inline int64_t table_dhat_t_offset(int N) {
     const int nf = N + 1;
     return (int64_t)nf * nf * 2 + 5 * nf;
 }

@@ -169,6 +169,8 @@ std::vector<double> pack_tables(const Tables &t, int N) {
     for (int i = 0; i < nf; ++i) out.push_back(1.0 / t.tab[NDG_TABLE_WEIGHTS][N][i]);
     const auto &M = t.tab[NDG_TABLE_POLYNOMIALS][N];
     out.insert(out.end(), M.begin(), M.end());
+    for (int a = 0; a < nf; ++a)
+        for (int i = 0; i < nf; ++i) out.push_back(Dh[i * nf + a]);
     return out;
 }
 
