@@ -243,6 +243,13 @@ ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *rec, double
                                 int64_t checkpoint_interval, const double *data, double *J, double *dJ_dp0,
                                 double *dJ_du0, double *dJ_dv0, double *dJ_dc);
 
+/* Device memory (bytes) ndg_adjoint_gradient may spend on stage states beyond one segment (a
+ * segment is checkpoint_interval steps, 3 states per step): the forward pass keeps the stage states
+ * of the last 1 + floor(bytes / segment bytes) segments as it goes, and the reverse sweep recomputes
+ * only the segments before them. 0 (the default) keeps the last segment only: every other segment
+ * is recomputed from its checkpoint. The result is the same bit for bit. No reference counterpart. */
+ndg_status ndg_adjoint_set_store_budget(ndg_mesh *mesh, int64_t bytes);
+
 /* lambda = G^T mu for the stage right-hand side G (ndg_compute_rhs without boundary terms), packed
  * fields; for dot-product tests of the adjoint. */
 ndg_status ndg_apply_rhs_transpose(ndg_mesh *mesh, const double *mp, const double *mu, const double *mv,

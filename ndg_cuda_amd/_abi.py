@@ -34,7 +34,7 @@ EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tabl
             "ndg_set_state", "ndg_get_state", "ndg_get_solution", "ndg_get_traces", "ndg_compute_rhs",
             "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_run_steps", "ndg_synchronize",
             "ndg_estimate_error", "ndg_mesh_set_graph", "ndg_mesh_autotune", "ndg_mesh_tuning",
-            "ndg_mesh_set_wave_speed", "ndg_record_receivers", "ndg_adjoint_gradient", "ndg_apply_rhs_transpose",
+            "ndg_mesh_set_wave_speed", "ndg_record_receivers", "ndg_adjoint_set_store_budget", "ndg_adjoint_gradient", "ndg_apply_rhs_transpose",
             "ndg_adapt", "ndg_mesh_set_adaptivity", "ndg_mesh_orders"]
 
 
@@ -111,6 +111,7 @@ def lib():
             "ndg_mesh_set_wave_speed": [vp, f64],
             "ndg_record_receivers": [vp, vp, f64, i64, vp],
             "ndg_adjoint_gradient": [vp, vp, f64, i64, i64, vp, vp, vp, vp, vp, vp],
+            "ndg_adjoint_set_store_budget": [vp, i64],
             "ndg_apply_rhs_transpose": [vp, vp, vp, vp, vp, vp, vp],
             "ndg_adapt": [vp, i32, i32, vp],
             "ndg_mesh_set_adaptivity": [vp, i64],

@@ -46,6 +46,13 @@ def record_receivers(mesh, receivers, dt, n_steps):
     return out
 
 
+def set_store_budget(mesh, nbytes):
+    """Device memory adjoint_gradient may spend on stage states beyond one segment: the forward pass
+    keeps the last 1 + nbytes // (segment bytes) segments, which are then not recomputed. 0 (the
+    default) recomputes every segment but the last. The gradient is the same bit for bit."""
+    check(lib().ndg_adjoint_set_store_budget(mesh.handle, int(nbytes)))
+
+
 def adjoint_gradient(mesh, receivers, dt, n_steps, checkpoint_interval, data=None):
     """(J, dJ/dp0, dJ/du0, dJ/dv0, dJ/dc) from the mesh's current state, which is left unchanged."""
     n = mesh.n_dofs_per_field_

@@ -112,6 +112,7 @@ struct ndg_mesh {
     std::vector<int32_t> status, rotation, split_level;  // Hilbert status per element
     int initial_N = 0, maximum_N = 0, max_split_level = 0;
     int64_t adapt_interval = 0;
+    int64_t adjoint_store_budget = 0;        // bytes of stage states kept beyond one segment
 };
 
 extern "C" ndg_status ndg_last_error(char *buf, size_t buf_len) {
@@ -1869,6 +1870,12 @@ extern "C" ndg_status ndg_apply_rhs_transpose(ndg_mesh *mesh, const double *mp, 
     return copy_state(mesh, lam.as<double>(), outs, true, "download adjoint");
 }
 
+extern "C" ndg_status ndg_adjoint_set_store_budget(ndg_mesh *mesh, int64_t bytes) {
+    if (!mesh || bytes < 0) return fail(NDG_ERR_INVALID, "ndg_adjoint_set_store_budget: null mesh or negative budget");
+    mesh->adjoint_store_budget = bytes;
+    return NDG_OK;
+}
+
 extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *rec, double dt, int64_t n_steps,
                                            int64_t checkpoint_interval, const double *data, double *J,
                                            double *dJ_dp0, double *dJ_du0, double *dJ_dv0, double *dJ_dc) {
@@ -1884,9 +1891,16 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     const int64_t n_ck = (n_steps + checkpoint_interval - 1) / checkpoint_interval;
     const int64_t seg = std::min(checkpoint_interval, n_steps);
     const size_t qb = sizeof(double) * d.n_q;
-    DevBuf ck, store, lam, mu, chi0, chi1, cpart, recs, res;
+    // Segments k >= first_kept keep their stage states from the forward pass (one block each);
+    // the others are recomputed into block 0 once the kept ones have been consumed.
+    const int64_t seg_bytes = (int64_t)qb * 3 * seg;
+    const int64_t n_kept = std::max<int64_t>(1, std::min<int64_t>(n_ck, 1 + mesh->adjoint_store_budget / seg_bytes));
+    const int64_t first_kept = n_ck - n_kept;
+    DevBuf ck, lam, mu, chi0, chi1, cpart, recs, res;
+    std::vector<DevBuf> store((size_t)n_kept);
     if ((st = dev_alloc(ck, qb * n_ck, "hipMalloc checkpoints")) != NDG_OK) return st;
-    if ((st = dev_alloc(store, qb * 3 * seg, "hipMalloc stage store")) != NDG_OK) return st;
+    for (DevBuf &b : store)
+        if ((st = dev_alloc(b, (size_t)seg_bytes, "hipMalloc stage store")) != NDG_OK) return st;
     if ((st = dev_alloc(lam, qb, "hipMalloc adjoint")) != NDG_OK) return st;
     if ((st = dev_alloc(mu, qb, "hipMalloc adjoint")) != NDG_OK) return st;
     if ((st = dev_alloc(chi0, sizeof(double) * d.n_tr, "hipMalloc adjoint")) != NDG_OK) return st;
@@ -1896,20 +1910,24 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     if ((st = dev_alloc(recs, sizeof(double) * R.n * n_steps, "hipMalloc records")) != NDG_OK) return st;
     if ((st = dev_alloc(res, sizeof(double) * R.n * n_steps, "hipMalloc residuals")) != NDG_OK) return st;
     auto ckp = [&](int64_t k) { return ck.as<double>() + k * d.n_q; };
-    auto stp = [&](int64_t j, int stage) { return store.as<double>() + (j * 3 + stage) * d.n_q; };
+    // stage state (step j of segment k, relative to the segment start)
+    auto stp = [&](int64_t k, int64_t j, int stage) {
+        const DevBuf &b = store[(size_t)(k >= first_kept ? k - first_kept : 0)];
+        return b.as<double>() + (j * 3 + stage) * d.n_q;
+    };
     double *const q_main = d.q;
 
     // Steps n0..n1-1 from the state at q_start, keeping every stage input: the stage kernels write
     // their output state into the next store slot (q_start itself stands for slot (n0, 0)); the
     // output of the last step goes to the mesh's own state buffer. Records the receivers after
     // every step when asked.
-    auto run_stored = [&](const double *q_start, int64_t n0, int64_t n1, bool record) -> ndg_status {
+    auto run_stored = [&](int64_t k, const double *q_start, int64_t n0, int64_t n1, bool record) -> ndg_status {
         d.q = const_cast<double *>(q_start);
         HIP_TRY(launch_traces(d, s), "traces kernel");
         for (int64_t j = n0; j < n1; ++j)
             for (int stage = 0; stage < 3; ++stage) {
-                double *in = (j == n0 && stage == 0) ? const_cast<double *>(q_start) : stp(j - n0, stage);
-                double *out = stage < 2 ? stp(j - n0, stage + 1) : (j + 1 < n1 ? stp(j + 1 - n0, 0) : q_main);
+                double *in = (j == n0 && stage == 0) ? const_cast<double *>(q_start) : stp(k, j - n0, stage);
+                double *out = stage < 2 ? stp(k, j - n0, stage + 1) : (j + 1 < n1 ? stp(k, j + 1 - n0, 0) : q_main);
                 d.q = in;
                 d.q_out = out;
                 HIP_TRY(launch_stage(d, stage, kAm[stage], kGm[stage] * dt, s), "stage kernel");
@@ -1924,20 +1942,23 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
         return NDG_OK;
     };
     auto stage_input = [&](int64_t k, int64_t j, int stage) -> const double * {
-        return (j == k * checkpoint_interval && stage == 0) ? ckp(k) : stp(j - k * checkpoint_interval, stage);
+        return (j == k * checkpoint_interval && stage == 0) ? ckp(k) : stp(k, j - k * checkpoint_interval, stage);
     };
 
-    // Forward pass: checkpoints; the last segment keeps its stage inputs right away.
+    // Forward pass: checkpoints; the kept segments keep their stage inputs right away.
     HIP_TRY(launch_clock_begin(d, 0.0, 1, dt, s), "clock kernel");
-    for (int64_t n = 0; n < (n_ck - 1) * checkpoint_interval; ++n) {
+    for (int64_t n = 0; n < first_kept * checkpoint_interval; ++n) {
         if (n % checkpoint_interval == 0)
             HIP_TRY(hipMemcpyAsync(ckp(n / checkpoint_interval), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
         if ((st = step_from_clock(mesh, dt)) != NDG_OK) return st;
         HIP_TRY(launch_record(d, R.elem.as<int64_t>(), R.w.as<double>(), R.n, recs.as<double>() + n * R.n, s),
                 "record kernel");
     }
-    HIP_TRY(hipMemcpyAsync(ckp(n_ck - 1), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
-    if ((st = run_stored(ckp(n_ck - 1), (n_ck - 1) * checkpoint_interval, n_steps, true)) != NDG_OK) return st;
+    for (int64_t k = first_kept; k < n_ck; ++k) {
+        HIP_TRY(hipMemcpyAsync(ckp(k), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
+        const int64_t n0 = k * checkpoint_interval, n1 = std::min(n0 + checkpoint_interval, n_steps);
+        if ((st = run_stored(k, ckp(k), n0, n1, true)) != NDG_OK) return st;
+    }
     std::vector<double> hres((size_t)(R.n * n_steps));
     HIP_TRY(hipMemcpyAsync(hres.data(), recs.p, sizeof(double) * hres.size(), hipMemcpyDeviceToHost, s), "download records");
     HIP_TRY(hipStreamSynchronize(s), "adjoint sync");
@@ -1972,7 +1993,7 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     }
     for (int64_t k = n_ck - 1; k >= 0; --k) {
         const int64_t n0 = k * checkpoint_interval, n1 = std::min(n0 + checkpoint_interval, n_steps);
-        if (k != n_ck - 1 && (st = run_stored(ckp(k), n0, n1, false)) != NDG_OK) return st;
+        if (k < first_kept && (st = run_stored(k, ckp(k), n0, n1, false)) != NDG_OK) return st;
         for (int64_t j = n1 - 1; j >= n0; --j)
             for (int stage = 2; stage >= 0; --stage) {
                 AdjointArgs A;
