@@ -142,6 +142,59 @@ def batched_leg(args, nd, torch, stream):
                          "note": "all launches of one stage (mortar kernel + one stage kernel per order)"}}
 
 
+def adaptive_leg(nd, torch, stream, steps=1000, interval=20):
+    """hp-adaptive run on the mixed-order path: 256 x 256 walls, N = 4 initially, orders up to 12,
+    split level up to 1, the 8 seeded pulses, adaptation (estimate, split / raise, rebuild,
+    transfer) every 20 steps, 1000 steps. Stepping throughput (captured step graph, tuned shapes)
+    and the adaptation time are reported separately."""
+    pulses = batched_workload()
+    desc = nd.square_mesh(B_RES, jitter=0.0)
+    mesh = nd.Mesh2D_t(desc, 4, 12, 5, 1, 0, 0, 1e-6, 1e-14, 0.0, nd.NDG_t(12, 5))
+    mesh.set_stream(stream.cuda_stream)
+    mesh.initial_conditions(pulses)
+    mesh.set_graph("parallel")
+    solver = nd.Solver2D_t(CFL, [0.0])
+    t, step_ms, adapt_s, tune_s, dof_stages = 0.0, 0.0, 0.0, 0.0, 0
+    tuned, changed = set(), 0
+    for block in range(steps // interval):
+        if block:
+            w0 = time.perf_counter()
+            changed += mesh.adapt(12, 1)
+            mesh.synchronize()
+            adapt_s += time.perf_counter() - w0
+        orders = set(int(n) for n in np.unique(mesh.orders_))
+        if not orders <= tuned:
+            w0 = time.perf_counter()
+            mesh.autotune(3)
+            tune_s += time.perf_counter() - w0
+            tuned |= orders
+        mesh.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        t = solver.run_steps(mesh, interval, t)
+        stop.record(stream)
+        stop.synchronize()
+        step_ms += start.elapsed_time(stop)
+        dof_stages += mesh.n_dofs * 3 * interval
+        print(f"[adaptive] block {block}: K={mesh.n_elements_} dofs={mesh.n_dofs} t={t:.5f}", file=sys.stderr,
+              flush=True)
+    state = mesh.get_solution()
+    if not all(np.isfinite(a).all() for a in state):
+        raise SystemExit("bench.py: adaptive run blew up")
+    counts = {int(n): int((mesh.orders_ == n).sum()) for n in np.unique(mesh.orders_)}
+    out = {"value": dof_stages / (step_ms / 1e3), "unit": "DOF*stage/s",
+           "workload": f"{B_RES}x{B_RES} walls, N0=4, N<=12, max split level 1, 8 seeded pulses, tol_min 1e-6, "
+                       f"adapt every {interval} steps, {steps} steps, FP64",
+           "stepping_s": step_ms / 1e3, "adaptation_s": adapt_s, "autotune_s": tune_s,
+           "adapted_elements": changed, "final_elements": mesh.n_elements_, "final_dofs": mesh.n_dofs,
+           "final_elements_per_order": counts, "t_final": t,
+           "note": "stepping time from HIP events around the 20-step blocks (graph capture after each "
+                   "rebuild included); adaptation and tuning are host-synchronous wall times"}
+    mesh.close()
+    return out
+
+
 def timed_steps(solver, mesh, stream, torch, steps, t=0.0):
     """Runs `steps` RK3 steps on the mesh stream between two HIP events; returns ms."""
     mesh.synchronize()
@@ -245,6 +298,27 @@ def adjoint_leg(nd, torch, stream, steps=2000):
             "note": "wall times of the library calls (host-synchronous), forward = one recorded 2000-step run"}
 
 
+def copy_rate(torch, stream, gib=2, reps=10):
+    """Device-to-device copy rate (bytes read + written per second) of a plain torch copy of
+    `gib` GiB, timed with HIP events on the bench stream: the practical HBM rate on this GPU."""
+    n = (gib << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    with torch.cuda.stream(stream):
+        b.copy_(a)
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(reps):
+            b.copy_(a)
+        stop.record(stream)
+    stop.synchronize()
+    rate = 2 * n * 8 * reps / (start.elapsed_time(stop) / 1e3)
+    del a, b
+    torch.cuda.empty_cache()
+    return rate
+
+
 def separate_forms(args, legs, nd, torch, stream):
     """The form legs that run on workloads of their own (the FP32 leg reuses the main mesh)."""
     forms = {}
@@ -254,6 +328,8 @@ def separate_forms(args, legs, nd, torch, stream):
         forms["precision_1024"] = precision_leg(nd, torch, stream)
     if "adjoint" in legs:
         forms["adjoint_128"] = adjoint_leg(nd, torch, stream, args.adjoint_steps)
+    if "adaptive" in legs:
+        forms["adaptive_256"] = adaptive_leg(nd, torch, stream)
     return forms
 
 
@@ -265,7 +341,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="skip every form leg (same as --forms none)")
-    ap.add_argument("--forms", default="batched,precision,adjoint,fp32c",
+    ap.add_argument("--forms", default="batched,adaptive,precision,adjoint,fp32c",
                     help="comma-separated form legs to run after the main line (or 'none')")
     ap.add_argument("--forms-only", action="store_true",
                     help="profiler runs: skip the main line and print only the selected form legs")
@@ -340,6 +416,7 @@ def main():
     launch_s = ms / 1e3 / stages
     bytes_per_launch = algorithmic_bytes_per_element(N) * mesh.n_elements_
     achieved = bytes_per_launch / launch_s / 1e9
+    copy_gbs = copy_rate(torch, stream) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
@@ -380,7 +457,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "ndg::stage_kernel<8,false>", "avg_launch_ms": launch_s * 1e3,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "copy_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "forms": forms,
         "wall_s_timed_region": wall,
     }

@@ -1277,6 +1277,13 @@ extern "C" ndg_status ndg_mesh_tuning(const ndg_mesh *mesh, int32_t *orders, int
 
 namespace {
 
+// Launch shapes chosen by ndg_mesh_autotune stay with their order when the mesh is rebuilt.
+void carry_tuning(const DeviceMesh &from, DeviceMesh &to) {
+    for (Group &G : to.groups)
+        for (const Group &H : from.groups)
+            if (H.N == G.N) std::copy(H.tune, H.tune + 2, G.tune);
+}
+
 // quad_map (include/functions/quad_map.tpp:3-8)
 v2 quad_point(double xi, double eta, const v2 P[4]) {
     return v2{(P[0].x * (1 - xi) * (1 - eta) + P[1].x * (xi + 1) * (1 - eta) + P[2].x * (xi + 1) * (eta + 1) +
@@ -1524,6 +1531,7 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
     nd.clock = d.clock;
     d.clock = nullptr;
     nd.wave_speed = d.wave_speed;
+    carry_tuning(d, nd);
     free_device(d);
     d = nd;
     HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * d.n_q, mesh->stream), "hipMemset g");
@@ -1618,6 +1626,7 @@ extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split
     d.clock = nullptr;
     nd.wave_speed = d.wave_speed;
     nd.cur = 0;
+    carry_tuning(d, nd);
     free_device(d);
     d = nd;
     HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * d.n_q, mesh->stream), "hipMemset g");
