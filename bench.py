@@ -122,8 +122,9 @@ def batched_leg(args, nd, torch, stream):
         return start.elapsed_time(stop), tuning, mesh.get_solution()
 
     ms_direct, _, ref = run("off", False)
+    ms_seq, _, got_seq = run("sequential", True)
     ms_graph, tuning, got = run("parallel", True)
-    diff = max(float(np.abs(a - b).max()) for a, b in zip(got, ref))
+    diff = max(float(np.abs(a - b).max()) for a, b in zip(got + got_seq, ref + ref))
     orders = desc.quad_N
     nf = orders.astype(np.int64) + 1
     bytes_launch = int((88 * nf * nf + 192 * nf + 80).sum()) + int(mortar_bytes(orders, desc))
@@ -131,6 +132,7 @@ def batched_leg(args, nd, torch, stream):
     counts = {int(n): int((orders == n).sum()) for n in np.unique(orders)}
     return {"value": dofs * 3 * steps / (ms_graph / 1e3), "ms_per_step": ms_graph / steps,
             "value_direct_default_shapes": dofs * 3 * steps / (ms_direct / 1e3),
+            "value_sequential_graph_tuned": dofs * 3 * steps / (ms_seq / 1e3),
             "workload": f"{B_RES}x{B_RES} walls, 8 seeded pulses, orders 4..12 by distance to the pulses, "
                         f"FP64, {steps} steps",
             "elements_per_order": counts, "dofs": dofs,

@@ -952,8 +952,8 @@ __global__ void clock_end_kernel(double *clock, double dt) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Non-conforming (mortar) faces, one thread per face, FP64 arithmetic (the traces and the output are
-// of type T: double, or float in the FP32 form). Restates, for one face:
+// Non-conforming (mortar) faces, FP64 arithmetic (the traces and the output are of type T: double,
+// or float in the FP32 form). Restates, for one face:
 //   project_to_faces (src/meshes/Mesh2D_t.cpp:3803-3893): each side's trace is copied when the
 //     element order equals the face order, else interpolated barycentrically at
 //     xi = (x_face + 1) scale + 2 offset - 1 with the exact-node shortcut (right side reversed);
@@ -966,38 +966,64 @@ __device__ __forceinline__ bool ulp2_equal(double a, double b) {
     return fabs(a - b) <= DBL_EPSILON * fabs(a + b) * 2 || fabs(a - b) < DBL_MIN;
 }
 
-// One wave per face: lanes 0..31 handle the left side, lanes 32..63 the right side (face node i
-// for the interpolation, element node i for the projection); the face states and flux go through
-// LDS. The interpolation and projection matrices are built on the host with the reference's
-// exact-node shortcuts (see MortarFace).
+// Loads n <= 32 * kMaxItems values src[0..n) into dst through registers, every load in flight
+// before the first LDS write (lane i of a 32-lane half takes items i, i + 32, ...).
+template <typename T, int kMaxItems>
+__device__ __forceinline__ void stage_to_lds(const T *__restrict__ src, double *dst, int n, int i) {
+    double v[kMaxItems];
+#pragma unroll
+    for (int it = 0; it < kMaxItems; ++it) {
+        const int idx = i + 32 * it;
+        v[it] = idx < n ? (double)src[idx] : 0.0;
+    }
+#pragma unroll
+    for (int it = 0; it < kMaxItems; ++it) {
+        const int idx = i + 32 * it;
+        if (idx < n) dst[idx] = v[it];
+    }
+}
+
+// One wave per face: lanes 0..31 handle the left side, lanes 32..63 the right side. Each half first
+// stages its element trace and its interpolation and projection matrices (built on the host with
+// the reference's exact-node shortcuts, see MortarFace) into LDS with all loads in flight, then
+// lane i computes face node i (interpolation) and element node i (projection); face states and
+// flux go through LDS.
 template <typename T>
 __global__ void __launch_bounds__(64)
 mortar_kernel(const MortarFace *__restrict__ faces, const double *__restrict__ mats,
               const T *__restrict__ tr, T *__restrict__ side_flux) {
-    __shared__ double s_st[2][3][kMaxOrder + 1];  // face states, left and right
-    __shared__ double s_f[3][kMaxOrder + 1];      // face flux
+    constexpr int M = kMaxOrder + 1, kMat = M * M, kMatItems = (kMat + 31) / 32, kTrItems = (3 * M + 31) / 32;
+    __shared__ double s_tr[2][3 * M];   // element traces, left and right
+    __shared__ double s_mi[2][kMat];    // interpolation matrices [face node][element node]
+    __shared__ double s_mp[2][kMat];    // projection matrices [element node][face node]
+    __shared__ double s_st[2][3][M];    // face states, left and right
+    __shared__ double s_f[3][M];        // face flux
     const MortarFace fc = faces[blockIdx.x];
     const int lane = threadIdx.x;
     const int side = lane >> 5, i = lane & 31;
     const int Nf = fc.Nf;
     const int Ne = side ? fc.NR : fc.NL;
-    const T *te = tr + (side ? fc.trR : fc.trL);
+    const int64_t mi = fc.mat[side], mp = fc.mat[2 + side];
+    stage_to_lds<T, kTrItems>(tr + (side ? fc.trR : fc.trL), s_tr[side], 3 * (Ne + 1), i);
+    if (mi >= 0) stage_to_lds<double, kMatItems>(mats + mi, s_mi[side], (Nf + 1) * (Ne + 1), i);
+    if (mp >= 0) stage_to_lds<double, kMatItems>(mats + mp, s_mp[side], (Ne + 1) * (Nf + 1), i);
+    __syncthreads();
     // project_to_faces (src/meshes/Mesh2D_t.cpp:3803-3893)
     if (i <= Nf) {
-        const int64_t mi = fc.mat[side];
+        const double *te = s_tr[side];
         double v0 = 0.0, v1 = 0.0, v2 = 0.0;
         if (mi < 0) {
             const int k = side ? Nf - i : i;
-            v0 = (double)te[k];
-            v1 = (double)te[(Ne + 1) + k];
-            v2 = (double)te[2 * (Ne + 1) + k];
+            v0 = te[k];
+            v1 = te[(Ne + 1) + k];
+            v2 = te[2 * (Ne + 1) + k];
         } else {
-            const double *row = mats + mi + (int64_t)i * (Ne + 1);
+            const double *row = s_mi[side] + i * (Ne + 1);
             for (int j = 0; j <= Ne; ++j) {
                 const double w = row[j];
-                v0 += w * (double)te[j];
-                v1 += w * (double)te[(Ne + 1) + j];
-                v2 += w * (double)te[2 * (Ne + 1) + j];
+                v0 += w * te[j];
+                v1 += w * te[(Ne + 1) + j];
+                v2 += w * te[2 * (Ne + 1) + j];
             }
         }
         s_st[side][0][i] = v0;
@@ -1020,13 +1046,12 @@ mortar_kernel(const MortarFace *__restrict__ faces, const double *__restrict__ m
     if (slot < 0 || i > Ne) return;
     const double sf = side ? fc.sfR : fc.sfL;
     T *out = side_flux + slot;
-    const int64_t mp = fc.mat[2 + side];
     if (mp < 0) {
         for (int f = 0; f < 3; ++f)
             out[f * (Ne + 1) + i] = side ? (T)(-s_f[f][Nf - i] * sf) : (T)(s_f[f][i] * sf);
         return;
     }
-    const double *row = mats + mp + (int64_t)i * (Nf + 1);
+    const double *row = s_mp[side] + i * (Nf + 1);
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int j = 0; j <= Nf; ++j) {
         const double w = row[j];
