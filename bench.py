@@ -144,17 +144,17 @@ def batched_leg(args, nd, torch, stream):
                          "note": "all launches of one stage (mortar kernel + one stage kernel per order)"}}
 
 
-def adaptive_leg(nd, torch, stream, steps=1000, interval=20):
+def adaptive_leg(nd, torch, stream, steps=1000, interval=20, graph="off"):
     """hp-adaptive run on the mixed-order path: 256 x 256 walls, N = 4 initially, orders up to 12,
     split level up to 1, the 8 seeded pulses, adaptation (estimate, split / raise, rebuild,
-    transfer) every 20 steps, 1000 steps. Stepping throughput (captured step graph, tuned shapes)
-    and the adaptation time are reported separately."""
+    transfer) every 20 steps, 1000 steps. Stepping throughput (tuned shapes; direct launches, or
+    step graphs recaptured after each rebuild) and the adaptation time are reported separately."""
     pulses = batched_workload()
     desc = nd.square_mesh(B_RES, jitter=0.0)
     mesh = nd.Mesh2D_t(desc, 4, 12, 5, 1, 0, 0, 1e-6, 1e-14, 0.0, nd.NDG_t(12, 5))
     mesh.set_stream(stream.cuda_stream)
     mesh.initial_conditions(pulses)
-    mesh.set_graph("parallel")
+    mesh.set_graph(graph)
     solver = nd.Solver2D_t(CFL, [0.0])
     t, step_ms, adapt_s, tune_s, dof_stages = 0.0, 0.0, 0.0, 0.0, 0
     tuned, changed = set(), 0
@@ -191,8 +191,9 @@ def adaptive_leg(nd, torch, stream, steps=1000, interval=20):
            "stepping_s": step_ms / 1e3, "adaptation_s": adapt_s, "autotune_s": tune_s,
            "adapted_elements": changed, "final_elements": mesh.n_elements_, "final_dofs": mesh.n_dofs,
            "final_elements_per_order": counts, "t_final": t,
-           "note": "stepping time from HIP events around the 20-step blocks (graph capture after each "
-                   "rebuild included); adaptation and tuning are host-synchronous wall times"}
+           "launches": graph,
+           "note": "stepping time from HIP events around the 20-step blocks (with graphs, the capture after "
+                   "each rebuild included); adaptation and tuning are host-synchronous wall times"}
     mesh.close()
     return out
 
@@ -332,6 +333,7 @@ def separate_forms(args, legs, nd, torch, stream):
         forms["adjoint_128"] = adjoint_leg(nd, torch, stream, args.adjoint_steps)
     if "adaptive" in legs:
         forms["adaptive_256"] = adaptive_leg(nd, torch, stream)
+        forms["adaptive_256"]["value_parallel_graphs"] = adaptive_leg(nd, torch, stream, graph="parallel")["value"]
     return forms
 
 
