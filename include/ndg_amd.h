@@ -135,6 +135,17 @@ ndg_status ndg_square_mesh_sizes(const ndg_square_mesh_params *params, int64_t *
 ndg_status ndg_square_mesh_fill(const ndg_square_mesh_params *params, double *xy, int64_t *quads,
                                 int64_t *bars, int32_t *bar_type, int64_t *bar_donor);
 
+/* ---- mesh files: Mesh2D_t::read_su2 (src/meshes/Mesh2D_t.cpp:78-330), the .su2 branch of the
+ * Mesh2D_t constructor (:56-70). NDIME= 2, NPOIN= nodes, NELEM= QUAD_4 (type 9) elements, NMARK=
+ * BAR_2 (type 3) markers: farfield -> NDG_BC_OUTFLOW, wall or airfoil -> NDG_BC_WALL, inlet ->
+ * NDG_BC_INFLOW, symmetry -> NDG_BC_SYMMETRY. Errors (the reference's exits 69-75) return
+ * NDG_ERR_INVALID with the reason in ndg_last_error. CGNS files are not read (no CGNS library). ---- */
+ndg_status ndg_su2_sizes(const char *path, int64_t *n_nodes, int64_t *n_quads, int64_t *n_bars);
+/* Fills caller-allocated arrays of the sizes given by ndg_su2_sizes; the bars come in the
+ * reference's ghost order (walls, symmetry, inflow, outflow, each in file order). Any pointer may
+ * be NULL. */
+ndg_status ndg_su2_fill(const char *path, double *xy, int64_t *quads, int64_t *bars, int32_t *bar_type);
+
 /* ---- Mesh2D_t: include/meshes/Mesh2D_t.h:21 (constructor), :105-128 (phase methods) ---- */
 ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_tables *tables, ndg_mesh **out);
 ndg_status ndg_mesh_destroy(ndg_mesh *mesh);

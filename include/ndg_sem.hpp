@@ -16,8 +16,10 @@
 #define NDG_SEM_HPP
 
 #include <array>
+#include <cctype>
 #include <cstddef>
 #include <cstdint>
+#include <filesystem>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -126,6 +128,31 @@ inline MeshDescription square_mesh(const ndg_square_mesh_params &params) {
     return m;
 }
 
+// Mesh2D_t::read_su2 (src/meshes/Mesh2D_t.cpp:78-330): nodes, QUAD_4 elements and the
+// farfield / wall / inlet / symmetry markers of an SU2 file (bars in the reference's ghost order).
+inline MeshDescription read_su2(const std::filesystem::path &filename) {
+    const std::string path = filename.string();
+    int64_t nn = 0, nq = 0, nb = 0;
+    check(ndg_su2_sizes(path.c_str(), &nn, &nq, &nb));
+    MeshDescription m;
+    m.xy.resize(2 * nn);
+    m.quads.resize(4 * nq);
+    m.bars.resize(2 * nb);
+    m.bar_type.resize(nb);
+    m.bar_donor.assign(nb, -1);
+    check(ndg_su2_fill(path.c_str(), m.xy.data(), m.quads.data(), m.bars.data(), m.bar_type.data()));
+    return m;
+}
+
+// The file branch of the reference's constructor (src/meshes/Mesh2D_t.cpp:56-70): .su2 is read,
+// any other extension (.cgns included: no CGNS library here) is refused.
+inline MeshDescription read_mesh_file(const std::filesystem::path &filename) {
+    std::string ext = filename.extension().string();
+    for (char &c : ext) c = (char)std::tolower((unsigned char)c);
+    if (ext != ".su2") throw Error(NDG_ERR_UNSUPPORTED, "Mesh2D_t: extension '" + ext + "' not read (only .su2)");
+    return read_su2(filename);
+}
+
 class Mesh2D_t {
   public:
     template <typename Polynomial>
@@ -178,6 +205,15 @@ class Mesh2D_t {
         refresh();
         return raised;
     }
+    // Mesh2D_t(path filename, ...) (include/meshes/Mesh2D_t.h:21).
+    template <typename Polynomial>
+    Mesh2D_t(const std::filesystem::path &filename, int initial_N, int maximum_N, std::size_t n_interpolation_points,
+             int max_split_level, std::size_t adaptivity_interval, std::size_t load_balancing_interval,
+             double tolerance_min, double tolerance_max, double load_balancing_threshold,
+             const Entities::NDG_t<Polynomial> &NDG, void *stream = nullptr)
+        : Mesh2D_t(read_mesh_file(filename), initial_N, maximum_N, n_interpolation_points, max_split_level,
+                   adaptivity_interval, load_balancing_interval, tolerance_min, tolerance_max,
+                   load_balancing_threshold, NDG, stream) {}
     Mesh2D_t(const Mesh2D_t &) = delete;
     Mesh2D_t &operator=(const Mesh2D_t &) = delete;
     ~Mesh2D_t() { ndg_mesh_destroy(handle_); }

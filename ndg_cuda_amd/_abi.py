@@ -29,7 +29,8 @@ TABLE_NAMES = ["nodes", "weights", "barycentric_weights", "lagrange_interpolant_
 
 # Every symbol include/ndg_amd.h declares (checked by the CPU tests).
 EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tables_destroy",
-            "ndg_tables_get", "ndg_square_mesh_sizes", "ndg_square_mesh_fill", "ndg_mesh_create",
+            "ndg_tables_get", "ndg_square_mesh_sizes", "ndg_square_mesh_fill", "ndg_su2_sizes", "ndg_su2_fill",
+            "ndg_mesh_create",
             "ndg_mesh_destroy", "ndg_mesh_set_stream", "ndg_mesh_info", "ndg_mesh_set_precision", "ndg_set_ic",
             "ndg_set_state", "ndg_get_state", "ndg_get_solution", "ndg_get_traces", "ndg_compute_rhs",
             "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_run_steps", "ndg_synchronize",
@@ -88,6 +89,8 @@ def lib():
             "ndg_tables_get": [vp, i32, i32, vp, i64, vp],
             "ndg_square_mesh_sizes": [vp, vp, vp, vp],
             "ndg_square_mesh_fill": [vp, vp, vp, vp, vp, vp],
+            "ndg_su2_sizes": [ctypes.c_char_p, vp, vp, vp],
+            "ndg_su2_fill": [ctypes.c_char_p, vp, vp, vp, vp],
             "ndg_mesh_create": [vp, vp, vp],
             "ndg_mesh_destroy": [vp],
             "ndg_mesh_set_stream": [vp, vp],

@@ -14,6 +14,7 @@ memory); square_mesh() is the equivalent of the reference's mesh generator. Erro
 instead of exiting the process.
 """
 import ctypes
+import os
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -93,6 +94,32 @@ def square_mesh(resolution: int, x_min=-1.0, x_max=1.0, y_min=-1.0, y_max=1.0,
     return MeshDescription(xy, quads, bars, bar_type, bar_donor)
 
 
+def read_su2(path) -> MeshDescription:
+    """Mesh2D_t::read_su2 (src/meshes/Mesh2D_t.cpp:78-330): nodes, QUAD_4 elements and the
+    farfield / wall / inlet / symmetry markers of an SU2 file as a MeshDescription (bars in the
+    reference's ghost order: walls, symmetry, inflow, outflow)."""
+    p = os.fsencode(path)
+    nn, nq, nb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    check(lib().ndg_su2_sizes(p, ctypes.byref(nn), ctypes.byref(nq), ctypes.byref(nb)))
+    xy = np.empty((nn.value, 2), dtype=np.float64)
+    quads = np.empty((nq.value, 4), dtype=np.int64)
+    bars = np.empty((nb.value, 2), dtype=np.int64)
+    bar_type = np.empty(nb.value, dtype=np.int32)
+    check(lib().ndg_su2_fill(p, ptr(xy), ptr(quads), ptr(bars), ptr(bar_type)))
+    return MeshDescription(xy, quads, bars, bar_type, np.full(nb.value, -1, dtype=np.int64))
+
+
+def _mesh_from(mesh) -> MeshDescription:
+    """The constructor's mesh argument: a MeshDescription, or a file path like the reference's
+    (src/meshes/Mesh2D_t.cpp:56-70; .su2 only, there is no CGNS reader here)."""
+    if isinstance(mesh, MeshDescription):
+        return mesh
+    ext = os.path.splitext(os.fsdecode(mesh))[1].lower()
+    if ext == ".su2":
+        return read_su2(mesh)
+    raise NDGError(_abi.NDG_ERR_UNSUPPORTED, f"Mesh2D_t: mesh file extension '{ext}' is not read here (only .su2)")
+
+
 class Mesh2D_t:
     """Device-resident mesh and solution (one HIP stream)."""
 
@@ -109,6 +136,7 @@ class Mesh2D_t:
         self.tolerance_min_ = float(tolerance_min)
         self.tolerance_max_ = float(tolerance_max)
         self._ndg = NDG
+        mesh = _mesh_from(mesh)
         self._arrays = [_abi.as_f64(mesh.xy), _abi.as_i64(mesh.quads), _abi.as_i64(mesh.bars),
                         _abi.as_i32(mesh.bar_type), _abi.as_i64(mesh.bar_donor),
                         None if mesh.quad_N is None else _abi.as_i32(mesh.quad_N)]

@@ -1,18 +1,21 @@
 // Drives the C++ mirror of the reference host API (include/ndg_sem.hpp) the way the reference's
 // src/main.cpp does: tables, mesh, initial conditions, solver, solve, solution. Writes the final
-// packed p, u, v as raw doubles to argv[1] and prints the step count and final time.
-//   host_api_main <out.bin> <resolution> <N> <t_end>
-// The mesh is the square generator's [-1,1]^2, periodic in x and y, with one Gaussian pulse.
+// packed p, u, v as raw doubles to argv[1], the output-point x, y, p, u, v to argv[1].out, and
+// prints the step count and final time.
+//   host_api_main <out.bin> <resolution> <N> <t_end> [mesh.su2]
+// The mesh is the square generator's [-1,1]^2, periodic in x and y, or the given SU2 file (the
+// reference's Mesh2D_t(path, ...) constructor), with one Gaussian pulse.
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #include "ndg_sem.hpp"
 
 using namespace SEM::Device;
 
 int main(int argc, char **argv) {
-    if (argc != 5) {
-        std::fprintf(stderr, "usage: %s out.bin resolution N t_end\n", argv[0]);
+    if (argc != 5 && argc != 6) {
+        std::fprintf(stderr, "usage: %s out.bin resolution N t_end [mesh.su2]\n", argv[0]);
         return 2;
     }
     const int res = std::atoi(argv[2]), N = std::atoi(argv[3]);
@@ -25,7 +28,9 @@ int main(int argc, char **argv) {
         for (int k = 0; k < 4; ++k) params.bc[k] = NDG_BC_WALL;
         params.x_periodic = 1;
         params.y_periodic = 1;
-        Meshes::Mesh2D_t mesh(Meshes::square_mesh(params), N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, NDG);
+        Meshes::Mesh2D_t mesh = argc == 6
+            ? Meshes::Mesh2D_t(std::filesystem::path(argv[5]), N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, NDG)
+            : Meshes::Mesh2D_t(Meshes::square_mesh(params), N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, NDG);
         mesh.initial_conditions({{0.1, -0.2, 0.2, 1.0}});
         Solvers::Solver2D_t solver(0.5, {t_end}, 0.0);
         solver.solve(NDG, mesh);
@@ -37,6 +42,15 @@ int main(int argc, char **argv) {
         std::fwrite(u.data(), sizeof(double), u.size(), f);
         std::fwrite(v.data(), sizeof(double), v.size(), f);
         std::fclose(f);
+        // Mesh2D_t::get_solution: the output-point values, written after the state as x, y, p, u, v
+        mesh.get_solution(NDG);
+        const std::string out2 = std::string(argv[1]) + ".out";
+        FILE *g = std::fopen(out2.c_str(), "wb");
+        if (!g) return 3;
+        for (const std::vector<double> *a : {&mesh.x_output_, &mesh.y_output_, &mesh.p_output_, &mesh.u_output_,
+                                             &mesh.v_output_})
+            std::fwrite(a->data(), sizeof(double), a->size(), g);
+        std::fclose(g);
         std::printf("steps %lld time %.17g dofs %zu\n", (long long)solver.n_steps_, solver.time_, 3 * p.size());
     } catch (const Error &e) {
         std::fprintf(stderr, "error %d: %s\n", e.code(), e.what());

@@ -53,3 +53,25 @@ def interior_elements(desc):
     """Elements with no node on the boundary bars."""
     bar_nodes = set(np.asarray(desc.bars).ravel().tolist())
     return [e for e, q in enumerate(np.asarray(desc.quads)) if not bar_nodes.intersection(q.tolist())]
+
+
+SU2_TAGS = {nd.BC_WALL: "wall", nd.BC_SYMMETRY: "symmetry", nd.BC_INFLOW: "inlet", nd.BC_OUTFLOW: "farfield"}
+
+
+def write_su2(desc, path, tags=None):
+    """Writes a description without periodic bars as an SU2 file (one marker per boundary kind,
+    in the order the kinds first appear), the format the reference's Mesh2D_t::read_su2 reads."""
+    tags = tags or SU2_TAGS
+    bar_type = np.asarray(desc.bar_type)
+    lines = ["NDIME= 2", f"NPOIN= {len(desc.xy)}"]
+    lines += [f"{x!r} {y!r} {i}" for i, (x, y) in enumerate(np.asarray(desc.xy).tolist())]
+    lines.append(f"NELEM= {len(desc.quads)}")
+    lines += ["9 " + " ".join(str(int(v)) for v in q) + f" {i}" for i, q in enumerate(np.asarray(desc.quads))]
+    kinds = list(dict.fromkeys(int(t) for t in bar_type))
+    lines.append(f"NMARK= {len(kinds)}")
+    for k in kinds:
+        sel = np.nonzero(bar_type == k)[0]
+        lines += [f"MARKER_TAG= {tags[k]}", f"MARKER_ELEMS= {len(sel)}"]
+        lines += [f"3 {int(desc.bars[b][0])} {int(desc.bars[b][1])}" for b in sel]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
