@@ -184,15 +184,33 @@ ndg_status ndg_step(ndg_mesh *mesh, double t, double dt);
  * as the reference does, and the mesh's adaptivity (ndg_mesh_set_adaptivity). Waits for
  * completion. */
 ndg_status ndg_solve(ndg_mesh *mesh, double cfl, double t_end, double *t_final, int64_t *n_steps);
+/* Solver2D_t::solve with its output events (include/solvers/Solver2D_t.tpp:21-29, :74-83,
+ * :112-127): runs to output_times[n_times - 1] as ndg_solve does and calls fn(mesh, time, user)
+ * where the reference calls write_complete_data: before the first step for every output time t_o
+ * with 0 in [t_o, t_o + dt); after a step for the first t_o with time in [t_o, t_o + dt); at the
+ * end if the last step wrote nothing. Output times do not shorten steps. The callback may read
+ * the mesh (ndg_get_state, ndg_get_solution, ndg_estimate_error, ...); a status other than NDG_OK
+ * from it stops the solve with that status. No reference counterpart for the callback itself: the
+ * reference writes VTK files there (DataWriter_t). */
+typedef ndg_status (*ndg_output_fn)(ndg_mesh *mesh, double time, void *user);
+ndg_status ndg_solve_output(ndg_mesh *mesh, double cfl, const double *output_times, int64_t n_times,
+                            ndg_output_fn fn, void *user, double *t_final, int64_t *n_steps);
 /* n_steps steps of the same loop without clipping, from *time_io (updated). Stream-ordered. */
 ndg_status ndg_run_steps(ndg_mesh *mesh, double cfl, int64_t n_steps, double *time_io);
-/* Mesh2D_t::adapt without splitting (src/meshes/Mesh2D_t.cpp:1644-1706, p_adapt :5026-5049):
- * estimates the error, then raises by 2 the order of every element with refine && sigma >= 1 &&
- * N + 2 <= N_max (Element2D_t.cpp:802-812), interpolating its state to the new nodes
- * (Element2D_t.cpp:464-579); ghosts and faces follow (adjust_boundaries / adjust_interfaces /
- * adjust_faces, :7482-7538), including the reference's zero edge scaling factor of a resized
- * ghost. Returns NDG_ERR_UNSUPPORTED (and changes nothing) when an element would be split
- * (refine && sigma < 1 && max_split_level > 0). *n_raised (may be NULL) gets the count. */
+/* Mesh2D_t::adapt (src/meshes/Mesh2D_t.cpp:1644-2268): estimates the error, then
+ * - when some element would h-refine (refine && sigma < 1 && split level < max_split_level,
+ *   Element2D_t.cpp:802-812), the hp_adapt branch (:5186-5659): each such element becomes four
+ *   children in Hilbert order (child order and statuses, Hilbert_splitting.cpp:23-81) carrying the
+ *   parent polynomial (inverse bilinear map + interpolate_from), ghosts on split sides become two
+ *   (split_boundaries, :6232-6343), the would-p-refine elements are raised by 2, and 2:1 hanging
+ *   faces appear; splitting an element with a periodic side, or a small hanging element whose
+ *   larger neighbour does not split, returns NDG_ERR_UNSUPPORTED and changes nothing;
+ * - otherwise the p_adapt branch (:5026-5049): every element with refine && sigma >= 1 &&
+ *   N + 2 <= N_max is raised by 2 (Element2D_t.cpp:464-579); ghosts and faces follow
+ *   (adjust_boundaries / adjust_interfaces / adjust_faces, :7482-7538), including the reference's
+ *   zero edge scaling factor of a resized ghost.
+ * The element count can change (ndg_mesh_info). *n_raised (may be NULL) gets the number of split
+ * plus raised elements. */
 ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split_level, int64_t *n_raised);
 /* The order of every element (caller's order); it changes with ndg_adapt. */
 ndg_status ndg_mesh_orders(const ndg_mesh *mesh, int32_t *orders);

@@ -19,6 +19,7 @@
 #include <cctype>
 #include <cstddef>
 #include <cstdint>
+#include <exception>
 #include <filesystem>
 #include <stdexcept>
 #include <string>
@@ -314,6 +315,37 @@ class Solver2D_t {
         (void)NDG;
         if (output_times_.empty()) throw Error(NDG_ERR_INVALID, "Solver2D_t::solve: no output times");
         check(ndg_solve(mesh.handle(), CFL_, output_times_.back(), &time_, &n_steps_));
+        mesh.refresh();
+    }
+
+    // The same with the reference's output events (include/solvers/Solver2D_t.tpp:21-29, :74-83,
+    // :112-127): data_writer(time, mesh) is called where the reference calls
+    // write_complete_data, and may read the mesh. An exception it throws stops the solve and is
+    // rethrown here.
+    template <typename Polynomial, typename Writer>
+    void solve(const Entities::NDG_t<Polynomial> &NDG, Meshes::Mesh2D_t &mesh, Writer &&data_writer) {
+        (void)NDG;
+        if (output_times_.empty()) throw Error(NDG_ERR_INVALID, "Solver2D_t::solve: no output times");
+        struct Context {
+            Meshes::Mesh2D_t *mesh;
+            Writer *writer;
+            std::exception_ptr error;
+        } ctx{&mesh, &data_writer, nullptr};
+        auto event = [](ndg_mesh *, double time, void *user) -> ndg_status {
+            Context &c = *static_cast<Context *>(user);
+            try {
+                c.mesh->refresh();
+                (*c.writer)(time, *c.mesh);
+                return NDG_OK;
+            } catch (...) {
+                c.error = std::current_exception();
+                return NDG_ERR_INVALID;
+            }
+        };
+        const ndg_status st = ndg_solve_output(mesh.handle(), CFL_, output_times_.data(), (int64_t)output_times_.size(),
+                                               event, &ctx, &time_, &n_steps_);
+        if (ctx.error) std::rethrow_exception(ctx.error);
+        check(st);
         mesh.refresh();
     }
 

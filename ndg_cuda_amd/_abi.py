@@ -21,6 +21,9 @@ IC_PLANE_WAVE, IC_PULSES = 0, 1
 PRECISION_FP64, PRECISION_FP32_COMPENSATED = 0, 1
 GRAPH_OFF, GRAPH_SEQUENTIAL, GRAPH_PARALLEL = 0, 1, 2
 
+# ndg_output_fn: status (mesh, time, user)
+OUTPUT_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p)
+
 TABLE_NAMES = ["nodes", "weights", "barycentric_weights", "lagrange_interpolant_left",
                "lagrange_interpolant_right", "lagrange_interpolant_derivative_left",
                "lagrange_interpolant_derivative_right", "derivative_matrices",
@@ -33,7 +36,7 @@ EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tabl
             "ndg_mesh_create",
             "ndg_mesh_destroy", "ndg_mesh_set_stream", "ndg_mesh_info", "ndg_mesh_set_precision", "ndg_set_ic",
             "ndg_set_state", "ndg_get_state", "ndg_get_solution", "ndg_get_traces", "ndg_compute_rhs",
-            "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_run_steps", "ndg_synchronize",
+            "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_solve_output", "ndg_run_steps", "ndg_synchronize",
             "ndg_estimate_error", "ndg_mesh_set_graph", "ndg_mesh_autotune", "ndg_mesh_tuning",
             "ndg_mesh_set_wave_speed", "ndg_record_receivers", "ndg_adjoint_set_store_budget", "ndg_adjoint_gradient", "ndg_apply_rhs_transpose",
             "ndg_adapt", "ndg_mesh_set_adaptivity", "ndg_mesh_orders"]
@@ -105,6 +108,7 @@ def lib():
             "ndg_get_delta_t": [vp, f64, vp],
             "ndg_step": [vp, f64, f64],
             "ndg_solve": [vp, f64, f64, vp, vp],
+            "ndg_solve_output": [vp, f64, vp, i64, vp, vp, vp, vp],
             "ndg_run_steps": [vp, f64, i64, vp],
             "ndg_synchronize": [vp],
             "ndg_estimate_error": [vp, vp, vp, vp, vp],

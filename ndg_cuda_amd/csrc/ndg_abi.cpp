@@ -1651,17 +1651,32 @@ extern "C" ndg_status ndg_mesh_set_adaptivity(ndg_mesh *mesh, int64_t adaptivity
     return NDG_OK;
 }
 
-extern "C" ndg_status ndg_solve(ndg_mesh *mesh, double cfl, double t_end, double *t_final, int64_t *n_steps) {
-    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_solve: null mesh");
+namespace {
+
+// Solver2D_t::solve (include/solvers/Solver2D_t.tpp:7-135): steps of the CFL size up to the last
+// output time (the last step clipped to it), the output events of the reference (before the first
+// step, every output time t_o in [0, dt); after a step, the first t_o with time in [t_o, t_o + dt);
+// at the end, once more if the last step wrote nothing), and adaptation every adapt_interval steps
+// with the tables' N_max, after which dt follows the new orders.
+ndg_status solve_impl(ndg_mesh *mesh, double cfl, const double *times, int64_t n_times, ndg_output_fn fn,
+                      void *user, double *t_final, int64_t *n_steps) {
+    const double t_end = times[n_times - 1];
     double dt0 = 0.0;
     ndg_status st = ndg_get_delta_t(mesh, cfl, &dt0);
     if (st != NDG_OK) return st;
-    double time = 0.0;
+    double time = 0.0, dt = dt0;
     int64_t steps = 0;
+    auto output = [&]() -> ndg_status {
+        const ndg_status r = fn(mesh, time, user);
+        return r == NDG_OK ? NDG_OK : fail(r, "ndg_solve_output: the output callback returned an error");
+    };
+    if (fn)
+        for (int64_t k = 0; k < n_times; ++k)
+            if (time >= times[k] && time < times[k] + dt && (st = output()) != NDG_OK) return st;
     HIP_TRY(launch_clock_begin(mesh->d, time, 1, dt0, mesh->stream), "clock kernel");
     while (time < t_end) {
         ++steps;
-        double dt = dt0;
+        dt = dt0;
         if (time + dt > t_end) {
             dt = t_end - time;
             if ((st = ndg_step(mesh, time, dt)) != NDG_OK) return st;
@@ -1669,6 +1684,12 @@ extern "C" ndg_status ndg_solve(ndg_mesh *mesh, double cfl, double t_end, double
             return st;
         }
         time += dt;
+        if (fn)
+            for (int64_t k = 0; k < n_times; ++k)
+                if (time >= times[k] && time < times[k] + dt) {
+                    if ((st = output()) != NDG_OK) return st;
+                    break;
+                }
         // include/solvers/Solver2D_t.tpp:88-96: estimate_error + adapt every adaptivity_interval
         // steps, with the tables' N_max; Δt then follows the new orders.
         if (mesh->adapt_interval > 0 && steps % mesh->adapt_interval == 0) {
@@ -1680,10 +1701,28 @@ extern "C" ndg_status ndg_solve(ndg_mesh *mesh, double cfl, double t_end, double
             }
         }
     }
+    if (fn) {
+        bool wrote = false;
+        for (int64_t k = 0; k < n_times && !wrote; ++k) wrote = time >= times[k] && time < times[k] + dt;
+        if (!wrote && (st = output()) != NDG_OK) return st;
+    }
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_solve sync");
     if (t_final) *t_final = time;
     if (n_steps) *n_steps = steps;
     return NDG_OK;
+}
+
+}  // namespace
+
+extern "C" ndg_status ndg_solve(ndg_mesh *mesh, double cfl, double t_end, double *t_final, int64_t *n_steps) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_solve: null mesh");
+    return solve_impl(mesh, cfl, &t_end, 1, nullptr, nullptr, t_final, n_steps);
+}
+
+extern "C" ndg_status ndg_solve_output(ndg_mesh *mesh, double cfl, const double *output_times, int64_t n_times,
+                                       ndg_output_fn fn, void *user, double *t_final, int64_t *n_steps) {
+    if (!mesh || !output_times || n_times < 1) return fail(NDG_ERR_INVALID, "ndg_solve_output: null mesh or no output times");
+    return solve_impl(mesh, cfl, output_times, n_times, fn, user, t_final, n_steps);
 }
 
 extern "C" ndg_status ndg_run_steps(ndg_mesh *mesh, double cfl, int64_t n_steps, double *time_io) {

@@ -303,12 +303,36 @@ class Solver2D_t:
         check(lib().ndg_get_delta_t(mesh.handle, self.CFL_, ctypes.byref(dt)))
         return dt.value
 
-    def solve(self, NDG: NDG_t, mesh: Mesh2D_t):
+    def solve(self, NDG: NDG_t, mesh: Mesh2D_t, data_writer=None):
         """Advances to output_times[-1] with the reference's step clipping; returns the number of
-        steps taken."""
+        steps taken. data_writer, if given, is called as data_writer(time, mesh) at the reference's
+        output events (include/solvers/Solver2D_t.tpp:21-29, :74-83, :112-127), where the reference
+        writes its VTK files; it may read the mesh (get_solution, get_interpolated_solution,
+        estimate_error)."""
         t_final, steps = ctypes.c_double(), ctypes.c_int64()
-        check(lib().ndg_solve(mesh.handle, self.CFL_, self.output_times_[-1], ctypes.byref(t_final),
-                              ctypes.byref(steps)))
+        if data_writer is None:
+            check(lib().ndg_solve(mesh.handle, self.CFL_, self.output_times_[-1], ctypes.byref(t_final),
+                                  ctypes.byref(steps)))
+        else:
+            times = np.asarray(self.output_times_, dtype=np.float64)
+            raised = []
+
+            def event(_mesh, time, _user):
+                try:
+                    mesh._refresh()
+                    data_writer(time, mesh)
+                    return _abi.NDG_OK
+                except BaseException as exc:  # re-raised after the library call returns
+                    raised.append(exc)
+                    return _abi.NDG_ERR_INVALID
+
+            cb = _abi.OUTPUT_FN(event)
+            status = lib().ndg_solve_output(mesh.handle, self.CFL_, ptr(times), times.size,
+                                            ctypes.cast(cb, ctypes.c_void_p), None, ctypes.byref(t_final),
+                                            ctypes.byref(steps))
+            if raised:
+                raise raised[0]
+            check(status)
         self.time_, self.n_steps_ = t_final.value, steps.value
         mesh._refresh()
         return steps.value

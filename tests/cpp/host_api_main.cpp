@@ -32,8 +32,13 @@ int main(int argc, char **argv) {
             ? Meshes::Mesh2D_t(std::filesystem::path(argv[5]), N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, NDG)
             : Meshes::Mesh2D_t(Meshes::square_mesh(params), N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, NDG);
         mesh.initial_conditions({{0.1, -0.2, 0.2, 1.0}});
-        Solvers::Solver2D_t solver(0.5, {t_end}, 0.0);
-        solver.solve(NDG, mesh);
+        // output events at t_end / 2 and t_end (the reference's write_complete_data calls)
+        Solvers::Solver2D_t solver(0.5, {t_end / 2, t_end}, 0.0);
+        std::vector<double> events;
+        solver.solve(NDG, mesh, [&](double time, Meshes::Mesh2D_t &) { events.push_back(time); });
+        std::printf("events");
+        for (double e : events) std::printf(" %.17g", e);
+        std::printf("\n");
         std::vector<double> p, u, v;
         mesh.get_solution(p, u, v);
         FILE *f = std::fopen(argv[1], "wb");

@@ -30,7 +30,10 @@ def test_driver_matches_oracle(tmp_path):
     r = subprocess.run([DRIVER, str(tmp_path / "o.bin"), str(res), str(N), str(t_end)], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    steps = int(r.stdout.split()[1])
+    lines = r.stdout.split("\n")
+    events = [float(x) for x in lines[0].split()[1:]]
+    steps = int(lines[1].split()[1])
+    assert len(events) == 2 and events[1] == t_end and t_end / 2 <= events[0] < t_end / 2 + 0.0018
     got = np.fromfile(tmp_path / "o.bin", dtype=np.float64).reshape(3, -1)
     desc = nd.square_mesh(res, x_periodic=True, y_periodic=True)
     om = O.Mesh(O.Tables(N, 5), desc, N)
@@ -57,7 +60,7 @@ def test_driver_reads_su2_file(tmp_path):
     r = subprocess.run([DRIVER, str(tmp_path / "o.bin"), str(res), str(N), str(t_end), str(path)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    steps = int(r.stdout.split()[1])
+    steps = int(r.stdout.split("\n")[1].split()[1])
     got = np.fromfile(tmp_path / "o.bin", dtype=np.float64).reshape(3, -1)
     om = O.Mesh(O.Tables(N, 5), nd.read_su2(path), N)
     om.initial_conditions([(0.1, -0.2, 0.2, 1.0)])
