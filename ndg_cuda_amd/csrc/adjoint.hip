@@ -29,6 +29,12 @@
 // adjoints (lambda, mu) of (q', g'):  mu~ = mu + gs lambda;  lambda <- lambda + G^T mu~;
 // mu <- a_s mu~.  The c-derivative of the misfit gathers <mu~, dG/dc q_s> = <G^T mu~, q_s> / c
 // over every stage (G is linear in c), from the stage input state q_s.
+//
+// The kernel keeps lambda and mu divided by the Jacobian J at their node (lambda / J, mu / J): the
+// RK combinations are the same in these variables, nu = -c mu~ / J is then -c times the stored
+// value, and only the new contribution G^T mu~ is divided by J, once per node and field. The
+// driver scales the receivers' misfit source and the final gradient accordingly
+// (jacobian_scale_kernel).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -46,7 +52,7 @@ constexpr int kAdjApply = 8;    // do the transpose (off: export only)
 constexpr int kAdjSource = 16;  // add the receivers' misfit source to lambda' before the export
 
 template <int N>
-__global__ void __launch_bounds__(192)
+__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(N <= 10 ? 3 : 2)))
 adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_nb,
                      const double *__restrict__ tab, double *__restrict__ lam, double *__restrict__ mu,
                      const double *__restrict__ chi_in, double *__restrict__ chi_out,
@@ -137,15 +143,9 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
     }
     __syncthreads();
 
-    // -c / J at node (a, b) of element e (J from the metric lines, as the forward stage).
-    auto scale_at = [&](int e, int a, int b) {
-        const double *met = s_met + e * 4 * NF;
-        return -cw * recip(met[2 * NF + b] * met[NF + a] - met[a] * met[3 * NF + b]);
-    };
-
     if (apply) {
-        // P2: mu~ = mu + gs lambda (to s_n; scaled to nu = -c mu~ / J where it is read);
-        // mu' = a_s mu~ (stored). Lanes past the last element hold zeros and store nothing.
+        // P2: mu~ = mu + gs lambda, nu = -c mu~ (J-scaled variables) to s_n; mu' = a_s mu~
+        // (stored). Lanes past the last element hold zeros and store nothing.
         double *mo = mu + e0 * 3 * NE;
 #pragma unroll
         for (int it = 0; it < QPT; ++it) {
@@ -154,7 +154,7 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
             mreg[it] = as * mt;
             if (r < nel * 3 * NE) {
                 if (flags & kAdjMuOut) mo[r] = mreg[it];
-                s_n[r] = mt;
+                s_n[r] = -cw * mt;
             }
         }
         __syncthreads();
@@ -168,8 +168,8 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
             double col[NF];
 #pragma unroll
             for (int m = 0; m < NF; ++m) {
-                col[m] = nb[m * NF + l] * scale_at(el, m, l);
-                row[m] = nb[l * NF + m] * scale_at(el, l, m);
+                col[m] = nb[m * NF + l];
+                row[m] = nb[l * NF + m];
             }
             double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
 #pragma unroll
@@ -228,6 +228,8 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         // the fields that f's fluxes depend on, plus the trace transposes:
         //   p: y_eta(i) Fbar_u - x_eta(i) Fbar_v - y_xi(b) Gbar_u + x_xi(b) Gbar_v,
         //   u: y_eta(i) Fbar_p - y_xi(b) Gbar_p,   v: -x_eta(i) Fbar_p + x_xi(b) Gbar_p.
+        // After the barrier d lambda goes to s_n (for the c-derivative term) and d lambda / J to
+        // s_a (J-scaled variables).
         double da[NF];
         if (active) {
             const int i = l;
@@ -252,13 +254,19 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         }
         __syncthreads();
         if (active) {
-            double *dst = s_a + el * 3 * NE + f * NE + l * NF;
+            const double *met = s_met + el * 4 * NF;
+            const double ye = met[NF + l], xe = met[l];
+            double *dst = s_a + el * 3 * NE + f * NE + l * NF, *raw = s_n + el * 3 * NE + f * NE + l * NF;
 #pragma unroll
-            for (int b = 0; b < NF; ++b) dst[b] = da[b];
+            for (int b = 0; b < NF; ++b) {
+                raw[b] = da[b];
+                dst[b] = da[b] * recip(met[2 * NF + b] * ye - xe * met[3 * NF + b]);
+            }
         }
         __syncthreads();
 
-        // P6 (coalesced): lambda' = lambda + d lambda, and the c-derivative term <d lambda, q_s>.
+        // P6 (coalesced): lambda' = lambda + d lambda / J, and the c-derivative term
+        // <d lambda, q_s>.
         double dot = 0.0;
         const double *qg = qs ? qs + e0 * 3 * NE : nullptr;
 #pragma unroll
@@ -266,9 +274,9 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
             const int r = tid + it * T;
             const bool in = r < nel * 3 * NE;
             const int rr = in ? r : 0;
-            const double dl = in ? s_a[rr] : 0.0;
-            if (qg) dot += dl * qg[rr];
-            lreg[it] += dl;
+            const double dl = s_a[rr];
+            if (qg) dot += (in ? s_n[rr] : 0.0) * qg[rr];
+            lreg[it] += in ? dl : 0.0;
             if (in) s_a[r] = lreg[it];
         }
         if (cpart) {
@@ -277,8 +285,8 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         }
         __syncthreads();
         if (cpart && tid == 0) cpart[blockIdx.x] += (s_red[0] + s_red[1]) + s_red[2];
-        // misfit source of the step end this stage reaches: lambda_p += res_r w_r (receivers in
-        // order, each node always handled by the same thread)
+        // misfit source of the step end this stage reaches: lambda_p += res_r w_r / J (the weights
+        // come divided by J; receivers in order, each node always handled by the same thread)
         if (flags & kAdjSource) {
             for (int r = 0; r < n_rec; ++r) {
                 const int64_t le = rec_elem[r] - e0;
@@ -306,11 +314,11 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
     }
     if (!(flags & kAdjExport)) return;
 
-    // P7: mu~ of the next reverse stage, mu' + gnext lambda' (scaled to nu where it is read).
+    // P7: nu of the next reverse stage, -c (mu' + gnext lambda') (J-scaled variables).
 #pragma unroll
     for (int it = 0; it < QPT; ++it) {
         const int r = tid + it * T;
-        const double v = mreg[it] + gnext * lreg[it];
+        const double v = -cw * (mreg[it] + gnext * lreg[it]);
         if (r < nel * 3 * NE) s_n[r] = v;
     }
     __syncthreads();
@@ -320,7 +328,7 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
 #pragma unroll
         for (int m = 0; m < NF; ++m) {
-            const double col = nf_[m * NF + l] * scale_at(el, m, l), row = nf_[l * NF + m] * scale_at(el, l, m);
+            const double col = nf_[m * NF + l], row = nf_[l * NF + m];
             r1 += lp[m] * winv[m] * col;
             r3 += lm[m] * winv[m] * col;
             r0 += lm[m] * winv[m] * row;
@@ -353,7 +361,8 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
 
 // Receivers: p at point r = sum over the element's nodes of w_r[node] p[node] (one wave per
 // receiver), and the misfit source lambda_p += res_r w_r (one workgroup, receivers in order so
-// that receivers sharing an element add without races).
+// that receivers sharing an element add without races; w_r divided by J for the J-scaled
+// adjoint).
 __global__ void __launch_bounds__(64)
 record_kernel(const double *__restrict__ q, const int64_t *__restrict__ elem, const double *__restrict__ w,
               int ne, double *__restrict__ out) {
@@ -378,6 +387,41 @@ source_kernel(double *__restrict__ lam, const int64_t *__restrict__ elem, const 
     }
 }
 
+// x[e][f][i*(N+1)+j] *= J(i, j)^power (power = 1 or -1) for every element of the single order
+// group, J from the metric lines exactly as in the stage kernels; with rec (n_rec receivers, their
+// device elements), scales the receiver weight arrays [n_rec][(N+1)^2] instead (one field).
+template <int N>
+__global__ void __launch_bounds__(256)
+jacobian_scale_kernel(const double *__restrict__ verts, const double *__restrict__ tab, double *__restrict__ x,
+                      int64_t n_blocks, const int64_t *__restrict__ rec_elem, int power) {
+    constexpr int NF = N + 1, NE = NF * NF;
+    const double *xn = tab + NF * NF + 3 * NF;  // nodes (table layout: Dhat, l(-1), l(+1), w, x, ...)
+    const int64_t blk = blockIdx.x;
+    if (blk >= n_blocks) return;
+    const int64_t e = rec_elem ? rec_elem[blk] : blk;
+    const int nf = rec_elem ? 1 : 3;
+    const double *P = verts + 8 * e;
+    double *xe = x + blk * nf * NE;
+    for (int r = threadIdx.x; r < nf * NE; r += 256) {
+        const int node = r % NE, i = node / NF, j = node % NF;
+        const double xi = xn[i], eta = xn[j];
+        const double xeta = ((1 - xi) * (P[6] - P[0]) + (1 + xi) * (P[4] - P[2])) / 4;
+        const double yeta = ((1 - xi) * (P[7] - P[1]) + (1 + xi) * (P[5] - P[3])) / 4;
+        const double xxi = ((1 - eta) * (P[2] - P[0]) + (1 + eta) * (P[4] - P[6])) / 4;
+        const double yxi = ((1 - eta) * (P[3] - P[1]) + (1 + eta) * (P[5] - P[7])) / 4;
+        const double jac = xxi * yeta - xeta * yxi;
+        xe[r] = power > 0 ? xe[r] * jac : xe[r] * recip(jac);
+    }
+}
+
+template <int N>
+static hipError_t jacobian_scale_n(const DeviceMesh &m, double *x, int64_t n, const int64_t *rec_elem, int power,
+                                   hipStream_t s) {
+    hipLaunchKernelGGL(jacobian_scale_kernel<N>, dim3((unsigned)n), dim3(256), 0, s, m.verts, m.groups[0].tab, x, n,
+                       rec_elem, power);
+    return hipGetLastError();
+}
+
 template <int N>
 static hipError_t adjoint_n(DeviceMesh &m, const AdjointArgs &A, hipStream_t s) {
     using S = LineShape<N>;
@@ -400,6 +444,15 @@ static hipError_t adjoint_n(DeviceMesh &m, const AdjointArgs &A, hipStream_t s) 
 hipError_t launch_adjoint_stage(DeviceMesh &m, const AdjointArgs &A, hipStream_t s) {
     if (m.groups.size() != 1) return hipErrorInvalidValue;
     switch (m.groups[0].N) { ADJ_CASES(adjoint_n, m, A, s) }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_jacobian_scale(const DeviceMesh &m, double *x, const int64_t *rec_elem, int64_t n_rec, int power,
+                                 hipStream_t s) {
+    if (m.groups.size() != 1) return hipErrorInvalidValue;
+    const int64_t n = rec_elem ? n_rec : m.groups[0].K;
+    if (n <= 0) return hipSuccess;
+    switch (m.groups[0].N) { ADJ_CASES(jacobian_scale_n, m, x, n, rec_elem, power, s) }
     return hipErrorInvalidValue;
 }
 

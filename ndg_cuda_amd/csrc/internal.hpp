@@ -189,6 +189,10 @@ struct AdjointArgs {
 };
 hipError_t launch_adjoint_stage(DeviceMesh &m, const AdjointArgs &A, hipStream_t s);
 int64_t adjoint_blocks(const DeviceMesh &m);
+// x *= J^power per node (power 1 or -1), single-order meshes: the packed state layout, or with
+// rec_elem the receiver weight arrays (n_rec of (N+1)^2, element rec_elem[r] in device order).
+hipError_t launch_jacobian_scale(const DeviceMesh &m, double *x, const int64_t *rec_elem, int64_t n_rec, int power,
+                                 hipStream_t s);
 // Receivers (element in device order, (N+1)^2 interpolation weights each): p at the receivers of
 // the current state, and lambda_p += res_r w_r.
 hipError_t launch_record(const DeviceMesh &m, const int64_t *elem, const double *w, int n, double *out,

@@ -1833,7 +1833,7 @@ ndg_status check_adjoint_mesh(const ndg_mesh *m, const char *who) {
 }
 
 struct DeviceReceivers {
-    DevBuf elem, w;
+    DevBuf elem, w, w_src;  // w_src: the weights divided by J at their node (adjoint source)
     int n = 0;
 };
 
@@ -1861,6 +1861,10 @@ ndg_status upload_receivers(const ndg_mesh *m, const ndg_receivers *rec, DeviceR
     HIP_TRY(hipMemcpy(out.elem.p, elem.data(), elem.size() * sizeof(int64_t), hipMemcpyHostToDevice), "upload receivers");
     HIP_TRY(hipMemcpy(out.w.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice), "upload receivers");
     out.n = rec->n;
+    if ((st = dev_alloc(out.w_src, w.size() * sizeof(double), "hipMalloc receivers")) != NDG_OK) return st;
+    HIP_TRY(hipMemcpy(out.w_src.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice), "upload receivers");
+    HIP_TRY(launch_jacobian_scale(m->d, out.w_src.as<double>(), out.elem.as<int64_t>(), out.n, -1, m->stream),
+            "jacobian scale kernel");
     return NDG_OK;
 }
 
@@ -1903,6 +1907,8 @@ extern "C" ndg_status ndg_apply_rhs_transpose(ndg_mesh *mesh, const double *mp, 
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_apply_rhs_transpose sync");
     double *const ins[3] = {const_cast<double *>(mp), const_cast<double *>(mu), const_cast<double *>(mv)};
     if ((st = copy_state(mesh, muv.as<double>(), ins, false, "upload adjoint")) != NDG_OK) return st;
+    // the kernel works on values divided by J at their node
+    HIP_TRY(launch_jacobian_scale(d, muv.as<double>(), nullptr, 0, -1, mesh->stream), "jacobian scale kernel");
     HIP_TRY(hipMemsetAsync(lam.p, 0, sizeof(double) * d.n_q, mesh->stream), "hipMemset adjoint");
     AdjointArgs A;
     A.lam = lam.as<double>();
@@ -1914,6 +1920,7 @@ extern "C" ndg_status ndg_apply_rhs_transpose(ndg_mesh *mesh, const double *mp, 
     A.chi_out = nullptr;
     A.flags = 1 | 8;  // mu input, apply
     HIP_TRY(launch_adjoint_stage(d, A, mesh->stream), "adjoint kernel");
+    HIP_TRY(launch_jacobian_scale(d, lam.as<double>(), nullptr, 0, 1, mesh->stream), "jacobian scale kernel");
     double *const outs[3] = {lp, lu, lv};
     return copy_state(mesh, lam.as<double>(), outs, true, "download adjoint");
 }
@@ -2027,7 +2034,7 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     HIP_TRY(hipMemsetAsync(cpart.p, 0, sizeof(double) * nblk, s), "hipMemset adjoint");
     double *chi[2] = {chi0.as<double>(), chi1.as<double>()};
     int cc = 0;
-    HIP_TRY(launch_source(lam.as<double>(), d, R.elem.as<int64_t>(), R.w.as<double>(),
+    HIP_TRY(launch_source(lam.as<double>(), d, R.elem.as<int64_t>(), R.w_src.as<double>(),
                           res.as<double>() + (n_steps - 1) * R.n, R.n, s), "source kernel");
     {
         AdjointArgs A;
@@ -2058,7 +2065,7 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
                 A.as = kAm[stage];
                 A.gnext = stage > 0 ? kGm[stage - 1] * dt : kGm[2] * dt;
                 A.rec_elem = R.elem.as<int64_t>();
-                A.rec_w = R.w.as<double>();
+                A.rec_w = R.w_src.as<double>();
                 A.rec_res = j >= 1 ? res.as<double>() + (j - 1) * R.n : nullptr;
                 A.n_rec = R.n;
                 HIP_TRY(launch_adjoint_stage(d, A, s), "adjoint kernel");
@@ -2066,7 +2073,8 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
             }
     }
 
-    // Outputs; the mesh state goes back to the initial fields.
+    // Outputs (lambda back from its J-scaled form); the mesh state goes back to the initial fields.
+    HIP_TRY(launch_jacobian_scale(d, lam.as<double>(), nullptr, 0, 1, s), "jacobian scale kernel");
     std::vector<double> hc((size_t)nblk);
     HIP_TRY(hipMemcpyAsync(hc.data(), cpart.p, sizeof(double) * nblk, hipMemcpyDeviceToHost, s), "download c-gradient");
     double *const outs[3] = {dJ_dp0, dJ_du0, dJ_dv0};
