@@ -272,6 +272,20 @@ ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *rec, double
                                 int64_t checkpoint_interval, const double *data, double *J, double *dJ_dp0,
                                 double *dJ_du0, double *dJ_dv0, double *dJ_dc);
 
+/* How ndg_adjoint_gradient obtains dJ/dc (no reference counterpart):
+ * NDG_ADJOINT_C_RECOMPUTE (default): checkpoints, per-segment recompute of the stage states, and
+ *   <G^T mu~, q_s> / c gathered by the transposed stages;
+ * NDG_ADJOINT_C_STAGE_RECORDS: from the forward pass alone. The operator is linear and
+ *   time-invariant, so one step is q_n = P(z) q_{n-1} with z = dt G = c dt G1 and P the RK3
+ *   stability polynomial, dq_n/dc = (n / c) (z + z^2 + z^3 / 2) q_{n-1}, and the three stage
+ *   right-hand sides of step n (recorded at the receivers) give z q, z^2 q and z^3 q. The
+ *   transposed stages then need no forward state: no checkpoints, no recompute, no stage store
+ *   (checkpoint_interval and the store budget are not used). dJ/dp0, du0, dv0 are the same bit for
+ *   bit; dJ/dc agrees to rounding. */
+#define NDG_ADJOINT_C_RECOMPUTE 0
+#define NDG_ADJOINT_C_STAGE_RECORDS 1
+ndg_status ndg_adjoint_set_c_derivative(ndg_mesh *mesh, int32_t mode);
+
 /* Device memory (bytes) ndg_adjoint_gradient may spend on stage states beyond one segment (a
  * segment is checkpoint_interval steps, 3 states per step): the forward pass keeps the stage states
  * of the last 1 + floor(bytes / segment bytes) segments as it goes, and the reverse sweep recomputes

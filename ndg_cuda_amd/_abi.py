@@ -20,6 +20,7 @@ BC_NONE, BC_WALL, BC_SYMMETRY, BC_INFLOW, BC_OUTFLOW, BC_PERIODIC = range(6)
 IC_PLANE_WAVE, IC_PULSES = 0, 1
 PRECISION_FP64, PRECISION_FP32_COMPENSATED = 0, 1
 GRAPH_OFF, GRAPH_SEQUENTIAL, GRAPH_PARALLEL = 0, 1, 2
+ADJOINT_C_RECOMPUTE, ADJOINT_C_STAGE_RECORDS = 0, 1
 
 # ndg_output_fn: status (mesh, time, user)
 OUTPUT_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p)
@@ -38,7 +39,7 @@ EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tabl
             "ndg_set_state", "ndg_get_state", "ndg_get_solution", "ndg_get_traces", "ndg_compute_rhs",
             "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_solve_output", "ndg_run_steps", "ndg_synchronize",
             "ndg_estimate_error", "ndg_mesh_set_graph", "ndg_mesh_autotune", "ndg_mesh_tuning",
-            "ndg_mesh_set_wave_speed", "ndg_record_receivers", "ndg_adjoint_set_store_budget", "ndg_adjoint_gradient", "ndg_apply_rhs_transpose",
+            "ndg_mesh_set_wave_speed", "ndg_record_receivers", "ndg_adjoint_set_store_budget", "ndg_adjoint_set_c_derivative", "ndg_adjoint_gradient", "ndg_apply_rhs_transpose",
             "ndg_adapt", "ndg_mesh_set_adaptivity", "ndg_mesh_orders"]
 
 
@@ -119,6 +120,7 @@ def lib():
             "ndg_record_receivers": [vp, vp, f64, i64, vp],
             "ndg_adjoint_gradient": [vp, vp, f64, i64, i64, vp, vp, vp, vp, vp, vp],
             "ndg_adjoint_set_store_budget": [vp, i64],
+            "ndg_adjoint_set_c_derivative": [vp, i32],
             "ndg_apply_rhs_transpose": [vp, vp, vp, vp, vp, vp, vp],
             "ndg_adapt": [vp, i32, i32, vp],
             "ndg_mesh_set_adaptivity": [vp, i64],

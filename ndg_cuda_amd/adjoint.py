@@ -46,6 +46,15 @@ def record_receivers(mesh, receivers, dt, n_steps):
     return out
 
 
+def set_c_derivative(mesh, mode):
+    """'recompute' (default: checkpoints, recompute, <G^T mu~, q_s> / c in the transposed stages) or
+    'stage_records' (dJ/dc from the RK stage right-hand sides recorded at the receivers in the
+    forward pass, valid because the operator is linear and time-invariant; no checkpoints, no
+    recompute)."""
+    code = {"recompute": _abi.ADJOINT_C_RECOMPUTE, "stage_records": _abi.ADJOINT_C_STAGE_RECORDS}[mode]
+    check(lib().ndg_adjoint_set_c_derivative(mesh.handle, code))
+
+
 def set_store_budget(mesh, nbytes):
     """Device memory adjoint_gradient may spend on stage states beyond one segment: the forward pass
     keeps the last 1 + nbytes // (segment bytes) segments, which are then not recomputed. 0 (the

@@ -463,9 +463,9 @@ int64_t adjoint_blocks(const DeviceMesh &m) {
 }
 
 hipError_t launch_record(const DeviceMesh &m, const int64_t *elem, const double *w, int n, double *out,
-                         hipStream_t s) {
+                         hipStream_t s, const double *src) {
     const int ne = (m.groups[0].N + 1) * (m.groups[0].N + 1);
-    hipLaunchKernelGGL(record_kernel, dim3(n), dim3(64), 0, s, m.q, elem, w, ne, out);
+    hipLaunchKernelGGL(record_kernel, dim3(n), dim3(64), 0, s, src ? src : m.q, elem, w, ne, out);
     return hipGetLastError();
 }
 

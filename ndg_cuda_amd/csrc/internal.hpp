@@ -195,8 +195,9 @@ hipError_t launch_jacobian_scale(const DeviceMesh &m, double *x, const int64_t *
                                  hipStream_t s);
 // Receivers (element in device order, (N+1)^2 interpolation weights each): p at the receivers of
 // the current state, and lambda_p += res_r w_r.
+// Receiver values sum_k w[r][k] src[elem r][field 0][k] (src: the state q by default).
 hipError_t launch_record(const DeviceMesh &m, const int64_t *elem, const double *w, int n, double *out,
-                         hipStream_t s);
+                         hipStream_t s, const double *src = nullptr);
 hipError_t launch_source(double *lam, const DeviceMesh &m, const int64_t *elem, const double *w,
                          const double *res, int n, hipStream_t s);
 

@@ -113,6 +113,7 @@ struct ndg_mesh {
     int initial_N = 0, maximum_N = 0, max_split_level = 0;
     int64_t adapt_interval = 0;
     int64_t adjoint_store_budget = 0;        // bytes of stage states kept beyond one segment
+    int adjoint_c_mode = NDG_ADJOINT_C_RECOMPUTE;
 };
 
 extern "C" ndg_status ndg_last_error(char *buf, size_t buf_len) {
@@ -1834,6 +1835,7 @@ ndg_status check_adjoint_mesh(const ndg_mesh *m, const char *who) {
 
 struct DeviceReceivers {
     DevBuf elem, w, w_src;  // w_src: the weights divided by J at their node (adjoint source)
+    DevBuf w_t;             // the weights in the RK register's transposed node order
     int n = 0;
 };
 
@@ -1861,6 +1863,13 @@ ndg_status upload_receivers(const ndg_mesh *m, const ndg_receivers *rec, DeviceR
     HIP_TRY(hipMemcpy(out.elem.p, elem.data(), elem.size() * sizeof(int64_t), hipMemcpyHostToDevice), "upload receivers");
     HIP_TRY(hipMemcpy(out.w.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice), "upload receivers");
     out.n = rec->n;
+    std::vector<double> wt(w.size());
+    for (int r = 0; r < rec->n; ++r)
+        for (int i = 0; i <= N; ++i)
+            for (int j = 0; j <= N; ++j)
+                wt[(size_t)r * ne + j * (N + 1) + i] = w[(size_t)r * ne + i * (N + 1) + j];
+    if ((st = dev_alloc(out.w_t, wt.size() * sizeof(double), "hipMalloc receivers")) != NDG_OK) return st;
+    HIP_TRY(hipMemcpy(out.w_t.p, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice), "upload receivers");
     if ((st = dev_alloc(out.w_src, w.size() * sizeof(double), "hipMalloc receivers")) != NDG_OK) return st;
     HIP_TRY(hipMemcpy(out.w_src.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice), "upload receivers");
     HIP_TRY(launch_jacobian_scale(m->d, out.w_src.as<double>(), out.elem.as<int64_t>(), out.n, -1, m->stream),
@@ -1925,6 +1934,14 @@ extern "C" ndg_status ndg_apply_rhs_transpose(ndg_mesh *mesh, const double *mp, 
     return copy_state(mesh, lam.as<double>(), outs, true, "download adjoint");
 }
 
+extern "C" ndg_status ndg_adjoint_set_c_derivative(ndg_mesh *mesh, int32_t mode) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_adjoint_set_c_derivative: null mesh");
+    if (mode != NDG_ADJOINT_C_RECOMPUTE && mode != NDG_ADJOINT_C_STAGE_RECORDS)
+        return fail(NDG_ERR_INVALID, "ndg_adjoint_set_c_derivative: unknown mode");
+    mesh->adjoint_c_mode = mode;
+    return NDG_OK;
+}
+
 extern "C" ndg_status ndg_adjoint_set_store_budget(ndg_mesh *mesh, int64_t bytes) {
     if (!mesh || bytes < 0) return fail(NDG_ERR_INVALID, "ndg_adjoint_set_store_budget: null mesh or negative budget");
     mesh->adjoint_store_budget = bytes;
@@ -1943,15 +1960,20 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     hipStream_t s = mesh->stream;
     DeviceReceivers R;
     if ((st = upload_receivers(mesh, rec, R)) != NDG_OK) return st;
-    const int64_t n_ck = (n_steps + checkpoint_interval - 1) / checkpoint_interval;
-    const int64_t seg = std::min(checkpoint_interval, n_steps);
+    // With the c-derivative from stage records no forward state is needed in the reverse sweep:
+    // one segment, no stage store, the initial state as the only checkpoint.
+    const bool records = mesh->adjoint_c_mode == NDG_ADJOINT_C_STAGE_RECORDS;
+    const int64_t interval = records ? n_steps : checkpoint_interval;
+    const int64_t n_ck = (n_steps + interval - 1) / interval;
+    const int64_t seg = std::min(interval, n_steps);
     const size_t qb = sizeof(double) * d.n_q;
     // Segments k >= first_kept keep their stage states from the forward pass (one block each);
     // the others are recomputed into block 0 once the kept ones have been consumed.
     const int64_t seg_bytes = (int64_t)qb * 3 * seg;
-    const int64_t n_kept = std::max<int64_t>(1, std::min<int64_t>(n_ck, 1 + mesh->adjoint_store_budget / seg_bytes));
+    const int64_t n_kept =
+        records ? 0 : std::max<int64_t>(1, std::min<int64_t>(n_ck, 1 + mesh->adjoint_store_budget / seg_bytes));
     const int64_t first_kept = n_ck - n_kept;
-    DevBuf ck, lam, mu, chi0, chi1, cpart, recs, res;
+    DevBuf ck, lam, mu, chi0, chi1, cpart, recs, res, recs_g;
     std::vector<DevBuf> store((size_t)n_kept);
     if ((st = dev_alloc(ck, qb * n_ck, "hipMalloc checkpoints")) != NDG_OK) return st;
     for (DevBuf &b : store)
@@ -1964,6 +1986,8 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     if ((st = dev_alloc(cpart, sizeof(double) * nblk, "hipMalloc adjoint")) != NDG_OK) return st;
     if ((st = dev_alloc(recs, sizeof(double) * R.n * n_steps, "hipMalloc records")) != NDG_OK) return st;
     if ((st = dev_alloc(res, sizeof(double) * R.n * n_steps, "hipMalloc residuals")) != NDG_OK) return st;
+    if (records && (st = dev_alloc(recs_g, sizeof(double) * R.n * 3 * n_steps, "hipMalloc records")) != NDG_OK)
+        return st;
     auto ckp = [&](int64_t k) { return ck.as<double>() + k * d.n_q; };
     // stage state (step j of segment k, relative to the segment start)
     auto stp = [&](int64_t k, int64_t j, int stage) {
@@ -1997,22 +2021,37 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
         return NDG_OK;
     };
     auto stage_input = [&](int64_t k, int64_t j, int stage) -> const double * {
-        return (j == k * checkpoint_interval && stage == 0) ? ckp(k) : stp(k, j - k * checkpoint_interval, stage);
+        return (j == k * interval && stage == 0) ? ckp(k) : stp(k, j - k * interval, stage);
     };
 
-    // Forward pass: checkpoints; the kept segments keep their stage inputs right away.
+    // Forward pass: checkpoints; the kept segments keep their stage inputs right away. With stage
+    // records: the RK register's pressure at the receivers after every stage as well.
     HIP_TRY(launch_clock_begin(d, 0.0, 1, dt, s), "clock kernel");
-    for (int64_t n = 0; n < first_kept * checkpoint_interval; ++n) {
-        if (n % checkpoint_interval == 0)
-            HIP_TRY(hipMemcpyAsync(ckp(n / checkpoint_interval), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
-        if ((st = step_from_clock(mesh, dt)) != NDG_OK) return st;
-        HIP_TRY(launch_record(d, R.elem.as<int64_t>(), R.w.as<double>(), R.n, recs.as<double>() + n * R.n, s),
-                "record kernel");
-    }
-    for (int64_t k = first_kept; k < n_ck; ++k) {
-        HIP_TRY(hipMemcpyAsync(ckp(k), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
-        const int64_t n0 = k * checkpoint_interval, n1 = std::min(n0 + checkpoint_interval, n_steps);
-        if ((st = run_stored(k, ckp(k), n0, n1, true)) != NDG_OK) return st;
+    if (records) {
+        HIP_TRY(hipMemcpyAsync(ckp(0), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
+        for (int64_t n = 0; n < n_steps; ++n) {
+            for (int stage = 0; stage < 3; ++stage) {
+                HIP_TRY(launch_stage(d, stage, kAm[stage], kGm[stage] * dt, s), "stage kernel");
+                HIP_TRY(launch_record(d, R.elem.as<int64_t>(), R.w_t.as<double>(), R.n,
+                                      recs_g.as<double>() + (n * 3 + stage) * R.n, s, d.g),
+                        "record kernel");
+            }
+            HIP_TRY(launch_record(d, R.elem.as<int64_t>(), R.w.as<double>(), R.n, recs.as<double>() + n * R.n, s),
+                    "record kernel");
+        }
+    } else {
+        for (int64_t n = 0; n < first_kept * interval; ++n) {
+            if (n % interval == 0)
+                HIP_TRY(hipMemcpyAsync(ckp(n / interval), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
+            if ((st = step_from_clock(mesh, dt)) != NDG_OK) return st;
+            HIP_TRY(launch_record(d, R.elem.as<int64_t>(), R.w.as<double>(), R.n, recs.as<double>() + n * R.n, s),
+                    "record kernel");
+        }
+        for (int64_t k = first_kept; k < n_ck; ++k) {
+            HIP_TRY(hipMemcpyAsync(ckp(k), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
+            const int64_t n0 = k * interval, n1 = std::min(n0 + interval, n_steps);
+            if ((st = run_stored(k, ckp(k), n0, n1, true)) != NDG_OK) return st;
+        }
     }
     std::vector<double> hres((size_t)(R.n * n_steps));
     HIP_TRY(hipMemcpyAsync(hres.data(), recs.p, sizeof(double) * hres.size(), hipMemcpyDeviceToHost, s), "download records");
@@ -2024,6 +2063,26 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     }
     Jv /= 2;
     HIP_TRY(hipMemcpyAsync(res.p, hres.data(), sizeof(double) * hres.size(), hipMemcpyHostToDevice, s), "upload residuals");
+    // dJ/dc from the stage records. With z = dt G = c dt G1 one step is q_n = P(z) q_{n-1},
+    // P(z) = 1 + z + z^2/2 + z^3/6 (the RK3 stability polynomial), so dq_n/dc = (n / c) P'(z) z
+    // q_{n-1} = (n / c) (z + z^2 + z^3 / 2) q_{n-1}. The stage right-hand sides k_s = dt G q_s of step
+    // n, k_s = dt (g_s - a_s g_{s-1}), satisfy k0 = z q, k1 = z q + gm0 z^2 q,
+    // k2 = z q + beta z^2 q + gm0 gm1 z^3 q with beta = gm0 + gm1 (a1 + 1).
+    double dc_records = 0.0;
+    if (records) {
+        std::vector<double> hg((size_t)(R.n * 3 * n_steps));
+        HIP_TRY(hipMemcpy(hg.data(), recs_g.p, sizeof(double) * hg.size(), hipMemcpyDeviceToHost), "download records");
+        const double beta = kGm[0] + kGm[1] * (kAm[1] + 1.0);
+        for (int64_t n = 0; n < n_steps; ++n)
+            for (int r = 0; r < R.n; ++r) {
+                const double g0 = hg[(size_t)((n * 3 + 0) * R.n + r)], g1 = hg[(size_t)((n * 3 + 1) * R.n + r)],
+                             g2 = hg[(size_t)((n * 3 + 2) * R.n + r)];
+                const double k0 = dt * g0, k1 = dt * (g1 - kAm[1] * g0), k2 = dt * (g2 - kAm[2] * g1);
+                const double z2 = (k1 - k0) / kGm[0];
+                const double z3 = (k2 - k0 - beta * z2) / (kGm[0] * kGm[1]);
+                dc_records += hres[(size_t)(n * R.n + r)] * ((double)(n + 1) / d.wave_speed) * (k0 + z2 + z3 / 2);
+            }
+    }
 
     // Reverse sweep: per segment (the last one is already stored) recompute from its checkpoint
     // keeping the stage inputs, then the transposed stages backwards. The misfit source of each
@@ -2047,8 +2106,8 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
         cc = 1;
     }
     for (int64_t k = n_ck - 1; k >= 0; --k) {
-        const int64_t n0 = k * checkpoint_interval, n1 = std::min(n0 + checkpoint_interval, n_steps);
-        if (k < first_kept && (st = run_stored(k, ckp(k), n0, n1, false)) != NDG_OK) return st;
+        const int64_t n0 = k * interval, n1 = std::min(n0 + interval, n_steps);
+        if (!records && k < first_kept && (st = run_stored(k, ckp(k), n0, n1, false)) != NDG_OK) return st;
         for (int64_t j = n1 - 1; j >= n0; --j)
             for (int stage = 2; stage >= 0; --stage) {
                 AdjointArgs A;
@@ -2056,8 +2115,8 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
                 A.mu = mu.as<double>();
                 A.chi_in = chi[cc];
                 A.chi_out = chi[1 - cc];
-                A.qs = stage_input(k, j, stage);
-                A.cpart = cpart.as<double>();
+                A.qs = records ? nullptr : stage_input(k, j, stage);
+                A.cpart = records ? nullptr : cpart.as<double>();
                 const bool more = stage != 0 || j >= 1;  // a reverse stage follows
                 A.flags = 8 | (stage != 2 ? 1 : 0) | (stage != 0 ? 2 : 0) | (more ? 4 : 0) |
                           (stage == 0 && j >= 1 ? 16 : 0);
@@ -2086,6 +2145,6 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     HIP_TRY(launch_traces(d, s), "traces kernel");
     HIP_TRY(hipStreamSynchronize(s), "ndg_adjoint_gradient sync");
     if (J) *J = Jv;
-    if (dJ_dc) *dJ_dc = dc / d.wave_speed;
+    if (dJ_dc) *dJ_dc = records ? dc_records : dc / d.wave_speed;
     return NDG_OK;
 }
