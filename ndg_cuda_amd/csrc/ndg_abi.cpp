@@ -1285,6 +1285,39 @@ void carry_tuning(const DeviceMesh &from, DeviceMesh &to) {
             if (H.N == G.N) std::copy(H.tune, H.tune + 2, G.tune);
 }
 
+// The host-side description, topology and ordering of a mesh, swapped out while the splitting
+// branch of ndg_adapt builds its replacement.
+struct MeshHostState {
+    HostMesh host;
+    std::vector<int64_t> link, hang_second, bc_bar, bar_side, dev_of, host_off;
+    std::vector<char> periodic_link, hang_kind, bar_zero;
+    std::vector<BcSide> bc_base;
+    std::vector<int32_t> status, rotation, split_level, order;
+    std::vector<double> host_verts;
+    int64_t n_faces = 0, n_dofs = 0;
+};
+
+void swap_host_state(ndg_mesh *m, MeshHostState &s) {
+    std::swap(m->host, s.host);
+    std::swap(m->link, s.link);
+    std::swap(m->hang_second, s.hang_second);
+    std::swap(m->bc_bar, s.bc_bar);
+    std::swap(m->bar_side, s.bar_side);
+    std::swap(m->dev_of, s.dev_of);
+    std::swap(m->host_off, s.host_off);
+    std::swap(m->periodic_link, s.periodic_link);
+    std::swap(m->hang_kind, s.hang_kind);
+    std::swap(m->bar_zero, s.bar_zero);
+    std::swap(m->bc_base, s.bc_base);
+    std::swap(m->status, s.status);
+    std::swap(m->rotation, s.rotation);
+    std::swap(m->split_level, s.split_level);
+    std::swap(m->order, s.order);
+    std::swap(m->host_verts, s.host_verts);
+    std::swap(m->n_faces, s.n_faces);
+    std::swap(m->n_dofs, s.n_dofs);
+}
+
 // quad_map (include/functions/quad_map.tpp:3-8)
 v2 quad_point(double xi, double eta, const v2 P[4]) {
     return v2{(P[0].x * (1 - xi) * (1 - eta) + P[1].x * (xi + 1) * (1 - eta) + P[2].x * (xi + 1) * (eta + 1) +
@@ -1472,8 +1505,11 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
     }
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_adapt sync");
     drop_graphs(mesh);
-    // commit the new host topology and build the device mesh
+    // commit the new host topology and build the device mesh (the previous host state is kept
+    // until the new device mesh holds the transferred state, and restored on a failure before)
     DeviceMesh &d = mesh->d;
+    MeshHostState previous;
+    swap_host_state(mesh, previous);
     mesh->host = std::move(nh);
     mesh->link = std::move(topo.link);
     mesh->periodic_link = std::move(topo.periodic_link);
@@ -1497,6 +1533,7 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
     DeviceMesh nd;
     if ((st = build_device(mesh, order, nd)) != NDG_OK) {
         free_device(nd);
+        swap_host_state(mesh, previous);
         return st;
     }
     auto offset = [](const DeviceMesh &dm, int N, int64_t dev) -> int64_t {
@@ -1527,6 +1564,7 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
     if (coords_dev) (void)hipFree(coords_dev);
     if (st != NDG_OK) {
         free_device(nd);
+        swap_host_state(mesh, previous);
         return st;
     }
     nd.clock = d.clock;
