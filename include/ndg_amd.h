@@ -168,6 +168,28 @@ ndg_status ndg_get_state(ndg_mesh *mesh, double *p, double *u, double *v);
  * element e's point (i, j) at e * n * n + i * n + j (caller's element order); a null array is
  * skipped. Requires n >= 2. */
 ndg_status ndg_get_solution(ndg_mesh *mesh, double *x, double *y, double *p, double *u, double *v);
+/* Output of Mesh2D_t::get_complete_solution (src/meshes/Mesh2D_t.cu:1861-1950, :5096-5155,
+ * Element2D_t::interpolate_complete_solution src/entities/Element2D_t.cu:623-665). Point fields
+ * hold n_elements * n * n values laid out as in ndg_get_solution; element fields hold n_elements
+ * values in the caller's element order. Any pointer may be null (skipped). */
+typedef struct ndg_complete_solution {
+    double *x, *y, *p, *u, *v;
+    double *dp_dt, *du_dt, *dv_dt;
+    double *p_analytical_error, *u_analytical_error, *v_analytical_error;
+    int32_t *N;
+    double *p_error, *u_error, *v_error;
+    double *p_sigma, *u_sigma, *v_sigma;
+    int32_t *refine, *coarsen, *split_level, *status, *rotation;
+} ndg_complete_solution;
+/* Point fields: x, y, p, u, v as ndg_get_solution; dp_dt, du_dt, dv_dt the right-hand side of the
+ * current state at `time` (the reference writes its last RK stage's G there, the right-hand side
+ * of that stage's state); *_analytical_error the node-wise |g(x, time) - q| of the reference plane
+ * wave (include/functions/analytical_solution.tpp:5-11, moving at the mesh's wave speed),
+ * interpolated like the state. Element fields: N; the last ndg_estimate_error result (p_error,
+ * p_sigma, refine, coarsen; zero before the first estimate and for elements created or raised by
+ * an adaptation since); u/v error and sigma are 0 as in the reference, whose device estimator is
+ * the pressure one (src/meshes/Mesh2D_t.cu:5158-5171); split level, Hilbert status and rotation. */
+ndg_status ndg_get_complete_solution(ndg_mesh *mesh, double time, const ndg_complete_solution *out);
 /* Element traces (Mesh2D_t::interpolate_to_boundaries, src/meshes/Mesh2D_t.cpp:3794-3801) of the
  * current state, packed [element][side][field][N+1]. */
 ndg_status ndg_get_traces(ndg_mesh *mesh, double *out);

@@ -8,10 +8,12 @@
 //       tolerance_max, load_balancing_threshold, NDG, stream)              include/meshes/Mesh2D_t.cuh:23
 //   mesh.initial_conditions(...)                                              src/meshes/Mesh2D_t.cpp:1192-1216
 //   SEM::Device::Solvers::Solver2D_t(CFL, output_times, viscosity)            include/solvers/Solver2D_t.cuh
-//   solver.solve(NDG, mesh)                                                   include/solvers/Solver2D_t.tcu:7-135
-// Differences: the mesh comes from an in-memory description (or the square generator) instead of
-// a CGNS/SU2 path, no VTK writer is attached, and errors throw SEM::Device::Error (carrying the
-// status code) instead of calling exit().
+//   solver.solve(NDG, mesh, data_writer)                                      include/solvers/Solver2D_t.tcu:7-135
+//   SEM::Helpers::DataWriter_t(path)           (ndg_data_writer.hpp)           include/helpers/DataWriter_t.h:14-33
+// Differences: the mesh comes from an in-memory description, the square generator or an SU2 file
+// (no CGNS), the writer writes VTK XML itself (no VTK library), the mesh's write_data /
+// write_complete_data take the NDG_t instead of its device tables, and errors throw
+// SEM::Device::Error (carrying the status code) instead of calling exit().
 #ifndef NDG_SEM_HPP
 #define NDG_SEM_HPP
 
@@ -23,9 +25,11 @@
 #include <filesystem>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "ndg_amd.h"
+#include "ndg_data_writer.hpp"
 
 namespace SEM {
 namespace Device {
@@ -256,6 +260,59 @@ class Mesh2D_t {
                                v_output_.data()));
     }
     std::vector<double> x_output_, y_output_, p_output_, u_output_, v_output_;
+    // Mesh2D_t::get_complete_solution (src/meshes/Mesh2D_t.cu:1861-1950): fills the output
+    // members as get_solution does and the other fields (ndg_get_complete_solution: dp_dt etc. are
+    // the right-hand side at `time`, the element errors those of the last estimate_error). The
+    // tables are the library's; the two table arguments keep the reference's call shape.
+    void get_complete_solution(double time, const std::vector<std::vector<double>> &polynomial_nodes,
+                               const std::vector<std::vector<double>> &interpolation_matrices, std::vector<int> &N,
+                               std::vector<double> &dp_dt, std::vector<double> &du_dt, std::vector<double> &dv_dt,
+                               std::vector<double> &p_error, std::vector<double> &u_error,
+                               std::vector<double> &v_error, std::vector<double> &p_sigma,
+                               std::vector<double> &u_sigma, std::vector<double> &v_sigma, std::vector<int> &refine,
+                               std::vector<int> &coarsen, std::vector<int> &split_level,
+                               std::vector<double> &p_analytical_error, std::vector<double> &u_analytical_error,
+                               std::vector<double> &v_analytical_error, std::vector<int> &status,
+                               std::vector<int> &rotation) {
+        (void)polynomial_nodes;
+        (void)interpolation_matrices;
+        const std::size_t points = n_elements_ * n_interpolation_points_ * n_interpolation_points_;
+        for (std::vector<double> *a : {&x_output_, &y_output_, &p_output_, &u_output_, &v_output_, &dp_dt, &du_dt,
+                                       &dv_dt, &p_analytical_error, &u_analytical_error, &v_analytical_error})
+            a->resize(points);
+        for (std::vector<double> *a : {&p_error, &u_error, &v_error, &p_sigma, &u_sigma, &v_sigma})
+            a->resize(n_elements_);
+        for (std::vector<int> *a : {&N, &refine, &coarsen, &split_level, &status, &rotation}) a->resize(n_elements_);
+        ndg_complete_solution out{x_output_.data(), y_output_.data(), p_output_.data(), u_output_.data(),
+                                  v_output_.data(), dp_dt.data(), du_dt.data(), dv_dt.data(),
+                                  p_analytical_error.data(), u_analytical_error.data(), v_analytical_error.data(),
+                                  N.data(), p_error.data(), u_error.data(), v_error.data(), p_sigma.data(),
+                                  u_sigma.data(), v_sigma.data(), refine.data(), coarsen.data(),
+                                  split_level.data(), status.data(), rotation.data()};
+        check(ndg_get_complete_solution(handle_, time, &out));
+    }
+    // Mesh2D_t::write_data (src/meshes/Mesh2D_t.cpp:1601-1609): get_solution, then the writer.
+    template <typename Polynomial>
+    void write_data(double time, const Entities::NDG_t<Polynomial> &NDG, const Helpers::DataWriter_t &data_writer) {
+        get_solution(NDG);
+        data_writer.write_data(n_interpolation_points_, n_elements_, time, x_output_, y_output_, p_output_,
+                               u_output_, v_output_);
+    }
+    // Mesh2D_t::write_complete_data (src/meshes/Mesh2D_t.cu:1861-1950): get_complete_solution,
+    // then the writer with rank 0.
+    template <typename Polynomial>
+    void write_complete_data(double time, const Entities::NDG_t<Polynomial> &NDG,
+                             const Helpers::DataWriter_t &data_writer) {
+        std::vector<int> N, refine, coarsen, split_level, status, rotation;
+        std::vector<double> dp_dt, du_dt, dv_dt, p_error, u_error, v_error, p_sigma, u_sigma, v_sigma, pa, ua, va;
+        get_complete_solution(time, NDG.nodes_, NDG.interpolation_matrices_, N, dp_dt, du_dt, dv_dt, p_error, u_error,
+                              v_error, p_sigma, u_sigma, v_sigma, refine, coarsen, split_level, pa, ua, va, status,
+                              rotation);
+        data_writer.write_complete_data(n_interpolation_points_, n_elements_, time, 0, x_output_, y_output_, p_output_,
+                                        u_output_, v_output_, N, dp_dt, du_dt, dv_dt, p_error, u_error, v_error,
+                                        p_sigma, u_sigma, v_sigma, refine, coarsen, split_level, pa, ua, va, status,
+                                        rotation);
+    }
     // Traces of the current state, [element][side][field][N_e+1] packed element after element.
     void interpolate_to_boundaries(std::vector<double> &traces) {
         traces.resize(n_trace_values_);
@@ -318,24 +375,35 @@ class Solver2D_t {
         mesh.refresh();
     }
 
-    // The same with the reference's output events (include/solvers/Solver2D_t.tpp:21-29, :74-83,
-    // :112-127): data_writer(time, mesh) is called where the reference calls
-    // write_complete_data, and may read the mesh. An exception it throws stops the solve and is
+    // The same with the reference's output events (include/solvers/Solver2D_t.tcu:21-29, :74-83,
+    // :112-127). A Helpers::DataWriter_t gets what the reference writes there: estimate_error
+    // (after the first step) and write_complete_data. Any other data_writer is called as
+    // data_writer(time, mesh) and may read the mesh. An exception it throws stops the solve and is
     // rethrown here.
     template <typename Polynomial, typename Writer>
     void solve(const Entities::NDG_t<Polynomial> &NDG, Meshes::Mesh2D_t &mesh, Writer &&data_writer) {
-        (void)NDG;
         if (output_times_.empty()) throw Error(NDG_ERR_INVALID, "Solver2D_t::solve: no output times");
+        using W = std::remove_reference_t<Writer>;
         struct Context {
+            const Entities::NDG_t<Polynomial> *NDG;
             Meshes::Mesh2D_t *mesh;
-            Writer *writer;
+            W *writer;
             std::exception_ptr error;
-        } ctx{&mesh, &data_writer, nullptr};
+        } ctx{&NDG, &mesh, &data_writer, nullptr};
         auto event = [](ndg_mesh *, double time, void *user) -> ndg_status {
             Context &c = *static_cast<Context *>(user);
             try {
                 c.mesh->refresh();
-                (*c.writer)(time, *c.mesh);
+                if constexpr (std::is_base_of_v<Helpers::DataWriter_t, std::decay_t<Writer>>) {
+                    if (time > 0.0) {
+                        std::vector<double> error, sigma;
+                        std::vector<int32_t> refine, coarsen;
+                        c.mesh->estimate_error(error, sigma, refine, coarsen);
+                    }
+                    c.mesh->write_complete_data(time, *c.NDG, *c.writer);
+                } else {
+                    (*c.writer)(time, *c.mesh);
+                }
                 return NDG_OK;
             } catch (...) {
                 c.error = std::current_exception();

@@ -36,7 +36,7 @@ EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tabl
             "ndg_tables_get", "ndg_square_mesh_sizes", "ndg_square_mesh_fill", "ndg_su2_sizes", "ndg_su2_fill",
             "ndg_mesh_create",
             "ndg_mesh_destroy", "ndg_mesh_set_stream", "ndg_mesh_info", "ndg_mesh_set_precision", "ndg_set_ic",
-            "ndg_set_state", "ndg_get_state", "ndg_get_solution", "ndg_get_traces", "ndg_compute_rhs",
+            "ndg_set_state", "ndg_get_state", "ndg_get_solution", "ndg_get_complete_solution", "ndg_get_traces", "ndg_compute_rhs",
             "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_solve_output", "ndg_run_steps", "ndg_synchronize",
             "ndg_estimate_error", "ndg_mesh_set_graph", "ndg_mesh_autotune", "ndg_mesh_tuning",
             "ndg_mesh_set_wave_speed", "ndg_record_receivers", "ndg_adjoint_set_store_budget", "ndg_adjoint_set_c_derivative", "ndg_adjoint_gradient", "ndg_apply_rhs_transpose",
@@ -67,6 +67,20 @@ class SquareMeshParams(ctypes.Structure):
 
 class Receivers(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("element", ctypes.c_void_p), ("xi_eta", ctypes.c_void_p)]
+
+
+COMPLETE_POINT_FIELDS = ("x", "y", "p", "u", "v", "dp_dt", "du_dt", "dv_dt", "p_analytical_error",
+                         "u_analytical_error", "v_analytical_error")
+COMPLETE_ELEMENT_FIELDS = (("N", np.int32), ("p_error", np.float64), ("u_error", np.float64),
+                           ("v_error", np.float64), ("p_sigma", np.float64), ("u_sigma", np.float64),
+                           ("v_sigma", np.float64), ("refine", np.int32), ("coarsen", np.int32),
+                           ("split_level", np.int32), ("status", np.int32), ("rotation", np.int32))
+
+
+class CompleteSolution(ctypes.Structure):
+    """ndg_complete_solution: one pointer per output field, in the header's order."""
+    _fields_ = [(name, ctypes.c_void_p) for name in COMPLETE_POINT_FIELDS] + \
+               [(name, ctypes.c_void_p) for name, _ in COMPLETE_ELEMENT_FIELDS]
 
 
 class ICDesc(ctypes.Structure):
@@ -104,6 +118,7 @@ def lib():
             "ndg_set_state": [vp, vp, vp, vp],
             "ndg_get_state": [vp, vp, vp, vp],
             "ndg_get_solution": [vp, vp, vp, vp, vp, vp],
+            "ndg_get_complete_solution": [vp, f64, vp],
             "ndg_get_traces": [vp, vp],
             "ndg_compute_rhs": [vp, f64, vp, vp, vp],
             "ndg_get_delta_t": [vp, f64, vp],

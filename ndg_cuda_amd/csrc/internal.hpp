@@ -211,8 +211,11 @@ struct OutputRec {
     int64_t i_off;   // its order's [n][N+1] interpolation matrix in the concatenated table
     int32_t N, pad;
 };
-hipError_t launch_output(const double *q, const double *verts, const OutputRec *recs, const double *interp, int n,
-                         int64_t K, double *out, hipStream_t s);
+// rhs == nullptr: 5 output planes (x, y, p, u, v); else 11 (plus the right-hand side and the
+// plane-wave errors at `time`).
+hipError_t launch_output(const double *q, const double *rhs, const double *verts, const OutputRec *recs,
+                         const double *interp, const double *tab1d, int n, int64_t K, double time, double cw,
+                         double *out, hipStream_t s);
 
 struct TransferRec {
     int64_t old_off, new_off;

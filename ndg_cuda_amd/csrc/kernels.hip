@@ -1144,17 +1144,45 @@ hipError_t launch_transfer(const double *q_old, double *q_new, const TransferRec
 // workgroup, caller's order) and output point (i, j) of the n x n uniform grid, x and y by the
 // bilinear map at ((i / (n - 1)) * 2 - 1, (j / (n - 1)) * 2 - 1), and each field as
 // sum_m sum_k q[m][k] I[i][m] I[j][k] with the order's uniform-grid interpolation matrix I.
-// Output layout [x, y, p, u, v][element][i * n + j].
-__global__ __launch_bounds__(64) void output_kernel(const double *__restrict__ q, const double *__restrict__ verts,
+// With rhs given, also Mesh2D_t::get_complete_solution (src/meshes/Mesh2D_t.cu:5096-5155,
+// src/entities/Element2D_t.cu:623-665): the right-hand side (laid out as q) and
+// |g(x_mk, time) - q[m][k]| of the plane wave at the solution nodes, interpolated
+// with the same weights. The element's state, right-hand side and node errors are staged in LDS.
+// Output layout [x, y, p, u, v (, dp, du, dv, |dp|, |du|, |dv|)][element][i * n + j].
+constexpr int kOutMaxNe = (kMaxOrder + 1) * (kMaxOrder + 1);
+
+__global__ __launch_bounds__(64) void output_kernel(const double *__restrict__ q, const double *__restrict__ rhs,
+                                                    const double *__restrict__ verts,
                                                     const OutputRec *__restrict__ recs,
-                                                    const double *__restrict__ interp, int n, int64_t K,
-                                                    double *__restrict__ out) {
+                                                    const double *__restrict__ interp,
+                                                    const double *__restrict__ tab1d, int n, int64_t K,
+                                                    double time, double cw, double *__restrict__ out) {
+    __shared__ double s_q[3 * kOutMaxNe], s_r[3 * kOutMaxNe], s_a[3 * kOutMaxNe];
     const int64_t e = blockIdx.x;
     const OutputRec R = recs[e];
     const int nf = R.N + 1, ne = nf * nf, nn = n * n;
     const double *I = interp + R.i_off;
     const double *P = verts + 8 * R.dev;
     const double *src = q + R.q_off;
+    const bool complete = rhs != nullptr;
+    const double *nodes = tab1d + 3 * R.N * kTab1dStride;
+    for (int r = threadIdx.x; r < 3 * ne; r += 64) {
+        const double v = src[r];
+        s_q[r] = v;
+        if (complete) {
+            const int c = r / ne, mk = r - c * ne, m = mk / nf, k = mk - m * nf;
+            s_r[r] = rhs[R.q_off + r];
+            const double xi = nodes[m], eta = nodes[k];
+            const double w0 = (1 - xi) * (1 - eta), w1 = (xi + 1) * (1 - eta), w2 = (xi + 1) * (eta + 1),
+                         w3 = (1 - xi) * (eta + 1);
+            const double X = (P[0] * w0 + P[2] * w1 + P[4] * w2 + P[6] * w3) / 4;
+            const double Y = (P[1] * w0 + P[3] * w1 + P[5] * w2 + P[7] * w3) / 4;
+            double g[3];
+            plane_wave(X, Y, time, cw, g[0], g[1], g[2]);
+            s_a[r] = fabs(g[c] - v);
+        }
+    }
+    __syncthreads();
     const int64_t plane = K * nn;
     for (int r = threadIdx.x; r < nn; r += 64) {
         const int i = r / n, j = r % n;
@@ -1163,25 +1191,34 @@ __global__ __launch_bounds__(64) void output_kernel(const double *__restrict__ q
                      w3 = (1 - xi) * (eta + 1);
         const double x = (P[0] * w0 + P[2] * w1 + P[4] * w2 + P[6] * w3) / 4;
         const double y = (P[1] * w0 + P[3] * w1 + P[5] * w2 + P[7] * w3) / 4;
-        double f[3] = {0.0, 0.0, 0.0};
+        double f[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int m = 0; m < nf; ++m) {
             const double a = I[i * nf + m];
             for (int k = 0; k < nf; ++k) {
                 const double b = I[j * nf + k];
-                for (int c = 0; c < 3; ++c) f[c] += src[c * ne + m * nf + k] * a * b;
+                const int mk = m * nf + k;
+                for (int c = 0; c < 3; ++c) f[c] += s_q[c * ne + mk] * a * b;
+                if (complete)
+                    for (int c = 0; c < 3; ++c) {
+                        f[3 + c] += s_r[c * ne + mk] * a * b;
+                        f[6 + c] += s_a[c * ne + mk] * a * b;
+                    }
             }
         }
         const int64_t at = e * nn + r;
         out[at] = x;
         out[plane + at] = y;
-        for (int c = 0; c < 3; ++c) out[(2 + c) * plane + at] = f[c];
+        const int nfield = complete ? 9 : 3;
+        for (int c = 0; c < nfield; ++c) out[(2 + c) * plane + at] = f[c];
     }
 }
 
-hipError_t launch_output(const double *q, const double *verts, const OutputRec *recs, const double *interp, int n,
-                         int64_t K, double *out, hipStream_t s) {
+hipError_t launch_output(const double *q, const double *rhs, const double *verts, const OutputRec *recs,
+                         const double *interp, const double *tab1d, int n, int64_t K, double time, double cw,
+                         double *out, hipStream_t s) {
     if (K <= 0) return hipSuccess;
-    hipLaunchKernelGGL(output_kernel, dim3((unsigned)K), dim3(64), 0, s, q, verts, recs, interp, n, K, out);
+    hipLaunchKernelGGL(output_kernel, dim3((unsigned)K), dim3(64), 0, s, q, rhs, verts, recs, interp, tab1d, n, K,
+                       time, cw, out);
     return hipGetLastError();
 }
 

@@ -110,6 +110,10 @@ struct ndg_mesh {
     std::vector<char> bar_zero;              // per bar: ghost storage resized to zero by an adaptation
     HostMesh host;                           // the description the topology was built from
     std::vector<int32_t> status, rotation, split_level;  // Hilbert status per element
+    // Last estimate per element (caller's order): error, sigma, refine, coarsen. Zero until the
+    // first estimate and for elements an adaptation creates, as the reference's element fields.
+    std::vector<double> est_error, est_sigma;
+    std::vector<int32_t> est_refine, est_coarsen;
     int initial_N = 0, maximum_N = 0, max_split_level = 0;
     int64_t adapt_interval = 0;
     int64_t adjoint_store_budget = 0;        // bytes of stage states kept beyond one segment
@@ -990,16 +994,26 @@ extern "C" ndg_status ndg_get_state(ndg_mesh *mesh, double *p, double *u, double
     return copy_state(mesh, mesh->d.q, outs, true, "download state");
 }
 
-extern "C" ndg_status ndg_get_solution(ndg_mesh *mesh, double *x, double *y, double *p, double *u, double *v) {
-    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_get_solution: null mesh");
+namespace {
+
+// The uniform-grid output of get_solution (5 planes: x, y, p, u, v) or, with `complete`, of
+// get_complete_solution (11 planes: also dq/dt of the current state at `time` and the plane-wave
+// errors), each plane [K][n][n] in the caller's element order; a null plane pointer is skipped.
+ndg_status interpolated_output(ndg_mesh *mesh, const char *who, bool complete, double time,
+                               double *const *planes) {
     const Tables &tab = mesh->tables->t;
     const int64_t n = tab.n_interp;
-    if (n < 2) return fail(NDG_ERR_INVALID, "ndg_get_solution: the tables have fewer than 2 output points");
-    if (n > 4096) return fail(NDG_ERR_INVALID, "ndg_get_solution: too many output points");
+    if (n < 2) return fail(NDG_ERR_INVALID, (std::string(who) + ": the tables have fewer than 2 output points").c_str());
+    if (n > 4096) return fail(NDG_ERR_INVALID, (std::string(who) + ": too many output points").c_str());
     ndg_status st = refresh_fp64(mesh);
     if (st != NDG_OK) return st;
-    const DeviceMesh &d = mesh->d;
+    DeviceMesh &d = mesh->d;
+    if (complete) {
+        if (!d.rhs) HIP_TRY(hipMalloc((void **)&d.rhs, sizeof(double) * d.n_q), "hipMalloc rhs");
+        HIP_TRY(launch_rhs(d, time, mesh->stream), "rhs kernel");
+    }
     const int64_t K = d.K, nn = n * n;
+    const int n_planes = complete ? 11 : 5;
     std::vector<double> interp;
     std::vector<int64_t> i_off(kMaxOrder + 1, -1);
     for (const Group &G : d.groups) {
@@ -1025,15 +1039,15 @@ extern "C" ndg_status ndg_get_solution(ndg_mesh *mesh, double *x, double *y, dou
     st = device_upload(&recs_dev, recs, "upload output records");
     if (st == NDG_OK) st = device_upload(&interp_dev, interp, "upload interpolation matrices");
     if (st == NDG_OK) {
-        const hipError_t e = hipMalloc((void **)&out_dev, sizeof(double) * 5 * K * nn);
+        const hipError_t e = hipMalloc((void **)&out_dev, sizeof(double) * n_planes * K * nn);
         if (e != hipSuccess) st = hip_fail(e, "hipMalloc output");
     }
     if (st == NDG_OK) {
-        hipError_t e = launch_output(d.q, d.verts, recs_dev, interp_dev, (int)n, K, out_dev, mesh->stream);
-        double *const outs[5] = {x, y, p, u, v};
-        for (int f = 0; f < 5 && e == hipSuccess; ++f)
-            if (outs[f])
-                e = hipMemcpyAsync(outs[f], out_dev + f * K * nn, sizeof(double) * K * nn, hipMemcpyDeviceToHost,
+        hipError_t e = launch_output(d.q, complete ? d.rhs : nullptr, d.verts, recs_dev, interp_dev, d.tab1d, (int)n,
+                                     K, time, d.wave_speed, out_dev, mesh->stream);
+        for (int f = 0; f < n_planes && e == hipSuccess; ++f)
+            if (planes[f])
+                e = hipMemcpyAsync(planes[f], out_dev + f * K * nn, sizeof(double) * K * nn, hipMemcpyDeviceToHost,
                                    mesh->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(mesh->stream);
         if (e != hipSuccess) st = hip_fail(e, "output kernel");
@@ -1041,6 +1055,42 @@ extern "C" ndg_status ndg_get_solution(ndg_mesh *mesh, double *x, double *y, dou
     for (void *ptr : {(void *)recs_dev, (void *)interp_dev, (void *)out_dev})
         if (ptr) (void)hipFree(ptr);
     return st;
+}
+
+}  // namespace
+
+extern "C" ndg_status ndg_get_solution(ndg_mesh *mesh, double *x, double *y, double *p, double *u, double *v) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_get_solution: null mesh");
+    double *const planes[5] = {x, y, p, u, v};
+    return interpolated_output(mesh, "ndg_get_solution", false, 0.0, planes);
+}
+
+extern "C" ndg_status ndg_get_complete_solution(ndg_mesh *mesh, double time, const ndg_complete_solution *out) {
+    if (!mesh || !out) return fail(NDG_ERR_INVALID, "ndg_get_complete_solution: null argument");
+    if (!std::isfinite(time)) return fail(NDG_ERR_INVALID, "ndg_get_complete_solution: time is not finite");
+    double *const planes[11] = {out->x,     out->y,     out->p,     out->u,
+                                out->v,     out->dp_dt, out->du_dt, out->dv_dt,
+                                out->p_analytical_error, out->u_analytical_error, out->v_analytical_error};
+    ndg_status st = interpolated_output(mesh, "ndg_get_complete_solution", true, time, planes);
+    if (st != NDG_OK) return st;
+    const size_t K = (size_t)mesh->d.K;
+    const bool have_est = mesh->est_error.size() == K;
+    for (size_t e = 0; e < K; ++e) {
+        if (out->N) out->N[e] = mesh->order[e];
+        if (out->p_error) out->p_error[e] = have_est ? mesh->est_error[e] : 0.0;
+        if (out->p_sigma) out->p_sigma[e] = have_est ? mesh->est_sigma[e] : 0.0;
+        if (out->refine) out->refine[e] = have_est ? mesh->est_refine[e] : 0;
+        if (out->coarsen) out->coarsen[e] = have_est ? mesh->est_coarsen[e] : 0;
+        // the reference estimates the pressure only (Mesh2D_t.cu:5158-5171 calls estimate_p_error)
+        if (out->u_error) out->u_error[e] = 0.0;
+        if (out->v_error) out->v_error[e] = 0.0;
+        if (out->u_sigma) out->u_sigma[e] = 0.0;
+        if (out->v_sigma) out->v_sigma[e] = 0.0;
+        if (out->split_level) out->split_level[e] = mesh->split_level[e];
+        if (out->status) out->status[e] = mesh->status[e];
+        if (out->rotation) out->rotation[e] = mesh->rotation[e];
+    }
+    return NDG_OK;
 }
 
 extern "C" ndg_status ndg_get_traces(ndg_mesh *mesh, double *out) {
@@ -1365,6 +1415,27 @@ v2 reference_point(v2 g, const v2 P[4]) {
 // raised gets fresh geometry. The faces are rebuilt from the new elements and ghosts. Elements
 // with a periodic side, or whose split would cover a side of an unsplit neighbour with more than
 // two faces, are refused (NDG_ERR_UNSUPPORTED, nothing changed).
+// The last estimate of the elements an adaptation keeps (new element k is old element from[k],
+// unchanged when keep[k]); created and raised elements start from zero, as the reference's new
+// Element2D_t objects do (src/meshes/Mesh2D_t.cu:6206-6232).
+void carry_estimates(ndg_mesh *m, size_t K_old, const std::vector<int64_t> &from, const std::vector<char> &keep) {
+    const size_t Kn = from.size();
+    const bool have = m->est_error.size() == K_old;
+    std::vector<double> er(Kn, 0.0), sg(Kn, 0.0);
+    std::vector<int32_t> rf(Kn, 0), cs(Kn, 0);
+    for (size_t k = 0; have && k < Kn; ++k)
+        if (keep[k]) {
+            er[k] = m->est_error[(size_t)from[k]];
+            sg[k] = m->est_sigma[(size_t)from[k]];
+            rf[k] = m->est_refine[(size_t)from[k]];
+            cs[k] = m->est_coarsen[(size_t)from[k]];
+        }
+    m->est_error = std::move(er);
+    m->est_sigma = std::move(sg);
+    m->est_refine = std::move(rf);
+    m->est_coarsen = std::move(cs);
+}
+
 ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const std::vector<double> &sig,
                        int32_t N_max, int32_t max_split_level, int64_t *n_changed) {
     const HostMesh &h = mesh->host;
@@ -1573,6 +1644,11 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
     carry_tuning(d, nd);
     free_device(d);
     d = nd;
+    {
+        std::vector<char> keep((size_t)Knew);
+        for (int64_t k = 0; k < Knew; ++k) keep[(size_t)k] = !child[(size_t)k] && !raise[(size_t)parent[(size_t)k]];
+        carry_estimates(mesh, (size_t)K, parent, keep);
+    }
     HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * d.n_q, mesh->stream), "hipMemset g");
     HIP_TRY(launch_traces(d, mesh->stream), "traces kernel");
     if (precision == NDG_PRECISION_FP32_COMPENSATED) {
@@ -1668,6 +1744,15 @@ extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split
     carry_tuning(d, nd);
     free_device(d);
     d = nd;
+    {
+        std::vector<int64_t> same((size_t)K);
+        std::vector<char> keep((size_t)K);
+        for (int64_t e = 0; e < K; ++e) {
+            same[(size_t)e] = e;
+            keep[(size_t)e] = order[e] == old_order[e];
+        }
+        carry_estimates(mesh, (size_t)K, same, keep);
+    }
     HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * d.n_q, mesh->stream), "hipMemset g");
     HIP_TRY(launch_traces(d, mesh->stream), "traces kernel");
     if (precision == NDG_PRECISION_FP32_COMPENSATED) {
@@ -1803,12 +1888,21 @@ extern "C" ndg_status ndg_estimate_error(ndg_mesh *mesh, double *error, double *
     HIP_TRY(hipMemcpyAsync(host.data(), mesh->est_dev, sizeof(double) * 4 * d.K, hipMemcpyDeviceToHost, mesh->stream),
             "download estimator");
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_estimate_error sync");
+    const size_t K = (size_t)d.K;
+    mesh->est_error.resize(K);
+    mesh->est_sigma.resize(K);
+    mesh->est_refine.resize(K);
+    mesh->est_coarsen.resize(K);
     for (int64_t e = 0; e < d.K; ++e) {
         const double *h = host.data() + 4 * mesh->dev_of[e];
-        if (error) error[e] = h[0];
-        if (sigma) sigma[e] = h[1];
-        if (refine) refine[e] = h[2] != 0.0;
-        if (coarsen) coarsen[e] = h[3] != 0.0;
+        mesh->est_error[e] = h[0];
+        mesh->est_sigma[e] = h[1];
+        mesh->est_refine[e] = h[2] != 0.0;
+        mesh->est_coarsen[e] = h[3] != 0.0;
+        if (error) error[e] = mesh->est_error[e];
+        if (sigma) sigma[e] = mesh->est_sigma[e];
+        if (refine) refine[e] = mesh->est_refine[e];
+        if (coarsen) coarsen[e] = mesh->est_coarsen[e];
     }
     return NDG_OK;
 }

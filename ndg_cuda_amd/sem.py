@@ -252,6 +252,38 @@ class Mesh2D_t:
         self.x_output_, self.y_output_, self.p_output_, self.u_output_, self.v_output_ = out
         return tuple(out)
 
+    def get_complete_solution(self, time: float):
+        """Mesh2D_t::get_complete_solution (src/meshes/Mesh2D_t.cu:1861-1950): a dict of the point
+        fields x, y, p, u, v, dp_dt, du_dt, dv_dt, p/u/v_analytical_error, each of shape
+        (n_elements, n, n), and the element fields N, p/u/v_error, p/u/v_sigma, refine, coarsen,
+        split_level, status, rotation, each of shape (n_elements,). dp_dt etc. are the right-hand
+        side of the current state at `time`; the analytical errors are against the reference plane
+        wave at `time`; the element errors are those of the last estimate_error."""
+        n, K = self._ndg.N_interpolation_points_, self.n_elements_
+        out = {name: np.empty((K, n, n), dtype=np.float64) for name in _abi.COMPLETE_POINT_FIELDS}
+        out.update({name: np.empty(K, dtype=dt) for name, dt in _abi.COMPLETE_ELEMENT_FIELDS})
+        cs = _abi.CompleteSolution(*(out[name].ctypes.data for name, _ in _abi.CompleteSolution._fields_))
+        check(lib().ndg_get_complete_solution(self._h, float(time), ctypes.byref(cs)))
+        self.x_output_, self.y_output_ = out["x"], out["y"]
+        self.p_output_, self.u_output_, self.v_output_ = out["p"], out["u"], out["v"]
+        return out
+
+    def write_data(self, time: float, data_writer):
+        """Mesh2D_t::write_data (src/meshes/Mesh2D_t.cpp:1601-1609): get_solution, then
+        data_writer.write_data with the output points of every element."""
+        x, y, p, u, v = self.get_interpolated_solution()
+        return data_writer.write_data(self._ndg.N_interpolation_points_, self.n_elements_, time, x, y, p, u, v)
+
+    def write_complete_data(self, time: float, data_writer):
+        """Mesh2D_t::write_complete_data (src/meshes/Mesh2D_t.cu:1861-1950):
+        get_complete_solution, then data_writer.write_complete_data (rank 0)."""
+        c = self.get_complete_solution(time)
+        return data_writer.write_complete_data(
+            self._ndg.N_interpolation_points_, self.n_elements_, time, 0, c["x"], c["y"], c["p"], c["u"], c["v"],
+            c["N"], c["dp_dt"], c["du_dt"], c["dv_dt"], c["p_error"], c["u_error"], c["v_error"], c["p_sigma"],
+            c["u_sigma"], c["v_sigma"], c["refine"], c["coarsen"], c["split_level"], c["p_analytical_error"],
+            c["u_analytical_error"], c["v_analytical_error"], c["status"], c["rotation"])
+
     def interpolate_to_boundaries(self):
         """Element traces of the current state: shape (K, 4, 3, N+1) when every element has the
         same order, else a list of K arrays of shape (4, 3, N_e+1)."""
@@ -305,10 +337,19 @@ class Solver2D_t:
 
     def solve(self, NDG: NDG_t, mesh: Mesh2D_t, data_writer=None):
         """Advances to output_times[-1] with the reference's step clipping; returns the number of
-        steps taken. data_writer, if given, is called as data_writer(time, mesh) at the reference's
-        output events (include/solvers/Solver2D_t.tpp:21-29, :74-83, :112-127), where the reference
-        writes its VTK files; it may read the mesh (get_solution, get_interpolated_solution,
-        estimate_error)."""
+        steps taken. At the reference's output events (include/solvers/Solver2D_t.tcu:21-29,
+        :74-83, :112-127), a DataWriter_t gets what the reference writes there: estimate_error
+        (after the first step) and write_complete_data. Any other callable is called as
+        data_writer(time, mesh) and may read the mesh (get_solution, get_interpolated_solution,
+        get_complete_solution, estimate_error)."""
+        from .data_writer import DataWriter_t
+        if isinstance(data_writer, DataWriter_t):
+            writer = data_writer
+
+            def data_writer(time, mesh):
+                if time > 0.0:
+                    mesh.estimate_error()
+                mesh.write_complete_data(time, writer)
         t_final, steps = ctypes.c_double(), ctypes.c_int64()
         if data_writer is None:
             check(lib().ndg_solve(mesh.handle, self.CFL_, self.output_times_[-1], ctypes.byref(t_final),

@@ -872,6 +872,44 @@ void o_get_solution(const o_mesh *m, double *x, double *y, double *p, double *u,
     }
 }
 
+/* The point fields of Mesh2D_t::get_complete_solution (src/meshes/Mesh2D_t.cu:5096-5155) with
+ * Element2D_t::interpolate_complete_solution (src/entities/Element2D_t.cu:623-665): those of
+ * o_get_solution, the element's G (what the last right-hand side evaluation left there, the
+ * reference's G_p_) and |g(x_mk, time) - q_mk| of the plane wave at the solution nodes, each
+ * interpolated with I[i][m] I[j][k]. out holds 11 planes of K n n values:
+ * x, y, p, u, v, Gp, Gu, Gv, |dp|, |du|, |dv|. */
+void o_get_complete_solution(const o_mesh *m, double time, double *out) {
+    const int n = m->t->n_interp;
+    const int64_t plane = m->K * (int64_t)n * n;
+    for (int64_t k = 0; k < m->K; ++k) {
+        const o_elem *e = &m->el[k];
+        const int nf = e->N + 1;
+        const double *I = m->t->tab[O_INTERP][e->N];
+        const double *xn = m->t->tab[O_NODES][e->N];
+        const v2 P[4] = {m->xy[e->nodes[0]], m->xy[e->nodes[1]], m->xy[e->nodes[2]], m->xy[e->nodes[3]]};
+        const double *src[6] = {e->p, e->u, e->v, e->Gp, e->Gu, e->Gv};
+        const int64_t off = k * (int64_t)n * n;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                const v2 g = quad_map((double)i / (double)(n - 1) * 2 - 1, (double)j / (double)(n - 1) * 2 - 1, P);
+                const int64_t at = off + (int64_t)i * n + j;
+                double f[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int a = 0; a <= e->N; ++a)
+                    for (int b = 0; b <= e->N; ++b) {
+                        const double wa = I[i * nf + a], wb = I[j * nf + b];
+                        for (int c = 0; c < 6; ++c) f[c] += src[c][a * nf + b] * wa * wb;
+                        const v2 xg = quad_map(xn[a], xn[b], P);
+                        double q[3];
+                        g_exact_c(xg.x, xg.y, time, m->c, q);
+                        for (int c = 0; c < 3; ++c) f[6 + c] += fabs(q[c] - src[c][a * nf + b]) * wa * wb;
+                    }
+                out[at] = g.x;
+                out[plane + at] = g.y;
+                for (int c = 0; c < 9; ++c) out[(2 + c) * plane + at] = f[c];
+            }
+    }
+}
+
 static void get_sides(const o_mesh *m, int flux, double *out) {
     int64_t off = 0;
     for (int64_t i = 0; i < m->K; ++i) {

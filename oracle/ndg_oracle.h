@@ -74,6 +74,7 @@ void o_get_rhs(const o_mesh *m, double *Gp, double *Gu, double *Gv);
 /* Mesh2D_t::get_solution: x, y, p, u, v at the tables' n x n uniform output points per element,
  * element k's point (i, j) at k*n*n + i*n + j. */
 void o_get_solution(const o_mesh *m, double *x, double *y, double *p, double *u, double *v);
+void o_get_complete_solution(const o_mesh *m, double time, double *out);
 void o_get_traces(const o_mesh *m, double *out);
 /* Element flux traces, packed [element][side][field][Nf]. */
 void o_get_flux_traces(const o_mesh *m, double *out);

@@ -43,6 +43,7 @@ def lib():
             "o_set_state": ([vp, vp, vp, vp], None), "o_get_state": ([vp, vp, vp, vp], None),
             "o_get_rhs": ([vp, vp, vp, vp], None), "o_get_traces": ([vp, vp], None),
             "o_get_solution": ([vp, vp, vp, vp, vp, vp], None),
+            "o_get_complete_solution": ([vp, ctypes.c_double, vp], None),
             "o_get_flux_traces": ([vp, vp], None), "o_get_delta_t": ([vp, f64], f64),
             "o_compute_rhs": ([vp, f64], None), "o_mesh_set_wave_speed": ([vp, f64], None),
             "o_adapt": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int64),
@@ -211,6 +212,13 @@ class Mesh:
         out = [np.empty((self.K, n, n)) for _ in range(5)]
         lib().o_get_solution(self.h, *(_p(a) for a in out))
         return tuple(out)
+
+    def get_complete_solution(self, n, time):
+        """The 11 point fields of get_complete_solution, shape (11, K, n, n): x, y, p, u, v, the
+        right-hand side left by the last compute_rhs / step, and the plane-wave errors at time."""
+        out = np.empty((11, self.K, n, n))
+        lib().o_get_complete_solution(self.h, float(time), _p(out))
+        return out
 
     def estimate_error(self):
         err, sig = np.empty(self.K), np.empty(self.K)

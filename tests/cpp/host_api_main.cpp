@@ -1,7 +1,7 @@
 // Drives the C++ mirror of the reference host API (include/ndg_sem.hpp) the way the reference's
 // src/main.cpp does: tables, mesh, initial conditions, solver, solve, solution. Writes the final
-// packed p, u, v as raw doubles to argv[1], the output-point x, y, p, u, v to argv[1].out, and
-// prints the step count and final time.
+// packed p, u, v as raw doubles to argv[1], the output-point x, y, p, u, v to argv[1].out, VTK
+// files to argv[1]_vtk/ and argv[1]_vtk2/, and prints the step count and final time.
 //   host_api_main <out.bin> <resolution> <N> <t_end> [mesh.su2]
 // The mesh is the square generator's [-1,1]^2, periodic in x and y, or the given SU2 file (the
 // reference's Mesh2D_t(path, ...) constructor), with one Gaussian pulse.
@@ -57,6 +57,15 @@ int main(int argc, char **argv) {
             std::fwrite(a->data(), sizeof(double), a->size(), g);
         std::fclose(g);
         std::printf("steps %lld time %.17g dofs %zu\n", (long long)solver.n_steps_, solver.time_, 3 * p.size());
+        // The reference's output path: the final state through DataWriter_t (write_complete_data,
+        // into argv[1]_vtk/), then a solve from t = 0 with a writer as its data_writer (output
+        // times 0 and t_end / 8, into argv[1]_vtk2/).
+        SEM::Helpers::DataWriter_t writer(std::string(argv[1]) + "_vtk/output.pvtu");
+        mesh.write_complete_data(solver.time_, NDG, writer);
+        SEM::Helpers::DataWriter_t writer2(std::string(argv[1]) + "_vtk2/output.vtu");
+        Solvers::Solver2D_t solver2(0.5, {0.0, t_end / 8}, 0.0);
+        solver2.solve(NDG, mesh, writer2);
+        std::printf("written %.17g\n", solver2.time_);
     } catch (const Error &e) {
         std::fprintf(stderr, "error %d: %s\n", e.code(), e.what());
         return 1;

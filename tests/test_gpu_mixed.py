@@ -240,6 +240,73 @@ def test_adaptive_split_solve_matches_oracle(tables, bc, ic):
         assert rel(g, r) < TOL
 
 
+def test_complete_solution(tables):
+    """Mesh2D_t::get_complete_solution on a mixed-order mesh with hanging faces, on the state 3
+    steps give: the 11 point fields (state, right-hand side at the output time, plane-wave errors)
+    against the oracle's restatement on the same state; element fields zero before the first
+    estimate, then exactly the last estimate's."""
+    say("complete solution")
+    desc, gm, om = hanging_build(tables, 8, dict(jitter=0.1), (3, 5, 6), 21)
+    gm.initial_conditions(PULSES)
+    om.initial_conditions(PULSES)
+    t = nd.Solver2D_t(0.5, [0.0]).run_steps(gm, 3)
+    om.set_state(*gm.get_solution())  # the output arithmetic on one state (steps are pinned elsewhere)
+    got = gm.get_complete_solution(t)
+    om.compute_rhs(t)
+    ref = om.get_complete_solution(5, t)
+    names = nd._abi.COMPLETE_POINT_FIELDS
+    for k, name in enumerate(names):
+        assert got[name].shape == (om.K, 5, 5)
+        if k < 2:
+            assert np.max(np.abs(got[name] - ref[k])) < 1e-15, name
+        else:
+            assert rel(got[name], ref[k]) < TOL, name
+    assert np.max(got["p_analytical_error"]) > 1e-3  # the pulses are not the plane wave
+    assert np.array_equal(got["N"], om.N)
+    _, status, rotation, _ = om.elements()
+    assert np.array_equal(got["status"], status) and np.array_equal(got["rotation"], rotation)
+    for name in ("p_error", "p_sigma", "refine", "coarsen", "u_error", "v_sigma"):
+        assert not np.any(got[name]), name
+    err, sig, ref_f, coa = gm.estimate_error()  # the estimator itself is pinned by the estimator tests
+    got = gm.get_complete_solution(t)
+    assert np.array_equal(got["p_error"], err) and np.array_equal(got["p_sigma"], sig)
+    assert np.array_equal(got["refine"], ref_f) and np.array_equal(got["coarsen"], coa)
+    assert not np.any(got["u_error"]) and not np.any(got["v_sigma"])
+
+
+def test_complete_solution_after_adaptation():
+    """Element fields across adaptations: a p-adaptation keeps the estimate of the elements it
+    leaves alone and zeroes those it raises; a splitting adaptation zeroes the children. Split
+    levels count the children; Hilbert statuses and rotations equal the oracle's."""
+    say("complete solution after adaptation")
+    desc = nd.square_mesh(8, jitter=0.1, seed=7)
+    gm = nd.Mesh2D_t(desc, 3, 9, 5, 1, 0, 0, 1e-4, 1e-14, 0.0, nd.NDG_t(9, 5))
+    gm.initial_conditions(PULSES)
+    nd.Solver2D_t(0.5, [0.0]).run_steps(gm, 4)
+    err, sig, refine, coarsen = gm.estimate_error()
+    before = gm.orders_.copy()
+    gm.adapt(9, 0)
+    c = gm.get_complete_solution(0.0)
+    raised = gm.orders_ != before
+    assert raised.any() and not raised.all()
+    assert np.array_equal(c["p_error"][~raised], err[~raised]) and not np.any(c["p_error"][raised])
+    assert np.array_equal(c["refine"][~raised], refine[~raised].astype(np.int32))
+
+    om = O.Mesh(O.Tables(9, 5), desc, 3, tol_min=1e-4, tol_max=1e-14)
+    om.initial_conditions(PULSES)
+    gm2 = nd.Mesh2D_t(desc, 3, 9, 5, 1, 4, 0, 1e-4, 1e-14, 0.0, nd.NDG_t(9, 5))
+    gm2.initial_conditions(PULSES)
+    solver = nd.Solver2D_t(0.5, [0.2])
+    solver.solve(None, gm2)
+    om.solve_adaptive(0.5, 0.2, 4, 9, 1)
+    c = gm2.get_complete_solution(solver.time_)
+    assert gm2.n_elements_ == om.K > 64
+    _, status, rotation, _ = om.elements()
+    assert np.array_equal(c["status"], status) and np.array_equal(c["rotation"], rotation)
+    assert set(np.unique(c["split_level"])) <= {0, 1}
+    assert 3 * int(np.sum(c["split_level"] == 1)) == 4 * (om.K - 64)
+
+
 HANGING = [
     ("periodic-uniform", dict(periodic=True, jitter=0.15), (5,)),
     ("periodic-mixed", dict(periodic=True, jitter=0.1), (3, 5, 6)),

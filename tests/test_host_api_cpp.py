@@ -45,6 +45,26 @@ def test_driver_matches_oracle(tmp_path):
     outs = np.fromfile(str(tmp_path / "o.bin") + ".out", dtype=np.float64).reshape(5, res * res, 5, 5)
     for g, ref in zip(outs, om.get_solution(5)):
         assert np.max(np.abs(g - ref)) <= 1e-12 * max(np.max(np.abs(ref)), 1.0)
+    # DataWriter_t files: write_complete_data of the final state, then a solve with the writer
+    import json
+    from ndg_cuda_amd.data_writer import read_vtu
+    vtk = tmp_path / "o.bin_vtk"
+    pts = read_vtu(vtk / "output_t0_500000000s.pvtu")
+    assert np.array_equal(pts["point_data"]["Pressure"], outs[2].ravel())
+    assert np.array_equal(pts["points"][:, 0], outs[0].ravel())
+    assert np.array_equal(pts["point_data"]["Velocity"][:, 1], outs[4].ravel())
+    els = read_vtu(vtk / "output_element_t0_500000000s.pvtu")
+    assert np.all(els["cell_data"]["N"] == N) and np.array_equal(els["cell_data"]["Index"], np.arange(res * res))
+    assert not np.any(els["cell_data"]["PressureError"])  # no estimate was made
+    series = json.loads((vtk / "output.pvtu.series").read_text())
+    assert series["files"] == [{"name": "output_t0_500000000s.pvtu", "time": 0.5}]
+    written = float(lines[2].split()[1])
+    files = json.loads((tmp_path / "o.bin_vtk2" / "output.vtu.series").read_text())["files"]
+    assert [f["time"] for f in files] == [0.0, written] and written >= t_end / 8
+    late = read_vtu(tmp_path / "o.bin_vtk2" / files[1]["name"])
+    el2 = read_vtu(tmp_path / "o.bin_vtk2" / files[1]["name"].replace("output_", "output_element_"))
+    assert np.any(el2["cell_data"]["PressureError"])  # estimated before the write after a step
+    assert late["point_data"]["PressureDerivative"].shape == (res * res * 25,)
 
 
 @pytest.mark.gpu
