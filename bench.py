@@ -304,9 +304,16 @@ def adjoint_leg(nd, torch, stream, steps=2000):
 
     fd = (misfit(1) - misfit(-1)) / (2 * eps)
     ad = float(gp @ d[0] + gu @ d[1] + gv @ d[2]) + gc
+    ad3 = float(gp3 @ d[0] + gu3 @ d[1] + gv3 @ d[2]) + gc3
+    stage_records["fd_rel_diff"] = abs(ad3 - fd) / abs(fd)
     mesh.close()
+    ratios = {"checkpoint_recompute": t_adj / t_fwd, "stage_records": t_rec / t_fwd}
+    if kept["fits"]:
+        ratios["all_segments_kept"] = kept["gradient_over_forward"]
+    best = min(ratios, key=ratios.get)
     return {"workload": f"{res}x{res} periodic, N={n}, {steps} steps, checkpoints every {interval}, 32 receivers",
             "J": J, "dJ_dc": gc, "forward_s": t_fwd, "gradient_s": t_adj, "gradient_over_forward": t_adj / t_fwd,
+            "gradient_over_forward_best": ratios[best], "best_method": best,
             "all_segments_kept": kept, "c_derivative_from_stage_records": stage_records,
             "fd_rel_diff": abs(ad - fd) / abs(fd),
             "note": "wall times of the library calls (host-synchronous), forward = one recorded 2000-step run"}

@@ -2,7 +2,7 @@
 
     python -m ndg_cuda_amd.profiling trace <run_results.db> [--kernel SUBSTR] [--min-us T] [--max-us T]
     python -m ndg_cuda_amd.profiling traffic --fetch DB --write DB --copy-fetch DB --copy-write DB
-    python -m ndg_cuda_amd.profiling counters DB [DB ...] [--kernel SUBSTR]
+    python -m ndg_cuda_amd.profiling counters DB [DB ...] [--kernel SUBSTR] [--by-grid]
 
 `trace` prints per-kernel call counts and durations from a --kernel-trace run. `traffic` turns one
 FETCH_SIZE run and one WRITE_SIZE run of the benchmark into HBM bytes per launch of the kernel,
@@ -39,11 +39,24 @@ def kernel_stats(path, kernel=None, min_us=0.0, max_us=float("inf")):
     return out
 
 
-def counter_means(path, kernel):
+def counter_means(path, kernel, grid=None):
+    """Mean counter value per dispatch of the kernel; with `grid`, only dispatches of that grid
+    size (threads), i.e. one launch shape (the autotuner launches several)."""
     c = _db(path)
-    rows = c.execute("select counter_name, count(*), avg(value) from counters_collection "
-                     "where kernel_name like ? group by counter_name", (f"%{kernel}%",)).fetchall()
+    sql = "select counter_name, count(*), avg(value) from counters_collection where kernel_name like ?"
+    args = [f"%{kernel}%"]
+    if grid is not None:
+        sql += " and grid_size = ?"
+        args.append(int(grid))
+    rows = c.execute(sql + " group by counter_name", args).fetchall()
     return {name: {"dispatches": n, "mean": v} for name, n, v in rows}
+
+
+def grid_sizes(path, kernel):
+    """(grid size, dispatches) of every launch shape of the kernel in a counter run."""
+    c = _db(path)
+    return c.execute("select grid_size, count(distinct dispatch_id) from counters_collection "
+                     "where kernel_name like ? group by grid_size order by grid_size", (f"%{kernel}%",)).fetchall()
 
 
 def traffic(fetch, write, copy_fetch, copy_write, kernel, copy_kernel="copyBuffer", copy_bytes=1 << 30):
@@ -74,6 +87,7 @@ def main(argv=None):
     cc = sub.add_parser("counters")
     cc.add_argument("dbs", nargs="+")
     cc.add_argument("--kernel", default="stage_kernel<8, false>")
+    cc.add_argument("--by-grid", action="store_true", help="one block per launch shape (grid size)")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         for r in kernel_stats(a.db, a.kernel, a.min_us, a.max_us):
@@ -83,8 +97,12 @@ def main(argv=None):
         print(json.dumps(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.kernel), indent=1))
     else:
         for db in a.dbs:
-            for name, v in sorted(counter_means(db, a.kernel).items()):
-                print(f"{name:28s} {v['mean']:.6g}  ({v['dispatches']} dispatches)")
+            grids = [(g, n) for g, n in grid_sizes(db, a.kernel)] if a.by_grid else [(None, None)]
+            for g, n in grids:
+                if g is not None:
+                    print(f"-- grid {g} threads ({n} dispatches)")
+                for name, v in sorted(counter_means(db, a.kernel, g).items()):
+                    print(f"{name:28s} {v['mean']:.6g}  ({v['dispatches']} dispatches)")
     return 0
 
 
