@@ -112,6 +112,7 @@ struct DeviceMesh {
     int64_t n_bc = 0;
     MortarFace *faces = nullptr; // [n_faces]
     int64_t n_faces = 0;
+    int mortar_nodes = 0;        // largest N + 1 over the faces' NL, NR and Nf (sizes the mortar LDS)
     double *side_flux = nullptr; // projected mortar side fluxes
     float *side_flux32 = nullptr;
     int64_t n_side_flux = 0;

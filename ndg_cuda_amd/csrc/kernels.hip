@@ -987,12 +987,13 @@ __device__ __forceinline__ void stage_to_lds(const T *__restrict__ src, double *
 // stages its element trace and its interpolation and projection matrices (built on the host with
 // the reference's exact-node shortcuts, see MortarFace) into LDS with all loads in flight, then
 // lane i computes face node i (interpolation) and element node i (projection); face states and
-// flux go through LDS.
-template <typename T>
+// flux go through LDS. M bounds N + 1 of every face and side (checked on the host when the kernel
+// is chosen): a smaller M gives a smaller LDS footprint and more faces resident per CU.
+template <typename T, int M>
 __global__ void __launch_bounds__(64)
 mortar_kernel(const MortarFace *__restrict__ faces, const double *__restrict__ mats,
               const T *__restrict__ tr, T *__restrict__ side_flux) {
-    constexpr int M = kMaxOrder + 1, kMat = M * M, kMatItems = (kMat + 31) / 32, kTrItems = (3 * M + 31) / 32;
+    constexpr int kMat = M * M, kMatItems = (kMat + 31) / 32, kTrItems = (3 * M + 31) / 32;
     __shared__ double s_tr[2][3 * M];   // element traces, left and right
     __shared__ double s_mi[2][kMat];    // interpolation matrices [face node][element node]
     __shared__ double s_mp[2][kMat];    // projection matrices [element node][face node]
@@ -1357,16 +1358,25 @@ hipError_t launch_traces(const DeviceMesh &m, hipStream_t s) {
     });
 }
 
-hipError_t launch_mortar(DeviceMesh &m, hipStream_t s) {
-    if (m.n_faces == 0) return hipSuccess;
+template <int M>
+static hipError_t mortar_m(DeviceMesh &m, hipStream_t s) {
     const unsigned blocks = (unsigned)m.n_faces;
     if (m.precision == NDG_PRECISION_FP32_COMPENSATED)
-        hipLaunchKernelGGL(mortar_kernel<float>, dim3(blocks), dim3(64), 0, s, m.faces, m.mortar_mats,
+        hipLaunchKernelGGL((mortar_kernel<float, M>), dim3(blocks), dim3(64), 0, s, m.faces, m.mortar_mats,
                            m.trace32[m.cur], m.side_flux32);
     else
-        hipLaunchKernelGGL(mortar_kernel<double>, dim3(blocks), dim3(64), 0, s, m.faces, m.mortar_mats,
+        hipLaunchKernelGGL((mortar_kernel<double, M>), dim3(blocks), dim3(64), 0, s, m.faces, m.mortar_mats,
                            m.trace[m.cur], m.side_flux);
     return hipGetLastError();
+}
+
+hipError_t launch_mortar(DeviceMesh &m, hipStream_t s) {
+    if (m.n_faces == 0) return hipSuccess;
+    static_assert(kMaxOrder + 1 <= 32, "a mortar side is handled by one 32-lane half wave");
+    if (m.mortar_nodes < 1 || m.mortar_nodes > kMaxOrder + 1) return hipErrorInvalidValue;
+    if (m.mortar_nodes <= 9) return mortar_m<9>(m, s);
+    if (m.mortar_nodes <= 13) return mortar_m<13>(m, s);
+    return mortar_m<kMaxOrder + 1>(m, s);
 }
 
 hipError_t launch_stage(DeviceMesh &m, int stage, double a, double gdt, hipStream_t s) {

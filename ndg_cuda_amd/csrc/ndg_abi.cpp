@@ -526,6 +526,9 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
     if ((st = device_upload(&d.faces, faces, "upload mortar faces")) != NDG_OK) return st;
     if ((st = device_upload(&d.mortar_mats, mats.data(), "upload mortar matrices")) != NDG_OK) return st;
     d.n_faces = (int64_t)faces.size();
+    d.mortar_nodes = 0;
+    for (const MortarFace &fc : faces)
+        d.mortar_nodes = std::max(d.mortar_nodes, std::max(fc.Nf, std::max(fc.NL, fc.NR)) + 1);
     d.n_side_flux = n_slot;
     if (n_slot > 0) HIP_TRY(hipMalloc((void **)&d.side_flux, sizeof(double) * n_slot), "hipMalloc side fluxes");
     {
