@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 output (rocpd SQLite databases) for the stage kernels.
 
-    python -m ndg_cuda_amd.profiling trace <run_results.db> [--kernel SUBSTR] [--min-us T] [--max-us T]
+    python -m ndg_cuda_amd.profiling trace <run_results.db> [--kernel SUBSTR] [--min-us T] [--max-us T] [--grid G]
     python -m ndg_cuda_amd.profiling traffic --fetch DB --write DB --copy-fetch DB --copy-write DB
     python -m ndg_cuda_amd.profiling counters DB [DB ...] [--kernel SUBSTR] [--by-grid]
 
@@ -26,11 +26,17 @@ def _db(path):
     return sqlite3.connect(path)
 
 
-def kernel_stats(path, kernel=None, min_us=0.0, max_us=float("inf")):
+def kernel_stats(path, kernel=None, min_us=0.0, max_us=float("inf"), grid=None):
+    """Per-kernel call counts and durations; with `grid`, only dispatches of that grid size
+    (threads), i.e. one launch shape."""
     c = _db(path)
-    rows = c.execute("select name, count(*), avg(duration), min(duration), max(duration), sum(duration) "
-                     "from kernels where duration >= ? and duration <= ? group by name order by sum(duration) desc",
-                     (min_us * 1e3, min(max_us * 1e3, 1e18))).fetchall()
+    sql = ("select name, count(*), avg(duration), min(duration), max(duration), sum(duration) "
+           "from kernels where duration >= ? and duration <= ?")
+    args = [min_us * 1e3, min(max_us * 1e3, 1e18)]
+    if grid is not None:
+        sql += " and grid_x * grid_y * grid_z = ?"
+        args.append(int(grid))
+    rows = c.execute(sql + " group by name order by sum(duration) desc", args).fetchall()
     out = []
     for name, n, avg, mn, mx, tot in rows:
         if kernel and kernel not in name:
@@ -80,6 +86,7 @@ def main(argv=None):
     t.add_argument("--kernel")
     t.add_argument("--min-us", type=float, default=0.0, help="only dispatches at least this long")
     t.add_argument("--max-us", type=float, default=float("inf"), help="only dispatches at most this long")
+    t.add_argument("--grid", type=int, help="only dispatches of this grid size (threads)")
     tr = sub.add_parser("traffic")
     for k in ("fetch", "write", "copy_fetch", "copy_write"):
         tr.add_argument("--" + k.replace("_", "-"), required=True)
@@ -90,7 +97,7 @@ def main(argv=None):
     cc.add_argument("--by-grid", action="store_true", help="one block per launch shape (grid size)")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
-        for r in kernel_stats(a.db, a.kernel, a.min_us, a.max_us):
+        for r in kernel_stats(a.db, a.kernel, a.min_us, a.max_us, a.grid):
             print(f"{r['calls']:6d} calls  avg {r['avg_ns'] / 1e3:10.2f} us  min {r['min_ns'] / 1e3:10.2f}  "
                   f"max {r['max_ns'] / 1e3:10.2f}  total {r['total_ns'] / 1e6:9.3f} ms  {r['kernel'][:90]}")
     elif a.cmd == "traffic":
