@@ -625,15 +625,32 @@ ndg_status build_topology(const HostMesh &h, Topology &t) {
     auto P = [&](int64_t node) { return v2{h.xy[2 * node], h.xy[2 * node + 1]}; };
     auto corner = [&](int64_t e, int k) { return P(h.quads[4 * e + (k & 3)]); };
 
-    // Directed edges of the domain elements (build_faces, src/meshes/Mesh2D_t.cpp:1110-1154).
-    std::unordered_map<uint64_t, int64_t> edges;
-    edges.reserve((size_t)(4 * K) * 2);
-    for (int64_t e = 0; e < K; ++e)
-        for (int s = 0; s < 4; ++s) {
-            const int64_t a = h.quads[4 * e + s], b = h.quads[4 * e + (s + 1) % 4];
-            if (!edges.emplace(edge_key(a, b), e * 4 + s).second)
-                return fail(NDG_ERR_INVALID, "mesh topology: an edge is used twice with the same orientation");
-        }
+    // Directed edges of the domain elements (build_faces, src/meshes/Mesh2D_t.cpp:1110-1154): the
+    // element sides bucketed by start node (CSR, sides in increasing order within a bucket), so the
+    // side a -> b is found by scanning the few sides that start at a.
+    const int64_t n_nodes = (int64_t)h.xy.size() / 2;
+    for (int64_t v : h.quads)
+        if (v < 0 || v >= n_nodes) return fail(NDG_ERR_INVALID, "mesh topology: element node id out of range");
+    for (int64_t v : h.bars)
+        if (v < 0 || v >= n_nodes) return fail(NDG_ERR_INVALID, "mesh topology: boundary node id out of range");
+    auto end_node = [&](int64_t es) { return h.quads[4 * (es / 4) + (es % 4 + 1) % 4]; };
+    std::vector<int64_t> first((size_t)n_nodes + 1, 0), by_start((size_t)(4 * K));
+    for (int64_t es = 0; es < 4 * K; ++es) ++first[(size_t)h.quads[es] + 1];
+    for (int64_t v = 0; v < n_nodes; ++v) first[(size_t)v + 1] += first[(size_t)v];
+    {
+        std::vector<int64_t> fill(first.begin(), first.end() - 1);
+        for (int64_t es = 0; es < 4 * K; ++es) by_start[(size_t)fill[(size_t)h.quads[es]]++] = es;
+    }
+    auto find_side = [&](int64_t a, int64_t b) -> int64_t {  // the element side a -> b, or -1
+        for (int64_t i = first[(size_t)a]; i < first[(size_t)a + 1]; ++i)
+            if (end_node(by_start[(size_t)i]) == b) return by_start[(size_t)i];
+        return -1;
+    };
+    for (int64_t a = 0; a < n_nodes; ++a)
+        for (int64_t i = first[(size_t)a]; i < first[(size_t)a + 1]; ++i)
+            for (int64_t j = first[(size_t)a]; j < i; ++j)
+                if (end_node(by_start[(size_t)i]) == end_node(by_start[(size_t)j]))
+                    return fail(NDG_ERR_INVALID, "mesh topology: an edge is used twice with the same orientation");
     // link[e*4+s]: the element side whose trace is the exterior state of side s of element e (a
     // neighbour, or the donor side of a periodic pair), or -(1 + boundary record) / INT64_MIN.
     std::vector<int64_t> &link = t.link;
@@ -642,25 +659,23 @@ ndg_status build_topology(const HostMesh &h, Topology &t) {
     periodic_link.assign((size_t)(4 * K), 0);
     int64_t &n_faces = t.n_faces;
     n_faces = 0;
-    for (int64_t e = 0; e < K; ++e)
-        for (int s = 0; s < 4; ++s) {
-            const int64_t a = h.quads[4 * e + s], b = h.quads[4 * e + (s + 1) % 4];
-            auto it = edges.find(edge_key(b, a));
-            if (it != edges.end()) {
-                link[4 * e + s] = it->second;
-                if (it->second > e * 4 + s) ++n_faces;
-            }
+    for (int64_t es = 0; es < 4 * K; ++es) {
+        const int64_t r = find_side(end_node(es), h.quads[es]);
+        if (r >= 0) {
+            link[es] = r;
+            if (r > es) ++n_faces;
         }
+    }
     // Ghost (bar) elements attach to the domain side with the same two nodes
     // (src/meshes/Mesh2D_t.cpp:1156-1181). Their nodes after attachment are {P(s+1), P(s)}.
     std::vector<int64_t> &bar_side = t.bar_side;
     bar_side.assign((size_t)B, -1);
     for (int64_t b = 0; b < B; ++b) {
         const int64_t a = h.bars[2 * b], c = h.bars[2 * b + 1];
-        auto it = edges.find(edge_key(c, a));
-        if (it == edges.end()) it = edges.find(edge_key(a, c));
-        if (it == edges.end()) return fail(NDG_ERR_INVALID, "mesh topology: boundary bar matches no element edge");
-        bar_side[b] = it->second;
+        int64_t es = find_side(c, a);
+        if (es < 0) es = find_side(a, c);
+        if (es < 0) return fail(NDG_ERR_INVALID, "mesh topology: boundary bar matches no element edge");
+        bar_side[b] = es;
         ++n_faces;
     }
     std::vector<BcSide> &bcs = t.bcs;
@@ -718,27 +733,23 @@ ndg_status build_topology(const HostMesh &h, Topology &t) {
     hang_kind.assign((size_t)(4 * K), 0);
     hang_second.assign((size_t)(4 * K), -1);
     {
-        std::unordered_map<int64_t, std::vector<int64_t>> out_edges;  // node -> element sides starting there
-        for (int64_t es = 0; es < 4 * K; ++es)
-            if (link[es] == INT64_MIN) out_edges[h.quads[es]].push_back(es);
-        auto end_node = [&](int64_t es) { return h.quads[4 * (es / 4) + (es % 4 + 1) % 4]; };
+        // sides still unlinked, by start node (the buckets above, linked sides skipped)
         for (int64_t es = 0; es < 4 * K; ++es) {
             if (link[es] != INT64_MIN || hang_kind[es]) continue;
             const int64_t a = h.quads[es], b = end_node(es);
-            auto it = out_edges.find(b);
-            if (it == out_edges.end()) continue;
-            for (int64_t s1 : it->second) {  // b -> m
+            for (int64_t ib = first[(size_t)b]; ib < first[(size_t)b + 1]; ++ib) {  // b -> m
+                const int64_t s1 = by_start[(size_t)ib];
                 if (s1 == es || hang_kind[s1] || link[s1] != INT64_MIN) continue;
                 const int64_t mnode = end_node(s1);
                 const v2 pa = P(a), pb = P(b), pm = P(mnode);
                 const double len = mag(pb.x - pa.x, pb.y - pa.y);
                 if (std::fabs(pm.x - (pa.x + pb.x) / 2) > 1e-12 * len || std::fabs(pm.y - (pa.y + pb.y) / 2) > 1e-12 * len)
                     continue;
-                auto jt = out_edges.find(mnode);
-                if (jt == out_edges.end()) continue;
                 int64_t s2 = -1;  // m -> a
-                for (int64_t c : jt->second)
+                for (int64_t ic = first[(size_t)mnode]; ic < first[(size_t)mnode + 1]; ++ic) {
+                    const int64_t c = by_start[(size_t)ic];
                     if (end_node(c) == a && !hang_kind[c] && link[c] == INT64_MIN) s2 = c;
+                }
                 if (s2 < 0) continue;
                 hang_kind[es] = 2;
                 link[es] = s2;          // the half starting at a first

@@ -113,3 +113,35 @@ def test_mesh_create_without_gpu_reports_no_device():
     with pytest.raises(nd.NDGError) as e:
         nd.Mesh2D_t(nd.square_mesh(4), 4, 8, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
     assert e.value.code == _abi.NDG_ERR_NO_DEVICE
+
+
+@pytest.mark.skipif(nd.device_count() > 0, reason="the host topology checks run before the no-device error")
+def test_mesh_topology_errors_and_hanging_faces_on_the_host():
+    """ndg_mesh_create builds the topology on the host before it looks for a device: malformed
+    descriptions are refused with their own message, and a mesh with 2:1 hanging faces passes
+    (reaching the no-device error), so every side found a neighbour, a half face or a bar."""
+    import meshes
+    ndg = nd.NDG_t(8, 5)
+
+    def create(desc):
+        with pytest.raises(nd.NDGError) as e:
+            nd.Mesh2D_t(desc, 4, 8, 5, 1, 0, 0, 1e-6, 1e-14, 0.0, ndg)
+        return e.value
+
+    base = nd.square_mesh(8)
+    split = meshes.split_elements(base, meshes.interior_elements(base)[::3])
+    assert create(split).code == _abi.NDG_ERR_NO_DEVICE
+    twice = nd.square_mesh(8)
+    twice.quads = np.concatenate([twice.quads, twice.quads[:1]])
+    err = create(twice)
+    assert err.code == _abi.NDG_ERR_INVALID and "used twice" in str(err)
+    stray = nd.square_mesh(8)
+    stray.bars = stray.bars.copy()
+    stray.bars[0] = [stray.quads[0, 0], stray.quads[0, 2]]  # a diagonal, no element edge
+    err = create(stray)
+    assert err.code == _abi.NDG_ERR_INVALID and "matches no element edge" in str(err)
+    open_side = nd.square_mesh(8)
+    keep = np.arange(1, len(open_side.bars))
+    open_side.bars, open_side.bar_type = open_side.bars[keep], open_side.bar_type[keep]
+    err = create(open_side)
+    assert err.code == _abi.NDG_ERR_INVALID and "neither a neighbour nor a boundary bar" in str(err)
