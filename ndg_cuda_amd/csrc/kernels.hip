@@ -1004,7 +1004,8 @@ mortar_kernel(const MortarFace *__restrict__ faces, const double *__restrict__ m
     const int side = lane >> 5, i = lane & 31;
     const int Nf = fc.Nf;
     const int Ne = side ? fc.NR : fc.NL;
-    const int64_t mi = fc.mat[side], mp = fc.mat[2 + side];
+    // selects, not fc.mat[side]: a lane-dependent index would put the whole record in scratch
+    const int64_t mi = side ? fc.mat[1] : fc.mat[0], mp = side ? fc.mat[3] : fc.mat[2];
     stage_to_lds<T, kTrItems>(tr + (side ? fc.trR : fc.trL), s_tr[side], 3 * (Ne + 1), i);
     if (mi >= 0) stage_to_lds<double, kMatItems>(mats + mi, s_mi[side], (Nf + 1) * (Ne + 1), i);
     if (mp >= 0) stage_to_lds<double, kMatItems>(mats + mp, s_mp[side], (Ne + 1) * (Nf + 1), i);
@@ -1019,12 +1020,17 @@ mortar_kernel(const MortarFace *__restrict__ faces, const double *__restrict__ m
             v1 = te[(Ne + 1) + k];
             v2 = te[2 * (Ne + 1) + k];
         } else {
+            // unrolled to M with every LDS read issued unconditionally (all in bounds: i <= Nf < M,
+            // j < M), only the terms j <= Ne accumulated, in the same order
             const double *row = s_mi[side] + i * (Ne + 1);
-            for (int j = 0; j <= Ne; ++j) {
-                const double w = row[j];
-                v0 += w * te[j];
-                v1 += w * te[(Ne + 1) + j];
-                v2 += w * te[2 * (Ne + 1) + j];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                const double w = row[j], a0 = te[j], a1 = te[(Ne + 1) + j], a2 = te[2 * (Ne + 1) + j];
+                if (j <= Ne) {
+                    v0 += w * a0;
+                    v1 += w * a1;
+                    v2 += w * a2;
+                }
             }
         }
         s_st[side][0][i] = v0;
@@ -1054,11 +1060,14 @@ mortar_kernel(const MortarFace *__restrict__ faces, const double *__restrict__ m
     }
     const double *row = s_mp[side] + i * (Nf + 1);
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int j = 0; j <= Nf; ++j) {
-        const double w = row[j];
-        a0 += w * s_f[0][j];
-        a1 += w * s_f[1][j];
-        a2 += w * s_f[2][j];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {  // as the interpolation: i <= Ne < M, j < M
+        const double w = row[j], f0 = s_f[0][j], f1 = s_f[1][j], f2 = s_f[2][j];
+        if (j <= Nf) {
+            a0 += w * f0;
+            a1 += w * f1;
+            a2 += w * f2;
+        }
     }
     out[i] = (T)(a0 * sf);
     out[(Ne + 1) + i] = (T)(a1 * sf);

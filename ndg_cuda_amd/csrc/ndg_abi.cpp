@@ -377,11 +377,19 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
     d.K = K;
     m->order = order;
 
-    // Order groups: elements sorted by N (stable, so each group keeps the caller's order, e.g. a
-    // space-filling curve).
+    // Order groups: elements sorted by N with a counting sort (stable, so each group keeps the
+    // caller's order, e.g. a space-filling curve).
     std::vector<int64_t> perm((size_t)K);
-    for (int64_t e = 0; e < K; ++e) perm[e] = e;
-    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return order[a] < order[b]; });
+    {
+        std::vector<int64_t> at(kMaxOrder + 2, 0);
+        for (int64_t e = 0; e < K; ++e) {
+            if (order[e] < 1 || order[e] > kMaxOrder)
+                return fail(NDG_ERR_UNSUPPORTED, "element order out of the compiled range");
+            ++at[(size_t)order[e] + 1];
+        }
+        for (int n = 0; n <= kMaxOrder; ++n) at[(size_t)n + 1] += at[(size_t)n];
+        for (int64_t e = 0; e < K; ++e) perm[(size_t)at[(size_t)order[e]]++] = e;
+    }
     m->dev_of.assign((size_t)K, 0);
     for (int64_t k = 0; k < K; ++k) m->dev_of[perm[k]] = k;
     std::vector<int> group_of_N(kMaxOrder + 1, -1);
