@@ -145,6 +145,24 @@ ndg_status ndg_su2_sizes(const char *path, int64_t *n_nodes, int64_t *n_quads, i
  * reference's ghost order (walls, symmetry, inflow, outflow, each in file order). Any pointer may
  * be NULL. */
 ndg_status ndg_su2_fill(const char *path, double *xy, int64_t *quads, int64_t *bars, int32_t *bar_type);
+/* Periodic boundaries for SU2 meshes. The reference has none (its SU2 reader has no
+ * self-interfaces, src/meshes/Mesh2D_t.cpp:398-400; periodic meshes come from CGNS). As SU2 itself
+ * does in its configuration (MARKER_PERIODIC), pairs of marker tags name the boundaries joined by
+ * a translation. Those tags (case-sensitive) are read as NDG_BC_PERIODIC bars, after the others,
+ * pair after pair (all bars of marker_a[i], then all of marker_b[i], each in file order). Each
+ * bar's donor (bar_donor, as ndg_mesh_desc takes it) is the bar of the other marker whose midpoint
+ * is its own moved by the difference of the two markers' bar-midpoint centroids; a bar without
+ * exactly one such partner (within 1e-8 of its length) is an error. n_pairs = 0 or a NULL pointer
+ * reads as ndg_su2_sizes / ndg_su2_fill do. */
+typedef struct ndg_su2_periodic {
+    int32_t n_pairs;
+    const char *const *marker_a;
+    const char *const *marker_b;
+} ndg_su2_periodic;
+ndg_status ndg_su2_sizes_periodic(const char *path, const ndg_su2_periodic *periodic, int64_t *n_nodes,
+                                  int64_t *n_quads, int64_t *n_bars);
+ndg_status ndg_su2_fill_periodic(const char *path, const ndg_su2_periodic *periodic, double *xy, int64_t *quads,
+                                 int64_t *bars, int32_t *bar_type, int64_t *bar_donor);
 
 /* ---- Mesh2D_t: include/meshes/Mesh2D_t.h:21 (constructor), :105-128 (phase methods) ---- */
 ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_tables *tables, ndg_mesh **out);

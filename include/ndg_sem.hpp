@@ -26,6 +26,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "ndg_amd.h"
@@ -135,17 +136,27 @@ inline MeshDescription square_mesh(const ndg_square_mesh_params &params) {
 
 // Mesh2D_t::read_su2 (src/meshes/Mesh2D_t.cpp:78-330): nodes, QUAD_4 elements and the
 // farfield / wall / inlet / symmetry markers of an SU2 file (bars in the reference's ghost order).
-inline MeshDescription read_su2(const std::filesystem::path &filename) {
+// With `periodic`, the named marker pairs are joined by a translation, as SU2's MARKER_PERIODIC
+// does (ndg_su2_fill_periodic): their bars come last, as NDG_BC_PERIODIC with donors.
+inline MeshDescription read_su2(const std::filesystem::path &filename,
+                                const std::vector<std::pair<std::string, std::string>> &periodic = {}) {
     const std::string path = filename.string();
+    std::vector<const char *> tags_a, tags_b;
+    for (const auto &pr : periodic) {
+        tags_a.push_back(pr.first.c_str());
+        tags_b.push_back(pr.second.c_str());
+    }
+    const ndg_su2_periodic per{(int32_t)periodic.size(), tags_a.data(), tags_b.data()};
     int64_t nn = 0, nq = 0, nb = 0;
-    check(ndg_su2_sizes(path.c_str(), &nn, &nq, &nb));
+    check(ndg_su2_sizes_periodic(path.c_str(), &per, &nn, &nq, &nb));
     MeshDescription m;
     m.xy.resize(2 * nn);
     m.quads.resize(4 * nq);
     m.bars.resize(2 * nb);
     m.bar_type.resize(nb);
-    m.bar_donor.assign(nb, -1);
-    check(ndg_su2_fill(path.c_str(), m.xy.data(), m.quads.data(), m.bars.data(), m.bar_type.data()));
+    m.bar_donor.resize(nb);
+    check(ndg_su2_fill_periodic(path.c_str(), &per, m.xy.data(), m.quads.data(), m.bars.data(), m.bar_type.data(),
+                                m.bar_donor.data()));
     return m;
 }
 

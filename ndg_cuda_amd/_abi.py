@@ -33,7 +33,7 @@ TABLE_NAMES = ["nodes", "weights", "barycentric_weights", "lagrange_interpolant_
 
 # Every symbol include/ndg_amd.h declares (checked by the CPU tests).
 EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tables_destroy",
-            "ndg_tables_get", "ndg_square_mesh_sizes", "ndg_square_mesh_fill", "ndg_su2_sizes", "ndg_su2_fill",
+            "ndg_tables_get", "ndg_square_mesh_sizes", "ndg_square_mesh_fill", "ndg_su2_sizes", "ndg_su2_fill", "ndg_su2_sizes_periodic", "ndg_su2_fill_periodic",
             "ndg_mesh_create",
             "ndg_mesh_destroy", "ndg_mesh_set_stream", "ndg_mesh_info", "ndg_mesh_set_precision", "ndg_set_ic",
             "ndg_set_state", "ndg_get_state", "ndg_get_solution", "ndg_get_complete_solution", "ndg_get_traces", "ndg_compute_rhs",
@@ -83,6 +83,12 @@ class CompleteSolution(ctypes.Structure):
                [(name, ctypes.c_void_p) for name, _ in COMPLETE_ELEMENT_FIELDS]
 
 
+class SU2Periodic(ctypes.Structure):
+    """ndg_su2_periodic: n_pairs marker-tag pairs joined by a translation."""
+    _fields_ = [("n_pairs", ctypes.c_int32), ("marker_a", ctypes.POINTER(ctypes.c_char_p)),
+                ("marker_b", ctypes.POINTER(ctypes.c_char_p))]
+
+
 class ICDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("n_pulses", ctypes.c_int32), ("pulses", ctypes.c_void_p)]
 
@@ -109,6 +115,8 @@ def lib():
             "ndg_square_mesh_fill": [vp, vp, vp, vp, vp, vp],
             "ndg_su2_sizes": [ctypes.c_char_p, vp, vp, vp],
             "ndg_su2_fill": [ctypes.c_char_p, vp, vp, vp, vp],
+            "ndg_su2_sizes_periodic": [ctypes.c_char_p, vp, vp, vp, vp],
+            "ndg_su2_fill_periodic": [ctypes.c_char_p, vp, vp, vp, vp, vp, vp],
             "ndg_mesh_create": [vp, vp, vp],
             "ndg_mesh_destroy": [vp],
             "ndg_mesh_set_stream": [vp, vp],

@@ -8,13 +8,17 @@ that holds them onto the square [-1, 1]^2 and sorts them by their Hilbert index 
 of the solver, so it sets memory locality on the GPU.
 
     python -m ndg_cuda_amd.mesh_tools renumber -i in.su2 -o out.su2 [-a circular|hilbert-circular]
+                                             [-p TAG_A:TAG_B ...]
+
+SU2 files carry no periodic pairing; write_su2 names each pair of periodic boundaries
+periodic_<k>_a / periodic_<k>_b and read_su2(path, periodic=[(a, b), ...]) joins them again.
 """
 import argparse
 import sys
 
 import numpy as np
 
-from .sem import BC_INFLOW, BC_OUTFLOW, BC_SYMMETRY, BC_WALL, MeshDescription, read_su2
+from .sem import BC_INFLOW, BC_OUTFLOW, BC_PERIODIC, BC_SYMMETRY, BC_WALL, MeshDescription, read_su2
 
 SU2_MARKERS = {BC_WALL: "wall", BC_SYMMETRY: "symmetry", BC_INFLOW: "inlet", BC_OUTFLOW: "farfield"}
 
@@ -87,24 +91,68 @@ def renumber(desc: MeshDescription, algorithm: str = "circular") -> MeshDescript
                            np.asarray(desc.bar_donor).copy(), quad_N)
 
 
-def write_su2(desc: MeshDescription, path) -> None:
-    """Writes a description without periodic bars as an SU2 file (one marker per boundary kind,
-    in the order the kinds first appear); read back by read_su2 / Mesh2D_t(path, ...)."""
+def periodic_groups(desc: MeshDescription):
+    """The periodic bars of a description grouped as marker pairs: bars whose donor lies at the
+    same translation t form marker a, their donors (at -t) marker b. Returns a list of
+    (bars of a, bars of b), each in bar order."""
     bar_type = np.asarray(desc.bar_type)
-    if any(int(t) not in SU2_MARKERS for t in bar_type):
-        raise ValueError("write_su2: SU2 files hold wall, symmetry, inlet and farfield markers only")
+    per = np.nonzero(bar_type == BC_PERIODIC)[0]
+    if per.size == 0:
+        return []
+    xy, bars = np.asarray(desc.xy), np.asarray(desc.bars)
+    donor = np.asarray(desc.bar_donor)
+    mid = (xy[bars[:, 0]] + xy[bars[:, 1]]) / 2
+    scale = float(np.max(np.abs(xy))) or 1.0
+    groups = {}
+    for b in per:
+        d = int(donor[b])
+        if d < 0 or bar_type[d] != BC_PERIODIC or int(donor[d]) != b:
+            raise ValueError("write_su2: periodic bars must come in mutual donor pairs")
+        key = tuple(np.round((mid[d] - mid[b]) / scale, 9))
+        groups.setdefault(key, []).append(int(b))
+    pairs, used = [], set()
+    for key, members in groups.items():
+        neg = tuple(-k + 0.0 for k in key)
+        if key in used:
+            continue
+        if neg not in groups:
+            raise ValueError("write_su2: a periodic group has no opposite group")
+        used.update({key, neg})
+        pairs.append((members, groups[neg]))
+    return pairs
+
+
+def write_su2(desc: MeshDescription, path):
+    """Writes a description as an SU2 file (one marker per boundary kind, in the order the kinds
+    first appear), read back by read_su2 / Mesh2D_t(path, ...). Periodic bars become marker pairs
+    "periodic_<k>_a" / "periodic_<k>_b" (one pair per translation); the list of those tag pairs is
+    returned, to be given to read_su2(path, periodic=...)."""
+    bar_type = np.asarray(desc.bar_type)
+    if any(int(t) not in SU2_MARKERS and int(t) != BC_PERIODIC for t in bar_type):
+        raise ValueError("write_su2: SU2 files hold wall, symmetry, inlet, farfield and periodic markers only")
+    groups = periodic_groups(desc)
     lines = ["NDIME= 2", f"NELEM= {len(desc.quads)}"]
     lines += ["9 " + " ".join(str(int(v)) for v in q) + f" {i}" for i, q in enumerate(np.asarray(desc.quads))]
     lines.append(f"NPOIN= {len(desc.xy)}")
     lines += [f"{x!r} {y!r} {i}" for i, (x, y) in enumerate(np.asarray(desc.xy).tolist())]
-    kinds = list(dict.fromkeys(int(t) for t in bar_type))
-    lines.append(f"NMARK= {len(kinds)}")
+    kinds = list(dict.fromkeys(int(t) for t in bar_type if int(t) != BC_PERIODIC))
+    lines.append(f"NMARK= {len(kinds) + 2 * len(groups)}")
+    bars = np.asarray(desc.bars)
+
+    def marker(tag, sel):
+        lines.extend([f"MARKER_TAG= {tag}", f"MARKER_ELEMS= {len(sel)}"])
+        lines.extend(f"3 {int(bars[b][0])} {int(bars[b][1])}" for b in sel)
+
     for k in kinds:
-        sel = np.nonzero(bar_type == k)[0]
-        lines += [f"MARKER_TAG= {SU2_MARKERS[k]}", f"MARKER_ELEMS= {len(sel)}"]
-        lines += [f"3 {int(desc.bars[b][0])} {int(desc.bars[b][1])}" for b in sel]
+        marker(SU2_MARKERS[k], np.nonzero(bar_type == k)[0])
+    tags = []
+    for i, (a, b) in enumerate(groups):
+        marker(f"periodic_{i}_a", a)
+        marker(f"periodic_{i}_b", b)
+        tags.append((f"periodic_{i}_a", f"periodic_{i}_b"))
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
+    return tags
 
 
 def main(argv=None) -> int:
@@ -114,8 +162,17 @@ def main(argv=None) -> int:
     r.add_argument("-i", "--input", required=True, help="SU2 mesh to re-number")
     r.add_argument("-o", "--output", required=True, help="path of the re-numbered mesh")
     r.add_argument("-a", "--algorithm", default="circular", choices=["circular", "hilbert-circular"])
+    r.add_argument("-p", "--periodic", action="append", default=[], metavar="A:B",
+                   help="marker tags A and B joined by a translation (repeatable)")
     a = ap.parse_args(argv)
-    write_su2(renumber(read_su2(a.input), a.algorithm), a.output)
+    pairs = []
+    for spec in a.periodic:
+        if spec.count(":") != 1:
+            ap.error(f"--periodic expects A:B, got '{spec}'")
+        pairs.append(tuple(spec.split(":")))
+    tags = write_su2(renumber(read_su2(a.input, periodic=pairs), a.algorithm), a.output)
+    for ta, tb in tags:
+        print(f"periodic markers in {a.output}: {ta}:{tb}")
     return 0
 
 

@@ -94,19 +94,27 @@ def square_mesh(resolution: int, x_min=-1.0, x_max=1.0, y_min=-1.0, y_max=1.0,
     return MeshDescription(xy, quads, bars, bar_type, bar_donor)
 
 
-def read_su2(path) -> MeshDescription:
+def read_su2(path, periodic=None) -> MeshDescription:
     """Mesh2D_t::read_su2 (src/meshes/Mesh2D_t.cpp:78-330): nodes, QUAD_4 elements and the
     farfield / wall / inlet / symmetry markers of an SU2 file as a MeshDescription (bars in the
-    reference's ghost order: walls, symmetry, inflow, outflow)."""
+    reference's ghost order: walls, symmetry, inflow, outflow). `periodic` lists (marker_a,
+    marker_b) tag pairs joined by a translation, as SU2's MARKER_PERIODIC does; their bars come
+    last, as periodic bars with their donors (ndg_su2_fill_periodic)."""
     p = os.fsencode(path)
+    pairs = list(periodic or [])
+    tags_a = (ctypes.c_char_p * max(1, len(pairs)))(*[a.encode() for a, _ in pairs])
+    tags_b = (ctypes.c_char_p * max(1, len(pairs)))(*[b.encode() for _, b in pairs])
+    per = _abi.SU2Periodic(len(pairs), tags_a, tags_b)
     nn, nq, nb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-    check(lib().ndg_su2_sizes(p, ctypes.byref(nn), ctypes.byref(nq), ctypes.byref(nb)))
+    check(lib().ndg_su2_sizes_periodic(p, ctypes.byref(per), ctypes.byref(nn), ctypes.byref(nq), ctypes.byref(nb)))
     xy = np.empty((nn.value, 2), dtype=np.float64)
     quads = np.empty((nq.value, 4), dtype=np.int64)
     bars = np.empty((nb.value, 2), dtype=np.int64)
     bar_type = np.empty(nb.value, dtype=np.int32)
-    check(lib().ndg_su2_fill(p, ptr(xy), ptr(quads), ptr(bars), ptr(bar_type)))
-    return MeshDescription(xy, quads, bars, bar_type, np.full(nb.value, -1, dtype=np.int64))
+    bar_donor = np.empty(nb.value, dtype=np.int64)
+    check(lib().ndg_su2_fill_periodic(p, ctypes.byref(per), ptr(xy), ptr(quads), ptr(bars), ptr(bar_type),
+                                      ptr(bar_donor)))
+    return MeshDescription(xy, quads, bars, bar_type, bar_donor)
 
 
 def _mesh_from(mesh) -> MeshDescription:

@@ -2,9 +2,10 @@
 // src/main.cpp does: tables, mesh, initial conditions, solver, solve, solution. Writes the final
 // packed p, u, v as raw doubles to argv[1], the output-point x, y, p, u, v to argv[1].out, VTK
 // files to argv[1]_vtk/ and argv[1]_vtk2/, and prints the step count and final time.
-//   host_api_main <out.bin> <resolution> <N> <t_end> [mesh.su2]
+//   host_api_main <out.bin> <resolution> <N> <t_end> [mesh.su2 [TAG_A:TAG_B ...]]
 // The mesh is the square generator's [-1,1]^2, periodic in x and y, or the given SU2 file (the
-// reference's Mesh2D_t(path, ...) constructor), with one Gaussian pulse.
+// reference's Mesh2D_t(path, ...) constructor; with marker pairs, read_su2 joins them
+// periodically), with one Gaussian pulse.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -14,8 +15,8 @@
 using namespace SEM::Device;
 
 int main(int argc, char **argv) {
-    if (argc != 5 && argc != 6) {
-        std::fprintf(stderr, "usage: %s out.bin resolution N t_end [mesh.su2]\n", argv[0]);
+    if (argc < 5) {
+        std::fprintf(stderr, "usage: %s out.bin resolution N t_end [mesh.su2 [TAG_A:TAG_B ...]]\n", argv[0]);
         return 2;
     }
     const int res = std::atoi(argv[2]), N = std::atoi(argv[3]);
@@ -28,9 +29,17 @@ int main(int argc, char **argv) {
         for (int k = 0; k < 4; ++k) params.bc[k] = NDG_BC_WALL;
         params.x_periodic = 1;
         params.y_periodic = 1;
-        Meshes::Mesh2D_t mesh = argc == 6
-            ? Meshes::Mesh2D_t(std::filesystem::path(argv[5]), N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, NDG)
-            : Meshes::Mesh2D_t(Meshes::square_mesh(params), N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, NDG);
+        std::vector<std::pair<std::string, std::string>> periodic;  // SU2 marker pairs joined periodically
+        for (int i = 6; i < argc; ++i) {
+            const std::string spec = argv[i];
+            const size_t colon = spec.find(':');
+            if (colon == std::string::npos) return 2;
+            periodic.emplace_back(spec.substr(0, colon), spec.substr(colon + 1));
+        }
+        Meshes::Mesh2D_t mesh =
+            argc == 6   ? Meshes::Mesh2D_t(std::filesystem::path(argv[5]), N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, NDG)
+            : argc > 6  ? Meshes::Mesh2D_t(Meshes::read_su2(argv[5], periodic), N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, NDG)
+                        : Meshes::Mesh2D_t(Meshes::square_mesh(params), N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, NDG);
         mesh.initial_conditions({{0.1, -0.2, 0.2, 1.0}});
         // output events at t_end / 2 and t_end (the reference's write_complete_data calls)
         Solvers::Solver2D_t solver(0.5, {t_end / 2, t_end}, 0.0);

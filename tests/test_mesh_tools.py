@@ -91,5 +91,73 @@ def test_su2_writer_round_trip(tmp_path):
     out = tmp_path / "c.su2"
     assert mt.main(["renumber", "-i", str(path), "-o", str(out), "-a", "circular"]) == 0
     assert np.array_equal(nd.read_su2(out).quads, mt.renumber(got, "circular").quads)
+    bad = nd.square_mesh(4)
+    bad.bar_type = np.full_like(np.asarray(bad.bar_type), nd.BC_NONE)  # no SU2 marker for this kind
     with pytest.raises(ValueError):
-        mt.write_su2(nd.square_mesh(4, x_periodic=True), tmp_path / "p.su2")
+        mt.write_su2(bad, tmp_path / "n.su2")
+
+
+def canonical_periodic(desc):
+    """{element-edge node pair of a periodic bar: that of its donor}, independent of bar order."""
+    bars = np.asarray(desc.bars)
+    out = {}
+    for b in np.nonzero(np.asarray(desc.bar_type) == nd.BC_PERIODIC)[0]:
+        d = int(desc.bar_donor[b])
+        out[frozenset(map(int, bars[b]))] = frozenset(map(int, bars[d]))
+    return out
+
+
+@pytest.mark.parametrize("xp,yp,jitter", [(True, True, 0.0), (True, False, 0.1), (False, True, 0.15)])
+def test_periodic_su2_round_trip(tmp_path, xp, yp, jitter):
+    """A generator mesh with periodic sides written as SU2 marker pairs and read back with those
+    pairs: same nodes, elements, boundary kinds and periodic couplings (ndg_su2_fill_periodic)."""
+    desc = nd.square_mesh(8, bc=(nd.BC_WALL, nd.BC_OUTFLOW, nd.BC_SYMMETRY, nd.BC_INFLOW), x_periodic=xp,
+                          y_periodic=yp, jitter=jitter, seed=4)
+    path = tmp_path / "p.su2"
+    tags = mt.write_su2(desc, path)
+    assert len(tags) == int(xp) + int(yp)
+    back = nd.read_su2(path, periodic=tags)
+    assert np.array_equal(back.xy, desc.xy) and np.array_equal(back.quads, desc.quads)
+    assert canonical_periodic(back) == canonical_periodic(desc)
+    kinds = lambda d: sorted(map(int, np.asarray(d.bar_type)))
+    assert kinds(back) == kinds(desc)
+    without = nd.read_su2  # without the pairs the periodic tags are unknown markers
+    with pytest.raises(nd.NDGError) as e:
+        without(path)
+    assert "periodic_0_a" in str(e.value)
+
+
+def test_periodic_su2_errors(tmp_path):
+    desc = nd.square_mesh(4, x_periodic=True, y_periodic=True)
+    path = tmp_path / "p.su2"
+    tags = mt.write_su2(desc, path)
+    with pytest.raises(nd.NDGError) as e:
+        nd.read_su2(path, periodic=tags + [("ghost_a", "ghost_b")])
+    assert "'ghost_a' not found" in str(e.value)
+    # pairing a marker with the other pair's marker: no translated partners
+    with pytest.raises(nd.NDGError) as e:
+        nd.read_su2(path, periodic=[(tags[0][0], tags[1][0]), (tags[0][1], tags[1][1])])
+    assert "partner" in str(e.value)
+
+
+@pytest.mark.skipif(nd.device_count() > 0, reason="the host topology runs before the no-device error")
+def test_periodic_su2_mesh_passes_the_host_topology(tmp_path):
+    desc = nd.square_mesh(8, x_periodic=True, y_periodic=True, seed=1)
+    path = tmp_path / "p.su2"
+    tags = mt.write_su2(desc, path)
+    with pytest.raises(nd.NDGError) as e:
+        nd.Mesh2D_t(nd.read_su2(path, periodic=tags), 4, 4, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, nd.NDG_t(4, 5))
+    assert e.value.code == nd._abi.NDG_ERR_NO_DEVICE
+
+
+def test_renumber_cli_keeps_periodic_pairs(tmp_path, capsys):
+    desc = nd.square_mesh(8, x_periodic=True, y_periodic=True)
+    src, dst = tmp_path / "in.su2", tmp_path / "out.su2"
+    tags = mt.write_su2(desc, src)
+    args = ["renumber", "-i", str(src), "-o", str(dst), "-a", "hilbert-circular"]
+    for a, b in tags:
+        args += ["-p", f"{a}:{b}"]
+    assert mt.main(args) == 0
+    printed = [l.split(": ")[1].split(":") for l in capsys.readouterr().out.strip().splitlines()]
+    back = nd.read_su2(dst, periodic=[tuple(p) for p in printed])
+    assert len(canonical_periodic(back)) == len(canonical_periodic(desc))
