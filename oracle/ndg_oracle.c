@@ -877,7 +877,10 @@ void o_get_solution(const o_mesh *m, double *x, double *y, double *p, double *u,
  * o_get_solution, the element's G (what the last right-hand side evaluation left there, the
  * reference's G_p_) and |g(x_mk, time) - q_mk| of the plane wave at the solution nodes, each
  * interpolated with I[i][m] I[j][k]. out holds 11 planes of K n n values:
- * x, y, p, u, v, Gp, Gu, Gv, |dp|, |du|, |dv|. */
+ * x, y, p, u, v, Gp, Gu, Gv, |dp|, |du|, |dv|. Pinned (tests/test_oracle_output.py): the first
+ * five planes equal o_get_solution's; the error planes vanish on the plane-wave state at t = 0
+ * and approach g itself on a zero state; the G planes hold the constant right-hand side of a
+ * linear field on interior elements. */
 void o_get_complete_solution(const o_mesh *m, double time, double *out) {
     const int n = m->t->n_interp;
     const int64_t plane = m->K * (int64_t)n * n;

@@ -66,3 +66,46 @@ def test_physical_polynomials_on_mixed_orders():
     x, y, *vals = om.get_solution(n)
     for g, got in zip(f, vals):
         assert np.max(np.abs(got - g(x, y))) < 1e-12
+
+
+def plane_wave(x, y, t, c=1.0):
+    """The reference's g (include/functions/analytical_solution.tpp:5-11, include/helpers/
+    constants.h:15-18): k = (sqrt(2)/2, sqrt(2)/2), x0 = (-0.2, -0.2), d = 0.2 / (2 sqrt(ln 2))."""
+    k = np.sqrt(2.0) / 2
+    d = 0.2 / (2.0 * np.sqrt(np.log(2.0)))
+    p = np.exp(-((k * (x + 0.2) + k * (y + 0.2) - c * t) ** 2) / d**2)
+    return p, p * k, p * k
+
+
+def test_complete_solution_point_fields():
+    """The oracle's get_complete_solution (src/entities/Element2D_t.cu:623-665):
+    - its first five planes are get_solution's;
+    - on the plane-wave initial state at time 0 the analytical-error planes are exactly zero;
+    - with a zero state they interpolate |g| = g, so they approach g at the output points as the
+      order grows;
+    - after compute_rhs of a physically linear p (u = v = 0) on an affine mesh the right-hand side
+      planes hold the constant (0, -a, -b) on the elements away from the walls (the interior
+      upwind fluxes of a continuous field vanish)."""
+    n, N = 6, 8
+    T, desc, om = build(4, N, n, jitter=0.0)
+    om.initial_conditions(None)  # the plane wave
+    full = om.get_complete_solution(n, 0.0)
+    for a, b in zip(full[:5], om.get_solution(n)):
+        assert np.array_equal(a, b)
+    assert not np.any(full[8:])
+    om.set_state(*(np.zeros(om.n) for _ in range(3)))
+    for t in (0.0, 0.3):
+        full = om.get_complete_solution(n, t)
+        exact = plane_wave(full[0], full[1], t)
+        for got, want in zip(full[8:], exact):
+            assert np.max(np.abs(got - want)) < 5e-3 * np.max(want)
+    X, Y = tm.packed_nodes(desc, T, om.N)
+    a, b = 0.7, -1.3
+    om.set_state(a * X + b * Y + 0.5, np.zeros(om.n), np.zeros(om.n))
+    om.compute_rhs(0.0)
+    full = om.get_complete_solution(n, 0.0)
+    cx, cy = full[0].mean(axis=(1, 2)), full[1].mean(axis=(1, 2))
+    inner = (np.abs(cx) < 0.5) & (np.abs(cy) < 0.5)
+    assert inner.sum() == 4
+    for plane, want in zip(full[5:8], (0.0, -a, -b)):
+        assert np.max(np.abs(plane[inner] - want)) < 1e-11
