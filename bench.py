@@ -222,7 +222,7 @@ def timed_steps(solver, mesh, stream, torch, steps, t=0.0):
     return start.elapsed_time(stop)
 
 
-def fp32c_config1(nd):
+def fp32c_periodic_16x16(nd):
     """16 x 16 periodic, N = 6, one pulse, CFL 0.5 to t = 0.5 (289 steps): the FP32 compensated
     form's relative l2 distance to the FP64 run (which matches the reference to 1e-12)."""
     desc = nd.square_mesh(16, x_periodic=True, y_periodic=True)
@@ -414,7 +414,7 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copies": 3}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the DG stage "
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); the DG stage "
               "runs on a single MI355X and multi-GPU sharding is not part of it", file=sys.stderr)
         sys.exit(2)
 
@@ -496,7 +496,7 @@ def main():
                          "algorithmic_bytes_per_launch": b32, "kernel": "ndg::stage_c32_kernel<8>"},
             "note": "FP32 right-hand side and RK register, state as a two-sum-compensated float pair; "
                     "error is against the FP64 run of the same steps",
-            "config_16x16_N6": fp32c_config1(nd)}
+            "periodic_16x16_N6": fp32c_periodic_16x16(nd)}
     out = {
         "metric": "DOF*stage updates/s (FP64 plain DG-SEM wave RK3 stage)",
         "value": value,

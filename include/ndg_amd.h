@@ -31,7 +31,7 @@ typedef int ndg_status;
 #define NDG_OK 0
 #define NDG_ERR_INVALID 1     /* bad argument or inconsistent mesh description */
 #define NDG_ERR_HIP 2         /* a HIP runtime call or kernel launch failed */
-#define NDG_ERR_UNSUPPORTED 3 /* valid input this build does not handle (message says which) */
+#define NDG_ERR_UNSUPPORTED 3 /* valid input the library does not handle (message says which) */
 #define NDG_ERR_NO_DEVICE 4   /* no HIP device visible */
 
 /* Boundary kinds of boundary (BAR_2) edges. Values follow the CGNS BC types the reference reads

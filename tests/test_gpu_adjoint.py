@@ -129,7 +129,7 @@ def test_gradient_vs_cpu_tangent_linear(tables):
         assert abs(dJ - dJ_cpu) <= 1e-9 * max(abs(dJ_cpu), 1e-300)
 
 
-def config5_setup(tables):
+def adjoint_128_setup(tables):
     N, res, n_steps, interval = 8, 128, 2000, 50
     desc, gm, _ = periodic_mesh(tables, res, N, 0.0, 1)
     rec = A.Receivers.seeded(gm.n_elements_, 32, 2024)
@@ -142,9 +142,9 @@ def config5_setup(tables):
     return gm, rec, dt, n_steps, interval, data
 
 
-def test_config5_gradient_vs_finite_differences(tables):
-    say("adjoint config 5: 128x128 N=8 periodic, 2000 steps, checkpoints 50, FD along 8 directions")
-    gm, rec, dt, n_steps, interval, data = config5_setup(tables)
+def test_adjoint_128x128_gradient_vs_finite_differences(tables):
+    say("adjoint 128x128 N=8 periodic, 2000 steps, checkpoints 50, FD along 8 directions")
+    gm, rec, dt, n_steps, interval, data = adjoint_128_setup(tables)
     q0 = [np.array(a) for a in gm.get_solution()]
     J, gp, gu, gv, gc = A.adjoint_gradient(gm, rec, dt, n_steps, interval, data)
     say("J", J, "dJ/dc", gc)

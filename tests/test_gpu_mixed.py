@@ -168,10 +168,10 @@ def test_graph_steps_after_autotune(tables, mode):
         assert rel(g, r) < TOL
 
 
-def test_graph_solve_config1(tables):
+def test_graph_solve_16x16_N6(tables):
     """The reference time loop in graph mode: 288 replayed steps plus the clipped 289th launched
     directly (16 x 16 periodic, N = 6, t = 0.5)."""
-    say("graph solve config 1")
+    say("graph solve 16x16 periodic N=6")
     ndg, ot = tables
     desc = nd.square_mesh(16, x_periodic=True, y_periodic=True)
     gm = nd.Mesh2D_t(desc, 6, 6, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)

@@ -116,10 +116,10 @@ def test_delta_t_matches(tables):
     assert solver.get_delta_t(gm) == om.delta_t(0.5)
 
 
-def test_config1_full_solve(tables):
+def test_periodic_16x16_N6_full_solve(tables):
     """16 x 16 periodic, N = 6, CFL 0.5, pulse at (0.1, -0.2) width 0.2, t_end = 0.5: the reference
     loop takes 289 steps (accumulated time falls short of 0.5 after 288)."""
-    say("config 1")
+    say("16x16 periodic N=6 to t=0.5")
     desc, gm, om = build(tables, 16, 6, periodic=True)
     pulse = [(0.1, -0.2, 0.2, 1.0)]
     gm.initial_conditions(pulse)
@@ -132,7 +132,7 @@ def test_config1_full_solve(tables):
         assert rel(g, r) < TOL
 
 
-def test_config1_exact_solution(tables):
+def test_periodic_16x16_N6_exact_solution(tables):
     """Same configuration with the periodised pulse against the Fourier-series exact solution."""
     ndg, ot = tables
     desc, gm, _ = build(tables, 16, 6, periodic=True)
@@ -146,9 +146,9 @@ def test_config1_exact_solution(tables):
     assert max(np.abs(p - pe).max(), np.abs(u - ue).max(), np.abs(v - ve).max()) < 5e-7
 
 
-def test_config2_reduced(tables):
-    """Config-2 form (jittered walls, N = 8, two pulses) at 32 x 32 for 20 steps."""
-    say("config 2 reduced")
+def test_jittered_walls_N8_reduced(tables):
+    """The 512 x 512 benchmark's form (jittered walls, N = 8, two pulses) at 32 x 32 for 20 steps."""
+    say("jittered walls N=8 at 32x32")
     desc, gm, om = build(tables, 32, 8, jitter=0.15, seed=2024)
     pulses = [(0.1, -0.2, 0.2, 1.0), (-0.4, 0.3, 0.2, 0.5)]
     gm.initial_conditions(pulses)
@@ -243,8 +243,8 @@ def test_fp32c_close_to_fp64_oracle(tables):
         assert rel(g, r) < 2e-5
 
 
-def test_fp32c_exact_solution_config1(tables):
-    """Config 1 (16x16 periodic, N = 6, t = 0.5) in the FP32 form: its distance to the Fourier exact
+def test_fp32c_exact_solution_16x16_N6(tables):
+    """16x16 periodic, N = 6, t = 0.5 in the FP32 form: its distance to the Fourier exact
     solution stays at the FP64 discretisation error (5e-7 bound used for FP64 above)."""
     ndg, ot = tables
     desc, gm, _ = build(tables, 16, 6, periodic=True)

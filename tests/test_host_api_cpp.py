@@ -92,7 +92,7 @@ def test_driver_reads_su2_file(tmp_path):
 
 @pytest.mark.gpu
 def test_driver_reads_periodic_su2_file(tmp_path):
-    """Config 1's periodic 16 x 16 mesh written as an SU2 file with periodic marker pairs and read
+    """The periodic 16 x 16 mesh (N = 6, to t = 0.5) written as an SU2 file with periodic marker pairs and read
     back by the C++ mirror's read_su2(path, pairs): the solve to t = 0.5 matches the oracle on the
     generator's description."""
     from ndg_cuda_amd import mesh_tools

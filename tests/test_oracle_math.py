@@ -265,14 +265,14 @@ def test_rk3_is_the_third_order_power_series():
 
 def test_delta_t_closed_form():
     desc, T, m = make(16, 6, periodic=True)
-    assert m.delta_t(0.5) == 0.5 * 0.125 / 36      # config 1: 1/576
+    assert m.delta_t(0.5) == 0.5 * 0.125 / 36      # 16x16 on [-1,1]^2, N = 6: 1/576
     n, tf = m.solve(0.5, 0.01)
     assert tf == 0.01 and n == 6
 
 
 @pytest.mark.parametrize("N,tol", [(4, 2e-5), (6, 5e-7), (8, 1e-7)])
 def test_periodic_pulse_exact_solution(N, tol):
-    """Config-1 geometry (16x16 periodic on [-1,1]^2, CFL 0.5) with the periodised pulse, against
+    """The 16x16 periodic geometry ( on [-1,1]^2, CFL 0.5) with the periodised pulse, against
     the Fourier-series exact solution; spectral convergence in N."""
     desc, T, m = make(16, N, periodic=True)
     base = [(0.1, -0.2, 0.2, 1.0)]
