@@ -257,3 +257,29 @@ def test_fp32c_exact_solution_16x16_N6(tables):
     pe, ue, ve = exact.periodic_pulse(X, Y, solver.time_, [base], n_modes=24)
     p, u, v = gm.get_solution()
     assert max(np.abs(p - pe).max(), np.abs(u - ue).max(), np.abs(v - ve).max()) < 1e-6
+
+
+@pytest.mark.parametrize("res,periodic", [(1, False), (1, True), (2, True)])
+@pytest.mark.parametrize("N", [1, 16])
+def test_tiny_meshes(tables, res, periodic, N):
+    """The smallest meshes at the smallest and largest compiled orders: one element with four wall
+    ghosts, one element coupled to itself across both periodic directions, and 2 x 2 periodic.
+    Right-hand side of a random state and 5 steps of a pulse against the oracle (errors scaled by
+    the largest value of the three fields, since a field can vanish)."""
+    say("tiny mesh", res, periodic, N)
+    desc, gm, om = build(tables, res, N, periodic=periodic)
+    q = random_state(gm.n_dofs_per_field_, 5)
+    gm.set_solution(*q)
+    om.set_state(*q)
+    got, ref = gm.compute_rhs(0.0), om.compute_rhs(0.0)
+    scale = max(float(np.abs(r).max()) for r in ref)
+    for g, r in zip(got, ref):
+        assert np.max(np.abs(g - r)) <= TOL * scale
+    gm.initial_conditions([(0.1, -0.2, 0.6, 1.0)])
+    om.initial_conditions([(0.1, -0.2, 0.6, 1.0)])
+    nd.Solver2D_t(0.5, [0.0]).run_steps(gm, 5)
+    om.run_steps(0.5, 5)
+    got, ref = gm.get_solution(), om.get_state()
+    scale = max(float(np.abs(r).max()) for r in ref)
+    for g, r in zip(got, ref):
+        assert np.max(np.abs(g - r)) <= TOL * scale
