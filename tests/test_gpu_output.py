@@ -67,3 +67,33 @@ def test_writer_exception_stops_the_solve():
 
     with pytest.raises(KeyError):
         nd.Solver2D_t(0.5, [0.0, 0.02, 0.05]).solve(None, gm, writer)
+
+
+def test_solve_with_data_writer(tmp_path):
+    """Solver2D_t.solve(NDG, mesh, DataWriter_t): the reference's output events write
+    write_complete_data's files (point grid and element grid) and the .series index, estimating
+    the error before every write but the one before the first step."""
+    import json
+    from ndg_cuda_amd.data_writer import read_vtu
+    print("[gpu] solve with DataWriter_t", flush=True)
+    desc = nd.square_mesh(8, x_periodic=True, y_periodic=True, jitter=0.1, seed=2)
+    ndg = nd.NDG_t(5, 4)
+    gm = nd.Mesh2D_t(desc, 5, 5, 4, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
+    gm.initial_conditions([(0.1, -0.2, 0.2, 1.0)])
+    writer = nd.DataWriter_t(tmp_path / "out" / "run.pvtu")
+    solver = nd.Solver2D_t(0.5, [0.0, 0.01, 0.02])
+    solver.solve(ndg, gm, writer)
+    files = json.loads((tmp_path / "out" / "run.pvtu.series").read_text())["files"]
+    assert [f["time"] for f in files][0] == 0.0 and files[-1]["time"] == solver.time_ == 0.02
+    assert len(files) == 3
+    last = read_vtu(tmp_path / "out" / files[-1]["name"])
+    x, y, p, u, v = gm.get_interpolated_solution()
+    assert np.array_equal(last["point_data"]["Pressure"], p.ravel())
+    assert np.array_equal(last["points"][:, 0], x.ravel())
+    elements = json.loads((tmp_path / "out" / "run_element.pvtu.series").read_text())["files"]
+    first = read_vtu(tmp_path / "out" / elements[0]["name"])
+    final = read_vtu(tmp_path / "out" / elements[-1]["name"])
+    assert not np.any(first["cell_data"]["PressureError"])  # no estimate before the first step
+    err, sig, _, _ = gm.estimate_error()
+    assert np.array_equal(final["cell_data"]["PressureError"], err)
+    assert np.all(final["cell_data"]["N"] == 5)
