@@ -135,7 +135,10 @@ struct DeviceMesh {
     // FP32 form with compensated state (allocated by ndg_mesh_set_precision): the state is the
     // unevaluated sum qh + ql of two floats per value, the RK register and traces are floats.
     int precision = NDG_PRECISION_FP64;
-    float *qh = nullptr, *ql = nullptr, *g32 = nullptr;
+    float *qh = nullptr;
+    // RK register (x) and low state word (y) of each value, interleaved so that a line owner
+    // reads and writes both with one 8-byte access; transposed within each (element, field) block
+    float2 *gl32 = nullptr;
     float *trace32[2] = {nullptr, nullptr};
 };
 

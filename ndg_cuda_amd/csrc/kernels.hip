@@ -443,11 +443,11 @@ __device__ __forceinline__ void two_sum_add(float &hi, float &lo, float d) {
     hi = h;
 }
 
-// q and qh are in reference node order, ql, g and g32 transposed within each (element, field)
-// block; nf = N + 1.
+// q and qh are in reference node order; g and gl32 (RK register in x, low state word in y) are
+// transposed within each (element, field) block; nf = N + 1. Thread r writes the x word of
+// element r and the y word of element t: 4-byte stores to distinct words.
 __global__ void to_c32_kernel(const double *__restrict__ q, const double *__restrict__ g,
-                              float *__restrict__ qh, float *__restrict__ ql, float *__restrict__ g32,
-                              int64_t n, int nf) {
+                              float *__restrict__ qh, float2 *__restrict__ gl32, int64_t n, int nf) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int ne = nf * nf, node = (int)(r % ne), i = node / nf, j = node % nf;
@@ -455,19 +455,18 @@ __global__ void to_c32_kernel(const double *__restrict__ q, const double *__rest
     const double v = q[r];
     const float h = (float)v;
     qh[r] = h;
-    ql[t] = (float)(v - (double)h);
-    g32[r] = (float)g[r];
+    gl32[t].y = (float)(v - (double)h);
+    gl32[r].x = (float)g[r];
 }
 
-__global__ void from_c32_kernel(const float *__restrict__ qh, const float *__restrict__ ql,
-                                const float *__restrict__ g32, double *__restrict__ q,
-                                double *__restrict__ g, int64_t n, int nf) {
+__global__ void from_c32_kernel(const float *__restrict__ qh, const float2 *__restrict__ gl32,
+                                double *__restrict__ q, double *__restrict__ g, int64_t n, int nf) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int ne = nf * nf, node = (int)(r % ne), i = node / nf, j = node % nf;
     const int64_t t = r - node + j * nf + i;
-    q[r] = (double)qh[r] + (double)ql[t];
-    g[r] = (double)g32[r];
+    q[r] = (double)qh[r] + (double)gl32[t].y;
+    g[r] = (double)gl32[r].x;
 }
 
 template <int N>
@@ -507,8 +506,8 @@ template <int N>
 __global__ void __launch_bounds__(192)
 stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_nb,
                  const BcSide *__restrict__ bc, const double *__restrict__ tab64,
-                 const float *__restrict__ tab, float *__restrict__ qh, float *__restrict__ ql,
-                 float *__restrict__ g, const float *__restrict__ tr_in, float *__restrict__ tr_out,
+                 const float *__restrict__ tab, float *__restrict__ qh, float2 *__restrict__ gl,
+                 const float *__restrict__ tr_in, float *__restrict__ tr_out,
                  const float *__restrict__ side_flux, int64_t K, int first, float a, float gdt,
                  const double *__restrict__ t_ptr, int epb, int remap, double cw) {
     using S = LineShape<N>;
@@ -554,15 +553,17 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
         const int r = tid + it * T;
         vreg[it] = r < nel * 8 ? verts[e0 * 8 + r] : 0.0;
     }
-    // RK register and low words stored transposed within an element ([e][f][j][i]): the row owner's
-    // lines are read and written with consecutive lanes on consecutive addresses.
+    // RK register and low words, interleaved and stored transposed within an element
+    // ([e][f][j][i]): the row owner's lines are read and written with consecutive lanes on
+    // consecutive addresses, both words with one 8-byte access.
     float g_old[NF], lo_row[NF];
     if (active) {
         const int64_t off = (e0 + el) * 3 * NE + f * NE + l;
 #pragma unroll
         for (int j = 0; j < NF; ++j) {
-            g_old[j] = first ? 0.f : g[off + j * NF];
-            lo_row[j] = ql[off + j * NF];
+            const float2 w = gl[off + j * NF];
+            g_old[j] = first ? 0.f : w.x;
+            lo_row[j] = w.y;
         }
     } else {
 #pragma unroll
@@ -757,11 +758,10 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
             const float jac = met[2 * NF + j] * yeta_i - xeta_i * met[3 * NF + j];
             const float Gv = (float)cw * ((-d[j] - acc) / jac);
             const float gn = first ? Gv : a * g_old[j] + Gv;
-            g[(e0 + el) * 3 * NE + f * NE + j * NF + i] = gn;
             float h = qrow[j], lo = lo_row[j];
             two_sum_add(h, lo, gdt * gn);
             hn[j] = h;
-            ql[(e0 + el) * 3 * NE + f * NE + j * NF + i] = lo;
+            gl[(e0 + el) * 3 * NE + f * NE + j * NF + i] = make_float2(gn, lo);
         }
     }
     __syncthreads();
@@ -1318,8 +1318,8 @@ static hipError_t stage_c32_n(DeviceMesh &m, const Group &G, int stage, double a
     const int epb = group_epb<N>(G, NDG_PRECISION_FP32_COMPENSATED);
     const int64_t blocks = (G.K + epb - 1) / epb;
     hipLaunchKernelGGL(stage_c32_kernel<N>, dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts + G.e_begin * 8,
-                       m.side_nb + G.e_begin * 4, m.bc, G.tab, G.tab32, m.qh + G.q_off, m.ql + G.q_off,
-                       m.g32 + G.q_off, m.trace32[m.cur] + G.tr_off, m.trace32[1 - m.cur] + G.tr_off,
+                       m.side_nb + G.e_begin * 4, m.bc, G.tab, G.tab32, m.qh + G.q_off, m.gl32 + G.q_off,
+                       m.trace32[m.cur] + G.tr_off, m.trace32[1 - m.cur] + G.tr_off,
                        m.side_flux32, G.K, stage == 0 ? 1 : 0, (float)a, (float)gdt, m.clock + 1 + stage, epb,
                        G.tune[NDG_PRECISION_FP32_COMPENSATED].remap, m.wave_speed);
     return hipGetLastError();
@@ -1429,13 +1429,13 @@ hipError_t launch_traces_c32(const DeviceMesh &m, hipStream_t s) {
 }
 
 
-// q and qh are in reference node order; ql, g and g32 transposed within each (element, field)
+// q and qh are in reference node order; g and gl32 transposed within each (element, field)
 // block. Run per group (the transposition depends on N).
 hipError_t launch_to_c32(DeviceMesh &m, hipStream_t s) {
     return for_groups(m, [&](const Group &G) -> hipError_t {
         const int64_t n = G.K * 3 * (int64_t)(G.N + 1) * (G.N + 1);
         hipLaunchKernelGGL(to_c32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m.q + G.q_off,
-                           m.g + G.q_off, m.qh + G.q_off, m.ql + G.q_off, m.g32 + G.q_off, n, G.N + 1);
+                           m.g + G.q_off, m.qh + G.q_off, m.gl32 + G.q_off, n, G.N + 1);
         return hipGetLastError();
     });
 }
@@ -1444,7 +1444,7 @@ hipError_t launch_from_c32(DeviceMesh &m, hipStream_t s) {
     return for_groups(m, [&](const Group &G) -> hipError_t {
         const int64_t n = G.K * 3 * (int64_t)(G.N + 1) * (G.N + 1);
         hipLaunchKernelGGL(from_c32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m.qh + G.q_off,
-                           m.ql + G.q_off, m.g32 + G.q_off, m.q + G.q_off, m.g + G.q_off, n, G.N + 1);
+                           m.gl32 + G.q_off, m.q + G.q_off, m.g + G.q_off, n, G.N + 1);
         return hipGetLastError();
     });
 }

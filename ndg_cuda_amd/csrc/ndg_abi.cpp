@@ -317,7 +317,7 @@ void free_device(DeviceMesh &d) {
                                 (void *)d.side_flux, (void *)d.side_flux32, (void *)d.tab1d,
                                 (void *)d.mortar_mats, (void *)d.q,
                                 (void *)d.g, (void *)d.trace[0], (void *)d.trace[1], (void *)d.rhs,
-                                (void *)d.scratch, (void *)d.qh, (void *)d.ql, (void *)d.g32,
+                                (void *)d.scratch, (void *)d.qh, (void *)d.gl32,
                                 (void *)d.trace32[0], (void *)d.trace32[1], (void *)d.clock};
     for (const Group &G : d.groups) {
         ptrs.push_back(G.tab);
@@ -917,7 +917,7 @@ ndg_status copy_state(ndg_mesh *m, double *dev, double *const fields[3], bool do
     return NDG_OK;
 }
 
-// FP32 compensated form: refresh the FP64 state, RK register and traces from (qh, ql, g32).
+// FP32 compensated form: refresh the FP64 state, RK register and traces from (qh, gl32).
 ndg_status refresh_fp64(ndg_mesh *m) {
     if (m->d.precision != NDG_PRECISION_FP32_COMPENSATED) return NDG_OK;
     HIP_TRY(launch_from_c32(m->d, m->stream), "from_c32 kernel");
@@ -925,7 +925,7 @@ ndg_status refresh_fp64(ndg_mesh *m) {
     return NDG_OK;
 }
 
-// After the FP64 state changed: rebuild (qh, ql, g32) and their traces.
+// After the FP64 state changed: rebuild (qh, gl32) and their traces.
 ndg_status refresh_c32(ndg_mesh *m) {
     if (m->d.precision != NDG_PRECISION_FP32_COMPENSATED) return NDG_OK;
     HIP_TRY(launch_to_c32(m->d, m->stream), "to_c32 kernel");
@@ -944,8 +944,7 @@ extern "C" ndg_status ndg_mesh_set_precision(ndg_mesh *mesh, int32_t precision) 
     if (precision == NDG_PRECISION_FP32_COMPENSATED) {
         if (!d.qh) {
             HIP_TRY(hipMalloc((void **)&d.qh, sizeof(float) * d.n_q), "hipMalloc qh");
-            HIP_TRY(hipMalloc((void **)&d.ql, sizeof(float) * d.n_q), "hipMalloc ql");
-            HIP_TRY(hipMalloc((void **)&d.g32, sizeof(float) * d.n_q), "hipMalloc g32");
+            HIP_TRY(hipMalloc((void **)&d.gl32, sizeof(float2) * d.n_q), "hipMalloc gl32");
             HIP_TRY(hipMalloc((void **)&d.trace32[0], sizeof(float) * d.n_tr), "hipMalloc traces32");
             HIP_TRY(hipMalloc((void **)&d.trace32[1], sizeof(float) * d.n_tr), "hipMalloc traces32");
             if (d.n_side_flux > 0)
