@@ -2,6 +2,7 @@
 point and cell layout, and the arrays read back from the XML (binary and ASCII, .vtu and .pvtu).
 CPU only: the inputs are synthetic output grids."""
 import json
+import os
 
 import numpy as np
 import pytest
@@ -121,3 +122,60 @@ def test_write_complete_data(tmp_path):
 def test_bad_extension(tmp_path):
     with pytest.raises(ValueError):
         DataWriter_t(tmp_path / "out.vtk")
+
+
+CPP_WRITER = r"""
+#include <cstdlib>
+#include "ndg_data_writer.hpp"
+int main(int argc, char **argv) {
+    const bool ascii = argc > 2 && std::atoi(argv[2]);
+    SEM::Helpers::DataWriter_t w(argv[1], ascii);
+    const std::size_t n = 3, K = 2;
+    std::vector<double> x(K * n * n), y(K * n * n), f(K * n * n);
+    for (std::size_t i = 0; i < x.size(); ++i) {
+        x[i] = 0.25 * (double)i;
+        y[i] = 1.0 - 0.125 * (double)i;
+        f[i] = 1.0 / (1.0 + (double)i);
+    }
+    w.write_data(n, K, 0.125, x, y, f, y, x);
+    std::vector<int> N{4, 6}, ints{1, 0};
+    std::vector<double> e{1e-3, 2e-4};
+    w.write_complete_data(n, K, 0.25, 0, x, y, f, y, x, N, f, y, x, e, e, e, e, e, e, ints, ints, ints, f, y, x, ints,
+                          ints);
+    return 0;
+}
+"""
+
+
+@pytest.mark.parametrize("ascii_mode", [0, 1])
+def test_cpp_writer_matches_python_writer(tmp_path, ascii_mode):
+    """The header-only C++ DataWriter_t (include/ndg_data_writer.hpp) writes the same arrays, file
+    names and series as the Python one."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src, exe = tmp_path / "w.cpp", tmp_path / "w"
+    src.write_text(CPP_WRITER)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(root, "include"), str(src), "-o", str(exe)],
+                   check=True, capture_output=True)
+    subprocess.run([str(exe), str(tmp_path / "cpp" / "o.pvtu"), str(ascii_mode)], check=True)
+    K, n = 2, 3
+    i = np.arange(K * n * n, dtype=np.float64)
+    x, y, f = 0.25 * i, 1.0 - 0.125 * i, 1.0 / (1.0 + i)
+    e = np.array([1e-3, 2e-4])
+    ints = np.array([1, 0])
+    w = DataWriter_t(tmp_path / "py" / "o.pvtu", fmt="ascii" if ascii_mode else "binary")
+    w.write_data(n, K, 0.125, x, y, f, y, x)
+    w.write_complete_data(n, K, 0.25, 0, x, y, f, y, x, np.array([4, 6]), f, y, x, e, e, e, e, e, e, ints, ints, ints,
+                          f, y, x, ints, ints)
+    for name in ("o_t0_125000000s.pvtu", "o_t0_250000000s.pvtu", "o_element_t0_250000000s.pvtu"):
+        a, b = read_vtu(tmp_path / "cpp" / name), read_vtu(tmp_path / "py" / name)
+        assert np.array_equal(a["points"], b["points"]) and np.array_equal(a["connectivity"], b["connectivity"])
+        for kind in ("point_data", "cell_data"):
+            assert list(a[kind]) == list(b[kind])
+            for key in a[kind]:
+                assert a[kind][key].dtype == b[kind][key].dtype and np.array_equal(a[kind][key], b[kind][key]), key
+    for series in ("o.pvtu.series", "o_element.pvtu.series"):
+        assert json.loads((tmp_path / "cpp" / series).read_text()) == json.loads((tmp_path / "py" / series).read_text())
