@@ -1017,6 +1017,20 @@ extern "C" ndg_status ndg_get_state(ndg_mesh *mesh, double *p, double *u, double
 
 namespace {
 
+// The right-hand side at time t without disturbing a running solve: launch_rhs sets the device
+// clock (which the next step of ndg_solve continues from) to t, so its 4 values are saved before
+// and put back after.
+ndg_status rhs_keeping_clock(ndg_mesh *mesh, double t) {
+    DeviceMesh &d = mesh->d;
+    double saved[4];
+    HIP_TRY(hipMemcpyAsync(saved, d.clock, sizeof saved, hipMemcpyDeviceToHost, mesh->stream), "save clock");
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "save clock sync");
+    HIP_TRY(launch_rhs(d, t, mesh->stream), "rhs kernel");
+    HIP_TRY(hipMemcpyAsync(d.clock, saved, sizeof saved, hipMemcpyHostToDevice, mesh->stream), "restore clock");
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "restore clock sync");
+    return NDG_OK;
+}
+
 // The uniform-grid output of get_solution (5 planes: x, y, p, u, v) or, with `complete`, of
 // get_complete_solution (11 planes: also dq/dt of the current state at `time` and the plane-wave
 // errors), each plane [K][n][n] in the caller's element order; a null plane pointer is skipped.
@@ -1031,7 +1045,7 @@ ndg_status interpolated_output(ndg_mesh *mesh, const char *who, bool complete, d
     DeviceMesh &d = mesh->d;
     if (complete) {
         if (!d.rhs) HIP_TRY(hipMalloc((void **)&d.rhs, sizeof(double) * d.n_q), "hipMalloc rhs");
-        HIP_TRY(launch_rhs(d, time, mesh->stream), "rhs kernel");
+        if ((st = rhs_keeping_clock(mesh, time)) != NDG_OK) return st;
     }
     const int64_t K = d.K, nn = n * n;
     const int n_planes = complete ? 11 : 5;
@@ -1144,7 +1158,7 @@ extern "C" ndg_status ndg_compute_rhs(ndg_mesh *mesh, double t, double *Gp, doub
     if (st != NDG_OK) return st;
     DeviceMesh &d = mesh->d;
     if (!d.rhs) HIP_TRY(hipMalloc((void **)&d.rhs, sizeof(double) * d.n_q), "hipMalloc rhs");
-    HIP_TRY(launch_rhs(d, t, mesh->stream), "rhs kernel");
+    if ((st = rhs_keeping_clock(mesh, t)) != NDG_OK) return st;
     double *const outs[3] = {Gp, Gu, Gv};
     return copy_state(mesh, d.rhs, outs, true, "download rhs");
 }

@@ -97,3 +97,26 @@ def test_solve_with_data_writer(tmp_path):
     err, sig, _, _ = gm.estimate_error()
     assert np.array_equal(final["cell_data"]["PressureError"], err)
     assert np.all(final["cell_data"]["N"] == 5)
+
+
+def test_output_calls_do_not_move_the_solver_clock():
+    """A writer that evaluates the right-hand side or the complete solution at other times (which
+    run the right-hand-side kernels) leaves the solve's device clock alone: with an inflow
+    boundary, whose ghost state depends on the stage times, the final state is the same bit for
+    bit as without those calls."""
+    print("[gpu] output calls and the device clock", flush=True)
+    desc = nd.square_mesh(8, bc=(nd.BC_INFLOW, nd.BC_OUTFLOW, nd.BC_WALL, nd.BC_INFLOW), jitter=0.1, seed=6)
+    states = []
+    for disturb in (False, True):
+        gm = nd.Mesh2D_t(desc, 4, 4, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, nd.NDG_t(4, 5))
+        gm.initial_conditions(None)
+
+        def writer(t, m):
+            if disturb:
+                m.compute_rhs(123.0)
+                m.get_complete_solution(45.0)
+
+        nd.Solver2D_t(0.5, [0.01, 0.02, 0.03]).solve(None, gm, writer)
+        states.append([a.copy() for a in gm.get_solution()])
+    for a, b in zip(*states):
+        assert np.array_equal(a, b)
