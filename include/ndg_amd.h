@@ -212,7 +212,8 @@ ndg_status ndg_get_complete_solution(ndg_mesh *mesh, double time, const ndg_comp
  * current state, packed [element][side][field][N+1]. */
 ndg_status ndg_get_traces(ndg_mesh *mesh, double *out);
 /* One stage's right-hand side G at time t (phases interpolate_to_boundaries .. compute_dg_wave_
- * derivative, include/solvers/Solver2D_t.tpp:46-51), packed; the state is not advanced. */
+ * derivative, include/solvers/Solver2D_t.tpp:46-51), packed; the state is not advanced, and the
+ * stage clock a running ndg_solve / ndg_solve_output continues from is left as it was. */
 ndg_status ndg_compute_rhs(ndg_mesh *mesh, double t, double *Gp, double *Gu, double *Gv);
 /* Solver2D_t::get_delta_t (src/solvers/Solver2D_t.cpp:19-29): a device reduction. */
 ndg_status ndg_get_delta_t(ndg_mesh *mesh, double cfl, double *dt);
