@@ -150,6 +150,7 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
     const int nel = (int)min<int64_t>(epb, K - e0);
     const bool active = el < nel;
 
+    __builtin_amdgcn_s_setprio(3);  // issue this workgroup's loads ahead of other waves' math
     // P0
     int32_t code[FPT];
 #pragma unroll
@@ -194,6 +195,7 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
             nbv[it][2] = nb[2 * NF];
         }
     }
+    __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int it = 0; it < S::VPT; ++it) {
         const int r = tid + it * T;
@@ -533,6 +535,7 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
     const int nel = (int)min<int64_t>(epb, K - e0);
     const bool active = el < nel;
 
+    __builtin_amdgcn_s_setprio(3);  // issue this workgroup's loads ahead of other waves' math
     // P0
     int32_t code[FPT];
 #pragma unroll
@@ -582,6 +585,7 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
             nbv[it][2] = nb[2 * NF];
         }
     }
+    __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int it = 0; it < S::VPT; ++it) {
         const int r = tid + it * T;
