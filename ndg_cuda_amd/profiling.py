@@ -95,6 +95,7 @@ def main(argv=None):
     cc.add_argument("dbs", nargs="+")
     cc.add_argument("--kernel", default="stage_kernel<8, false>")
     cc.add_argument("--by-grid", action="store_true", help="one block per launch shape (grid size)")
+    cc.add_argument("--grid", type=int, help="only dispatches of this grid size (threads)")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         for r in kernel_stats(a.db, a.kernel, a.min_us, a.max_us, a.grid):
@@ -104,7 +105,12 @@ def main(argv=None):
         print(json.dumps(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.kernel), indent=1))
     else:
         for db in a.dbs:
-            grids = [(g, n) for g, n in grid_sizes(db, a.kernel)] if a.by_grid else [(None, None)]
+            if a.grid is not None:
+                grids = [(g, n) for g, n in grid_sizes(db, a.kernel) if g == a.grid]
+            elif a.by_grid:
+                grids = grid_sizes(db, a.kernel)
+            else:
+                grids = [(None, None)]
             for g, n in grids:
                 if g is not None:
                     print(f"-- grid {g} threads ({n} dispatches)")
@@ -114,4 +120,11 @@ def main(argv=None):
 
 
 if __name__ == "__main__":
-    sys.exit(main())
+    try:
+        rc = main()
+        sys.stdout.flush()
+    except BrokenPipeError:
+        # the reader of stdout (head, a pager) closed early: not an error of the summary
+        os.dup2(os.open(os.devnull, os.O_WRONLY), sys.stdout.fileno())
+        rc = 0
+    sys.exit(rc)

@@ -57,3 +57,21 @@ def test_counters_by_grid_and_traffic_calibration(counter_db):
     t = profiling.traffic(fetch, write, cfetch, cwrite, k)
     assert t["fetch_kib_per_dispatch"] == 1000.0 and t["write_kib_per_dispatch"] == 3000.0
     assert t["hbm_bytes_per_launch"] == pytest.approx(1000 * 1024 * 2 + 3000 * 1024)
+
+
+def test_counters_cli_grid_filter_and_closed_pipe(counter_db, capsys):
+    import subprocess
+    import sys
+    k = "stage_kernel<8, false>"
+    db = counter_db("g.db", [(k, 7190400, 1, "SQ_WAVES", 100.0), (k, 12582912, 2, "SQ_WAVES", 1000.0)])
+    assert profiling.main(["counters", str(db), "--kernel", k, "--grid", "12582912"]) == 0
+    out = capsys.readouterr().out
+    assert "-- grid 12582912 threads (1 dispatches)" in out and "1000" in out and "7190400" not in out
+    # a reader that closes the pipe after one line does not turn the summary into a failure
+    root = __import__("os").path.dirname(__import__("os").path.dirname(__import__("os").path.abspath(__file__)))
+    many = " ".join([db.name] * 2000)  # about 300 KB of summary, more than a pipe holds
+    env = dict(__import__("os").environ, PYTHONPATH=root)
+    r = subprocess.run(f"{sys.executable} -m ndg_cuda_amd.profiling counters {many} --kernel '{k}' --by-grid"
+                       " | head -1", shell=True, cwd=db.parent, env=env, capture_output=True, text=True,
+                       executable="/bin/bash")
+    assert r.returncode == 0 and "Traceback" not in r.stderr
