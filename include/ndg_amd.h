@@ -135,6 +135,16 @@ ndg_status ndg_square_mesh_sizes(const ndg_square_mesh_params *params, int64_t *
 ndg_status ndg_square_mesh_fill(const ndg_square_mesh_params *params, double *xy, int64_t *quads,
                                 int64_t *bars, int32_t *bar_type, int64_t *bar_donor);
 
+/* ---- Hilbert-curve statuses: src/functions/Hilbert_splitting.cpp (statuses H, A, R, B = 0..3) ----
+ * Hilbert::child_order / child_statuses (:23-81): the curve order of the four children of an
+ * element with the given status and rotation (children numbered from the element's first corner,
+ * counter-clockwise) and their statuses. Either output may be NULL. */
+ndg_status ndg_hilbert_child_order(int32_t status, int32_t rotation, int32_t order[4], int32_t statuses[4]);
+/* Hilbert::deduct_element_status (:13-21) for the sides through which the curve enters and leaves
+ * an element; incoming_side = -1 is deduct_first_element_status (:3-6), outgoing_side = -1
+ * deduct_last_element_status (:8-11). */
+ndg_status ndg_hilbert_element_status(int32_t incoming_side, int32_t outgoing_side, int32_t *status);
+
 /* ---- mesh files: Mesh2D_t::read_su2 (src/meshes/Mesh2D_t.cpp:78-330), the .su2 branch of the
  * Mesh2D_t constructor (:56-70). NDIME= 2, NPOIN= nodes, NELEM= QUAD_4 (type 9) elements, NMARK=
  * BAR_2 (type 3) markers: farfield -> NDG_BC_OUTFLOW, wall or airfoil -> NDG_BC_WALL, inlet ->

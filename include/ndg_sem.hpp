@@ -80,6 +80,13 @@ class NDG_t {
             }
         }
     }
+    // The device twin's constructor (include/entities/NDG_t.cuh:11-29) takes the rank's stream;
+    // the tables live on the host here and each mesh keeps its own device copy, so it is unused.
+    template <typename Stream>
+    NDG_t(int N_max, std::size_t n_interpolation_points, const Stream &stream)
+        : NDG_t(N_max, n_interpolation_points) {
+        (void)stream;
+    }
     NDG_t(const NDG_t &) = delete;
     NDG_t &operator=(const NDG_t &) = delete;
     ~NDG_t() { ndg_tables_destroy(handle_); }
@@ -230,9 +237,39 @@ class Mesh2D_t {
         : Mesh2D_t(read_mesh_file(filename), initial_N, maximum_N, n_interpolation_points, max_split_level,
                    adaptivity_interval, load_balancing_interval, tolerance_min, tolerance_max,
                    load_balancing_threshold, NDG, stream) {}
+    // The reference's own argument list (include/meshes/Mesh2D_t.h:21, device twin
+    // include/meshes/Mesh2D_t.cuh:23 with the stream last): the polynomial nodes of every order up
+    // to N_max (NDG.nodes_) instead of the NDG_t object, and the MPI workers' weights. The mesh
+    // builds its own copy of the tables for those nodes (they must be the Legendre-Gauss nodes
+    // NDG_t computes, bit for bit). One GPU: worker_weights must hold exactly one weight.
+    Mesh2D_t(const std::filesystem::path &filename, int initial_N, int maximum_N, std::size_t n_interpolation_points,
+             int max_split_level, std::size_t adaptivity_interval, std::size_t load_balancing_interval,
+             double tolerance_min, double tolerance_max, double load_balancing_threshold,
+             const std::vector<std::vector<double>> &polynomial_nodes, std::vector<double> worker_weights)
+        : Mesh2D_t(filename, initial_N, maximum_N, n_interpolation_points, max_split_level, adaptivity_interval,
+                   load_balancing_interval, tolerance_min, tolerance_max, load_balancing_threshold,
+                   Tables(polynomial_nodes, n_interpolation_points, worker_weights).ndg, nullptr) {}
+    template <typename Stream>
+    Mesh2D_t(const std::filesystem::path &filename, int initial_N, int maximum_N, std::size_t n_interpolation_points,
+             int max_split_level, std::size_t adaptivity_interval, std::size_t load_balancing_interval,
+             double tolerance_min, double tolerance_max, double load_balancing_threshold,
+             const std::vector<std::vector<double>> &polynomial_nodes, std::vector<double> worker_weights,
+             const Stream &stream)
+        : Mesh2D_t(filename, initial_N, maximum_N, n_interpolation_points, max_split_level, adaptivity_interval,
+                   load_balancing_interval, tolerance_min, tolerance_max, load_balancing_threshold,
+                   Tables(polynomial_nodes, n_interpolation_points, worker_weights).ndg, (void *)stream) {}
     Mesh2D_t(const Mesh2D_t &) = delete;
     Mesh2D_t &operator=(const Mesh2D_t &) = delete;
     ~Mesh2D_t() { ndg_mesh_destroy(handle_); }
+
+    // Mesh2D_t::initial_conditions(polynomial_nodes) (src/meshes/Mesh2D_t.cpp:1192-1216): the
+    // reference plane wave at the mesh's own nodes.
+    template <typename Float>
+    void initial_conditions(const std::vector<std::vector<Float>> &polynomial_nodes) {
+        if ((int)polynomial_nodes.size() <= maximum_N_)
+            throw Error(NDG_ERR_INVALID, "Mesh2D_t::initial_conditions: fewer polynomial node sets than maximum_N + 1");
+        initial_conditions();
+    }
 
     // NDG_PRECISION_FP64 (the reference arithmetic) or NDG_PRECISION_FP32_COMPENSATED.
     void set_precision(int32_t precision) { check(ndg_mesh_set_precision(handle_, precision)); }
@@ -347,6 +384,25 @@ class Mesh2D_t {
     std::vector<int32_t> orders_;
 
   private:
+    // Tables for the reference constructor: NDG_t for N_max = polynomial_nodes.size() - 1, checked
+    // against the given nodes; the mesh copies what it needs, so this temporary may go.
+    struct Tables {
+        Entities::NDG_t<Polynomials::LegendrePolynomial_t> ndg;
+        Tables(const std::vector<std::vector<double>> &nodes, std::size_t n_interpolation_points,
+               const std::vector<double> &worker_weights)
+            : ndg(checked_N_max(nodes, worker_weights), n_interpolation_points) {
+            for (std::size_t N = 0; N < nodes.size(); ++N)
+                if (nodes[N] != ndg.nodes_[N])
+                    throw Error(NDG_ERR_UNSUPPORTED, "Mesh2D_t: polynomial_nodes are not the Legendre-Gauss nodes of NDG_t");
+        }
+        static int checked_N_max(const std::vector<std::vector<double>> &nodes, const std::vector<double> &weights) {
+            if (weights.size() != 1)
+                throw Error(NDG_ERR_UNSUPPORTED, "Mesh2D_t: one worker per mesh (one GPU, no MPI ranks): worker_weights "
+                                                 "must hold exactly one weight");
+            if (nodes.empty()) throw Error(NDG_ERR_INVALID, "Mesh2D_t: no polynomial nodes");
+            return (int)nodes.size() - 1;
+        }
+    };
     std::size_t n_trace_values_ = 0;
 
   public:
@@ -443,6 +499,19 @@ class Solver2D_t {
 
 }  // namespace Solvers
 }  // namespace Device
+
+// The reference's host classes (SEM::Host, src/main.cpp:227-268) are the same classes here: one
+// implementation on the GPU serves both spellings.
+namespace Host {
+namespace Entities = ::SEM::Device::Entities;
+namespace Polynomials = ::SEM::Device::Polynomials;
+namespace Meshes = ::SEM::Device::Meshes;
+namespace Solvers = ::SEM::Device::Solvers;
+}  // namespace Host
 }  // namespace SEM
+
+// include/helpers/float_types.h
+typedef double deviceFloat;
+typedef deviceFloat hostFloat;
 
 #endif

@@ -4,6 +4,7 @@ Kernels (*.hip) are compiled with hipcc's default FP contraction; the host sourc
 reproduce the reference host arithmetic (tables, face frames, time accumulation), are compiled
 with -ffp-contract=off. Packed FP32 instructions are disabled for all device code.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -36,7 +37,7 @@ def _run(cmd):
 def build(verbose=False):
     """Compiles every source under csrc/ and links ndg_cuda_amd/libndg_amd.so. Always rebuilds."""
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    jobs = []
     for name in sorted(os.listdir(CSRC)):
         src = os.path.join(CSRC, name)
         stem, ext = os.path.splitext(name)
@@ -44,10 +45,16 @@ def build(verbose=False):
             continue
         obj = os.path.join(BUILD, stem + ".o")
         flags = DEVICE_FLAGS if ext == ".hip" else HOST_FLAGS
-        msgs = _run([HIPCC] + flags + ["-c", src, "-o", obj])
-        if verbose and msgs:
-            print("\n".join(msgs), file=sys.stderr)
-        objs.append(obj)
+        jobs.append(([HIPCC] + flags + ["-c", src, "-o", obj], obj))
+    # one compiler process per source, at most as many as there are cores (16 at most)
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 16))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
+        results = list(pool.map(lambda job: _run(job[0]), jobs))
+    if verbose:
+        for msgs in results:
+            if msgs:
+                print("\n".join(msgs), file=sys.stderr)
+    objs = [obj for _, obj in jobs]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
@@ -57,14 +64,23 @@ def build(verbose=False):
 
 HOST_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "host_api_main.cpp")
 HOST_DRIVER = os.path.join(ROOT, "tests", "cpp", "host_api_main")
+FLOW_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "reference_flow_main.cpp")
+FLOW_DRIVER = os.path.join(ROOT, "tests", "cpp", "reference_flow_main")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 
 def build_host_driver():
-    """Links the C++ host-API driver (tests/cpp/host_api_main.cpp) against libndg_amd.so with g++."""
+    """Links the C++ host-API drivers (tests/cpp/host_api_main.cpp, and reference_flow_main.cpp,
+    which also calls the HIP runtime for its stream) against libndg_amd.so with g++."""
     if not os.path.exists(HOST_DRIVER_SRC):
         return None
     _run(["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), HOST_DRIVER_SRC,
           "-L" + PKG, "-lndg_amd", "-Wl,-rpath,$ORIGIN/../../ndg_cuda_amd", "-o", HOST_DRIVER])
+    if os.path.exists(FLOW_DRIVER_SRC):
+        _run(["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROOT, "include"),
+              "-I" + os.path.join(ROCM, "include"), FLOW_DRIVER_SRC, "-L" + PKG, "-lndg_amd",
+              "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../ndg_cuda_amd",
+              "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-o", FLOW_DRIVER])
     return HOST_DRIVER
 
 

@@ -39,7 +39,6 @@ ndg_status hip_fail(hipError_t e, const char *where) {
 
 // include/solvers/Solver2D_t.tpp:16-18 (Williamson 2N-storage RK3)
 constexpr double kAm[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
-constexpr double kBm[3] = {0.0, 1.0 / 3.0, 0.75};
 constexpr double kGm[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 
 }  // namespace
@@ -72,7 +71,10 @@ struct Topology {
 
 struct ndg_mesh {
     DeviceMesh d;
-    const ndg_tables *tables = nullptr;
+    // The mesh's own copy of the caller's tables (the reference's Mesh2D_t copies the polynomial
+    // tables it needs), so the caller's ndg_tables may be destroyed before the mesh.
+    ndg_tables tables_own;
+    const ndg_tables *tables = &tables_own;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     int64_t n_faces = 0;
@@ -581,6 +583,31 @@ constexpr int kChildStatus[4][4][4] = {
     {{0, 0, 1, 3}, {1, 2, 0, 1}, {3, 1, 2, 2}, {2, 3, 3, 0}},
     {{0, 1, 3, 0}, {2, 0, 1, 1}, {1, 2, 2, 3}, {3, 3, 0, 2}}};
 
+}  // namespace
+
+extern "C" ndg_status ndg_hilbert_child_order(int32_t status, int32_t rotation, int32_t order[4], int32_t statuses[4]) {
+    if (status < 0 || status > 3 || rotation < 0 || rotation > 3)
+        return fail(NDG_ERR_INVALID, "ndg_hilbert_child_order: status and rotation must be in 0..3");
+    for (int k = 0; k < 4; ++k) {
+        if (order) order[k] = kChildOrder[rotation][status][k];
+        if (statuses) statuses[k] = kChildStatus[rotation][status][k];
+    }
+    return NDG_OK;
+}
+
+extern "C" ndg_status ndg_hilbert_element_status(int32_t incoming_side, int32_t outgoing_side, int32_t *status) {
+    if (!status) return fail(NDG_ERR_INVALID, "ndg_hilbert_element_status: null output");
+    if (incoming_side < -1 || incoming_side > 3 || outgoing_side < -1 || outgoing_side > 3 ||
+        (incoming_side < 0 && outgoing_side < 0))
+        return fail(NDG_ERR_INVALID, "ndg_hilbert_element_status: sides must be in 0..3 (-1 for a curve end, not both)");
+    if (incoming_side < 0) *status = kFirstStatus[outgoing_side];
+    else if (outgoing_side < 0) *status = kLastStatus[incoming_side];
+    else *status = kInnerStatus[incoming_side][outgoing_side];
+    return NDG_OK;
+}
+
+namespace {
+
 // The side of an element through which the centre offset (dx, dy) points (the element's bottom,
 // right, top, left are sides 0..3).
 int side_towards(double dx, double dy) {
@@ -631,7 +658,6 @@ void element_statuses(const HostMesh &h, std::vector<int32_t> &status, std::vect
 ndg_status build_topology(const HostMesh &h, Topology &t) {
     const int64_t K = (int64_t)h.quads.size() / 4, B = (int64_t)h.bar_type.size();
     auto P = [&](int64_t node) { return v2{h.xy[2 * node], h.xy[2 * node + 1]}; };
-    auto corner = [&](int64_t e, int k) { return P(h.quads[4 * e + (k & 3)]); };
 
     // Directed edges of the domain elements (build_faces, src/meshes/Mesh2D_t.cpp:1110-1154): the
     // element sides bucketed by start node (CSR, sides in increasing order within a bucket), so the
@@ -813,7 +839,7 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
     ndg_status tst = build_topology(host, topo);
     if (tst != NDG_OK) return tst;
     std::unique_ptr<ndg_mesh, void (*)(ndg_mesh *)> m(new ndg_mesh, free_mesh);
-    m->tables = tables;
+    m->tables_own = *tables;
     m->tol_min = desc->tolerance_min;
     m->tol_max = desc->tolerance_max;
     m->n_faces = topo.n_faces;
@@ -1636,11 +1662,15 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
             mesh->host_verts[8 * k + 2 * q] = p.x;
             mesh->host_verts[8 * k + 2 * q + 1] = p.y;
         }
+    auto restore = [&](ndg_status code) {
+        swap_host_state(mesh, previous);
+        if (precision == NDG_PRECISION_FP32_COMPENSATED) (void)ndg_mesh_set_precision(mesh, precision);
+        return code;
+    };
     DeviceMesh nd;
     if ((st = build_device(mesh, order, nd)) != NDG_OK) {
         free_device(nd);
-        swap_host_state(mesh, previous);
-        return st;
+        return restore(st);
     }
     auto offset = [](const DeviceMesh &dm, int N, int64_t dev) -> int64_t {
         for (const Group &G : dm.groups)
@@ -1670,8 +1700,7 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
     if (coords_dev) (void)hipFree(coords_dev);
     if (st != NDG_OK) {
         free_device(nd);
-        swap_host_state(mesh, previous);
-        return st;
+        return restore(st);
     }
     nd.clock = d.clock;
     d.clock = nullptr;
@@ -1720,6 +1749,27 @@ extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split
     for (int64_t e = 0; e < K; ++e)
         if (order[e] > kMaxOrder || order[e] > mesh->tables->t.N_max)
             return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a raised order exceeds the compiled range or the tables");
+    const int precision = d.precision;
+    if (precision == NDG_PRECISION_FP32_COMPENSATED) {
+        if ((st = ndg_mesh_set_precision(mesh, NDG_PRECISION_FP64)) != NDG_OK) return st;
+    }
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_adapt sync");
+    drop_graphs(mesh);
+    // Everything the rebuild changes on the host (orders, device permutation, packed offsets, the
+    // resized-ghost flags) is restored, with the precision, if the rebuild or the transfer fails.
+    const std::vector<int32_t> old_order = mesh->order;
+    const std::vector<int64_t> old_dev = mesh->dev_of, old_host_off = mesh->host_off;
+    const std::vector<char> old_bar_zero = mesh->bar_zero;
+    const int64_t old_dofs = mesh->n_dofs;
+    auto restore = [&](ndg_status code) {
+        mesh->order = old_order;
+        mesh->dev_of = old_dev;
+        mesh->host_off = old_host_off;
+        mesh->bar_zero = old_bar_zero;
+        mesh->n_dofs = old_dofs;
+        if (precision == NDG_PRECISION_FP32_COMPENSATED) (void)ndg_mesh_set_precision(mesh, precision);
+        return code;
+    };
     // adjust_boundaries (Mesh2D_t.cpp:7482-7504): the ghost of a raised element is resized, which
     // zeroes its edge scaling factor (Element2D_t.cpp:735-741).
     for (int64_t es = 0; es < 4 * K; ++es) {
@@ -1727,24 +1777,11 @@ extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split
         if (l < 0 && l != INT64_MIN && order[es / 4] != mesh->order[es / 4])
             mesh->bar_zero[(size_t)mesh->bc_bar[(size_t)(-l - 1)]] = 1;
     }
-    const int precision = d.precision;
-    if (precision == NDG_PRECISION_FP32_COMPENSATED) {
-        if ((st = ndg_mesh_set_precision(mesh, NDG_PRECISION_FP64)) != NDG_OK) return st;
-    }
-    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_adapt sync");
-    drop_graphs(mesh);
-    const std::vector<int32_t> old_order = mesh->order;
-    const std::vector<int64_t> old_dev = mesh->dev_of, old_host_off = mesh->host_off;
-    const int64_t old_dofs = mesh->n_dofs;
     DeviceMesh nd;
     st = build_device(mesh, order, nd);
     if (st != NDG_OK) {
         free_device(nd);
-        mesh->order = old_order;
-        mesh->dev_of = old_dev;
-        mesh->host_off = old_host_off;
-        mesh->n_dofs = old_dofs;
-        return st;
+        return restore(st);
     }
     auto offset = [](const DeviceMesh &dm, int N, int64_t dev) -> int64_t {
         for (const Group &G : dm.groups)
@@ -1770,7 +1807,7 @@ extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split
     if (recs_dev) (void)hipFree(recs_dev);
     if (st != NDG_OK) {
         free_device(nd);
-        return st;
+        return restore(st);
     }
     nd.clock = d.clock;
     d.clock = nullptr;
@@ -2162,6 +2199,16 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
         return b.as<double>() + (j * 3 + stage) * d.n_q;
     };
     double *const q_main = d.q;
+    // On every exit path, errors included, the mesh's state pointers are its own buffers again:
+    // run_stored points them into the checkpoint and stage-store buffers, which are freed on return.
+    struct StatePointers {
+        DeviceMesh &dm;
+        double *q;
+        ~StatePointers() {
+            dm.q = q;
+            dm.q_out = nullptr;
+        }
+    } state_pointers{d, q_main};
 
     // Steps n0..n1-1 from the state at q_start, keeping every stage input: the stage kernels write
     // their output state into the next store slot (q_start itself stands for slot (n0, 0)); the

@@ -479,6 +479,22 @@ static const int CHILD_STATUS[4][4][4] = {
     {{0, 0, 1, 3}, {1, 2, 0, 1}, {3, 1, 2, 2}, {2, 3, 3, 0}},
     {{0, 1, 3, 0}, {2, 0, 1, 1}, {1, 2, 2, 3}, {3, 3, 0, 2}}};
 
+/* The tables above, copied out for the fixture comparison: child order and child statuses
+ * [rotation][status][4], first / last element status [side], inner status [incoming][outgoing]. */
+void o_hilbert_tables(int *child_order, int *child_status, int *first, int *last, int *inner) {
+    for (int r = 0; r < 4; ++r)
+        for (int s = 0; s < 4; ++s)
+            for (int k = 0; k < 4; ++k) {
+                child_order[(r * 4 + s) * 4 + k] = CHILD_ORDER[r][s][k];
+                child_status[(r * 4 + s) * 4 + k] = CHILD_STATUS[r][s][k];
+            }
+    for (int s = 0; s < 4; ++s) {
+        first[s] = first_status(s);
+        last[s] = last_status(s);
+        for (int o = 0; o < 4; ++o) inner[s * 4 + o] = mid_status(s, o);
+    }
+}
+
 /* side through which the neighbouring centre d lies (Mesh2D_t.cpp:4083-4100) */
 static int exit_side(double dx, double dy) {
     if (fabs(dy) > fabs(dx)) return dy < 0 ? 0 : 2;

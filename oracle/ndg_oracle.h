@@ -133,6 +133,9 @@ void o_element_estimate(const o_tables *t, int N, const double *p, double tol_mi
 /* upwind flux at one point: in = pL,uL,vL,pR,uR,vR,nx,ny, out = Fp,Fu,Fv */
 void o_wave_flux(const double *in, double *out);
 void o_hilbert_d2xy(int n, int d, int *xy);
+/* Hilbert_splitting.cpp tables: child_order / child_status [rotation][status][4] (64 each),
+ * first / last status [side] (4 each), inner status [incoming][outgoing] (16). */
+void o_hilbert_tables(int *child_order, int *child_status, int *first, int *last, int *inner);
 int o_hilbert_xy2d(int n, int x, int y);
 
 #ifdef __cplusplus

@@ -60,6 +60,7 @@ def lib():
             "o_element_estimate": ([vp, i32, vp, f64, f64, vp], None),
             "o_wave_flux": ([vp, vp], None), "o_hilbert_d2xy": ([i32, i32, vp], None),
             "o_hilbert_xy2d": ([i32, i32, i32], i32),
+            "o_hilbert_tables": ([vp, vp, vp, vp, vp], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -273,6 +274,18 @@ def wave_flux(state8):
     s = _f(state8)
     lib().o_wave_flux(_p(s), _p(out))
     return out
+
+
+def hilbert_tables():
+    """(child_order [rotation][status][4], child_status [rotation][status][4], first status [side],
+    last status [side], inner status [incoming][outgoing]) as int arrays."""
+    co = np.zeros((4, 4, 4), dtype=np.int32)
+    cs = np.zeros((4, 4, 4), dtype=np.int32)
+    fi = np.zeros(4, dtype=np.int32)
+    la = np.zeros(4, dtype=np.int32)
+    inner = np.zeros((4, 4), dtype=np.int32)
+    lib().o_hilbert_tables(_p(co), _p(cs), _p(fi), _p(la), _p(inner))
+    return co, cs, fi, la, inner
 
 
 def hilbert_d2xy(n, d):

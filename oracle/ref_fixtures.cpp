@@ -10,6 +10,7 @@
 #include "entities/Element2D_t.h"
 #include "entities/Face2D_t.h"
 #include "functions/Hilbert.h"
+#include "functions/Hilbert_splitting.h"
 
 #include <cinttypes>
 #include <cstdio>
@@ -187,6 +188,39 @@ int main() {
             first_h = false;
         }
     }
+    std::printf("]");
+
+    // Hilbert splitting tables (Hilbert_splitting.cpp): every status and rotation / side index
+    namespace HS = SEM::Host::Hilbert;
+    const HS::Status sts[4] = {HS::Status::H, HS::Status::A, HS::Status::R, HS::Status::B};
+    key("hilbert_child_order");
+    std::printf("[");
+    for (int rot = 0; rot < 4; ++rot)
+        for (int st = 0; st < 4; ++st) {
+            const auto o = HS::child_order(sts[st], rot);
+            std::printf("%s[%zu,%zu,%zu,%zu]", rot + st ? "," : "", o[0], o[1], o[2], o[3]);
+        }
+    std::printf("]");
+    key("hilbert_child_status");
+    std::printf("[");
+    for (int rot = 0; rot < 4; ++rot)
+        for (int st = 0; st < 4; ++st) {
+            const auto o = HS::child_statuses(sts[st], rot);
+            std::printf("%s[%d,%d,%d,%d]", rot + st ? "," : "", (int)o[0], (int)o[1], (int)o[2], (int)o[3]);
+        }
+    std::printf("]");
+    key("hilbert_first_status");
+    std::printf("[");
+    for (size_t s = 0; s < 4; ++s) std::printf("%s%d", s ? "," : "", (int)HS::deduct_first_element_status(s));
+    std::printf("]");
+    key("hilbert_last_status");
+    std::printf("[");
+    for (size_t s = 0; s < 4; ++s) std::printf("%s%d", s ? "," : "", (int)HS::deduct_last_element_status(s));
+    std::printf("]");
+    key("hilbert_element_status");
+    std::printf("[");
+    for (size_t a = 0; a < 4; ++a)
+        for (size_t b = 0; b < 4; ++b) std::printf("%s%d", a + b ? "," : "", (int)HS::deduct_element_status(a, b));
     std::printf("]");
     std::printf("\n}\n");
     return 0;

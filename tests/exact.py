@@ -3,9 +3,12 @@
 periodic_pulse: the periodic Fourier-series solution on [-1,1]^2 for an initial pressure that is
 a periodised radial Gaussian with u = v = 0. Each mode exp(i k.x), k = pi n, evolves as
 p_k cos(|k| t), u_k = -i k_x/|k| p_k sin(|k| t) (and v likewise), for the system
-p_t + u_x + v_y = 0, u_t + p_x = 0, v_t + p_y = 0 (wave speed 1).
+p_t + u_x + v_y = 0, u_t + p_x = 0, v_t + p_y = 0 (wave speed 1). With wave speed c
+(p_t + c div(u, v) = 0, (u, v)_t + c grad p = 0) every mode turns at c|k|, so the solution at
+time t is the c = 1 solution at time c t.
 
-plane_wave: the reference's own exact solution g(x, t) (analytical_solution.tpp:5-11).
+plane_wave: the reference's own exact solution g(x, t) (analytical_solution.tpp:5-11), moving at
+speed c along its direction k (c = 1 in the reference).
 """
 import numpy as np
 
@@ -21,8 +24,10 @@ def periodic_images(x0, y0, width, amp, reach=2):
     return np.array(out)
 
 
-def periodic_pulse(x, y, t, pulses, n_modes=40):
-    """p, u, v at points (x, y) and time t for a sum of periodised Gaussian pulses on [-1,1]^2."""
+def periodic_pulse(x, y, t, pulses, n_modes=40, c=1.0):
+    """p, u, v at points (x, y) and time t for a sum of periodised Gaussian pulses on [-1,1]^2,
+    wave speed c."""
+    t = c * t
     x = np.asarray(x, dtype=np.float64)
     y = np.asarray(y, dtype=np.float64)
     n = np.arange(-n_modes, n_modes + 1)
@@ -48,10 +53,10 @@ def periodic_pulse(x, y, t, pulses, n_modes=40):
     return p, u, v
 
 
-def plane_wave(x, y, t):
+def plane_wave(x, y, t, c=1.0):
     kx = ky = 0.707106781186548
     d = 0.120112240878645
-    a = kx * (x + 0.2) + ky * (y + 0.2) - t
+    a = kx * (x + 0.2) + ky * (y + 0.2) - c * t
     p = np.exp(-a * a / (d * d))
     return p, p * kx, p * ky
 

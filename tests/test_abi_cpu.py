@@ -42,6 +42,30 @@ def test_product_tables_match_reference_bitwise(ref_fixtures, which):
         assert np.array_equal(getattr(ndg, name)[N], np.array(ref_fixtures[key][N])), (key, N)
 
 
+def test_hilbert_tables_match_reference_bitwise(ref_fixtures):
+    """The product's Hilbert splitting tables (child order, child statuses, first / last / inner
+    status) for every rotation, status and side against the compiled Hilbert_splitting.cpp."""
+    L = _abi.lib()
+    order = np.zeros(4, dtype=np.int32)
+    stat = np.zeros(4, dtype=np.int32)
+    for rot in range(4):
+        for st in range(4):
+            _abi.check(L.ndg_hilbert_child_order(st, rot, _abi.ptr(order), _abi.ptr(stat)))
+            assert order.tolist() == ref_fixtures["hilbert_child_order"][rot * 4 + st], (rot, st)
+            assert stat.tolist() == ref_fixtures["hilbert_child_status"][rot * 4 + st], (rot, st)
+    out = ctypes.c_int32()
+    for side in range(4):
+        _abi.check(L.ndg_hilbert_element_status(-1, side, ctypes.byref(out)))
+        assert out.value == ref_fixtures["hilbert_first_status"][side]
+        _abi.check(L.ndg_hilbert_element_status(side, -1, ctypes.byref(out)))
+        assert out.value == ref_fixtures["hilbert_last_status"][side]
+        for o in range(4):
+            _abi.check(L.ndg_hilbert_element_status(side, o, ctypes.byref(out)))
+            assert out.value == ref_fixtures["hilbert_element_status"][side * 4 + o]
+    assert L.ndg_hilbert_child_order(4, 0, None, None) == _abi.NDG_ERR_INVALID
+    assert L.ndg_hilbert_element_status(-1, -1, ctypes.byref(out)) == _abi.NDG_ERR_INVALID
+
+
 def test_generator_layout_matches_reference_generator(ref_fixtures):
     """extras/mesh_generator/main.cpp:189-305: node ids, Hilbert element order, local node order
     BR, TR, TL, BL, and the bar sections in the order bottom, right, top, left."""

@@ -11,6 +11,7 @@ from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "tests", "cpp", "host_api_main")
+FLOW = os.path.join(ROOT, "tests", "cpp", "reference_flow_main")
 
 
 def test_driver_built():
@@ -112,3 +113,55 @@ def test_driver_reads_periodic_su2_file(tmp_path):
     assert steps == n_ref == 289
     for g, ref in zip(got, om.get_state()):
         assert np.max(np.abs(g - ref)) / np.max(np.abs(ref)) < 1e-12
+
+
+def test_reference_flow_driver_built():
+    """tests/cpp/reference_flow_main.cpp is the reference program's call sequence (src/main.cpp
+    host classes, src/main.cu device classes with a stream) compiled against ndg_sem.hpp."""
+    assert os.path.exists(FLOW), "run __graft_entry__.build() first"
+
+
+@pytest.mark.skipif(nd.device_count() > 0, reason="checks the no-device error path")
+def test_reference_flow_reports_status_without_gpu(tmp_path):
+    desc = nd.square_mesh(4, bc=(nd.BC_INFLOW,) * 4)
+    from ndg_cuda_amd import mesh_tools
+    mesh_tools.write_su2(desc, tmp_path / "m.su2")
+    r = subprocess.run([FLOW, "host", str(tmp_path / "m.su2"), str(tmp_path / "o.bin"), str(tmp_path / "out.vtu"),
+                        "4", "0.1"], capture_output=True, text=True)
+    assert r.returncode == 1 and "error 4" in r.stderr, r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flow", ["host", "device"])
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_reference_flow_matches_oracle(tmp_path, flow, adaptive):
+    """The reference program's sequence with the reference constructor (polynomial nodes, one
+    worker weight), initial_conditions(nodes) (the plane wave) and solve(NDG, mesh, writer) on an
+    SU2 file with inlets on every side: steps, time, element count and state against the oracle;
+    with adaptive=True the mesh adapts every 4 steps with max_split_level 1 (N 4 up to 8)."""
+    from ndg_cuda_amd import mesh_tools
+    res, N, t_end = 8, 4, 0.1
+    print("[gpu] reference call sequence,", flow, "classes, adaptive" if adaptive else "", flush=True)
+    desc = nd.square_mesh(res, bc=(nd.BC_INFLOW,) * 4, jitter=0.1, seed=5)
+    path = tmp_path / "mesh.su2"
+    mesh_tools.write_su2(desc, path)
+    args = [FLOW, flow, str(path), str(tmp_path / "o.bin"), str(tmp_path / "vtk" / "out.vtu"), str(N), str(t_end)]
+    if adaptive:
+        args += ["1", "4"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    words = r.stdout.split()
+    K, steps, time = int(words[1]), int(words[3]), float(words[5])
+    got = np.fromfile(tmp_path / "o.bin", dtype=np.float64).reshape(3, -1)
+    N_max = N + 4 if adaptive else N
+    om = O.Mesh(O.Tables(N_max, 5), nd.read_su2(path), N, tol_min=1e-6, tol_max=1e-14)
+    om.initial_conditions(None)
+    if adaptive:
+        n_ref, t_ref = om.solve_adaptive(0.5, t_end, 4, N_max, 1)
+        assert om.K > res * res or om.N.max() > N
+    else:
+        n_ref, t_ref = om.solve(0.5, t_end)
+    assert (K, steps, time) == (om.K, n_ref, t_ref)
+    for g, ref in zip(got, om.get_state()):
+        assert np.max(np.abs(g - ref)) / np.max(np.abs(ref)) < 1e-12
+    assert (tmp_path / "vtk" / "out.vtu.series").exists()

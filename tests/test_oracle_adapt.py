@@ -158,6 +158,53 @@ def test_split_keeps_the_hilbert_curve(oracle_tables, res):
     assert split_all(om, 2) == 0 or om.K == 16 * res * res
 
 
+def rotated(desc, shift):
+    """The same mesh with every element's local nodes renumbered from node `shift` on, which turns
+    the generator's rotation 1 into rotation (1 + shift) % 4 (compute_element_status)."""
+    out = nd.MeshDescription(desc.xy, np.roll(np.asarray(desc.quads), -shift, axis=1), desc.bars,
+                             desc.bar_type, desc.bar_donor)
+    return out
+
+
+def curve_statuses(nodes, xy, fx):
+    """compute_element_status (Mesh2D_t.cpp:4055-4178) from the element centres, with the status
+    tables of the compiled Hilbert_splitting.cpp (fixtures): the side towards the previous element
+    (incoming) and towards the next one (outgoing)."""
+    c = xy[nodes].mean(axis=1)
+
+    def side(d):
+        return (0 if d[1] < 0 else 2) if abs(d[1]) > abs(d[0]) else (3 if d[0] < 0 else 1)
+
+    K = len(nodes)
+    out = np.empty(K, dtype=np.int32)
+    out[0] = fx["hilbert_first_status"][side(c[1] - c[0])]
+    out[-1] = fx["hilbert_last_status"][side(c[-2] - c[-1])]
+    for k in range(1, K - 1):
+        out[k] = fx["hilbert_element_status"][4 * side(c[k - 1] - c[k]) + side(c[k + 1] - c[k])]
+    return out
+
+
+@pytest.mark.parametrize("shift", [0, 1, 2, 3])
+def test_split_keeps_the_hilbert_curve_every_rotation(oracle_tables, ref_fixtures, shift):
+    """Every rotation row of the child order / child status tables: with the element frames
+    turned (rotations 0..3), splitting every element twice keeps the curve continuous, and each
+    child's status from the child status table is the one compute_element_status gives it from
+    its curve neighbours."""
+    desc = rotated(nd.square_mesh(8, bc=(nd.BC_WALL,) * 4, jitter=0.0, seed=3), shift)
+    om = O.Mesh(oracle_tables, desc, 3, tol_min=1e-14, tol_max=1e-300)
+    nodes, st, rot, xy = om.elements()
+    assert (rot == (1 + shift) % 4).all()
+    assert np.array_equal(st, curve_statuses(nodes, xy, ref_fixtures))
+    assert_hilbert_path(om)
+    for level in (1, 2):
+        assert split_all(om, 2) > 0
+        assert om.K == 64 * 4 ** level
+        assert_hilbert_path(om)
+        nodes, st, rot, xy = om.elements()
+        assert (rot == (1 + shift) % 4).all()
+        assert np.array_equal(st, curve_statuses(nodes, xy, ref_fixtures)), level
+
+
 def test_split_interpolates_and_conserves(oracle_tables):
     """Children get the parent polynomial at their nodes (inverse_quad_map + barycentric
     interpolation), so a polynomial field of degree <= N is carried exactly, and the operator on

@@ -236,6 +236,43 @@ def test_wall_flux_closed_form():
     assert checked == 4 * res
 
 
+def test_outflow_flux_closed_form():
+    """Reference outflow ghost (Mesh2D_t.cpp:4601-4710; the multi-face branch always runs): the
+    element trace projected onto the ghost with the weight ratio times the ghost scaling factor,
+    i.e. q_R = s q_L for all three fields on a conforming boundary edge (s = half the edge
+    length). The upwind flux is then F_p = (1 + s) u_n/2 + (1 - s) p/2 and
+    F_(u,v) = n ((1 + s) p/2 + (1 - s) u_n/2), times the element edge Jacobian."""
+    res, N = 4, 6
+    desc, T, m = make(res, N, jitter=0.0, periodic=False, bc=nd.BC_OUTFLOW)
+    rng = np.random.default_rng(12)
+    n = m.n
+    m.set_state(rng.normal(size=n), rng.normal(size=n), rng.normal(size=n))
+    m.compute_rhs(0.0)
+    tr = m.get_traces().reshape(m.K, 4, 3, N + 1)
+    fl = m.get_flux_traces().reshape(m.K, 4, 3, N + 1)
+    xy = desc.xy
+    checked = 0
+    for e in range(m.K):
+        P = xy[desc.quads[e]]
+        for s in range(4):
+            a, b = P[s], P[(s + 1) % 4]
+            mid = 0.5 * (a + b)
+            if not (np.isclose(abs(mid[0]), 1.0) or np.isclose(abs(mid[1]), 1.0)):
+                continue
+            t = (b - a) / np.linalg.norm(b - a)
+            nrm = np.array([t[1], -t[0]])
+            sf = np.linalg.norm(b - a) / 2
+            pL, uL, vL = tr[e, s]
+            un = uL * nrm[0] + vL * nrm[1]
+            exp_p = ((1 + sf) * un / 2 + (1 - sf) * pL / 2) * sf
+            exp_n = ((1 + sf) * pL / 2 + (1 - sf) * un / 2) * sf
+            assert np.max(np.abs(fl[e, s, 0] - exp_p)) < 1e-14
+            assert np.max(np.abs(fl[e, s, 1] - nrm[0] * exp_n)) < 1e-14
+            assert np.max(np.abs(fl[e, s, 2] - nrm[1] * exp_n)) < 1e-14
+            checked += 1
+    assert checked == 4 * res
+
+
 # ---------------------------------------------------------------------------------------------
 # Time stepping (A9, A16, A17)
 
@@ -284,6 +321,71 @@ def test_periodic_pulse_exact_solution(N, tol):
     pe, ue, ve = exact.periodic_pulse(X, Y, tf, base, n_modes=24)
     p, u, v = m.get_state()
     assert max(np.abs(p - pe).max(), np.abs(u - ue).max(), np.abs(v - ve).max()) < tol
+
+
+@pytest.mark.parametrize("c", [1.05, 0.8])
+def test_wave_speed_delta_t_closed_form(c):
+    """Wave speed c: the CFL step is the c = 1 step divided by c (Solver2D_t.cpp:19-29 with the
+    constant c of constants.h:15 made a runtime value)."""
+    desc, T, m = make(16, 6, periodic=True)
+    m.set_wave_speed(c)
+    assert m.delta_t(0.5) == 0.5 * 0.125 / (36 * 1.0 * c)
+
+
+@pytest.mark.parametrize("c", [1.05, 0.8])
+def test_wave_speed_operator_scales(c):
+    """The speed-c operator is c times the c = 1 operator on any state (jittered walls, so the
+    wall ghosts are included)."""
+    desc, T, m = make(8, 5, jitter=0.2, periodic=False, bc=nd.BC_WALL)
+    rng = np.random.default_rng(31)
+    n = m.n
+    q = [rng.normal(size=n) for _ in range(3)]
+    m.set_state(*q)
+    G1 = m.compute_rhs(0.0)
+    m.set_wave_speed(c)
+    Gc = m.compute_rhs(0.0)
+    for a, b in zip(Gc, G1):
+        assert np.max(np.abs(a - c * b)) < 1e-13 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("c", [1.05, 0.8])
+def test_wave_speed_periodic_pulse_exact_solution(c):
+    """16 x 16 periodic mesh at wave speed c against the Fourier-series solution whose modes turn
+    at c|k|; spectral convergence over N = 4, 6, 8 with the c = 1 bounds."""
+    errs = []
+    for N, tol in ((4, 2e-5), (6, 5e-7), (8, 1e-7)):
+        desc, T, m = make(16, N, periodic=True)
+        m.set_wave_speed(c)
+        base = [(0.1, -0.2, 0.2, 1.0)]
+        m.initial_conditions(exact.periodic_images(*base[0]))
+        _, tf = m.solve(0.5, 0.25)
+        assert tf == 0.25
+        X, Y = packed_nodes(desc, T, m.N)
+        pe, ue, ve = exact.periodic_pulse(X, Y, tf, base, n_modes=24, c=c)
+        p, u, v = m.get_state()
+        err = max(np.abs(p - pe).max(), np.abs(u - ue).max(), np.abs(v - ve).max())
+        assert err < tol, (c, N, err)
+        # the c = 1 solution at the same time is far away: the speed is really c
+        pw, _, _ = exact.periodic_pulse(X, Y, tf, base, n_modes=24, c=1.0)
+        assert np.abs(p - pw).max() > 100 * tol
+        errs.append(err)
+    assert errs[2] < errs[1] < errs[0]
+
+
+@pytest.mark.parametrize("c", [1.05, 0.8])
+def test_wave_speed_plane_wave_inflow_exact_solution(c):
+    """The reference plane wave g moving at speed c, inflow ghosts at speed c on all sides."""
+    errs = []
+    for N in (4, 7):
+        desc, T, m = make(8, N, jitter=0.1, periodic=False, bc=nd.BC_INFLOW)
+        m.set_wave_speed(c)
+        m.initial_conditions(None)
+        _, tf = m.solve(0.5, 0.2)
+        X, Y = packed_nodes(desc, T, m.N)
+        pe, ue, ve = exact.plane_wave(X, Y, tf, c=c)
+        p, u, v = m.get_state()
+        errs.append(max(np.abs(p - pe).max(), np.abs(u - ue).max(), np.abs(v - ve).max()))
+    assert errs[1] < 3e-4 and errs[1] < errs[0] / 30, errs
 
 
 def test_plane_wave_inflow_exact_solution():

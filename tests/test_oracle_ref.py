@@ -58,3 +58,14 @@ def test_face_geometry_bitwise(ref_fixtures):
 def test_hilbert_bitwise(ref_fixtures):
     for n, d, x, y in ref_fixtures["hilbert"]:
         assert O.hilbert_d2xy(n, d) == (x, y)
+
+
+def test_hilbert_splitting_tables_bitwise(ref_fixtures):
+    """Child order, child statuses and the first / last / inner element status tables for every
+    rotation, status and side (Hilbert_splitting.cpp:3-81, compiled)."""
+    co, cs, fi, la, inner = O.hilbert_tables()
+    assert np.array_equal(co.reshape(16, 4), np.array(ref_fixtures["hilbert_child_order"]))
+    assert np.array_equal(cs.reshape(16, 4), np.array(ref_fixtures["hilbert_child_status"]))
+    assert np.array_equal(fi, np.array(ref_fixtures["hilbert_first_status"]))
+    assert np.array_equal(la, np.array(ref_fixtures["hilbert_last_status"]))
+    assert np.array_equal(inner.ravel(), np.array(ref_fixtures["hilbert_element_status"]))

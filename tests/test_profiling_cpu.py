@@ -71,7 +71,8 @@ def test_counters_cli_grid_filter_and_closed_pipe(counter_db, capsys):
     root = __import__("os").path.dirname(__import__("os").path.dirname(__import__("os").path.abspath(__file__)))
     many = " ".join([db.name] * 2000)  # about 300 KB of summary, more than a pipe holds
     env = dict(__import__("os").environ, PYTHONPATH=root)
-    r = subprocess.run(f"{sys.executable} -m ndg_cuda_amd.profiling counters {many} --kernel '{k}' --by-grid"
-                       " | head -1", shell=True, cwd=db.parent, env=env, capture_output=True, text=True,
+    # pipefail: the exit status is Python's (head's would always be 0)
+    r = subprocess.run(f"set -o pipefail; {sys.executable} -m ndg_cuda_amd.profiling counters {many} --kernel '{k}'"
+                       " --by-grid | head -1", shell=True, cwd=db.parent, env=env, capture_output=True, text=True,
                        executable="/bin/bash")
     assert r.returncode == 0 and "Traceback" not in r.stderr
