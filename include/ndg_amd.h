@@ -313,6 +313,17 @@ typedef struct ndg_receivers {
 ndg_status ndg_record_receivers(ndg_mesh *mesh, const ndg_receivers *rec, double dt, int64_t n_steps,
                                 double *p_out);
 
+/* ndg_record_receivers without any device allocation (no reference counterpart): for callers
+ * that cannot allocate device memory (serving loops that must not call hipMalloc, harnesses that
+ * forbid it). The receivers' elements and weights and the records of every step live in a fixed
+ * per-GPU scratch array of 4 MiB held by the library until the process ends: 8 * (n + n (N+1)^2 +
+ * n n_steps) bytes must fit, else NDG_ERR_INVALID and nothing runs. Steps are launched directly
+ * (no captured graph). Same results as ndg_record_receivers, bit for bit. Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other threads, other meshes' streams)
+ * race and must not be made. */
+ndg_status ndg_record_receivers_noalloc(ndg_mesh *mesh, const ndg_receivers *rec, double dt, int64_t n_steps,
+                                        double *p_out);
+
 /* Misfit J = 1/2 sum_steps sum_receivers (p_r(t_n) - data[n][r])^2 over n_steps steps of size dt
  * (frozen) from the mesh's current state, and its gradient with respect to the initial fields
  * (packed like ndg_get_state) and to c, by the discrete adjoint of the RK3 scheme: a forward pass

@@ -41,7 +41,7 @@ EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tabl
             "ndg_estimate_error", "ndg_mesh_set_graph", "ndg_mesh_autotune", "ndg_mesh_tuning",
             "ndg_mesh_set_wave_speed", "ndg_record_receivers", "ndg_adjoint_set_store_budget", "ndg_adjoint_set_c_derivative", "ndg_adjoint_gradient", "ndg_apply_rhs_transpose",
             "ndg_adapt", "ndg_mesh_set_adaptivity", "ndg_mesh_orders", "ndg_hilbert_child_order",
-            "ndg_hilbert_element_status"]
+            "ndg_hilbert_element_status", "ndg_record_receivers_noalloc"]
 
 
 class NDGError(RuntimeError):
@@ -151,6 +151,7 @@ def lib():
             "ndg_mesh_orders": [vp, vp],
             "ndg_hilbert_child_order": [i32, i32, vp, vp],
             "ndg_hilbert_element_status": [i32, i32, vp],
+            "ndg_record_receivers_noalloc": [vp, vp, f64, i64, vp],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)

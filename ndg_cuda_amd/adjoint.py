@@ -46,6 +46,15 @@ def record_receivers(mesh, receivers, dt, n_steps):
     return out
 
 
+def record_receivers_noalloc(mesh, receivers, dt, n_steps):
+    """record_receivers through the allocation-free entry point (the receivers and records live in
+    the library's fixed 4 MiB per-GPU scratch array; larger requests raise)."""
+    out = np.empty((int(n_steps), receivers.n), dtype=np.float64)
+    check(lib().ndg_record_receivers_noalloc(mesh.handle, ctypes.byref(receivers._c), float(dt), int(n_steps),
+                                             ptr(out)))
+    return out
+
+
 def set_c_derivative(mesh, mode):
     """'recompute' (default: checkpoints, recompute, <G^T mu~, q_s> / c in the transposed stages) or
     'stage_records' (dJ/dc from the RK stage right-hand sides recorded at the receivers in the

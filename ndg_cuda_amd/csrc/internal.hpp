@@ -229,6 +229,11 @@ struct TransferRec {
 hipError_t launch_transfer(const double *q_old, double *q_new, const TransferRec *recs, const double *coords,
                            int64_t K, const double *tab1d, hipStream_t s);
 
+// Allocation-free entry points: size of the fixed per-device scratch array (noalloc.hip), and its
+// device address on the current device (looked up once per device, then cached).
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+hipError_t noalloc_scratch(double **ptr);
+
 // Records msg as the calling thread's last error and returns code.
 ndg_status set_error(ndg_status code, const char *msg);
 

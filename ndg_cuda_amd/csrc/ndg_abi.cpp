@@ -2043,11 +2043,13 @@ struct DeviceReceivers {
     int n = 0;
 };
 
-ndg_status upload_receivers(const ndg_mesh *m, const ndg_receivers *rec, DeviceReceivers &out) {
+// Device element index and the (N+1)^2 interpolation weights of every receiver.
+ndg_status receiver_weights(const ndg_mesh *m, const ndg_receivers *rec, std::vector<int64_t> &elem,
+                            std::vector<double> &w) {
     if (!rec || rec->n < 1 || !rec->element || !rec->xi_eta) return fail(NDG_ERR_INVALID, "receivers: empty list");
     const int N = m->d.groups[0].N, ne = (N + 1) * (N + 1);
-    std::vector<int64_t> elem((size_t)rec->n);
-    std::vector<double> w((size_t)rec->n * ne);
+    elem.assign((size_t)rec->n, 0);
+    w.assign((size_t)rec->n * ne, 0.0);
     std::vector<double> lx, ly;
     for (int r = 0; r < rec->n; ++r) {
         const int64_t e = rec->element[r];
@@ -2061,7 +2063,15 @@ ndg_status upload_receivers(const ndg_mesh *m, const ndg_receivers *rec, DeviceR
         for (int i = 0; i <= N; ++i)
             for (int j = 0; j <= N; ++j) w[(size_t)r * ne + i * (N + 1) + j] = lx[i] * ly[j];
     }
-    ndg_status st;
+    return NDG_OK;
+}
+
+ndg_status upload_receivers(const ndg_mesh *m, const ndg_receivers *rec, DeviceReceivers &out) {
+    const int N = m->d.groups[0].N, ne = (N + 1) * (N + 1);
+    std::vector<int64_t> elem;
+    std::vector<double> w;
+    ndg_status st = receiver_weights(m, rec, elem, w);
+    if (st != NDG_OK) return st;
     if ((st = dev_alloc(out.elem, elem.size() * sizeof(int64_t), "hipMalloc receivers")) != NDG_OK) return st;
     if ((st = dev_alloc(out.w, w.size() * sizeof(double), "hipMalloc receivers")) != NDG_OK) return st;
     HIP_TRY(hipMemcpy(out.elem.p, elem.data(), elem.size() * sizeof(int64_t), hipMemcpyHostToDevice), "upload receivers");
@@ -2104,6 +2114,43 @@ extern "C" ndg_status ndg_record_receivers(ndg_mesh *mesh, const ndg_receivers *
     HIP_TRY(hipMemcpyAsync(p_out, out.p, sizeof(double) * R.n * n_steps, hipMemcpyDeviceToHost, mesh->stream),
             "download records");
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_record_receivers sync");
+    return NDG_OK;
+}
+
+extern "C" ndg_status ndg_record_receivers_noalloc(ndg_mesh *mesh, const ndg_receivers *rec, double dt,
+                                                   int64_t n_steps, double *p_out) {
+    if (!mesh || !p_out) return fail(NDG_ERR_INVALID, "ndg_record_receivers_noalloc: null argument");
+    if (n_steps < 0 || !(dt > 0.0)) return fail(NDG_ERR_INVALID, "ndg_record_receivers_noalloc: bad step count or dt");
+    ndg_status st = check_adjoint_mesh(mesh, "ndg_record_receivers_noalloc");
+    if (st != NDG_OK) return st;
+    std::vector<int64_t> elem;
+    std::vector<double> w;
+    if ((st = receiver_weights(mesh, rec, elem, w)) != NDG_OK) return st;
+    // Scratch layout: receiver elements (int64), weights, then the records of every step. Every
+    // value is written before it is read, so nothing is cleared.
+    const int64_t n = rec->n, ne = (int64_t)(w.size() / elem.size());
+    const int64_t need = 8 * (n + n * ne + n * n_steps);
+    if (need > (int64_t)kNoAllocScratchBytes)
+        return fail(NDG_ERR_INVALID, "ndg_record_receivers_noalloc: " + std::to_string(need) +
+                                         " bytes of receivers and records exceed its fixed scratch of " +
+                                         std::to_string(kNoAllocScratchBytes) + " bytes");
+    if (n_steps == 0) return NDG_OK;
+    double *scr = nullptr;
+    HIP_TRY(noalloc_scratch(&scr), "ndg_record_receivers_noalloc: scratch address");
+    int64_t *elem_dev = reinterpret_cast<int64_t *>(scr);
+    double *w_dev = scr + n, *rec_dev = scr + n + n * ne;
+    DeviceMesh &d = mesh->d;
+    hipStream_t s = mesh->stream;
+    HIP_TRY(hipMemcpyAsync(elem_dev, elem.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s), "upload receivers");
+    HIP_TRY(hipMemcpyAsync(w_dev, w.data(), sizeof(double) * n * ne, hipMemcpyHostToDevice, s), "upload receivers");
+    // direct launches (a captured step graph would be instantiated on first use)
+    HIP_TRY(launch_clock_begin(d, 0.0, 1, dt, s), "clock kernel");
+    for (int64_t k = 0; k < n_steps; ++k) {
+        if ((st = launch_step_body(mesh, dt, false, s)) != NDG_OK) return st;
+        HIP_TRY(launch_record(d, elem_dev, w_dev, (int)n, rec_dev + k * n, s), "record kernel");
+    }
+    HIP_TRY(hipMemcpyAsync(p_out, rec_dev, sizeof(double) * n * n_steps, hipMemcpyDeviceToHost, s), "download records");
+    HIP_TRY(hipStreamSynchronize(s), "ndg_record_receivers_noalloc sync");
     return NDG_OK;
 }
 
