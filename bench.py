@@ -80,18 +80,30 @@ def mortar_bytes(orders, desc):
 
 
 def cpu_baseline(desc, steps):
-    """The oracle (single-thread C restatement) timed on a bounded sample of the same workload."""
+    """The oracle (the C restatement of the reference host path) timed on a bounded sample of the
+    same workload: single-threaded (the reference's one MPI rank per core), then with OpenMP over
+    the elements and faces of each stage on the cores this process may use (at most 16, one GPU's
+    share of the host). `value` is the multi-threaded rate."""
     from oracle import oracle as O
+    nproc = os.cpu_count() or 1
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    threads = max(1, min(16, avail))
     ot = O.Tables(N, 5)
     om = O.Mesh(ot, desc, N)
-    om.initial_conditions(PULSES)
-    dofs = 3 * om.n
-    t0 = time.perf_counter()
-    om.run_steps(CFL, steps)
-    dt = time.perf_counter() - t0
-    return {"value": dofs * 3 * steps / dt, "unit": "DOF*stage/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} RK3 steps ({3 * steps} stages) of the same {RES}x{RES} N={N} mesh, "
-                      f"oracle/liboracle.so, 1 thread, {dt:.1f} s"}
+    rates = {}
+    for nt, n_steps in ((1, steps), (threads, steps * 4)):
+        used = O.set_threads(nt)
+        om.initial_conditions(PULSES)
+        t0 = time.perf_counter()
+        om.run_steps(CFL, n_steps)
+        rates[nt] = (3 * om.n * 3 * n_steps / (time.perf_counter() - t0), n_steps, used)
+    O.set_threads(1)
+    v1, s1, _ = rates[1]
+    vt, st, used = rates[threads]
+    return {"value": vt, "unit": "DOF*stage/s", "cores": used, "kind": "port",
+            "single_thread": v1, "nproc": nproc, "cores_available": avail,
+            "sample": f"the same {RES}x{RES} N={N} mesh, oracle/liboracle.so: {s1} RK3 steps on 1 thread "
+                      f"({v1:.3g} DOF*stage/s), then {st} steps on {used} OpenMP threads over elements and faces"}
 
 
 def batched_leg(args, nd, torch, stream):
@@ -391,7 +403,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="RK3 steps of the single-thread CPU baseline (4x as many multi-threaded)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="skip every form leg (same as --forms none)")
     ap.add_argument("--forms", default="batched,adaptive,precision,adjoint,fp32c",

@@ -32,6 +32,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* include/helpers/constants.h:15-19 */
 static const double C_WAVE = 1.0;
@@ -1058,9 +1061,13 @@ static void inflow_boundary(const o_mesh *m, int64_t gi, double time) {
 
 /* src/meshes/Mesh2D_t.cpp:3731-3782 boundary_conditions (+ :4712-4730 local_interfaces) */
 static void boundary_conditions(o_mesh *m, double time) {
+#pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < m->n_wall; ++k) wall_boundary(m, m->wall[k]);
+#pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < m->n_sym; ++k) wall_boundary(m, m->sym[k]);
+#pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < m->n_in; ++k) inflow_boundary(m, m->in[k], time);
+#pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < m->n_out; ++k) ghost_project(m, &m->el[m->out[k]], m->out[k]);
     for (int64_t k = 0; k < m->n_if; ++k) {
         const o_elem *src = &m->el[m->if_origin[k]];
@@ -1079,6 +1086,7 @@ static int next_side(int s) { return (s + 1) * !(s + 1 == 4); }
 /* src/meshes/Mesh2D_t.cpp:3803-3893 project_to_faces */
 static void project_to_faces(o_mesh *m) {
     const o_tables *t = m->t;
+#pragma omp parallel for schedule(static)
     for (int64_t fi = 0; fi < m->F; ++fi) {
         o_face *f = &m->fa[fi];
         const int Nf = f->N;
@@ -1124,6 +1132,7 @@ static void project_to_faces(o_mesh *m) {
 
 /* Solver2D_t.cpp:48-63 calculate_wave_fluxes */
 static void wave_fluxes(o_mesh *m) {
+#pragma omp parallel for schedule(static)
     for (int64_t fi = 0; fi < m->F; ++fi) {
         o_face *f = &m->fa[fi];
         for (int i = 0; i <= f->N; ++i) {
@@ -1148,6 +1157,7 @@ void o_wave_flux(const double *in, double *out) {
 /* src/meshes/Mesh2D_t.cpp:3895-4006 project_to_elements */
 static void project_to_elements(o_mesh *m) {
     const o_tables *t = m->t;
+#pragma omp parallel for schedule(static)
     for (int64_t ei = 0; ei < m->K; ++ei) {
         o_elem *e = &m->el[ei];
         const int Ne = e->N;
@@ -1226,6 +1236,7 @@ static void matvec(int N, const double *A, const double *x, double *y) {
  * :31-37) */
 static void dg_derivative(o_mesh *m) {
     const o_tables *t = m->t;
+#pragma omp parallel for schedule(static)
     for (int64_t ei = 0; ei < m->K; ++ei) {
         o_elem *e = &m->el[ei];
         const int N = e->N, nf = N + 1;
@@ -1289,8 +1300,22 @@ static void dg_derivative(o_mesh *m) {
 
 void o_mesh_set_wave_speed(o_mesh *m, double c) { m->c = c; }
 
+/* Threads of the element and face loops of a stage (OpenMP; every loop writes disjoint elements,
+ * faces or ghosts, so any thread count gives the same results bit for bit). 0 = OpenMP's default.
+ * Returns the count in use. */
+int o_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
 /* Solver2D_t.cpp:138-172 rk3_first_step / rk3_step (a = 0 marks the first stage) */
 static void rk3_update(o_mesh *m, int first, double dt, double a, double g) {
+#pragma omp parallel for schedule(static)
     for (int64_t ei = 0; ei < m->K; ++ei) {
         o_elem *e = &m->el[ei];
         const int ne = (e->N + 1) * (e->N + 1);
@@ -1310,6 +1335,7 @@ static void rk3_update(o_mesh *m, int first, double dt, double a, double g) {
 }
 
 void o_compute_rhs(o_mesh *m, double time) {
+#pragma omp parallel for schedule(static)
     for (int64_t ei = 0; ei < m->K; ++ei) {
         o_elem *e = &m->el[ei];
         element_traces(e, m->t->tab[O_LAG_LEFT][e->N], m->t->tab[O_LAG_RIGHT][e->N]);

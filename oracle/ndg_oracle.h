@@ -132,6 +132,8 @@ void o_element_estimate(const o_tables *t, int N, const double *p, double tol_mi
                         double tol_max, double *out);
 /* upwind flux at one point: in = pL,uL,vL,pR,uR,vR,nx,ny, out = Fp,Fu,Fv */
 void o_wave_flux(const double *in, double *out);
+/* Threads of the stage loops (OpenMP, bitwise the same result for any count); 0 = default. */
+int o_set_threads(int n);
 void o_hilbert_d2xy(int n, int d, int *xy);
 /* Hilbert_splitting.cpp tables: child_order / child_status [rotation][status][4] (64 each),
  * first / last status [side] (4 each), inner status [incoming][outgoing] (16). */

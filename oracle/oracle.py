@@ -61,6 +61,7 @@ def lib():
             "o_wave_flux": ([vp, vp], None), "o_hilbert_d2xy": ([i32, i32, vp], None),
             "o_hilbert_xy2d": ([i32, i32, i32], i32),
             "o_hilbert_tables": ([vp, vp, vp, vp, vp], None),
+            "o_set_threads": ([i32], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -274,6 +275,12 @@ def wave_flux(state8):
     s = _f(state8)
     lib().o_wave_flux(_p(s), _p(out))
     return out
+
+
+def set_threads(n):
+    """Threads of the oracle's element and face loops (OpenMP); 0 keeps the default. Returns the
+    thread count in use. Results are the same bit for bit for any count."""
+    return int(lib().o_set_threads(int(n)))
 
 
 def hilbert_tables():

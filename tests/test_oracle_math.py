@@ -498,3 +498,22 @@ def test_hanging_conservation(mixed, jitter):
     Gp, Gu, Gv = m.compute_rhs(0.0)
     W = quadrature_weights(desc, T, m.N)
     assert abs(np.sum(W * Gp)) < 1e-12 * np.sum(W * np.abs(Gp))
+
+
+def test_openmp_threads_give_the_same_bits():
+    """The oracle's stage loops run under OpenMP (elements, faces and ghosts are written by one
+    thread each): 1 and 4 threads give the same state bit for bit on a mixed-order mesh with
+    hanging faces and every boundary kind."""
+    import meshes
+    base = nd.square_mesh(8, bc=(nd.BC_WALL, nd.BC_INFLOW, nd.BC_OUTFLOW, nd.BC_SYMMETRY), jitter=0.1, seed=2)
+    desc = meshes.split_elements(base, [e for e in meshes.interior_elements(base)][::5])
+    desc.quad_N = np.random.default_rng(5).choice([3, 4, 6], size=len(desc.quads)).astype(np.int32)
+    out = []
+    for threads in (1, 4):
+        assert O.set_threads(threads) == threads
+        m = O.Mesh(O.Tables(8, 5), desc, 3)
+        m.initial_conditions(None)
+        m.run_steps(0.5, 5)
+        out.append(np.concatenate(m.get_state()))
+    O.set_threads(1)
+    assert np.array_equal(out[0], out[1])
