@@ -222,6 +222,102 @@ def adaptive_leg(nd, torch, stream, steps=1000, interval=20, graph="off"):
     return out
 
 
+# Equal-error workload: periodic [-1, 1]^2 with an exact solution (the Fourier series of the
+# periodised pulses), the 8 seeded pulses of the batched workload, base mesh EE_RES^2, to t = EE_T.
+EE_RES, EE_T, EE_INTERVAL = 128, 0.25, 20
+EE_TARGETS = (1e-4, 1e-6, 1e-8, 1e-10)
+
+
+def equal_error_leg(nd, torch, stream, res=EE_RES, t_end=EE_T, orders=(4, 6, 8, 10, 12),
+                    tolerances=(1e-3, 1e-5, 1e-7, 1e-9, 1e-11)):
+    """The metric's comparison at equal L2 error. On the periodic multi-pulse workload, solved to
+    t_end with CFL 0.5, every run's relative L2 error against the exact solution (Gauss quadrature
+    of each element) and its time are measured for
+      - plain: uniform order N (FP64, and the FP32 compensated form), for N in `orders`;
+      - batched: the p-adaptive mixed-order path (N0 = 4, N <= 12, no splitting on the periodic
+        square), adapt every 20 steps, after an iterative pre-conditioning of the initial
+        condition (4 rounds of one step), for each estimator tolerance in `tolerances`.
+    For each target error, each form's cheapest run that reaches it is compared with the plain
+    FP64 form's: time ratio, and the plain run's DOF*stage count divided by the form's time (the
+    DOF*stage/s the plain path would need to reach the same error in the same time)."""
+    from ndg_cuda_amd import exact
+    pulses = batched_workload()
+    desc = nd.square_mesh(res, x_periodic=True, y_periodic=True)
+    ndg = nd.NDG_t(12, 5)
+    solver = nd.Solver2D_t(CFL, [t_end])
+
+    def nodes_of(n):
+        return ndg.nodes_[n], ndg.weights_[n]
+
+    def error_of(mesh):
+        X, Y, W = exact.mesh_nodes(desc, mesh.orders_, nodes_of)
+        ex = exact.periodic_pulses(X, Y, pulses, t_end)
+        return exact.l2_error(mesh.get_solution(), ex, W)
+
+    def timed_solve(mesh, pre=None):
+        """Pre-conditioning (if any), then the solve, between HIP events on the mesh stream; the
+        adaptations inside are host-synchronous and counted in the elapsed time."""
+        mesh.synchronize()
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        wall0 = time.perf_counter()
+        start.record(stream)
+        if pre:
+            pre()
+        solver.solve(None, mesh)
+        stop.record(stream)
+        stop.synchronize()
+        return start.elapsed_time(stop) / 1e3, time.perf_counter() - wall0
+
+    runs = []
+    for n in orders:
+        mesh = nd.Mesh2D_t(desc, n, n, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
+        mesh.set_stream(stream.cuda_stream)
+        for prec in ("fp64", "fp32c"):
+            mesh.set_precision("fp64")
+            mesh.initial_conditions(pulses)
+            mesh.set_precision(prec)
+            s, wall = timed_solve(mesh)
+            mesh.set_precision("fp64")
+            runs.append({"form": "plain_" + prec, "N": n, "seconds": s, "wall_s": wall, "steps": solver.n_steps_,
+                         "dof_stages": mesh.n_dofs * 3 * solver.n_steps_, "l2_error": error_of(mesh)})
+            print(f"[equal-error] {runs[-1]}", file=sys.stderr, flush=True)
+        mesh.close()
+    for tol in tolerances:
+        mesh = nd.Mesh2D_t(desc, 4, 12, 5, 0, EE_INTERVAL, 0, tol, 1e-14, 0.0, ndg)
+        mesh.set_stream(stream.cuda_stream)
+        mesh.initial_conditions(pulses)
+        s, wall = timed_solve(mesh, pre=lambda: solver.pre_condition_iterative(ndg, mesh, 4, 1))
+        counts = {int(k): int((mesh.orders_ == k).sum()) for k in np.unique(mesh.orders_)}
+        runs.append({"form": "batched_fp64", "tolerance_min": tol, "seconds": s, "wall_s": wall,
+                     "steps": solver.n_steps_, "final_dofs": mesh.n_dofs, "final_elements_per_order": counts,
+                     "l2_error": error_of(mesh)})
+        print(f"[equal-error] {runs[-1]}", file=sys.stderr, flush=True)
+        mesh.close()
+    table = []
+    for target in EE_TARGETS:
+        row = {"target_l2": target}
+        best = {}
+        for r in runs:
+            if r["l2_error"] <= target and (r["form"] not in best or r["seconds"] < best[r["form"]]["seconds"]):
+                best[r["form"]] = r
+        plain = best.get("plain_fp64")
+        for form, r in sorted(best.items()):
+            entry = {"seconds": r["seconds"], "l2_error": r["l2_error"]}
+            entry.update({k: r[k] for k in ("N", "tolerance_min") if k in r})
+            if plain is not None:
+                entry["speedup_vs_plain_fp64"] = plain["seconds"] / r["seconds"]
+                entry["equivalent_dof_stage_per_s"] = plain["dof_stages"] / r["seconds"]
+            row[form] = entry
+        table.append(row)
+    return {"workload": f"{res}x{res} periodic, 8 seeded pulses (widths 0.05-0.2), CFL {CFL}, to t = {t_end}; "
+                        "error = relative L2 (Gauss quadrature) against the exact Fourier-series solution",
+            "runs": runs, "at_equal_error": table,
+            "note": "seconds = HIP events around pre-conditioning + solve on the mesh stream (host adaptation "
+                    "included); speedup = plain FP64 time / form time, each form's cheapest run reaching the target"}
+
+
 def timed_steps(solver, mesh, stream, torch, steps, t=0.0):
     """Runs `steps` RK3 steps on the mesh stream between two HIP events; returns ms."""
     mesh.synchronize()
@@ -392,6 +488,8 @@ def separate_forms(args, legs, nd, torch, stream):
         forms["precision_1024"] = precision_leg(nd, torch, stream)
     if "adjoint" in legs:
         forms["adjoint_128"] = adjoint_leg(nd, torch, stream, args.adjoint_steps)
+    if "equal_error" in legs:
+        forms["equal_error"] = equal_error_leg(nd, torch, stream)
     if "adaptive" in legs:
         forms["adaptive_256"] = adaptive_leg(nd, torch, stream)
         forms["adaptive_256"]["value_parallel_graphs"] = adaptive_leg(nd, torch, stream, graph="parallel")["value"]
@@ -407,7 +505,7 @@ def main():
                     help="RK3 steps of the single-thread CPU baseline (4x as many multi-threaded)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="skip every form leg (same as --forms none)")
-    ap.add_argument("--forms", default="batched,adaptive,precision,adjoint,fp32c",
+    ap.add_argument("--forms", default="batched,adaptive,precision,adjoint,fp32c,equal_error",
                     help="comma-separated form legs to run after the main line (or 'none')")
     ap.add_argument("--forms-only", action="store_true",
                     help="profiler runs: skip the main line and print only the selected form legs")

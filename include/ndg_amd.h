@@ -248,6 +248,15 @@ ndg_status ndg_solve_output(ndg_mesh *mesh, double cfl, const double *output_tim
                             ndg_output_fn fn, void *user, double *t_final, int64_t *n_steps);
 /* n_steps steps of the same loop without clipping, from *time_io (updated). Stream-ordered. */
 ndg_status ndg_run_steps(ndg_mesh *mesh, double cfl, int64_t n_steps, double *time_io);
+/* Solver2D_t::pre_condition (iterative = 0, include/solvers/Solver2D_t.tpp:138-236) and
+ * pre_condition_iterative (iterative = 1, :238-336): from time 0, steps of the CFL size with an
+ * estimate + adapt (the tables' N_max, the mesh's max_split_level) every pre_condition_interval
+ * steps, until n_adaptivity_steps adaptations are done; the initial condition last given to
+ * ndg_set_ic is then applied to the adapted mesh (iterative: after every adaptation, with the
+ * time set back to 0). The reference exits where this returns NDG_ERR_INVALID (interval 0; here
+ * also no initial condition set). Waits for completion. */
+ndg_status ndg_pre_condition(ndg_mesh *mesh, double cfl, int64_t n_adaptivity_steps, int64_t pre_condition_interval,
+                             int32_t iterative);
 /* Mesh2D_t::adapt (src/meshes/Mesh2D_t.cpp:1644-2268): estimates the error, then
  * - when some element would h-refine (refine && sigma < 1 && split level < max_split_level,
  *   Element2D_t.cpp:802-812), the hp_adapt branch (:5186-5659): each such element becomes four

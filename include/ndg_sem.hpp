@@ -484,6 +484,26 @@ class Solver2D_t {
         mesh.refresh();
     }
 
+    // Solver2D_t::pre_condition (include/solvers/Solver2D_t.tpp:138-236): steps with an adapt every
+    // pre_condition_interval steps until n_adaptivity_steps adaptations are done, then the mesh's
+    // initial condition again on the adapted mesh.
+    template <typename Polynomial>
+    void pre_condition(const Entities::NDG_t<Polynomial> &NDG, Meshes::Mesh2D_t &mesh, std::size_t n_adaptivity_steps,
+                       std::size_t pre_condition_interval) const {
+        (void)NDG;
+        check(ndg_pre_condition(mesh.handle(), CFL_, (int64_t)n_adaptivity_steps, (int64_t)pre_condition_interval, 0));
+        mesh.refresh();
+    }
+    // Solver2D_t::pre_condition_iterative (:238-336): the initial condition again, and the time set
+    // back to 0, after every adaptation.
+    template <typename Polynomial>
+    void pre_condition_iterative(const Entities::NDG_t<Polynomial> &NDG, Meshes::Mesh2D_t &mesh,
+                                 std::size_t n_adaptivity_steps, std::size_t pre_condition_interval) const {
+        (void)NDG;
+        check(ndg_pre_condition(mesh.handle(), CFL_, (int64_t)n_adaptivity_steps, (int64_t)pre_condition_interval, 1));
+        mesh.refresh();
+    }
+
     double get_delta_t(Meshes::Mesh2D_t &mesh) const {
         double dt = 0.0;
         check(ndg_get_delta_t(mesh.handle(), CFL_, &dt));

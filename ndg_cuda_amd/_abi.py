@@ -37,7 +37,7 @@ EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tabl
             "ndg_mesh_create",
             "ndg_mesh_destroy", "ndg_mesh_set_stream", "ndg_mesh_info", "ndg_mesh_set_precision", "ndg_set_ic",
             "ndg_set_state", "ndg_get_state", "ndg_get_solution", "ndg_get_complete_solution", "ndg_get_traces", "ndg_compute_rhs",
-            "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_solve_output", "ndg_run_steps", "ndg_synchronize",
+            "ndg_get_delta_t", "ndg_step", "ndg_solve", "ndg_solve_output", "ndg_run_steps", "ndg_pre_condition", "ndg_synchronize",
             "ndg_estimate_error", "ndg_mesh_set_graph", "ndg_mesh_autotune", "ndg_mesh_tuning",
             "ndg_mesh_set_wave_speed", "ndg_record_receivers", "ndg_adjoint_set_store_budget", "ndg_adjoint_set_c_derivative", "ndg_adjoint_gradient", "ndg_apply_rhs_transpose",
             "ndg_adapt", "ndg_mesh_set_adaptivity", "ndg_mesh_orders", "ndg_hilbert_child_order",
@@ -135,6 +135,7 @@ def lib():
             "ndg_solve": [vp, f64, f64, vp, vp],
             "ndg_solve_output": [vp, f64, vp, i64, vp, vp, vp, vp],
             "ndg_run_steps": [vp, f64, i64, vp],
+            "ndg_pre_condition": [vp, f64, i64, i64, i32],
             "ndg_synchronize": [vp],
             "ndg_estimate_error": [vp, vp, vp, vp, vp],
             "ndg_mesh_set_graph": [vp, i32],

@@ -83,6 +83,8 @@ struct ndg_mesh {
     double *est_dev = nullptr;
     int64_t est_len = 0;
     double *pulses_dev = nullptr;
+    int ic_kind = -1;                        // last initial condition given (re-applied by pre_condition)
+    std::vector<double> ic_pulses;
     std::vector<double> staging;
     // Element e of the caller's order is element dev_of[e] of the device order (grouped by N).
     std::vector<int64_t> dev_of;
@@ -1011,6 +1013,8 @@ extern "C" ndg_status ndg_set_ic(ndg_mesh *mesh, const ndg_ic_desc *ic) {
         HIP_TRY(hipMemcpy(mesh->pulses_dev, ic->pulses, sizeof(double) * 4 * np, hipMemcpyHostToDevice), "upload pulses");
     }
     HIP_TRY(launch_ic(d, ic->kind, np, mesh->pulses_dev, mesh->stream), "ic kernel");
+    mesh->ic_kind = ic->kind;
+    mesh->ic_pulses.assign(ic->pulses, ic->pulses + 4 * np);
     HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * d.n_q, mesh->stream), "hipMemset g");
     HIP_TRY(launch_traces(d, mesh->stream), "traces kernel");
     ndg_status st = refresh_c32(mesh);
@@ -1919,6 +1923,42 @@ extern "C" ndg_status ndg_solve_output(ndg_mesh *mesh, double cfl, const double 
                                        ndg_output_fn fn, void *user, double *t_final, int64_t *n_steps) {
     if (!mesh || !output_times || n_times < 1) return fail(NDG_ERR_INVALID, "ndg_solve_output: null mesh or no output times");
     return solve_impl(mesh, cfl, output_times, n_times, fn, user, t_final, n_steps);
+}
+
+// Solver2D_t::pre_condition (include/solvers/Solver2D_t.tpp:138-236) and pre_condition_iterative
+// (:238-336): steps of the CFL size, estimate + adapt every pre_condition_interval steps until
+// n_adaptivity_steps adaptations are done, and the initial condition applied again to the adapted
+// mesh: once at the end, or (iterative) after every adaptation, with the time set back to 0.
+extern "C" ndg_status ndg_pre_condition(ndg_mesh *mesh, double cfl, int64_t n_adaptivity_steps,
+                                        int64_t pre_condition_interval, int32_t iterative) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_pre_condition: null mesh");
+    if (pre_condition_interval <= 0)
+        return fail(NDG_ERR_INVALID, "ndg_pre_condition: the pre-condition interval must be positive");
+    if (n_adaptivity_steps < 0) return fail(NDG_ERR_INVALID, "ndg_pre_condition: negative adaptation count");
+    if (mesh->ic_kind < 0) return fail(NDG_ERR_INVALID, "ndg_pre_condition: no initial condition was set");
+    const std::vector<double> pulses = mesh->ic_pulses;  // ndg_set_ic below rewrites the mesh's copy
+    const ndg_ic_desc ic_keep{mesh->ic_kind, (int32_t)(pulses.size() / 4), pulses.empty() ? nullptr : pulses.data()};
+    double time = 0.0, dt = 0.0;
+    int64_t timestep = 0, n_adaptivity = 0;
+    ndg_status st;
+    while (n_adaptivity < n_adaptivity_steps) {
+        ++timestep;
+        if ((st = ndg_get_delta_t(mesh, cfl, &dt)) != NDG_OK) return st;
+        if ((st = ndg_step(mesh, time, dt)) != NDG_OK) return st;
+        time += dt;
+        if (timestep % pre_condition_interval == 0) {
+            int64_t changed = 0;
+            if ((st = ndg_adapt(mesh, mesh->tables->t.N_max, mesh->max_split_level, &changed)) != NDG_OK) return st;
+            ++n_adaptivity;
+            if (iterative) {
+                if ((st = ndg_set_ic(mesh, &ic_keep)) != NDG_OK) return st;
+                time = 0.0;
+            }
+        }
+    }
+    if (!iterative && (st = ndg_set_ic(mesh, &ic_keep)) != NDG_OK) return st;
+    HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_pre_condition sync");
+    return NDG_OK;
 }
 
 extern "C" ndg_status ndg_run_steps(ndg_mesh *mesh, double cfl, int64_t n_steps, double *time_io) {

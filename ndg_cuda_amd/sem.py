@@ -386,6 +386,20 @@ class Solver2D_t:
         mesh._refresh()
         return steps.value
 
+    def pre_condition(self, NDG: NDG_t, mesh: Mesh2D_t, n_adaptivity_steps: int, pre_condition_interval: int):
+        """Solver2D_t::pre_condition (include/solvers/Solver2D_t.tpp:138-236): steps with an adapt
+        every pre_condition_interval steps until n_adaptivity_steps adaptations are done, then the
+        mesh's initial condition again on the adapted mesh."""
+        check(lib().ndg_pre_condition(mesh.handle, self.CFL_, int(n_adaptivity_steps), int(pre_condition_interval), 0))
+        mesh._refresh()
+
+    def pre_condition_iterative(self, NDG: NDG_t, mesh: Mesh2D_t, n_adaptivity_steps: int,
+                                pre_condition_interval: int):
+        """Solver2D_t::pre_condition_iterative (include/solvers/Solver2D_t.tpp:238-336): the same,
+        with the initial condition applied again (and the time reset) after every adaptation."""
+        check(lib().ndg_pre_condition(mesh.handle, self.CFL_, int(n_adaptivity_steps), int(pre_condition_interval), 1))
+        mesh._refresh()
+
     def step(self, mesh: Mesh2D_t, t: float, dt: float):
         check(lib().ndg_step(mesh.handle, float(t), float(dt)))
 

@@ -189,6 +189,29 @@ class Mesh:
         self._refresh()
         return n, tf.value
 
+    def pre_condition(self, cfl, n_adaptivity_steps, interval, N_max, max_split_level, pulses, iterative=False):
+        """Solver2D_t::pre_condition (include/solvers/Solver2D_t.tpp:138-236) and
+        pre_condition_iterative (:238-336): from time 0, steps of get_delta_t with estimate_error +
+        adapt every `interval` steps until n_adaptivity_steps adaptations are done; the initial
+        condition (pulses, or the plane wave for None) is applied again at the end, or after every
+        adaptation with the time set back to 0 (iterative)."""
+        time, timestep, n = 0.0, 0, 0
+        while n < n_adaptivity_steps:
+            timestep += 1
+            dt = self.delta_t(cfl)
+            self.step(time, dt)
+            time += dt
+            if timestep % interval == 0:
+                self.estimate_error()
+                if self.adapt(N_max, max_split_level) < 0:
+                    raise RuntimeError("oracle pre_condition: adaptation refused")
+                n += 1
+                if iterative:
+                    self.initial_conditions(pulses)
+                    time = 0.0
+        if not iterative:
+            self.initial_conditions(pulses)
+
     def set_flags(self, refine, sigma):
         r = np.ascontiguousarray(refine, dtype=np.int32)
         sg = _f(sigma)
