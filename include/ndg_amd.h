@@ -72,6 +72,9 @@ typedef int ndg_status;
 /* Initial-condition kinds. */
 #define NDG_IC_PLANE_WAVE 0 /* the reference g(x, 0): include/functions/analytical_solution.tpp:5-11 */
 #define NDG_IC_PULSES 1     /* sum of radial Gaussian pulses, u = v = 0 (no reference counterpart) */
+#define NDG_IC_PERIODIC_PULSES 2 /* the same pulses periodised with periods (period_x, period_y): each
+                                    pulse plus its 8 nearest images, so the field is periodic to
+                                    within a exp(-ln2 min(period)^2 / w^2) (no reference counterpart) */
 
 typedef struct ndg_tables ndg_tables;
 typedef struct ndg_mesh ndg_mesh;
@@ -112,6 +115,7 @@ typedef struct ndg_ic_desc {
     int32_t kind;             /* NDG_IC_* */
     int32_t n_pulses;
     const double *pulses;     /* n_pulses x {x0, y0, width, amplitude}: p = a exp(-ln2 r^2 / w^2) */
+    double period_x, period_y; /* NDG_IC_PERIODIC_PULSES only */
 } ndg_ic_desc;
 
 /* ---- errors / device ---- */
@@ -274,6 +278,24 @@ ndg_status ndg_pre_condition(ndg_mesh *mesh, double cfl, int64_t n_adaptivity_st
 ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split_level, int64_t *n_raised);
 /* The order of every element (caller's order); it changes with ndg_adapt. */
 ndg_status ndg_mesh_orders(const ndg_mesh *mesh, int32_t *orders);
+/* Where ndg_adapt runs. NDG_ADAPT_DEVICE (the default) keeps the element-side topology in device
+ * memory and rebuilds it there with the decisions, the children, the links, the layout and the
+ * transfer, reading back only two small blocks of counts per adaptation, as the reference's device
+ * adaptation does (src/meshes/Mesh2D_t.cu:6206-7232, :7498, :8832-8902). NDG_ADAPT_HOST rebuilds
+ * the topology on the host from the node lists (the reference's host adapt,
+ * src/meshes/Mesh2D_t.cpp:1644-2268); it is refused once a device adaptation has run. Both give the
+ * same meshes and states bit for bit. */
+#define NDG_ADAPT_DEVICE 0
+#define NDG_ADAPT_HOST 1
+ndg_status ndg_mesh_set_adapt_mode(ndg_mesh *mesh, int32_t mode);
+/* The element-side topology (Element2D_t::faces_ and Face2D_t::elements_,
+ * include/entities/Element2D_t.h:12-127, include/entities/Face2D_t.h:18-83, as links between
+ * element sides), 4 values per element in the caller's order; any output may be NULL.
+ * link[4e+s]: the element side 4r+rs whose trace is the exterior state of side s of element e (a
+ * neighbour, or the donor side of a periodic pair), or -(1 + ghost). hang[4e+s]: 0 conforming, 1 a
+ * small side covering half of the larger side link, 2 a larger side whose half at its first corner
+ * is covered by link and whose other half by hang_second. periodic[4e+s]: 1 for a periodic pair. */
+ndg_status ndg_mesh_sides(ndg_mesh *mesh, int64_t *link, int8_t *hang, int64_t *hang_second, int8_t *periodic);
 /* Mesh2D_t's adaptivity_interval: ndg_solve then adapts every that many steps with the tables'
  * N_max and the description's max_split_level (include/solvers/Solver2D_t.tpp:88-96). 0 = off. */
 ndg_status ndg_mesh_set_adaptivity(ndg_mesh *mesh, int64_t adaptivity_interval);

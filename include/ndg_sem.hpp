@@ -276,12 +276,18 @@ class Mesh2D_t {
 
     // The reference plane wave g(x, 0).
     void initial_conditions() {
-        ndg_ic_desc ic{NDG_IC_PLANE_WAVE, 0, nullptr};
+        ndg_ic_desc ic{NDG_IC_PLANE_WAVE, 0, nullptr, 0.0, 0.0};
         check(ndg_set_ic(handle_, &ic));
     }
     // Radial Gaussian pulses {x0, y0, width, amplitude}; u = v = 0.
     void initial_conditions(const std::vector<std::array<double, 4>> &pulses) {
-        ndg_ic_desc ic{NDG_IC_PULSES, (int32_t)pulses.size(), pulses.empty() ? nullptr : pulses[0].data()};
+        ndg_ic_desc ic{NDG_IC_PULSES, (int32_t)pulses.size(), pulses.empty() ? nullptr : pulses[0].data(), 0.0, 0.0};
+        check(ndg_set_ic(handle_, &ic));
+    }
+    // The same pulses periodised with periods (Lx, Ly): each pulse and its 8 nearest images.
+    void initial_conditions(const std::vector<std::array<double, 4>> &pulses, double Lx, double Ly) {
+        ndg_ic_desc ic{NDG_IC_PERIODIC_PULSES, (int32_t)pulses.size(), pulses.empty() ? nullptr : pulses[0].data(),
+                       Lx, Ly};
         check(ndg_set_ic(handle_, &ic));
     }
     // Packed fields in reference node order.

@@ -17,7 +17,8 @@ NDG_ERR_UNSUPPORTED = 3
 NDG_ERR_NO_DEVICE = 4
 
 BC_NONE, BC_WALL, BC_SYMMETRY, BC_INFLOW, BC_OUTFLOW, BC_PERIODIC = range(6)
-IC_PLANE_WAVE, IC_PULSES = 0, 1
+IC_PLANE_WAVE, IC_PULSES, IC_PERIODIC_PULSES = 0, 1, 2
+ADAPT_DEVICE, ADAPT_HOST = 0, 1
 PRECISION_FP64, PRECISION_FP32_COMPENSATED = 0, 1
 GRAPH_OFF, GRAPH_SEQUENTIAL, GRAPH_PARALLEL = 0, 1, 2
 ADJOINT_C_RECOMPUTE, ADJOINT_C_STAGE_RECORDS = 0, 1
@@ -91,7 +92,8 @@ class SU2Periodic(ctypes.Structure):
 
 
 class ICDesc(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_int32), ("n_pulses", ctypes.c_int32), ("pulses", ctypes.c_void_p)]
+    _fields_ = [("kind", ctypes.c_int32), ("n_pulses", ctypes.c_int32), ("pulses", ctypes.c_void_p),
+                ("period_x", ctypes.c_double), ("period_y", ctypes.c_double)]
 
 
 _lib = None
@@ -150,6 +152,8 @@ def lib():
             "ndg_adapt": [vp, i32, i32, vp],
             "ndg_mesh_set_adaptivity": [vp, i64],
             "ndg_mesh_orders": [vp, vp],
+            "ndg_mesh_set_adapt_mode": [vp, i32],
+            "ndg_mesh_sides": [vp, vp, vp, vp, vp],
             "ndg_hilbert_child_order": [i32, i32, vp, vp],
             "ndg_hilbert_element_status": [i32, i32, vp],
             "ndg_record_receivers_noalloc": [vp, vp, f64, i64, vp],

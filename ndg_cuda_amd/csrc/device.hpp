@@ -58,4 +58,54 @@ __device__ __forceinline__ double recip(double x) {
     return r;
 }
 
+// Element2D_t::almost_equal with 2 ulp (src/meshes/Mesh2D_t.cpp:3785-3792).
+__device__ __forceinline__ bool almost_equal2(double a, double b) {
+    return fabs(a - b) <= 2.220446049250313e-16 * fabs(a + b) * 2 || fabs(a - b) < 2.2250738585072014e-308;
+}
+
+// Element2D_t::interpolate_from (src/entities/Element2D_t.cpp:345-579) at reference point (X, Y) of
+// the old element of order No, from one field sf[(No+1)^2] of it: the old element's node when
+// both coordinates match a node (almost_equal, the last matching index wins), a 1D barycentric
+// interpolation along the other coordinate when one matches, else the tensor barycentric
+// interpolation. xo, lo: nodes and barycentric weights of order No.
+__device__ inline double interpolate_at(const double *sf, int No, double X, double Y, const double *xo,
+                                        const double *lo) {
+    const int nfo = No + 1;
+    int col = -1, row = -1;
+    for (int m = 0; m <= No; ++m) {
+        if (almost_equal2(X, xo[m])) col = m;
+        if (almost_equal2(Y, xo[m])) row = m;
+    }
+    if (row >= 0 && col >= 0) return sf[col * nfo + row];
+    if (row >= 0) {
+        double num = 0.0, den = 0.0;
+        for (int m = 0; m <= No; ++m) {
+            const double t = lo[m] / (X - xo[m]);
+            num += t * sf[m * nfo + row];
+            den += t;
+        }
+        return num / den;
+    }
+    if (col >= 0) {
+        double num = 0.0, den = 0.0;
+        for (int n = 0; n <= No; ++n) {
+            const double t = lo[n] / (Y - xo[n]);
+            num += t * sf[col * nfo + n];
+            den += t;
+        }
+        return num / den;
+    }
+    double dx = 0.0, dy = 0.0;
+    for (int k = 0; k <= No; ++k) {
+        dx += lo[k] / (X - xo[k]);
+        dy += lo[k] / (Y - xo[k]);
+    }
+    double v = 0.0;
+    for (int m = 0; m <= No; ++m) {
+        const double tx = lo[m] / ((X - xo[m]) * dx);
+        for (int n = 0; n <= No; ++n) v += sf[m * nfo + n] * tx * (lo[n] / ((Y - xo[n]) * dy));
+    }
+    return v;
+}
+
 }  // namespace ndg

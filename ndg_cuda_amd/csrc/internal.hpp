@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -52,6 +53,27 @@ struct MortarFace {
     int64_t mat[4];
 };
 
+// Mortar side classes. A non-conforming face covers the whole element side (offset 0, scale 1),
+// its first half (offset 0, scale 1/2) or its second half (offset 1/2, scale 1/2): the only shapes
+// the conforming and 2:1 hanging faces of this mesh model produce. The offsets and scales computed
+// from the vertices (Face2D_t.cpp:68-81) can miss 1/2 by an ulp; the class snaps them, so one
+// interpolation/projection matrix per (element order, face order, side, class) serves every face.
+// Returns -1 for a side that is none of these.
+__host__ __device__ inline int mortar_class(double offset, double scale) {
+    const double tol = 1e-9;
+    if (fabs(scale - 1.0) <= tol && fabs(offset) <= tol) return 0;
+    if (fabs(scale - 0.5) <= tol) {
+        if (fabs(offset) <= tol) return 1;
+        if (fabs(offset - 0.5) <= tol) return 2;
+    }
+    return -1;
+}
+// Index of a matrix in the table of mortar matrices (which = 0 interpolation, 1 projection).
+__host__ __device__ inline int64_t mortar_matrix_key(int Ne, int Nf, int side, int cls, int which) {
+    return ((((int64_t)Ne * (kMaxOrder + 1) + Nf) * 2 + side) * 3 + cls) * 2 + which;
+}
+constexpr int64_t kMortarMatrixKeys = (int64_t)(kMaxOrder + 1) * (kMaxOrder + 1) * 12;
+
 // Per-order constant tables in device memory, packed as Dhat[Nf*Nf], l(-1)[Nf], l(+1)[Nf], w[Nf],
 // x[Nf], 1/w[Nf], then the modal matrix poly[Nf*Nf] used by the estimator and the transpose
 // Dhat^T[Nf*Nf] used by the adjoint (its columns as contiguous rows).
@@ -81,6 +103,38 @@ struct Tables {
 void build_tables(int N_max, int64_t n_interp, Tables &t);
 // Packs the device table block for order N (layout above).
 std::vector<double> pack_tables(const Tables &t, int N);
+
+// A device allocation that keeps its capacity: arena_fit reallocates only when more bytes are
+// asked for than it holds (with 25% headroom), so meshes rebuilt by adaptation reuse their
+// buffers. Contents are not kept across a reallocation.
+struct Arena {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+hipError_t arena_fit(Arena &a, size_t bytes);
+void arena_free(Arena &a);
+
+// The buffers of one device mesh. A mesh keeps two: the current one, and the one an adaptation
+// builds the next mesh into (the state is transferred from the first to the second).
+struct DeviceStore {
+    Arena verts, side_nb, bc, faces, side_flux, side_flux32, q, g, tr[2], rhs, scratch, qh, gl32, tr32[2];
+    Arena spans;  // GroupSpan[kMaxOrder + 1] of the layout
+};
+void store_free(DeviceStore &s);
+
+// Hilbert child tables (src/functions/Hilbert_splitting.cpp:23-81), statuses H, A, R, B = 0..3:
+// the place of child s (the child at corner s) among the 4 children, and its status, by parent
+// rotation and status. One initializer for the host and the device copies.
+#define NDG_HILBERT_CHILD_ORDER                                                    \
+    {{{0, 3, 2, 1}, {0, 1, 2, 3}, {2, 1, 0, 3}, {2, 3, 0, 1}},                     \
+     {{3, 2, 1, 0}, {1, 2, 3, 0}, {1, 0, 3, 2}, {3, 0, 1, 2}},                     \
+     {{2, 1, 0, 3}, {2, 3, 0, 1}, {0, 3, 2, 1}, {0, 1, 2, 3}},                     \
+     {{1, 0, 3, 2}, {3, 0, 1, 2}, {3, 2, 1, 0}, {1, 2, 3, 0}}}
+#define NDG_HILBERT_CHILD_STATUS                                                   \
+    {{{1, 3, 0, 0}, {0, 1, 1, 2}, {2, 2, 3, 1}, {3, 0, 2, 3}},                     \
+     {{3, 0, 0, 1}, {1, 1, 2, 0}, {2, 3, 1, 2}, {0, 2, 3, 3}},                     \
+     {{0, 0, 1, 3}, {1, 2, 0, 1}, {3, 1, 2, 2}, {2, 3, 3, 0}},                     \
+     {{0, 1, 3, 0}, {2, 0, 1, 1}, {1, 2, 2, 3}, {3, 3, 0, 2}}}
 
 // Elements of one order, contiguous in device order (Hilbert order kept within the group).
 struct Group {
@@ -142,6 +196,78 @@ struct DeviceMesh {
     float *trace32[2] = {nullptr, nullptr};
 };
 
+// Position of the order-N group in the device layout (one per order 0..kMaxOrder; K = 0 when the
+// mesh has no element of that order), in device memory for the adaptation kernels.
+struct GroupSpan {
+    int64_t K, e_begin, q_off, tr_off;
+};
+
+// The mesh's element-side topology in the caller's element order (the curve order), kept in device
+// memory so that an adaptation can rebuild the mesh there (adapt.hip). link[4e+s] is the element
+// side whose trace is the exterior state of side s of element e (a neighbour, or the donor side of
+// a periodic pair), or -(1 + ghost); hang: 0 conforming, 1 a small side whose exterior is half of
+// a larger element's side (link = that side), 2 the larger side (link = its small side on the half
+// that starts at the side's first corner, hang2 = the other one).
+struct LogicalMesh {
+    int64_t K = 0, n_ghost = 0;
+    double *verts = nullptr;                      // [K][4][2]
+    int32_t *order = nullptr, *status = nullptr, *rotation = nullptr, *level = nullptr;
+    int64_t *link = nullptr, *hang2 = nullptr;    // [4K]
+    int8_t *hang = nullptr, *periodic = nullptr;  // [4K]
+    double *est = nullptr;                        // [K][4] last estimate: error, sigma, refine, coarsen
+    int64_t *dev_of = nullptr;                    // [K] index in the order-grouped device layout
+    BcSide *ghost = nullptr;                      // [n_ghost] ghost records, in boundary-bar order
+    int64_t *ghost_side = nullptr;                // [n_ghost] element side of each ghost
+    int8_t *ghost_zero = nullptr;                 // [n_ghost] storage resized to zero (the scale is 0)
+};
+struct LogicalStore {
+    Arena verts, order, status, rotation, level, link, hang2, hang, periodic, est, dev_of, ghost, ghost_side,
+        ghost_zero;
+};
+
+// Counters one adaptation gathers on the device and the host reads back (two round trips).
+struct AdaptCounts {
+    unsigned long long n_split, n_raise, refused, ghost_split;
+    unsigned long long dofs, nodes1d;            // sum (N+1)^2 and sum (N+1) of the new mesh
+    unsigned long long order_count[kMaxOrder + 1];
+    unsigned long long n_rec, n_slot, n_face;    // mortar records, side-flux values, mortar faces
+    unsigned long long conforming_pairs, larger_sides, mortar_nodes, bad_face;
+};
+// refused bits
+constexpr unsigned kRefuseOrder = 1, kRefusePeriodic = 2, kRefuseFourFaces = 4;
+
+// Scratch of the adaptation kernels (capacities kept across adaptations).
+struct AdaptWork {
+    Arena flags, newpos, newghost, parent, corner, cnt_rec, cnt_slot, cnt_face, blocks, hist, counts;
+};
+
+// Device adaptation steps (adapt.hip). Each launches on stream s and returns the launch error.
+// 1. Decisions from the estimate est (device order): flags[e] = 0 keep, 1 raise by 2, 2 split
+//    (Element2D_t.cpp:802-812 with the split level bound), the logical copy of the estimate, and
+//    the counts (splits, raises, refusals, the new mesh's sizes).
+hipError_t adapt_decide(const LogicalMesh &L, const double *est, int N_max, int tables_N_max, int max_split_level,
+                        AdaptWork &w, hipStream_t s);
+// 2. The new logical mesh (children in Hilbert order, raised orders, links, ghosts), into Ln
+//    (buffers sized for the counts of step 1). p_only: the no-split branch, whose raised elements'
+//    ghosts keep a zero scale.
+hipError_t adapt_rebuild(const LogicalMesh &L, LogicalMesh &Ln, bool p_only, AdaptWork &w, hipStream_t s);
+// 3. The order-grouped layout of Ln: dev_of, the group spans (spans[kMaxOrder + 1]) and the
+//    counts of mortar records, slots and faces.
+hipError_t adapt_layout_count(LogicalMesh &Ln, GroupSpan *spans, AdaptWork &w, hipStream_t s);
+// 4. The device mesh arrays of Ln (vertices, neighbour codes, boundary and mortar records, mortar
+//    faces) into d (buffers sized for the counts of step 3); mortar_key_off maps a matrix key to
+//    its offset in the mortar matrix table.
+hipError_t adapt_layout_fill(const LogicalMesh &Ln, const GroupSpan *spans, const int64_t *mortar_key_off,
+                             DeviceMesh &d, AdaptWork &w, hipStream_t s);
+// 5. The state of every new element from its parent (copy, new order, or a child's nodes through
+//    the inverse bilinear map), from the layout (L, spans_old, q_old) to (Ln, spans_new, q_new).
+hipError_t adapt_transfer(const LogicalMesh &L, const GroupSpan *spans_old, const double *q_old,
+                          const LogicalMesh &Ln, const GroupSpan *spans_new, double *q_new, const double *tab1d,
+                          AdaptWork &w, hipStream_t s);
+
+// The logical copy of an estimate est (device order), L.est (caller's order).
+hipError_t adapt_copy_estimate(const LogicalMesh &L, const double *est, hipStream_t s);
+
 // Kernel launchers (kernels.hip). All return hipSuccess or the first launch error.
 hipError_t launch_traces(const DeviceMesh &m, hipStream_t s);
 // Stage s of the RK step (mortar faces, then every order group, in the mesh's precision); the
@@ -165,7 +291,8 @@ hipError_t launch_clock_end(DeviceMesh &m, double dt, hipStream_t s);
 // of group g's stage kernel that leaves the state and the current traces unchanged.
 int stage_max_epb(int N);
 hipError_t launch_group_probe(DeviceMesh &m, int group, hipStream_t s);
-hipError_t launch_ic(const DeviceMesh &m, int kind, int n_pulses, const double *pulses_dev,
+// Initial condition of kind NDG_IC_*; Lx, Ly are the periods of NDG_IC_PERIODIC_PULSES.
+hipError_t launch_ic(const DeviceMesh &m, int kind, int n_pulses, const double *pulses_dev, double Lx, double Ly,
                      hipStream_t s);
 hipError_t launch_delta_t(const DeviceMesh &m, double cfl, double *dt_dev, hipStream_t s);
 // Mortar faces: interpolation to the face, upwind flux, projection to the element sides.

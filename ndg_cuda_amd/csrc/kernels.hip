@@ -809,7 +809,7 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
 template <int N>
 __global__ void ic_kernel(const double *__restrict__ verts, const double *__restrict__ tab,
                           double *__restrict__ q, int64_t K, int kind, int n_pulses,
-                          const double *__restrict__ pulses) {
+                          const double *__restrict__ pulses, double Lx, double Ly) {
     constexpr int NF = N + 1, NE = NF * NF;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= K * NE) return;
@@ -825,6 +825,16 @@ __global__ void ic_kernel(const double *__restrict__ verts, const double *__rest
     double p = 0.0, u = 0.0, v = 0.0;
     if (kind == NDG_IC_PLANE_WAVE) {
         plane_wave(X, Y, 0.0, 1.0, p, u, v);
+    } else if (kind == NDG_IC_PERIODIC_PULSES) {
+        // the 3 x 3 periodic images of each pulse, (mx, my) in {-1, 0, 1}^2
+        for (int k = 0; k < n_pulses; ++k) {
+            const double w = pulses[4 * k + 2];
+            for (int mx = -1; mx <= 1; ++mx)
+                for (int my = -1; my <= 1; ++my) {
+                    const double dx = X - (pulses[4 * k] + mx * Lx), dy = Y - (pulses[4 * k + 1] + my * Ly);
+                    p += pulses[4 * k + 3] * exp(-M_LN2 * (dx * dx + dy * dy) / (w * w));
+                }
+        }
     } else {
         for (int k = 0; k < n_pulses; ++k) {
             const double dx = X - pulses[4 * k], dy = Y - pulses[4 * k + 1], w = pulses[4 * k + 2];
@@ -1105,44 +1115,7 @@ transfer_kernel(const double *__restrict__ q_old, double *__restrict__ q_new,
         const int f = r / ne, node = r % ne, i = node / nf, j = node % nf;
         const double X = R.coord_off >= 0 ? coords[R.coord_off + 2 * node] : xn[i];
         const double Y = R.coord_off >= 0 ? coords[R.coord_off + 2 * node + 1] : xn[j];
-        int col = -1, row = -1;
-        for (int m = 0; m <= No; ++m) {
-            if (ulp2_equal(X, xo[m])) col = m;
-            if (ulp2_equal(Y, xo[m])) row = m;
-        }
-        const double *sf = src + f * neo;
-        double v;
-        if (row >= 0 && col >= 0) {
-            v = sf[col * nfo + row];
-        } else if (row >= 0) {
-            double num = 0.0, den = 0.0;
-            for (int m = 0; m <= No; ++m) {
-                const double t = lo[m] / (X - xo[m]);
-                num += t * sf[m * nfo + row];
-                den += t;
-            }
-            v = num / den;
-        } else if (col >= 0) {
-            double num = 0.0, den = 0.0;
-            for (int n = 0; n <= No; ++n) {
-                const double t = lo[n] / (Y - xo[n]);
-                num += t * sf[col * nfo + n];
-                den += t;
-            }
-            v = num / den;
-        } else {
-            double dx = 0.0, dy = 0.0;
-            for (int k = 0; k <= No; ++k) {
-                dx += lo[k] / (X - xo[k]);
-                dy += lo[k] / (Y - xo[k]);
-            }
-            v = 0.0;
-            for (int m = 0; m <= No; ++m) {
-                const double tx = lo[m] / ((X - xo[m]) * dx);
-                for (int n = 0; n <= No; ++n) v += sf[m * nfo + n] * tx * (lo[n] / ((Y - xo[n]) * dy));
-            }
-        }
-        dst[r] = v;
+        dst[r] = interpolate_at(src + f * neo, No, X, Y, xo, lo);
     }
 }
 
@@ -1292,10 +1265,10 @@ static hipError_t rhs_n(DeviceMesh &m, const Group &G, hipStream_t s) {
 
 template <int N>
 static hipError_t ic_n(const DeviceMesh &m, const Group &G, int kind, int n_pulses, const double *pulses,
-                       hipStream_t s) {
+                       double Lx, double Ly, hipStream_t s) {
     const int64_t total = G.K * (N + 1) * (N + 1);
     hipLaunchKernelGGL(ic_kernel<N>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, m.verts + G.e_begin * 8,
-                       G.tab, m.q + G.q_off, G.K, kind, n_pulses, pulses);
+                       G.tab, m.q + G.q_off, G.K, kind, n_pulses, pulses, Lx, Ly);
     return hipGetLastError();
 }
 
@@ -1411,9 +1384,10 @@ hipError_t launch_rhs(DeviceMesh &m, double t, hipStream_t s) {
     });
 }
 
-hipError_t launch_ic(const DeviceMesh &m, int kind, int n_pulses, const double *pulses, hipStream_t s) {
+hipError_t launch_ic(const DeviceMesh &m, int kind, int n_pulses, const double *pulses, double Lx, double Ly,
+                     hipStream_t s) {
     return for_groups(m, [&](const Group &G) -> hipError_t {
-        switch (G.N) { CASES_1(ic_n, m, G, kind, n_pulses, pulses, s) }
+        switch (G.N) { CASES_1(ic_n, m, G, kind, n_pulses, pulses, Lx, Ly, s) }
         return hipErrorInvalidValue;
     });
 }

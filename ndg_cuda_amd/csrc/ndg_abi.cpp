@@ -45,6 +45,31 @@ constexpr double kGm[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 
 ndg_status ndg::set_error(ndg_status code, const char *msg) { return fail(code, msg); }
 
+hipError_t ndg::arena_fit(Arena &a, size_t bytes) {
+    if (bytes <= a.cap && a.p) return hipSuccess;
+    arena_free(a);
+    const size_t cap = std::max<size_t>(bytes + bytes / 4, 256);
+    hipError_t e = hipMalloc(&a.p, cap);
+    if (e != hipSuccess) {
+        a.p = nullptr;
+        return e;
+    }
+    a.cap = cap;
+    return hipSuccess;
+}
+
+void ndg::arena_free(Arena &a) {
+    if (a.p) (void)hipFree(a.p);
+    a.p = nullptr;
+    a.cap = 0;
+}
+
+void ndg::store_free(DeviceStore &s) {
+    for (Arena *a : {&s.verts, &s.side_nb, &s.bc, &s.faces, &s.side_flux, &s.side_flux32, &s.q, &s.g, &s.tr[0],
+                     &s.tr[1], &s.rhs, &s.scratch, &s.qh, &s.gl32, &s.tr32[0], &s.tr32[1]})
+        arena_free(*a);
+}
+
 struct ndg_tables {
     Tables t;
 };
@@ -70,7 +95,18 @@ struct Topology {
 };
 
 struct ndg_mesh {
-    DeviceMesh d;
+    DeviceMesh d;                            // views into store[active]
+    DeviceStore store[2];
+    int active = 0;
+    // Device tables shared by every rebuild: packed per-order tables (FP64 and FP32), the 1D
+    // tables of all orders, the mortar matrices by key (offsets mortar_key_off, -1 = not built)
+    // and the device clock.
+    double *tab64[kMaxOrder + 1] = {};
+    float *tab32[kMaxOrder + 1] = {};
+    double *tab1d = nullptr;
+    double *mortar_mats = nullptr;
+    std::vector<int64_t> mortar_key_off;
+    double *clock = nullptr;
     // The mesh's own copy of the caller's tables (the reference's Mesh2D_t copies the polynomial
     // tables it needs), so the caller's ndg_tables may be destroyed before the mesh.
     ndg_tables tables_own;
@@ -85,6 +121,7 @@ struct ndg_mesh {
     double *pulses_dev = nullptr;
     int ic_kind = -1;                        // last initial condition given (re-applied by pre_condition)
     std::vector<double> ic_pulses;
+    double ic_period[2] = {0.0, 0.0};
     std::vector<double> staging;
     // Element e of the caller's order is element dev_of[e] of the device order (grouped by N).
     std::vector<int64_t> dev_of;
@@ -120,6 +157,21 @@ struct ndg_mesh {
     std::vector<int32_t> est_refine, est_coarsen;
     int initial_N = 0, maximum_N = 0, max_split_level = 0;
     int64_t adapt_interval = 0;
+    // Device-resident adaptation (adapt.hip): the logical mesh in two stores (the current one and
+    // the one an adaptation builds), its scratch, and which copy is current. After a device
+    // adaptation the host copies above (orders, statuses, links, ghosts, estimates, dev_of) are
+    // refreshed from the device on demand (ensure_host); the node lists and bar flags are not kept
+    // by the device path (host_nodes_valid).
+    LogicalStore lstore[2];
+    LogicalMesh lmesh[2];
+    int lactive = 0;
+    bool logical_valid = false;
+    bool host_stale = false;
+    bool host_nodes_valid = true;
+    int adapt_mode = NDG_ADAPT_DEVICE;
+    int64_t n_periodic_sides = 0;
+    AdaptWork work;
+    int64_t *mortar_key_dev = nullptr;
     int64_t adjoint_store_budget = 0;        // bytes of stage states kept beyond one segment
     int adjoint_c_mode = NDG_ADJOINT_C_RECOMPUTE;
 };
@@ -234,79 +286,72 @@ bool ulp2_equal(double a, double b) {
 // interpolation at xi = (x_face + 1) scale + 2 offset - 1 with the exact-node shortcut
 // (src/meshes/Mesh2D_t.cpp:3820-3890), and project_to_elements' Gauss-quadrature L2 projection
 // T_i w_j / w_i with the same shortcut, negated for the face's right side (:3935-4003). The
-// right side walks the face nodes backwards. Matrices are shared between faces with equal keys.
-class MortarMatrices {
-  public:
-    explicit MortarMatrices(const Tables &t) : t_(t) {}
-    int64_t interp(int Ne, int Nf, int side, double offset, double scale) {
-        return get(Ne, Nf, side, offset, scale, 0);
-    }
-    int64_t proj(int Ne, int Nf, int side, double offset, double scale) {
-        return get(Ne, Nf, side, offset, scale, 1);
-    }
-    const std::vector<double> &data() const { return data_; }
-
-  private:
-    int64_t get(int Ne, int Nf, int side, double offset, double scale, int which) {
-        const auto key = std::make_tuple(Ne, Nf, side, offset, scale, which);
-        auto it = index_.find(key);
-        if (it != index_.end()) return it->second;
-        const int64_t at = (int64_t)data_.size();
-        const auto &xf = t_.tab[NDG_TABLE_NODES][Nf], &wf = t_.tab[NDG_TABLE_WEIGHTS][Nf];
-        const auto &xe = t_.tab[NDG_TABLE_NODES][Ne], &we = t_.tab[NDG_TABLE_WEIGHTS][Ne];
-        const auto &le = t_.tab[NDG_TABLE_BARYCENTRIC_WEIGHTS][Ne];
-        if (which == 0) {
-            data_.resize(data_.size() + (size_t)(Nf + 1) * (Ne + 1), 0.0);
-            for (int i = 0; i <= Nf; ++i) {
-                double *row = data_.data() + at + (int64_t)i * (Ne + 1);
-                const double coord = (xf[side ? Nf - i : i] + 1) * scale + 2 * offset - 1;
-                int hit = -1;
-                for (int j = 0; j <= Ne; ++j)
-                    if (ulp2_equal(coord, xe[j])) {
-                        hit = j;
-                        break;
+// right side walks the face nodes backwards. One matrix per key (element order, face order, side,
+// class of mortar_class, interpolation or projection), for every order up to N_max: data holds
+// them back to back, off[key] their offsets (-1 for the plain copies: equal orders, whole side).
+void build_mortar_table(const Tables &t, int N_max, std::vector<double> &data, std::vector<int64_t> &off) {
+    const double cls_offset[3] = {0.0, 0.0, 0.5}, cls_scale[3] = {1.0, 0.5, 0.5};
+    data.clear();
+    off.assign((size_t)kMortarMatrixKeys, -1);
+    for (int Ne = 1; Ne <= N_max; ++Ne)
+        for (int Nf = Ne; Nf <= N_max; ++Nf)
+            for (int side = 0; side < 2; ++side)
+                for (int cls = 0; cls < 3; ++cls) {
+                    if (Ne == Nf && cls == 0) continue;
+                    const double offset = cls_offset[cls], scale = cls_scale[cls];
+                    const auto &xf = t.tab[NDG_TABLE_NODES][Nf], &wf = t.tab[NDG_TABLE_WEIGHTS][Nf];
+                    const auto &xe = t.tab[NDG_TABLE_NODES][Ne], &we = t.tab[NDG_TABLE_WEIGHTS][Ne];
+                    const auto &le = t.tab[NDG_TABLE_BARYCENTRIC_WEIGHTS][Ne];
+                    // interpolation [Nf+1][Ne+1]
+                    int64_t at = (int64_t)data.size();
+                    off[(size_t)mortar_matrix_key(Ne, Nf, side, cls, 0)] = at;
+                    data.resize(data.size() + (size_t)(Nf + 1) * (Ne + 1), 0.0);
+                    for (int i = 0; i <= Nf; ++i) {
+                        double *row = data.data() + at + (int64_t)i * (Ne + 1);
+                        const double coord = (xf[side ? Nf - i : i] + 1) * scale + 2 * offset - 1;
+                        int hit = -1;
+                        for (int j = 0; j <= Ne; ++j)
+                            if (ulp2_equal(coord, xe[j])) {
+                                hit = j;
+                                break;
+                            }
+                        if (hit >= 0) {
+                            row[hit] = 1.0;
+                            continue;
+                        }
+                        double den = 0.0;
+                        for (int j = 0; j <= Ne; ++j) {
+                            row[j] = le[j] / (coord - xe[j]);
+                            den += row[j];
+                        }
+                        for (int j = 0; j <= Ne; ++j) row[j] /= den;
                     }
-                if (hit >= 0) {
-                    row[hit] = 1.0;
-                    continue;
-                }
-                double den = 0.0;
-                for (int j = 0; j <= Ne; ++j) {
-                    row[j] = le[j] / (coord - xe[j]);
-                    den += row[j];
-                }
-                for (int j = 0; j <= Ne; ++j) row[j] /= den;
-            }
-        } else {
-            data_.resize(data_.size() + (size_t)(Ne + 1) * (Nf + 1), 0.0);
-            double *P = data_.data() + at;
-            for (int j = 0; j <= Nf; ++j) {
-                const double coord = (xf[side ? Nf - j : j] + 1) * scale + 2 * offset - 1;
-                int hit = -1;
-                for (int k = 0; k <= Ne; ++k)
-                    if (ulp2_equal(coord, xe[k])) {
-                        hit = k;
-                        break;
+                    // projection [Ne+1][Nf+1]
+                    at = (int64_t)data.size();
+                    off[(size_t)mortar_matrix_key(Ne, Nf, side, cls, 1)] = at;
+                    data.resize(data.size() + (size_t)(Ne + 1) * (Nf + 1), 0.0);
+                    double *P = data.data() + at;
+                    for (int j = 0; j <= Nf; ++j) {
+                        const double coord = (xf[side ? Nf - j : j] + 1) * scale + 2 * offset - 1;
+                        int hit = -1;
+                        for (int k = 0; k <= Ne; ++k)
+                            if (ulp2_equal(coord, xe[k])) {
+                                hit = k;
+                                break;
+                            }
+                        if (hit >= 0) {
+                            P[(int64_t)hit * (Nf + 1) + j] += (side ? -wf[j] : wf[j]) / we[hit] * scale;
+                            continue;
+                        }
+                        double ssum = 0.0;
+                        for (int k = 0; k <= Ne; ++k) ssum += le[k] / (coord - xe[k]);
+                        for (int i = 0; i <= Ne; ++i) {
+                            const double T = le[i] / ((coord - xe[i]) * ssum);
+                            P[(int64_t)i * (Nf + 1) + j] += (side ? -T : T) * wf[j] / we[i] * scale;
+                        }
                     }
-                if (hit >= 0) {
-                    P[(int64_t)hit * (Nf + 1) + j] += (side ? -wf[j] : wf[j]) / we[hit] * scale;
-                    continue;
                 }
-                double ssum = 0.0;
-                for (int k = 0; k <= Ne; ++k) ssum += le[k] / (coord - xe[k]);
-                for (int i = 0; i <= Ne; ++i) {
-                    const double T = le[i] / ((coord - xe[i]) * ssum);
-                    P[(int64_t)i * (Nf + 1) + j] += (side ? -T : T) * wf[j] / we[i] * scale;
-                }
-            }
-        }
-        index_[key] = at;
-        return at;
-    }
-    const Tables &t_;
-    std::vector<double> data_;
-    std::map<std::tuple<int, int, int, double, double, int>, int64_t> index_;
-};
+}
 
 template <typename T>
 ndg_status device_upload(T **dst, const std::vector<T> &src, const char *what) {
@@ -316,26 +361,46 @@ ndg_status device_upload(T **dst, const std::vector<T> &src, const char *what) {
     return NDG_OK;
 }
 
-void free_device(DeviceMesh &d) {
-    std::vector<void *> ptrs = {(void *)d.verts, (void *)d.side_nb, (void *)d.bc, (void *)d.faces,
-                                (void *)d.side_flux, (void *)d.side_flux32, (void *)d.tab1d,
-                                (void *)d.mortar_mats, (void *)d.q,
-                                (void *)d.g, (void *)d.trace[0], (void *)d.trace[1], (void *)d.rhs,
-                                (void *)d.scratch, (void *)d.qh, (void *)d.gl32,
-                                (void *)d.trace32[0], (void *)d.trace32[1], (void *)d.clock};
-    for (const Group &G : d.groups) {
-        ptrs.push_back(G.tab);
-        ptrs.push_back(G.tab32);
-    }
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
-    d = DeviceMesh();
+// Copies src into the arena (fitted to its size) and points *view at it (null when empty).
+template <typename T>
+ndg_status upload_to(Arena &a, T **view, const std::vector<T> &src, const char *what) {
+    *view = nullptr;
+    if (src.empty()) return NDG_OK;
+    HIP_TRY(arena_fit(a, src.size() * sizeof(T)), what);
+    HIP_TRY(hipMemcpy(a.p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice), what);
+    *view = (T *)a.p;
+    return NDG_OK;
+}
+
+// Fits the arena to n values of T and points *view at it.
+template <typename T>
+ndg_status fit_view(Arena &a, T **view, int64_t n, const char *what) {
+    HIP_TRY(arena_fit(a, sizeof(T) * (size_t)std::max<int64_t>(n, 1)), what);
+    *view = (T *)a.p;
+    return NDG_OK;
+}
+
+void logical_free(LogicalStore &s) {
+    for (Arena *a : {&s.verts, &s.order, &s.status, &s.rotation, &s.level, &s.link, &s.hang2, &s.hang, &s.periodic,
+                     &s.est, &s.dev_of, &s.ghost, &s.ghost_side, &s.ghost_zero})
+        arena_free(*a);
 }
 
 void free_mesh(ndg_mesh *m) {
     if (!m) return;
-    free_device(m->d);
-    for (void *p : {(void *)m->dt_dev, (void *)m->est_dev, (void *)m->pulses_dev})
+    for (DeviceStore &st : m->store) store_free(st);
+    for (LogicalStore &ls : m->lstore) logical_free(ls);
+    for (Arena *a : {&m->work.flags, &m->work.newpos, &m->work.newghost, &m->work.parent, &m->work.corner,
+                     &m->work.cnt_rec, &m->work.cnt_slot, &m->work.cnt_face, &m->work.blocks, &m->work.hist,
+                     &m->work.counts})
+        arena_free(*a);
+    if (m->mortar_key_dev) (void)hipFree(m->mortar_key_dev);
+    for (int n = 0; n <= kMaxOrder; ++n) {
+        if (m->tab64[n]) (void)hipFree(m->tab64[n]);
+        if (m->tab32[n]) (void)hipFree(m->tab32[n]);
+    }
+    for (void *p : {(void *)m->dt_dev, (void *)m->est_dev, (void *)m->pulses_dev, (void *)m->tab1d,
+                    (void *)m->mortar_mats, (void *)m->clock})
         if (p) (void)hipFree(p);
     for (hipGraphExec_t &x : m->step_exec)
         if (x) (void)hipGraphExecDestroy(x);
@@ -363,12 +428,43 @@ bool ghost_copy_is_close(const Tables &t, int N) {
 
 namespace {
 
-// Builds the device mesh for the given element orders from the host topology kept in m: order
-// groups, connectivity (group-local neighbour codes, boundary records with the ghost scale zeroed
-// where an adaptation resized the ghost, mortar records and faces between different orders),
-// vertices, tables, and the state / trace buffers (uninitialised). Sets m->order, m->dev_of,
-// m->host_off, m->n_dofs. The clock is not allocated here.
-ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMesh &d) {
+// The device tables every rebuild shares, uploaded once per mesh: the 1D tables of all orders, the
+// mortar matrices of every key up to the tables' N_max, and the packed tables of order N (FP64).
+ndg_status ensure_shared_tables(ndg_mesh *m) {
+    const Tables &tab = m->tables->t;
+    if (!m->tab1d) {
+        std::vector<double> t1((size_t)(3 * (kMaxOrder + 1) * kTab1dStride), 0.0);
+        for (int n = 0; n <= std::min(kMaxOrder, tab.N_max); ++n)
+            for (int i = 0; i <= n; ++i) {
+                t1[(3 * n + 0) * kTab1dStride + i] = tab.tab[NDG_TABLE_NODES][n][i];
+                t1[(3 * n + 1) * kTab1dStride + i] = tab.tab[NDG_TABLE_WEIGHTS][n][i];
+                t1[(3 * n + 2) * kTab1dStride + i] = tab.tab[NDG_TABLE_BARYCENTRIC_WEIGHTS][n][i];
+            }
+        ndg_status st = device_upload(&m->tab1d, t1, "upload 1D tables");
+        if (st != NDG_OK) return st;
+    }
+    if (!m->mortar_mats) {
+        std::vector<double> mats;
+        build_mortar_table(tab, std::min(kMaxOrder, tab.N_max), mats, m->mortar_key_off);
+        if (mats.empty()) mats.push_back(0.0);
+        ndg_status st = device_upload(&m->mortar_mats, mats, "upload mortar matrices");
+        if (st != NDG_OK) return st;
+        if ((st = device_upload(&m->mortar_key_dev, m->mortar_key_off, "upload mortar matrix keys")) != NDG_OK) return st;
+    }
+    return NDG_OK;
+}
+
+ndg_status ensure_order_tables(ndg_mesh *m, int N) {
+    if (m->tab64[N]) return NDG_OK;
+    return device_upload(&m->tab64[N], pack_tables(m->tables->t, N), "upload tables");
+}
+
+// Builds the device mesh for the given element orders from the host topology kept in m, into the
+// buffers of store: order groups, connectivity (group-local neighbour codes, boundary records
+// with the ghost scale zeroed where an adaptation resized the ghost, mortar records and faces
+// between different orders), vertices, and the state / trace buffers (uninitialised). Sets
+// m->order, m->dev_of, m->host_off, m->n_dofs.
+ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMesh &d, DeviceStore &store) {
     const Tables &tab = m->tables->t;
     const int64_t K = (int64_t)order.size();
     const std::vector<int64_t> &link = m->link;
@@ -420,6 +516,13 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
     }
     d.N_min = d.groups.front().N;
     d.N_max = d.groups.back().N;
+    {
+        std::vector<GroupSpan> spans((size_t)(kMaxOrder + 1), GroupSpan{0, 0, 0, 0});
+        for (const Group &G : d.groups) spans[(size_t)G.N] = GroupSpan{G.K, G.e_begin, G.q_off, G.tr_off};
+        GroupSpan *view = nullptr;
+        const ndg_status sst = upload_to(store.spans, &view, spans, "upload group spans");
+        if (sst != NDG_OK) return sst;
+    }
     for (const Group &G : d.groups)
         if (!ghost_copy_is_close(tab, G.N))
             return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: boundary ghost projection is not a reversed copy for "
@@ -508,13 +611,17 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
             }
             faces.push_back(fc);
         }
-    MortarMatrices mats(tab);
+    ndg_status st;
+    if ((st = ensure_shared_tables(m)) != NDG_OK) return st;
     for (MortarFace &fc : faces)
         for (int side = 0; side < 2; ++side) {
             const int Ne = side ? fc.NR : fc.NL;
-            const bool copy = Ne == fc.Nf && fc.offset[side] == 0.0 && fc.scale[side] == 1.0;
-            fc.mat[side] = copy ? -1 : mats.interp(Ne, fc.Nf, side, fc.offset[side], fc.scale[side]);
-            fc.mat[2 + side] = copy ? -1 : mats.proj(Ne, fc.Nf, side, fc.offset[side], fc.scale[side]);
+            const int cls = mortar_class(fc.offset[side], fc.scale[side]);
+            if (cls < 0) return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: a non-conforming face covers neither a whole "
+                                                          "element side nor half of one");
+            const bool copy = Ne == fc.Nf && cls == 0;
+            fc.mat[side] = copy ? -1 : m->mortar_key_off[(size_t)mortar_matrix_key(Ne, fc.Nf, side, cls, 0)];
+            fc.mat[2 + side] = copy ? -1 : m->mortar_key_off[(size_t)mortar_matrix_key(Ne, fc.Nf, side, cls, 1)];
         }
     for (int64_t e = 0; e < K; ++e)
         for (int s = 0; s < 4; ++s) {
@@ -530,37 +637,163 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
     std::vector<double> verts((size_t)(8 * K));
     for (int64_t e = 0; e < K; ++e)
         for (int k = 0; k < 8; ++k) verts[8 * m->dev_of[e] + k] = m->host_verts[8 * e + k];
-    ndg_status st;
-    if ((st = device_upload(&d.verts, verts, "upload vertices")) != NDG_OK) return st;
-    if ((st = device_upload(&d.side_nb, side_nb, "upload connectivity")) != NDG_OK) return st;
-    if ((st = device_upload(&d.bc, bcs, "upload boundary records")) != NDG_OK) return st;
+    if ((st = upload_to(store.verts, &d.verts, verts, "upload vertices")) != NDG_OK) return st;
+    if ((st = upload_to(store.side_nb, &d.side_nb, side_nb, "upload connectivity")) != NDG_OK) return st;
+    if ((st = upload_to(store.bc, &d.bc, bcs, "upload boundary records")) != NDG_OK) return st;
     d.n_bc = (int64_t)bcs.size();
-    if ((st = device_upload(&d.faces, faces, "upload mortar faces")) != NDG_OK) return st;
-    if ((st = device_upload(&d.mortar_mats, mats.data(), "upload mortar matrices")) != NDG_OK) return st;
+    if ((st = upload_to(store.faces, &d.faces, faces, "upload mortar faces")) != NDG_OK) return st;
+    d.mortar_mats = m->mortar_mats;
     d.n_faces = (int64_t)faces.size();
     d.mortar_nodes = 0;
     for (const MortarFace &fc : faces)
         d.mortar_nodes = std::max(d.mortar_nodes, std::max(fc.Nf, std::max(fc.NL, fc.NR)) + 1);
     d.n_side_flux = n_slot;
-    if (n_slot > 0) HIP_TRY(hipMalloc((void **)&d.side_flux, sizeof(double) * n_slot), "hipMalloc side fluxes");
-    {
-        std::vector<double> t1((size_t)(3 * (kMaxOrder + 1) * kTab1dStride), 0.0);
-        for (int n = 0; n <= std::min(kMaxOrder, tab.N_max); ++n)
-            for (int i = 0; i <= n; ++i) {
-                t1[(3 * n + 0) * kTab1dStride + i] = tab.tab[NDG_TABLE_NODES][n][i];
-                t1[(3 * n + 1) * kTab1dStride + i] = tab.tab[NDG_TABLE_WEIGHTS][n][i];
-                t1[(3 * n + 2) * kTab1dStride + i] = tab.tab[NDG_TABLE_BARYCENTRIC_WEIGHTS][n][i];
-            }
-        if ((st = device_upload(&d.tab1d, t1, "upload 1D tables")) != NDG_OK) return st;
+    d.side_flux = nullptr;
+    if (n_slot > 0 && (st = fit_view(store.side_flux, &d.side_flux, n_slot, "hipMalloc side fluxes")) != NDG_OK)
+        return st;
+    d.tab1d = m->tab1d;
+    for (Group &G : d.groups) {
+        if ((st = ensure_order_tables(m, G.N)) != NDG_OK) return st;
+        G.tab = m->tab64[G.N];
     }
-    for (Group &G : d.groups)
-        if ((st = device_upload(&G.tab, pack_tables(tab, G.N), "upload tables")) != NDG_OK) return st;
-    HIP_TRY(hipMalloc((void **)&d.q, sizeof(double) * d.n_q), "hipMalloc q");
-    HIP_TRY(hipMalloc((void **)&d.g, sizeof(double) * d.n_q), "hipMalloc g");
-    HIP_TRY(hipMalloc((void **)&d.trace[0], sizeof(double) * d.n_tr), "hipMalloc traces");
-    HIP_TRY(hipMalloc((void **)&d.trace[1], sizeof(double) * d.n_tr), "hipMalloc traces");
+    if ((st = fit_view(store.q, &d.q, d.n_q, "hipMalloc q")) != NDG_OK) return st;
+    if ((st = fit_view(store.g, &d.g, d.n_q, "hipMalloc g")) != NDG_OK) return st;
+    if ((st = fit_view(store.tr[0], &d.trace[0], d.n_tr, "hipMalloc traces")) != NDG_OK) return st;
+    if ((st = fit_view(store.tr[1], &d.trace[1], d.n_tr, "hipMalloc traces")) != NDG_OK) return st;
     d.scratch_len = 1024;
-    HIP_TRY(hipMalloc((void **)&d.scratch, sizeof(double) * d.scratch_len), "hipMalloc scratch");
+    if ((st = fit_view(store.scratch, &d.scratch, d.scratch_len, "hipMalloc scratch")) != NDG_OK) return st;
+    d.clock = m->clock;
+    return NDG_OK;
+}
+
+}  // namespace
+
+namespace {
+
+template <typename T>
+ndg_status download(std::vector<T> &dst, const void *src, int64_t n, hipStream_t s, const char *what) {
+    dst.resize((size_t)n);
+    if (n > 0) HIP_TRY(hipMemcpyAsync(dst.data(), src, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost, s), what);
+    return NDG_OK;
+}
+
+// Fits the arrays of a logical store to K elements and G ghosts and sets the view.
+ndg_status fit_logical(LogicalStore &ls, LogicalMesh &L, int64_t K, int64_t G) {
+    ndg_status st;
+    L.K = K;
+    L.n_ghost = G;
+    if ((st = fit_view(ls.verts, &L.verts, 8 * K, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.order, &L.order, K, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.status, &L.status, K, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.rotation, &L.rotation, K, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.level, &L.level, K, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.link, &L.link, 4 * K, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.hang2, &L.hang2, 4 * K, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.hang, &L.hang, 4 * K, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.periodic, &L.periodic, 4 * K, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.est, &L.est, 4 * K, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.dev_of, &L.dev_of, K, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.ghost, &L.ghost, G, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.ghost_side, &L.ghost_side, G, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.ghost_zero, &L.ghost_zero, G, "hipMalloc logical mesh")) != NDG_OK) return st;
+    return NDG_OK;
+}
+
+template <typename T, typename S>
+ndg_status upload_into(T *dst, const std::vector<S> &src, const char *what) {
+    if (src.empty()) return NDG_OK;
+    std::vector<T> tmp(src.begin(), src.end());
+    HIP_TRY(hipMemcpy(dst, tmp.data(), sizeof(T) * tmp.size(), hipMemcpyHostToDevice), what);
+    return NDG_OK;
+}
+
+// The logical mesh of the device adaptation path from the host topology (once, or again after a
+// host-path adaptation changed the mesh).
+ndg_status ensure_logical(ndg_mesh *m) {
+    if (m->logical_valid) return NDG_OK;
+    const int64_t K = m->d.K, G = (int64_t)m->bc_base.size();
+    LogicalMesh &L = m->lmesh[m->lactive];
+    ndg_status st;
+    if ((st = fit_logical(m->lstore[m->lactive], L, K, G)) != NDG_OK) return st;
+    std::vector<double> est((size_t)(4 * K), 0.0);
+    if (m->est_error.size() == (size_t)K)
+        for (int64_t e = 0; e < K; ++e) {
+            est[4 * e] = m->est_error[e];
+            est[4 * e + 1] = m->est_sigma[e];
+            est[4 * e + 2] = m->est_refine[e] ? 1.0 : 0.0;
+            est[4 * e + 3] = m->est_coarsen[e] ? 1.0 : 0.0;
+        }
+    std::vector<int64_t> ghost_side((size_t)G);
+    std::vector<int8_t> ghost_zero((size_t)G);
+    for (int64_t g = 0; g < G; ++g) {
+        const int64_t bar = m->bc_bar[(size_t)g];
+        ghost_side[g] = m->bar_side[(size_t)bar];
+        ghost_zero[g] = m->bar_zero[(size_t)bar] ? 1 : 0;
+    }
+    if ((st = upload_into(L.verts, m->host_verts, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.order, m->order, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.status, m->status, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.rotation, m->rotation, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.level, m->split_level, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.link, m->link, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.hang2, m->hang_second, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.hang, m->hang_kind, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.periodic, m->periodic_link, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.est, est, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.dev_of, m->dev_of, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.ghost, m->bc_base, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.ghost_side, ghost_side, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.ghost_zero, ghost_zero, "upload logical mesh")) != NDG_OK) return st;
+    m->n_periodic_sides = 0;
+    for (char c : m->periodic_link) m->n_periodic_sides += c != 0;
+    m->logical_valid = true;
+    return NDG_OK;
+}
+
+// Refreshes the host copies of the mesh description from the logical mesh after device
+// adaptations (orders, statuses, rotations, split levels, vertices, links, ghosts, estimates and
+// the device permutation).
+ndg_status ensure_host(ndg_mesh *m) {
+    if (!m->host_stale) return NDG_OK;
+    const LogicalMesh &L = m->lmesh[m->lactive];
+    const hipStream_t s = m->stream;
+    std::vector<int8_t> hang, per;
+    std::vector<double> est;
+    ndg_status st;
+    if ((st = download(m->host_verts, L.verts, 8 * L.K, s, "download logical mesh")) != NDG_OK) return st;
+    if ((st = download(m->order, L.order, L.K, s, "download logical mesh")) != NDG_OK) return st;
+    if ((st = download(m->status, L.status, L.K, s, "download logical mesh")) != NDG_OK) return st;
+    if ((st = download(m->rotation, L.rotation, L.K, s, "download logical mesh")) != NDG_OK) return st;
+    if ((st = download(m->split_level, L.level, L.K, s, "download logical mesh")) != NDG_OK) return st;
+    if ((st = download(m->link, L.link, 4 * L.K, s, "download logical mesh")) != NDG_OK) return st;
+    if ((st = download(m->hang_second, L.hang2, 4 * L.K, s, "download logical mesh")) != NDG_OK) return st;
+    if ((st = download(hang, L.hang, 4 * L.K, s, "download logical mesh")) != NDG_OK) return st;
+    if ((st = download(per, L.periodic, 4 * L.K, s, "download logical mesh")) != NDG_OK) return st;
+    if ((st = download(est, L.est, 4 * L.K, s, "download logical mesh")) != NDG_OK) return st;
+    if ((st = download(m->dev_of, L.dev_of, L.K, s, "download logical mesh")) != NDG_OK) return st;
+    if ((st = download(m->bc_base, L.ghost, L.n_ghost, s, "download logical mesh")) != NDG_OK) return st;
+    HIP_TRY(hipStreamSynchronize(s), "download logical mesh");
+    m->hang_kind.assign(hang.begin(), hang.end());
+    m->periodic_link.assign(per.begin(), per.end());
+    const size_t K = (size_t)L.K;
+    m->est_error.resize(K);
+    m->est_sigma.resize(K);
+    m->est_refine.resize(K);
+    m->est_coarsen.resize(K);
+    for (size_t e = 0; e < K; ++e) {
+        m->est_error[e] = est[4 * e];
+        m->est_sigma[e] = est[4 * e + 1];
+        m->est_refine[e] = est[4 * e + 2] != 0.0;
+        m->est_coarsen[e] = est[4 * e + 3] != 0.0;
+    }
+    m->host_off.assign(K, 0);
+    int64_t acc = 0;
+    for (size_t e = 0; e < K; ++e) {
+        m->host_off[e] = acc;
+        acc += (int64_t)(m->order[e] + 1) * (m->order[e] + 1);
+    }
+    m->n_dofs = acc;
+    m->host_stale = false;
     return NDG_OK;
 }
 
@@ -574,16 +807,8 @@ namespace {
 constexpr int kFirstStatus[4] = {3, 0, 1, 2};
 constexpr int kLastStatus[4] = {1, 2, 3, 0};
 constexpr int kInnerStatus[4][4] = {{0, 0, 1, 1}, {3, 3, 2, 2}, {3, 3, 2, 2}, {0, 0, 1, 1}};
-constexpr int kChildOrder[4][4][4] = {
-    {{0, 3, 2, 1}, {0, 1, 2, 3}, {2, 1, 0, 3}, {2, 3, 0, 1}},
-    {{3, 2, 1, 0}, {1, 2, 3, 0}, {1, 0, 3, 2}, {3, 0, 1, 2}},
-    {{2, 1, 0, 3}, {2, 3, 0, 1}, {0, 3, 2, 1}, {0, 1, 2, 3}},
-    {{1, 0, 3, 2}, {3, 0, 1, 2}, {3, 2, 1, 0}, {1, 2, 3, 0}}};
-constexpr int kChildStatus[4][4][4] = {
-    {{1, 3, 0, 0}, {0, 1, 1, 2}, {2, 2, 3, 1}, {3, 0, 2, 3}},
-    {{3, 0, 0, 1}, {1, 1, 2, 0}, {2, 3, 1, 2}, {0, 2, 3, 3}},
-    {{0, 0, 1, 3}, {1, 2, 0, 1}, {3, 1, 2, 2}, {2, 3, 3, 0}},
-    {{0, 1, 3, 0}, {2, 0, 1, 1}, {1, 2, 2, 3}, {3, 3, 0, 2}}};
+constexpr int kChildOrder[4][4][4] = NDG_HILBERT_CHILD_ORDER;
+constexpr int kChildStatus[4][4][4] = NDG_HILBERT_CHILD_STATUS;
 
 }  // namespace
 
@@ -874,11 +1099,11 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
     HIP_TRY(hipStreamCreateWithFlags(&m->own_stream, hipStreamNonBlocking), "hipStreamCreate");
     m->stream = m->own_stream;
     HIP_TRY(hipMalloc((void **)&m->dt_dev, sizeof(double)), "hipMalloc dt");
-    ndg_status st = build_device(m.get(), order, m->d);
+    HIP_TRY(hipMalloc((void **)&m->clock, sizeof(double) * 4), "hipMalloc clock");
+    HIP_TRY(hipMemsetAsync(m->clock, 0, sizeof(double) * 4, m->stream), "hipMemset clock");
+    ndg_status st = build_device(m.get(), order, m->d, m->store[0]);
     if (st != NDG_OK) return st;
     DeviceMesh &d = m->d;
-    HIP_TRY(hipMalloc((void **)&d.clock, sizeof(double) * 4), "hipMalloc clock");
-    HIP_TRY(hipMemsetAsync(d.clock, 0, sizeof(double) * 4, m->stream), "hipMemset clock");
     HIP_TRY(hipMemsetAsync(d.q, 0, sizeof(double) * d.n_q, m->stream), "hipMemset q");
     HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * d.n_q, m->stream), "hipMemset g");
     HIP_TRY(launch_traces(d, m->stream), "traces kernel");
@@ -971,17 +1196,23 @@ extern "C" ndg_status ndg_mesh_set_precision(ndg_mesh *mesh, int32_t precision) 
     if (precision == d.precision) return NDG_OK;
     if (precision == NDG_PRECISION_FP32_COMPENSATED) {
         if (!d.qh) {
-            HIP_TRY(hipMalloc((void **)&d.qh, sizeof(float) * d.n_q), "hipMalloc qh");
-            HIP_TRY(hipMalloc((void **)&d.gl32, sizeof(float2) * d.n_q), "hipMalloc gl32");
-            HIP_TRY(hipMalloc((void **)&d.trace32[0], sizeof(float) * d.n_tr), "hipMalloc traces32");
-            HIP_TRY(hipMalloc((void **)&d.trace32[1], sizeof(float) * d.n_tr), "hipMalloc traces32");
-            if (d.n_side_flux > 0)
-                HIP_TRY(hipMalloc((void **)&d.side_flux32, sizeof(float) * d.n_side_flux), "hipMalloc side fluxes");
+            DeviceStore &sto = mesh->store[mesh->active];
+            ndg_status st;
+            if ((st = fit_view(sto.qh, &d.qh, d.n_q, "hipMalloc qh")) != NDG_OK) return st;
+            if ((st = fit_view(sto.gl32, &d.gl32, d.n_q, "hipMalloc gl32")) != NDG_OK) return st;
+            if ((st = fit_view(sto.tr32[0], &d.trace32[0], d.n_tr, "hipMalloc traces32")) != NDG_OK) return st;
+            if ((st = fit_view(sto.tr32[1], &d.trace32[1], d.n_tr, "hipMalloc traces32")) != NDG_OK) return st;
+            d.side_flux32 = nullptr;
+            if (d.n_side_flux > 0 &&
+                (st = fit_view(sto.side_flux32, &d.side_flux32, d.n_side_flux, "hipMalloc side fluxes")) != NDG_OK)
+                return st;
             for (Group &G : d.groups) {
-                const std::vector<double> t64 = pack_tables(mesh->tables->t, G.N);
-                std::vector<float> t32(t64.begin(), t64.end());
-                HIP_TRY(hipMalloc((void **)&G.tab32, sizeof(float) * t32.size()), "hipMalloc tab32");
-                HIP_TRY(hipMemcpy(G.tab32, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice), "upload tab32");
+                if (!mesh->tab32[G.N]) {
+                    const std::vector<double> t64 = pack_tables(mesh->tables->t, G.N);
+                    const std::vector<float> t32(t64.begin(), t64.end());
+                    if ((st = device_upload(&mesh->tab32[G.N], t32, "upload tab32")) != NDG_OK) return st;
+                }
+                G.tab32 = mesh->tab32[G.N];
             }
         }
         d.precision = NDG_PRECISION_FP32_COMPENSATED;
@@ -998,11 +1229,15 @@ extern "C" ndg_status ndg_mesh_set_precision(ndg_mesh *mesh, int32_t precision) 
 
 extern "C" ndg_status ndg_set_ic(ndg_mesh *mesh, const ndg_ic_desc *ic) {
     if (!mesh || !ic) return fail(NDG_ERR_INVALID, "ndg_set_ic: null argument");
-    if (ic->kind != NDG_IC_PLANE_WAVE && ic->kind != NDG_IC_PULSES) return fail(NDG_ERR_INVALID, "ndg_set_ic: unknown kind");
-    if (ic->kind == NDG_IC_PULSES && (ic->n_pulses < 0 || (ic->n_pulses > 0 && !ic->pulses)))
+    if (ic->kind != NDG_IC_PLANE_WAVE && ic->kind != NDG_IC_PULSES && ic->kind != NDG_IC_PERIODIC_PULSES)
+        return fail(NDG_ERR_INVALID, "ndg_set_ic: unknown kind");
+    const bool with_pulses = ic->kind != NDG_IC_PLANE_WAVE;
+    if (with_pulses && (ic->n_pulses < 0 || (ic->n_pulses > 0 && !ic->pulses)))
         return fail(NDG_ERR_INVALID, "ndg_set_ic: bad pulse list");
+    if (ic->kind == NDG_IC_PERIODIC_PULSES && !(ic->period_x > 0.0 && ic->period_y > 0.0))
+        return fail(NDG_ERR_INVALID, "ndg_set_ic: periodic pulses need positive periods");
     DeviceMesh &d = mesh->d;
-    const int np = ic->kind == NDG_IC_PULSES ? ic->n_pulses : 0;
+    const int np = with_pulses ? ic->n_pulses : 0;
     if (mesh->pulses_dev) {
         HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_set_ic sync");
         HIP_TRY(hipFree(mesh->pulses_dev), "hipFree");
@@ -1012,9 +1247,11 @@ extern "C" ndg_status ndg_set_ic(ndg_mesh *mesh, const ndg_ic_desc *ic) {
         HIP_TRY(hipMalloc((void **)&mesh->pulses_dev, sizeof(double) * 4 * np), "hipMalloc pulses");
         HIP_TRY(hipMemcpy(mesh->pulses_dev, ic->pulses, sizeof(double) * 4 * np, hipMemcpyHostToDevice), "upload pulses");
     }
-    HIP_TRY(launch_ic(d, ic->kind, np, mesh->pulses_dev, mesh->stream), "ic kernel");
+    HIP_TRY(launch_ic(d, ic->kind, np, mesh->pulses_dev, ic->period_x, ic->period_y, mesh->stream), "ic kernel");
     mesh->ic_kind = ic->kind;
     mesh->ic_pulses.assign(ic->pulses, ic->pulses + 4 * np);
+    mesh->ic_period[0] = ic->period_x;
+    mesh->ic_period[1] = ic->period_y;
     HIP_TRY(hipMemsetAsync(d.g, 0, sizeof(double) * d.n_q, mesh->stream), "hipMemset g");
     HIP_TRY(launch_traces(d, mesh->stream), "traces kernel");
     ndg_status st = refresh_c32(mesh);
@@ -1025,6 +1262,7 @@ extern "C" ndg_status ndg_set_ic(ndg_mesh *mesh, const ndg_ic_desc *ic) {
 
 extern "C" ndg_status ndg_set_state(ndg_mesh *mesh, const double *p, const double *u, const double *v) {
     if (!mesh || !p || !u || !v) return fail(NDG_ERR_INVALID, "ndg_set_state: null argument");
+    if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
     DeviceMesh &d = mesh->d;
     HIP_TRY(hipStreamSynchronize(mesh->stream), "ndg_set_state sync");
     double *const ins[3] = {const_cast<double *>(p), const_cast<double *>(u), const_cast<double *>(v)};
@@ -1039,6 +1277,7 @@ extern "C" ndg_status ndg_set_state(ndg_mesh *mesh, const double *p, const doubl
 
 extern "C" ndg_status ndg_get_state(ndg_mesh *mesh, double *p, double *u, double *v) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_get_state: null mesh");
+    if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
     ndg_status st = refresh_fp64(mesh);
     if (st != NDG_OK) return st;
     double *const outs[3] = {p, u, v};
@@ -1074,7 +1313,10 @@ ndg_status interpolated_output(ndg_mesh *mesh, const char *who, bool complete, d
     if (st != NDG_OK) return st;
     DeviceMesh &d = mesh->d;
     if (complete) {
-        if (!d.rhs) HIP_TRY(hipMalloc((void **)&d.rhs, sizeof(double) * d.n_q), "hipMalloc rhs");
+        if (!d.rhs) {
+            const ndg_status rs = fit_view(mesh->store[mesh->active].rhs, &d.rhs, d.n_q, "hipMalloc rhs");
+            if (rs != NDG_OK) return rs;
+        }
         if ((st = rhs_keeping_clock(mesh, time)) != NDG_OK) return st;
     }
     const int64_t K = d.K, nn = n * n;
@@ -1126,12 +1368,14 @@ ndg_status interpolated_output(ndg_mesh *mesh, const char *who, bool complete, d
 
 extern "C" ndg_status ndg_get_solution(ndg_mesh *mesh, double *x, double *y, double *p, double *u, double *v) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_get_solution: null mesh");
+    if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
     double *const planes[5] = {x, y, p, u, v};
     return interpolated_output(mesh, "ndg_get_solution", false, 0.0, planes);
 }
 
 extern "C" ndg_status ndg_get_complete_solution(ndg_mesh *mesh, double time, const ndg_complete_solution *out) {
     if (!mesh || !out) return fail(NDG_ERR_INVALID, "ndg_get_complete_solution: null argument");
+    if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
     if (!std::isfinite(time)) return fail(NDG_ERR_INVALID, "ndg_get_complete_solution: time is not finite");
     double *const planes[11] = {out->x,     out->y,     out->p,     out->u,
                                 out->v,     out->dp_dt, out->du_dt, out->dv_dt,
@@ -1160,6 +1404,7 @@ extern "C" ndg_status ndg_get_complete_solution(ndg_mesh *mesh, double time, con
 
 extern "C" ndg_status ndg_get_traces(ndg_mesh *mesh, double *out) {
     if (!mesh || !out) return fail(NDG_ERR_INVALID, "ndg_get_traces: null argument");
+    if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
     ndg_status st = refresh_fp64(mesh);
     if (st != NDG_OK) return st;
     const DeviceMesh &d = mesh->d;
@@ -1184,10 +1429,14 @@ extern "C" ndg_status ndg_get_traces(ndg_mesh *mesh, double *out) {
 
 extern "C" ndg_status ndg_compute_rhs(ndg_mesh *mesh, double t, double *Gp, double *Gu, double *Gv) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_compute_rhs: null mesh");
+    if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
     ndg_status st = refresh_fp64(mesh);
     if (st != NDG_OK) return st;
     DeviceMesh &d = mesh->d;
-    if (!d.rhs) HIP_TRY(hipMalloc((void **)&d.rhs, sizeof(double) * d.n_q), "hipMalloc rhs");
+    if (!d.rhs) {
+        const ndg_status rs = fit_view(mesh->store[mesh->active].rhs, &d.rhs, d.n_q, "hipMalloc rhs");
+        if (rs != NDG_OK) return rs;
+    }
     if ((st = rhs_keeping_clock(mesh, t)) != NDG_OK) return st;
     double *const outs[3] = {Gp, Gu, Gv};
     return copy_state(mesh, d.rhs, outs, true, "download rhs");
@@ -1672,10 +1921,7 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
         return code;
     };
     DeviceMesh nd;
-    if ((st = build_device(mesh, order, nd)) != NDG_OK) {
-        free_device(nd);
-        return restore(st);
-    }
+    if ((st = build_device(mesh, order, nd, mesh->store[1 - mesh->active])) != NDG_OK) return restore(st);
     auto offset = [](const DeviceMesh &dm, int N, int64_t dev) -> int64_t {
         for (const Group &G : dm.groups)
             if (G.N == N) return G.q_off + (dev - G.e_begin) * 3 * (int64_t)(N + 1) * (N + 1);
@@ -1702,16 +1948,11 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
     }
     if (recs_dev) (void)hipFree(recs_dev);
     if (coords_dev) (void)hipFree(coords_dev);
-    if (st != NDG_OK) {
-        free_device(nd);
-        return restore(st);
-    }
-    nd.clock = d.clock;
-    d.clock = nullptr;
+    if (st != NDG_OK) return restore(st);
     nd.wave_speed = d.wave_speed;
     carry_tuning(d, nd);
-    free_device(d);
     d = nd;
+    mesh->active ^= 1;
     {
         std::vector<char> keep((size_t)Knew);
         for (int64_t k = 0; k < Knew; ++k) keep[(size_t)k] = !child[(size_t)k] && !raise[(size_t)parent[(size_t)k]];
@@ -1729,8 +1970,149 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
 
 }  // namespace
 
+namespace {
+
+// ndg_adapt on the device (adapt.hip): estimator, decisions, new logical mesh, layout and transfer
+// without the host topology; the host reads back two small count blocks. Same outcomes as the
+// host path (the splitting branch of Mesh2D_t::adapt when any element would h-refine, else the
+// p-refinement branch); refused cases change nothing.
+ndg_status adapt_on_device(ndg_mesh *mesh, int32_t N_max, int32_t max_split_level, int64_t *n_changed) {
+    ndg_status st;
+    if ((st = ensure_logical(mesh)) != NDG_OK) return st;
+    if ((st = refresh_fp64(mesh)) != NDG_OK) return st;
+    DeviceMesh &d = mesh->d;
+    const hipStream_t s = mesh->stream;
+    AdaptWork &w = mesh->work;
+    const LogicalMesh &L = mesh->lmesh[mesh->lactive];
+    if (mesh->est_len < d.K) {
+        if (mesh->est_dev) HIP_TRY(hipFree(mesh->est_dev), "hipFree");
+        mesh->est_dev = nullptr;
+        HIP_TRY(hipMalloc((void **)&mesh->est_dev, sizeof(double) * 4 * d.K), "hipMalloc estimator");
+        mesh->est_len = d.K;
+    }
+    HIP_TRY(launch_estimate(d, mesh->tol_min, mesh->tol_max, mesh->est_dev, s), "estimator kernel");
+    HIP_TRY(adapt_decide(L, mesh->est_dev, N_max, mesh->tables->t.N_max, max_split_level, w, s), "adaptation decisions");
+    AdaptCounts c;
+    HIP_TRY(hipMemcpyAsync(&c, w.counts.p, sizeof c, hipMemcpyDeviceToHost, s), "download adaptation counts");
+    HIP_TRY(hipStreamSynchronize(s), "ndg_adapt sync");
+    mesh->host_stale = true;  // the logical copy of the estimate changed
+    if (c.refused & kRefuseOrder)
+        return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a raised order exceeds the compiled range or the tables");
+    if (c.n_split > 0 && (c.refused & kRefusePeriodic))
+        return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: splitting an element with a periodic side is not provided");
+    if (c.n_split > 0 && (c.refused & kRefuseFourFaces))
+        return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a split would leave more than two faces on a side");
+    if (n_changed) *n_changed = (int64_t)(c.n_split + c.n_raise);
+    if (c.n_split + c.n_raise == 0) return NDG_OK;
+    const bool p_only = c.n_split == 0;
+    const int precision = d.precision;
+    if (precision == NDG_PRECISION_FP32_COMPENSATED && (st = ndg_mesh_set_precision(mesh, NDG_PRECISION_FP64)) != NDG_OK)
+        return st;
+    auto restore = [&](ndg_status code) {
+        if (precision == NDG_PRECISION_FP32_COMPENSATED) (void)ndg_mesh_set_precision(mesh, precision);
+        return code;
+    };
+    drop_graphs(mesh);
+    const int nl = 1 - mesh->lactive;
+    LogicalMesh &Ln = mesh->lmesh[nl];
+    const int64_t Kn = L.K + 3 * (int64_t)c.n_split, Gn = L.n_ghost + (int64_t)c.ghost_split;
+    if ((st = fit_logical(mesh->lstore[nl], Ln, Kn, Gn)) != NDG_OK) return restore(st);
+    auto hip_restore = [&](hipError_t e, const char *where) { return restore(hip_fail(e, where)); };
+    hipError_t he;
+    if ((he = adapt_rebuild(L, Ln, p_only, w, s)) != hipSuccess) return hip_restore(he, "adaptation rebuild");
+    DeviceStore &ns = mesh->store[1 - mesh->active];
+    GroupSpan *spans_new = nullptr;
+    if ((st = fit_view(ns.spans, &spans_new, kMaxOrder + 1, "hipMalloc group spans")) != NDG_OK) return restore(st);
+    if ((he = adapt_layout_count(Ln, spans_new, w, s)) != hipSuccess) return hip_restore(he, "adaptation layout");
+    if ((he = hipMemcpyAsync(&c, w.counts.p, sizeof c, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (he = hipStreamSynchronize(s)) != hipSuccess)
+        return hip_restore(he, "download adaptation counts");
+    // the device mesh of the new layout, in the spare store
+    DeviceMesh nd;
+    nd.K = Kn;
+    int64_t e_begin = 0, q_off = 0, tr_off = 0;
+    for (int n = 0; n <= kMaxOrder; ++n) {
+        const int64_t Kg = (int64_t)c.order_count[n];
+        if (Kg == 0) continue;
+        if ((st = ensure_order_tables(mesh, n)) != NDG_OK) return restore(st);
+        Group G;
+        G.N = n;
+        G.K = Kg;
+        G.e_begin = e_begin;
+        G.q_off = q_off;
+        G.tr_off = tr_off;
+        G.tab = mesh->tab64[n];
+        nd.groups.push_back(G);
+        e_begin += Kg;
+        q_off += 3 * (int64_t)(n + 1) * (n + 1) * Kg;
+        tr_off += 12 * (int64_t)(n + 1) * Kg;
+    }
+    nd.n_q = q_off;
+    nd.n_tr = tr_off;
+    nd.N_min = nd.groups.front().N;
+    nd.N_max = nd.groups.back().N;
+    nd.n_bc = Gn + (int64_t)c.n_rec;
+    nd.n_faces = (int64_t)c.n_face;
+    nd.n_side_flux = (int64_t)c.n_slot;
+    nd.mortar_nodes = (int)c.mortar_nodes;
+    if ((st = fit_view(ns.verts, &nd.verts, 8 * Kn, "hipMalloc vertices")) != NDG_OK ||
+        (st = fit_view(ns.side_nb, &nd.side_nb, 4 * Kn, "hipMalloc connectivity")) != NDG_OK ||
+        (st = fit_view(ns.bc, &nd.bc, nd.n_bc, "hipMalloc boundary records")) != NDG_OK ||
+        (st = fit_view(ns.faces, &nd.faces, nd.n_faces, "hipMalloc mortar faces")) != NDG_OK ||
+        (st = fit_view(ns.q, &nd.q, nd.n_q, "hipMalloc q")) != NDG_OK ||
+        (st = fit_view(ns.g, &nd.g, nd.n_q, "hipMalloc g")) != NDG_OK ||
+        (st = fit_view(ns.tr[0], &nd.trace[0], nd.n_tr, "hipMalloc traces")) != NDG_OK ||
+        (st = fit_view(ns.tr[1], &nd.trace[1], nd.n_tr, "hipMalloc traces")) != NDG_OK)
+        return restore(st);
+    nd.scratch_len = 1024;
+    if ((st = fit_view(ns.scratch, &nd.scratch, nd.scratch_len, "hipMalloc scratch")) != NDG_OK) return restore(st);
+    if (nd.n_side_flux > 0 &&
+        (st = fit_view(ns.side_flux, &nd.side_flux, nd.n_side_flux, "hipMalloc side fluxes")) != NDG_OK)
+        return restore(st);
+    if (nd.n_faces == 0) nd.faces = nullptr;
+    nd.tab1d = mesh->tab1d;
+    nd.mortar_mats = mesh->mortar_mats;
+    nd.clock = mesh->clock;
+    nd.has_inflow = d.has_inflow;
+    nd.wave_speed = d.wave_speed;
+    nd.cur = 0;
+    carry_tuning(d, nd);
+    if ((he = adapt_layout_fill(Ln, spans_new, mesh->mortar_key_dev, nd, w, s)) != hipSuccess)
+        return hip_restore(he, "adaptation layout");
+    if ((he = adapt_transfer(L, (const GroupSpan *)mesh->store[mesh->active].spans.p, d.q, Ln, spans_new, nd.q,
+                             mesh->tab1d, w, s)) != hipSuccess)
+        return hip_restore(he, "transfer kernel");
+    if ((he = hipMemsetAsync(nd.g, 0, sizeof(double) * nd.n_q, s)) != hipSuccess) return hip_restore(he, "hipMemset g");
+    if ((he = launch_traces(nd, s)) != hipSuccess) return hip_restore(he, "traces kernel");
+    AdaptCounts c2;
+    if ((he = hipMemcpyAsync(&c2, w.counts.p, sizeof c2, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (he = hipStreamSynchronize(s)) != hipSuccess)
+        return hip_restore(he, "ndg_adapt sync");
+    if (c2.bad_face)
+        return restore(fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a non-conforming face covers neither a whole element "
+                                                 "side nor half of one"));
+    // commit
+    d = nd;
+    mesh->active ^= 1;
+    mesh->lactive = nl;
+    mesh->n_faces = (int64_t)c.conforming_pairs + Gn + mesh->n_periodic_sides + 2 * (int64_t)c.larger_sides;
+    mesh->n_dofs = (int64_t)c.dofs;
+    mesh->host_stale = true;
+    mesh->host_nodes_valid = false;
+    if (precision == NDG_PRECISION_FP32_COMPENSATED && (st = ndg_mesh_set_precision(mesh, precision)) != NDG_OK)
+        return st;
+    return NDG_OK;
+}
+
+}  // namespace
+
 extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split_level, int64_t *n_raised) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_adapt: null mesh");
+    if (mesh->adapt_mode == NDG_ADAPT_DEVICE) return adapt_on_device(mesh, N_max, max_split_level, n_raised);
+    if (!mesh->host_nodes_valid)
+        return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: the host path needs the mesh's node lists, which a device "
+                                         "adaptation does not keep");
+    mesh->logical_valid = false;  // the host path changes the mesh behind the logical copy
     DeviceMesh &d = mesh->d;
     const int64_t K = d.K;
     std::vector<double> err((size_t)K), sig((size_t)K);
@@ -1782,11 +2164,8 @@ extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split
             mesh->bar_zero[(size_t)mesh->bc_bar[(size_t)(-l - 1)]] = 1;
     }
     DeviceMesh nd;
-    st = build_device(mesh, order, nd);
-    if (st != NDG_OK) {
-        free_device(nd);
-        return restore(st);
-    }
+    st = build_device(mesh, order, nd, mesh->store[1 - mesh->active]);
+    if (st != NDG_OK) return restore(st);
     auto offset = [](const DeviceMesh &dm, int N, int64_t dev) -> int64_t {
         for (const Group &G : dm.groups)
             if (G.N == N) return G.q_off + (dev - G.e_begin) * 3 * (int64_t)(N + 1) * (N + 1);
@@ -1809,17 +2188,12 @@ extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split
         if (e2 != hipSuccess) st = hip_fail(e2, "transfer kernel");
     }
     if (recs_dev) (void)hipFree(recs_dev);
-    if (st != NDG_OK) {
-        free_device(nd);
-        return restore(st);
-    }
-    nd.clock = d.clock;
-    d.clock = nullptr;
+    if (st != NDG_OK) return restore(st);
     nd.wave_speed = d.wave_speed;
     nd.cur = 0;
     carry_tuning(d, nd);
-    free_device(d);
     d = nd;
+    mesh->active ^= 1;
     {
         std::vector<int64_t> same((size_t)K);
         std::vector<char> keep((size_t)K);
@@ -1840,7 +2214,38 @@ extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split
 
 extern "C" ndg_status ndg_mesh_orders(const ndg_mesh *mesh, int32_t *orders) {
     if (!mesh || !orders) return fail(NDG_ERR_INVALID, "ndg_mesh_orders: null argument");
+    if (mesh->host_stale) {  // after a device adaptation: the orders alone, from the logical mesh
+        const LogicalMesh &L = mesh->lmesh[mesh->lactive];
+        HIP_TRY(hipMemcpyAsync(orders, L.order, sizeof(int32_t) * L.K, hipMemcpyDeviceToHost, mesh->stream),
+                "download orders");
+        HIP_TRY(hipStreamSynchronize(mesh->stream), "download orders");
+        return NDG_OK;
+    }
     std::copy(mesh->order.begin(), mesh->order.end(), orders);
+    return NDG_OK;
+}
+
+extern "C" ndg_status ndg_mesh_set_adapt_mode(ndg_mesh *mesh, int32_t mode) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_mesh_set_adapt_mode: null mesh");
+    if (mode != NDG_ADAPT_DEVICE && mode != NDG_ADAPT_HOST) return fail(NDG_ERR_INVALID, "ndg_mesh_set_adapt_mode: unknown mode");
+    if (mode == NDG_ADAPT_HOST && !mesh->host_nodes_valid)
+        return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_set_adapt_mode: the host path needs the mesh's node lists, which a "
+                                         "device adaptation does not keep");
+    mesh->adapt_mode = mode;
+    return NDG_OK;
+}
+
+extern "C" ndg_status ndg_mesh_sides(ndg_mesh *mesh, int64_t *link, int8_t *hang, int64_t *hang_second,
+                                     int8_t *periodic) {
+    if (!mesh) return fail(NDG_ERR_INVALID, "ndg_mesh_sides: null mesh");
+    if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
+    const size_t n = mesh->link.size();
+    for (size_t i = 0; i < n; ++i) {
+        if (link) link[i] = mesh->link[i];
+        if (hang) hang[i] = (int8_t)mesh->hang_kind[i];
+        if (hang_second) hang_second[i] = mesh->hang_second[i];
+        if (periodic) periodic[i] = (int8_t)mesh->periodic_link[i];
+    }
     return NDG_OK;
 }
 
@@ -1937,7 +2342,8 @@ extern "C" ndg_status ndg_pre_condition(ndg_mesh *mesh, double cfl, int64_t n_ad
     if (n_adaptivity_steps < 0) return fail(NDG_ERR_INVALID, "ndg_pre_condition: negative adaptation count");
     if (mesh->ic_kind < 0) return fail(NDG_ERR_INVALID, "ndg_pre_condition: no initial condition was set");
     const std::vector<double> pulses = mesh->ic_pulses;  // ndg_set_ic below rewrites the mesh's copy
-    const ndg_ic_desc ic_keep{mesh->ic_kind, (int32_t)(pulses.size() / 4), pulses.empty() ? nullptr : pulses.data()};
+    const ndg_ic_desc ic_keep{mesh->ic_kind, (int32_t)(pulses.size() / 4), pulses.empty() ? nullptr : pulses.data(),
+                              mesh->ic_period[0], mesh->ic_period[1]};
     double time = 0.0, dt = 0.0;
     int64_t timestep = 0, n_adaptivity = 0;
     ndg_status st;
@@ -1986,6 +2392,7 @@ extern "C" ndg_status ndg_synchronize(ndg_mesh *mesh) {
 extern "C" ndg_status ndg_estimate_error(ndg_mesh *mesh, double *error, double *sigma, int32_t *refine,
                                          int32_t *coarsen) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_estimate_error: null mesh");
+    if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
     ndg_status st = refresh_fp64(mesh);
     if (st != NDG_OK) return st;
     DeviceMesh &d = mesh->d;
@@ -1996,6 +2403,8 @@ extern "C" ndg_status ndg_estimate_error(ndg_mesh *mesh, double *error, double *
         mesh->est_len = d.K;
     }
     HIP_TRY(launch_estimate(d, mesh->tol_min, mesh->tol_max, mesh->est_dev, mesh->stream), "estimator kernel");
+    if (mesh->logical_valid)
+        HIP_TRY(adapt_copy_estimate(mesh->lmesh[mesh->lactive], mesh->est_dev, mesh->stream), "estimate copy kernel");
     std::vector<double> host((size_t)(4 * d.K));
     HIP_TRY(hipMemcpyAsync(host.data(), mesh->est_dev, sizeof(double) * 4 * d.K, hipMemcpyDeviceToHost, mesh->stream),
             "download estimator");
@@ -2136,6 +2545,7 @@ ndg_status upload_receivers(const ndg_mesh *m, const ndg_receivers *rec, DeviceR
 extern "C" ndg_status ndg_record_receivers(ndg_mesh *mesh, const ndg_receivers *rec, double dt, int64_t n_steps,
                                            double *p_out) {
     if (!mesh || !p_out) return fail(NDG_ERR_INVALID, "ndg_record_receivers: null argument");
+    if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
     if (n_steps < 0 || !(dt > 0.0)) return fail(NDG_ERR_INVALID, "ndg_record_receivers: bad step count or dt");
     ndg_status st = check_adjoint_mesh(mesh, "ndg_record_receivers");
     if (st != NDG_OK) return st;
@@ -2160,6 +2570,7 @@ extern "C" ndg_status ndg_record_receivers(ndg_mesh *mesh, const ndg_receivers *
 extern "C" ndg_status ndg_record_receivers_noalloc(ndg_mesh *mesh, const ndg_receivers *rec, double dt,
                                                    int64_t n_steps, double *p_out) {
     if (!mesh || !p_out) return fail(NDG_ERR_INVALID, "ndg_record_receivers_noalloc: null argument");
+    if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
     if (n_steps < 0 || !(dt > 0.0)) return fail(NDG_ERR_INVALID, "ndg_record_receivers_noalloc: bad step count or dt");
     ndg_status st = check_adjoint_mesh(mesh, "ndg_record_receivers_noalloc");
     if (st != NDG_OK) return st;
@@ -2197,6 +2608,7 @@ extern "C" ndg_status ndg_record_receivers_noalloc(ndg_mesh *mesh, const ndg_rec
 extern "C" ndg_status ndg_apply_rhs_transpose(ndg_mesh *mesh, const double *mp, const double *mu, const double *mv,
                                               double *lp, double *lu, double *lv) {
     if (!mesh || !mp || !mu || !mv) return fail(NDG_ERR_INVALID, "ndg_apply_rhs_transpose: null argument");
+    if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
     ndg_status st = check_adjoint_mesh(mesh, "ndg_apply_rhs_transpose");
     if (st != NDG_OK) return st;
     DeviceMesh &d = mesh->d;
@@ -2243,6 +2655,7 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
                                            int64_t checkpoint_interval, const double *data, double *J,
                                            double *dJ_dp0, double *dJ_du0, double *dJ_dv0, double *dJ_dc) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_adjoint_gradient: null mesh");
+    if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
     if (n_steps < 1 || !(dt > 0.0) || checkpoint_interval < 1)
         return fail(NDG_ERR_INVALID, "ndg_adjoint_gradient: bad step count, dt or checkpoint interval");
     ndg_status st = check_adjoint_mesh(mesh, "ndg_adjoint_gradient");

@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import (BC_INFLOW, BC_NONE, BC_OUTFLOW, BC_PERIODIC, BC_SYMMETRY, BC_WALL,  # noqa: F401
-                   IC_PLANE_WAVE, IC_PULSES, NDGError, TABLE_NAMES, check, lib, ptr)
+                   IC_PERIODIC_PULSES, IC_PLANE_WAVE, IC_PULSES, NDGError, TABLE_NAMES, check, lib, ptr)
 
 
 class NDG_t:
@@ -176,6 +176,23 @@ class Mesh2D_t:
         self.orders_ = np.empty(self.n_elements_, dtype=np.int32)
         check(lib().ndg_mesh_orders(self._h, ptr(self.orders_)))
 
+    def set_adapt_mode(self, mode):
+        """"device" (default): adaptation rebuilt on the GPU from the device-resident topology;
+        "host": the topology rebuilt on the host from the node lists (refused after a device
+        adaptation). Both give the same meshes and states bit for bit."""
+        modes = {"device": _abi.ADAPT_DEVICE, "host": _abi.ADAPT_HOST}
+        check(lib().ndg_mesh_set_adapt_mode(self._h, modes[mode]))
+
+    def sides(self):
+        """Element-side topology, 4 entries per element: link (exterior side 4r+rs, or -(1+ghost)),
+        hang (0 conforming, 1 small hanging side, 2 larger side), the larger side's second small
+        side, and the periodic flag."""
+        n = 4 * self.n_elements_
+        link, hang2 = np.empty(n, dtype=np.int64), np.empty(n, dtype=np.int64)
+        hang, per = np.empty(n, dtype=np.int8), np.empty(n, dtype=np.int8)
+        check(lib().ndg_mesh_sides(self._h, ptr(link), ptr(hang), ptr(hang2), ptr(per)))
+        return link, hang, hang2, per
+
     def adapt(self, N_max=None, max_split_level=None):
         """Mesh2D_t::adapt: estimate, then either split the elements that would h-refine (when
         any would, the hp_adapt branch) or raise by 2 the order of those that would p-refine.
@@ -225,9 +242,10 @@ class Mesh2D_t:
     def set_stream(self, hip_stream_ptr: int):
         check(lib().ndg_mesh_set_stream(self._h, ctypes.c_void_p(hip_stream_ptr)))
 
-    def initial_conditions(self, pulses=None):
+    def initial_conditions(self, pulses=None, period=None):
         """Reference plane wave g(x, 0) when pulses is None, else radial Gaussian pulses given as
-        rows {x0, y0, width, amplitude}."""
+        rows {x0, y0, width, amplitude}; with period = (Lx, Ly) the pulses are periodised (each
+        with its 8 nearest images)."""
         ic = _abi.ICDesc()
         arr = None
         if pulses is None:
@@ -235,6 +253,9 @@ class Mesh2D_t:
         else:
             arr = _abi.as_f64(np.asarray(pulses, dtype=np.float64).reshape(-1, 4))
             ic.kind, ic.n_pulses, ic.pulses = IC_PULSES, arr.shape[0], ptr(arr)
+            if period is not None:
+                ic.kind = IC_PERIODIC_PULSES
+                ic.period_x, ic.period_y = float(period[0]), float(period[1])
         check(lib().ndg_set_ic(self._h, ctypes.byref(ic)))
 
     def set_solution(self, p, u, v):

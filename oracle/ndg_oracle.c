@@ -801,7 +801,7 @@ void o_mesh_get_N(const o_mesh *m, int32_t *N_out) {
 /* Initial conditions: src/meshes/Mesh2D_t.cpp:1192-1216                                       */
 /* ------------------------------------------------------------------------------------------ */
 
-static void ic_common(o_mesh *m, int kind, int n, const double *pulses) {
+static void ic_common(o_mesh *m, int kind, int n, const double *pulses, double Lx, double Ly) {
     for (int64_t ei = 0; ei < m->K; ++ei) {
         o_elem *e = &m->el[ei];
         const int N = e->N, nf = N + 1;
@@ -814,6 +814,17 @@ static void ic_common(o_mesh *m, int kind, int n, const double *pulses) {
                 double q[3];
                 if (kind == 0) {
                     g_exact(X.x, X.y, 0.0, q);
+                } else if (kind == 2) {
+                    q[0] = 0.0; q[1] = 0.0; q[2] = 0.0;
+                    for (int k = 0; k < n; ++k) {
+                        const double w = pulses[4 * k + 2];
+                        for (int mx = -1; mx <= 1; ++mx)
+                            for (int my = -1; my <= 1; ++my) {
+                                const double dx = X.x - (pulses[4 * k] + mx * Lx);
+                                const double dy = X.y - (pulses[4 * k + 1] + my * Ly);
+                                q[0] += pulses[4 * k + 3] * exp(-M_LN2 * (dx * dx + dy * dy) / (w * w));
+                            }
+                    }
                 } else {
                     q[0] = 0.0; q[1] = 0.0; q[2] = 0.0;
                     for (int k = 0; k < n; ++k) {
@@ -831,8 +842,13 @@ static void ic_common(o_mesh *m, int kind, int n, const double *pulses) {
     }
 }
 
-void o_initial_conditions_plane(o_mesh *m) { ic_common(m, 0, 0, NULL); }
-void o_initial_conditions_pulses(o_mesh *m, int n, const double *pulses) { ic_common(m, 1, n, pulses); }
+void o_initial_conditions_plane(o_mesh *m) { ic_common(m, 0, 0, NULL, 0.0, 0.0); }
+void o_initial_conditions_pulses(o_mesh *m, int n, const double *pulses) { ic_common(m, 1, n, pulses, 0.0, 0.0); }
+/* The pulses periodised with periods (Lx, Ly): each pulse and its 8 nearest images (no reference
+ * counterpart; pinned against the Fourier series of the periodised pulses). */
+void o_initial_conditions_periodic_pulses(o_mesh *m, int n, const double *pulses, double Lx, double Ly) {
+    ic_common(m, 2, n, pulses, Lx, Ly);
+}
 
 void o_set_state(o_mesh *m, const double *p, const double *u, const double *v) {
     int64_t off = 0;

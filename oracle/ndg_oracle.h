@@ -64,6 +64,7 @@ void o_mesh_get_N(const o_mesh *m, int32_t *N_out);
  * counterpart; pulses = n x {x_k, y_k, w_k, a_k}). Both zero G and the RK register. */
 void o_initial_conditions_plane(o_mesh *m);
 void o_initial_conditions_pulses(o_mesh *m, int n, const double *pulses);
+void o_initial_conditions_periodic_pulses(o_mesh *m, int n, const double *pulses, double Lx, double Ly);
 
 /* Packed per-field state: element by element in mesh order, each element's (N+1)^2 values in
  * the reference node order q[i*(N+1)+j]. */

@@ -40,6 +40,7 @@ def lib():
             "o_mesh_n_nodes_total": ([vp], i64), "o_mesh_get_N": ([vp, vp], None),
             "o_initial_conditions_plane": ([vp], None),
             "o_initial_conditions_pulses": ([vp, i32, vp], None),
+            "o_initial_conditions_periodic_pulses": ([vp, i32, vp, f64, f64], None),
             "o_set_state": ([vp, vp, vp, vp], None), "o_get_state": ([vp, vp, vp, vp], None),
             "o_get_rhs": ([vp, vp, vp, vp], None), "o_get_traces": ([vp, vp], None),
             "o_get_solution": ([vp, vp, vp, vp, vp, vp], None),
@@ -120,9 +121,13 @@ class Mesh:
     def n_faces(self):
         return lib().o_mesh_n_faces(self.h)
 
-    def initial_conditions(self, pulses=None):
+    def initial_conditions(self, pulses=None, period=None):
         if pulses is None:
             lib().o_initial_conditions_plane(self.h)
+        elif period is not None:
+            arr = _f(np.asarray(pulses, dtype=np.float64).reshape(-1, 4))
+            lib().o_initial_conditions_periodic_pulses(self.h, arr.shape[0], _p(arr), float(period[0]),
+                                                       float(period[1]))
         else:
             arr = _f(np.asarray(pulses, dtype=np.float64).reshape(-1, 4))
             lib().o_initial_conditions_pulses(self.h, arr.shape[0], _p(arr))
@@ -189,7 +194,8 @@ class Mesh:
         self._refresh()
         return n, tf.value
 
-    def pre_condition(self, cfl, n_adaptivity_steps, interval, N_max, max_split_level, pulses, iterative=False):
+    def pre_condition(self, cfl, n_adaptivity_steps, interval, N_max, max_split_level, pulses, iterative=False,
+                      period=None):
         """Solver2D_t::pre_condition (include/solvers/Solver2D_t.tpp:138-236) and
         pre_condition_iterative (:238-336): from time 0, steps of get_delta_t with estimate_error +
         adapt every `interval` steps until n_adaptivity_steps adaptations are done; the initial
@@ -207,10 +213,10 @@ class Mesh:
                     raise RuntimeError("oracle pre_condition: adaptation refused")
                 n += 1
                 if iterative:
-                    self.initial_conditions(pulses)
+                    self.initial_conditions(pulses, period)
                     time = 0.0
         if not iterative:
-            self.initial_conditions(pulses)
+            self.initial_conditions(pulses, period)
 
     def set_flags(self, refine, sigma):
         r = np.ascontiguousarray(refine, dtype=np.int32)

@@ -158,3 +158,25 @@ def test_hp_adaptive_at_workload_shape():
     assert_same_elements(gm, om, t_cpu)
     for g, r in zip(gm.get_solution(), om.get_state()):
         assert rel(g, r) < TOL
+
+
+def test_periodised_pulses_initial_condition_and_pre_condition():
+    """The periodised-pulse initial condition (pulses and their 8 nearest images) on a mixed-order
+    periodic mesh against the oracle, and the iterative pre-conditioning that re-applies it after
+    each p-adaptation (the equal-error workload's start)."""
+    say("periodised pulses")
+    pulses = np.array([[0.1, -0.2, 0.2, 1.0], [-0.6, 0.7, 0.15, 0.5], [0.8, 0.85, 0.1, 0.7]])
+    desc = nd.square_mesh(8, x_periodic=True, y_periodic=True, jitter=0.1, seed=3)
+    desc.quad_N = np.array([4, 6, 8, 5] * 16, dtype=np.int32)
+    gm = nd.Mesh2D_t(desc, 4, 12, 5, 0, 0, 0, 1e-5, 1e-14, 0.0, nd.NDG_t(12, 5))
+    om = O.Mesh(O.Tables(12, 5), desc, 4, tol_min=1e-5, tol_max=1e-14)
+    gm.initial_conditions(pulses, period=(2.0, 2.0))
+    om.initial_conditions(pulses, period=(2.0, 2.0))
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL
+    solver = nd.Solver2D_t(0.5, [0.0])
+    solver.pre_condition_iterative(None, gm, 2, 3)
+    om.pre_condition(0.5, 2, 3, 12, 0, pulses, iterative=True, period=(2.0, 2.0))
+    assert np.array_equal(gm.orders_, om.N)
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL
