@@ -101,9 +101,12 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
             }
         }
     }
-    double mreg[QPT], lreg[QPT];
+    double mreg[QPT], lreg[QPT], qreg[QPT];
     const double *lg = lam + e0 * 3 * NE;
     const double *mg = mu + e0 * 3 * NE;
+    // the stage input state for the c-derivative, loaded with the other tiles so that its latency
+    // is hidden behind P1-P5
+    const double *qg = (qs && cpart && apply) ? qs + e0 * 3 * NE : nullptr;
 #pragma unroll
     for (int it = 0; it < QPT; ++it) {
         const int r = tid + it * T;
@@ -111,8 +114,10 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         const int rr = in ? r : 0;  // in-range address for every lane, value selected below
         const double lv = lg[rr];
         const double mv = (flags & kAdjMuIn) ? mg[rr] : 0.0;
+        const double qv = qg ? qg[rr] : 0.0;
         lreg[it] = in ? lv : 0.0;
         mreg[it] = in ? mv : 0.0;
+        qreg[it] = in ? qv : 0.0;
     }
     for (int r = tid; r < nel * 8; r += T) s_v[r] = verts[e0 * 8 + r];
     __syncthreads();
@@ -268,14 +273,13 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         // P6 (coalesced): lambda' = lambda + d lambda / J, and the c-derivative term
         // <d lambda, q_s>.
         double dot = 0.0;
-        const double *qg = qs ? qs + e0 * 3 * NE : nullptr;
 #pragma unroll
         for (int it = 0; it < QPT; ++it) {
             const int r = tid + it * T;
             const bool in = r < nel * 3 * NE;
             const int rr = in ? r : 0;
             const double dl = s_a[rr];
-            if (qg) dot += (in ? s_n[rr] : 0.0) * qg[rr];
+            if (qg) dot += (in ? s_n[rr] : 0.0) * qreg[it];
             lreg[it] += in ? dl : 0.0;
             if (in) s_a[r] = lreg[it];
         }
@@ -284,7 +288,9 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
             if (lane == 0) s_red[f] = dot;
         }
         __syncthreads();
-        if (cpart && tid == 0) cpart[blockIdx.x] += (s_red[0] + s_red[1]) + s_red[2];
+        // one adder per slot and launch: the no-return atomic gives the same sum without waiting
+        // for a read of the slot at the end of the workgroup
+        if (cpart && tid == 0) unsafeAtomicAdd(cpart + blockIdx.x, (s_red[0] + s_red[1]) + s_red[2]);
         // misfit source of the step end this stage reaches: lambda_p += res_r w_r / J (the weights
         // come divided by J; receivers in order, each node always handled by the same thread)
         if (flags & kAdjSource) {

@@ -2677,11 +2677,19 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     const int64_t n_kept =
         records ? 0 : std::max<int64_t>(1, std::min<int64_t>(n_ck, 1 + mesh->adjoint_store_budget / seg_bytes));
     const int64_t first_kept = n_ck - n_kept;
-    DevBuf ck, lam, mu, chi0, chi1, cpart, recs, res, recs_g;
+    // Recomputed segments alternate between two stage-store blocks, so that the recompute of
+    // segment k - 1 (on a second stream) runs beside the reverse sweep of segment k: the kept
+    // blocks of segments first_kept and first_kept + 1, or one extra block when only one is kept.
+    const bool overlap = !records && first_kept > 0;
+    DevBuf ck, lam, mu, chi0, chi1, cpart, recs, res, recs_g, extra;
     std::vector<DevBuf> store((size_t)n_kept);
     if ((st = dev_alloc(ck, qb * n_ck, "hipMalloc checkpoints")) != NDG_OK) return st;
     for (DevBuf &b : store)
         if ((st = dev_alloc(b, (size_t)seg_bytes, "hipMalloc stage store")) != NDG_OK) return st;
+    if (overlap && n_kept < 2 && (st = dev_alloc(extra, (size_t)seg_bytes, "hipMalloc stage store")) != NDG_OK)
+        return st;
+    double *const rbuf[2] = {n_kept >= 1 ? store[0].as<double>() : nullptr,
+                             n_kept >= 2 ? store[1].as<double>() : extra.as<double>()};
     if ((st = dev_alloc(lam, qb, "hipMalloc adjoint")) != NDG_OK) return st;
     if ((st = dev_alloc(mu, qb, "hipMalloc adjoint")) != NDG_OK) return st;
     if ((st = dev_alloc(chi0, sizeof(double) * d.n_tr, "hipMalloc adjoint")) != NDG_OK) return st;
@@ -2695,8 +2703,8 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     auto ckp = [&](int64_t k) { return ck.as<double>() + k * d.n_q; };
     // stage state (step j of segment k, relative to the segment start)
     auto stp = [&](int64_t k, int64_t j, int stage) {
-        const DevBuf &b = store[(size_t)(k >= first_kept ? k - first_kept : 0)];
-        return b.as<double>() + (j * 3 + stage) * d.n_q;
+        double *b = k >= first_kept ? store[(size_t)(k - first_kept)].as<double>() : rbuf[(first_kept - k) % 2];
+        return b + (j * 3 + stage) * d.n_q;
     };
     double *const q_main = d.q;
     // On every exit path, errors included, the mesh's state pointers are its own buffers again:
@@ -2714,24 +2722,27 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     // their output state into the next store slot (q_start itself stands for slot (n0, 0)); the
     // output of the last step goes to the mesh's own state buffer. Records the receivers after
     // every step when asked.
-    auto run_stored = [&](int64_t k, const double *q_start, int64_t n0, int64_t n1, bool record) -> ndg_status {
-        d.q = const_cast<double *>(q_start);
-        HIP_TRY(launch_traces(d, s), "traces kernel");
+    // dm: the mesh (forward pass) or a copy of it whose state pointers the recompute moves (on
+    // stream st, beside the reverse sweep, which reads none of the buffers the stage kernels write).
+    auto run_stored = [&](DeviceMesh &dm, hipStream_t st_, int64_t k, const double *q_start, int64_t n0, int64_t n1,
+                          bool record) -> ndg_status {
+        dm.q = const_cast<double *>(q_start);
+        HIP_TRY(launch_traces(dm, st_), "traces kernel");
         for (int64_t j = n0; j < n1; ++j)
             for (int stage = 0; stage < 3; ++stage) {
                 double *in = (j == n0 && stage == 0) ? const_cast<double *>(q_start) : stp(k, j - n0, stage);
                 double *out = stage < 2 ? stp(k, j - n0, stage + 1) : (j + 1 < n1 ? stp(k, j + 1 - n0, 0) : q_main);
-                d.q = in;
-                d.q_out = out;
-                HIP_TRY(launch_stage(d, stage, kAm[stage], kGm[stage] * dt, s), "stage kernel");
+                dm.q = in;
+                dm.q_out = out;
+                HIP_TRY(launch_stage(dm, stage, kAm[stage], kGm[stage] * dt, st_), "stage kernel");
                 if (stage == 2 && record) {
-                    d.q = out;
-                    HIP_TRY(launch_record(d, R.elem.as<int64_t>(), R.w.as<double>(), R.n,
-                                          recs.as<double>() + j * R.n, s), "record kernel");
+                    dm.q = out;
+                    HIP_TRY(launch_record(dm, R.elem.as<int64_t>(), R.w.as<double>(), R.n,
+                                          recs.as<double>() + j * R.n, st_), "record kernel");
                 }
             }
-        d.q = q_main;
-        d.q_out = nullptr;
+        dm.q = q_main;
+        dm.q_out = nullptr;
         return NDG_OK;
     };
     auto stage_input = [&](int64_t k, int64_t j, int stage) -> const double * {
@@ -2764,7 +2775,7 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
         for (int64_t k = first_kept; k < n_ck; ++k) {
             HIP_TRY(hipMemcpyAsync(ckp(k), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
             const int64_t n0 = k * interval, n1 = std::min(n0 + interval, n_steps);
-            if ((st = run_stored(k, ckp(k), n0, n1, true)) != NDG_OK) return st;
+            if ((st = run_stored(d, s, k, ckp(k), n0, n1, true)) != NDG_OK) return st;
         }
     }
     std::vector<double> hres((size_t)(R.n * n_steps));
@@ -2819,9 +2830,45 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
         HIP_TRY(launch_adjoint_stage(d, A, s), "adjoint export kernel");
         cc = 1;
     }
+    // The recompute stream (default priority) and its events: ev_rec[k] after segment k is
+    // recomputed, ev_rev[k] after its reverse sweep (its block is then free for segment k - 2).
+    struct Overlap {
+        hipStream_t rs = nullptr;
+        std::vector<hipEvent_t> ev_rec, ev_rev;
+        ~Overlap() {
+            if (rs) {
+                (void)hipStreamSynchronize(rs);
+                (void)hipStreamDestroy(rs);
+            }
+            for (hipEvent_t e : ev_rec) (void)hipEventDestroy(e);
+            for (hipEvent_t e : ev_rev) (void)hipEventDestroy(e);
+        }
+    } ov;
+    if (overlap) {
+        HIP_TRY(hipStreamCreateWithFlags(&ov.rs, hipStreamNonBlocking), "hipStreamCreate");
+        for (std::vector<hipEvent_t> *v : {&ov.ev_rec, &ov.ev_rev})
+            for (int64_t k = 0; k < n_ck; ++k) {
+                hipEvent_t e;
+                HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+                v->push_back(e);
+            }
+    }
+    auto recompute = [&](int64_t k) -> ndg_status {
+        const int64_t n0 = k * interval, n1 = std::min(n0 + interval, n_steps);
+        if (k + 2 < n_ck) HIP_TRY(hipStreamWaitEvent(ov.rs, ov.ev_rev[(size_t)(k + 2)], 0), "hipStreamWaitEvent");
+        DeviceMesh dr = d;
+        ndg_status rst = run_stored(dr, ov.rs, k, ckp(k), n0, n1, false);
+        if (rst != NDG_OK) return rst;
+        HIP_TRY(hipEventRecord(ov.ev_rec[(size_t)k], ov.rs), "hipEventRecord");
+        return NDG_OK;
+    };
     for (int64_t k = n_ck - 1; k >= 0; --k) {
         const int64_t n0 = k * interval, n1 = std::min(n0 + interval, n_steps);
-        if (!records && k < first_kept && (st = run_stored(k, ckp(k), n0, n1, false)) != NDG_OK) return st;
+        if (!records && k < first_kept) {
+            if (overlap) HIP_TRY(hipStreamWaitEvent(s, ov.ev_rec[(size_t)k], 0), "hipStreamWaitEvent");
+            else if ((st = run_stored(d, s, k, ckp(k), n0, n1, false)) != NDG_OK) return st;
+        }
+        if (overlap && k >= 1 && k - 1 < first_kept && (st = recompute(k - 1)) != NDG_OK) return st;
         for (int64_t j = n1 - 1; j >= n0; --j)
             for (int stage = 2; stage >= 0; --stage) {
                 AdjointArgs A;
@@ -2844,6 +2891,7 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
                 HIP_TRY(launch_adjoint_stage(d, A, s), "adjoint kernel");
                 if (more) cc = 1 - cc;
             }
+        if (overlap) HIP_TRY(hipEventRecord(ov.ev_rev[(size_t)k], s), "hipEventRecord");
     }
 
     // Outputs (lambda back from its J-scaled form); the mesh state goes back to the initial fields.
