@@ -276,6 +276,11 @@ ndg_status ndg_pre_condition(ndg_mesh *mesh, double cfl, int64_t n_adaptivity_st
  * The element count can change (ndg_mesh_info). *n_raised (may be NULL) gets the number of split
  * plus raised elements. */
 ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split_level, int64_t *n_raised);
+/* SEM::inverse_quad_map (include/functions/inverse_quad_map.tpp:46-88): the reference coordinates
+ * (xi, eta) of n physical points xy (x, y pairs) in the bilinear quad of the 4 vertices (x, y pairs,
+ * the element's local node order), from the closed-form root of the map's quadratic. The
+ * adaptation's solution transfer maps every child node into its parent with this function. */
+ndg_status ndg_inverse_quad_map(const double *vertices, int64_t n, const double *xy, double *xi_eta);
 /* The order of every element (caller's order); it changes with ndg_adapt. */
 ndg_status ndg_mesh_orders(const ndg_mesh *mesh, int32_t *orders);
 /* Where ndg_adapt runs. NDG_ADAPT_DEVICE (the default) keeps the element-side topology in device

@@ -2106,6 +2106,18 @@ ndg_status adapt_on_device(ndg_mesh *mesh, int32_t N_max, int32_t max_split_leve
 
 }  // namespace
 
+extern "C" ndg_status ndg_inverse_quad_map(const double *vertices, int64_t n, const double *xy, double *xi_eta) {
+    if (!vertices || n < 0 || (n > 0 && (!xy || !xi_eta))) return fail(NDG_ERR_INVALID, "ndg_inverse_quad_map: bad argument");
+    const v2 P[4] = {{vertices[0], vertices[1]}, {vertices[2], vertices[3]}, {vertices[4], vertices[5]},
+                     {vertices[6], vertices[7]}};
+    for (int64_t i = 0; i < n; ++i) {
+        const v2 r = reference_point(v2{xy[2 * i], xy[2 * i + 1]}, P);
+        xi_eta[2 * i] = r.x;
+        xi_eta[2 * i + 1] = r.y;
+    }
+    return NDG_OK;
+}
+
 extern "C" ndg_status ndg_adapt(ndg_mesh *mesh, int32_t N_max, int32_t max_split_level, int64_t *n_raised) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_adapt: null mesh");
     if (mesh->adapt_mode == NDG_ADAPT_DEVICE) return adapt_on_device(mesh, N_max, max_split_level, n_raised);

@@ -70,6 +70,16 @@ class MeshDescription:
         return int(self.quads.shape[0])
 
 
+def inverse_quad_map(vertices, xy):
+    """SEM::inverse_quad_map (include/functions/inverse_quad_map.tpp:46-88): reference coordinates
+    (n, 2) of the physical points xy (n, 2) in the bilinear quad of vertices (4, 2)."""
+    P = _abi.as_f64(np.asarray(vertices, dtype=np.float64).reshape(4, 2))
+    X = _abi.as_f64(np.asarray(xy, dtype=np.float64).reshape(-1, 2))
+    out = np.empty_like(X)
+    check(lib().ndg_inverse_quad_map(ptr(P), X.shape[0], ptr(X), ptr(out)))
+    return out
+
+
 def square_mesh(resolution: int, x_min=-1.0, x_max=1.0, y_min=-1.0, y_max=1.0,
                 bc: Sequence[int] = (BC_WALL, BC_WALL, BC_WALL, BC_WALL),
                 x_periodic=False, y_periodic=False, jitter=0.0, seed=0) -> MeshDescription:

@@ -1099,6 +1099,34 @@ static void boundary_conditions(o_mesh *m, double time) {
 
 static int next_side(int s) { return (s + 1) * !(s + 1 == 4); }
 
+/* project_to_faces' barycentric interpolation of one element trace (order Ne) to the nodes of one
+ * side of a face (order Nf) at xi = (x_face + 1) scale + 2 offset - 1, with the exact-node shortcut
+ * (src/meshes/Mesh2D_t.cpp:3820-3890); the right side (side 1) walks the face nodes backwards. */
+static void face_side_values(const o_tables *t, int Ne, const double *tp, const double *tu, const double *tv,
+                             int Nf, int side, double offset, double scale, double *op, double *ou, double *ov) {
+    const double *xf = t->tab[O_NODES][Nf];
+    const double *xe = t->tab[O_NODES][Ne], *le = t->tab[O_BARY][Ne];
+    for (int i = 0; i <= Nf; ++i) {
+        const double coord = (xf[side == 0 ? i : Nf - i] + 1) * scale + 2 * offset - 1;
+        double pn = 0.0, un = 0.0, vn = 0.0, den = 0.0;
+        for (int j = 0; j <= Ne; ++j) {
+            if (almost_equal(coord, xe[j])) {
+                pn = tp[j]; un = tu[j]; vn = tv[j];
+                den = 1.0;
+                break;
+            }
+            const double tt = le[j] / (coord - xe[j]);
+            pn += tt * tp[j];
+            un += tt * tu[j];
+            vn += tt * tv[j];
+            den += tt;
+        }
+        op[i] = pn / den;
+        ou[i] = un / den;
+        ov[i] = vn / den;
+    }
+}
+
 /* src/meshes/Mesh2D_t.cpp:3803-3893 project_to_faces */
 static void project_to_faces(o_mesh *m) {
     const o_tables *t = m->t;
@@ -1106,7 +1134,6 @@ static void project_to_faces(o_mesh *m) {
     for (int64_t fi = 0; fi < m->F; ++fi) {
         o_face *f = &m->fa[fi];
         const int Nf = f->N;
-        const double *xf = t->tab[O_NODES][Nf];
         for (int side = 0; side < 2; ++side) {
             const o_elem *e = &m->el[f->elements[side]];
             const int es = (int)f->sides[side];
@@ -1121,26 +1148,8 @@ static void project_to_faces(o_mesh *m) {
                     f->v[side][o] = e->ve[es][i];
                 }
             } else {
-                const double *xe = t->tab[O_NODES][e->N], *le = t->tab[O_BARY][e->N];
-                for (int i = 0; i <= Nf; ++i) {
-                    const double coord = (xf[side == 0 ? i : Nf - i] + 1) * f->scale[side] + 2 * f->offset[side] - 1;
-                    double pn = 0.0, un = 0.0, vn = 0.0, den = 0.0;
-                    for (int j = 0; j <= e->N; ++j) {
-                        if (almost_equal(coord, xe[j])) {
-                            pn = e->pe[es][j]; un = e->ue[es][j]; vn = e->ve[es][j];
-                            den = 1.0;
-                            break;
-                        }
-                        const double tt = le[j] / (coord - xe[j]);
-                        pn += tt * e->pe[es][j];
-                        un += tt * e->ue[es][j];
-                        vn += tt * e->ve[es][j];
-                        den += tt;
-                    }
-                    f->p[side][i] = pn / den;
-                    f->u[side][i] = un / den;
-                    f->v[side][i] = vn / den;
-                }
+                face_side_values(t, e->N, e->pe[es], e->ue[es], e->ve[es], Nf, side, f->offset[side], f->scale[side],
+                                 f->p[side], f->u[side], f->v[side]);
             }
         }
     }
@@ -1170,6 +1179,51 @@ void o_wave_flux(const double *in, double *out) {
     out[2] = ny * (wL + wR) / 2;
 }
 
+/* project_to_elements' Gauss-quadrature L2 projection of one face's flux (order Nf) onto an element
+ * side (order Ne), accumulated into out: T_i w_j / w_i F_j sf_i scale with the exact-node shortcut,
+ * negated and walked backwards for the face's right side (src/meshes/Mesh2D_t.cpp:3935-4003). */
+static void side_flux_from_face(const o_tables *t, int Ne, const double *sf, int Nf, int fwd, double offset,
+                                double scale, const double *fp, const double *fu, const double *fv, double *op,
+                                double *ou, double *ov) {
+    const double *xe = t->tab[O_NODES][Ne], *we = t->tab[O_WEIGHTS][Ne], *le = t->tab[O_BARY][Ne];
+    const double *xf = t->tab[O_NODES][Nf], *wf = t->tab[O_WEIGHTS][Nf];
+    for (int j = 0; j <= Nf; ++j) {
+        const double coord = (xf[fwd ? j : Nf - j] + 1) * scale + 2 * offset - 1;
+        int found = 0;
+        for (int i = 0; i <= Ne; ++i) {
+            if (almost_equal(coord, xe[i])) {
+                if (fwd) {
+                    op[i] += wf[j] / we[i] * fp[j] * sf[i] * scale;
+                    ou[i] += wf[j] / we[i] * fu[j] * sf[i] * scale;
+                    ov[i] += wf[j] / we[i] * fv[j] * sf[i] * scale;
+                } else {
+                    op[i] += -wf[j] / we[i] * fp[j] * sf[i] * scale;
+                    ou[i] += -wf[j] / we[i] * fu[j] * sf[i] * scale;
+                    ov[i] += -wf[j] / we[i] * fv[j] * sf[i] * scale;
+                }
+                found = 1;
+                break;
+            }
+        }
+        if (!found) {
+            double ssum = 0.0;
+            for (int i = 0; i <= Ne; ++i) ssum += le[i] / (coord - xe[i]);
+            for (int i = 0; i <= Ne; ++i) {
+                const double T = le[i] / ((coord - xe[i]) * ssum);
+                if (fwd) {
+                    op[i] += T * wf[j] / we[i] * fp[j] * sf[i] * scale;
+                    ou[i] += T * wf[j] / we[i] * fu[j] * sf[i] * scale;
+                    ov[i] += T * wf[j] / we[i] * fv[j] * sf[i] * scale;
+                } else {
+                    op[i] += -T * wf[j] / we[i] * fp[j] * sf[i] * scale;
+                    ou[i] += -T * wf[j] / we[i] * fu[j] * sf[i] * scale;
+                    ov[i] += -T * wf[j] / we[i] * fv[j] * sf[i] * scale;
+                }
+            }
+        }
+    }
+}
+
 /* src/meshes/Mesh2D_t.cpp:3895-4006 project_to_elements */
 static void project_to_elements(o_mesh *m) {
     const o_tables *t = m->t;
@@ -1193,47 +1247,12 @@ static void project_to_elements(o_mesh *m) {
                 }
             } else {
                 for (int i = 0; i <= Ne; ++i) { e->pf[s][i] = 0.0; e->uf[s][i] = 0.0; e->vf[s][i] = 0.0; }
-                const double *xe = t->tab[O_NODES][Ne], *we = t->tab[O_WEIGHTS][Ne], *le = t->tab[O_BARY][Ne];
                 for (int fk = 0; fk < e->nf[s]; ++fk) {
                     const o_face *f = &m->fa[e->faces[s][fk]];
-                    const double *xf = t->tab[O_NODES][f->N], *wf = t->tab[O_WEIGHTS][f->N];
                     const int fwd = ei == f->elements[0];
                     const int fs = fwd ? 0 : 1;
-                    for (int j = 0; j <= f->N; ++j) {
-                        const double coord = (xf[fwd ? j : f->N - j] + 1) * f->scale[fs] + 2 * f->offset[fs] - 1;
-                        int found = 0;
-                        for (int i = 0; i <= Ne; ++i) {
-                            if (almost_equal(coord, xe[i])) {
-                                if (fwd) {
-                                    e->pf[s][i] += wf[j] / we[i] * f->pflux[j] * e->sf[s][i] * f->scale[0];
-                                    e->uf[s][i] += wf[j] / we[i] * f->uflux[j] * e->sf[s][i] * f->scale[0];
-                                    e->vf[s][i] += wf[j] / we[i] * f->vflux[j] * e->sf[s][i] * f->scale[0];
-                                } else {
-                                    e->pf[s][i] += -wf[j] / we[i] * f->pflux[j] * e->sf[s][i] * f->scale[1];
-                                    e->uf[s][i] += -wf[j] / we[i] * f->uflux[j] * e->sf[s][i] * f->scale[1];
-                                    e->vf[s][i] += -wf[j] / we[i] * f->vflux[j] * e->sf[s][i] * f->scale[1];
-                                }
-                                found = 1;
-                                break;
-                            }
-                        }
-                        if (!found) {
-                            double ssum = 0.0;
-                            for (int i = 0; i <= Ne; ++i) ssum += le[i] / (coord - xe[i]);
-                            for (int i = 0; i <= Ne; ++i) {
-                                const double T = le[i] / ((coord - xe[i]) * ssum);
-                                if (fwd) {
-                                    e->pf[s][i] += T * wf[j] / we[i] * f->pflux[j] * e->sf[s][i] * f->scale[0];
-                                    e->uf[s][i] += T * wf[j] / we[i] * f->uflux[j] * e->sf[s][i] * f->scale[0];
-                                    e->vf[s][i] += T * wf[j] / we[i] * f->vflux[j] * e->sf[s][i] * f->scale[0];
-                                } else {
-                                    e->pf[s][i] += -T * wf[j] / we[i] * f->pflux[j] * e->sf[s][i] * f->scale[1];
-                                    e->uf[s][i] += -T * wf[j] / we[i] * f->uflux[j] * e->sf[s][i] * f->scale[1];
-                                    e->vf[s][i] += -T * wf[j] / we[i] * f->vflux[j] * e->sf[s][i] * f->scale[1];
-                                }
-                            }
-                        }
-                    }
+                    side_flux_from_face(t, Ne, e->sf[s], f->N, fwd, f->offset[fs], f->scale[fs], f->pflux, f->uflux,
+                                        f->vflux, e->pf[s], e->uf[s], e->vf[s]);
                 }
             }
         }
@@ -2012,4 +2031,62 @@ int o_hilbert_xy2d(int n, int x, int y) {
         hilbert_rot(n, &x, &y, rx, ry);
     }
     return d;
+}
+
+
+/* ------------------------------------------------------------------------------------------ */
+/* Single-operation entry points for the reference's own known-answer tests                    */
+/* (tests/element_to_face_projection_test.cu, face_to_element_projection_test.cu,              */
+/* element_to_element_interpolation_test.cu, inverse_quad_mapping_test.cu): the same helpers the */
+/* mesh code above uses, on caller-given traces, fluxes and elements.                          */
+/* ------------------------------------------------------------------------------------------ */
+
+#define O_MAX_NF 64  /* face / element orders up to 63 */
+
+void o_kat_face_values(const o_tables *t, int Ne, int Nf, int side, double offset, double scale,
+                       const double *trace, double *out) {
+    double u[O_MAX_NF], v[O_MAX_NF];
+    face_side_values(t, Ne, trace, trace, trace, Nf, side, offset, scale, out, u, v);
+}
+
+void o_kat_side_flux(const o_tables *t, int Ne, int Nf, int fwd, double offset, double scale, double sf,
+                     const double *flux, double *out) {
+    double sfv[O_MAX_NF], u[O_MAX_NF], v[O_MAX_NF];
+    for (int i = 0; i <= Ne; ++i) { sfv[i] = sf; u[i] = 0.0; v[i] = 0.0; }
+    side_flux_from_face(t, Ne, sfv, Nf, fwd, offset, scale, flux, flux, flux, out, u, v);
+}
+
+void o_kat_interpolate(const o_tables *t, int No, const double *Po, int N, const double *Pn, const double *q_old,
+                       double *q_new) {
+    const int neo = (No + 1) * (No + 1), ne = (N + 1) * (N + 1);
+    double *zo = (double *)calloc((size_t)neo, sizeof(double));
+    double *zn = (double *)calloc((size_t)ne * 5, sizeof(double));
+    o_elem o, e;
+    memset(&o, 0, sizeof o);
+    memset(&e, 0, sizeof e);
+    o.N = No;
+    o.p = (double *)q_old; o.u = zo; o.v = zo; o.Gp = zo; o.Gu = zo; o.Gv = zo;
+    e.N = N;
+    e.p = q_new; e.u = zn; e.v = zn + ne; e.Gp = zn + 2 * ne; e.Gu = zn + 3 * ne; e.Gv = zn + 4 * ne;
+    if (Po && Pn) {
+        v2 PO[4], PN[4];
+        for (int k = 0; k < 4; ++k) {
+            PO[k].x = Po[2 * k]; PO[k].y = Po[2 * k + 1];
+            PN[k].x = Pn[2 * k]; PN[k].y = Pn[2 * k + 1];
+        }
+        interpolate_h(t, &o, PO, &e, PN);
+    } else {
+        interpolate_p(t, &o, &e);
+    }
+    free(zo);
+    free(zn);
+}
+
+void o_kat_inverse_quad_map(const double *P, double x, double y, double *xi_eta) {
+    v2 PP[4];
+    for (int k = 0; k < 4; ++k) { PP[k].x = P[2 * k]; PP[k].y = P[2 * k + 1]; }
+    v2 g; g.x = x; g.y = y;
+    const v2 r = inverse_quad_map(g, PP);
+    xi_eta[0] = r.x;
+    xi_eta[1] = r.y;
 }

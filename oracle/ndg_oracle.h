@@ -141,6 +141,19 @@ void o_hilbert_d2xy(int n, int d, int *xy);
 void o_hilbert_tables(int *child_order, int *child_status, int *first, int *last, int *inner);
 int o_hilbert_xy2d(int n, int x, int y);
 
+
+/* Known-answer-test entry points: one project_to_faces side (trace of order Ne -> Nf face nodes),
+ * one project_to_elements face (flux of order Nf accumulated into an order-Ne side, edge scale sf),
+ * Element2D_t::interpolate_from (same geometry when Po or Pn is NULL, else a child element Pn
+ * inside the parent Po, vertices as 4 x (x, y)), and inverse_quad_map. */
+void o_kat_face_values(const o_tables *t, int Ne, int Nf, int side, double offset, double scale,
+                       const double *trace, double *out);
+void o_kat_side_flux(const o_tables *t, int Ne, int Nf, int fwd, double offset, double scale, double sf,
+                     const double *flux, double *out);
+void o_kat_interpolate(const o_tables *t, int No, const double *Po, int N, const double *Pn, const double *q_old,
+                       double *q_new);
+void o_kat_inverse_quad_map(const double *P, double x, double y, double *xi_eta);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,10 @@ def lib():
             "o_initial_conditions_plane": ([vp], None),
             "o_initial_conditions_pulses": ([vp, i32, vp], None),
             "o_initial_conditions_periodic_pulses": ([vp, i32, vp, f64, f64], None),
+            "o_kat_face_values": ([vp, i32, i32, i32, f64, f64, vp, vp], None),
+            "o_kat_side_flux": ([vp, i32, i32, i32, f64, f64, f64, vp, vp], None),
+            "o_kat_interpolate": ([vp, i32, vp, i32, vp, vp, vp], None),
+            "o_kat_inverse_quad_map": ([vp, f64, f64, vp], None),
             "o_set_state": ([vp, vp, vp, vp], None), "o_get_state": ([vp, vp, vp, vp], None),
             "o_get_rhs": ([vp, vp, vp, vp], None), "o_get_traces": ([vp, vp], None),
             "o_get_solution": ([vp, vp, vp, vp, vp, vp], None),
@@ -328,3 +332,39 @@ def hilbert_d2xy(n, d):
     xy = np.zeros(2, dtype=np.int32)
     lib().o_hilbert_d2xy(n, d, _p(xy))
     return int(xy[0]), int(xy[1])
+
+
+# Single operations for the reference's known-answer tests.
+def kat_face_values(tables, Ne, Nf, side, offset, scale, trace):
+    """project_to_faces on one face side: the element trace (order Ne) at the Nf face nodes."""
+    tr = _f(np.asarray(trace, dtype=np.float64))
+    out = np.empty(Nf + 1)
+    lib().o_kat_face_values(tables.h, Ne, Nf, side, offset, scale, _p(tr), _p(out))
+    return out
+
+
+def kat_side_flux(tables, Ne, Nf, fwd, offset, scale, sf, flux, out=None):
+    """project_to_elements of one face's flux (order Nf) into an order-Ne side, accumulated into out."""
+    fl = _f(np.asarray(flux, dtype=np.float64))
+    out = np.zeros(Ne + 1) if out is None else out
+    lib().o_kat_side_flux(tables.h, Ne, Nf, int(fwd), offset, scale, sf, _p(fl), _p(out))
+    return out
+
+
+def kat_interpolate(tables, No, q_old, N, P_old=None, P_new=None):
+    """Element2D_t::interpolate_from: the old element's field (order No) at the new element's nodes
+    (same geometry, or a child P_new inside P_old, vertices (4, 2))."""
+    q = _f(np.asarray(q_old, dtype=np.float64))
+    out = np.empty((N + 1) ** 2)
+    Po = None if P_old is None else _f(np.asarray(P_old, dtype=np.float64).ravel())
+    Pn = None if P_new is None else _f(np.asarray(P_new, dtype=np.float64).ravel())
+    lib().o_kat_interpolate(tables.h, No, _p(Po), N, _p(Pn), _p(q), _p(out))
+    return out
+
+
+def kat_inverse_quad_map(P, x, y):
+    """inverse_quad_map: reference coordinates of (x, y) in the quad P (4, 2)."""
+    PP = _f(np.asarray(P, dtype=np.float64).ravel())
+    out = np.empty(2)
+    lib().o_kat_inverse_quad_map(_p(PP), float(x), float(y), _p(out))
+    return out
