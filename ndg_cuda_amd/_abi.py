@@ -42,7 +42,8 @@ EXPORTED = ["ndg_last_error", "ndg_device_count", "ndg_tables_create", "ndg_tabl
             "ndg_estimate_error", "ndg_mesh_set_graph", "ndg_mesh_autotune", "ndg_mesh_tuning",
             "ndg_mesh_set_wave_speed", "ndg_record_receivers", "ndg_adjoint_set_store_budget", "ndg_adjoint_set_c_derivative", "ndg_adjoint_gradient", "ndg_apply_rhs_transpose",
             "ndg_adapt", "ndg_mesh_set_adaptivity", "ndg_mesh_orders", "ndg_hilbert_child_order",
-            "ndg_hilbert_element_status", "ndg_record_receivers_noalloc"]
+            "ndg_hilbert_element_status", "ndg_record_receivers_noalloc", "ndg_mesh_set_adapt_mode",
+            "ndg_mesh_sides", "ndg_inverse_quad_map"]
 
 
 class NDGError(RuntimeError):
