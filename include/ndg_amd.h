@@ -364,8 +364,11 @@ ndg_status ndg_record_receivers_noalloc(ndg_mesh *mesh, const ndg_receivers *rec
  * (frozen) from the mesh's current state, and its gradient with respect to the initial fields
  * (packed like ndg_get_state) and to c, by the discrete adjoint of the RK3 scheme: a forward pass
  * keeping a checkpoint every checkpoint_interval steps, then per segment a recompute from its
- * checkpoint that keeps the stage states, and the transposed stages backwards. The mesh state is
- * the initial one again on return. data may be NULL (zero); any output may be NULL. */
+ * checkpoint that keeps the stage states (on a second stream, beside the reverse sweep of the
+ * following segment), and the transposed stages backwards. The mesh state is the initial one
+ * again on return. data may be NULL (zero); any output may be NULL. The checkpoints and
+ * stage-store blocks stay allocated with the mesh for the next call (grown when a call needs
+ * more, released by ndg_mesh_destroy). */
 ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *rec, double dt, int64_t n_steps,
                                 int64_t checkpoint_interval, const double *data, double *J, double *dJ_dp0,
                                 double *dJ_du0, double *dJ_dv0, double *dJ_dc);

@@ -112,6 +112,8 @@ struct Arena {
     size_t cap = 0;
 };
 hipError_t arena_fit(Arena &a, size_t bytes);
+// The same without headroom (for very large blocks sized by the caller).
+hipError_t arena_fit_exact(Arena &a, size_t bytes);
 void arena_free(Arena &a);
 
 // The buffers of one device mesh. A mesh keeps two: the current one, and the one an adaptation

@@ -58,6 +58,18 @@ hipError_t ndg::arena_fit(Arena &a, size_t bytes) {
     return hipSuccess;
 }
 
+hipError_t ndg::arena_fit_exact(Arena &a, size_t bytes) {
+    if (bytes <= a.cap && a.p) return hipSuccess;
+    arena_free(a);
+    hipError_t e = hipMalloc(&a.p, std::max<size_t>(bytes, 256));
+    if (e != hipSuccess) {
+        a.p = nullptr;
+        return e;
+    }
+    a.cap = std::max<size_t>(bytes, 256);
+    return hipSuccess;
+}
+
 void ndg::arena_free(Arena &a) {
     if (a.p) (void)hipFree(a.p);
     a.p = nullptr;
@@ -173,6 +185,9 @@ struct ndg_mesh {
     AdaptWork work;
     int64_t *mortar_key_dev = nullptr;
     int64_t adjoint_store_budget = 0;        // bytes of stage states kept beyond one segment
+    // The adjoint's checkpoints and stage-store blocks, kept for the next gradient call (grown when
+    // a call needs more, released with the mesh).
+    Arena adj_ck, adj_stage;
     int adjoint_c_mode = NDG_ADJOINT_C_RECOMPUTE;
 };
 
@@ -395,6 +410,8 @@ void free_mesh(ndg_mesh *m) {
                      &m->work.counts})
         arena_free(*a);
     if (m->mortar_key_dev) (void)hipFree(m->mortar_key_dev);
+    arena_free(m->adj_ck);
+    arena_free(m->adj_stage);
     for (int n = 0; n <= kMaxOrder; ++n) {
         if (m->tab64[n]) (void)hipFree(m->tab64[n]);
         if (m->tab32[n]) (void)hipFree(m->tab32[n]);
@@ -2693,15 +2710,14 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     // segment k - 1 (on a second stream) runs beside the reverse sweep of segment k: the kept
     // blocks of segments first_kept and first_kept + 1, or one extra block when only one is kept.
     const bool overlap = !records && first_kept > 0;
-    DevBuf ck, lam, mu, chi0, chi1, cpart, recs, res, recs_g, extra;
-    std::vector<DevBuf> store((size_t)n_kept);
-    if ((st = dev_alloc(ck, qb * n_ck, "hipMalloc checkpoints")) != NDG_OK) return st;
-    for (DevBuf &b : store)
-        if ((st = dev_alloc(b, (size_t)seg_bytes, "hipMalloc stage store")) != NDG_OK) return st;
-    if (overlap && n_kept < 2 && (st = dev_alloc(extra, (size_t)seg_bytes, "hipMalloc stage store")) != NDG_OK)
-        return st;
-    double *const rbuf[2] = {n_kept >= 1 ? store[0].as<double>() : nullptr,
-                             n_kept >= 2 ? store[1].as<double>() : extra.as<double>()};
+    DevBuf lam, mu, chi0, chi1, cpart, recs, res, recs_g;
+    // checkpoints and stage-store blocks in the mesh's arenas (reused by the next call)
+    const int64_t n_blocks = n_kept + (overlap && n_kept < 2 ? 1 : 0);
+    HIP_TRY(arena_fit_exact(mesh->adj_ck, qb * (size_t)n_ck), "hipMalloc checkpoints");
+    if (n_blocks > 0) HIP_TRY(arena_fit_exact(mesh->adj_stage, (size_t)seg_bytes * (size_t)n_blocks), "hipMalloc stage store");
+    double *const ck_base = (double *)mesh->adj_ck.p;
+    auto block = [&](int64_t b) { return (double *)mesh->adj_stage.p + b * (seg_bytes / (int64_t)sizeof(double)); };
+    double *const rbuf[2] = {n_kept >= 1 ? block(0) : nullptr, n_kept >= 2 ? block(1) : (n_blocks > 1 ? block(1) : nullptr)};
     if ((st = dev_alloc(lam, qb, "hipMalloc adjoint")) != NDG_OK) return st;
     if ((st = dev_alloc(mu, qb, "hipMalloc adjoint")) != NDG_OK) return st;
     if ((st = dev_alloc(chi0, sizeof(double) * d.n_tr, "hipMalloc adjoint")) != NDG_OK) return st;
@@ -2712,15 +2728,15 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     if ((st = dev_alloc(res, sizeof(double) * R.n * n_steps, "hipMalloc residuals")) != NDG_OK) return st;
     if (records && (st = dev_alloc(recs_g, sizeof(double) * R.n * 3 * n_steps, "hipMalloc records")) != NDG_OK)
         return st;
-    auto ckp = [&](int64_t k) { return ck.as<double>() + k * d.n_q; };
+    auto ckp = [&](int64_t k) { return ck_base + k * d.n_q; };
     // stage state (step j of segment k, relative to the segment start)
     auto stp = [&](int64_t k, int64_t j, int stage) {
-        double *b = k >= first_kept ? store[(size_t)(k - first_kept)].as<double>() : rbuf[(first_kept - k) % 2];
+        double *b = k >= first_kept ? block(k - first_kept) : rbuf[(first_kept - k) % 2];
         return b + (j * 3 + stage) * d.n_q;
     };
     double *const q_main = d.q;
     // On every exit path, errors included, the mesh's state pointers are its own buffers again:
-    // run_stored points them into the checkpoint and stage-store buffers, which are freed on return.
+    // run_stored points them into the checkpoint and stage-store blocks.
     struct StatePointers {
         DeviceMesh &dm;
         double *q;
