@@ -267,8 +267,11 @@ ndg_status ndg_pre_condition(ndg_mesh *mesh, double cfl, int64_t n_adaptivity_st
  *   children in Hilbert order (child order and statuses, Hilbert_splitting.cpp:23-81) carrying the
  *   parent polynomial (inverse bilinear map + interpolate_from), ghosts on split sides become two
  *   (split_boundaries, :6232-6343), the would-p-refine elements are raised by 2, and 2:1 hanging
- *   faces appear; splitting an element with a periodic side, or a small hanging element whose
- *   larger neighbour does not split, returns NDG_ERR_UNSUPPORTED and changes nothing;
+ *   faces appear, also across periodic sides (split_interfaces, :6343-6756: each end of the
+ *   period gets its own faces against the other end's trace); splitting a small hanging element
+ *   (across the interior or a period) whose larger neighbour does not split returns
+ *   NDG_ERR_UNSUPPORTED and changes nothing, as does, in the NDG_ADAPT_HOST mode, splitting an
+ *   element with a periodic side;
  * - otherwise the p_adapt branch (:5026-5049): every element with refine && sigma >= 1 &&
  *   N + 2 <= N_max is raised by 2 (Element2D_t.cpp:464-579); ghosts and faces follow
  *   (adjust_boundaries / adjust_interfaces / adjust_faces, :7482-7538), including the reference's

@@ -276,9 +276,9 @@ __global__ void __launch_bounds__(kThreads) decide_kernel(LogicalMesh L, const d
     if (bad) atomicOr(&c->refused, (unsigned long long)kRefuseOrder);
 }
 
-// Refusals of the split branch (a split element with a periodic side, or a small hanging element
-// that splits while its larger neighbour does not: four faces on one side) and the ghosts that
-// split with their element.
+// Refusals of the split branch (a small hanging element, across an interior or a periodic side,
+// that splits while its larger neighbour does not: more than two faces on one side) and the
+// ghosts that split with their element.
 __global__ void __launch_bounds__(kThreads) check_kernel(LogicalMesh L, const int8_t *__restrict__ flags,
                                                           AdaptCounts *c) {
     const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -286,7 +286,6 @@ __global__ void __launch_bounds__(kThreads) check_kernel(LogicalMesh L, const in
     if (i < L.K && flags[i] == 2)
         for (int s = 0; s < 4; ++s) {
             const int64_t es = 4 * i + s;
-            if (L.periodic[es]) refused |= kRefusePeriodic;
             if (L.hang[es] == 1 && flags[L.link[es] / 4] != 2) refused |= kRefuseFourFaces;
         }
     if (i < L.n_ghost && flags[L.ghost_side[i] / 4] == 2) gs = 1;
@@ -304,11 +303,12 @@ __global__ void count_kernel(const int8_t *__restrict__ flags, int64_t K, int64_
     if (i < n_ghost) newghost[i] = flags[ghost_side[i] / 4] == 2 ? 2 : 1;
 }
 
-// The middle of side s of old element e: the middle node of a larger hanging side (the start of
-// its first small side), else the midpoint of the side's corners.
+// The middle of side s of old element e: the middle node of a larger interior hanging side (the
+// start of its first small side), else the midpoint of the side's corners (the small sides of a
+// larger periodic side lie at the other end of the period).
 __device__ v2 side_middle(const LogicalMesh &L, int64_t e, int s) {
     const int64_t es = 4 * e + s;
-    if (L.hang[es] == 2) {
+    if (L.hang[es] == 2 && !L.periodic[es]) {
         const int64_t f = L.link[es];
         return corner(L, f / 4, (int)(f % 4));
     }
@@ -387,14 +387,14 @@ __global__ void __launch_bounds__(kThreads) links_kernel(LogicalMesh L, LogicalM
     const int s = (int)(t % 4), c = corner_of[k];
     int64_t link = 0, hang2 = -1;
     int hang = 0, per = 0;
+    // A periodic side follows the interior rules: through the period the partner's side runs
+    // the other way, as an interior neighbour's does.
     if (c < 0) {  // unsplit element: side s of e
         const int64_t es = 4 * e + s, l = L.link[es];
         const int hk = L.hang[es];
+        per = L.periodic[es];
         if (l < 0) {
             link = -(1 + newghost[-l - 1]);
-        } else if (L.periodic[es]) {  // the donor does not split (a split periodic side is refused)
-            link = M.same(l / 4, (int)(l % 4));
-            per = 1;
         } else if (hk == 0) {
             const int64_t r = l / 4;
             const int rs = (int)(l % 4);
@@ -434,6 +434,7 @@ __global__ void __launch_bounds__(kThreads) links_kernel(LogicalMesh L, LogicalM
             const int ps = rel == 0 ? c : (c + 3) & 3, h = rel == 0 ? 0 : 1;
             const int64_t es = 4 * e + ps, l = L.link[es];
             const int hk = L.hang[es];
+            per = L.periodic[es];
             if (l < 0) {  // the ghost splits: ghost g + 1 covers the first half, g the second
                 link = -(1 + newghost[-l - 1] + (h == 0 ? 1 : 0));
             } else if (hk == 0) {
@@ -571,38 +572,47 @@ __global__ void __launch_bounds__(kThreads) order_place_kernel(const int32_t *__
 }
 
 // A side is a mortar side (its flux comes from the mortar kernel) when it is hanging or its
-// exterior has another order; it owns a mortar face when it is a small hanging side, or a
-// non-conforming side whose face belongs to it (periodic, or the lower-numbered element).
-__device__ void side_roles(const LogicalMesh &L, int64_t t, bool &mortar, bool &owner) {
+// exterior has another order. The mortar faces a side owns: an interior small hanging side owns
+// the face it shares with the larger side, an interior side of full length the face of a pair of
+// different orders when it is the lower-numbered element's; a periodic mortar side owns its own
+// faces against its partner's trace, one per face on the side (two on a larger side), as the
+// reference's periodic ghosts give each end of the period its own faces.
+__device__ void side_roles(const LogicalMesh &L, int64_t t, bool &mortar, int &owned) {
     const int64_t k = t / 4, l = L.link[t];
     const int hk = L.hang[t];
     const bool other = l >= 0 && L.order[l / 4] != L.order[k];
     mortar = hk != 0 || other;
-    owner = hk == 1 || (hk == 0 && other && (L.periodic[t] || l / 4 > k));
+    if (L.periodic[t]) owned = mortar ? (hk == 2 ? 2 : 1) : 0;
+    else owned = hk == 1 || (hk == 0 && other && l / 4 > k);
 }
 
 __global__ void __launch_bounds__(kThreads) side_count_kernel(LogicalMesh L, int64_t *__restrict__ cnt_rec,
                                                                int64_t *__restrict__ cnt_slot,
                                                                int64_t *__restrict__ cnt_face, AdaptCounts *c) {
     const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    unsigned long long pairs = 0, larger = 0;
+    unsigned long long pairs = 0, larger = 0, pfaces = 0;
     if (t < 4 * L.K) {
-        bool mortar, owner;
-        side_roles(L, t, mortar, owner);
+        bool mortar;
+        int owned;
+        side_roles(L, t, mortar, owned);
         const int64_t k = t / 4, l = L.link[t];
         const int N = L.order[k], hk = L.hang[t];
+        const bool per = L.periodic[t] != 0;
         cnt_rec[t] = mortar;
         cnt_slot[t] = mortar ? 3 * (int64_t)(N + 1) * (hk == 2 ? 2 : 1) : 0;
-        cnt_face[t] = owner;
-        pairs = l >= 0 && hk == 0 && !L.periodic[t] && l > t;
-        larger = hk == 2;
-        if (owner) {
-            const int Nr = L.order[l / 4];
+        cnt_face[t] = owned;
+        pairs = l >= 0 && hk == 0 && !per && l > t;
+        larger = hk == 2 && !per;
+        pfaces = per ? (hk == 2 ? 2 : 1) : 0;
+        if (owned) {
+            int Nr = L.order[l / 4];
+            if (hk == 2) Nr = max(Nr, (int)L.order[L.hang2[t] / 4]);
             atomicMax(&c->mortar_nodes, (unsigned long long)(max(N, Nr) + 1));
         }
     }
     wave_add(&c->conforming_pairs, pairs);
     wave_add(&c->larger_sides, larger);
+    wave_add(&c->periodic_faces, pfaces);
 }
 
 __global__ void totals_kernel(const int64_t *rec, const int64_t *slot, const int64_t *face, int64_t n, AdaptCounts *c) {
@@ -626,6 +636,23 @@ struct FillArgs {
     AdaptCounts *c;
 };
 
+// The matrix offsets of a mortar face's two sides from their offset/scale classes (-1: a plain
+// copy); a face whose sides fit no class is counted as bad.
+__device__ void face_matrices(MortarFace &fc, const int64_t *key_off, AdaptCounts *c) {
+    for (int side = 0; side < 2; ++side) {
+        const int Ne = side ? fc.NR : fc.NL;
+        const int cls = mortar_class(fc.offset[side], fc.scale[side]);
+        if (cls < 0) {
+            atomicAdd(&c->bad_face, 1ull);
+            fc.mat[side] = fc.mat[2 + side] = -1;
+            continue;
+        }
+        const bool copy = Ne == fc.Nf && cls == 0;
+        fc.mat[side] = copy ? -1 : key_off[mortar_matrix_key(Ne, fc.Nf, side, cls, 0)];
+        fc.mat[2 + side] = copy ? -1 : key_off[mortar_matrix_key(Ne, fc.Nf, side, cls, 1)];
+    }
+}
+
 __global__ void __launch_bounds__(kThreads) side_fill_kernel(LogicalMesh L, FillArgs A) {
     const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (t >= 4 * L.K) return;
@@ -636,8 +663,9 @@ __global__ void __launch_bounds__(kThreads) side_fill_kernel(LogicalMesh L, Fill
     const int64_t local = dv - G.e_begin;
     if (s == 0)
         for (int i = 0; i < 8; ++i) A.verts[8 * dv + i] = L.verts[8 * k + i];
-    bool mortar, owner;
-    side_roles(L, t, mortar, owner);
+    bool mortar;
+    int owned;
+    side_roles(L, t, mortar, owned);
     int64_t code;
     if (mortar) {
         code = -(1 + A.n_ghost + A.rec_idx[t]);
@@ -653,47 +681,65 @@ __global__ void __launch_bounds__(kThreads) side_fill_kernel(LogicalMesh L, Fill
         code = (L.dev_of[r] - A.spans[L.order[r]].e_begin) * 4 + l % 4;
     }
     A.side_nb[4 * dv + s] = (int32_t)code;
-    if (!owner) return;
+    if (!owned) return;
+    auto trace_of = [&](int64_t side) {
+        const int64_t e = side / 4;
+        const int n = L.order[e] + 1;
+        const GroupSpan &S = A.spans[n - 1];
+        return S.tr_off + (L.dev_of[e] - S.e_begin) * 12 * n + (side % 4) * 3 * n;
+    };
+    const v2 c0 = corner(L, k, s), c1 = corner(L, k, s + 1);
+    const v2 PL[4] = {corner(L, k, 0), corner(L, k, 1), corner(L, k, 2), corner(L, k, 3)};
+    const double sfL = side_scale(PL, s);
+    if (L.periodic[t]) {
+        // One face per face of this side, against a ghost holding the partner side's trace (in
+        // the partner's direction, which runs against this side through the period): a whole side
+        // against a whole partner or against a larger partner's half (first half when this side
+        // is the partner's first small side), or half of a larger side against a whole small
+        // partner.
+        for (int h = 0; h < owned; ++h) {
+            const int64_t lr = hk == 2 && h == 1 ? L.hang2[t] : l;
+            const int NR = L.order[lr / 4];
+            MortarFace fc{};
+            fc.NL = N;
+            fc.NR = NR;
+            fc.Nf = max(N, NR);
+            fc.trL = G.tr_off + local * 12 * nf + s * 3 * nf;
+            fc.trR = trace_of(lr);
+            const v2 md = midpoint(c0, c1);
+            const v2 f0 = hk == 2 && h == 1 ? md : c0, f1 = hk == 2 && h == 0 ? md : c1;
+            double tx, ty;
+            face_frame(f0, f1, f0, f1, fc.nx, fc.ny, tx, ty);
+            fc.offset[0] = hk == 2 ? 0.5 * h : 0.0;
+            fc.scale[0] = hk == 2 ? 0.5 : 1.0;
+            fc.offset[1] = hk == 1 && L.link[l] != t ? 0.5 : 0.0;
+            fc.scale[1] = hk == 1 ? 0.5 : 1.0;
+            fc.sfL = sfL;
+            fc.slotL = A.slot_off[t] + h * 3 * nf;
+            fc.slotR = -1;
+            face_matrices(fc, A.key_off, A.c);
+            A.faces[A.face_idx[t] + h] = fc;
+        }
+        return;
+    }
     const int64_t r = l / 4;
     const int rs = (int)(l % 4), NR = L.order[r], nfr = NR + 1;
-    const bool per = L.periodic[t] != 0;
-    const GroupSpan &GR = A.spans[NR];
     MortarFace fc{};
     fc.NL = N;
     fc.NR = NR;
     fc.Nf = max(N, NR);
     fc.trL = G.tr_off + local * 12 * nf + s * 3 * nf;
-    fc.trR = GR.tr_off + (L.dev_of[r] - GR.e_begin) * 12 * nfr + rs * 3 * nfr;
-    const v2 f0 = corner(L, k, s), f1 = corner(L, k, s + 1);
+    fc.trR = trace_of(l);
     double tx, ty;
-    face_frame(f0, f1, f0, f1, fc.nx, fc.ny, tx, ty);
-    v2 en[4] = {f0, f1, f1, f0};
-    if (!per) {
-        en[2] = corner(L, r, rs);
-        en[3] = corner(L, r, rs + 1);
-    }
-    face_offsets(f0, f1, en, fc.offset, fc.scale);
-    const v2 PL[4] = {corner(L, k, 0), corner(L, k, 1), corner(L, k, 2), corner(L, k, 3)};
-    fc.sfL = side_scale(PL, s);
+    face_frame(c0, c1, c0, c1, fc.nx, fc.ny, tx, ty);
+    const v2 en[4] = {c0, c1, corner(L, r, rs), corner(L, r, rs + 1)};
+    face_offsets(c0, c1, en, fc.offset, fc.scale);
+    fc.sfL = sfL;
     fc.slotL = A.slot_off[t];
-    fc.slotR = -1;
-    if (hk == 1 || !per) {
-        const v2 PR[4] = {corner(L, r, 0), corner(L, r, 1), corner(L, r, 2), corner(L, r, 3)};
-        fc.sfR = side_scale(PR, rs);
-        fc.slotR = hk == 1 ? (L.link[l] == t ? A.slot_off[l] : A.slot_off[l] + 3 * nfr) : A.slot_off[l];
-    }
-    for (int side = 0; side < 2; ++side) {
-        const int Ne = side ? NR : N;
-        const int cls = mortar_class(fc.offset[side], fc.scale[side]);
-        if (cls < 0) {
-            atomicAdd(&A.c->bad_face, 1ull);
-            fc.mat[side] = fc.mat[2 + side] = -1;
-            continue;
-        }
-        const bool copy = Ne == fc.Nf && cls == 0;
-        fc.mat[side] = copy ? -1 : A.key_off[mortar_matrix_key(Ne, fc.Nf, side, cls, 0)];
-        fc.mat[2 + side] = copy ? -1 : A.key_off[mortar_matrix_key(Ne, fc.Nf, side, cls, 1)];
-    }
+    const v2 PR[4] = {corner(L, r, 0), corner(L, r, 1), corner(L, r, 2), corner(L, r, 3)};
+    fc.sfR = side_scale(PR, rs);
+    fc.slotR = hk == 1 ? (L.link[l] == t ? A.slot_off[l] : A.slot_off[l] + 3 * nfr) : A.slot_off[l];
+    face_matrices(fc, A.key_off, A.c);
     A.faces[A.face_idx[t]] = fc;
 }
 

@@ -234,9 +234,10 @@ struct AdaptCounts {
     unsigned long long order_count[kMaxOrder + 1];
     unsigned long long n_rec, n_slot, n_face;    // mortar records, side-flux values, mortar faces
     unsigned long long conforming_pairs, larger_sides, mortar_nodes, bad_face;
+    unsigned long long periodic_faces;           // faces of periodic sides (two on a larger side)
 };
 // refused bits
-constexpr unsigned kRefuseOrder = 1, kRefusePeriodic = 2, kRefuseFourFaces = 4;
+constexpr unsigned kRefuseOrder = 1, kRefuseFourFaces = 4;
 
 // Scratch of the adaptation kernels (capacities kept across adaptations).
 struct AdaptWork {

@@ -181,7 +181,6 @@ struct ndg_mesh {
     bool host_stale = false;
     bool host_nodes_valid = true;
     int adapt_mode = NDG_ADAPT_DEVICE;
-    int64_t n_periodic_sides = 0;
     AdaptWork work;
     int64_t *mortar_key_dev = nullptr;
     int64_t adjoint_store_budget = 0;        // bytes of stage states kept beyond one segment
@@ -591,6 +590,8 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
         for (int s = 0; s < 4; ++s) {
             const int64_t es = 4 * e + s, l = link[es];
             const int hk = m->hang_kind[es];
+            if (hk != 0 && periodic_link[es])  // only the device adaptation makes these
+                return fail(NDG_ERR_UNSUPPORTED, "mesh build: hanging periodic sides are built by the device path");
             if (hk == 2) continue;  // built from its two halves
             if (l < 0 || (hk == 0 && order[l / 4] == order[e])) continue;
             const int64_t r = l / 4;
@@ -761,8 +762,6 @@ ndg_status ensure_logical(ndg_mesh *m) {
     if ((st = upload_into(L.ghost, m->bc_base, "upload logical mesh")) != NDG_OK) return st;
     if ((st = upload_into(L.ghost_side, ghost_side, "upload logical mesh")) != NDG_OK) return st;
     if ((st = upload_into(L.ghost_zero, ghost_zero, "upload logical mesh")) != NDG_OK) return st;
-    m->n_periodic_sides = 0;
-    for (char c : m->periodic_link) m->n_periodic_sides += c != 0;
     m->logical_valid = true;
     return NDG_OK;
 }
@@ -1744,8 +1743,9 @@ v2 reference_point(v2 g, const v2 P[4]) {
 // the child status and the parent's rotation; would_p_refine elements are raised by 2 orders; a
 // boundary ghost on a split side becomes two ghosts (fresh geometry), a ghost whose element is
 // raised gets fresh geometry. The faces are rebuilt from the new elements and ghosts. Elements
-// with a periodic side, or whose split would cover a side of an unsplit neighbour with more than
-// two faces, are refused (NDG_ERR_UNSUPPORTED, nothing changed).
+// with a periodic side (split across the period by the device path only), or whose split would
+// cover a side of an unsplit neighbour with more than two faces, are refused (NDG_ERR_UNSUPPORTED,
+// nothing changed).
 // The last estimate of the elements an adaptation keeps (new element k is old element from[k],
 // unchanged when keep[k]); created and raised elements start from zero, as the reference's new
 // Element2D_t objects do (src/meshes/Mesh2D_t.cu:6206-6232).
@@ -2015,8 +2015,6 @@ ndg_status adapt_on_device(ndg_mesh *mesh, int32_t N_max, int32_t max_split_leve
     mesh->host_stale = true;  // the logical copy of the estimate changed
     if (c.refused & kRefuseOrder)
         return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a raised order exceeds the compiled range or the tables");
-    if (c.n_split > 0 && (c.refused & kRefusePeriodic))
-        return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: splitting an element with a periodic side is not provided");
     if (c.n_split > 0 && (c.refused & kRefuseFourFaces))
         return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a split would leave more than two faces on a side");
     if (n_changed) *n_changed = (int64_t)(c.n_split + c.n_raise);
@@ -2112,7 +2110,7 @@ ndg_status adapt_on_device(ndg_mesh *mesh, int32_t N_max, int32_t max_split_leve
     d = nd;
     mesh->active ^= 1;
     mesh->lactive = nl;
-    mesh->n_faces = (int64_t)c.conforming_pairs + Gn + mesh->n_periodic_sides + 2 * (int64_t)c.larger_sides;
+    mesh->n_faces = (int64_t)c.conforming_pairs + Gn + (int64_t)c.periodic_faces + 2 * (int64_t)c.larger_sides;
     mesh->n_dofs = (int64_t)c.dofs;
     mesh->host_stale = true;
     mesh->host_nodes_valid = false;

@@ -345,6 +345,7 @@ struct o_mesh {
     int initial_N;
     int32_t *bar_type;  /* per ghost (bar order): kind and periodic donor bar */
     int64_t *bar_donor;
+    int64_t *bar_origin; /* per periodic ghost: the element and side whose trace it copies */
 };
 
 static void elem_alloc_domain(o_elem *e) {
@@ -536,10 +537,14 @@ static void element_status(o_mesh *m) {
     }
 }
 
-o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const double *xy,
-                      int64_t n_quads, const int64_t *quads, const int32_t *quad_N, int64_t n_bars,
-                      const int64_t *bars, const int32_t *bar_type, const int64_t *bar_donor,
-                      double tol_min, double tol_max) {
+/* Mesh creation from nodes, quads and boundary bars. A periodic bar names its donor bar (the bar
+ * on the other end of the period, at mesh creation) or, when bar_origin is given, the element and
+ * side whose trace it copies (after a split, where one whole bar faces two half bars). A periodic
+ * ghost may be half of its element's side or cover two elements' sides (2:1 across the period). */
+static o_mesh *mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const double *xy, int64_t n_quads,
+                           const int64_t *quads, const int32_t *quad_N, int64_t n_bars, const int64_t *bars,
+                           const int32_t *bar_type, const int64_t *bar_donor, const int64_t *bar_origin,
+                           double tol_min, double tol_max) {
     o_mesh *m = (o_mesh *)calloc(1, sizeof(o_mesh));
     m->t = t;
     m->tol_min = tol_min;
@@ -548,9 +553,12 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
     m->initial_N = initial_N;
     m->bar_type = (int32_t *)malloc((size_t)(n_bars + 1) * sizeof(int32_t));
     m->bar_donor = (int64_t *)malloc((size_t)(n_bars + 1) * sizeof(int64_t));
+    m->bar_origin = (int64_t *)malloc((size_t)(2 * n_bars + 2) * sizeof(int64_t));
     for (int64_t b = 0; b < n_bars; ++b) {
         m->bar_type[b] = bar_type[b];
         m->bar_donor[b] = bar_donor ? bar_donor[b] : -1;
+        m->bar_origin[2 * b] = bar_origin ? bar_origin[2 * b] : -1;
+        m->bar_origin[2 * b + 1] = bar_origin ? bar_origin[2 * b + 1] : -1;
     }
     m->n_nodes = n_nodes;
     m->xy = (v2 *)malloc((size_t)n_nodes * sizeof(v2));
@@ -617,6 +625,7 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
         e2f[i][0] = -1;
         for (int k = 0; k < n2f_count[a]; ++k) {
             o_face *f = &m->fa[n2f[a][k]];
+            if (f->elements[0] >= n_quads) continue;
             if (f->nodes[0] == b && f->nodes[1] == a) {
                 f->elements[1] = i; f->sides[1] = 0; e2f[i][0] = n2f[a][k];
                 break;
@@ -626,6 +635,22 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
                 e->nodes[0] = b; e->nodes[1] = a; /* "inside out" boundary element */
                 break;
             }
+        }
+        if (e2f[i][0] < 0 && bar_type[i - n_quads] == O_BC_PERIODIC) {
+            /* half of an element side, or two element sides: a face of its own, matched below */
+            if (m->F == cap) {
+                cap *= 2;
+                m->fa = (o_face *)realloc(m->fa, (size_t)cap * sizeof(o_face));
+            }
+            e2f[i][0] = m->F;
+            face_add_to_node(n2f, n2f_count, n2f_cap, a, m->F);
+            face_add_to_node(n2f, n2f_count, n2f_cap, b, m->F);
+            o_face *f = &m->fa[m->F++];
+            memset(f, 0, sizeof(*f));
+            f->N = initial_N;
+            f->nodes[0] = a; f->nodes[1] = b;
+            f->elements[0] = i; f->elements[1] = -1;
+            f->sides[0] = 0; f->sides[1] = -1;
         }
     }
     /* Hanging (2:1) sides: a domain side a -> b without a neighbour or a ghost, covered by the
@@ -662,6 +687,39 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
                 break;
             }
         }
+    /* A periodic ghost a -> b covering the sides b -> m and m -> a of two elements (the ghost of a
+     * larger side across the period whose partner split): those faces take the ghost as their
+     * right element, the one ending at a first, and the ghost's own face is dropped. */
+    int64_t (*ghang)[2] = (int64_t(*)[2])malloc((size_t)(n_bars + 1) * sizeof(int64_t[2]));
+    for (int64_t b = 0; b < n_bars; ++b) {
+        ghang[b][0] = ghang[b][1] = -1;
+        const int64_t i = n_quads + b, fb = e2f[i][0];
+        if (bar_type[b] != O_BC_PERIODIC || fb < 0 || m->fa[fb].elements[0] != i || m->fa[fb].elements[1] != -1)
+            continue;
+        const int64_t a = m->fa[fb].nodes[0], bb = m->fa[fb].nodes[1];
+        for (int k = 0; k < n2f_count[bb]; ++k) {
+            const int64_t f1 = n2f[bb][k];
+            const o_face *c1 = &m->fa[f1];
+            if (c1->nodes[0] != bb || c1->elements[1] != -1 || c1->elements[0] >= n_quads) continue;
+            const int64_t mid = c1->nodes[1];
+            int64_t f2 = -1;
+            for (int q = 0; q < n2f_count[mid]; ++q) {
+                const o_face *c = &m->fa[n2f[mid][q]];
+                if (c->nodes[0] == mid && c->nodes[1] == a && c->elements[1] == -1 && c->elements[0] < n_quads)
+                    f2 = n2f[mid][q];
+            }
+            if (f2 < 0) continue;
+            const double mx = (m->xy[a].x + m->xy[bb].x) / 2, my = (m->xy[a].y + m->xy[bb].y) / 2;
+            const double len = mag(sub(m->xy[bb], m->xy[a]));
+            if (fabs(m->xy[mid].x - mx) > 1e-12 * len || fabs(m->xy[mid].y - my) > 1e-12 * len) continue;
+            m->fa[f1].elements[1] = i; m->fa[f1].sides[1] = 0;
+            m->fa[f2].elements[1] = i; m->fa[f2].sides[1] = 0;
+            ghang[b][0] = f2;
+            ghang[b][1] = f1;
+            dropped[fb] = 1;
+            break;
+        }
+    }
     {   /* compact the face array */
         int64_t *remap = (int64_t *)malloc((size_t)(m->F + 1) * sizeof(int64_t));
         int64_t nf = 0;
@@ -675,6 +733,8 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
                 if (e2f[i][j] >= 0) e2f[i][j] = remap[e2f[i][j]];
         for (int64_t es = 0; es < 4 * n_quads; ++es)
             if (hang[es][0] >= 0) { hang[es][0] = remap[hang[es][0]]; hang[es][1] = remap[hang[es][1]]; }
+        for (int64_t b = 0; b < n_bars; ++b)
+            if (ghang[b][0] >= 0) { ghang[b][0] = remap[ghang[b][0]]; ghang[b][1] = remap[ghang[b][1]]; }
         free(remap);
     }
     free(dropped);
@@ -690,6 +750,14 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
             case O_BC_INFLOW: push(&m->in, &m->n_in, g); break;
             case O_BC_OUTFLOW: push(&m->out, &m->n_out, g); break;
             case O_BC_PERIODIC: {
+                if (bar_origin) {
+                    int64_t tmp = m->n_if;
+                    push(&m->if_origin, &tmp, bar_origin[2 * b]);
+                    tmp = m->n_if;
+                    push(&m->if_side, &tmp, bar_origin[2 * b + 1]);
+                    push(&m->if_dest, &m->n_if, g);
+                    break;
+                }
                 const int64_t donor_ghost = n_quads + bar_donor[b];
                 const int64_t fi = e2f[donor_ghost][0];
                 const int side_index = m->fa[fi].elements[0] == donor_ghost;
@@ -746,9 +814,17 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
             element_geometry(e, P, t->tab[O_NODES][e->N]);
         } else {
             elem_alloc_ghost(e);
-            e->nf[0] = 1;
-            e->faces[0] = (int64_t *)malloc(sizeof(int64_t));
-            e->faces[0][0] = e2f[i][0];
+            const int64_t b = i - n_quads;
+            if (ghang[b][0] >= 0) {
+                e->nf[0] = 2;
+                e->faces[0] = (int64_t *)malloc(2 * sizeof(int64_t));
+                e->faces[0][0] = ghang[b][0];
+                e->faces[0][1] = ghang[b][1];
+            } else {
+                e->nf[0] = 1;
+                e->faces[0] = (int64_t *)malloc(sizeof(int64_t));
+                e->faces[0][0] = e2f[i][0];
+            }
             v2 P[4];
             for (int k = 0; k < 4; ++k) P[k] = m->xy[e->nodes[k]];
             boundary_geometry(e, P, t->tab[O_NODES][e->N]);
@@ -768,7 +844,20 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const d
     }
     free(e2f);
     free(hang);
+    free(ghang);
+    for (int64_t k = 0; k < m->n_if; ++k) {
+        m->bar_origin[2 * (m->if_dest[k] - n_quads)] = m->if_origin[k];
+        m->bar_origin[2 * (m->if_dest[k] - n_quads) + 1] = m->if_side[k];
+    }
     return m;
+}
+
+o_mesh *o_mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, const double *xy,
+                      int64_t n_quads, const int64_t *quads, const int32_t *quad_N, int64_t n_bars,
+                      const int64_t *bars, const int32_t *bar_type, const int64_t *bar_donor,
+                      double tol_min, double tol_max) {
+    return mesh_create(t, initial_N, n_nodes, xy, n_quads, quads, quad_N, n_bars, bars, bar_type, bar_donor, NULL,
+                       tol_min, tol_max);
 }
 
 void o_mesh_destroy(o_mesh *m) {
@@ -781,7 +870,7 @@ void o_mesh_destroy(o_mesh *m) {
     free(m->el); free(m->fa); free(m->xy);
     free(m->wall); free(m->sym); free(m->in); free(m->out);
     free(m->if_origin); free(m->if_side); free(m->if_dest);
-    free(m->bar_type); free(m->bar_donor);
+    free(m->bar_type); free(m->bar_donor); free(m->bar_origin);
     free(m);
 }
 
@@ -1714,16 +1803,17 @@ static int64_t edge_mid_find(const int64_t *keys, int64_t n, int64_t a, int64_t 
 }
 
 /* The splitting branch of Mesh2D_t::adapt (src/meshes/Mesh2D_t.cpp:1993-2268, hp_adapt
- * :5186-5659, split_boundaries :6232-6343): elements with would_h_refine become four children
- * (corner s: {corner s, midpoint of side s, centre, midpoint of side s-1} in local positions s,
- * s+1, s+2, s+3), placed at position child_order[s] of the Hilbert child order, with the parent's
- * order, split level + 1, child status and the parent's rotation, state interpolated from the
- * parent; elements with would_p_refine are raised by 2 orders; a boundary ghost on a split side
- * becomes two ghosts (geometry recomputed), a ghost whose order changes is resized and its
- * geometry recomputed. The new topology (faces, 2:1 hanging sides) is built from the new element
- * and ghost lists as at mesh creation. Splitting an element with a periodic side, or one that
- * would leave a side covered by more than two faces, is not restated (returns -2, nothing
- * changed). Returns the number of split plus raised elements. */
+ * :5186-5659, split_boundaries :6232-6343, split_interfaces :6343-6756): elements with
+ * would_h_refine become four children (corner s: {corner s, midpoint of side s, centre, midpoint
+ * of side s-1} in local positions s, s+1, s+2, s+3), placed at position child_order[s] of the
+ * Hilbert child order, with the parent's order, split level + 1, child status and the parent's
+ * rotation, state interpolated from the parent; elements with would_p_refine are raised by 2
+ * orders; a boundary ghost on a split side becomes two ghosts (geometry recomputed), a ghost whose
+ * order changes is resized and its geometry recomputed; a periodic ghost splits with its origin
+ * (the element across the period). The new topology (faces, 2:1 hanging sides, also across the
+ * period) is built from the new element and ghost lists as at mesh creation. A split that would
+ * leave a side covered by more than two faces is not restated (returns -2, nothing changed).
+ * Returns the number of split plus raised elements. */
 static int64_t adapt_split(o_mesh *m, int N_max, int max_split_level) {
     const int64_t K = m->K, NB = m->n_elements_total - m->K;
     char *split = (char *)calloc((size_t)K + 1, 1);
@@ -1734,33 +1824,37 @@ static int64_t adapt_split(o_mesh *m, int N_max, int max_split_level) {
         if (e->refine && e->sigma < 1.0 && e->split_level < max_split_level) { split[i] = 1; ++n_split; }
         else if (e->refine && e->sigma >= 1.0 && e->N + 2 <= N_max) { raise[i] = 1; ++n_raise; }
     }
-    /* limits of this restatement */
+    /* limit of this restatement: a half of a larger unsplit side, across the interior or across
+     * the period (the small element's ghost then covers two sides), would be split again */
     for (int64_t i = 0; i < K; ++i) {
         if (!split[i]) continue;
         const o_elem *e = &m->el[i];
-        for (int s = 0; s < 4; ++s)
-            for (int fk = 0; fk < e->nf[s]; ++fk) {
-                const o_face *f = &m->fa[e->faces[s][fk]];
-                const int64_t other = f->elements[0] == i ? f->elements[1] : f->elements[0];
-                if (other >= K && m->bar_type[other - K] == O_BC_PERIODIC) { free(split); free(raise); return -2; }
-                /* a half of a larger unsplit neighbour's side would be split again */
-                if (other < K && !split[other] && e->nf[s] == 1) {
-                    const o_elem *nb = &m->el[other];
-                    const int ns = (int)(f->elements[0] == i ? f->sides[1] : f->sides[0]);
-                    if (nb->nf[ns] == 2) { free(split); free(raise); return -2; }
-                }
+        for (int s = 0; s < 4; ++s) {
+            if (e->nf[s] != 1) continue;
+            const o_face *f = &m->fa[e->faces[s][0]];
+            const int64_t other = f->elements[0] == i ? f->elements[1] : f->elements[0];
+            if (other < K && !split[other]) {
+                const o_elem *nb = &m->el[other];
+                const int ns = (int)(f->elements[0] == i ? f->sides[1] : f->sides[0]);
+                if (nb->nf[ns] == 2) { free(split); free(raise); return -2; }
             }
+            if (other >= K && m->bar_type[other - K] == O_BC_PERIODIC && m->el[other].nf[0] == 2 &&
+                !split[m->bar_origin[2 * (other - K)]]) {
+                free(split); free(raise); return -2;
+            }
+        }
     }
-    /* new nodes: existing, then per split element its side midpoints (shared) and centre */
-    int64_t cap_nodes = m->n_nodes + 5 * n_split + 1;
+    /* new nodes: existing, then per split element its side midpoints (shared) and centre, then the
+     * middles of periodic ghosts that split with their origin and had none */
+    int64_t cap_nodes = m->n_nodes + 5 * n_split + NB + 1;
     double *xy = (double *)malloc((size_t)cap_nodes * 2 * sizeof(double));
     for (int64_t k = 0; k < m->n_nodes; ++k) { xy[2 * k] = m->xy[k].x; xy[2 * k + 1] = m->xy[k].y; }
     int64_t n_nodes = m->n_nodes;
-    int64_t *mids = (int64_t *)malloc((size_t)(4 * K + 4 * n_split + 1) * 3 * sizeof(int64_t));
+    int64_t *mids = (int64_t *)malloc((size_t)(4 * K + 4 * n_split + 2 * NB + 1) * 3 * sizeof(int64_t));
     int64_t n_mids = 0;
-    /* midpoints already present: hanging sides (two faces) */
-    for (int64_t i = 0; i < K; ++i)
-        for (int s = 0; s < 4; ++s)
+    /* midpoints already present: hanging sides (two faces), of elements and of periodic ghosts */
+    for (int64_t i = 0; i < m->n_elements_total; ++i)
+        for (int s = 0; s < (i < K ? 4 : 1); ++s)
             if (m->el[i].nf[s] == 2) {
                 const int64_t a = m->el[i].nodes[s], b = m->el[i].nodes[(s + 1) % 4];
                 mids[3 * n_mids] = a < b ? a : b; mids[3 * n_mids + 1] = a < b ? b : a;
@@ -1772,9 +1866,11 @@ static int64_t adapt_split(o_mesh *m, int N_max, int max_split_level) {
     int32_t *qN = (int32_t *)malloc((size_t)Knew * sizeof(int32_t));
     int64_t *parent = (int64_t *)malloc((size_t)Knew * sizeof(int64_t));
     int *child_status = (int *)malloc((size_t)Knew * sizeof(int));
+    int64_t *newpos = (int64_t *)malloc((size_t)(K + 1) * sizeof(int64_t));
     int64_t pos = 0;
     for (int64_t i = 0; i < K; ++i) {
         const o_elem *e = &m->el[i];
+        newpos[i] = pos;
         if (!split[i]) {
             for (int k = 0; k < 4; ++k) quads[4 * pos + k] = e->nodes[k];
             qN[pos] = e->N + (raise[i] ? 2 : 0);
@@ -1816,34 +1912,70 @@ static int64_t adapt_split(o_mesh *m, int N_max, int max_split_level) {
         pos += 4;
     }
     /* ghosts: split on split sides (two consecutive ghosts), else kept; resized ones and new ones
-     * get fresh geometry, the others keep a zero scaling factor an earlier resize left */
+     * get fresh geometry, the others keep a zero scaling factor an earlier resize left. A periodic
+     * ghost follows its origin instead (split_interfaces, src/meshes/Mesh2D_t.cpp:6343-6756): it
+     * splits when the origin splits, its halves copying the origin's children at the side's first
+     * and second corner, and stays whole otherwise, also when its own element splits (then it
+     * covers two element sides). */
     int64_t *bars = (int64_t *)malloc((size_t)(2 * NB + 1) * 2 * sizeof(int64_t));
     int32_t *btype = (int32_t *)malloc((size_t)(2 * NB + 1) * sizeof(int32_t));
-    int64_t *bdonor = (int64_t *)malloc((size_t)(2 * NB + 1) * sizeof(int64_t));
+    int64_t *borig = (int64_t *)malloc((size_t)(2 * NB + 1) * 2 * sizeof(int64_t));
     char *bzero = (char *)calloc((size_t)(2 * NB + 1), 1);
-    int64_t *bremap = (int64_t *)malloc((size_t)(NB + 1) * sizeof(int64_t));
     int64_t nb = 0;
     for (int64_t b = 0; b < NB; ++b) {
         const o_elem *g = &m->el[K + b];
+        if (m->bar_type[b] == O_BC_PERIODIC) {
+            const int64_t o = m->bar_origin[2 * b], so = m->bar_origin[2 * b + 1];
+            if (split[o]) {
+                int64_t md = edge_mid_find(mids, n_mids, g->nodes[0], g->nodes[1]);
+                if (md < 0) {
+                    const int64_t a = g->nodes[0], c = g->nodes[1];
+                    md = n_nodes++;
+                    xy[2 * md] = (xy[2 * a] + xy[2 * c]) / 2;
+                    xy[2 * md + 1] = (xy[2 * a + 1] + xy[2 * c + 1]) / 2;
+                    mids[3 * n_mids] = a < c ? a : c; mids[3 * n_mids + 1] = a < c ? c : a; mids[3 * n_mids + 2] = md;
+                    ++n_mids;
+                }
+                const int *order = CHILD_ORDER[m->el[o].rotation][m->el[o].status];
+                for (int h = 0; h < 2; ++h) {
+                    bars[2 * nb] = h == 0 ? g->nodes[0] : md;
+                    bars[2 * nb + 1] = h == 0 ? md : g->nodes[1];
+                    btype[nb] = O_BC_PERIODIC;
+                    borig[2 * nb] = newpos[o] + order[(so + h) % 4];
+                    borig[2 * nb + 1] = so;
+                    ++nb;
+                }
+            } else {
+                bars[2 * nb] = g->nodes[0]; bars[2 * nb + 1] = g->nodes[1];
+                btype[nb] = O_BC_PERIODIC;
+                borig[2 * nb] = newpos[o];
+                borig[2 * nb + 1] = so;
+                bzero[nb] = !raise[o] && g->sf[0][0] == 0.0;
+                ++nb;
+            }
+            continue;
+        }
         const o_face *f = &m->fa[g->faces[0][0]];
         const int64_t src = f->elements[0];
-        bremap[b] = nb;
         if (split[src]) {
             const int64_t md = edge_mid_find(mids, n_mids, g->nodes[0], g->nodes[1]);
-            bars[2 * nb] = g->nodes[0]; bars[2 * nb + 1] = md; btype[nb] = m->bar_type[b]; bdonor[nb] = -1; ++nb;
-            bars[2 * nb] = md; bars[2 * nb + 1] = g->nodes[1]; btype[nb] = m->bar_type[b]; bdonor[nb] = -1; ++nb;
+            for (int h = 0; h < 2; ++h) {
+                bars[2 * nb] = h == 0 ? g->nodes[0] : md;
+                bars[2 * nb + 1] = h == 0 ? md : g->nodes[1];
+                btype[nb] = m->bar_type[b];
+                borig[2 * nb] = borig[2 * nb + 1] = -1;
+                ++nb;
+            }
         } else {
             bars[2 * nb] = g->nodes[0]; bars[2 * nb + 1] = g->nodes[1];
             btype[nb] = m->bar_type[b];
-            bdonor[nb] = m->bar_donor[b];
+            borig[2 * nb] = borig[2 * nb + 1] = -1;
             bzero[nb] = !raise[src] && g->sf[0][0] == 0.0;
             ++nb;
         }
     }
-    for (int64_t b = 0; b < nb; ++b)
-        if (btype[b] == O_BC_PERIODIC) bdonor[b] = bremap[bdonor[b]];
-    o_mesh *nm = o_mesh_create(m->t, m->initial_N, n_nodes, xy, Knew, quads, qN, nb, bars, btype, bdonor,
-                               m->tol_min, m->tol_max);
+    o_mesh *nm = mesh_create(m->t, m->initial_N, n_nodes, xy, Knew, quads, qN, nb, bars, btype, NULL, borig,
+                             m->tol_min, m->tol_max);
     nm->c = m->c;
     for (int64_t k = 0; k < Knew; ++k) {
         o_elem *e = &nm->el[k];
@@ -1880,7 +2012,7 @@ static int64_t adapt_split(o_mesh *m, int N_max, int max_split_level) {
     *nm = tmp;
     o_mesh_destroy(nm);
     free(xy); free(mids); free(quads); free(qN); free(parent); free(child_status);
-    free(bars); free(btype); free(bdonor); free(bzero); free(bremap); free(split); free(raise);
+    free(bars); free(btype); free(borig); free(bzero); free(newpos); free(split); free(raise);
     return n_split + n_raise;
 }
 

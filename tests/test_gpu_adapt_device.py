@@ -4,7 +4,8 @@ host from the node lists) give the same element counts, face counts, orders, ele
 Hilbert statuses, rotations, split levels, estimates and states bit for bit; refused adaptations
 change nothing; and the device path's adaptive loop matches the oracle at the batched workload's
 shape. The oracle comparisons of the other adaptive tests run through the device path too (it is
-the default).
+the default). Splits across periodic sides (which the host path does not provide) are compared
+with the oracle and with the same solve shifted by half a period.
 """
 import numpy as np
 import pytest
@@ -12,6 +13,7 @@ import pytest
 import ndg_cuda_amd as nd
 from oracle import oracle as O
 from test_gpu_pinned_paths import batched_hp_pulses, rotated
+from test_oracle_adapt import PERIODIC_SPLITS
 
 pytestmark = pytest.mark.gpu
 
@@ -102,22 +104,132 @@ def test_device_adaptation_twice_split_equals_host_path():
     assert_identical(dm, hm, t)
 
 
-def test_device_adaptation_refusal_changes_nothing():
-    """A split of an element with a periodic side is refused by the device path before anything
-    changes: same orders, element count and state as before the call."""
-    say("device refusal")
+def rough_state(om, pred, rng):
+    """Noise on the elements whose centre satisfies pred (they h-refine: at N = 6 a noise field's
+    spectral decay rate is below 1), zero elsewhere (no refinement), packed in element order."""
+    nodes, _, _, xy = om.elements()
+    out = []
+    for f in range(3):
+        parts = []
+        for e, (x, y) in enumerate(xy[nodes].mean(axis=1)):
+            n = (int(om.N[e]) + 1) ** 2
+            parts.append(rng.normal(size=n) if pred(x, y) else np.zeros(n))
+        out.append(np.concatenate(parts))
+    return out
+
+
+def periodic_pair(N0=6, N_max=8):
     desc = nd.square_mesh(8, x_periodic=True, y_periodic=True, jitter=0.1, seed=7)
-    m = nd.Mesh2D_t(desc, 3, 9, 5, 1, 0, 0, 1e-4, 1e-14, 0.0, nd.NDG_t(9, 5))
-    m.initial_conditions([(0.9, 0.9, 0.06, 1.0)])  # a narrow pulse on the periodic corner: h-refinement there
-    before = m.get_solution()
-    orders = m.orders_.copy()
+    m = nd.Mesh2D_t(desc, N0, N_max, 5, 2, 0, 0, 1e-4, 1e-14, 0.0, nd.NDG_t(N_max, 5))
+    om = O.Mesh(O.Tables(N_max, 5), desc, N0, tol_min=1e-4, tol_max=1e-14)
+    return m, om
+
+
+def split_both(m, om, pred, rng):
+    q = rough_state(om, pred, rng)
+    m.set_solution(*q)
+    om.set_state(*q)
+    om.estimate_error()
+    return om.adapt(8, 2), q
+
+
+@pytest.mark.parametrize("name", list(PERIODIC_SPLITS))
+def test_device_splits_across_the_period_match_oracle(name):
+    """Splits next to the periods (2:1 across them, at split levels 1 and 2, and conforming again
+    when the partners split): the device adaptation splits the same elements as the oracle
+    (split_interfaces, src/meshes/Mesh2D_t.cpp:6343-6756), with the same element and face counts
+    and orders; then the right-hand side of a periodised two-pulse state (relative to its largest
+    field) and the state after 20 RK3 steps match the oracle to 1e-12."""
+    say("periodic splits", name)
+    m, om = periodic_pair()
+    rng = np.random.default_rng(11)
+    for pred in PERIODIC_SPLITS[name]:
+        n_o, _ = split_both(m, om, pred, rng)
+        assert n_o > 0
+        assert m.adapt(8, 2) == n_o
+        assert m.n_elements_ == om.K and m.n_faces_ == om.n_faces
+        assert np.array_equal(m.orders_, om.N)
+    link, hang, hang2, per = m.sides()
+    assert ((hang != 0) & (per != 0)).any() or name == "then the partners"
+    say("K", m.n_elements_, "faces", m.n_faces_, "periodic hanging sides", int(((hang != 0) & (per != 0)).sum()))
+    pulses = [(0.9, 0.85, 0.2, 1.0), (-0.6, 0.2, 0.3, 0.5)]
+    m.initial_conditions(pulses, period=(2.0, 2.0))
+    om.initial_conditions(pulses, period=(2.0, 2.0))
+    G, R = m.compute_rhs(0.0), om.compute_rhs(0.0)
+    scale = max(np.abs(r).max() for r in R)  # the pressure rate of a state at rest is tiny
+    for g, r in zip(G, R):
+        assert float(np.max(np.abs(g - r))) < 1e-12 * scale
+    solver = nd.Solver2D_t(0.5)
+    t = solver.run_steps(m, 20)
+    assert t == om.run_steps(0.5, 20)
+    for g, r in zip(m.get_solution(), om.get_state()):
+        assert float(np.max(np.abs(g - r)) / np.max(np.abs(r))) < 1e-12
+
+
+def test_device_refuses_three_faces_across_the_period():
+    """A child on the period whose larger partner across it does not split would leave three
+    faces on the partner's side: the device path refuses before anything changes (same orders,
+    element count and state), as the oracle does."""
+    say("device refusal across the period")
+    m, om = periodic_pair()
+    rng = np.random.default_rng(11)
+    n_o, _ = split_both(m, om, lambda x, y: x > 0.75, rng)
+    assert m.adapt(8, 2) == n_o == 8
+    n_o, q = split_both(m, om, lambda x, y: x > 0.9 and y > 0.5, rng)
+    assert n_o == -2
+    K, faces, orders = m.n_elements_, m.n_faces_, m.orders_.copy()
     with pytest.raises(nd.NDGError) as info:
-        m.adapt()
+        m.adapt(8, 2)
     assert info.value.code == nd._abi.NDG_ERR_UNSUPPORTED
     m._refresh()
-    assert np.array_equal(m.orders_, orders) and m.n_elements_ == 64
-    for a, b in zip(m.get_solution(), before):
+    assert m.n_elements_ == K and m.n_faces_ == faces and np.array_equal(m.orders_, orders)
+    for a, b in zip(m.get_solution(), q):
         assert np.array_equal(a, b)
+
+
+def corner_pulse_solve(x0, y0):
+    desc = nd.square_mesh(8, x_periodic=True, y_periodic=True)
+    m = nd.Mesh2D_t(desc, 4, 10, 5, 1, 5, 0, 1e-5, 1e-14, 0.0, nd.NDG_t(10, 5))
+    m.initial_conditions([(x0, y0, 0.15, 1.0)], period=(2.0, 2.0))
+    solver = nd.Solver2D_t(0.5, [0.3])
+    solver.solve(None, m)
+    return m, solver
+
+
+def test_device_periodic_split_solve_matches_oracle_and_the_shifted_solve():
+    """The hp-adaptive solve with the pulse on the periodic corner (its elements split across both
+    periods) against the oracle (step count, time, element and face counts, orders, state <=
+    1e-12), then against the same device solve with the pulse moved by half a period to the
+    centre (split in the interior), shifted back element by element (<= 1e-12)."""
+    say("periodic corner pulse solve")
+    a, sa = corner_pulse_solve(0.95, 0.95)
+    om = O.Mesh(O.Tables(10, 5), nd.square_mesh(8, x_periodic=True, y_periodic=True), 4, tol_min=1e-5,
+                tol_max=1e-14)
+    om.initial_conditions([(0.95, 0.95, 0.15, 1.0)], period=(2.0, 2.0))
+    steps, t = om.solve_adaptive(0.5, 0.3, 5, 10, 1)
+    assert sa.n_steps_ == steps and sa.time_ == t
+    assert a.n_elements_ == om.K > 64 and a.n_faces_ == om.n_faces
+    assert np.array_equal(a.orders_, om.N)
+    for g, r in zip(a.get_solution(), om.get_state()):
+        assert float(np.max(np.abs(g - r)) / np.max(np.abs(r))) < 1e-12
+    b, sb = corner_pulse_solve(-0.05, -0.05)
+    assert sb.n_steps_ == steps and b.n_elements_ == a.n_elements_
+
+    def centres(m):
+        c = m.get_complete_solution(t)
+        return np.stack([c["x"].mean(axis=(1, 2)), c["y"].mean(axis=(1, 2))], axis=1)
+
+    key = lambda c: tuple(np.round(c, 9))
+    index_b = {key(c): k for k, c in enumerate(centres(b))}
+    off_a = np.concatenate([[0], np.cumsum((a.orders_.astype(np.int64) + 1) ** 2)])
+    off_b = np.concatenate([[0], np.cumsum((b.orders_.astype(np.int64) + 1) ** 2)])
+    qa, qb = a.get_solution(), b.get_solution()
+    scale = max(np.abs(q).max() for q in qb)
+    for e, c in enumerate(centres(a)):
+        j = index_b[key((c - 1.0 + 1.0) % 2.0 - 1.0)]
+        assert a.orders_[e] == b.orders_[j]
+        for f in range(3):
+            assert np.abs(qa[f][off_a[e]:off_a[e + 1]] - qb[f][off_b[j]:off_b[j + 1]]).max() < 1e-12 * scale
 
 
 def test_device_adaptation_at_workload_shape_matches_oracle():

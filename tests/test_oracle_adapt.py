@@ -236,8 +236,8 @@ def test_split_interpolates_and_conserves(oracle_tables):
 
 
 def test_split_walls_and_limits(oracle_tables):
-    """Splitting an element on a wall splits its ghost in two (geometry recomputed); a periodic
-    side or a half side of an unsplit neighbour is outside this restatement."""
+    """Splitting an element on a wall splits its ghost in two (geometry recomputed); splitting
+    every element of a periodic mesh splits every periodic ghost with its origin."""
     desc, om = mesh(oracle_tables, 4, 3, jitter=0.1)
     flags = np.zeros(om.K, dtype=np.int32)
     flags[0] = 1  # the first Hilbert element is a corner element: two wall sides
@@ -254,7 +254,132 @@ def test_split_walls_and_limits(oracle_tables):
     assert_hilbert_path(om)
     om.set_state(*(np.ones(om.n) for _ in range(3)))
     assert all(np.isfinite(g).all() for g in om.compute_rhs(0.0))
-    # periodic sides are refused
+    # every element of a periodic mesh: conforming again across the period
     d2, om2 = mesh(oracle_tables, 4, 3, periodic=True)
+    assert om2.n_faces == 2 * 4 * 3 + 4 * 4  # interior faces, then one ghost face per periodic side
     om2.set_flags(np.ones(om2.K, dtype=np.int32), np.zeros(om2.K))
-    assert om2.adapt(N_max=16, max_split_level=1) == -2
+    assert om2.adapt(N_max=16, max_split_level=1) == 16
+    assert om2.K == 64 and om2.n_faces == 2 * 8 * 7 + 4 * 8
+    assert_hilbert_path(om2)
+
+
+# ---------------------------------------------------------------------------------------------
+# Splitting across the period (split_interfaces, Mesh2D_t.cpp:6343-6756): a periodic ghost splits
+# with its origin, so a split element faces an unsplit partner through two half ghosts on one end
+# and one ghost covering two sides on the other (2:1 across the period).
+
+def centre_flags(om, pred):
+    nodes, _, _, xy = om.elements()
+    return np.array([1 if pred(x, y) else 0 for x, y in xy[nodes].mean(axis=1)], dtype=np.int32)
+
+
+def split_where(om, *preds):
+    for pred in preds:
+        om.set_flags(centre_flags(om, pred), np.zeros(om.K))
+        n = om.adapt(N_max=16, max_split_level=2)
+        if n < 0:
+            return n
+    return n
+
+
+def fine_description(om, desc):
+    nodes, _, _, xy = om.elements()
+    return nd.MeshDescription(xy, nodes, desc.bars, desc.bar_type, desc.bar_donor)
+
+
+PERIODIC_SPLITS = {
+    # elements at the periodic corner: 2:1 across both periods
+    "corner block": ((lambda x, y: x > 0.5 and y > 0.5),),
+    # one column and one row next to the periods (their corner element split on two sides)
+    "column and row": ((lambda x, y: x > 0.75 or y < -0.75),),
+    # the right column, then its partners on the left: conforming again across the period
+    "then the partners": ((lambda x, y: x > 0.75), (lambda x, y: x < -0.75)),
+    # the right column, then its partners, then the children on the period at level 2 (2:1 across
+    # the period between levels 1 and 2)
+    "level two": ((lambda x, y: x > 0.75), (lambda x, y: x < -0.75), (lambda x, y: x > 0.9 and y > 0.5)),
+    # the same, then their partners too
+    "two levels": ((lambda x, y: x > 0.75), (lambda x, y: x < -0.75),
+                   (lambda x, y: x > 0.9 and y > 0.5), (lambda x, y: x < -0.9 and y > 0.5)),
+}
+
+
+@pytest.mark.parametrize("name", list(PERIODIC_SPLITS))
+def test_periodic_split_keeps_the_operator_exact_and_conservative(oracle_tables, name):
+    """On meshes split next to the periods (2:1 across them): a constant state stays at rest, the
+    mass integrals of a random state's right-hand side vanish, and the right-hand side of a
+    smooth periodic field matches its exact derivative to the order-10 interpolation error
+    (observed 1.4e-12; a periodic ghost copying the wrong child of its origin gives 20)."""
+    N = 10
+    desc = nd.square_mesh(8, x_periodic=True, y_periodic=True)
+    om = O.Mesh(oracle_tables, desc, N, tol_min=1e-14, tol_max=1e-300)
+    assert split_where(om, *PERIODIC_SPLITS[name]) > 0
+    assert om.K > 64
+    import test_oracle_math as tm
+    fine = fine_description(om, desc)
+    X, Y = tm.packed_nodes(fine, oracle_tables, om.N)
+    W = tm.quadrature_weights(fine, oracle_tables, om.N)
+    om.set_state(np.ones(om.n), 0.3 * np.ones(om.n), -0.2 * np.ones(om.n))
+    assert max(np.abs(g).max() for g in om.compute_rhs(0.0)) < 1e-11
+    rng = np.random.default_rng(1)
+    om.set_state(*(rng.normal(size=om.n) for _ in range(3)))
+    for g in om.compute_rhs(0.0):
+        assert abs(float(W @ g)) < 1e-13
+    k = np.pi
+    om.set_state(np.sin(k * X) * np.cos(k * Y), np.cos(k * X + 0.3), np.sin(k * Y) * np.cos(k * X))
+    exact_rhs = (k * np.sin(k * X + 0.3) - k * np.cos(k * Y) * np.cos(k * X), -k * np.cos(k * X) * np.cos(k * Y),
+                 k * np.sin(k * X) * np.sin(k * Y))
+    for g, e in zip(om.compute_rhs(0.0), exact_rhs):
+        assert np.abs(g - e).max() < 1e-10
+
+
+def test_periodic_split_refuses_three_faces_on_a_side(oracle_tables):
+    """A child next to the period whose larger partner does not split again would leave three
+    faces on the partner's side: refused, nothing changed."""
+    desc = nd.square_mesh(8, x_periodic=True, y_periodic=True)
+    om = O.Mesh(oracle_tables, desc, 4, tol_min=1e-14, tol_max=1e-300)
+    assert split_where(om, lambda x, y: x > 0.75) == 8
+    K, faces = om.K, om.n_faces
+    assert split_where(om, lambda x, y: x > 0.8 and y > 0.5) == -2
+    assert om.K == K and om.n_faces == faces
+
+
+def periodic_pulse_solve(oracle_tables, x0, y0):
+    desc = nd.square_mesh(8, x_periodic=True, y_periodic=True)
+    om = O.Mesh(oracle_tables, desc, 4, tol_min=1e-5, tol_max=1e-14)
+    om.initial_conditions([(x0, y0, 0.15, 1.0)], period=(2.0, 2.0))
+    steps, t = om.solve_adaptive(0.5, 0.3, 5, 10, 1)
+    return om, steps, t
+
+
+def test_periodic_split_solve_equals_the_shifted_solve(oracle_tables):
+    """Translation invariance: an hp-adaptive solve with the pulse on the periodic corner (its
+    elements split across both periods) equals, element by element, the same solve with the
+    pulse moved by half a period to the centre (split in the interior), shifted back: same step
+    count, element count and orders, states equal to rounding (observed 4e-15)."""
+    a, steps_a, t_a = periodic_pulse_solve(oracle_tables, 0.95, 0.95)
+    b, steps_b, t_b = periodic_pulse_solve(oracle_tables, -0.05, -0.05)
+    assert steps_a == steps_b and t_a == t_b
+    assert a.K == b.K > 64
+    assert np.array_equal(np.bincount(a.N), np.bincount(b.N))
+    assert_same_up_to_shift(a, b, 1.0, 1.0, 1e-12)
+
+
+def assert_same_up_to_shift(a, b, dx, dy, tol):
+    """States of meshes a and b equal element by element, element e of a matching the element of
+    b whose centre is e's centre moved by (-dx, -dy) modulo the period 2."""
+    def centres(m):
+        nodes, _, _, xy = m.elements()
+        return xy[nodes].mean(axis=1)
+    key = lambda c: tuple(np.round(c, 9))
+    index_b = {key(c): k for k, c in enumerate(centres(b))}
+    off_a = np.concatenate([[0], np.cumsum((a.N.astype(np.int64) + 1) ** 2)])
+    off_b = np.concatenate([[0], np.cumsum((b.N.astype(np.int64) + 1) ** 2)])
+    sa, sb = a.get_state(), b.get_state()
+    scale = max(np.abs(s).max() for s in sb)
+    for e, c in enumerate(centres(a)):
+        shifted = (c - (dx, dy) + 1.0) % 2.0 - 1.0
+        j = index_b[key(shifted)]
+        assert a.N[e] == b.N[j]
+        for f in range(3):
+            d = np.abs(sa[f][off_a[e]:off_a[e + 1]] - sb[f][off_b[j]:off_b[j + 1]]).max()
+            assert d < tol * scale, (e, f, d)
