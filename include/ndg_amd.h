@@ -268,10 +268,11 @@ ndg_status ndg_pre_condition(ndg_mesh *mesh, double cfl, int64_t n_adaptivity_st
  *   parent polynomial (inverse bilinear map + interpolate_from), ghosts on split sides become two
  *   (split_boundaries, :6232-6343), the would-p-refine elements are raised by 2, and 2:1 hanging
  *   faces appear, also across periodic sides (split_interfaces, :6343-6756: each end of the
- *   period gets its own faces against the other end's trace); splitting a small hanging element
- *   (across the interior or a period) whose larger neighbour does not split returns
- *   NDG_ERR_UNSUPPORTED and changes nothing, as does, in the NDG_ADAPT_HOST mode, splitting an
- *   element with a periodic side;
+ *   period gets its own faces against the other end's trace), with any number of faces on a side
+ *   down to pieces of 1/16 of it; a split that would leave a smaller piece returns
+ *   NDG_ERR_UNSUPPORTED and changes nothing, as do, in the NDG_ADAPT_HOST mode, splitting an
+ *   element with a periodic side and splitting a small hanging element whose larger neighbour
+ *   does not split;
  * - otherwise the p_adapt branch (:5026-5049): every element with refine && sigma >= 1 &&
  *   N + 2 <= N_max is raised by 2 (Element2D_t.cpp:464-579); ghosts and faces follow
  *   (adjust_boundaries / adjust_interfaces / adjust_faces, :7482-7538), including the reference's
@@ -301,9 +302,14 @@ ndg_status ndg_mesh_set_adapt_mode(ndg_mesh *mesh, int32_t mode);
  * element sides), 4 values per element in the caller's order; any output may be NULL.
  * link[4e+s]: the element side 4r+rs whose trace is the exterior state of side s of element e (a
  * neighbour, or the donor side of a periodic pair), or -(1 + ghost). hang[4e+s]: 0 conforming, 1 a
- * small side covering half of the larger side link, 2 a larger side whose half at its first corner
- * is covered by link and whose other half by hang_second. periodic[4e+s]: 1 for a periodic pair. */
-ndg_status ndg_mesh_sides(ndg_mesh *mesh, int64_t *link, int8_t *hang, int64_t *hang_second, int8_t *periodic);
+ * small side covering a piece of the larger side link, 2 a larger side covered by several small
+ * sides, the first of them (at its first corner) being link. For a small side, span[4e+s] is the
+ * piece [j / 2^d, (j + 1) / 2^d] of the larger side it covers, in the larger side's direction, as
+ * 2^d - 1 + j (1, 2: the halves; 3..6: the quarters; d <= 4), and next_small[4e+s] the next small
+ * side along the larger side (-1 after the last, and on other sides). periodic[4e+s]: 1 for a
+ * periodic pair. */
+ndg_status ndg_mesh_sides(ndg_mesh *mesh, int64_t *link, int8_t *hang, int64_t *next_small, int8_t *periodic,
+                          int8_t *span);
 /* Mesh2D_t's adaptivity_interval: ndg_solve then adapts every that many steps with the tables'
  * N_max and the description's max_split_level (include/solvers/Solver2D_t.tpp:88-96). 0 = off. */
 ndg_status ndg_mesh_set_adaptivity(ndg_mesh *mesh, int64_t adaptivity_interval);

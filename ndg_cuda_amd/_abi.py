@@ -155,7 +155,7 @@ def lib():
             "ndg_mesh_orders": [vp, vp],
             "ndg_mesh_set_adapt_mode": [vp, i32],
             "ndg_inverse_quad_map": [vp, i64, vp, vp],
-            "ndg_mesh_sides": [vp, vp, vp, vp, vp],
+            "ndg_mesh_sides": [vp, vp, vp, vp, vp, vp],
             "ndg_hilbert_child_order": [i32, i32, vp, vp],
             "ndg_hilbert_element_status": [i32, i32, vp],
             "ndg_record_receivers_noalloc": [vp, vp, f64, i64, vp],

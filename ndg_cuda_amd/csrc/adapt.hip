@@ -276,9 +276,9 @@ __global__ void __launch_bounds__(kThreads) decide_kernel(LogicalMesh L, const d
     if (bad) atomicOr(&c->refused, (unsigned long long)kRefuseOrder);
 }
 
-// Refusals of the split branch (a small hanging element, across an interior or a periodic side,
-// that splits while its larger neighbour does not: more than two faces on one side) and the
-// ghosts that split with their element.
+// Refusals of the split branch (a small hanging element already covering 1 / 2^kMaxSpanLevel of a
+// larger side, across the interior or a period, that splits while the larger element does not)
+// and the ghosts that split with their element.
 __global__ void __launch_bounds__(kThreads) check_kernel(LogicalMesh L, const int8_t *__restrict__ flags,
                                                           AdaptCounts *c) {
     const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -286,7 +286,8 @@ __global__ void __launch_bounds__(kThreads) check_kernel(LogicalMesh L, const in
     if (i < L.K && flags[i] == 2)
         for (int s = 0; s < 4; ++s) {
             const int64_t es = 4 * i + s;
-            if (L.hang[es] == 1 && flags[L.link[es] / 4] != 2) refused |= kRefuseFourFaces;
+            if (L.hang[es] == 1 && class_level(L.span[es]) >= kMaxSpanLevel && flags[L.link[es] / 4] != 2)
+                refused |= kRefuseSpan;
         }
     if (i < L.n_ghost && flags[L.ghost_side[i] / 4] == 2) gs = 1;
     if (refused) atomicOr(&c->refused, refused);
@@ -303,14 +304,41 @@ __global__ void count_kernel(const int8_t *__restrict__ flags, int64_t K, int64_
     if (i < n_ghost) newghost[i] = flags[ghost_side[i] / 4] == 2 ? 2 : 1;
 }
 
-// The middle of side s of old element e: the middle node of a larger interior hanging side (the
-// start of its first small side), else the midpoint of the side's corners (the small sides of a
-// larger periodic side lie at the other end of the period).
+// Small sides along a larger side: the chain from its link through hang2, in the larger side's
+// direction (at most 2^kMaxSpanLevel of them; the bound only guards the walk).
+constexpr int kMaxChain = 1 << kMaxSpanLevel;
+
+__device__ int chain_length(const LogicalMesh &L, int64_t large) {
+    int n = 0;
+    for (int64_t c = L.link[large]; c >= 0 && n < kMaxChain; c = L.hang2[c]) ++n;
+    return n;
+}
+
+// Position of small side t in its larger side's chain.
+__device__ int chain_position(const LogicalMesh &L, int64_t t) {
+    int pos = 0;
+    for (int64_t c = L.link[L.link[t]]; c != t && c >= 0 && pos < kMaxChain; c = L.hang2[c]) ++pos;
+    return pos;
+}
+
+// Half (0 at the larger side's first corner) of the larger side that small side t lies in.
+__device__ int span_half(const LogicalMesh &L, int64_t t) {
+    const int cls = L.span[t], d = class_level(cls);
+    return class_index(cls) >> (d - 1);
+}
+
+// The middle of side s of old element e: on a larger interior hanging side the node its small
+// sides share there (the start of the small side whose piece ends at the middle: small sides run
+// against the larger side), else the midpoint of the side's corners (the small sides of a larger
+// periodic side lie at the other end of the period).
 __device__ v2 side_middle(const LogicalMesh &L, int64_t e, int s) {
     const int64_t es = 4 * e + s;
     if (L.hang[es] == 2 && !L.periodic[es]) {
-        const int64_t f = L.link[es];
-        return corner(L, f / 4, (int)(f % 4));
+        int n = 0;
+        for (int64_t f = L.link[es]; f >= 0 && n < kMaxChain; f = L.hang2[f], ++n) {
+            const int cls = L.span[f];
+            if ((class_index(cls) + 1) * 2 == (1 << class_level(cls))) return corner(L, f / 4, (int)(f % 4));
+        }
     }
     return midpoint(corner(L, e, s), corner(L, e, s + 1));
 }
@@ -366,6 +394,7 @@ __global__ void __launch_bounds__(kThreads) elements_kernel(LogicalMesh L, Logic
 struct SideMap {
     const LogicalMesh &L;
     const int64_t *newpos;
+    const int8_t *flags;
     // Side rs of unsplit old element r in the new numbering.
     __device__ int64_t same(int64_t r, int rs) const { return 4 * newpos[r] + rs; }
     // The child of split old element r covering half h of its side rs (h = 0: the half at the
@@ -373,8 +402,21 @@ struct SideMap {
     __device__ int64_t half(int64_t r, int rs, int h) const {
         return 4 * (newpos[r] + dChildOrder[L.rotation[r]][L.status[r]][(rs + h) & 3]) + rs;
     }
+    // The first new side of old small side n in its larger side's direction (n runs the other way,
+    // so a split n's half at its second corner comes first); -1 for n < 0.
+    __device__ int64_t first_piece(int64_t n) const {
+        if (n < 0) return -1;
+        const int64_t r = n / 4;
+        const int rs = (int)(n % 4);
+        return flags[r] == 2 ? half(r, rs, 1) : same(r, rs);
+    }
 };
 
+// The exterior of every new side from the old topology alone, one thread per side. A small side
+// covers the piece span = (d, j) of its larger side (in the larger side's direction) and links to
+// the next small side along it; a larger side links to its first small side. Through a period the
+// partner's side runs against this one, as an interior neighbour's does, so periodic sides follow
+// the same rules and keep their flag.
 __global__ void __launch_bounds__(kThreads) links_kernel(LogicalMesh L, LogicalMesh Ln, const int8_t *__restrict__ flags,
                                                           const int64_t *__restrict__ newpos,
                                                           const int64_t *__restrict__ newghost,
@@ -382,13 +424,23 @@ __global__ void __launch_bounds__(kThreads) links_kernel(LogicalMesh L, LogicalM
                                                           const int8_t *__restrict__ corner_of) {
     const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (t >= 4 * Ln.K) return;
-    const SideMap M{L, newpos};
+    const SideMap M{L, newpos, flags};
     const int64_t k = t / 4, e = parent[k];
     const int s = (int)(t % 4), c = corner_of[k];
-    int64_t link = 0, hang2 = -1;
-    int hang = 0, per = 0;
-    // A periodic side follows the interior rules: through the period the partner's side runs
-    // the other way, as an interior neighbour's does.
+    int64_t link = 0, next = -1;
+    int hang = 0, per = 0, span = 0;
+    // A small side (old side es, piece (d, j) of larger side l of old element r) whose larger
+    // element splits: it lies in half H of l, covered by r's child there, in which its piece is
+    // (d - 1, j mod 2^(d-1)) refined by (dd, jj) (for a half of es: one level more).
+    auto under_split_large = [&](int64_t es, int64_t l, int dd, int jj, int64_t nxt) {
+        const int cls = L.span[es], d = class_level(cls), j = class_index(cls), H = j >> (d - 1);
+        const int d2 = d - 1 + dd, j2 = ((j & ((1 << (d - 1)) - 1)) << dd) + jj;
+        link = M.half(l / 4, (int)(l % 4), H);
+        if (d2 == 0) return;  // the whole side of r's child: conforming
+        hang = 1;
+        span = make_class(d2, j2);
+        next = nxt >= 0 && span_half(L, nxt) == H ? M.first_piece(nxt) : -1;
+    };
     if (c < 0) {  // unsplit element: side s of e
         const int64_t es = 4 * e + s, l = L.link[es];
         const int hk = L.hang[es];
@@ -400,24 +452,21 @@ __global__ void __launch_bounds__(kThreads) links_kernel(LogicalMesh L, LogicalM
             const int rs = (int)(l % 4);
             if (flags[r] != 2) {
                 link = M.same(r, rs);
-            } else {  // the neighbour splits: this side becomes the larger side of a 2:1 pair
+            } else {  // the neighbour splits: this side becomes the larger side of its two halves
                 hang = 2;
                 link = M.half(r, rs, 1);
-                hang2 = M.half(r, rs, 0);
             }
-        } else if (hk == 2) {  // the small elements do not split (that would be refused)
-            const int64_t l2 = L.hang2[es];
+        } else if (hk == 2) {  // still the larger side; its small elements may have split
             hang = 2;
-            link = M.same(l / 4, (int)(l % 4));
-            hang2 = M.same(l2 / 4, (int)(l2 % 4));
+            link = M.first_piece(l);
         } else {  // small side of the larger side l
-            const int64_t r = l / 4;
-            const int ls = (int)(l % 4);
-            if (flags[r] != 2) {
+            if (flags[l / 4] != 2) {
                 hang = 1;
-                link = M.same(r, ls);
-            } else {  // the larger element splits: its child on this half matches this side
-                link = M.half(r, ls, L.link[l] == es ? 0 : 1);
+                link = M.same(l / 4, (int)(l % 4));
+                span = L.span[es];
+                next = M.first_piece(L.hang2[es]);
+            } else {
+                under_split_large(es, l, 0, 0, L.hang2[es]);
             }
         }
     } else {  // child at corner c of split element e
@@ -435,6 +484,9 @@ __global__ void __launch_bounds__(kThreads) links_kernel(LogicalMesh L, LogicalM
             const int64_t es = 4 * e + ps, l = L.link[es];
             const int hk = L.hang[es];
             per = L.periodic[es];
+            // in the exterior side's direction this half is piece 1 - h of the parent side, the
+            // half h = 1 first, followed by its sibling half
+            const int64_t sibling = M.half(e, ps, 0);
             if (l < 0) {  // the ghost splits: ghost g + 1 covers the first half, g the second
                 link = -(1 + newghost[-l - 1] + (h == 0 ? 1 : 0));
             } else if (hk == 0) {
@@ -445,30 +497,39 @@ __global__ void __launch_bounds__(kThreads) links_kernel(LogicalMesh L, LogicalM
                 } else {
                     hang = 1;
                     link = M.same(r, rs);
+                    span = make_class(1, 1 - h);
+                    next = h == 1 ? sibling : -1;
                 }
-            } else if (hk == 2) {  // the small element of this half
-                const int64_t sm = h == 0 ? l : L.hang2[es];
-                const int64_t r = sm / 4;
-                const int rs = (int)(sm % 4);
-                if (flags[r] != 2) {
-                    link = M.same(r, rs);
+            } else if (hk == 1) {  // a piece of the larger side l
+                const int cls = L.span[es], d = class_level(cls), j = class_index(cls);
+                const int64_t nxt = h == 1 ? -2 : L.hang2[es];
+                if (flags[l / 4] != 2) {  // one level finer (refused beyond kMaxSpanLevel)
+                    hang = 1;
+                    link = M.same(l / 4, (int)(l % 4));
+                    span = make_class(d + 1, 2 * j + 1 - h);
+                    next = h == 1 ? sibling : M.first_piece(nxt);
+                } else {
+                    under_split_large(es, l, 1, 1 - h, nxt >= 0 ? nxt : -1);
+                    if (h == 1) next = sibling;
+                }
+            } else {  // the larger side: the small sides on this half
+                int64_t f = l;
+                for (int n = 0; f >= 0 && n < kMaxChain && span_half(L, f) != h; ++n) f = L.hang2[f];
+                const int cls = f >= 0 ? L.span[f] : 0;
+                if (f >= 0 && class_level(cls) == 1 && flags[f / 4] != 2) {  // one small side, unsplit
+                    link = M.same(f / 4, (int)(f % 4));
                 } else {
                     hang = 2;
-                    link = M.half(r, rs, 1);
-                    hang2 = M.half(r, rs, 0);
+                    link = M.first_piece(f);
                 }
-            } else {  // small side of a larger side that splits too (anything else is refused)
-                const int64_t r = l / 4;
-                const int ls = (int)(l % 4);
-                hang = 1;
-                link = M.half(r, ls, L.link[l] == es ? 0 : 1);
             }
         }
     }
     Ln.link[t] = link;
-    Ln.hang2[t] = hang2;
+    Ln.hang2[t] = next;
     Ln.hang[t] = (int8_t)hang;
     Ln.periodic[t] = (int8_t)per;
+    Ln.span[t] = (int8_t)span;
 }
 
 // Ghosts: a ghost on a split side becomes the ghosts of bars (P(s+1), m) and (m, P(s)), on the
@@ -490,7 +551,7 @@ __global__ void __launch_bounds__(kThreads) ghosts_kernel(LogicalMesh L, Logical
         Ln.ghost_zero[ng] = (int8_t)(p_only ? (z || f == 1) : (z && f != 1));
         return;
     }
-    const SideMap M{L, newpos};
+    const SideMap M{L, newpos, flags};
     const v2 u0 = corner(L, e, s), u1 = corner(L, e, s + 1), md = midpoint(u0, u1);
     for (int h2 = 0; h2 < 2; ++h2) {
         const v2 a0 = h2 == 0 ? md : u0, a1 = h2 == 0 ? u1 : md;  // the element side a0 -> a1
@@ -503,7 +564,6 @@ __global__ void __launch_bounds__(kThreads) ghosts_kernel(LogicalMesh L, Logical
         rec.sf = side_scale(G, 0);
         face_frame(a0, a1, a0, a1, rec.nx, rec.ny, rec.tx, rec.ty);
         rec.slot = -1;
-        rec.slot2 = -1;
         Ln.ghost[ng + h2] = rec;
         Ln.ghost_side[ng + h2] = M.half(e, s, h2 == 0 ? 1 : 0);
         Ln.ghost_zero[ng + h2] = 0;
@@ -582,7 +642,7 @@ __device__ void side_roles(const LogicalMesh &L, int64_t t, bool &mortar, int &o
     const int hk = L.hang[t];
     const bool other = l >= 0 && L.order[l / 4] != L.order[k];
     mortar = hk != 0 || other;
-    if (L.periodic[t]) owned = mortar ? (hk == 2 ? 2 : 1) : 0;
+    if (L.periodic[t]) owned = mortar ? (hk == 2 ? chain_length(L, t) : 1) : 0;
     else owned = hk == 1 || (hk == 0 && other && l / 4 > k);
 }
 
@@ -590,7 +650,7 @@ __global__ void __launch_bounds__(kThreads) side_count_kernel(LogicalMesh L, int
                                                                int64_t *__restrict__ cnt_slot,
                                                                int64_t *__restrict__ cnt_face, AdaptCounts *c) {
     const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    unsigned long long pairs = 0, larger = 0, pfaces = 0;
+    unsigned long long pairs = 0, hfaces = 0, pfaces = 0;
     if (t < 4 * L.K) {
         bool mortar;
         int owned;
@@ -598,20 +658,25 @@ __global__ void __launch_bounds__(kThreads) side_count_kernel(LogicalMesh L, int
         const int64_t k = t / 4, l = L.link[t];
         const int N = L.order[k], hk = L.hang[t];
         const bool per = L.periodic[t] != 0;
+        const int n_faces = hk == 2 ? chain_length(L, t) : 1;
         cnt_rec[t] = mortar;
-        cnt_slot[t] = mortar ? 3 * (int64_t)(N + 1) * (hk == 2 ? 2 : 1) : 0;
+        cnt_slot[t] = mortar ? 3 * (int64_t)(N + 1) * n_faces : 0;
         cnt_face[t] = owned;
         pairs = l >= 0 && hk == 0 && !per && l > t;
-        larger = hk == 2 && !per;
-        pfaces = per ? (hk == 2 ? 2 : 1) : 0;
+        hfaces = hk == 2 && !per ? n_faces : 0;
+        pfaces = per ? n_faces : 0;
         if (owned) {
             int Nr = L.order[l / 4];
-            if (hk == 2) Nr = max(Nr, (int)L.order[L.hang2[t] / 4]);
+            if (hk == 2) {
+                int n = 0;
+                for (int64_t f = L.hang2[l]; f >= 0 && n < kMaxChain; f = L.hang2[f], ++n)
+                    Nr = max(Nr, (int)L.order[f / 4]);
+            }
             atomicMax(&c->mortar_nodes, (unsigned long long)(max(N, Nr) + 1));
         }
     }
     wave_add(&c->conforming_pairs, pairs);
-    wave_add(&c->larger_sides, larger);
+    wave_add(&c->hanging_faces, hfaces);
     wave_add(&c->periodic_faces, pfaces);
 }
 
@@ -672,7 +737,7 @@ __global__ void __launch_bounds__(kThreads) side_fill_kernel(LogicalMesh L, Fill
         BcSide rec{};
         rec.kind = kKindMortar;
         rec.slot = A.slot_off[t];
-        rec.slot2 = hk == 2 ? A.slot_off[t] + 3 * nf : -1;
+        rec.n_faces = hk == 2 ? chain_length(L, t) : 1;
         A.bc[A.n_ghost + A.rec_idx[t]] = rec;
     } else if (l < 0) {
         code = l;
@@ -694,11 +759,12 @@ __global__ void __launch_bounds__(kThreads) side_fill_kernel(LogicalMesh L, Fill
     if (L.periodic[t]) {
         // One face per face of this side, against a ghost holding the partner side's trace (in
         // the partner's direction, which runs against this side through the period): a whole side
-        // against a whole partner or against a larger partner's half (first half when this side
-        // is the partner's first small side), or half of a larger side against a whole small
-        // partner.
-        for (int h = 0; h < owned; ++h) {
-            const int64_t lr = hk == 2 && h == 1 ? L.hang2[t] : l;
+        // against a whole partner or against its piece of a larger partner, or each piece of a
+        // larger side against its whole small partner, in the side's direction.
+        double tx, ty, nx, ny;
+        face_frame(c0, c1, c0, c1, nx, ny, tx, ty);
+        int64_t lr = l;
+        for (int h = 0; h < owned; ++h, lr = L.hang2[lr]) {
             const int NR = L.order[lr / 4];
             MortarFace fc{};
             fc.NL = N;
@@ -706,14 +772,13 @@ __global__ void __launch_bounds__(kThreads) side_fill_kernel(LogicalMesh L, Fill
             fc.Nf = max(N, NR);
             fc.trL = G.tr_off + local * 12 * nf + s * 3 * nf;
             fc.trR = trace_of(lr);
-            const v2 md = midpoint(c0, c1);
-            const v2 f0 = hk == 2 && h == 1 ? md : c0, f1 = hk == 2 && h == 0 ? md : c1;
-            double tx, ty;
-            face_frame(f0, f1, f0, f1, fc.nx, fc.ny, tx, ty);
-            fc.offset[0] = hk == 2 ? 0.5 * h : 0.0;
-            fc.scale[0] = hk == 2 ? 0.5 : 1.0;
-            fc.offset[1] = hk == 1 && L.link[l] != t ? 0.5 : 0.0;
-            fc.scale[1] = hk == 1 ? 0.5 : 1.0;
+            fc.nx = nx;
+            fc.ny = ny;
+            const int own = hk == 2 ? L.span[lr] : 0, ghost = hk == 1 ? L.span[t] : 0;
+            fc.offset[0] = class_offset(own);
+            fc.scale[0] = class_scale(own);
+            fc.offset[1] = class_offset(ghost);
+            fc.scale[1] = class_scale(ghost);
             fc.sfL = sfL;
             fc.slotL = A.slot_off[t] + h * 3 * nf;
             fc.slotR = -1;
@@ -738,7 +803,7 @@ __global__ void __launch_bounds__(kThreads) side_fill_kernel(LogicalMesh L, Fill
     fc.slotL = A.slot_off[t];
     const v2 PR[4] = {corner(L, r, 0), corner(L, r, 1), corner(L, r, 2), corner(L, r, 3)};
     fc.sfR = side_scale(PR, rs);
-    fc.slotR = hk == 1 ? (L.link[l] == t ? A.slot_off[l] : A.slot_off[l] + 3 * nfr) : A.slot_off[l];
+    fc.slotR = A.slot_off[l] + (hk == 1 ? chain_position(L, t) * 3 * (int64_t)nfr : 0);
     face_matrices(fc, A.key_off, A.c);
     A.faces[A.face_idx[t]] = fc;
 }

@@ -21,16 +21,16 @@ constexpr int kMaxOrder = 16;       // largest order with a compiled kernel inst
 // face normal and tangent of the reference (Face2D_t::compute_geometry with the domain element on
 // the left); a/b the ghost element's first two nodes after face building; sf the ghost's edge
 // scaling factor (half the edge length, src/entities/Element2D_t.cpp:779-790). For mortar sides
-// `slot` is the offset (in values) of the side's projected flux [3][N+1] in the side-flux buffer
-// (a side covered by two half faces adds the one at `slot2`).
+// `slot` is the offset (in values) of the side's projected flux [3][N+1] in the side-flux buffer;
+// a larger side covered by n_faces faces sums n_faces such blocks, one per face in the side's
+// direction, from `slot` on.
 struct BcSide {
     double nx, ny, tx, ty;
     double ax, ay, bx, by;
     double sf;
     int32_t kind;
-    int32_t pad;
+    int32_t n_faces = 1;
     int64_t slot;
-    int64_t slot2 = -1;  // a larger hanging side: the second half face's projected flux
 };
 
 // Element side whose face is non-conforming: its element flux trace is produced by the mortar
@@ -53,26 +53,42 @@ struct MortarFace {
     int64_t mat[4];
 };
 
-// Mortar side classes. A non-conforming face covers the whole element side (offset 0, scale 1),
-// its first half (offset 0, scale 1/2) or its second half (offset 1/2, scale 1/2): the only shapes
-// the conforming and 2:1 hanging faces of this mesh model produce. The offsets and scales computed
-// from the vertices (Face2D_t.cpp:68-81) can miss 1/2 by an ulp; the class snaps them, so one
-// interpolation/projection matrix per (element order, face order, side, class) serves every face.
-// Returns -1 for a side that is none of these.
+// Mortar side classes. Splitting halves sides, so a non-conforming face covers a dyadic piece of an
+// element side: [j / 2^d, (j + 1) / 2^d] (offset j / 2^d, scale 1 / 2^d), class 2^d - 1 + j, with d
+// up to kMaxSpanLevel (class 0 the whole side, 1 and 2 its halves, 3..6 its quarters, ...). The
+// offsets and scales computed from the vertices (Face2D_t.cpp:68-81) can miss these by an ulp; the
+// class snaps them, so one interpolation/projection matrix per (element order, face order, side,
+// class) serves every face. Returns -1 for a side that is none of these.
+constexpr int kMaxSpanLevel = 4;
+constexpr int kMortarClasses = (2 << kMaxSpanLevel) - 1;
 __host__ __device__ inline int mortar_class(double offset, double scale) {
     const double tol = 1e-9;
-    if (fabs(scale - 1.0) <= tol && fabs(offset) <= tol) return 0;
-    if (fabs(scale - 0.5) <= tol) {
-        if (fabs(offset) <= tol) return 1;
-        if (fabs(offset - 0.5) <= tol) return 2;
+    for (int d = 0; d <= kMaxSpanLevel; ++d) {
+        const double w = 1.0 / (double)(1 << d);
+        if (fabs(scale - w) > tol) continue;
+        const double jj = offset / w;
+        const int j = (int)floor(jj + 0.5);
+        if (j < 0 || j >= (1 << d) || fabs(offset - j * w) > tol) return -1;
+        return (1 << d) - 1 + j;
     }
     return -1;
 }
+__host__ __device__ inline int class_level(int cls) {
+    int d = 0;
+    while ((2 << d) - 1 <= cls) ++d;
+    return d;
+}
+__host__ __device__ inline int class_index(int cls) { return cls + 1 - (1 << class_level(cls)); }
+__host__ __device__ inline int make_class(int d, int j) { return (1 << d) - 1 + j; }
+__host__ __device__ inline double class_offset(int cls) {
+    return (double)class_index(cls) / (double)(1 << class_level(cls));
+}
+__host__ __device__ inline double class_scale(int cls) { return 1.0 / (double)(1 << class_level(cls)); }
 // Index of a matrix in the table of mortar matrices (which = 0 interpolation, 1 projection).
 __host__ __device__ inline int64_t mortar_matrix_key(int Ne, int Nf, int side, int cls, int which) {
-    return ((((int64_t)Ne * (kMaxOrder + 1) + Nf) * 2 + side) * 3 + cls) * 2 + which;
+    return ((((int64_t)Ne * (kMaxOrder + 1) + Nf) * 2 + side) * kMortarClasses + cls) * 2 + which;
 }
-constexpr int64_t kMortarMatrixKeys = (int64_t)(kMaxOrder + 1) * (kMaxOrder + 1) * 12;
+constexpr int64_t kMortarMatrixKeys = (int64_t)(kMaxOrder + 1) * (kMaxOrder + 1) * 4 * kMortarClasses;
 
 // Per-order constant tables in device memory, packed as Dhat[Nf*Nf], l(-1)[Nf], l(+1)[Nf], w[Nf],
 // x[Nf], 1/w[Nf], then the modal matrix poly[Nf*Nf] used by the estimator and the transpose
@@ -207,15 +223,17 @@ struct GroupSpan {
 // The mesh's element-side topology in the caller's element order (the curve order), kept in device
 // memory so that an adaptation can rebuild the mesh there (adapt.hip). link[4e+s] is the element
 // side whose trace is the exterior state of side s of element e (a neighbour, or the donor side of
-// a periodic pair), or -(1 + ghost); hang: 0 conforming, 1 a small side whose exterior is half of
-// a larger element's side (link = that side), 2 the larger side (link = its small side on the half
-// that starts at the side's first corner, hang2 = the other one).
+// a periodic pair), or -(1 + ghost); hang: 0 conforming, 1 a small side whose exterior is a piece
+// of a larger element's side (link = that side, span = the piece's mortar class in the larger
+// side's direction, hang2 = the next small side along the larger side, -1 after the last), 2 the
+// larger side (link = its first small side, the one at the side's first corner; hang2 = -1).
 struct LogicalMesh {
     int64_t K = 0, n_ghost = 0;
     double *verts = nullptr;                      // [K][4][2]
     int32_t *order = nullptr, *status = nullptr, *rotation = nullptr, *level = nullptr;
     int64_t *link = nullptr, *hang2 = nullptr;    // [4K]
     int8_t *hang = nullptr, *periodic = nullptr;  // [4K]
+    int8_t *span = nullptr;                       // [4K]
     double *est = nullptr;                        // [K][4] last estimate: error, sigma, refine, coarsen
     int64_t *dev_of = nullptr;                    // [K] index in the order-grouped device layout
     BcSide *ghost = nullptr;                      // [n_ghost] ghost records, in boundary-bar order
@@ -223,7 +241,7 @@ struct LogicalMesh {
     int8_t *ghost_zero = nullptr;                 // [n_ghost] storage resized to zero (the scale is 0)
 };
 struct LogicalStore {
-    Arena verts, order, status, rotation, level, link, hang2, hang, periodic, est, dev_of, ghost, ghost_side,
+    Arena verts, order, status, rotation, level, link, hang2, hang, periodic, span, est, dev_of, ghost, ghost_side,
         ghost_zero;
 };
 
@@ -233,11 +251,12 @@ struct AdaptCounts {
     unsigned long long dofs, nodes1d;            // sum (N+1)^2 and sum (N+1) of the new mesh
     unsigned long long order_count[kMaxOrder + 1];
     unsigned long long n_rec, n_slot, n_face;    // mortar records, side-flux values, mortar faces
-    unsigned long long conforming_pairs, larger_sides, mortar_nodes, bad_face;
-    unsigned long long periodic_faces;           // faces of periodic sides (two on a larger side)
+    unsigned long long conforming_pairs, mortar_nodes, bad_face;
+    unsigned long long periodic_faces;           // faces of periodic sides (one per face on the side)
+    unsigned long long hanging_faces;            // faces of interior larger sides
 };
-// refused bits
-constexpr unsigned kRefuseOrder = 1, kRefuseFourFaces = 4;
+// refused bits: a raised order beyond the tables, a face smaller than 1 / 2^kMaxSpanLevel of its side
+constexpr unsigned kRefuseOrder = 1, kRefuseSpan = 4;
 
 // Scratch of the adaptation kernels (capacities kept across adaptations).
 struct AdaptWork {

@@ -271,8 +271,8 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
                 if (rec.kind == kKindMortar) {  // projected by the mortar kernel (already x sf)
                     const double *sfl = side_flux + rec.slot + k;
                     double f0 = sfl[0], f1 = sfl[NF], f2 = sfl[2 * NF];
-                    if (rec.slot2 >= 0) {  // a side covered by two half faces
-                        const double *s2 = side_flux + rec.slot2 + k;
+                    for (int m = 1; m < rec.n_faces; ++m) {  // a side covered by several faces
+                        const double *s2 = sfl + m * 3 * NF;
                         f0 += s2[0];
                         f1 += s2[NF];
                         f2 += s2[2 * NF];
@@ -661,8 +661,8 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
                 if (rec.kind == kKindMortar) {
                     const float *sfl = side_flux + rec.slot + k;
                     float f0 = sfl[0], f1 = sfl[NF], f2 = sfl[2 * NF];
-                    if (rec.slot2 >= 0) {
-                        const float *s2 = side_flux + rec.slot2 + k;
+                    for (int m = 1; m < rec.n_faces; ++m) {
+                        const float *s2 = sfl + m * 3 * NF;
                         f0 += s2[0];
                         f1 += s2[NF];
                         f2 += s2[2 * NF];

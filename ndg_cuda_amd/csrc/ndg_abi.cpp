@@ -100,6 +100,7 @@ struct Topology {
     std::vector<char> periodic_link;
     std::vector<char> hang_kind;
     std::vector<int64_t> hang_second;
+    std::vector<int8_t> span;
     std::vector<BcSide> bcs;       // boundary (ghost) records
     std::vector<int64_t> bc_bar;   // bar of each record
     std::vector<int64_t> bar_side; // element side of each bar
@@ -152,11 +153,13 @@ struct ndg_mesh {
     std::vector<double> host_verts;          // [K][4][2], caller's order
     std::vector<int64_t> link;               // exterior side of each element side (see ndg_mesh_create)
     std::vector<char> periodic_link;
-    // Hanging (2:1) sides: 1 = a small side whose exterior is half of a larger element's side
-    // (link = that side), 2 = the larger side (link = its first small side, hang_second = the
-    // second, in the order of the larger side).
+    // Hanging sides: 1 = a small side whose exterior is a piece of a larger element's side (link =
+    // that side, span = the piece's mortar class in the larger side's direction, hang_second = the
+    // next small side along the larger side or -1), 2 = the larger side (link = its first small
+    // side, at its first corner).
     std::vector<char> hang_kind;
     std::vector<int64_t> hang_second;
+    std::vector<int8_t> span;
     std::vector<BcSide> bc_base;             // boundary (ghost) records
     std::vector<int64_t> bc_bar;             // bar (ghost) of each record
     std::vector<int64_t> bar_side;           // element side of each bar
@@ -304,15 +307,14 @@ bool ulp2_equal(double a, double b) {
 // class of mortar_class, interpolation or projection), for every order up to N_max: data holds
 // them back to back, off[key] their offsets (-1 for the plain copies: equal orders, whole side).
 void build_mortar_table(const Tables &t, int N_max, std::vector<double> &data, std::vector<int64_t> &off) {
-    const double cls_offset[3] = {0.0, 0.0, 0.5}, cls_scale[3] = {1.0, 0.5, 0.5};
     data.clear();
     off.assign((size_t)kMortarMatrixKeys, -1);
     for (int Ne = 1; Ne <= N_max; ++Ne)
         for (int Nf = Ne; Nf <= N_max; ++Nf)
             for (int side = 0; side < 2; ++side)
-                for (int cls = 0; cls < 3; ++cls) {
+                for (int cls = 0; cls < kMortarClasses; ++cls) {
                     if (Ne == Nf && cls == 0) continue;
-                    const double offset = cls_offset[cls], scale = cls_scale[cls];
+                    const double offset = class_offset(cls), scale = class_scale(cls);
                     const auto &xf = t.tab[NDG_TABLE_NODES][Nf], &wf = t.tab[NDG_TABLE_WEIGHTS][Nf];
                     const auto &xe = t.tab[NDG_TABLE_NODES][Ne], &we = t.tab[NDG_TABLE_WEIGHTS][Ne];
                     const auto &le = t.tab[NDG_TABLE_BARYCENTRIC_WEIGHTS][Ne];
@@ -396,7 +398,7 @@ ndg_status fit_view(Arena &a, T **view, int64_t n, const char *what) {
 
 void logical_free(LogicalStore &s) {
     for (Arena *a : {&s.verts, &s.order, &s.status, &s.rotation, &s.level, &s.link, &s.hang2, &s.hang, &s.periodic,
-                     &s.est, &s.dev_of, &s.ghost, &s.ghost_side, &s.ghost_zero})
+                     &s.span, &s.est, &s.dev_of, &s.ghost, &s.ghost_side, &s.ghost_zero})
         arena_free(*a);
 }
 
@@ -579,12 +581,20 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
         bcs.push_back(rec);
         return rec.slot;
     };
-    // A larger hanging side sums two projected half-face fluxes (slot, slot2).
+    // A larger hanging side sums the projected fluxes of its faces, one block per small side
+    // along it (the chain from link through hang_second).
+    auto chain_pos = [&](int64_t small) {
+        int64_t pos = 0;
+        for (int64_t c = link[(size_t)link[small]]; c != small && c >= 0; c = m->hang_second[(size_t)c]) ++pos;
+        return pos;
+    };
     for (int64_t es = 0; es < 4 * K; ++es)
         if (m->hang_kind[es] == 2) {
+            int n = 0;
+            for (int64_t c = link[es]; c >= 0; c = m->hang_second[(size_t)c]) ++n;
             add_mortar_side(es);
-            bcs.back().slot2 = n_slot;
-            n_slot += 3 * (int64_t)(order[es / 4] + 1);
+            bcs.back().n_faces = n;
+            n_slot += 3 * (int64_t)(n - 1) * (order[es / 4] + 1);
         }
     for (int64_t e = 0; e < K; ++e)
         for (int s = 0; s < 4; ++s) {
@@ -617,11 +627,11 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
             fc.sfL = side_scale(PL, s);
             fc.slotL = add_mortar_side(es);
             fc.slotR = -1;
-            if (hk == 1) {  // half of the larger side l
+            if (hk == 1) {  // a piece of the larger side l
                 v2 PR[4] = {corner(r, 0), corner(r, 1), corner(r, 2), corner(r, 3)};
                 fc.sfR = side_scale(PR, rs);
                 const BcSide &big = bcs[(size_t)mortar_rec[l]];
-                fc.slotR = link[l] == es ? big.slot : big.slot2;
+                fc.slotR = big.slot + chain_pos(es) * 3 * (int64_t)(order[r] + 1);
             } else if (!periodic) {
                 v2 PR[4] = {corner(r, 0), corner(r, 1), corner(r, 2), corner(r, 3)};
                 fc.sfR = side_scale(PR, rs);
@@ -709,6 +719,7 @@ ndg_status fit_logical(LogicalStore &ls, LogicalMesh &L, int64_t K, int64_t G) {
     if ((st = fit_view(ls.hang2, &L.hang2, 4 * K, "hipMalloc logical mesh")) != NDG_OK) return st;
     if ((st = fit_view(ls.hang, &L.hang, 4 * K, "hipMalloc logical mesh")) != NDG_OK) return st;
     if ((st = fit_view(ls.periodic, &L.periodic, 4 * K, "hipMalloc logical mesh")) != NDG_OK) return st;
+    if ((st = fit_view(ls.span, &L.span, 4 * K, "hipMalloc logical mesh")) != NDG_OK) return st;
     if ((st = fit_view(ls.est, &L.est, 4 * K, "hipMalloc logical mesh")) != NDG_OK) return st;
     if ((st = fit_view(ls.dev_of, &L.dev_of, K, "hipMalloc logical mesh")) != NDG_OK) return st;
     if ((st = fit_view(ls.ghost, &L.ghost, G, "hipMalloc logical mesh")) != NDG_OK) return st;
@@ -755,6 +766,7 @@ ndg_status ensure_logical(ndg_mesh *m) {
     if ((st = upload_into(L.level, m->split_level, "upload logical mesh")) != NDG_OK) return st;
     if ((st = upload_into(L.link, m->link, "upload logical mesh")) != NDG_OK) return st;
     if ((st = upload_into(L.hang2, m->hang_second, "upload logical mesh")) != NDG_OK) return st;
+    if ((st = upload_into(L.span, m->span, "upload logical mesh")) != NDG_OK) return st;
     if ((st = upload_into(L.hang, m->hang_kind, "upload logical mesh")) != NDG_OK) return st;
     if ((st = upload_into(L.periodic, m->periodic_link, "upload logical mesh")) != NDG_OK) return st;
     if ((st = upload_into(L.est, est, "upload logical mesh")) != NDG_OK) return st;
@@ -783,6 +795,7 @@ ndg_status ensure_host(ndg_mesh *m) {
     if ((st = download(m->split_level, L.level, L.K, s, "download logical mesh")) != NDG_OK) return st;
     if ((st = download(m->link, L.link, 4 * L.K, s, "download logical mesh")) != NDG_OK) return st;
     if ((st = download(m->hang_second, L.hang2, 4 * L.K, s, "download logical mesh")) != NDG_OK) return st;
+    if ((st = download(m->span, L.span, 4 * L.K, s, "download logical mesh")) != NDG_OK) return st;
     if ((st = download(hang, L.hang, 4 * L.K, s, "download logical mesh")) != NDG_OK) return st;
     if ((st = download(per, L.periodic, 4 * L.K, s, "download logical mesh")) != NDG_OK) return st;
     if ((st = download(est, L.est, 4 * L.K, s, "download logical mesh")) != NDG_OK) return st;
@@ -1009,6 +1022,7 @@ ndg_status build_topology(const HostMesh &h, Topology &t) {
     std::vector<int64_t> &hang_second = t.hang_second;
     hang_kind.assign((size_t)(4 * K), 0);
     hang_second.assign((size_t)(4 * K), -1);
+    t.span.assign((size_t)(4 * K), 0);
     {
         // sides still unlinked, by start node (the buckets above, linked sides skipped)
         for (int64_t es = 0; es < 4 * K; ++es) {
@@ -1029,10 +1043,12 @@ ndg_status build_topology(const HostMesh &h, Topology &t) {
                 }
                 if (s2 < 0) continue;
                 hang_kind[es] = 2;
-                link[es] = s2;          // the half starting at a first
-                hang_second[es] = s1;
+                link[es] = s2;          // the half starting at a first, then s1
+                hang_second[s2] = s1;
                 hang_kind[s1] = hang_kind[s2] = 1;
                 link[s1] = link[s2] = es;
+                t.span[s2] = (int8_t)make_class(1, 0);
+                t.span[s1] = (int8_t)make_class(1, 1);
                 n_faces += 2;           // the two half faces
                 break;
             }
@@ -1093,6 +1109,7 @@ extern "C" ndg_status ndg_mesh_create(const ndg_mesh_desc *desc, const ndg_table
     m->periodic_link = std::move(topo.periodic_link);
     m->hang_kind = std::move(topo.hang_kind);
     m->hang_second = std::move(topo.hang_second);
+    m->span = std::move(topo.span);
     m->bc_base = std::move(topo.bcs);
     m->bc_bar = std::move(topo.bc_bar);
     m->bar_side = std::move(topo.bar_side);
@@ -1671,6 +1688,7 @@ struct MeshHostState {
     HostMesh host;
     std::vector<int64_t> link, hang_second, bc_bar, bar_side, dev_of, host_off;
     std::vector<char> periodic_link, hang_kind, bar_zero;
+    std::vector<int8_t> span;
     std::vector<BcSide> bc_base;
     std::vector<int32_t> status, rotation, split_level, order;
     std::vector<double> host_verts;
@@ -1681,6 +1699,7 @@ void swap_host_state(ndg_mesh *m, MeshHostState &s) {
     std::swap(m->host, s.host);
     std::swap(m->link, s.link);
     std::swap(m->hang_second, s.hang_second);
+    std::swap(m->span, s.span);
     std::swap(m->bc_bar, s.bc_bar);
     std::swap(m->bar_side, s.bar_side);
     std::swap(m->dev_of, s.dev_of);
@@ -1917,6 +1936,7 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
     mesh->periodic_link = std::move(topo.periodic_link);
     mesh->hang_kind = std::move(topo.hang_kind);
     mesh->hang_second = std::move(topo.hang_second);
+    mesh->span = std::move(topo.span);
     mesh->bc_base = std::move(topo.bcs);
     mesh->bc_bar = std::move(topo.bc_bar);
     mesh->bar_side = std::move(topo.bar_side);
@@ -2015,8 +2035,8 @@ ndg_status adapt_on_device(ndg_mesh *mesh, int32_t N_max, int32_t max_split_leve
     mesh->host_stale = true;  // the logical copy of the estimate changed
     if (c.refused & kRefuseOrder)
         return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a raised order exceeds the compiled range or the tables");
-    if (c.n_split > 0 && (c.refused & kRefuseFourFaces))
-        return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a split would leave more than two faces on a side");
+    if (c.n_split > 0 && (c.refused & kRefuseSpan))
+        return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a split would leave a face smaller than 1/16 of its element side");
     if (n_changed) *n_changed = (int64_t)(c.n_split + c.n_raise);
     if (c.n_split + c.n_raise == 0) return NDG_OK;
     const bool p_only = c.n_split == 0;
@@ -2110,7 +2130,7 @@ ndg_status adapt_on_device(ndg_mesh *mesh, int32_t N_max, int32_t max_split_leve
     d = nd;
     mesh->active ^= 1;
     mesh->lactive = nl;
-    mesh->n_faces = (int64_t)c.conforming_pairs + Gn + (int64_t)c.periodic_faces + 2 * (int64_t)c.larger_sides;
+    mesh->n_faces = (int64_t)c.conforming_pairs + Gn + (int64_t)c.periodic_faces + (int64_t)c.hanging_faces;
     mesh->n_dofs = (int64_t)c.dofs;
     mesh->host_stale = true;
     mesh->host_nodes_valid = false;
@@ -2262,16 +2282,17 @@ extern "C" ndg_status ndg_mesh_set_adapt_mode(ndg_mesh *mesh, int32_t mode) {
     return NDG_OK;
 }
 
-extern "C" ndg_status ndg_mesh_sides(ndg_mesh *mesh, int64_t *link, int8_t *hang, int64_t *hang_second,
-                                     int8_t *periodic) {
+extern "C" ndg_status ndg_mesh_sides(ndg_mesh *mesh, int64_t *link, int8_t *hang, int64_t *next_small,
+                                     int8_t *periodic, int8_t *span) {
     if (!mesh) return fail(NDG_ERR_INVALID, "ndg_mesh_sides: null mesh");
     if (const ndg_status hs = ensure_host(mesh); hs != NDG_OK) return hs;
     const size_t n = mesh->link.size();
     for (size_t i = 0; i < n; ++i) {
         if (link) link[i] = mesh->link[i];
         if (hang) hang[i] = (int8_t)mesh->hang_kind[i];
-        if (hang_second) hang_second[i] = mesh->hang_second[i];
+        if (next_small) next_small[i] = mesh->hang_second[i];
         if (periodic) periodic[i] = (int8_t)mesh->periodic_link[i];
+        if (span) span[i] = mesh->span[i];
     }
     return NDG_OK;
 }

@@ -194,14 +194,16 @@ class Mesh2D_t:
         check(lib().ndg_mesh_set_adapt_mode(self._h, modes[mode]))
 
     def sides(self):
-        """Element-side topology, 4 entries per element: link (exterior side 4r+rs, or -(1+ghost)),
-        hang (0 conforming, 1 small hanging side, 2 larger side), the larger side's second small
-        side, and the periodic flag."""
+        """Element-side topology, 4 entries per element: link (exterior side 4r+rs, or -(1+ghost);
+        a larger side's first small side), hang (0 conforming, 1 small hanging side, 2 larger
+        side), a small side's next small side along its larger side (-1 after the last), the
+        periodic flag, and a small side's piece of its larger side as 2^d - 1 + j for
+        [j / 2^d, (j + 1) / 2^d]."""
         n = 4 * self.n_elements_
-        link, hang2 = np.empty(n, dtype=np.int64), np.empty(n, dtype=np.int64)
-        hang, per = np.empty(n, dtype=np.int8), np.empty(n, dtype=np.int8)
-        check(lib().ndg_mesh_sides(self._h, ptr(link), ptr(hang), ptr(hang2), ptr(per)))
-        return link, hang, hang2, per
+        link, nxt = np.empty(n, dtype=np.int64), np.empty(n, dtype=np.int64)
+        hang, per, span = np.empty(n, dtype=np.int8), np.empty(n, dtype=np.int8), np.empty(n, dtype=np.int8)
+        check(lib().ndg_mesh_sides(self._h, ptr(link), ptr(hang), ptr(nxt), ptr(per), ptr(span)))
+        return link, hang, nxt, per, span
 
     def adapt(self, N_max=None, max_split_level=None):
         """Mesh2D_t::adapt: estimate, then either split the elements that would h-refine (when

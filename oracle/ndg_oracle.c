@@ -653,71 +653,64 @@ static o_mesh *mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, co
             f->sides[0] = 0; f->sides[1] = -1;
         }
     }
-    /* Hanging (2:1) sides: a domain side a -> b without a neighbour or a ghost, covered by the
-     * unmatched sides b -> m and m -> a of two elements meeting at its midpoint m (the configuration
-     * h-refinement leaves behind, src/meshes/Mesh2D_t.cpp:5661-5985). Those two sides' faces take
-     * the larger element as their right element (its side then has two faces, the one starting at
-     * a first) and the larger side's own face is dropped. */
-    int64_t (*hang)[2] = (int64_t(*)[2])malloc((size_t)(4 * n_quads + 1) * sizeof(int64_t[2]));
+    /* Hanging sides: a side a -> b without a neighbour or a ghost (an element side, or a periodic
+     * ghost's own face), covered by a chain of unmatched faces b -> m1 -> ... -> a along it, made
+     * by the sides of smaller elements or half ghosts (the configuration h-refinement leaves behind,
+     * src/meshes/Mesh2D_t.cpp:5661-5985, any number of faces per side). Those faces take the larger
+     * side's element as their right element (its side then lists them from a to b) and the larger
+     * side's own face is dropped. */
+    int64_t *hpool = NULL, hpool_n = 0, hpool_cap = 0;
+    int64_t *hstart = (int64_t *)malloc((size_t)(4 * n_quads + n_bars + 1) * sizeof(int64_t));
+    int *hcount = (int *)calloc((size_t)(4 * n_quads + n_bars + 1), sizeof(int));
     char *dropped = (char *)calloc((size_t)m->F + 1, 1);
-    for (int64_t es = 0; es < 4 * n_quads; ++es) hang[es][0] = hang[es][1] = -1;
-    for (int64_t i = 0; i < n_quads; ++i)
-        for (int j = 0; j < 4; ++j) {
+    for (int64_t q = 0; q < 4 * n_quads + n_bars; ++q) hstart[q] = -1;
+    for (int64_t i = 0; i < m->n_elements_total; ++i) {
+        const int ghost = i >= n_quads;
+        if (ghost && bar_type[i - n_quads] != O_BC_PERIODIC) continue;
+        for (int j = 0; j < (ghost ? 1 : 4); ++j) {
             const int64_t fb = e2f[i][j];
-            if (m->fa[fb].elements[1] != -1 || m->fa[fb].elements[0] != i) continue;
+            if (fb < 0 || m->fa[fb].elements[1] != -1 || m->fa[fb].elements[0] != i) continue;
             const int64_t a = m->fa[fb].nodes[0], b = m->fa[fb].nodes[1];
-            for (int k = 0; k < n2f_count[b]; ++k) {
-                const int64_t f1 = n2f[b][k];
-                if (m->fa[f1].nodes[0] != b || m->fa[f1].elements[1] != -1 || f1 == fb) continue;
-                const int64_t mid = m->fa[f1].nodes[1];
-                int64_t f2 = -1;
-                for (int q = 0; q < n2f_count[mid]; ++q) {
-                    const o_face *c = &m->fa[n2f[mid][q]];
-                    if (c->nodes[0] == mid && c->nodes[1] == a && c->elements[1] == -1) f2 = n2f[mid][q];
+            const v2 pa = m->xy[a], pb = m->xy[b], ab = sub(pb, pa);
+            const double len2 = ab.x * ab.x + ab.y * ab.y, len = sqrt(len2);
+            int64_t chain[64];
+            int n = 0;
+            int64_t x = b;
+            double tx = 1.0;
+            while (x != a && n < 64) {
+                int64_t found = -1;
+                double ty = 0.0;
+                for (int k = 0; k < n2f_count[x] && found < 0; ++k) {
+                    const int64_t fk = n2f[x][k];
+                    const o_face *c = &m->fa[fk];
+                    if (fk == fb || c->nodes[0] != x || c->elements[1] != -1 || dropped[fk]) continue;
+                    const int64_t y = c->nodes[1];
+                    const v2 ay = sub(m->xy[y], pa);
+                    const double t = y == a ? 0.0 : (ay.x * ab.x + ay.y * ab.y) / len2;
+                    const double off = fabs(ay.x * ab.y - ay.y * ab.x) / len;
+                    if (y != a && (off > 1e-12 * len || t <= 1e-12 || t >= tx - 1e-12)) continue;
+                    found = fk;
+                    ty = t;
                 }
-                if (f2 < 0) continue;
-                const double mx = (m->xy[a].x + m->xy[b].x) / 2, my = (m->xy[a].y + m->xy[b].y) / 2;
-                const double len = mag(sub(m->xy[b], m->xy[a]));
-                if (fabs(m->xy[mid].x - mx) > 1e-12 * len || fabs(m->xy[mid].y - my) > 1e-12 * len) continue;
-                m->fa[f1].elements[1] = i; m->fa[f1].sides[1] = j;
-                m->fa[f2].elements[1] = i; m->fa[f2].sides[1] = j;
-                hang[4 * i + j][0] = f2;
-                hang[4 * i + j][1] = f1;
-                dropped[fb] = 1;
-                break;
+                if (found < 0) break;
+                chain[n++] = found;
+                x = m->fa[found].nodes[1];
+                tx = ty;
             }
-        }
-    /* A periodic ghost a -> b covering the sides b -> m and m -> a of two elements (the ghost of a
-     * larger side across the period whose partner split): those faces take the ghost as their
-     * right element, the one ending at a first, and the ghost's own face is dropped. */
-    int64_t (*ghang)[2] = (int64_t(*)[2])malloc((size_t)(n_bars + 1) * sizeof(int64_t[2]));
-    for (int64_t b = 0; b < n_bars; ++b) {
-        ghang[b][0] = ghang[b][1] = -1;
-        const int64_t i = n_quads + b, fb = e2f[i][0];
-        if (bar_type[b] != O_BC_PERIODIC || fb < 0 || m->fa[fb].elements[0] != i || m->fa[fb].elements[1] != -1)
-            continue;
-        const int64_t a = m->fa[fb].nodes[0], bb = m->fa[fb].nodes[1];
-        for (int k = 0; k < n2f_count[bb]; ++k) {
-            const int64_t f1 = n2f[bb][k];
-            const o_face *c1 = &m->fa[f1];
-            if (c1->nodes[0] != bb || c1->elements[1] != -1 || c1->elements[0] >= n_quads) continue;
-            const int64_t mid = c1->nodes[1];
-            int64_t f2 = -1;
-            for (int q = 0; q < n2f_count[mid]; ++q) {
-                const o_face *c = &m->fa[n2f[mid][q]];
-                if (c->nodes[0] == mid && c->nodes[1] == a && c->elements[1] == -1 && c->elements[0] < n_quads)
-                    f2 = n2f[mid][q];
+            if (x != a || n < 2) continue;
+            if (hpool_n + n > hpool_cap) {
+                hpool_cap = 2 * (hpool_n + n) + 64;
+                hpool = (int64_t *)realloc(hpool, (size_t)hpool_cap * sizeof(int64_t));
             }
-            if (f2 < 0) continue;
-            const double mx = (m->xy[a].x + m->xy[bb].x) / 2, my = (m->xy[a].y + m->xy[bb].y) / 2;
-            const double len = mag(sub(m->xy[bb], m->xy[a]));
-            if (fabs(m->xy[mid].x - mx) > 1e-12 * len || fabs(m->xy[mid].y - my) > 1e-12 * len) continue;
-            m->fa[f1].elements[1] = i; m->fa[f1].sides[1] = 0;
-            m->fa[f2].elements[1] = i; m->fa[f2].sides[1] = 0;
-            ghang[b][0] = f2;
-            ghang[b][1] = f1;
+            const int64_t key = ghost ? 4 * n_quads + (i - n_quads) : 4 * i + j;
+            hstart[key] = hpool_n;
+            hcount[key] = n;
+            for (int q = n - 1; q >= 0; --q) {  /* from a to b */
+                m->fa[chain[q]].elements[1] = i;
+                m->fa[chain[q]].sides[1] = j;
+                hpool[hpool_n++] = chain[q];
+            }
             dropped[fb] = 1;
-            break;
         }
     }
     {   /* compact the face array */
@@ -731,10 +724,7 @@ static o_mesh *mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, co
         for (int64_t i = 0; i < m->n_elements_total; ++i)
             for (int j = 0; j < (i < n_quads ? 4 : 1); ++j)
                 if (e2f[i][j] >= 0) e2f[i][j] = remap[e2f[i][j]];
-        for (int64_t es = 0; es < 4 * n_quads; ++es)
-            if (hang[es][0] >= 0) { hang[es][0] = remap[hang[es][0]]; hang[es][1] = remap[hang[es][1]]; }
-        for (int64_t b = 0; b < n_bars; ++b)
-            if (ghang[b][0] >= 0) { ghang[b][0] = remap[ghang[b][0]]; ghang[b][1] = remap[ghang[b][1]]; }
+        for (int64_t q = 0; q < hpool_n; ++q) hpool[q] = remap[hpool[q]];
         free(remap);
     }
     free(dropped);
@@ -798,11 +788,11 @@ static o_mesh *mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, co
         if (i < n_quads) {
             elem_alloc_domain(e);
             for (int s = 0; s < 4; ++s) {
-                if (hang[4 * i + s][0] >= 0) {
-                    e->nf[s] = 2;
-                    e->faces[s] = (int64_t *)malloc(2 * sizeof(int64_t));
-                    e->faces[s][0] = hang[4 * i + s][0];
-                    e->faces[s][1] = hang[4 * i + s][1];
+                if (hcount[4 * i + s] > 0) {
+                    const int n = hcount[4 * i + s];
+                    e->nf[s] = n;
+                    e->faces[s] = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+                    for (int q = 0; q < n; ++q) e->faces[s][q] = hpool[hstart[4 * i + s] + q];
                     continue;
                 }
                 e->nf[s] = 1;
@@ -814,12 +804,12 @@ static o_mesh *mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, co
             element_geometry(e, P, t->tab[O_NODES][e->N]);
         } else {
             elem_alloc_ghost(e);
-            const int64_t b = i - n_quads;
-            if (ghang[b][0] >= 0) {
-                e->nf[0] = 2;
-                e->faces[0] = (int64_t *)malloc(2 * sizeof(int64_t));
-                e->faces[0][0] = ghang[b][0];
-                e->faces[0][1] = ghang[b][1];
+            const int64_t key = 4 * n_quads + (i - n_quads);
+            if (hcount[key] > 0) {
+                const int n = hcount[key];
+                e->nf[0] = n;
+                e->faces[0] = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+                for (int q = 0; q < n; ++q) e->faces[0][q] = hpool[hstart[key] + q];
             } else {
                 e->nf[0] = 1;
                 e->faces[0] = (int64_t *)malloc(sizeof(int64_t));
@@ -843,8 +833,9 @@ static o_mesh *mesh_create(const o_tables *t, int initial_N, int64_t n_nodes, co
         face_geometry(fc, fn, en);
     }
     free(e2f);
-    free(hang);
-    free(ghang);
+    free(hpool);
+    free(hstart);
+    free(hcount);
     for (int64_t k = 0; k < m->n_if; ++k) {
         m->bar_origin[2 * (m->if_dest[k] - n_quads)] = m->if_origin[k];
         m->bar_origin[2 * (m->if_dest[k] - n_quads) + 1] = m->if_side[k];
@@ -876,6 +867,10 @@ void o_mesh_destroy(o_mesh *m) {
 
 int64_t o_mesh_n_elements(const o_mesh *m) { return m->K; }
 int64_t o_mesh_n_faces(const o_mesh *m) { return m->F; }
+void o_mesh_side_faces(const o_mesh *m, int32_t *nf) {
+    for (int64_t i = 0; i < m->K; ++i)
+        for (int s = 0; s < 4; ++s) nf[4 * i + s] = m->el[i].nf[s];
+}
 int64_t o_mesh_n_ghosts(const o_mesh *m) { return m->n_elements_total - m->K; }
 int64_t o_mesh_n_nodes_total(const o_mesh *m) {
     int64_t n = 0;
@@ -1811,9 +1806,10 @@ static int64_t edge_mid_find(const int64_t *keys, int64_t n, int64_t a, int64_t 
  * orders; a boundary ghost on a split side becomes two ghosts (geometry recomputed), a ghost whose
  * order changes is resized and its geometry recomputed; a periodic ghost splits with its origin
  * (the element across the period). The new topology (faces, 2:1 hanging sides, also across the
- * period) is built from the new element and ghost lists as at mesh creation. A split that would
- * leave a side covered by more than two faces is not restated (returns -2, nothing changed).
- * Returns the number of split plus raised elements. */
+ * period, any number of faces per side) is built from the new element and ghost lists as at mesh
+ * creation. A split that would leave a face smaller than 1/16 of its larger side is refused
+ * (returns -2, nothing changed), as the device path does. Returns the number of split plus raised
+ * elements. */
 static int64_t adapt_split(o_mesh *m, int N_max, int max_split_level) {
     const int64_t K = m->K, NB = m->n_elements_total - m->K;
     char *split = (char *)calloc((size_t)K + 1, 1);
@@ -1824,24 +1820,23 @@ static int64_t adapt_split(o_mesh *m, int N_max, int max_split_level) {
         if (e->refine && e->sigma < 1.0 && e->split_level < max_split_level) { split[i] = 1; ++n_split; }
         else if (e->refine && e->sigma >= 1.0 && e->N + 2 <= N_max) { raise[i] = 1; ++n_raise; }
     }
-    /* limit of this restatement: a half of a larger unsplit side, across the interior or across
-     * the period (the small element's ghost then covers two sides), would be split again */
+    /* limit shared with the device path: a small side covering 1/16 of a larger side (a face scale
+     * of 1/16 on the larger side), across the interior or the period, is not split again while the
+     * larger element stays */
     for (int64_t i = 0; i < K; ++i) {
         if (!split[i]) continue;
         const o_elem *e = &m->el[i];
         for (int s = 0; s < 4; ++s) {
             if (e->nf[s] != 1) continue;
             const o_face *f = &m->fa[e->faces[s][0]];
-            const int64_t other = f->elements[0] == i ? f->elements[1] : f->elements[0];
-            if (other < K && !split[other]) {
-                const o_elem *nb = &m->el[other];
-                const int ns = (int)(f->elements[0] == i ? f->sides[1] : f->sides[0]);
-                if (nb->nf[ns] == 2) { free(split); free(raise); return -2; }
+            const int side = f->elements[0] == i ? 1 : 0;
+            const int64_t other = f->elements[side];
+            int64_t larger = other;
+            if (other >= K) {
+                if (m->bar_type[other - K] != O_BC_PERIODIC) continue;
+                larger = m->bar_origin[2 * (other - K)];
             }
-            if (other >= K && m->bar_type[other - K] == O_BC_PERIODIC && m->el[other].nf[0] == 2 &&
-                !split[m->bar_origin[2 * (other - K)]]) {
-                free(split); free(raise); return -2;
-            }
+            if (f->scale[side] < 1.5 / 16 && !split[larger]) { free(split); free(raise); return -2; }
         }
     }
     /* new nodes: existing, then per split element its side midpoints (shared) and centre, then the
@@ -1852,13 +1847,23 @@ static int64_t adapt_split(o_mesh *m, int N_max, int max_split_level) {
     int64_t n_nodes = m->n_nodes;
     int64_t *mids = (int64_t *)malloc((size_t)(4 * K + 4 * n_split + 2 * NB + 1) * 3 * sizeof(int64_t));
     int64_t n_mids = 0;
-    /* midpoints already present: hanging sides (two faces), of elements and of periodic ghosts */
+    /* midpoints already present: the node at the middle of a side covered by several faces, of
+     * elements and of periodic ghosts */
     for (int64_t i = 0; i < m->n_elements_total; ++i)
         for (int s = 0; s < (i < K ? 4 : 1); ++s)
-            if (m->el[i].nf[s] == 2) {
+            if (m->el[i].nf[s] >= 2) {
                 const int64_t a = m->el[i].nodes[s], b = m->el[i].nodes[(s + 1) % 4];
+                const double mx = (m->xy[a].x + m->xy[b].x) / 2, my = (m->xy[a].y + m->xy[b].y) / 2;
+                const double len = mag(sub(m->xy[b], m->xy[a]));
+                int64_t md = -1;
+                for (int q = 0; q < m->el[i].nf[s] && md < 0; ++q)
+                    for (int r = 0; r < 2; ++r) {
+                        const int64_t v = m->fa[m->el[i].faces[s][q]].nodes[r];
+                        if (fabs(m->xy[v].x - mx) <= 1e-12 * len && fabs(m->xy[v].y - my) <= 1e-12 * len) md = v;
+                    }
+                if (md < 0) continue;
                 mids[3 * n_mids] = a < b ? a : b; mids[3 * n_mids + 1] = a < b ? b : a;
-                mids[3 * n_mids + 2] = m->fa[m->el[i].faces[s][0]].nodes[0];
+                mids[3 * n_mids + 2] = md;
                 ++n_mids;
             }
     const int64_t Knew = K + 3 * n_split;

@@ -53,6 +53,8 @@ o_mesh *o_mesh_create(const o_tables *t, int initial_N,
 void o_mesh_destroy(o_mesh *m);
 int64_t o_mesh_n_elements(const o_mesh *m);
 int64_t o_mesh_n_faces(const o_mesh *m);
+/* faces per element side [K][4] */
+void o_mesh_side_faces(const o_mesh *m, int32_t *nf);
 int64_t o_mesh_n_ghosts(const o_mesh *m);
 /* Sum over domain elements of (N+1)^2: the length of one packed field. */
 int64_t o_mesh_n_nodes_total(const o_mesh *m);

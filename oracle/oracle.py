@@ -36,7 +36,7 @@ def lib():
             "o_tables_get": ([vp, i32, i32, vp], i32),
             "o_mesh_create": ([vp, i32, i64, vp, i64, vp, vp, i64, vp, vp, vp, f64, f64], vp),
             "o_mesh_destroy": ([vp], None), "o_mesh_n_elements": ([vp], i64),
-            "o_mesh_n_faces": ([vp], i64), "o_mesh_n_ghosts": ([vp], i64),
+            "o_mesh_n_faces": ([vp], i64), "o_mesh_side_faces": ([vp, vp], None), "o_mesh_n_ghosts": ([vp], i64),
             "o_mesh_n_nodes_total": ([vp], i64), "o_mesh_get_N": ([vp, vp], None),
             "o_initial_conditions_plane": ([vp], None),
             "o_initial_conditions_pulses": ([vp, i32, vp], None),
@@ -124,6 +124,12 @@ class Mesh:
     @property
     def n_faces(self):
         return lib().o_mesh_n_faces(self.h)
+
+    def side_faces(self):
+        """Faces per element side, shape (K, 4)."""
+        out = np.empty((self.K, 4), dtype=np.int32)
+        lib().o_mesh_side_faces(self.h, _p(out))
+        return out
 
     def initial_conditions(self, pulses=None, period=None):
         if pulses is None:

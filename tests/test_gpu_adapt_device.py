@@ -4,8 +4,9 @@ host from the node lists) give the same element counts, face counts, orders, ele
 Hilbert statuses, rotations, split levels, estimates and states bit for bit; refused adaptations
 change nothing; and the device path's adaptive loop matches the oracle at the batched workload's
 shape. The oracle comparisons of the other adaptive tests run through the device path too (it is
-the default). Splits across periodic sides (which the host path does not provide) are compared
-with the oracle and with the same solve shifted by half a period.
+the default). Splits across periodic sides and sides covered by more than two faces (which the
+host path does not provide) are compared with the oracle, face count per side included, and with
+the same solve shifted by half a period.
 """
 import numpy as np
 import pytest
@@ -13,7 +14,7 @@ import pytest
 import ndg_cuda_amd as nd
 from oracle import oracle as O
 from test_gpu_pinned_paths import batched_hp_pulses, rotated
-from test_oracle_adapt import PERIODIC_SPLITS
+from test_oracle_adapt import PERIODIC_SPLITS, SHIFT_CASES
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +42,7 @@ def assert_identical(dm, hm, t):
     assert dm.n_elements_ == hm.n_elements_ and dm.n_faces_ == hm.n_faces_
     assert dm.n_dofs_per_field_ == hm.n_dofs_per_field_
     assert np.array_equal(dm.orders_, hm.orders_)
-    for a, b, name in zip(dm.sides(), hm.sides(), ("link", "hang", "hang_second", "periodic")):
+    for a, b, name in zip(dm.sides(), hm.sides(), ("link", "hang", "next_small", "periodic", "span")):
         assert np.array_equal(a, b), name
     cd, ch = dm.get_complete_solution(t), hm.get_complete_solution(t)
     for name in ("N", "status", "rotation", "split_level", "p_error", "p_sigma", "refine", "coarsen", "x", "y"):
@@ -118,40 +119,57 @@ def rough_state(om, pred, rng):
     return out
 
 
-def periodic_pair(N0=6, N_max=8):
-    desc = nd.square_mesh(8, x_periodic=True, y_periodic=True, jitter=0.1, seed=7)
+def periodic_pair(N0=6, N_max=8, jitter=0.1):
+    desc = nd.square_mesh(8, x_periodic=True, y_periodic=True, jitter=jitter, seed=7)
     m = nd.Mesh2D_t(desc, N0, N_max, 5, 2, 0, 0, 1e-4, 1e-14, 0.0, nd.NDG_t(N_max, 5))
     om = O.Mesh(O.Tables(N_max, 5), desc, N0, tol_min=1e-4, tol_max=1e-14)
     return m, om
 
 
-def split_both(m, om, pred, rng):
+def split_both(m, om, pred, rng, level=5):
     q = rough_state(om, pred, rng)
     m.set_solution(*q)
     om.set_state(*q)
     om.estimate_error()
-    return om.adapt(8, 2), q
+    return om.adapt(8, level), q
+
+
+def faces_per_side(m):
+    """Faces on each element side from the device topology: a larger side's small sides (its chain
+    of next links), one on any other side."""
+    link, hang, nxt, _, _ = m.sides()
+    out = np.ones(len(link), dtype=np.int64)
+    for t in np.nonzero(hang == 2)[0]:
+        n, c = 0, link[t]
+        while c >= 0:
+            n, c = n + 1, nxt[c]
+        out[t] = n
+    return out.reshape(-1, 4)
 
 
 @pytest.mark.parametrize("name", list(PERIODIC_SPLITS))
 def test_device_splits_across_the_period_match_oracle(name):
     """Splits next to the periods (2:1 across them, at split levels 1 and 2, and conforming again
-    when the partners split): the device adaptation splits the same elements as the oracle
-    (split_interfaces, src/meshes/Mesh2D_t.cpp:6343-6756), with the same element and face counts
-    and orders; then the right-hand side of a periodised two-pulse state (relative to its largest
-    field) and the state after 20 RK3 steps match the oracle to 1e-12."""
+    when the partners split) and small elements split again next to unsplit ones (sides with up to
+    five faces, pieces down to 1/16): the device adaptation splits the same elements as the oracle
+    (split_interfaces, src/meshes/Mesh2D_t.cpp:6343-6756), with the same element and face counts,
+    faces on every side and orders; then the right-hand side of a periodised two-pulse state
+    (relative to its largest field) and the state after 20 RK3 steps match the oracle to 1e-12."""
     say("periodic splits", name)
-    m, om = periodic_pair()
+    # the 1/16 pieces are picked by their centres near the corner (0, 0), which jitter would move
+    m, om = periodic_pair(jitter=0.0 if name == "sixteenth" else 0.1)
     rng = np.random.default_rng(11)
     for pred in PERIODIC_SPLITS[name]:
         n_o, _ = split_both(m, om, pred, rng)
         assert n_o > 0
-        assert m.adapt(8, 2) == n_o
+        assert m.adapt(8, 5) == n_o
         assert m.n_elements_ == om.K and m.n_faces_ == om.n_faces
         assert np.array_equal(m.orders_, om.N)
-    link, hang, hang2, per = m.sides()
-    assert ((hang != 0) & (per != 0)).any() or name == "then the partners"
-    say("K", m.n_elements_, "faces", m.n_faces_, "periodic hanging sides", int(((hang != 0) & (per != 0)).sum()))
+        assert np.array_equal(faces_per_side(m), om.side_faces())
+    link, hang, nxt, per, span = m.sides()
+    assert ((hang != 0) & (per != 0)).any() or name in ("then the partners", "four faces", "sixteenth")
+    say("K", m.n_elements_, "faces", m.n_faces_, "periodic hanging sides", int(((hang != 0) & (per != 0)).sum()),
+        "most faces on a side", int(faces_per_side(m).max()))
     pulses = [(0.9, 0.85, 0.2, 1.0), (-0.6, 0.2, 0.3, 0.5)]
     m.initial_conditions(pulses, period=(2.0, 2.0))
     om.initial_conditions(pulses, period=(2.0, 2.0))
@@ -166,20 +184,21 @@ def test_device_splits_across_the_period_match_oracle(name):
         assert float(np.max(np.abs(g - r)) / np.max(np.abs(r))) < 1e-12
 
 
-def test_device_refuses_three_faces_across_the_period():
-    """A child on the period whose larger partner across it does not split would leave three
-    faces on the partner's side: the device path refuses before anything changes (same orders,
-    element count and state), as the oracle does."""
-    say("device refusal across the period")
-    m, om = periodic_pair()
+def test_device_refuses_faces_below_a_sixteenth():
+    """A small element covering 1/16 of a larger side that splits while the larger element does
+    not would leave a face of 1/32 of the side: the device path refuses before anything changes
+    (same orders, element count and state), as the oracle does."""
+    say("device refusal below 1/16")
+    m, om = periodic_pair(jitter=0.0)
     rng = np.random.default_rng(11)
-    n_o, _ = split_both(m, om, lambda x, y: x > 0.75, rng)
-    assert m.adapt(8, 2) == n_o == 8
-    n_o, q = split_both(m, om, lambda x, y: x > 0.9 and y > 0.5, rng)
+    for pred in PERIODIC_SPLITS["sixteenth"]:
+        n_o, _ = split_both(m, om, pred, rng, level=6)
+        assert m.adapt(8, 6) == n_o > 0
+    n_o, q = split_both(m, om, lambda x, y: 0 < x < 0.01 and 0 < y < 0.01, rng, level=6)
     assert n_o == -2
     K, faces, orders = m.n_elements_, m.n_faces_, m.orders_.copy()
     with pytest.raises(nd.NDGError) as info:
-        m.adapt(8, 2)
+        m.adapt(8, 6)
     assert info.value.code == nd._abi.NDG_ERR_UNSUPPORTED
     m._refresh()
     assert m.n_elements_ == K and m.n_faces_ == faces and np.array_equal(m.orders_, orders)
@@ -187,32 +206,35 @@ def test_device_refuses_three_faces_across_the_period():
         assert np.array_equal(a, b)
 
 
-def corner_pulse_solve(x0, y0):
+def corner_pulse_solve(x0, y0, split, tol, width):
     desc = nd.square_mesh(8, x_periodic=True, y_periodic=True)
-    m = nd.Mesh2D_t(desc, 4, 10, 5, 1, 5, 0, 1e-5, 1e-14, 0.0, nd.NDG_t(10, 5))
-    m.initial_conditions([(x0, y0, 0.15, 1.0)], period=(2.0, 2.0))
+    m = nd.Mesh2D_t(desc, 4, 10, 5, split, 5, 0, tol, 1e-14, 0.0, nd.NDG_t(10, 5))
+    m.initial_conditions([(x0, y0, width, 1.0)], period=(2.0, 2.0))
     solver = nd.Solver2D_t(0.5, [0.3])
     solver.solve(None, m)
     return m, solver
 
 
-def test_device_periodic_split_solve_matches_oracle_and_the_shifted_solve():
+@pytest.mark.parametrize("split,tol,width,most", SHIFT_CASES)
+def test_device_periodic_split_solve_matches_oracle_and_the_shifted_solve(split, tol, width, most):
     """The hp-adaptive solve with the pulse on the periodic corner (its elements split across both
-    periods) against the oracle (step count, time, element and face counts, orders, state <=
-    1e-12), then against the same device solve with the pulse moved by half a period to the
-    centre (split in the interior), shifted back element by element (<= 1e-12)."""
-    say("periodic corner pulse solve")
-    a, sa = corner_pulse_solve(0.95, 0.95)
-    om = O.Mesh(O.Tables(10, 5), nd.square_mesh(8, x_periodic=True, y_periodic=True), 4, tol_min=1e-5,
+    periods; at split level 2, sides with up to four faces) against the oracle (step count, time,
+    element and face counts, faces per side, orders, state <= 1e-12), then against the same device
+    solve with the pulse moved by half a period to the centre (split in the interior), shifted back
+    element by element (<= 1e-12)."""
+    say("periodic corner pulse solve, split level", split)
+    a, sa = corner_pulse_solve(0.95, 0.95, split, tol, width)
+    om = O.Mesh(O.Tables(10, 5), nd.square_mesh(8, x_periodic=True, y_periodic=True), 4, tol_min=tol,
                 tol_max=1e-14)
-    om.initial_conditions([(0.95, 0.95, 0.15, 1.0)], period=(2.0, 2.0))
-    steps, t = om.solve_adaptive(0.5, 0.3, 5, 10, 1)
+    om.initial_conditions([(0.95, 0.95, width, 1.0)], period=(2.0, 2.0))
+    steps, t = om.solve_adaptive(0.5, 0.3, 5, 10, split)
     assert sa.n_steps_ == steps and sa.time_ == t
     assert a.n_elements_ == om.K > 64 and a.n_faces_ == om.n_faces
+    assert np.array_equal(faces_per_side(a), om.side_faces()) and om.side_faces().max() == most
     assert np.array_equal(a.orders_, om.N)
     for g, r in zip(a.get_solution(), om.get_state()):
         assert float(np.max(np.abs(g - r)) / np.max(np.abs(r))) < 1e-12
-    b, sb = corner_pulse_solve(-0.05, -0.05)
+    b, sb = corner_pulse_solve(-0.05, -0.05, split, tol, width)
     assert sb.n_steps_ == steps and b.n_elements_ == a.n_elements_
 
     def centres(m):

@@ -266,17 +266,19 @@ def test_split_walls_and_limits(oracle_tables):
 # ---------------------------------------------------------------------------------------------
 # Splitting across the period (split_interfaces, Mesh2D_t.cpp:6343-6756): a periodic ghost splits
 # with its origin, so a split element faces an unsplit partner through two half ghosts on one end
-# and one ghost covering two sides on the other (2:1 across the period).
+# and one ghost covering two sides on the other (2:1 across the period). Sides covered by more than
+# two faces (a small element split again next to an unsplit one), in the interior and across the
+# period.
 
 def centre_flags(om, pred):
     nodes, _, _, xy = om.elements()
     return np.array([1 if pred(x, y) else 0 for x, y in xy[nodes].mean(axis=1)], dtype=np.int32)
 
 
-def split_where(om, *preds):
+def split_where(om, *preds, level=5):
     for pred in preds:
         om.set_flags(centre_flags(om, pred), np.zeros(om.K))
-        n = om.adapt(N_max=16, max_split_level=2)
+        n = om.adapt(N_max=16, max_split_level=level)
         if n < 0:
             return n
     return n
@@ -297,15 +299,31 @@ PERIODIC_SPLITS = {
     # the right column, then its partners, then the children on the period at level 2 (2:1 across
     # the period between levels 1 and 2)
     "level two": ((lambda x, y: x > 0.75), (lambda x, y: x < -0.75), (lambda x, y: x > 0.9 and y > 0.5)),
+    # the right column, then its children on the period again: four quarter faces on the unsplit
+    # partners' sides
+    "four faces across the period": ((lambda x, y: x > 0.75), (lambda x, y: x > 0.9 and y > 0.5)),
+    # one element, then its two children on its left side: four quarter faces on the left
+    # neighbour's side, and two quarters and a half on the lower neighbour's
+    "four faces": ((lambda x, y: 0 < x < 0.25 and 0 < y < 0.25), (lambda x, y: 0 < x < 0.1 and 0 < y < 0.25)),
+    # one element, then its corner child three more times: pieces down to 1/16 of the
+    # neighbours' sides
+    "sixteenth": ((lambda x, y: 0 < x < 0.25 and 0 < y < 0.25), (lambda x, y: 0 < x < 0.1 and 0 < y < 0.1),
+                  (lambda x, y: 0 < x < 0.05 and 0 < y < 0.05), (lambda x, y: 0 < x < 0.02 and 0 < y < 0.02)),
     # the same, then their partners too
     "two levels": ((lambda x, y: x > 0.75), (lambda x, y: x < -0.75),
                    (lambda x, y: x > 0.9 and y > 0.5), (lambda x, y: x < -0.9 and y > 0.5)),
 }
 
 
+# the most faces on one element side each configuration leaves
+MOST_FACES = {"corner block": 2, "column and row": 2, "then the partners": 2, "level two": 2,
+              "four faces across the period": 4, "four faces": 4, "sixteenth": 5, "two levels": 2}
+
+
 @pytest.mark.parametrize("name", list(PERIODIC_SPLITS))
 def test_periodic_split_keeps_the_operator_exact_and_conservative(oracle_tables, name):
-    """On meshes split next to the periods (2:1 across them): a constant state stays at rest, the
+    """On meshes split next to the periods (2:1 across them) or with sides covered by up to four
+    faces (pieces down to 1/16 of a side): a constant state stays at rest, the
     mass integrals of a random state's right-hand side vanish, and the right-hand side of a
     smooth periodic field matches its exact derivative to the order-10 interpolation error
     (observed 1.4e-12; a periodic ghost copying the wrong child of its origin gives 20)."""
@@ -314,6 +332,7 @@ def test_periodic_split_keeps_the_operator_exact_and_conservative(oracle_tables,
     om = O.Mesh(oracle_tables, desc, N, tol_min=1e-14, tol_max=1e-300)
     assert split_where(om, *PERIODIC_SPLITS[name]) > 0
     assert om.K > 64
+    assert om.side_faces().max() == MOST_FACES[name]
     import test_oracle_math as tm
     fine = fine_description(om, desc)
     X, Y = tm.packed_nodes(fine, oracle_tables, om.N)
@@ -332,34 +351,42 @@ def test_periodic_split_keeps_the_operator_exact_and_conservative(oracle_tables,
         assert np.abs(g - e).max() < 1e-10
 
 
-def test_periodic_split_refuses_three_faces_on_a_side(oracle_tables):
-    """A child next to the period whose larger partner does not split again would leave three
-    faces on the partner's side: refused, nothing changed."""
+def test_split_refuses_faces_below_a_sixteenth(oracle_tables):
+    """A small element covering 1/16 of a larger side that splits while the larger element does
+    not would leave a face of 1/32 of the side: refused, nothing changed (the limit the device
+    path shares)."""
     desc = nd.square_mesh(8, x_periodic=True, y_periodic=True)
     om = O.Mesh(oracle_tables, desc, 4, tol_min=1e-14, tol_max=1e-300)
-    assert split_where(om, lambda x, y: x > 0.75) == 8
+    assert split_where(om, *PERIODIC_SPLITS["sixteenth"]) > 0
     K, faces = om.K, om.n_faces
-    assert split_where(om, lambda x, y: x > 0.8 and y > 0.5) == -2
+    assert split_where(om, lambda x, y: 0 < x < 0.01 and 0 < y < 0.01) == -2
     assert om.K == K and om.n_faces == faces
 
 
-def periodic_pulse_solve(oracle_tables, x0, y0):
+def periodic_pulse_solve(oracle_tables, x0, y0, split=1, tol=1e-5, width=0.15):
     desc = nd.square_mesh(8, x_periodic=True, y_periodic=True)
-    om = O.Mesh(oracle_tables, desc, 4, tol_min=1e-5, tol_max=1e-14)
-    om.initial_conditions([(x0, y0, 0.15, 1.0)], period=(2.0, 2.0))
-    steps, t = om.solve_adaptive(0.5, 0.3, 5, 10, 1)
+    om = O.Mesh(oracle_tables, desc, 4, tol_min=tol, tol_max=1e-14)
+    om.initial_conditions([(x0, y0, width, 1.0)], period=(2.0, 2.0))
+    steps, t = om.solve_adaptive(0.5, 0.3, 5, 10, split)
     return om, steps, t
 
 
-def test_periodic_split_solve_equals_the_shifted_solve(oracle_tables):
+# (max split level, tolerance_min, pulse width, most faces on one side)
+SHIFT_CASES = [(1, 1e-5, 0.15, 2), (2, 1e-4, 0.08, 4)]
+
+
+@pytest.mark.parametrize("split,tol,width,most", SHIFT_CASES)
+def test_periodic_split_solve_equals_the_shifted_solve(oracle_tables, split, tol, width, most):
     """Translation invariance: an hp-adaptive solve with the pulse on the periodic corner (its
-    elements split across both periods) equals, element by element, the same solve with the
-    pulse moved by half a period to the centre (split in the interior), shifted back: same step
-    count, element count and orders, states equal to rounding (observed 4e-15)."""
-    a, steps_a, t_a = periodic_pulse_solve(oracle_tables, 0.95, 0.95)
-    b, steps_b, t_b = periodic_pulse_solve(oracle_tables, -0.05, -0.05)
-    assert steps_a == steps_b and t_a == t_b
+    elements split across both periods; at split level 2 sides with up to four faces appear)
+    equals, element by element, the same solve with the pulse moved by half a period to the centre
+    (split in the interior), shifted back: same step count, element count and orders, states equal
+    to rounding (observed 4e-15)."""
+    a, steps_a, t_a = periodic_pulse_solve(oracle_tables, 0.95, 0.95, split, tol, width)
+    b, steps_b, t_b = periodic_pulse_solve(oracle_tables, -0.05, -0.05, split, tol, width)
+    assert steps_a == steps_b > 0 and t_a == t_b
     assert a.K == b.K > 64
+    assert a.side_faces().max() == most
     assert np.array_equal(np.bincount(a.N), np.bincount(b.N))
     assert_same_up_to_shift(a, b, 1.0, 1.0, 1e-12)
 
