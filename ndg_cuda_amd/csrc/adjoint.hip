@@ -381,6 +381,25 @@ record_kernel(const double *__restrict__ q, const int64_t *__restrict__ elem, co
     if (threadIdx.x == 0) out[r] = acc;
 }
 
+// record_kernel on the RK register in the stage kernel's tile order: element e is element
+// el = e mod epb of its tile of nel elements, node (i, j) of field 0 at ((j nel + el)(N+1) + i).
+__global__ void __launch_bounds__(64)
+record_rk_kernel(const double *__restrict__ g, const int64_t *__restrict__ elem, const double *__restrict__ w_t,
+                 int nf, int64_t K, int epb, double *__restrict__ out) {
+    const int r = blockIdx.x, ne = nf * nf;
+    const int64_t e = elem[r], e0 = e - e % epb;
+    const int el = (int)(e - e0), nel = (int)min<int64_t>(epb, K - e0);
+    const double *tile = g + e0 * 3 * ne;
+    const double *wr = w_t + (int64_t)r * ne;
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < ne; k += 64) {
+        const int j = k / nf, i = k % nf;
+        acc += wr[k] * tile[(j * nel + el) * nf + i];
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (threadIdx.x == 0) out[r] = acc;
+}
+
 __global__ void __launch_bounds__(256)
 source_kernel(double *__restrict__ lam, const int64_t *__restrict__ elem, const double *__restrict__ w,
               const double *__restrict__ res, int n, int ne) {
@@ -469,9 +488,17 @@ int64_t adjoint_blocks(const DeviceMesh &m) {
 }
 
 hipError_t launch_record(const DeviceMesh &m, const int64_t *elem, const double *w, int n, double *out,
-                         hipStream_t s, const double *src) {
+                         hipStream_t s) {
     const int ne = (m.groups[0].N + 1) * (m.groups[0].N + 1);
-    hipLaunchKernelGGL(record_kernel, dim3(n), dim3(64), 0, s, src ? src : m.q, elem, w, ne, out);
+    hipLaunchKernelGGL(record_kernel, dim3(n), dim3(64), 0, s, m.q, elem, w, ne, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_record_rk(const DeviceMesh &m, const int64_t *elem, const double *w_t, int n, double *out,
+                            hipStream_t s) {
+    const Group &G = m.groups[0];
+    hipLaunchKernelGGL(record_rk_kernel, dim3(n), dim3(64), 0, s, m.g, elem, w_t, G.N + 1, G.K, stage_tile_epb(G),
+                       out);
     return hipGetLastError();
 }
 

@@ -191,7 +191,7 @@ struct DeviceMesh {
     double *tab1d = nullptr;     // 1D tables of all orders (solution transfer)
     double *mortar_mats = nullptr;  // interpolation / projection matrices of the mortar faces
     double *q = nullptr;
-    double *g = nullptr;         // RK register, transposed within each (element, field) block
+    double *g = nullptr;         // RK register (alive within a step; the stage kernel's tile order)
     double *trace[2] = {nullptr, nullptr};
     double *rhs = nullptr;       // compute_rhs only (allocated on demand)
     double *scratch = nullptr;   // reductions
@@ -348,9 +348,15 @@ hipError_t launch_jacobian_scale(const DeviceMesh &m, double *x, const int64_t *
                                  hipStream_t s);
 // Receivers (element in device order, (N+1)^2 interpolation weights each): p at the receivers of
 // the current state, and lambda_p += res_r w_r.
-// Receiver values sum_k w[r][k] src[elem r][field 0][k] (src: the state q by default).
+// Receiver values sum_k w[r][k] q[elem r][field 0][k] of the state q.
 hipError_t launch_record(const DeviceMesh &m, const int64_t *elem, const double *w, int n, double *out,
-                         hipStream_t s, const double *src = nullptr);
+                         hipStream_t s);
+// The same of the RK register g (single-order meshes), which the FP64 stage kernel keeps in its
+// tile order; w_t holds the weights in transposed node order (k = j (N+1) + i for node (i, j)).
+hipError_t launch_record_rk(const DeviceMesh &m, const int64_t *elem, const double *w_t, int n, double *out,
+                            hipStream_t s);
+// Elements per workgroup tile of the FP64 stage kernel for group G (the RK register's tile order).
+int stage_tile_epb(const Group &G);
 hipError_t launch_source(double *lam, const DeviceMesh &m, const int64_t *elem, const double *w,
                          const double *res, int n, hipStream_t s);
 

@@ -1,7 +1,9 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of one low-storage RK3 stage of the DG-SEM wave solver.
 //
 // Data layout in HBM (structure of arrays, element-major, elements in Hilbert-curve order):
-//   q, g   [K][3][(N+1)^2]   state (p, u, v) and RK register, reference node order i*(N+1)+j
+//   q      [K][3][(N+1)^2]   state (p, u, v), reference node order i*(N+1)+j
+//   g      [K][3][(N+1)^2]   RK register, alive within a step only; the stage kernel's own order
+//                            within each workgroup tile (see stage_kernel)
 //   trace  [K][4][3][N+1]    element traces of the current state (two buffers, ping-pong)
 //   verts  [K][4][2]         element vertices (metrics are recomputed from them per node)
 //   side_nb[K][4]            neighbour element*4+side, or -(1+index) into the BcSide records
@@ -171,13 +173,16 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
         const int r = tid + it * T;
         vreg[it] = r < nel * 8 ? verts[e0 * 8 + r] : 0.0;
     }
-    // The RK register is stored transposed within an element ([e][f][j][i]), so that the row
-    // owner's line of it is read and written with consecutive lanes on consecutive addresses.
+    // The RK register lives only within a step (the first stage writes it without reading it), so
+    // its layout is the kernel's own: within the workgroup's tile of nel elements it is stored as
+    // [f][j][el][i], so that the row owners of all the tile's elements read and write one line
+    // index j with consecutive lanes on consecutive addresses (one contiguous run per wave
+    // instruction instead of one short run per element).
     double g_old[NF];
     if (!RHS_ONLY && !first && active) {
-        const double *gg = g + (e0 + el) * 3 * NE + f * NE + l;
+        const double *gg = g + e0 * 3 * NE + ((int64_t)f * NF * nel + el) * NF + l;
 #pragma unroll
-        for (int j = 0; j < NF; ++j) g_old[j] = gg[j * NF];
+        for (int j = 0; j < NF; ++j) g_old[j] = gg[j * nel * NF];
     } else {
 #pragma unroll
         for (int j = 0; j < NF; ++j) g_old[j] = 0.0;
@@ -345,8 +350,8 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
     }
     __syncthreads();
 
-    // P4: thread owns eta-line i = l of field f. The new RK register goes straight to HBM
-    // (transposed layout, coalesced); the new state waits for the barrier.
+    // P4: thread owns eta-line i = l of field f. The new RK register goes straight to HBM (tile
+    // layout, coalesced); the new state waits for the barrier.
     double qn[NF];
     if (active) {
         const int i = l;
@@ -375,7 +380,7 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
                 d[j] = Gv;
             } else {
                 const double gn = first ? Gv : a * g_old[j] + Gv;
-                g[(e0 + el) * 3 * NE + f * NE + j * NF + i] = gn;
+                g[e0 * 3 * NE + ((f * NF + j) * nel + el) * NF + i] = gn;
                 qn[j] = qrow[j] + gdt * gn;
             }
         }
@@ -445,9 +450,10 @@ __device__ __forceinline__ void two_sum_add(float &hi, float &lo, float d) {
     hi = h;
 }
 
-// q and qh are in reference node order; g and gl32 (RK register in x, low state word in y) are
+// q and qh are in reference node order; gl32 (RK register in x, low state word in y) is
 // transposed within each (element, field) block; nf = N + 1. Thread r writes the x word of
-// element r and the y word of element t: 4-byte stores to distinct words.
+// element r and the y word of element t: 4-byte stores to distinct words. The RK register
+// carries no value from one step to the next, so g is copied word for word whatever its order.
 __global__ void to_c32_kernel(const double *__restrict__ q, const double *__restrict__ g,
                               float *__restrict__ qh, float2 *__restrict__ gl32, int64_t n, int nf) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1240,6 +1246,16 @@ static int group_epb(const Group &G, int precision) {
 }
 
 template <int N>
+static int stage_epb_n(const Group &G) {
+    return group_epb<N>(G, NDG_PRECISION_FP64);
+}
+
+int stage_tile_epb(const Group &G) {
+    switch (G.N) { CASES_1(stage_epb_n, G) }
+    return 1;
+}
+
+template <int N>
 static hipError_t stage_n(DeviceMesh &m, const Group &G, int stage, double a, double gdt, hipStream_t s) {
     using S = LineShape<N>;
     const int epb = group_epb<N>(G, NDG_PRECISION_FP64);
@@ -1407,8 +1423,8 @@ hipError_t launch_traces_c32(const DeviceMesh &m, hipStream_t s) {
 }
 
 
-// q and qh are in reference node order; g and gl32 transposed within each (element, field)
-// block. Run per group (the transposition depends on N).
+// q and qh are in reference node order; gl32 transposed within each (element, field) block. Run
+// per group (the transposition depends on N).
 hipError_t launch_to_c32(DeviceMesh &m, hipStream_t s) {
     return for_groups(m, [&](const Group &G) -> hipError_t {
         const int64_t n = G.K * 3 * (int64_t)(G.N + 1) * (G.N + 1);

@@ -2804,8 +2804,8 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
         for (int64_t n = 0; n < n_steps; ++n) {
             for (int stage = 0; stage < 3; ++stage) {
                 HIP_TRY(launch_stage(d, stage, kAm[stage], kGm[stage] * dt, s), "stage kernel");
-                HIP_TRY(launch_record(d, R.elem.as<int64_t>(), R.w_t.as<double>(), R.n,
-                                      recs_g.as<double>() + (n * 3 + stage) * R.n, s, d.g),
+                HIP_TRY(launch_record_rk(d, R.elem.as<int64_t>(), R.w_t.as<double>(), R.n,
+                                         recs_g.as<double>() + (n * 3 + stage) * R.n, s),
                         "record kernel");
             }
             HIP_TRY(launch_record(d, R.elem.as<int64_t>(), R.w.as<double>(), R.n, recs.as<double>() + n * R.n, s),
