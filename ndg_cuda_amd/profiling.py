@@ -1,7 +1,7 @@
 """Summaries of rocprofv3 output (rocpd SQLite databases) for the stage kernels.
 
     python -m ndg_cuda_amd.profiling trace <run_results.db> [--kernel SUBSTR] [--min-us T] [--max-us T] [--grid G]
-    python -m ndg_cuda_amd.profiling traffic --fetch DB --write DB --copy-fetch DB --copy-write DB
+    python -m ndg_cuda_amd.profiling traffic --fetch DB --write DB --copy-fetch DB --copy-write DB [--grid G] [--last N]
     python -m ndg_cuda_amd.profiling counters DB [DB ...] [--kernel SUBSTR] [--by-grid]
 
 `trace` prints per-kernel call counts and durations from a --kernel-trace run. `traffic` turns one
@@ -45,17 +45,28 @@ def kernel_stats(path, kernel=None, min_us=0.0, max_us=float("inf"), grid=None):
     return out
 
 
-def counter_means(path, kernel, grid=None):
+def counter_means(path, kernel, grid=None, last=None):
     """Mean counter value per dispatch of the kernel; with `grid`, only dispatches of that grid
-    size (threads), i.e. one launch shape (the autotuner launches several)."""
+    size (threads), i.e. one launch shape (the autotuner launches several); with `last`, only the
+    last that many dispatches (the timed steps after the tuner's probes, which are mostly
+    first-stage launches that read no RK register)."""
     c = _db(path)
-    sql = "select counter_name, count(*), avg(value) from counters_collection where kernel_name like ?"
+    sql = "select counter_name, dispatch_id, value from counters_collection where kernel_name like ?"
     args = [f"%{kernel}%"]
     if grid is not None:
         sql += " and grid_size = ?"
         args.append(int(grid))
-    rows = c.execute(sql + " group by counter_name", args).fetchall()
-    return {name: {"dispatches": n, "mean": v} for name, n, v in rows}
+    per = {}
+    for name, did, v in c.execute(sql + " order by dispatch_id", args).fetchall():
+        per.setdefault(name, {}).setdefault(did, 0.0)
+        per[name][did] += v
+    out = {}
+    for name, by in per.items():
+        vals = [by[d] for d in sorted(by)]
+        if last:
+            vals = vals[-last:]
+        out[name] = {"dispatches": len(vals), "mean": sum(vals) / len(vals)}
+    return out
 
 
 def grid_sizes(path, kernel):
@@ -65,9 +76,10 @@ def grid_sizes(path, kernel):
                      "where kernel_name like ? group by grid_size order by grid_size", (f"%{kernel}%",)).fetchall()
 
 
-def traffic(fetch, write, copy_fetch, copy_write, kernel, copy_kernel="copyBuffer", copy_bytes=1 << 30):
-    f = counter_means(fetch, kernel)["FETCH_SIZE"]["mean"]
-    w = counter_means(write, kernel)["WRITE_SIZE"]["mean"]
+def traffic(fetch, write, copy_fetch, copy_write, kernel, copy_kernel="copyBuffer", copy_bytes=1 << 30, grid=None,
+            last=None):
+    f = counter_means(fetch, kernel, grid, last)["FETCH_SIZE"]["mean"]
+    w = counter_means(write, kernel, grid, last)["WRITE_SIZE"]["mean"]
     cf = counter_means(copy_fetch, copy_kernel)["FETCH_SIZE"]["mean"]
     cw = counter_means(copy_write, copy_kernel)["WRITE_SIZE"]["mean"]
     fs, ws = copy_bytes / (cf * 1024.0), copy_bytes / (cw * 1024.0)
@@ -91,6 +103,8 @@ def main(argv=None):
     for k in ("fetch", "write", "copy_fetch", "copy_write"):
         tr.add_argument("--" + k.replace("_", "-"), required=True)
     tr.add_argument("--kernel", default="stage_kernel<8, false>")
+    tr.add_argument("--grid", type=int, help="only dispatches of this grid size (threads)")
+    tr.add_argument("--last", type=int, help="only the last this many dispatches (the timed steps)")
     cc = sub.add_parser("counters")
     cc.add_argument("dbs", nargs="+")
     cc.add_argument("--kernel", default="stage_kernel<8, false>")
@@ -102,7 +116,8 @@ def main(argv=None):
             print(f"{r['calls']:6d} calls  avg {r['avg_ns'] / 1e3:10.2f} us  min {r['min_ns'] / 1e3:10.2f}  "
                   f"max {r['max_ns'] / 1e3:10.2f}  total {r['total_ns'] / 1e6:9.3f} ms  {r['kernel'][:90]}")
     elif a.cmd == "traffic":
-        print(json.dumps(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.kernel), indent=1))
+        print(json.dumps(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.kernel, grid=a.grid, last=a.last),
+                         indent=1))
     else:
         for db in a.dbs:
             if a.grid is not None:
