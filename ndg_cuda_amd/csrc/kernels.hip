@@ -450,31 +450,40 @@ __device__ __forceinline__ void two_sum_add(float &hi, float &lo, float d) {
     hi = h;
 }
 
-// q and qh are in reference node order; gl32 (RK register in x, low state word in y) is
-// transposed within each (element, field) block; nf = N + 1. Thread r writes the x word of
-// element r and the y word of element t: 4-byte stores to distinct words. The RK register
-// carries no value from one step to the next, so g is copied word for word whatever its order.
+// Position in the FP32 form's gl32 (tile order [f][j][el][i] of tiles of epb elements, the last
+// tile of K - tile start elements) of value (element e, field f, node (i, j)); nf = N + 1.
+__device__ __forceinline__ int64_t c32_tile_index(int64_t e, int f, int i, int j, int nf, int epb, int64_t K) {
+    const int64_t e0 = e - e % epb;
+    const int nel = (int)min<int64_t>(epb, K - e0), el = (int)(e - e0);
+    return e0 * 3 * nf * nf + ((int64_t)(f * nf + j) * nel + el) * nf + i;
+}
+
+// q and qh are in reference node order; gl32 (RK register in x, low state word in y) is in the
+// FP32 stage kernel's tile order. Thread r writes one float2. The RK register carries no value
+// from one step to the next, so g is copied word for word whatever its order.
 __global__ void to_c32_kernel(const double *__restrict__ q, const double *__restrict__ g,
-                              float *__restrict__ qh, float2 *__restrict__ gl32, int64_t n, int nf) {
+                              float *__restrict__ qh, float2 *__restrict__ gl32, int64_t n, int nf, int epb) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int ne = nf * nf, node = (int)(r % ne), i = node / nf, j = node % nf;
-    const int64_t t = r - node + j * nf + i;
+    const int64_t e = r / (3 * ne);
+    const int f = (int)((r / ne) % 3);
     const double v = q[r];
     const float h = (float)v;
     qh[r] = h;
-    gl32[t].y = (float)(v - (double)h);
-    gl32[r].x = (float)g[r];
+    gl32[c32_tile_index(e, f, i, j, nf, epb, n / (3 * ne))] = make_float2((float)g[r], (float)(v - (double)h));
 }
 
 __global__ void from_c32_kernel(const float *__restrict__ qh, const float2 *__restrict__ gl32,
-                                double *__restrict__ q, double *__restrict__ g, int64_t n, int nf) {
+                                double *__restrict__ q, double *__restrict__ g, int64_t n, int nf, int epb) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int ne = nf * nf, node = (int)(r % ne), i = node / nf, j = node % nf;
-    const int64_t t = r - node + j * nf + i;
-    q[r] = (double)qh[r] + (double)gl32[t].y;
-    g[r] = (double)gl32[r].x;
+    const int64_t e = r / (3 * ne);
+    const int f = (int)((r / ne) % 3);
+    const float2 w = gl32[c32_tile_index(e, f, i, j, nf, epb, n / (3 * ne))];
+    q[r] = (double)qh[r] + (double)w.y;
+    g[r] = (double)w.x;
 }
 
 template <int N>
@@ -562,15 +571,16 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
         const int r = tid + it * T;
         vreg[it] = r < nel * 8 ? verts[e0 * 8 + r] : 0.0;
     }
-    // RK register and low words, interleaved and stored transposed within an element
-    // ([e][f][j][i]): the row owner's lines are read and written with consecutive lanes on
-    // consecutive addresses, both words with one 8-byte access.
+    // RK register and low words, interleaved, in the tile order [f][j][el][i] of the shape's
+    // full tiles (the launch always uses them, so the low words keep their places from one step to
+    // the next): the row owners of the tile's elements read and write one line index j with
+    // consecutive lanes on consecutive addresses, both words with one 8-byte access.
     float g_old[NF], lo_row[NF];
     if (active) {
-        const int64_t off = (e0 + el) * 3 * NE + f * NE + l;
+        const int64_t off = e0 * 3 * NE + ((int64_t)f * NF * nel + el) * NF + l;
 #pragma unroll
         for (int j = 0; j < NF; ++j) {
-            const float2 w = gl[off + j * NF];
+            const float2 w = gl[off + j * nel * NF];
             g_old[j] = first ? 0.f : w.x;
             lo_row[j] = w.y;
         }
@@ -771,7 +781,7 @@ stage_c32_kernel(const double *__restrict__ verts, const int32_t *__restrict__ s
             float h = qrow[j], lo = lo_row[j];
             two_sum_add(h, lo, gdt * gn);
             hn[j] = h;
-            gl[(e0 + el) * 3 * NE + f * NE + j * NF + i] = make_float2(gn, lo);
+            gl[e0 * 3 * NE + ((f * NF + j) * nel + el) * NF + i] = make_float2(gn, lo);
         }
     }
     __syncthreads();
@@ -1308,7 +1318,7 @@ static hipError_t traces_c32_n(const DeviceMesh &m, const Group &G, hipStream_t 
 template <int N>
 static hipError_t stage_c32_n(DeviceMesh &m, const Group &G, int stage, double a, double gdt, hipStream_t s) {
     using S = LineShape<N>;
-    const int epb = group_epb<N>(G, NDG_PRECISION_FP32_COMPENSATED);
+    const int epb = S::EPB;  // the low state words are kept in the full tiles' order
     const int64_t blocks = (G.K + epb - 1) / epb;
     hipLaunchKernelGGL(stage_c32_kernel<N>, dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts + G.e_begin * 8,
                        m.side_nb + G.e_begin * 4, m.bc, G.tab, G.tab32, m.qh + G.q_off, m.gl32 + G.q_off,
@@ -1423,13 +1433,13 @@ hipError_t launch_traces_c32(const DeviceMesh &m, hipStream_t s) {
 }
 
 
-// q and qh are in reference node order; gl32 transposed within each (element, field) block. Run
-// per group (the transposition depends on N).
+// q and qh are in reference node order; gl32 in the FP32 stage's tile order (tiles of the
+// shape's EPB elements). Run per group (the order depends on N).
 hipError_t launch_to_c32(DeviceMesh &m, hipStream_t s) {
     return for_groups(m, [&](const Group &G) -> hipError_t {
         const int64_t n = G.K * 3 * (int64_t)(G.N + 1) * (G.N + 1);
         hipLaunchKernelGGL(to_c32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m.q + G.q_off,
-                           m.g + G.q_off, m.qh + G.q_off, m.gl32 + G.q_off, n, G.N + 1);
+                           m.g + G.q_off, m.qh + G.q_off, m.gl32 + G.q_off, n, G.N + 1, stage_max_epb(G.N));
         return hipGetLastError();
     });
 }
@@ -1438,7 +1448,7 @@ hipError_t launch_from_c32(DeviceMesh &m, hipStream_t s) {
     return for_groups(m, [&](const Group &G) -> hipError_t {
         const int64_t n = G.K * 3 * (int64_t)(G.N + 1) * (G.N + 1);
         hipLaunchKernelGGL(from_c32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m.qh + G.q_off,
-                           m.gl32 + G.q_off, m.q + G.q_off, m.g + G.q_off, n, G.N + 1);
+                           m.gl32 + G.q_off, m.q + G.q_off, m.g + G.q_off, n, G.N + 1, stage_max_epb(G.N));
         return hipGetLastError();
     });
 }

@@ -1627,6 +1627,8 @@ extern "C" ndg_status ndg_mesh_autotune(ndg_mesh *mesh, int32_t repeats) {
         std::vector<int> epbs;
         for (int e : {emax, (emax + 1) / 2, (emax + 3) / 4, 1})
             if (std::find(epbs.begin(), epbs.end(), e) == epbs.end()) epbs.push_back(e);
+        // the FP32 form keeps its low state words in the order of the full tiles: block order only
+        if (prec == NDG_PRECISION_FP32_COMPENSATED) epbs.assign(1, emax);
         float best = INFINITY;
         Group::Tune best_t = G.tune[prec];
         for (int e : epbs)
