@@ -28,11 +28,12 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_N8_512x512_traffic.j
 
 
 def algorithmic_bytes_per_element(n):
-    """FP64 bytes one fused stage moves per element, averaged over the 3 stages: state and RK
-    register (stage 1 reads q, writes q and g; stages 2-3 read and write both) 88*(N+1)^2,
+    """FP64 bytes one fused stage moves per element, averaged over the 3 stages: the state read
+    and written every stage and the RK register written by stages 1-2 and read by stages 2-3 (the
+    last stage's register is never read: the next step's first stage overwrites it) 80*(N+1)^2,
     neighbour traces read plus own traces written 192*(N+1), vertices and neighbour ids 80."""
     nf = n + 1
-    return 88 * nf * nf + 192 * nf + 80
+    return 80 * nf * nf + 192 * nf + 80
 
 
 def algorithmic_bytes_fp32c(n):
@@ -145,7 +146,7 @@ def batched_leg(args, nd, torch, stream):
     ref64 = np.concatenate(got)
     orders = desc.quad_N
     nf = orders.astype(np.int64) + 1
-    bytes_launch = int((88 * nf * nf + 192 * nf + 80).sum()) + int(mortar_bytes(orders, desc))
+    bytes_launch = int((80 * nf * nf + 192 * nf + 80).sum()) + int(mortar_bytes(orders, desc))
     stage_s = ms_graph / 1e3 / (3 * steps)
     counts = {int(n): int((orders == n).sum()) for n in np.unique(orders)}
     return {"value": dofs * 3 * steps / (ms_graph / 1e3), "ms_per_step": ms_graph / steps,
@@ -415,7 +416,7 @@ def precision_leg(nd, torch, stream):
         ms = timed_steps(solver, mesh, stream, torch, steps)
         states[prec] = np.concatenate(mesh.get_solution())
         nf = n + 1
-        b = (88 * nf * nf + 192 * nf + 80) if prec == "fp64" else (68 * nf * nf + 96 * nf + 80)
+        b = algorithmic_bytes_per_element(n) if prec == "fp64" else algorithmic_bytes_fp32c(n)
         b *= mesh.n_elements_
         out[prec] = {"value": mesh.n_dofs * 3 * steps / (ms / 1e3), "ms_per_step": ms / steps,
                      "roofline_frac": b / (ms / 1e3 / (3 * steps)) / 1e9 / HBM_PEAK_GBS}

@@ -33,6 +33,9 @@ struct BcSide {
     int64_t slot;
 };
 
+// Stage kernel flags: the step's first stage (the RK register is not read) and its last stage.
+constexpr int kRkFirst = 1, kRkLast = 2;
+
 // Element side whose face is non-conforming: its element flux trace is produced by the mortar
 // kernel (interpolation to the face, flux, L2 projection back) before the stage kernels run.
 constexpr int32_t kKindMortar = 100;
@@ -207,6 +210,9 @@ struct DeviceMesh {
     // FP32 form with compensated state (allocated by ndg_mesh_set_precision): the state is the
     // unevaluated sum qh + ql of two floats per value, the RK register and traces are floats.
     int precision = NDG_PRECISION_FP64;
+    // Store the RK register at a step's last stage too (the adjoint's stage records read it after
+    // every stage); otherwise that store is skipped, no later stage reading it.
+    bool keep_last_rk = false;
     float *qh = nullptr;
     // RK register (x) and low state word (y) of each value, interleaved so that a line owner
     // reads and writes both with one 8-byte access; transposed within each (element, field) block

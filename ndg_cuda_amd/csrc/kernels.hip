@@ -112,7 +112,10 @@ traces_kernel(const double *__restrict__ tab, const double *__restrict__ q,
 //   P4  eta lines: D^ (-y_xi f + x_xi g) + lift, G = (-d_xi - d_eta)/J (:104-133), RK update
 //       (Solver2D_t.cpp:138-172)
 //   P5  coalesced stores of q and g, traces of the updated state for the next stage.
-// RHS_ONLY writes G (the compute_dg_wave_derivative output) and leaves the state alone.
+// RHS_ONLY writes G (the compute_dg_wave_derivative output) and leaves the state alone. rk_flags:
+// kRkFirst, the step's first stage (a = 0: the RK register is not read); kRkLast, the step's last
+// stage, whose RK register no one reads (the next step's first stage overwrites it), so it is not
+// stored.
 // The new state goes to q_out, which is q for the in-place step (q and q_out are therefore not
 // restrict; every value of q a workgroup reads precedes, through barriers, its writes).
 // A workgroup takes epb <= EPB consecutive elements (fewer gives more workgroups for a small
@@ -126,8 +129,9 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
              const BcSide *__restrict__ bc, const double *__restrict__ tab,
              const double *q, double *q_out, double *__restrict__ g, const double *__restrict__ tr_in,
              double *__restrict__ tr_out, double *__restrict__ rhs, const double *__restrict__ side_flux,
-             int64_t K, int first, double a, double gdt, const double *__restrict__ t_ptr, int epb,
+             int64_t K, int rk_flags, double a, double gdt, const double *__restrict__ t_ptr, int epb,
              int remap, double cw) {
+    const bool first = rk_flags & kRkFirst, store_g = !(rk_flags & kRkLast);
     using S = LineShape<N>;
     constexpr int NF = S::NF, NE = S::NE, EPB = S::EPB, T = S::T, FPT = S::FPT;
     constexpr double c = kWaveSpeed;
@@ -380,7 +384,7 @@ stage_kernel(const double *__restrict__ verts, const int32_t *__restrict__ side_
                 d[j] = Gv;
             } else {
                 const double gn = first ? Gv : a * g_old[j] + Gv;
-                g[e0 * 3 * NE + ((f * NF + j) * nel + el) * NF + i] = gn;
+                if (store_g) g[e0 * 3 * NE + ((f * NF + j) * nel + el) * NF + i] = gn;
                 qn[j] = qrow[j] + gdt * gn;
             }
         }
@@ -1273,7 +1277,8 @@ static hipError_t stage_n(DeviceMesh &m, const Group &G, int stage, double a, do
     hipLaunchKernelGGL((stage_kernel<N, false>), dim3((unsigned)blocks), dim3(S::T), 0, s, m.verts + G.e_begin * 8,
                        m.side_nb + G.e_begin * 4, m.bc, G.tab, m.q + G.q_off, (m.q_out ? m.q_out : m.q) + G.q_off,
                        m.g + G.q_off, m.trace[m.cur] + G.tr_off, m.trace[1 - m.cur] + G.tr_off, nullptr, m.side_flux,
-                       G.K, stage == 0 ? 1 : 0, a, gdt, m.clock + 1 + stage, epb, G.tune[NDG_PRECISION_FP64].remap,
+                       G.K, (stage == 0 ? kRkFirst : 0) | (stage == 2 && !m.keep_last_rk ? kRkLast : 0), a, gdt,
+                       m.clock + 1 + stage, epb, G.tune[NDG_PRECISION_FP64].remap,
                        m.wave_speed);
     return hipGetLastError();
 }

@@ -2803,6 +2803,12 @@ extern "C" ndg_status ndg_adjoint_gradient(ndg_mesh *mesh, const ndg_receivers *
     HIP_TRY(launch_clock_begin(d, 0.0, 1, dt, s), "clock kernel");
     if (records) {
         HIP_TRY(hipMemcpyAsync(ckp(0), d.q, qb, hipMemcpyDeviceToDevice, s), "checkpoint");
+        // the RK register is recorded after every stage, the last included
+        struct KeepLastRk {
+            DeviceMesh &d;
+            KeepLastRk(DeviceMesh &dm) : d(dm) { d.keep_last_rk = true; }
+            ~KeepLastRk() { d.keep_last_rk = false; }
+        } keep_last_rk{d};
         for (int64_t n = 0; n < n_steps; ++n) {
             for (int stage = 0; stage < 3; ++stage) {
                 HIP_TRY(launch_stage(d, stage, kAm[stage], kGm[stage] * dt, s), "stage kernel");
