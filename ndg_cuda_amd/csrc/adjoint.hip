@@ -67,7 +67,6 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
     __shared__ double s_met[EPB * 4 * NF];
     __shared__ double s_side[EPB * 4 * 3];
     __shared__ double s_v[EPB * 8];
-    __shared__ double s_red[3];
     const double *lm = tab + NF * NF;
     const double *lp = lm + NF;
     const double *x = lp + 2 * NF;
@@ -114,12 +113,34 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         const int rr = in ? r : 0;  // in-range address for every lane, value selected below
         const double lv = lg[rr];
         const double mv = (flags & kAdjMuIn) ? mg[rr] : 0.0;
-        const double qv = qg ? qg[rr] : 0.0;
         lreg[it] = in ? lv : 0.0;
         mreg[it] = in ? mv : 0.0;
+    }
+#pragma unroll
+    for (int it = 0; it < QPT; ++it) {
+        const int r = tid + it * T;
+        const bool in = r < nel * 3 * NE;
+        const double qv = qg ? qg[in ? r : 0] : 0.0;
         qreg[it] = in ? qv : 0.0;
     }
     for (int r = tid; r < nel * 8; r += T) s_v[r] = verts[e0 * 8 + r];
+
+    // P2 (needs only this thread's registers, so it shares P0's barrier): mu~ = mu + gs lambda,
+    // nu = -c mu~ (J-scaled variables) to s_n; mu' = a_s mu~ (stored). Lanes past the last element
+    // hold zeros and store nothing.
+    if (apply) {
+        double *mo = mu + e0 * 3 * NE;
+#pragma unroll
+        for (int it = 0; it < QPT; ++it) {
+            const int r = tid + it * T;
+            const double mt = mreg[it] + gs * lreg[it];
+            mreg[it] = as * mt;
+            if (r < nel * 3 * NE) {
+                if (flags & kAdjMuOut) mo[r] = mreg[it];
+                s_n[r] = -cw * mt;
+            }
+        }
+    }
     __syncthreads();
 
     // P1: metric lines, side normals and edge Jacobians (as the forward stage).
@@ -146,25 +167,9 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
             }
         }
     }
-    __syncthreads();
 
     if (apply) {
-        // P2: mu~ = mu + gs lambda, nu = -c mu~ (J-scaled variables) to s_n; mu' = a_s mu~
-        // (stored). Lanes past the last element hold zeros and store nothing.
-        double *mo = mu + e0 * 3 * NE;
-#pragma unroll
-        for (int it = 0; it < QPT; ++it) {
-            const int r = tid + it * T;
-            const double mt = mreg[it] + gs * lreg[it];
-            mreg[it] = as * mt;
-            if (r < nel * 3 * NE) {
-                if (flags & kAdjMuOut) mo[r] = mreg[it];
-                s_n[r] = -cw * mt;
-            }
-        }
-        __syncthreads();
-
-        // P3: per field f, the lift transposes rho of the four sides and the volume transpose
+        // P3 (same phase as P1, which writes only the metric and side arrays): per field f, the lift transposes rho of the four sides and the volume transpose
         // Fbar_f[a][l] = sum_i Dh[i][a] nu_f[i][l] (column l, into s_a); row l of nu_f stays in
         // registers for P4. One contraction per field and direction: the same work in each wave.
         double row[NF];
@@ -271,43 +276,37 @@ adjoint_stage_kernel(const double *__restrict__ verts, const int32_t *__restrict
         __syncthreads();
 
         // P6 (coalesced): lambda' = lambda + d lambda / J, and the c-derivative term
-        // <d lambda, q_s>.
+        // <d lambda, q_s>. Every thread reads only the LDS values at its own positions r from here
+        // on, which P7 then overwrites: no barrier until P7's.
         double dot = 0.0;
 #pragma unroll
         for (int it = 0; it < QPT; ++it) {
             const int r = tid + it * T;
-            const bool in = r < nel * 3 * NE;
-            const int rr = in ? r : 0;
-            const double dl = s_a[rr];
-            if (qg) dot += (in ? s_n[rr] : 0.0) * qreg[it];
-            lreg[it] += in ? dl : 0.0;
-            if (in) s_a[r] = lreg[it];
+            if (r < nel * 3 * NE) {
+                if (qg) dot += s_n[r] * qreg[it];
+                lreg[it] += s_a[r];
+            }
         }
+        // one slot per wave, one adder per slot and launch: the no-return atomic accumulates the
+        // same sum over the launches without waiting for a read of the slot
         if (cpart) {
             for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
-            if (lane == 0) s_red[f] = dot;
+            if (lane == 0) unsafeAtomicAdd(cpart + (int64_t)blockIdx.x * 3 + f, dot);
         }
-        __syncthreads();
-        // one adder per slot and launch: the no-return atomic gives the same sum without waiting
-        // for a read of the slot at the end of the workgroup
-        if (cpart && tid == 0) unsafeAtomicAdd(cpart + blockIdx.x, (s_red[0] + s_red[1]) + s_red[2]);
         // misfit source of the step end this stage reaches: lambda_p += res_r w_r / J (the weights
-        // come divided by J; receivers in order, each node always handled by the same thread)
+        // come divided by J), added by the thread that holds the node, receivers in order
         if (flags & kAdjSource) {
-            for (int r = 0; r < n_rec; ++r) {
-                const int64_t le = rec_elem[r] - e0;
+            for (int rc = 0; rc < n_rec; ++rc) {
+                const int64_t le = rec_elem[rc] - e0;
                 if (le < 0 || le >= nel) continue;
-                const double *w = rec_w + (int64_t)r * NE;
-                const double c = rec_res[r];
-                for (int k = tid; k < NE; k += T) s_a[le * 3 * NE + k] += c * w[k];
-            }
-            __syncthreads();
+                const double *w = rec_w + (int64_t)rc * NE;
+                const double c = rec_res[rc];
+                const int base = (int)le * 3 * NE;  // field p of the element
 #pragma unroll
-            for (int it = 0; it < QPT; ++it) {
-                const int r = tid + it * T;
-                const bool in = r < nel * 3 * NE;
-                const double v = s_a[in ? r : 0];
-                lreg[it] = in ? v : 0.0;
+                for (int it = 0; it < QPT; ++it) {
+                    const int k = tid + it * T - base;
+                    if (k >= 0 && k < NE) lreg[it] += c * w[k];
+                }
             }
         }
 
@@ -484,7 +483,7 @@ hipError_t launch_jacobian_scale(const DeviceMesh &m, double *x, const int64_t *
 int64_t adjoint_blocks(const DeviceMesh &m) {
     const int N = m.groups[0].N;
     const int epb = 64 / (N + 1);
-    return (m.groups[0].K + epb - 1) / epb;
+    return 3 * ((m.groups[0].K + epb - 1) / epb);  // one slot per wave
 }
 
 hipError_t launch_record(const DeviceMesh &m, const int64_t *elem, const double *w, int n, double *out,

@@ -338,7 +338,7 @@ struct AdjointArgs {
     const double *chi_in = nullptr;
     double *chi_out = nullptr;
     const double *qs = nullptr;  // stage input state for the c-derivative (may be null)
-    double *cpart = nullptr;     // per-workgroup accumulators of <d lambda, q_s> (may be null)
+    double *cpart = nullptr;     // per-wave accumulators of <d lambda, q_s> (may be null)
     int flags = 0;
     double gs = 0.0, as = 0.0, gnext = 0.0;
     // flag 16: receivers (element in device order, weights, residuals) added to lambda'
@@ -347,7 +347,7 @@ struct AdjointArgs {
     int n_rec = 0;
 };
 hipError_t launch_adjoint_stage(DeviceMesh &m, const AdjointArgs &A, hipStream_t s);
-int64_t adjoint_blocks(const DeviceMesh &m);
+int64_t adjoint_blocks(const DeviceMesh &m);  // accumulator slots of cpart (3 per workgroup)
 // x *= J^power per node (power 1 or -1), single-order meshes: the packed state layout, or with
 // rec_elem the receiver weight arrays (n_rec of (N+1)^2, element rec_elem[r] in device order).
 hipError_t launch_jacobian_scale(const DeviceMesh &m, double *x, const int64_t *rec_elem, int64_t n_rec, int power,
