@@ -646,7 +646,7 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
             const int Ne = side ? fc.NR : fc.NL;
             const int cls = mortar_class(fc.offset[side], fc.scale[side]);
             if (cls < 0) return fail(NDG_ERR_UNSUPPORTED, "ndg_mesh_create: a non-conforming face covers neither a whole "
-                                                          "element side nor half of one");
+                                                          "element side nor a dyadic piece of one down to 1/16");
             const bool copy = Ne == fc.Nf && cls == 0;
             fc.mat[side] = copy ? -1 : m->mortar_key_off[(size_t)mortar_matrix_key(Ne, fc.Nf, side, cls, 0)];
             fc.mat[2 + side] = copy ? -1 : m->mortar_key_off[(size_t)mortar_matrix_key(Ne, fc.Nf, side, cls, 1)];
@@ -1015,43 +1015,73 @@ ndg_status build_topology(const HostMesh &h, Topology &t) {
         bcs.push_back(rec);
         bc_bar.push_back(b);
     }
-    // Hanging (2:1) sides: a side a -> b with neither a neighbour nor a bar whose two halves are
-    // the unmatched sides b -> m and m -> a of two other elements, m its midpoint (what
-    // h-refinement leaves, src/meshes/Mesh2D_t.cpp:5661-5985).
+    // Hanging sides: a side a -> b with neither a neighbour nor a bar, covered by a chain of
+    // unmatched sides b -> m1 -> ... -> a of other elements whose nodes lie on it at dyadic
+    // positions, pieces of 1/2 down to 1/2^kMaxSpanLevel of the side (what h-refinement leaves,
+    // src/meshes/Mesh2D_t.cpp:5661-5985). The larger side links to the piece at a, each piece to
+    // the larger side and (hang_second) to the next piece towards b; span is the piece's mortar
+    // class along a -> b.
     std::vector<char> &hang_kind = t.hang_kind;
     std::vector<int64_t> &hang_second = t.hang_second;
     hang_kind.assign((size_t)(4 * K), 0);
     hang_second.assign((size_t)(4 * K), -1);
     t.span.assign((size_t)(4 * K), 0);
     {
-        // sides still unlinked, by start node (the buckets above, linked sides skipped)
+        std::vector<int64_t> chain;
+        std::vector<int> cls;
         for (int64_t es = 0; es < 4 * K; ++es) {
             if (link[es] != INT64_MIN || hang_kind[es]) continue;
             const int64_t a = h.quads[es], b = end_node(es);
-            for (int64_t ib = first[(size_t)b]; ib < first[(size_t)b + 1]; ++ib) {  // b -> m
-                const int64_t s1 = by_start[(size_t)ib];
-                if (s1 == es || hang_kind[s1] || link[s1] != INT64_MIN) continue;
-                const int64_t mnode = end_node(s1);
-                const v2 pa = P(a), pb = P(b), pm = P(mnode);
-                const double len = mag(pb.x - pa.x, pb.y - pa.y);
-                if (std::fabs(pm.x - (pa.x + pb.x) / 2) > 1e-12 * len || std::fabs(pm.y - (pa.y + pb.y) / 2) > 1e-12 * len)
-                    continue;
-                int64_t s2 = -1;  // m -> a
-                for (int64_t ic = first[(size_t)mnode]; ic < first[(size_t)mnode + 1]; ++ic) {
-                    const int64_t c = by_start[(size_t)ic];
-                    if (end_node(c) == a && !hang_kind[c] && link[c] == INT64_MIN) s2 = c;
+            const v2 pa = P(a), pb = P(b);
+            const double ex = pb.x - pa.x, ey = pb.y - pa.y, len = mag(ex, ey);
+            // position of node n along a -> b, or -1 off the segment
+            auto along = [&](int64_t n) {
+                const v2 p = P(n);
+                const double u = ((p.x - pa.x) * ex + (p.y - pa.y) * ey) / (len * len);
+                if (std::fabs(pa.x + u * ex - p.x) > 1e-12 * len || std::fabs(pa.y + u * ey - p.y) > 1e-12 * len)
+                    return -1.0;
+                return u;
+            };
+            chain.clear();
+            cls.clear();
+            int64_t cur = b;
+            double ucur = 1.0;
+            bool ok = true;
+            while (cur != a && ok) {
+                int64_t pick = -1;
+                double unext = -1.0;
+                for (int64_t i = first[(size_t)cur]; i < first[(size_t)cur + 1]; ++i) {  // cur -> m along the side
+                    const int64_t c = by_start[(size_t)i];
+                    if (c == es || hang_kind[c] || link[c] != INT64_MIN) continue;
+                    const int64_t mn = end_node(c);
+                    const double u = mn == a ? 0.0 : along(mn);
+                    if (u < 0.0 || u >= ucur - 1e-12) continue;
+                    if (u > unext) pick = c, unext = u;
                 }
-                if (s2 < 0) continue;
-                hang_kind[es] = 2;
-                link[es] = s2;          // the half starting at a first, then s1
-                hang_second[s2] = s1;
-                hang_kind[s1] = hang_kind[s2] = 1;
-                link[s1] = link[s2] = es;
-                t.span[s2] = (int8_t)make_class(1, 0);
-                t.span[s1] = (int8_t)make_class(1, 1);
-                n_faces += 2;           // the two half faces
-                break;
+                // the piece [unext, ucur] must be j / 2^d .. (j + 1) / 2^d with 1 <= d <= kMaxSpanLevel
+                int d = 1;
+                while (d <= kMaxSpanLevel && std::fabs((ucur - unext) * (1 << d) - 1.0) > 1e-9) ++d;
+                const double j = unext * (1 << d);
+                if (pick < 0 || d > kMaxSpanLevel || std::fabs(j - std::round(j)) > 1e-9) ok = false;
+                else {
+                    chain.push_back(pick);
+                    cls.push_back(make_class(d, (int)std::lround(j)));
+                    cur = end_node(pick);
+                    ucur = unext;
+                }
             }
+            if (!ok || chain.size() < 2) continue;
+            // chain runs from b to a; the larger side's first piece is the one at a
+            const size_t n = chain.size();
+            hang_kind[es] = 2;
+            link[es] = chain[n - 1];
+            for (size_t k = 0; k < n; ++k) {
+                hang_kind[chain[k]] = 1;
+                link[chain[k]] = es;
+                t.span[chain[k]] = (int8_t)cls[k];
+                hang_second[chain[k]] = k == 0 ? -1 : chain[k - 1];
+            }
+            n_faces += (int64_t)n;
         }
     }
     for (int64_t es = 0; es < 4 * K; ++es)
@@ -1806,8 +1836,11 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
             const int64_t es = 4 * e + s;
             if (mesh->periodic_link[es])
                 return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: splitting an element with a periodic side is not provided");
-            if (mesh->hang_kind[es] == 1 && !split[mesh->link[es] / 4])
-                return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a split would leave more than two faces on a side");
+            // a piece of 1/2^kMaxSpanLevel of a larger side that splits alone would leave faces of
+            // half that size
+            if (mesh->hang_kind[es] == 1 && class_level(mesh->span[es]) >= kMaxSpanLevel &&
+                !split[mesh->link[es] / 4])
+                return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: a split would leave a face smaller than 1/16 of a side");
         }
     }
     auto node_xy = [&](const std::vector<double> &xy, int64_t n) { return v2{xy[2 * n], xy[2 * n + 1]}; };
@@ -1816,8 +1849,13 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
     nh.xy = h.xy;
     std::unordered_map<uint64_t, int64_t> mids;
     auto mid_key = [](int64_t a, int64_t b) { return edge_key(std::min(a, b), std::max(a, b)); };
+    // a larger side's midpoint is the node where its pieces cross 1/2: the start of the piece that
+    // ends there (pieces run against the larger side)
     for (int64_t es = 0; es < 4 * K; ++es)
-        if (mesh->hang_kind[es] == 2) mids[mid_key(h.quads[es], end_node(es))] = h.quads[mesh->link[es]];
+        if (mesh->hang_kind[es] == 2)
+            for (int64_t c = mesh->link[es]; c >= 0; c = mesh->hang_second[(size_t)c])
+                if (class_offset(mesh->span[c]) + class_scale(mesh->span[c]) == 0.5)
+                    mids[mid_key(h.quads[es], end_node(es))] = h.quads[c];
     auto midpoint = [&](int64_t a, int64_t b) {
         auto it = mids.find(mid_key(a, b));
         if (it != mids.end()) return it->second;
