@@ -232,14 +232,17 @@ EE_TARGETS = (1e-4, 1e-6, 1e-8, 1e-10)
 
 
 def equal_error_leg(nd, torch, stream, res=EE_RES, t_end=EE_T, orders=(4, 6, 8, 10, 12),
-                    tolerances=(1e-3, 1e-5, 1e-7, 1e-9, 1e-11, 1e-13), small_cfl=(0.25, 0.125)):
+                    tolerances=(1e-3, 1e-5, 1e-7, 1e-9, 1e-11, 1e-13), small_cfl=(0.25, 0.125),
+                    fp32c_tolerances=(1e-3, 1e-5)):
     """The metric's comparison at equal L2 error. On the periodic multi-pulse workload, solved to
     t_end with CFL 0.5, every run's relative L2 error against the exact solution (Gauss quadrature
     of each element) and its time are measured for
       - plain: uniform order N (FP64, and the FP32 compensated form), for N in `orders`;
       - batched: the p-adaptive mixed-order path (N0 = 4, N <= 12, no splitting on the periodic
         square), adapt every 20 steps, after an iterative pre-conditioning of the initial
-        condition (4 rounds of one step), for each estimator tolerance in `tolerances`.
+        condition (4 rounds of one step), for each estimator tolerance in `tolerances`; and the
+        same path stepping in the FP32 compensated form (batched and mixed precision together)
+        for the tolerances in `fp32c_tolerances` (its error floor is near 2e-7).
     At CFL 0.5 the RK3 time error (dt ~ 1/N^2) bounds every run near 3e-10 on this mesh, so the
     highest orders and the smallest tolerance also run at the CFL numbers in `small_cfl`.
     For each target error, each form's cheapest run that reaches it is compared with the plain
@@ -298,14 +301,20 @@ def equal_error_leg(nd, torch, stream, res=EE_RES, t_end=EE_T, orders=(4, 6, 8, 
             print(f"[equal-error] {runs[-1]}", file=sys.stderr, flush=True)
         mesh.close()
 
-    def batched(tol, cfl):
+    def batched(tol, cfl, prec="fp64"):
         mesh = nd.Mesh2D_t(desc, 4, 12, 5, 0, EE_INTERVAL, 0, tol, 1e-14, 0.0, ndg)
         mesh.set_stream(stream.cuda_stream)
         mesh.initial_conditions(pulses, period=(2.0, 2.0))
         solver = solver_for(cfl)
-        s, wall = timed_solve(mesh, cfl, pre=lambda: solver.pre_condition_iterative(ndg, mesh, 4, 1))
+
+        def pre():
+            solver.pre_condition_iterative(ndg, mesh, 4, 1)
+            mesh.set_precision(prec)
+
+        s, wall = timed_solve(mesh, cfl, pre=pre)
+        mesh.set_precision("fp64")
         counts = {int(k): int((mesh.orders_ == k).sum()) for k in np.unique(mesh.orders_)}
-        runs.append({"form": "batched_fp64", "tolerance_min": tol, "cfl": cfl, "seconds": s, "wall_s": wall,
+        runs.append({"form": "batched_" + prec, "tolerance_min": tol, "cfl": cfl, "seconds": s, "wall_s": wall,
                      "steps": solver.n_steps_, "final_dofs": mesh.n_dofs, "final_elements_per_order": counts,
                      "l2_error": error_of(mesh)})
         print(f"[equal-error] {runs[-1]}", file=sys.stderr, flush=True)
@@ -315,6 +324,8 @@ def equal_error_leg(nd, torch, stream, res=EE_RES, t_end=EE_T, orders=(4, 6, 8, 
         plain(n, CFL, ("fp64", "fp32c"))
     for tol in tolerances:
         batched(tol, CFL)
+    for tol in fp32c_tolerances:
+        batched(tol, CFL, "fp32c")
     for cfl in small_cfl:
         for n in orders[-2:]:
             plain(n, cfl, ("fp64",))

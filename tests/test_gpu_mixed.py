@@ -214,6 +214,31 @@ def test_adaptive_solve_matches_oracle(tables, periodic, bc):
         assert rel(g, r) < TOL
 
 
+@pytest.mark.parametrize("split", [0, 1])
+def test_adaptive_solve_in_fp32c_form(split):
+    """The adaptive solve stepping in the FP32 compensated form (each adaptation converts the state
+    to FP64, rebuilds and converts back): the same step count, time, elements and orders as the
+    oracle's FP64 adaptive solve on this workload, and the state within 2e-5 of it (the FP32
+    form's bar against FP64, relative to the largest value of each field)."""
+    say("adaptive solve in the fp32c form, split level", split)
+    desc = nd.square_mesh(8, bc=(nd.BC_WALL,) * 4, jitter=0.1, seed=7)
+    gm = nd.Mesh2D_t(desc, 3, 9, 5, split, 4, 0, 1e-4, 1e-14, 0.0, nd.NDG_t(9, 5))
+    om = O.Mesh(O.Tables(9, 5), desc, 3, tol_min=1e-4, tol_max=1e-14)
+    gm.initial_conditions(PULSES)
+    om.initial_conditions(PULSES)
+    gm.set_precision("fp32c")
+    solver = nd.Solver2D_t(0.5, [0.2])
+    solver.solve(None, gm)
+    steps, t_cpu = om.solve_adaptive(0.5, 0.2, 4, 9, split)
+    say("steps", steps, "K", om.K, "orders", np.bincount(om.N).tolist())
+    assert solver.n_steps_ == steps and solver.time_ == t_cpu
+    assert gm.n_elements_ == om.K and np.array_equal(gm.orders_, om.N)
+    assert om.N.max() > 3 and (split == 0 or om.K > 64)
+    gm.set_precision("fp64")
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < 2e-5
+
+
 @pytest.mark.parametrize("bc,ic", [((nd.BC_WALL,) * 4, PULSES), (MIXED_BC, None),
                                    ((nd.BC_WALL,) * 4, [(0.1, -0.2, 0.08, 1.0)])])
 def test_adaptive_split_solve_matches_oracle(tables, bc, ic):
