@@ -219,6 +219,49 @@ def test_full_size_properties(tables):
     assert abs(np.sum(W * Gp)) < 1e-11 * np.sum(W * np.abs(Gp))
 
 
+def test_full_size_precision_workload_properties(tables):
+    """1024 x 1024 periodic elements, N = 7 (the precision workload's size): over 20 RK3 steps of
+    four pulses, the FP64 path keeps the mass sum(w w J p) to roundoff and does not gain energy
+    sum(w w J (p^2 + u^2 + v^2)) / 2 (upwind fluxes dissipate, and RK3 at CFL 0.5 is stable);
+    the FP32 form keeps the mass too (its state is a compensated pair; the energy may rise by
+    its rounding) and stays within 1e-6 (l2, relative; observed 2.1e-7) of the FP64 state. No
+    oracle: size-independent properties only."""
+    say("full size 1024x1024 N=7 periodic")
+    ndg, ot = tables
+    N, res = 7, 1024
+    desc = nd.square_mesh(res, x_periodic=True, y_periodic=True)
+    rng = np.random.default_rng(4)
+    pulses = [(float(rng.uniform(-0.7, 0.7)), float(rng.uniform(-0.7, 0.7)), 0.2, float(rng.uniform(0.2, 1.0)))
+              for _ in range(4)]
+    w = ot.get(O.O_WEIGHTS, N)
+    h = 2.0 / res  # the unrefined square: J = h^2 / 4 on every element
+    W = np.tile(np.outer(w, w).ravel() * h * h / 4, len(desc.quads))
+    mass = lambda q: float(np.sum(W * q[0]))
+    energy = lambda q: float(np.sum(W * (q[0] ** 2 + q[1] ** 2 + q[2] ** 2))) / 2
+    states = {}
+    for prec in ("fp64", "fp32c"):
+        gm = nd.Mesh2D_t(desc, N, N, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
+        gm.initial_conditions(pulses, period=(2.0, 2.0))
+        q0 = gm.get_solution()
+        gm.set_precision(prec)
+        solver = nd.Solver2D_t(0.5, [0.0])
+        solver.run_steps(gm, 20)
+        gm.set_precision("fp64")
+        q = gm.get_solution()
+        scale = float(np.sum(W * np.abs(q0[0])))
+        drift = abs(mass(q) - mass(q0)) / scale
+        say(prec, "mass drift", drift, "energy ratio", energy(q) / energy(q0))
+        assert drift < (1e-12 if prec == "fp64" else 1e-10)  # observed 6e-15 and 1.3e-14
+        if prec == "fp64":
+            assert energy(q) <= energy(q0)
+        states[prec] = q
+        gm.close()
+    d = np.sqrt(sum(float(np.sum((a - b) ** 2)) for a, b in zip(states["fp32c"], states["fp64"])))
+    r = np.sqrt(sum(float(np.sum(b ** 2)) for b in states["fp64"]))
+    say("fp32c vs fp64 l2", d / r)
+    assert d / r < 1e-6
+
+
 # ---------------------------------------------------------------------------------------------
 # FP32 form with compensated state (no reference counterpart): checked against the FP64 oracle
 # with a tolerance set by FP32 rounding of the right-hand side, and against the exact solution.
