@@ -270,8 +270,7 @@ ndg_status ndg_pre_condition(ndg_mesh *mesh, double cfl, int64_t n_adaptivity_st
  *   faces appear, also across periodic sides (split_interfaces, :6343-6756: each end of the
  *   period gets its own faces against the other end's trace), with any number of faces on a side
  *   down to pieces of 1/16 of it; a split that would leave a smaller piece returns
- *   NDG_ERR_UNSUPPORTED and changes nothing, as does, in the NDG_ADAPT_HOST mode, splitting an
- *   element with a periodic side;
+ *   NDG_ERR_UNSUPPORTED and changes nothing (in either NDG_ADAPT_* mode);
  * - otherwise the p_adapt branch (:5026-5049): every element with refine && sigma >= 1 &&
  *   N + 2 <= N_max is raised by 2 (Element2D_t.cpp:464-579); ghosts and faces follow
  *   (adjust_boundaries / adjust_interfaces / adjust_faces, :7482-7538), including the reference's

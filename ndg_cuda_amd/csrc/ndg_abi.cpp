@@ -93,7 +93,11 @@ struct HostMesh {
     std::vector<int64_t> bars;     // [B][2]
     std::vector<int32_t> bar_type; // [B]
     std::vector<int64_t> bar_donor;  // [B] (empty: no periodic bars)
+    // [B][2] translation from a periodic bar to its partner end (empty: the centre of the donor
+    // minus the bar's own centre, which holds while every periodic pair is conforming)
+    std::vector<double> bar_shift;
 };
+
 
 struct Topology {
     std::vector<int64_t> link;
@@ -247,6 +251,17 @@ namespace {
 struct v2 {
     double x, y;
 };
+// The translation taking periodic bar b onto its partner end.
+v2 bar_translation(const HostMesh &h, int64_t b) {
+    if (h.bar_shift.size() == h.bar_type.size() * 2) return v2{h.bar_shift[2 * b], h.bar_shift[2 * b + 1]};
+    const int64_t d = h.bar_donor[(size_t)b];
+    auto centre = [&](int64_t k) {
+        const int64_t u = h.bars[2 * k], w = h.bars[2 * k + 1];
+        return v2{(h.xy[2 * u] + h.xy[2 * w]) / 2, (h.xy[2 * u + 1] + h.xy[2 * w + 1]) / 2};
+    };
+    const v2 cb = centre(b), cd = centre(d);
+    return v2{cd.x - cb.x, cd.y - cb.y};
+}
 
 double mag(double x, double y) { return std::sqrt(x * x + y * y); }
 
@@ -600,13 +615,38 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
         for (int s = 0; s < 4; ++s) {
             const int64_t es = 4 * e + s, l = link[es];
             const int hk = m->hang_kind[es];
-            if (hk != 0 && periodic_link[es])  // only the device adaptation makes these
-                return fail(NDG_ERR_UNSUPPORTED, "mesh build: hanging periodic sides are built by the device path");
-            if (hk == 2) continue;  // built from its two halves
+            const bool periodic = periodic_link[es] != 0;
+            if (hk == 2 && periodic) {
+                // a larger periodic side owns one face per piece at the other end, against a ghost
+                // holding that piece's trace, each writing only this side's block for the piece
+                const v2 f0 = corner(e, s), f1 = corner(e, s + 1);
+                v2 PL[4] = {corner(e, 0), corner(e, 1), corner(e, 2), corner(e, 3)};
+                const int64_t slot = bcs[(size_t)mortar_rec[es]].slot;
+                int64_t h = 0;
+                for (int64_t c = l; c >= 0; c = m->hang_second[(size_t)c], ++h) {
+                    MortarFace fc{};
+                    fc.NL = order[e];
+                    fc.NR = order[c / 4];
+                    fc.Nf = std::max(fc.NL, fc.NR);
+                    fc.trL = trace_at(e, s);
+                    fc.trR = trace_at(c / 4, (int)(c % 4));
+                    double tx, ty;
+                    face_frame(f0, f1, f0, f1, fc.nx, fc.ny, tx, ty);
+                    fc.offset[0] = class_offset(m->span[(size_t)c]);
+                    fc.scale[0] = class_scale(m->span[(size_t)c]);
+                    fc.offset[1] = 0.0;
+                    fc.scale[1] = 1.0;
+                    fc.sfL = side_scale(PL, s);
+                    fc.slotL = slot + h * 3 * (int64_t)(order[e] + 1);
+                    fc.slotR = -1;
+                    faces.push_back(fc);
+                }
+                continue;
+            }
+            if (hk == 2) continue;  // built from its small sides
             if (l < 0 || (hk == 0 && order[l / 4] == order[e])) continue;
             const int64_t r = l / 4;
             const int rs = (int)(l % 4);
-            const bool periodic = periodic_link[es] != 0;
             if (hk == 0 && !periodic && r < e) continue;  // the face belongs to its first element
             MortarFace fc{};
             fc.NL = order[e];
@@ -627,7 +667,12 @@ ndg_status build_device(ndg_mesh *m, const std::vector<int32_t> &order, DeviceMe
             fc.sfL = side_scale(PL, s);
             fc.slotL = add_mortar_side(es);
             fc.slotR = -1;
-            if (hk == 1) {  // a piece of the larger side l
+            if (hk == 1 && periodic) {  // this whole side against its piece of the larger partner
+                fc.offset[0] = 0.0;
+                fc.scale[0] = 1.0;
+                fc.offset[1] = class_offset(m->span[(size_t)es]);
+                fc.scale[1] = class_scale(m->span[(size_t)es]);
+            } else if (hk == 1) {  // a piece of the larger side l
                 v2 PR[4] = {corner(r, 0), corner(r, 1), corner(r, 2), corner(r, 3)};
                 fc.sfR = side_scale(PR, rs);
                 const BcSide &big = bcs[(size_t)mortar_rec[l]];
@@ -1082,6 +1127,67 @@ ndg_status build_topology(const HostMesh &h, Topology &t) {
                 hang_second[chain[k]] = k == 0 ? -1 : chain[k - 1];
             }
             n_faces += (int64_t)n;
+        }
+    }
+    // Periodic sides of different lengths (splits next to a period, split_interfaces,
+    // src/meshes/Mesh2D_t.cpp:6343-6756): a periodic side whose partner end is covered by shorter
+    // bars is a larger side, hanging as an interior one would be; its pieces are the shorter bars
+    // whose donor it is, placed along it by their translation. Each end keeps its own faces, so the
+    // larger side adds one face per piece beyond the one its bar counted.
+    if (!h.bar_donor.empty()) {
+        auto blen = [&](int64_t k) {
+            const v2 u = P(h.bars[2 * k]), w = P(h.bars[2 * k + 1]);
+            return mag(w.x - u.x, w.y - u.y);
+        };
+        std::vector<std::vector<int64_t>> pieces((size_t)B);
+        for (int64_t x = 0; x < B; ++x)
+            if (h.bar_type[x] == NDG_BC_PERIODIC && blen(x) < blen(h.bar_donor[x]) * (1 - 1e-9))
+                pieces[(size_t)h.bar_donor[x]].push_back(x);
+        for (int64_t b = 0; b < B; ++b) {
+            if (pieces[(size_t)b].empty()) continue;
+            const int64_t es = bar_side[b];
+            const v2 pa = P(h.quads[es]), pb = P(end_node(es));
+            const double ex = pb.x - pa.x, ey = pb.y - pa.y, l2 = ex * ex + ey * ey;
+            struct Piece {
+                double lo;
+                int64_t side;
+                int cls;
+            };
+            std::vector<Piece> ps;
+            for (int64_t x : pieces[(size_t)b]) {
+                const v2 T = bar_translation(h, x);
+                double u[2];
+                for (int k = 0; k < 2; ++k) {
+                    const v2 q = P(h.bars[2 * x + k]);
+                    u[k] = ((q.x + T.x - pa.x) * ex + (q.y + T.y - pa.y) * ey) / l2;
+                }
+                const double lo = std::min(u[0], u[1]), size = std::fabs(u[1] - u[0]);
+                int d = 1;
+                while (d <= kMaxSpanLevel && std::fabs(size * (1 << d) - 1.0) > 1e-9) ++d;
+                const double j = lo * (1 << d);
+                if (d > kMaxSpanLevel || std::fabs(j - std::round(j)) > 1e-9)
+                    return fail(NDG_ERR_INVALID, "mesh topology: a periodic piece is not a dyadic part of its partner");
+                ps.push_back(Piece{lo, bar_side[x], make_class(d, (int)std::lround(j))});
+            }
+            std::sort(ps.begin(), ps.end(), [](const Piece &p1, const Piece &p2) { return p1.lo < p2.lo; });
+            double covered = 0.0;
+            for (const Piece &pc : ps) {
+                if (std::fabs(pc.lo - covered) > 1e-9)
+                    return fail(NDG_ERR_INVALID, "mesh topology: a periodic side is not covered by its partner's pieces");
+                covered = class_offset(pc.cls) + class_scale(pc.cls);
+            }
+            if (std::fabs(covered - 1.0) > 1e-9)
+                return fail(NDG_ERR_INVALID, "mesh topology: a periodic side is not covered by its partner's pieces");
+            hang_kind[es] = 2;
+            link[es] = ps[0].side;
+            for (size_t k = 0; k < ps.size(); ++k) {
+                const int64_t c = ps[k].side;
+                hang_kind[c] = 1;
+                link[c] = es;
+                t.span[c] = (int8_t)ps[k].cls;
+                hang_second[c] = k + 1 < ps.size() ? ps[k + 1].side : -1;
+            }
+            n_faces += (int64_t)ps.size() - 1;
         }
     }
     for (int64_t es = 0; es < 4 * K; ++es)
@@ -1834,8 +1940,6 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
         if (!split[e]) continue;
         for (int s = 0; s < 4; ++s) {
             const int64_t es = 4 * e + s;
-            if (mesh->periodic_link[es])
-                return fail(NDG_ERR_UNSUPPORTED, "ndg_adapt: splitting an element with a periodic side is not provided");
             // a piece of 1/2^kMaxSpanLevel of a larger side that splits alone would leave faces of
             // half that size
             if (mesh->hang_kind[es] == 1 && class_level(mesh->span[es]) >= kMaxSpanLevel &&
@@ -1851,8 +1955,10 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
     auto mid_key = [](int64_t a, int64_t b) { return edge_key(std::min(a, b), std::max(a, b)); };
     // a larger side's midpoint is the node where its pieces cross 1/2: the start of the piece that
     // ends there (pieces run against the larger side)
+    // (a larger periodic side's pieces lie at the other end of the period: it splits at its own
+    // midpoint)
     for (int64_t es = 0; es < 4 * K; ++es)
-        if (mesh->hang_kind[es] == 2)
+        if (mesh->hang_kind[es] == 2 && !mesh->periodic_link[es])
             for (int64_t c = mesh->link[es]; c >= 0; c = mesh->hang_second[(size_t)c])
                 if (class_offset(mesh->span[c]) + class_scale(mesh->span[c]) == 0.5)
                     mids[mid_key(h.quads[es], end_node(es))] = h.quads[c];
@@ -1911,30 +2017,65 @@ ndg_status adapt_split(ndg_mesh *mesh, const std::vector<int32_t> &ref, const st
         pos += 4;
     }
     std::vector<char> bar_zero;
-    std::vector<int64_t> bar_new((size_t)B, -1);
+    std::vector<int64_t> bar_new((size_t)B, -1), bar_count((size_t)B, 1), bar_parent;
+    const bool periodic_bars = !h.bar_donor.empty();
     for (int64_t b = 0; b < B; ++b) {
         const int64_t es = mesh->bar_side[b], e = es / 4;
         bar_new[b] = (int64_t)nh.bar_type.size();
+        const v2 T = periodic_bars && h.bar_type[b] == NDG_BC_PERIODIC ? bar_translation(h, b) : v2{0.0, 0.0};
+        auto push = [&](int64_t n0, int64_t n1, char zero) {
+            nh.bars.push_back(n0);
+            nh.bars.push_back(n1);
+            nh.bar_type.push_back(h.bar_type[b]);
+            if (periodic_bars) {
+                nh.bar_donor.push_back(-1);
+                nh.bar_shift.push_back(T.x);
+                nh.bar_shift.push_back(T.y);
+            }
+            bar_parent.push_back(b);
+            bar_zero.push_back(zero);
+        };
         if (split[e]) {  // two ghosts: (P(s+1), m) and (m, P(s))
             const int64_t u0 = h.quads[es], u1 = end_node(es), md = midpoint(u0, u1);
-            const int64_t halves[2][2] = {{u1, md}, {md, u0}};
-            for (const auto &pr : halves) {
-                nh.bars.push_back(pr[0]);
-                nh.bars.push_back(pr[1]);
-                nh.bar_type.push_back(h.bar_type[b]);
-                if (!h.bar_donor.empty()) nh.bar_donor.push_back(-1);
-                bar_zero.push_back(0);
-            }
+            push(u1, md, 0);
+            push(md, u0, 0);
+            bar_count[b] = 2;
         } else {
-            nh.bars.push_back(h.bars[2 * b]);
-            nh.bars.push_back(h.bars[2 * b + 1]);
-            nh.bar_type.push_back(h.bar_type[b]);
-            if (!h.bar_donor.empty()) nh.bar_donor.push_back(h.bar_donor[b]);
-            bar_zero.push_back(mesh->bar_zero[b] && !raise[e]);
+            push(h.bars[2 * b], h.bars[2 * b + 1], mesh->bar_zero[b] && !raise[e]);
         }
     }
-    for (size_t b = 0; b < nh.bar_donor.size(); ++b)
-        if (nh.bar_type[b] == NDG_BC_PERIODIC) nh.bar_donor[b] = bar_new[(size_t)nh.bar_donor[b]];
+    // A periodic bar's donor is the bar at the partner end holding its translated centre (its
+    // match, or the whole side it is a piece of), or, for a side whose partner is in pieces, the
+    // first piece that touches it. The candidates are the new bars of the parent's partners: its
+    // donor, the bars whose donor it is, and the other pieces of its donor.
+    std::vector<std::vector<int64_t>> donor_of((size_t)(periodic_bars ? B : 0));
+    for (int64_t b = 0; b < (periodic_bars ? B : 0); ++b)
+        if (h.bar_type[b] == NDG_BC_PERIODIC) donor_of[(size_t)h.bar_donor[b]].push_back(b);
+    for (size_t x = 0; x < nh.bar_donor.size(); ++x) {
+        if (nh.bar_type[x] != NDG_BC_PERIODIC) continue;
+        const int64_t b = bar_parent[x], d = h.bar_donor[(size_t)b];
+        std::vector<int64_t> partners{d};
+        for (int64_t k : donor_of[(size_t)b]) partners.push_back(k);
+        for (int64_t k : donor_of[(size_t)d]) partners.push_back(k);
+        const int64_t u = nh.bars[2 * x], w = nh.bars[2 * x + 1];
+        const v2 c{(nh.xy[2 * u] + nh.xy[2 * w]) / 2 + nh.bar_shift[2 * x],
+                   (nh.xy[2 * u + 1] + nh.xy[2 * w + 1]) / 2 + nh.bar_shift[2 * x + 1]};
+        int64_t pick = -1;
+        for (int pass = 0; pass < 2 && pick < 0; ++pass)
+            for (int64_t pb : partners) {
+                if (pb == b) continue;
+                for (int64_t y = bar_new[(size_t)pb]; y < bar_new[(size_t)pb] + bar_count[(size_t)pb] && pick < 0; ++y) {
+                    const v2 p0 = node_xy(nh.xy, nh.bars[2 * y]), p1 = node_xy(nh.xy, nh.bars[2 * y + 1]);
+                    const double vx = p1.x - p0.x, vy = p1.y - p0.y, l2 = vx * vx + vy * vy;
+                    const double t = ((c.x - p0.x) * vx + (c.y - p0.y) * vy) / l2;
+                    const double off = std::fabs((c.x - p0.x) * vy - (c.y - p0.y) * vx) / l2;  // over the length
+                    const double m = pass == 0 ? 1e-9 : -1e-9;
+                    if (off < 1e-9 && t > m && t < 1 - m) pick = y;
+                }
+            }
+        if (pick < 0) return fail(NDG_ERR_INVALID, "ndg_adapt: a periodic side has no partner after the split");
+        nh.bar_donor[x] = pick;
+    }
     Topology topo;
     ndg_status st = build_topology(nh, topo);
     if (st != NDG_OK) return st;

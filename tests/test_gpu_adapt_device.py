@@ -4,9 +4,9 @@ host from the node lists) give the same element counts, face counts, orders, ele
 Hilbert statuses, rotations, split levels, estimates and states bit for bit; refused adaptations
 change nothing; and the device path's adaptive loop matches the oracle at the batched workload's
 shape. The oracle comparisons of the other adaptive tests run through the device path too (it is
-the default). Splits across periodic sides (which the host path does not provide) and sides
-covered by more than two faces are compared with the oracle, face count per side included, and
-with the same solve shifted by half a period; the multi-face splits also against the host path.
+the default). Splits across periodic sides and sides covered by more than two faces are compared
+with the oracle, face count per side included, with the same solve shifted by half a period, and
+with the host path bit for bit.
 """
 import numpy as np
 import pytest
@@ -232,6 +232,38 @@ def test_multi_face_splits_equal_host_path_and_oracle(name):
     assert_identical(dm, hm, t)
 
 
+@pytest.mark.parametrize("name", list(PERIODIC_SPLITS))
+def test_periodic_splits_equal_host_path(name):
+    """The splits next to the periods and the multi-face splits of the oracle comparison above,
+    through the host path too (periodic bars split with their translation, donors found again
+    at the partner end, a periodic side covered by shorter partner bars hanging as an interior
+    side would): the same element-side links, pieces, periodic flags, statuses and states as the
+    device path bit for bit after every adaptation and after 20 RK3 steps."""
+    say("periodic splits, device vs host", name)
+    desc = nd.square_mesh(8, x_periodic=True, y_periodic=True, jitter=0.0 if name == "sixteenth" else 0.1, seed=7)
+    dm, hm = pair(desc, 6, 8, 5, 0, 1e-4, PULSES)
+    om = O.Mesh(O.Tables(8, 5), desc, 6, tol_min=1e-4, tol_max=1e-14)
+    rng = np.random.default_rng(11)
+    for pred in PERIODIC_SPLITS[name]:
+        q = rough_state(om, pred, rng)
+        for m in (dm, hm):
+            m.set_solution(*q)
+        om.set_state(*q)
+        om.estimate_error()
+        n_o = om.adapt(8, 5)
+        assert n_o > 0
+        assert dm.adapt(8, 5) == n_o and hm.adapt(8, 5) == n_o
+        assert_identical(dm, hm, 0.0)
+        assert np.array_equal(faces_per_side(hm), om.side_faces())
+    pulses = [(0.9, 0.85, 0.2, 1.0), (-0.6, 0.2, 0.3, 0.5)]
+    for m in (dm, hm):
+        m.initial_conditions(pulses, period=(2.0, 2.0))
+    solver = nd.Solver2D_t(0.5)
+    t = solver.run_steps(dm, 20)
+    assert solver.run_steps(hm, 20) == t
+    assert_identical(dm, hm, t)
+
+
 @pytest.mark.parametrize("mode", ["device", "host"])
 def test_refuses_faces_below_a_sixteenth(mode):
     """A small element covering 1/16 of a larger side that splits while the larger element does
@@ -256,9 +288,10 @@ def test_refuses_faces_below_a_sixteenth(mode):
         assert np.array_equal(a, b)
 
 
-def corner_pulse_solve(x0, y0, split, tol, width):
+def corner_pulse_solve(x0, y0, split, tol, width, mode="device"):
     desc = nd.square_mesh(8, x_periodic=True, y_periodic=True)
     m = nd.Mesh2D_t(desc, 4, 10, 5, split, 5, 0, tol, 1e-14, 0.0, nd.NDG_t(10, 5))
+    m.set_adapt_mode(mode)
     m.initial_conditions([(x0, y0, width, 1.0)], period=(2.0, 2.0))
     solver = nd.Solver2D_t(0.5, [0.3])
     solver.solve(None, m)
@@ -302,6 +335,18 @@ def test_device_periodic_split_solve_matches_oracle_and_the_shifted_solve(split,
         assert a.orders_[e] == b.orders_[j]
         for f in range(3):
             assert np.abs(qa[f][off_a[e]:off_a[e + 1]] - qb[f][off_b[j]:off_b[j + 1]]).max() < 1e-12 * scale
+
+
+@pytest.mark.parametrize("split,tol,width,most", SHIFT_CASES)
+def test_periodic_split_solve_equals_host_path(split, tol, width, most):
+    """The corner-pulse hp solve above through the host path: the same steps, time, topology and
+    state as the device path bit for bit."""
+    say("periodic corner pulse solve, device vs host, split level", split)
+    a, sa = corner_pulse_solve(0.95, 0.95, split, tol, width)
+    b, sb = corner_pulse_solve(0.95, 0.95, split, tol, width, mode="host")
+    assert sa.n_steps_ == sb.n_steps_ and sa.time_ == sb.time_
+    assert faces_per_side(b).max() == most
+    assert_identical(a, b, sa.time_)
 
 
 def test_device_adaptation_at_workload_shape_matches_oracle():
