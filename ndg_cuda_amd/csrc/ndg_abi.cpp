@@ -1640,8 +1640,17 @@ ndg_status launch_step_body(ndg_mesh *m, double dt, bool parallel, hipStream_t s
     for (int st = 0; st < 3; ++st) {
         HIP_TRY(launch_mortar(d, s), "mortar kernel");
         if (parallel && ng > 1) {
+            // one branch per group, the groups with the most bytes forked first (their launches
+            // are the longest)
+            std::vector<int> gorder(ng);
+            for (int g = 0; g < ng; ++g) gorder[g] = g;
+            auto bytes = [&](int g) {
+                const double nf = d.groups[g].N + 1;
+                return (double)d.groups[g].K * (88 * nf * nf + 192 * nf + 80);
+            };
+            std::stable_sort(gorder.begin(), gorder.end(), [&](int x, int y) { return bytes(x) > bytes(y); });
             HIP_TRY(hipEventRecord(m->side_events[0], s), "fork");
-            for (int g = 0; g < ng; ++g) {
+            for (int g : gorder) {
                 HIP_TRY(hipStreamWaitEvent(m->side_streams[g], m->side_events[0], 0), "fork wait");
                 HIP_TRY(launch_group_stage(d, g, st, kAm[st], kGm[st] * dt, m->side_streams[g]), "stage kernel");
                 HIP_TRY(hipEventRecord(m->side_events[1 + g], m->side_streams[g]), "join");
