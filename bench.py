@@ -233,7 +233,7 @@ EE_TARGETS = (1e-4, 1e-6, 1e-8, 1e-10)
 
 def equal_error_leg(nd, torch, stream, res=EE_RES, t_end=EE_T, orders=(4, 6, 8, 10, 12),
                     tolerances=(1e-3, 1e-5, 1e-7, 1e-9, 1e-11, 1e-13), small_cfl=(0.25, 0.125),
-                    fp32c_tolerances=(1e-3, 1e-5)):
+                    fp32c_tolerances=(1e-3, 1e-5), graph="off"):
     """The metric's comparison at equal L2 error. On the periodic multi-pulse workload, solved to
     t_end with CFL 0.5, every run's relative L2 error against the exact solution (Gauss quadrature
     of each element) and its time are measured for
@@ -289,6 +289,7 @@ def equal_error_leg(nd, torch, stream, res=EE_RES, t_end=EE_T, orders=(4, 6, 8, 
     def plain(n, cfl, precisions):
         mesh = nd.Mesh2D_t(desc, n, n, 5, 0, 0, 0, 1e-6, 1e-14, 0.0, ndg)
         mesh.set_stream(stream.cuda_stream)
+        mesh.set_graph(graph)
         for prec in precisions:
             mesh.set_precision("fp64")
             mesh.initial_conditions(pulses, period=(2.0, 2.0))
@@ -304,6 +305,7 @@ def equal_error_leg(nd, torch, stream, res=EE_RES, t_end=EE_T, orders=(4, 6, 8, 
     def batched(tol, cfl, prec="fp64"):
         mesh = nd.Mesh2D_t(desc, 4, 12, 5, 0, EE_INTERVAL, 0, tol, 1e-14, 0.0, ndg)
         mesh.set_stream(stream.cuda_stream)
+        mesh.set_graph(graph)
         mesh.initial_conditions(pulses, period=(2.0, 2.0))
         solver = solver_for(cfl)
 
