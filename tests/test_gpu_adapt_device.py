@@ -264,6 +264,35 @@ def test_periodic_splits_equal_host_path(name):
     assert_identical(dm, hm, t)
 
 
+@pytest.mark.parametrize("name", ["four faces", "sixteenth"])
+def test_multi_face_description_matches_oracle(name):
+    """A mesh description that already holds sides with several faces (the oracle's mesh after
+    the splits, given as nodes and quads): the mesh built from it finds every chain of pieces
+    (faces per side as the oracle's) and its right-hand side of a random state matches the
+    oracle's to 1e-12 relative to the largest field."""
+    say("multi-face description", name)
+    desc = nd.square_mesh(8, bc=(nd.BC_WALL,) * 4, jitter=0.0 if name == "sixteenth" else 0.1, seed=7)
+    om = O.Mesh(O.Tables(8, 5), desc, 6, tol_min=1e-4, tol_max=1e-14)
+    rng = np.random.default_rng(11)
+    for pred in MULTI_FACE_SPLITS[name]:
+        om.set_state(*rough_state(om, pred, rng))
+        om.estimate_error()
+        assert om.adapt(8, 5) > 0
+    nodes, _, _, xy = om.elements()
+    fine = nd.MeshDescription(xy, nodes, desc.bars, desc.bar_type, desc.bar_donor)
+    fine.quad_N = np.asarray(om.N, dtype=np.int32)
+    m = nd.Mesh2D_t(fine, 6, 8, 5, 0, 0, 0, 1e-4, 1e-14, 0.0, nd.NDG_t(8, 5))
+    assert m.n_elements_ == om.K and m.n_faces_ == om.n_faces
+    assert np.array_equal(faces_per_side(m), om.side_faces()) and om.side_faces().max() == MULTI_FACE_MOST[name]
+    q = [rng.normal(size=om.n) for _ in range(3)]
+    m.set_solution(*q)
+    om.set_state(*q)
+    G, R = m.compute_rhs(0.0), om.compute_rhs(0.0)
+    scale = max(np.abs(r).max() for r in R)
+    for g, r in zip(G, R):
+        assert float(np.max(np.abs(g - r))) < 1e-12 * scale
+
+
 @pytest.mark.parametrize("mode", ["device", "host"])
 def test_refuses_faces_below_a_sixteenth(mode):
     """A small element covering 1/16 of a larger side that splits while the larger element does
