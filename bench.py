@@ -462,10 +462,19 @@ def adjoint_leg(nd, torch, stream, steps=2000):
     t0 = time.perf_counter()
     A.record_receivers(mesh, rec, dt, steps)
     t_fwd = time.perf_counter() - t0
-    mesh.set_solution(*q0)
-    t0 = time.perf_counter()
-    J, gp, gu, gv, gc = A.adjoint_gradient(mesh, rec, dt, steps, interval, data)
-    t_adj = time.perf_counter() - t0
+
+    def gradient():
+        """One gradient call from q0, timed on its second run: the first sizes the mesh's
+        checkpoint and stage-store blocks (kept for later calls), as the forward solve timed
+        above is the second recorded run."""
+        for _ in range(2):
+            mesh.set_solution(*q0)
+            t0 = time.perf_counter()
+            out = A.adjoint_gradient(mesh, rec, dt, steps, interval, data)
+            t = time.perf_counter() - t0
+        return t, out
+
+    t_adj, (J, gp, gu, gv, gc) = gradient()
     seg_bytes = 3 * interval * 3 * mesh.n_dofs_per_field_ * 8
     n_seg = (steps + interval - 1) // interval
     free, _ = torch.cuda.mem_get_info()
@@ -474,10 +483,7 @@ def adjoint_leg(nd, torch, stream, steps=2000):
         """The same gradient with the stage states of the last n_kept segments kept from the forward
         pass (their recompute skipped); returns (seconds, identical to the checkpointed one)."""
         A.set_store_budget(mesh, (n_kept - 1) * seg_bytes)
-        mesh.set_solution(*q0)
-        t0 = time.perf_counter()
-        J2, gp2, gu2, gv2, gc2 = A.adjoint_gradient(mesh, rec, dt, steps, interval, data)
-        t2 = time.perf_counter() - t0
+        t2, (J2, gp2, gu2, gv2, gc2) = gradient()
         A.set_store_budget(mesh, 0)
         same = J2 == J and gc2 == gc and all(np.array_equal(a, b) for a, b in zip((gp2, gu2, gv2), (gp, gu, gv)))
         return t2, bool(same)
@@ -498,10 +504,7 @@ def adjoint_leg(nd, torch, stream, steps=2000):
                      "identical_to_checkpointed": same})
     # dJ/dc from the stage right-hand sides recorded in the forward pass: no checkpoints, no recompute.
     A.set_c_derivative(mesh, "stage_records")
-    mesh.set_solution(*q0)
-    t0 = time.perf_counter()
-    J3, gp3, gu3, gv3, gc3 = A.adjoint_gradient(mesh, rec, dt, steps, interval, data)
-    t_rec = time.perf_counter() - t0
+    t_rec, (J3, gp3, gu3, gv3, gc3) = gradient()
     A.set_c_derivative(mesh, "recompute")
     stage_records = {"gradient_s": t_rec, "gradient_over_forward": t_rec / t_fwd, "dJ_dc": gc3,
                      "dJ_dc_rel_diff_vs_recompute": abs(gc3 - gc) / abs(gc),
@@ -535,7 +538,9 @@ def adjoint_leg(nd, torch, stream, steps=2000):
             "all_segments_kept": kept, "sixteen_segments_kept": part,
             "c_derivative_from_stage_records": stage_records,
             "fd_rel_diff": abs(ad - fd) / abs(fd),
-            "note": "wall times of the library calls (host-synchronous), forward = one recorded 2000-step run"}
+            "note": "wall times of the library calls (host-synchronous), forward = one recorded 2000-step run "
+                    "(the second), each gradient timed on its second call (the first sizes the mesh's "
+                    "checkpoint and stage-store blocks)"}
 
 
 def copy_rate(torch, stream, gib=2, reps=10):
