@@ -90,6 +90,25 @@ def test_rhs(tables, name, kw, N):
         assert rel(g, r) < TOL, name
 
 
+@pytest.mark.parametrize("N", range(1, 17))
+def test_every_order_rhs_and_steps(tables, N):
+    """Every compiled order (one stage kernel instantiation each) on a jittered periodic 4 x 4 mesh:
+    the right-hand side of a random state and 3 RK3 steps of a pulse against the oracle."""
+    say("every order, N", N)
+    desc, gm, om = build(tables, 4, N, periodic=True, jitter=0.2, seed=N)
+    q = random_state(gm.n_dofs_per_field_, 100 + N)
+    gm.set_solution(*q)
+    om.set_state(*q)
+    for g, r in zip(gm.compute_rhs(0.0), om.compute_rhs(0.0)):
+        assert rel(g, r) < TOL
+    gm.initial_conditions([(0.1, -0.2, 0.4, 1.0)])
+    om.initial_conditions([(0.1, -0.2, 0.4, 1.0)])
+    solver = nd.Solver2D_t(0.5, [0.0])
+    assert solver.run_steps(gm, 3, 0.0) == om.run_steps(0.5, 3, 0.0)
+    for g, r in zip(gm.get_solution(), om.get_state()):
+        assert rel(g, r) < TOL
+
+
 @pytest.mark.parametrize("name,kw,N", CASES)
 def test_steps(tables, name, kw, N):
     say("steps", name, "N", N)
