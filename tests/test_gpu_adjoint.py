@@ -69,6 +69,26 @@ def test_transpose_dot_product(tables):
     assert abs(lhs - rhs) <= 1e-12 * scale
 
 
+@pytest.mark.parametrize("N", range(1, 17))
+def test_transpose_dot_product_every_order(N):
+    """<y, G x> = <G^T y, x> for every compiled order (one transposed-stage instantiation each) on
+    a jittered periodic 4 x 4 mesh, to 1e-12 of sum |y| |G x|."""
+    say("adjoint dot product, N", N)
+    desc, gm, _ = periodic_mesh((nd.NDG_t(N, 5), O.Tables(N, 5)), 4, N, 0.2, 40 + N)
+    A.set_wave_speed(gm, 0.9)
+    rng = np.random.default_rng(N)
+    n = gm.n_dofs_per_field_
+    x = [rng.normal(size=n) for _ in range(3)]
+    y = [rng.normal(size=n) for _ in range(3)]
+    gm.set_solution(*x)
+    Gx = gm.compute_rhs(0.0)
+    Gty = A.apply_rhs_transpose(gm, *y)
+    lhs = sum(float(a @ b) for a, b in zip(y, Gx))
+    rhs = sum(float(a @ b) for a, b in zip(Gty, x))
+    scale = sum(float(np.abs(a) @ np.abs(b)) for a, b in zip(y, Gx))
+    assert abs(lhs - rhs) <= 1e-12 * scale
+
+
 def tlm_run(om, ot, N, q0, dq0, c, dc, dt, n_steps, rec, data):
     """Forward run and tangent-linear model on the oracle's right-hand side: returns (J, dJ)."""
     om.set_wave_speed(c)
