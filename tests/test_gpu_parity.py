@@ -216,6 +216,23 @@ def test_estimator(tables):
     assert np.array_equal(rg[~tie], ro[~tie]) and np.array_equal(cg, co)
 
 
+@pytest.mark.parametrize("N", range(2, 17))
+def test_estimator_every_order(tables, N):
+    """The estimator for every compiled order from 2 (estimate_kernel<N>) on a random state of a
+    jittered 4 x 4 mesh: error estimates to 1e-12 relative, the decay rate sigma to 1e-8, and the
+    same refine/coarsen flags as the oracle."""
+    say("estimator, N", N)
+    desc, gm, om = build(tables, 4, N, jitter=0.2, seed=N)
+    q = random_state(gm.n_dofs_per_field_, 200 + N)
+    gm.set_solution(*q)
+    om.set_state(*q)
+    eg, sg, rg, cg = gm.estimate_error()
+    eo, so, ro, co = om.estimate_error()
+    assert np.max(np.abs(eg - eo) / np.abs(eo)) < 1e-12
+    assert np.max(np.abs(sg - so) / np.abs(so)) < 1e-8
+    assert np.array_equal(rg, ro) and np.array_equal(cg, co)
+
+
 def test_full_size_properties(tables):
     """512 x 512 elements, N = 8 (the roofline size): linearity of G on jittered walls and
     conservation of sum(w w J G_p) on a periodic mesh, without the oracle."""
