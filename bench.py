@@ -25,6 +25,7 @@ PULSES = [(0.1, -0.2, 0.2, 1.0), (-0.4, 0.3, 0.2, 0.5)]
 CFL = 0.5
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_N8_512x512_traffic.json")
+TRAFFIC_FILE_C32 = os.path.join(ROOT, "profiles", "stage_c32_kernel_N8_512x512_traffic.json")
 
 
 def algorithmic_bytes_per_element(n):
@@ -686,6 +687,10 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
     forms = separate_forms(args, legs, nd, torch, stream)
+    traffic_c32 = None
+    if os.path.exists(TRAFFIC_FILE_C32):
+        with open(TRAFFIC_FILE_C32) as f:
+            traffic_c32 = json.load(f).get("hbm_bytes_per_launch")
     if "fp32c" in legs:
         ms32, _, st32 = timed_run("fp32c")
         diff = np.concatenate([a - b for a, b in zip(st32, ref_state)])
@@ -697,7 +702,8 @@ def main():
             "max_rel_vs_fp64": float(np.abs(diff).max() / np.abs(refv).max()),
             "roofline": {"bound": "hbm", "achieved": b32 / (ms32 / 1e3 / stages) / 1e9, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": b32 / (ms32 / 1e3 / stages) / 1e9 / HBM_PEAK_GBS,
-                         "algorithmic_bytes_per_launch": b32, "kernel": "ndg::stage_c32_kernel<8>"},
+                         "algorithmic_bytes_per_launch": b32, "kernel": "ndg::stage_c32_kernel<8>",
+                         "traffic": traffic_c32},
             "note": "FP32 right-hand side and RK register, state as a two-sum-compensated float pair; "
                     "error is against the FP64 run of the same steps",
             "periodic_16x16_N6": fp32c_periodic_16x16(nd)}
